@@ -1,0 +1,128 @@
+/*
+ * p3smi.h -- C ABI of the MI355X batched polar Newton power-flow engine (libp3smi.so).
+ *
+ * Plain C types only: pointers, sizes, status codes. Every function returns 0 on success
+ * and a nonzero P3SMI_E_* code on failure; p3smi_last_error() then gives the message of
+ * the calling thread's last failure. Complex arrays are interleaved (re, im) doubles, the
+ * memory layout of numpy complex128. A (rows, cases) array is C-ordered: element
+ * (i, c) lives at [i * cases + c], the layout of a numpy (n, B) array.
+ *
+ * The entry points replace the reference's pybind11 module nr_klu
+ * (p3s/cpp/nr_klu.cpp:1315-1376) and the GPU solver's batch calls
+ * (p3s/cuda/nr_polar_solver.py:466-557); each declaration cites what it replaces.
+ */
+#ifndef P3SMI_H
+#define P3SMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define P3SMI_OK 0
+#define P3SMI_E_ARG 1      /* invalid argument (shape, range, NaN start voltage, ...) */
+#define P3SMI_E_DEVICE 2   /* HIP runtime failure or no device */
+#define P3SMI_E_MEMORY 3   /* device or host allocation failed */
+#define P3SMI_E_INTERNAL 4
+
+/* precision of the Newton-step linear solve */
+#define P3SMI_PREC_DOUBLE 0 /* FP64 static-pivot LU: the iteration nr_klu runs */
+#define P3SMI_PREC_MIXED 1  /* FP32 LU + FP64 iterative refinement, FP64 LU fallback per case */
+
+typedef struct p3smi_topology p3smi_topology; /* one grid topology, resident on the device */
+typedef struct p3smi_batch p3smi_batch;       /* device workspace for a batch of cases */
+
+typedef struct {
+    int32_t n, m, npv, npq, nnz_y, nnz_j;
+    int64_t nnz_lu, n_pairs;
+    int32_t n_levels, n_blevels, n_slots, ordering;
+    double rcond0;
+} p3smi_info;
+
+/* Per-case/aggregate counters of the last solve (the reference's last_stats,
+ * p3s/cuda/nr_polar_solver.py:671-681). Kernel times are filled only when timing is on. */
+typedef struct {
+    int32_t cases, newton_passes, line_search_passes;
+    int32_t failed_factorizations;
+    int64_t single_factorizations, double_factorizations, refinement_steps;
+    double ms_eval, ms_lu, ms_step, ms_decide, ms_other;
+    int32_t n_eval, n_lu, n_step, n_decide;
+} p3smi_stats;
+
+/* ---- library ------------------------------------------------------------------------ */
+const char* p3smi_last_error(void);
+int p3smi_version(void);
+/* Number of visible HIP devices (0 without a GPU); never initialises a context. */
+int p3smi_device_count(int32_t* count);
+
+/* ---- topology (replaces nr_klu.Solver.__init__, p3s/cpp/nr_klu.cpp:966-994, and
+ *      PolarNewtonSolverCUDA.__init__, p3s/cuda/nr_polar_solver.py:282-376) ---------------
+ * Yp (n+1), Yj (nnz): CSR pattern of Ybus, columns sorted per row; Yx (nnz) complex.
+ * pv (npv), pq (npq): bus index sets. ordering: 0 minimum degree, 2 natural.
+ * n_slots: task lists per workgroup of the LU kernel (0 = default).
+ * The host plan is built here; device buffers are uploaded on first use. */
+int p3smi_topology_create(const int32_t* Yp, const int32_t* Yj, const double* Yx, int32_t n, int32_t nnz,
+                          const int32_t* pv, int32_t npv, const int32_t* pq, int32_t npq, int32_t ordering,
+                          int32_t n_slots, p3smi_topology** out);
+void p3smi_topology_free(p3smi_topology* t);
+int p3smi_topology_info(const p3smi_topology* t, p3smi_info* out);
+/* Refresh Ybus values on the same pattern: nr_klu.Solver.update_Y (p3s/cpp/nr_klu.cpp:999-1011). */
+int p3smi_update_y(p3smi_topology* t, const double* Yx, int32_t nnz);
+
+/* ---- host-side checks of the plan (no GPU needed) ------------------------------------
+ * Jacobian (CSR by equation: Jp (m+1), Jj/Jx (nnz_j)) and residual R = Sspec - Scalc (m)
+ * at V (n complex) with injections S (n complex); the reference's debug_J
+ * (p3s/cpp/nr_klu.cpp:1284-1304), in this engine's row/entry order. */
+int p3smi_host_jacobian(const p3smi_topology* t, const double* S, const double* V, int32_t* Jp, int32_t* Jj,
+                        double* Jx, double* R);
+/* Factor J (entry order) with the frozen plan and solve J x = R on the host; x in Jacobian
+ * column order. pivots[0..2] = min |pivot|, max |pivot|, sum |pivot|. */
+int p3smi_host_factor_solve(const p3smi_topology* t, const double* Jx, const double* R, double* x, double* pivots);
+/* Bus -> variable maps and the elimination order (m): xva/xvm (n), perm (m). */
+int p3smi_host_maps(const p3smi_topology* t, int32_t* row_p, int32_t* row_q, int32_t* perm);
+
+/* ---- one-call batch solves (host arrays in and out) ----------------------------------
+ * nr_klu.Solver.solve_batch (p3s/cpp/nr_klu.cpp:1049-1166):
+ *   S (n, T) complex, V0 (n,) complex [v0_per_case = 0] or (n, T) [1].
+ *   Outputs V (n, T) complex, iterations (T), converged (T), and, if not NULL, the final
+ *   mismatch infinity norm of every case (T), computed on the device in FP64. */
+int p3smi_solve_batch(p3smi_topology* t, const double* S, const double* V0, int32_t v0_per_case, int32_t T,
+                      int32_t max_iter, double tol, int32_t line_search, int32_t precision, double* V_out,
+                      int32_t* iterations, uint8_t* converged, double* mismatch);
+
+/* nr_klu.Solver.solve_batch_contingency (p3s/cpp/nr_klu.cpp:1184-1281) with per-case
+ * Ybus magnitude/angle (nnz, L), S (n,) complex, V0 (n, L) complex, pin (n, L) uint8. */
+int p3smi_solve_batch_contingency(p3smi_topology* t, const double* Yx_mag, const double* Yx_ang, const double* S,
+                                  const double* V0, const uint8_t* pin, int32_t L, int32_t max_iter, double tol,
+                                  int32_t line_search, int32_t precision, double* V_out, int32_t* iterations,
+                                  uint8_t* converged, double* mismatch);
+
+/* PolarNewtonSolverCUDA.solve_batch_contingency_stamp (p3s/cuda/nr_polar_solver.py:531-557):
+ * case c runs on Ybus = base - stamp[:, c]; the stamp is a COO list (k, c, value complex)
+ * with each (k, c) at most once. The per-case Ybus is expanded on the device. */
+int p3smi_solve_batch_contingency_stamp(p3smi_topology* t, const int32_t* stamp_k, const int32_t* stamp_c,
+                                        const double* stamp_val, int64_t n_stamp, const double* S, const double* V0,
+                                        const uint8_t* pin, int32_t L, int32_t max_iter, double tol,
+                                        int32_t line_search, int32_t precision, double* V_out, int32_t* iterations,
+                                        uint8_t* converged, double* mismatch);
+
+/* ---- resident batches (inputs uploaded once, solved many times) ---------------------- */
+int p3smi_batch_create(p3smi_topology* t, int32_t cases, p3smi_batch** out);
+void p3smi_batch_free(p3smi_batch* b);
+/* Upload time-series inputs: S (n, cases) complex, V0 (n,) or (n, cases) complex. */
+int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_t v0_per_case);
+/* Run the whole Newton solve of every case on the device from the uploaded inputs. */
+int p3smi_batch_solve(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_search, int32_t precision);
+int p3smi_batch_download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* converged, double* mismatch);
+int p3smi_batch_stats(const p3smi_batch* b, p3smi_stats* out);
+/* 1 = record HIP events around every kernel of later solves (per-kernel times in stats). */
+int p3smi_batch_set_timing(p3smi_batch* b, int32_t on);
+/* Algorithmic bytes of one pass of each kernel for one case (eval, lu, step, decide). */
+int p3smi_batch_bytes_per_case(const p3smi_batch* b, double* out4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* P3SMI_H */

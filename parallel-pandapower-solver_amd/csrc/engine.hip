@@ -1,0 +1,1148 @@
+// engine.hip -- batched polar Newton-Raphson power flow on one MI355X (gfx950).
+//
+// The Newton iteration is the one of the reference's nr_klu (p3s/cpp/nr_klu.cpp:569-711):
+// polar mismatch and Jacobian, static-pivot LU on a frozen pattern with a pivot-ratio
+// guard, Armijo backtracking (beta 0.5, at most 10 halvings, c = 1e-4, best trial taken
+// when none passes). Every case of a batch iterates on its own; the device keeps one
+// status word per case and every kernel skips the cases it has nothing to do for.
+//
+// LAYOUT. A per-case quantity q of item i (bus, Jacobian entry, factor position, ...) is
+// stored at [i * S + c], S = number of cases rounded up to 64: one wavefront covers 64
+// cases of one item, so every value load is one contiguous 512-byte line and every index
+// (Ybus column, factor position, operand pair) is wave-uniform and comes through the
+// scalar unit once per wavefront.
+//
+// KERNELS (one pass each; the host loop below sequences them):
+//   k_eval      mismatch R = Sspec - Scalc, its infinity norm, and (masked) the Jacobian
+//   k_lu        static-pivot LU of the Jacobian + forward/backward substitution, one
+//               workgroup per 64 cases, level-synchronous over the elimination tree
+//   k_step      V = V0 + alpha * dx (full Newton step or line-search trial)
+//   k_decide    convergence test and Armijo bookkeeping per case
+//
+// STATUS per case: ACTIVE (iterating), CONVERGED, FAILED (non-finite mismatch or rejected
+// pivots), BACKTRACK (line-search trial), NEEDJ (trial accepted, Jacobian to rebuild),
+// STOPPED (iteration limit), IDLE (padding).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "p3smi.h"
+#include "plan.hpp"
+
+namespace {
+
+enum : int { ST_ACTIVE = 0, ST_CONVERGED = 1, ST_FAILED = 2, ST_BACKTRACK = 3, ST_NEEDJ = 4, ST_STOPPED = 5, ST_IDLE = 6 };
+constexpr int NSTAT = 8;
+constexpr double ARMIJO_C = 1e-4;
+constexpr int LS_MAX_TRIALS = 10;
+constexpr double REUSE_RCOND_RATIO = 1e-3;
+constexpr int EVAL_WAVES = 4;         // wavefronts per k_eval workgroup
+constexpr int EVAL_BUSES_PER_WAVE = 16;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                                 \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw Error(P3SMI_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+thread_local std::string g_err;
+
+// ===================================================================== device code
+__device__ __forceinline__ size_t at(int item, int S, int c) { return (size_t)item * (size_t)S + (size_t)c; }
+
+__device__ __forceinline__ double nan_max(double w, double a) { return (a > w || a != a) ? a : w; }
+
+__global__ void k_status_init(int* status, int B, int S) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < S) status[c] = c < B ? ST_ACTIVE : ST_IDLE;
+}
+
+__global__ void k_clear_int(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// start voltage -> polar state. V0: (n) shared complex (v0_stride 0) or (n, B) complex.
+__global__ void k_init(const double* __restrict__ V0, int v0_stride, double* __restrict__ Vm,
+                       double* __restrict__ Va, double* __restrict__ cs, double* __restrict__ sn, int n, int S,
+                       int B) {
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int bus = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (bus >= n) return;
+    const size_t o = at(bus, S, c);
+    if (c >= B) {
+        Vm[o] = 1.0; Va[o] = 0.0; cs[o] = 1.0; sn[o] = 0.0;
+        return;
+    }
+    const size_t v = 2 * (v0_stride ? (size_t)bus * v0_stride + c : (size_t)bus);
+    const double re = V0[v], im = V0[v + 1];
+    const double va = atan2(im, re);
+    double s, co;
+    sincos(va, &s, &co);
+    Vm[o] = hypot(re, im);
+    Va[o] = va;
+    cs[o] = co;
+    sn[o] = s;
+}
+
+// Per-case Ybus from magnitude/angle (nnz, L) -> G, B at [k * S + c].
+__global__ void k_y_polar(const double* __restrict__ mag, const double* __restrict__ ang, double* __restrict__ G,
+                          double* __restrict__ Bv, int nnz, int L, int ld, int S) {
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int k = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (k >= nnz || c >= L) return;
+    const double m = mag[(size_t)k * ld + c], a = ang[(size_t)k * ld + c];
+    double s, co;
+    sincos(a, &s, &co);
+    G[at(k, S, c)] = m * co;
+    Bv[at(k, S, c)] = m * s;
+}
+
+// Per-case Ybus = base - stamp: fill with the base, then subtract the COO stamp entries.
+__global__ void k_y_base(const double* __restrict__ G0, const double* __restrict__ B0, double* __restrict__ G,
+                         double* __restrict__ Bv, int nnz, int S) {
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int k = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (k >= nnz) return;
+    G[at(k, S, c)] = G0[k];
+    Bv[at(k, S, c)] = B0[k];
+}
+
+__global__ void k_y_stamp(const int* __restrict__ sk, const int* __restrict__ sc, const double* __restrict__ sv,
+                          double* __restrict__ G, double* __restrict__ Bv, long long nst, int c0, int nc, int S) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nst) return;
+    const int c = sc[e] - c0;
+    if (c < 0 || c >= nc) return;
+    G[at(sk[e], S, c)] -= sv[2 * e];
+    Bv[at(sk[e], S, c)] -= sv[2 * e + 1];
+}
+
+// Mismatch, its infinity norm and (for statuses in j_mask) the Jacobian. A wavefront walks
+// one Ybus row for 64 cases. With the unit phasor (cs, sn) of every bus,
+//   Ym cos(delta) = G cu + B su,  Ym sin(delta) = G su - B cu,
+//   cu = cos(Va_i - Va_k) = ci ck + si sk,  su = sin(Va_i - Va_k) = si ck - ci sk,
+// the polar derivatives of p3s/cpp/nr_klu.cpp:288-378 need no trigonometry per nonzero.
+// A pinned bus gets identity rows and zero mismatch (p3s/cpp/nr_klu.cpp:398-423).
+// part[blockIdx.x * S + c]: the largest |R| of the workgroup's buses (NaN kept).
+template <bool PER_CASE_Y, bool HAS_PIN>
+__global__ __launch_bounds__(64 * EVAL_WAVES) void k_eval(
+    const int* __restrict__ Yp, const int* __restrict__ Yj, const int* __restrict__ Ydiag,
+    const double* __restrict__ G, const double* __restrict__ Bv, const double* __restrict__ Vm,
+    const double* __restrict__ cs, const double* __restrict__ sn, const double* __restrict__ Ssp, int s_stride,
+    const int* __restrict__ row_p, const int* __restrict__ row_q, const unsigned char* __restrict__ pin,
+    const int* __restrict__ status, int run_mask, int j_mask, const int* __restrict__ posJ, double* __restrict__ Jx,
+    double* __restrict__ R, double* __restrict__ part, int n, int S) {
+    __shared__ double sw[EVAL_WAVES][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.y * 64 + lane;
+    const int st = status[c];
+    const bool run = (run_mask >> st) & 1;
+    const bool wj = (j_mask >> st) & 1;
+    double worst = 0.0;
+    const int b0 = blockIdx.x * (EVAL_WAVES * EVAL_BUSES_PER_WAVE);
+    const int b1 = min(n, b0 + EVAL_WAVES * EVAL_BUSES_PER_WAVE);
+    if (run) {
+        for (int row = b0 + wave; row < b1; row += EVAL_WAVES) {
+            const size_t oi = at(row, S, c);
+            const double vi = Vm[oi], ci = cs[oi], si = sn[oi];
+            const bool pinned = HAS_PIN && pin[oi];
+            double p = 0.0, q = 0.0, dPVa = 0.0, dQVa = 0.0, dPVm = 0.0, dQVm = 0.0;
+            const int k1 = Yp[row + 1];
+            for (int k = Yp[row]; k < k1; ++k) {
+                const int col = Yj[k];
+                const double g = PER_CASE_Y ? G[at(k, S, c)] : G[k];
+                const double b = PER_CASE_Y ? Bv[at(k, S, c)] : Bv[k];
+                if (col != row) {
+                    const size_t ok = at(col, S, c);
+                    const double vk = Vm[ok], ck = cs[ok], sk = sn[ok];
+                    const double cu = ci * ck + si * sk;
+                    const double su = si * ck - ci * sk;
+                    const double gc = g * cu + b * su;
+                    const double gs = g * su - b * cu;
+                    const double vgc = vi * gc, vgs = vi * gs;
+                    const double a = vgc * vk, bt = vgs * vk;
+                    p += a;
+                    q += bt;
+                    dPVa -= bt;
+                    dQVa += a;
+                    dPVm += vk * gc;
+                    dQVm += vk * gs;
+                    if (wj) {
+                        const int4 pj = reinterpret_cast<const int4*>(posJ)[k];
+                        if (pj.x >= 0) Jx[at(pj.x, S, c)] = pinned ? 0.0 : bt;
+                        if (pj.y >= 0) Jx[at(pj.y, S, c)] = pinned ? 0.0 : vgc;
+                        if (pj.z >= 0) Jx[at(pj.z, S, c)] = pinned ? 0.0 : -a;
+                        if (pj.w >= 0) Jx[at(pj.w, S, c)] = pinned ? 0.0 : vgs;
+                    }
+                } else {
+                    p += vi * vi * g;
+                    q -= vi * vi * b;
+                    dPVm += 2.0 * vi * g;
+                    dQVm -= 2.0 * vi * b;
+                }
+            }
+            const int d = Ydiag[row];
+            if (wj && d >= 0) {
+                const int4 pj = reinterpret_cast<const int4*>(posJ)[d];
+                if (pj.x >= 0) Jx[at(pj.x, S, c)] = pinned ? 1.0 : dPVa;
+                if (pj.y >= 0) Jx[at(pj.y, S, c)] = pinned ? 0.0 : dPVm;
+                if (pj.z >= 0) Jx[at(pj.z, S, c)] = pinned ? 0.0 : dQVa;
+                if (pj.w >= 0) Jx[at(pj.w, S, c)] = pinned ? 1.0 : dQVm;
+            }
+            const size_t so = 2 * (s_stride ? (size_t)row * s_stride + c : (size_t)row);
+            const int rp = row_p[row], rq = row_q[row];
+            if (rp >= 0) {
+                const double r = pinned ? 0.0 : Ssp[so] - p;
+                R[at(rp, S, c)] = r;
+                worst = nan_max(worst, fabs(r));
+            }
+            if (rq >= 0) {
+                const double r = pinned ? 0.0 : Ssp[so + 1] - q;
+                R[at(rq, S, c)] = r;
+                worst = nan_max(worst, fabs(r));
+            }
+        }
+    }
+    sw[wave][lane] = worst;
+    __syncthreads();
+    if (wave == 0 && run) {
+        double w = sw[0][lane];
+        for (int v = 1; v < EVAL_WAVES; ++v) w = nan_max(w, sw[v][lane]);
+        part[at(blockIdx.x, S, c)] = w;
+    }
+}
+
+// Static-pivot LU + substitution for the ACTIVE cases of 64 consecutive cases (lanes);
+// the NS wavefronts of the workgroup are NS task lists. Level l of the elimination tree:
+//   sub-level A: pivots and U entries   acc = a - sum L(i,t) U(t,k); pivot -> 1/acc
+//                forward rows            y_i = R[perm i] - sum L(i,j) y_j
+//   barrier
+//   sub-level B: L entries               acc = (a - sum L(i,t) U(t,k)) / U(k,k)
+//   barrier
+// then backward rows by depth: x_i = (y_i - sum U(i,j) x_j) / U(i,i).
+// Pivot check (p3s/cpp/lean_lu.hpp:146-177, p3s/cpp/nr_klu.cpp:610-620): every pivot
+// non-zero and finite and min|pivot| >= rcond_min * max|pivot|, else the case is FAILED.
+template <int NS>
+__global__ __launch_bounds__(64 * NS) void k_lu(
+    const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
+    const int4* __restrict__ taskB, const int* __restrict__ diagB, const int2* __restrict__ pairs,
+    const int* __restrict__ ptrF, const int4* __restrict__ fwd, const int2* __restrict__ fent,
+    const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int* __restrict__ Li,
+    const double* __restrict__ Jx, double* __restrict__ fac, const double* __restrict__ R, double* __restrict__ x,
+    int* __restrict__ status, int* __restrict__ iters, double* __restrict__ pstat, double rcond_min, int nlev,
+    int nblev, int S, int it) {
+    __shared__ double smin[NS][64], smax[NS][64], ssum[NS][64];
+    __shared__ int sbad[64];
+    const int lane = threadIdx.x & 63;
+    const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.x * 64 + lane;
+    bool act = status[c] == ST_ACTIVE;
+    if (!__syncthreads_or(act)) return;
+
+    double pmin = DBL_MAX, pmax = 0.0, psum = 0.0;
+    for (int l = 0; l < nlev; ++l) {
+        if (act) {
+            const int q0 = ptrA[l * NS + slot], q1 = ptrA[l * NS + slot + 1];
+            for (int q = q0; q < q1; ++q) {
+                const int4 tk = taskA[q];
+                const bool diag = tk.y < -1;
+                const int src = diag ? -tk.y - 2 : tk.y;
+                double acc = src >= 0 ? Jx[at(src, S, c)] : 0.0;
+                for (int e = tk.z; e < tk.w; ++e) {
+                    const int2 pp = pairs[e];
+                    acc = fma(-fac[at(pp.x, S, c)], fac[at(pp.y, S, c)], acc);
+                }
+                if (diag) {
+                    const double ab = fabs(acc);
+                    pmin = fmin(pmin, ab);
+                    pmax = fmax(pmax, ab);
+                    psum += ab;
+                    acc = 1.0 / acc;
+                }
+                fac[at(tk.x, S, c)] = acc;
+            }
+            const int f0 = ptrF[l * NS + slot], f1 = ptrF[l * NS + slot + 1];
+            for (int q = f0; q < f1; ++q) {
+                const int4 fr = fwd[q];
+                double acc = R[at(fr.y, S, c)];
+                for (int e = fr.z; e < fr.w; ++e) {
+                    const int2 fe = fent[e];
+                    acc = fma(-fac[at(fe.x, S, c)], x[at(fe.y, S, c)], acc);
+                }
+                x[at(fr.x, S, c)] = acc;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            const int q0 = ptrB[l * NS + slot], q1 = ptrB[l * NS + slot + 1];
+            for (int q = q0; q < q1; ++q) {
+                const int4 tk = taskB[q];
+                double acc = tk.y >= 0 ? Jx[at(tk.y, S, c)] : 0.0;
+                for (int e = tk.z; e < tk.w; ++e) {
+                    const int2 pp = pairs[e];
+                    acc = fma(-fac[at(pp.x, S, c)], fac[at(pp.y, S, c)], acc);
+                }
+                fac[at(tk.x, S, c)] = acc * fac[at(diagB[q], S, c)];
+            }
+        }
+        __syncthreads();
+    }
+    smin[slot][lane] = pmin;
+    smax[slot][lane] = pmax;
+    ssum[slot][lane] = psum;
+    __syncthreads();
+    if (slot == 0) {
+        int bad = 0;
+        if (act) {
+            for (int s = 1; s < NS; ++s) {
+                pmin = fmin(pmin, smin[s][lane]);
+                pmax = fmax(pmax, smax[s][lane]);
+                psum += ssum[s][lane];
+            }
+            pstat[2 * (size_t)c] = pmin;
+            pstat[2 * (size_t)c + 1] = pmax;
+            const bool ok = psum < DBL_MAX && pmin > 0.0 && pmin >= rcond_min * pmax;
+            if (!ok) {
+                bad = 1;
+                status[c] = ST_FAILED;
+                iters[c] = it;
+            }
+        }
+        sbad[lane] = bad;
+    }
+    __syncthreads();
+    if (sbad[lane]) act = false;
+    for (int l = 0; l < nblev; ++l) {
+        if (act) {
+            const int q0 = ptrW[l * NS + slot], q1 = ptrW[l * NS + slot + 1];
+            for (int q = q0; q < q1; ++q) {
+                const int4 br = bwd[q];
+                double acc = x[at(br.x, S, c)];
+                const int pos0 = br.y + 1 - br.z;
+                for (int e = br.z; e < br.w; ++e) acc = fma(-fac[at(pos0 + e, S, c)], x[at(Li[e], S, c)], acc);
+                x[at(br.x, S, c)] = acc * fac[at(br.y, S, c)];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// V = V0 + alpha * dx for the cases whose status is in mask. full: alpha = 1 and the
+// current point is saved as the base V0 of the step (p3s/cpp/nr_klu.cpp:665-669).
+__global__ void k_step(const double* __restrict__ x, const int* __restrict__ xva, const int* __restrict__ xvm,
+                       const int* __restrict__ status, int mask, int full, const double* __restrict__ alpha,
+                       double* __restrict__ Vm, double* __restrict__ Va, double* __restrict__ cs,
+                       double* __restrict__ sn, double* __restrict__ Vm0, double* __restrict__ Va0, int n, int S) {
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int bus = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (bus >= n) return;
+    if (!((mask >> status[c]) & 1)) return;
+    const double a = full ? 1.0 : alpha[c];
+    const size_t o = at(bus, S, c);
+    const int iva = xva[bus];
+    if (iva >= 0) {
+        double va0;
+        if (full) { va0 = Va[o]; Va0[o] = va0; } else va0 = Va0[o];
+        const double va = va0 + a * x[at(iva, S, c)];
+        double s, co;
+        sincos(va, &s, &co);
+        Va[o] = va;
+        cs[o] = co;
+        sn[o] = s;
+    }
+    const int ivm = xvm[bus];
+    if (ivm >= 0) {
+        double vm0;
+        if (full) { vm0 = Vm[o]; Vm0[o] = vm0; } else vm0 = Vm0[o];
+        Vm[o] = vm0 + a * x[at(ivm, S, c)];
+    }
+}
+
+// Per-case Newton bookkeeping after an eval pass (p3s/cpp/nr_klu.cpp:590-709):
+//   phase 0  start point: converged if ||F|| < tol, failed if non-finite
+//   phase 1  full step of iteration `it`: Armijo test, else start backtracking
+//   phase 2  backtracking trial: accept, halve, or after 10 trials take the best
+//   phase 3  Jacobian rebuilt at the accepted point
+// The loop of the reference ends after the step of iteration max_iter without a further
+// convergence test, so such a case ends STOPPED (not converged).
+__global__ void k_decide(const double* __restrict__ part, int nparts, double* __restrict__ nrm,
+                         double* __restrict__ alpha, double* __restrict__ best_alpha, double* __restrict__ best_nrm,
+                         int* __restrict__ trial, int* __restrict__ status, int* __restrict__ iters,
+                         int* __restrict__ counts, double tol, int line_search, int phase, int it, int max_iter,
+                         int S) {
+    __shared__ int cnt[NSTAT];
+    if (threadIdx.x < NSTAT) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < S) {
+        int st = status[c];
+        const int want = phase == 2 ? ST_BACKTRACK : phase == 3 ? ST_NEEDJ : ST_ACTIVE;
+        if (st == want) {
+            double nw = 0.0;
+            for (int b = 0; b < nparts; ++b) nw = nan_max(nw, part[at(b, S, c)]);
+            if (!(nw < DBL_MAX)) nw = DBL_MAX;
+            auto classify = [&](double v) {
+                if (it >= max_iter && phase != 0) return (int)ST_STOPPED;
+                return v < tol ? (int)ST_CONVERGED : (v >= DBL_MAX ? (int)ST_FAILED : (int)ST_ACTIVE);
+            };
+            if (phase == 0) {
+                nrm[c] = nw;
+                iters[c] = 0;
+                st = classify(nw);
+            } else if (phase == 1) {
+                iters[c] = it;
+                if (!line_search || nw <= (1.0 - ARMIJO_C) * nrm[c]) {
+                    nrm[c] = nw;
+                    st = classify(nw);
+                } else {
+                    best_alpha[c] = 1.0;
+                    best_nrm[c] = nw;
+                    alpha[c] = 0.5;
+                    trial[c] = 1;
+                    st = ST_BACKTRACK;
+                }
+            } else if (phase == 2) {
+                const double a = alpha[c];
+                if (nw < best_nrm[c]) { best_nrm[c] = nw; best_alpha[c] = a; }
+                if (nw <= (1.0 - ARMIJO_C * a) * nrm[c]) {
+                    st = ST_NEEDJ;
+                } else if (trial[c] >= LS_MAX_TRIALS) {
+                    alpha[c] = best_alpha[c];
+                    st = ST_NEEDJ;
+                } else {
+                    trial[c] += 1;
+                    alpha[c] = 0.5 * a;
+                }
+            } else {
+                nrm[c] = nw;
+                st = classify(nw);
+            }
+            status[c] = st;
+        }
+        atomicAdd(&cnt[st], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < NSTAT && cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], cnt[threadIdx.x]);
+}
+
+// polar state -> complex V (n, B), the layout of the numpy result.
+__global__ void k_finalize(const double* __restrict__ Vm, const double* __restrict__ cs,
+                           const double* __restrict__ sn, double* __restrict__ V, int n, int S, int B) {
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int bus = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (bus >= n || c >= B) return;
+    const size_t o = at(bus, S, c);
+    const double vm = Vm[o];
+    V[2 * ((size_t)bus * B + c)] = vm * cs[o];
+    V[2 * ((size_t)bus * B + c) + 1] = vm * sn[o];
+}
+
+// ===================================================================== host side
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) count = 1;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            throw Error(P3SMI_E_MEMORY, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) HIPCHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+struct DevTopo {
+    DevBuf<int> Yp, Yj, Ydiag, row_p, row_q, posJ, xva, xvm;
+    DevBuf<double> G, B;
+    DevBuf<int> ptrA, ptrB, ptrF, ptrW, diagB, Li;
+    DevBuf<int4> taskA, taskB, fwd, bwd;
+    DevBuf<int2> pairs, fent;
+    bool ready = false;
+};
+
+}  // namespace
+
+struct p3smi_topology {
+    p3smi::Topology host;
+    DevTopo dev;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<p3smi_batch> cached;  // workspace of the one-call entry points
+    std::mutex mu;
+    ~p3smi_topology();
+};
+
+struct p3smi_batch {
+    p3smi_topology* topo = nullptr;
+    int B = 0, S = 0;
+    int nparts = 0;
+    DevBuf<double> Vm, Va, cs, sn, Vm0, Va0, Jx, fac, R, x;
+    DevBuf<double> nrm, alpha, best_alpha, best_nrm, part, pstat;
+    DevBuf<int> status, iters, trial, counts;
+    DevBuf<double> Ssp, V0, Vout;       // inputs / output in the caller's layouts
+    DevBuf<double> Yg, Yb;              // per-case Ybus (N-1)
+    DevBuf<unsigned char> pin;          // per-case pin mask (N-1)
+    int s_stride = 0, v0_stride = 0;
+    bool per_case_y = false, has_pin = false, uploaded = false;
+    int* h_counts = nullptr;            // pinned host mirror of counts
+    bool timing = false;
+    p3smi_stats stats{};
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    size_t ev_used = 0;
+    ~p3smi_batch() {
+        if (h_counts) (void)hipHostFree(h_counts);
+        for (auto& e : ev) {
+            (void)hipEventDestroy(e.second.first);
+            (void)hipEventDestroy(e.second.second);
+        }
+    }
+};
+
+p3smi_topology::~p3smi_topology() {
+    cached.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+void ensure_device(p3smi_topology* t) {
+    if (t->dev.ready) return;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        throw Error(P3SMI_E_DEVICE, "no HIP device available (the MI355X engine needs a GPU)");
+    if (!t->stream) HIPCHECK(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    const p3smi::Topology& h = t->host;
+    const p3smi::Schedule& s = h.sched;
+    hipStream_t st = t->stream;
+    DevTopo& d = t->dev;
+    d.Yp.upload(h.Yp.data(), h.Yp.size(), st);
+    d.Yj.upload(h.Yj.data(), h.Yj.size(), st);
+    d.Ydiag.upload(h.Ydiag.data(), h.Ydiag.size(), st);
+    d.G.upload(h.G.data(), h.G.size(), st);
+    d.B.upload(h.B.data(), h.B.size(), st);
+    d.row_p.upload(h.row_p.data(), h.row_p.size(), st);
+    d.row_q.upload(h.row_q.data(), h.row_q.size(), st);
+    d.posJ.upload(h.posJ.data(), h.posJ.size(), st);
+    d.xva.upload(h.xva.data(), h.xva.size(), st);
+    d.xvm.upload(h.xvm.data(), h.xvm.size(), st);
+    d.ptrA.upload(s.ptrA.data(), s.ptrA.size(), st);
+    d.ptrB.upload(s.ptrB.data(), s.ptrB.size(), st);
+    d.ptrF.upload(s.ptrF.data(), s.ptrF.size(), st);
+    d.ptrW.upload(s.ptrW.data(), s.ptrW.size(), st);
+    d.diagB.upload(s.diagB.data(), s.diagB.size(), st);
+    d.Li.upload(h.Li.data(), h.Li.size(), st);
+    std::vector<int4> ta(s.taskA.size()), tb(s.taskB.size()), fw(s.fwd.size()), bw(s.bwd.size());
+    for (size_t i = 0; i < ta.size(); ++i) {
+        const auto& k = s.taskA[i];
+        int src = k.asrc;
+        if (s.isdiagA[i]) {
+            if (src < 0) throw Error(P3SMI_E_INTERNAL, "pivot without a Jacobian entry");
+            src = -src - 2;
+        }
+        ta[i] = make_int4(k.dst, src, k.pbeg, k.pend);
+    }
+    for (size_t i = 0; i < tb.size(); ++i) tb[i] = make_int4(s.taskB[i].dst, s.taskB[i].asrc, s.taskB[i].pbeg, s.taskB[i].pend);
+    for (size_t i = 0; i < fw.size(); ++i) fw[i] = make_int4(s.fwd[i].row, s.fwd[i].rsrc, s.fwd[i].ebeg, s.fwd[i].eend);
+    for (size_t i = 0; i < bw.size(); ++i) bw[i] = make_int4(s.bwd[i].row, s.bwd[i].base, s.bwd[i].lbeg, s.bwd[i].lend);
+    std::vector<int2> pr(h.pa.size()), fe(h.Fpos.size());
+    for (size_t i = 0; i < pr.size(); ++i) pr[i] = make_int2(h.pa[i], h.pb[i]);
+    for (size_t i = 0; i < fe.size(); ++i) fe[i] = make_int2(h.Fpos[i], h.Fcol[i]);
+    d.taskA.upload(ta.data(), ta.size(), st);
+    d.taskB.upload(tb.data(), tb.size(), st);
+    d.fwd.upload(fw.data(), fw.size(), st);
+    d.bwd.upload(bw.data(), bw.size(), st);
+    d.pairs.upload(pr.data(), pr.size(), st);
+    d.fent.upload(fe.data(), fe.size(), st);
+    HIPCHECK(hipStreamSynchronize(st));
+    d.ready = true;
+}
+
+int eval_nparts(int n) { return (n + EVAL_WAVES * EVAL_BUSES_PER_WAVE - 1) / (EVAL_WAVES * EVAL_BUSES_PER_WAVE); }
+
+void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
+    ensure_device(t);
+    const p3smi::Topology& h = t->host;
+    b->topo = t;
+    b->B = cases;
+    b->S = (cases + 63) / 64 * 64;
+    const size_t S = b->S;
+    b->nparts = eval_nparts(h.n);
+    for (auto* v : {&b->Vm, &b->Va, &b->cs, &b->sn, &b->Vm0, &b->Va0}) v->alloc((size_t)h.n * S);
+    b->Jx.alloc((size_t)h.nnzJ * S);
+    b->fac.alloc((size_t)h.nnz_lu * S);
+    b->R.alloc((size_t)h.m * S);
+    b->x.alloc((size_t)h.m * S);
+    for (auto* v : {&b->nrm, &b->alpha, &b->best_alpha, &b->best_nrm}) v->alloc(S);
+    b->part.alloc((size_t)b->nparts * S);
+    b->pstat.alloc(2 * S);
+    for (auto* v : {&b->status, &b->iters, &b->trial}) v->alloc(S);
+    b->counts.alloc(NSTAT);
+    if (!b->h_counts) HIPCHECK(hipHostMalloc((void**)&b->h_counts, NSTAT * sizeof(int), hipHostMallocDefault));
+}
+
+// kernel timing (optional): kinds 0 eval, 1 lu, 2 step, 3 decide, 4 other
+struct KTimer {
+    p3smi_batch* b;
+    int kind;
+    hipEvent_t stop = nullptr;
+    KTimer(p3smi_batch* bb, int k) : b(bb), kind(k) {
+        if (!b->timing) return;
+        if (b->ev_used == b->ev.size()) {
+            hipEvent_t e0, e1;
+            HIPCHECK(hipEventCreate(&e0));
+            HIPCHECK(hipEventCreate(&e1));
+            b->ev.push_back({k, {e0, e1}});
+        }
+        auto& slot = b->ev[b->ev_used++];
+        slot.first = k;
+        HIPCHECK(hipEventRecord(slot.second.first, b->topo->stream));
+        stop = slot.second.second;
+    }
+    ~KTimer() {
+        if (stop) (void)hipEventRecord(stop, b->topo->stream);
+    }
+};
+
+void collect_timing(p3smi_batch* b) {
+    if (!b->timing) return;
+    HIPCHECK(hipStreamSynchronize(b->topo->stream));
+    for (size_t i = 0; i < b->ev_used; ++i) {
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, b->ev[i].second.first, b->ev[i].second.second));
+        switch (b->ev[i].first) {
+            case 0: b->stats.ms_eval += ms; b->stats.n_eval++; break;
+            case 1: b->stats.ms_lu += ms; b->stats.n_lu++; break;
+            case 2: b->stats.ms_step += ms; b->stats.n_step++; break;
+            case 3: b->stats.ms_decide += ms; b->stats.n_decide++; break;
+            default: b->stats.ms_other += ms; break;
+        }
+    }
+    b->ev_used = 0;
+}
+
+dim3 grid_items(int items, int S) { return dim3((unsigned)(S / 64), (unsigned)((items + 3) / 4)); }
+
+void launch_eval(p3smi_batch* b, int run_mask, int j_mask) {
+    const p3smi::Topology& h = b->topo->host;
+    const DevTopo& d = b->topo->dev;
+    KTimer tm(b, 0);
+    dim3 grid((unsigned)b->nparts, (unsigned)(b->S / 64));
+    const double* G = b->per_case_y ? b->Yg.p : d.G.p;
+    const double* Bv = b->per_case_y ? b->Yb.p : d.B.p;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, dim3(64 * EVAL_WAVES), 0, b->topo->stream, d.Yp.p, d.Yj.p, d.Ydiag.p, G, Bv,
+                           b->Vm.p, b->cs.p, b->sn.p, b->Ssp.p, b->s_stride, d.row_p.p, d.row_q.p,
+                           b->has_pin ? b->pin.p : nullptr, b->status.p, run_mask, j_mask, d.posJ.p, b->Jx.p, b->R.p,
+                           b->part.p, h.n, b->S);
+    };
+    if (b->per_case_y && b->has_pin) go(k_eval<true, true>);
+    else if (b->per_case_y) go(k_eval<true, false>);
+    else if (b->has_pin) go(k_eval<false, true>);
+    else go(k_eval<false, false>);
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_lu(p3smi_batch* b, int it) {
+    const p3smi::Topology& h = b->topo->host;
+    const DevTopo& d = b->topo->dev;
+    const int ns = h.sched.nslots;
+    KTimer tm(b, 1);
+    dim3 grid((unsigned)(b->S / 64));
+    const double rcond_min = REUSE_RCOND_RATIO * h.rcond0;
+    auto go = [&](auto kern, int nsl) {
+        hipLaunchKernelGGL(kern, grid, dim3(64 * nsl), 0, b->topo->stream, d.ptrA.p, d.taskA.p, d.ptrB.p, d.taskB.p,
+                           d.diagB.p, d.pairs.p, d.ptrF.p, d.fwd.p, d.fent.p, d.ptrW.p, d.bwd.p, d.Li.p, b->Jx.p,
+                           b->fac.p, b->R.p, b->x.p, b->status.p, b->iters.p, b->pstat.p, rcond_min, h.sched.nlev,
+                           h.sched.nblev, b->S, it);
+    };
+    switch (ns) {
+        case 1: go(k_lu<1>, 1); break;
+        case 2: go(k_lu<2>, 2); break;
+        case 4: go(k_lu<4>, 4); break;
+        case 8: go(k_lu<8>, 8); break;
+        case 16: go(k_lu<16>, 16); break;
+        default: throw Error(P3SMI_E_ARG, "n_slots must be 1, 2, 4, 8 or 16");
+    }
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_step(p3smi_batch* b, int mask, int full) {
+    const p3smi::Topology& h = b->topo->host;
+    const DevTopo& d = b->topo->dev;
+    KTimer tm(b, 2);
+    hipLaunchKernelGGL(k_step, grid_items(h.n, b->S), dim3(256), 0, b->topo->stream, b->x.p, d.xva.p, d.xvm.p,
+                       b->status.p, mask, full, b->alpha.p, b->Vm.p, b->Va.p, b->cs.p, b->sn.p, b->Vm0.p, b->Va0.p,
+                       h.n, b->S);
+    HIPCHECK(hipGetLastError());
+}
+
+const int* launch_decide(p3smi_batch* b, double tol, int ls, int phase, int it, int max_iter) {
+    KTimer tm(b, 3);
+    hipStream_t st = b->topo->stream;
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->counts.p, NSTAT);
+    hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->part.p, b->nparts,
+                       b->nrm.p, b->alpha.p, b->best_alpha.p, b->best_nrm.p, b->trial.p, b->status.p, b->iters.p,
+                       b->counts.p, tol, ls, phase, it, max_iter, b->S);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return b->h_counts;
+}
+
+void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int precision) {
+    if (!b->uploaded) throw Error(P3SMI_E_ARG, "batch has no inputs (upload first)");
+    if (max_iter <= 0) throw Error(P3SMI_E_ARG, "max_iter must be > 0");
+    if (!(tol > 0.0)) throw Error(P3SMI_E_ARG, "tol must be > 0");
+    if (precision != P3SMI_PREC_DOUBLE && precision != P3SMI_PREC_MIXED)
+        throw Error(P3SMI_E_ARG, "precision must be 0 (double) or 1 (mixed)");
+    const p3smi::Topology& h = b->topo->host;
+    hipStream_t st = b->topo->stream;
+    std::memset(&b->stats, 0, sizeof b->stats);
+    b->stats.cases = b->B;
+    {
+        KTimer tm(b, 4);
+        hipLaunchKernelGGL(k_status_init, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p, b->B,
+                           b->S);
+        hipLaunchKernelGGL(k_init, grid_items(h.n, b->S), dim3(256), 0, st, b->V0.p, b->v0_stride, b->Vm.p, b->Va.p,
+                           b->cs.p, b->sn.p, h.n, b->S, b->B);
+        HIPCHECK(hipGetLastError());
+    }
+    const int A = 1 << ST_ACTIVE, BT = 1 << ST_BACKTRACK, NJ = 1 << ST_NEEDJ;
+    launch_eval(b, A, A);
+    const int* cnt = launch_decide(b, tol, line_search, 0, 0, max_iter);
+    int it = 0;
+    while (it < max_iter && cnt[ST_ACTIVE] > 0) {
+        ++it;
+        b->stats.newton_passes++;
+        b->stats.double_factorizations += cnt[ST_ACTIVE];
+        launch_lu(b, it);
+        launch_step(b, A, 1);
+        launch_eval(b, A, A);
+        cnt = launch_decide(b, tol, line_search, 1, it, max_iter);
+        while (cnt[ST_BACKTRACK] > 0) {
+            b->stats.line_search_passes++;
+            launch_step(b, BT, 0);
+            launch_eval(b, BT, 0);
+            cnt = launch_decide(b, tol, line_search, 2, it, max_iter);
+        }
+        if (cnt[ST_NEEDJ] > 0) {
+            launch_step(b, NJ, 0);
+            launch_eval(b, NJ, NJ);
+            cnt = launch_decide(b, tol, line_search, 3, it, max_iter);
+        }
+    }
+    b->stats.failed_factorizations = cnt[ST_FAILED];
+    collect_timing(b);
+}
+
+void download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* converged, double* mismatch) {
+    const p3smi::Topology& h = b->topo->host;
+    hipStream_t st = b->topo->stream;
+    b->Vout.alloc((size_t)2 * h.n * b->B);
+    hipLaunchKernelGGL(k_finalize, grid_items(h.n, b->S), dim3(256), 0, st, b->Vm.p, b->cs.p, b->sn.p, b->Vout.p, h.n,
+                       b->S, b->B);
+    HIPCHECK(hipGetLastError());
+    std::vector<int> status(b->B), it(b->B);
+    if (V_out) HIPCHECK(hipMemcpyAsync(V_out, b->Vout.p, (size_t)2 * h.n * b->B * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(status.data(), b->status.p, b->B * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(it.data(), b->iters.p, b->B * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (mismatch) HIPCHECK(hipMemcpyAsync(mismatch, b->nrm.p, b->B * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int c = 0; c < b->B; ++c) {
+        if (iterations) iterations[c] = it[c];
+        if (converged) converged[c] = status[c] == ST_CONVERGED;
+    }
+}
+
+void check_v0(const double* V0, size_t rows, size_t cols, const char* who) {
+    for (size_t i = 0; i < rows; ++i)
+        for (size_t c = 0; c < cols; ++c) {
+            const double re = V0[2 * (i * cols + c)], im = V0[2 * (i * cols + c) + 1];
+            if (!std::isfinite(re) || !std::isfinite(im))
+                throw Error(P3SMI_E_ARG, std::string(who) + "invalid initial voltage at bus " + std::to_string(i) +
+                                             ", step " + std::to_string(c));
+            const double vm = std::hypot(re, im);
+            if (vm <= 0.0 || vm > 10.0)
+                throw Error(P3SMI_E_ARG, std::string(who) + "invalid voltage magnitude " + std::to_string(vm) +
+                                             " at bus " + std::to_string(i));
+        }
+}
+
+// Upload a column block [c0, c0 + cols) of a (rows, ld) host array of `elem` bytes into a
+// device (rows, cols) array.
+void upload_cols(void* dst, const void* src, size_t rows, size_t ld, size_t c0, size_t cols, size_t elem,
+                 hipStream_t st) {
+    HIPCHECK(hipMemcpy2DAsync(dst, cols * elem, (const char*)src + c0 * elem, ld * elem, cols * elem, rows,
+                              hipMemcpyHostToDevice, st));
+}
+
+void download_cols(void* dst, const void* src, size_t rows, size_t ld, size_t c0, size_t cols, size_t elem,
+                   hipStream_t st) {
+    HIPCHECK(hipMemcpy2DAsync((char*)dst + c0 * elem, ld * elem, src, cols * elem, cols * elem, rows,
+                              hipMemcpyDeviceToHost, st));
+}
+
+size_t bytes_per_case(const p3smi::Topology& h, bool per_case_y, bool pin, bool per_case_s) {
+    size_t b = (size_t)h.n * 6 * 8 + (size_t)h.nnzJ * 8 + (size_t)h.nnz_lu * 8 + (size_t)h.m * 16;
+    b += (size_t)eval_nparts(h.n) * 8 + 64;
+    b += (size_t)h.n * 16 * 2;  // V0 / V out
+    if (per_case_s) b += (size_t)h.n * 16;
+    if (per_case_y) b += (size_t)h.nnzY * 16 * 2;  // G, B and the upload staging
+    if (pin) b += h.n;
+    return b;
+}
+
+int chunk_cases(p3smi_topology* t, int total, bool per_case_y, bool pin, bool per_case_s) {
+    size_t freeb = 0, totb = 0;
+    HIPCHECK(hipMemGetInfo(&freeb, &totb));
+    const size_t per = bytes_per_case(t->host, per_case_y, pin, per_case_s);
+    size_t cap = (size_t)(0.6 * (double)freeb) / per;
+    if (t->cached) cap += (size_t)t->cached->S;  // its memory is reusable
+    if (cap < 64) cap = 64;
+    if ((size_t)total <= cap) return total;
+    return (int)(cap / 64 * 64);
+}
+
+p3smi_batch* cached_batch(p3smi_topology* t, int cases) {
+    if (!t->cached || t->cached->B != cases) {
+        t->cached.reset(new p3smi_batch());
+        batch_alloc(t->cached.get(), t, cases);
+    }
+    return t->cached.get();
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return P3SMI_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return P3SMI_E_MEMORY;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return P3SMI_E_ARG;
+    }
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char* p3smi_last_error(void) { return g_err.c_str(); }
+int p3smi_version(void) { return 1; }
+
+int p3smi_device_count(int32_t* count) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return P3SMI_OK;
+}
+
+int p3smi_topology_create(const int32_t* Yp, const int32_t* Yj, const double* Yx, int32_t n, int32_t nnz,
+                          const int32_t* pv, int32_t npv, const int32_t* pq, int32_t npq, int32_t ordering,
+                          int32_t n_slots, p3smi_topology** out) {
+    return guarded([&] {
+        if (!out) throw Error(P3SMI_E_ARG, "out is NULL");
+        *out = nullptr;
+        if (ordering != 0 && ordering != 2) throw Error(P3SMI_E_ARG, "ordering must be 0 (minimum degree) or 2 (natural)");
+        if (n_slots == 0) n_slots = 8;
+        if (n_slots != 1 && n_slots != 2 && n_slots != 4 && n_slots != 8 && n_slots != 16)
+            throw Error(P3SMI_E_ARG, "n_slots must be 1, 2, 4, 8 or 16");
+        std::unique_ptr<p3smi_topology> t(new p3smi_topology());
+        p3smi::build_topology(t->host, Yp, Yj, Yx, n, nnz, pv, npv, pq, npq, ordering, n_slots);
+        *out = t.release();
+    });
+}
+
+void p3smi_topology_free(p3smi_topology* t) { delete t; }
+
+int p3smi_topology_info(const p3smi_topology* t, p3smi_info* o) {
+    return guarded([&] {
+        const p3smi::Topology& h = t->host;
+        o->n = h.n; o->m = h.m; o->npv = h.npv; o->npq = h.npq; o->nnz_y = h.nnzY; o->nnz_j = h.nnzJ;
+        o->nnz_lu = h.nnz_lu; o->n_pairs = h.n_pairs; o->n_levels = h.sched.nlev; o->n_blevels = h.sched.nblev;
+        o->n_slots = h.sched.nslots; o->ordering = h.ordering; o->rcond0 = h.rcond0;
+    });
+}
+
+int p3smi_update_y(p3smi_topology* t, const double* Yx, int32_t nnz) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(t->mu);
+        if (nnz != t->host.nnzY) throw Error(P3SMI_E_ARG, "update_Y: nnz mismatch (topology changed; rebuild Solver)");
+        p3smi::update_values(t->host, Yx);
+        if (t->dev.ready) {
+            t->dev.G.upload(t->host.G.data(), t->host.G.size(), t->stream);
+            t->dev.B.upload(t->host.B.data(), t->host.B.size(), t->stream);
+            HIPCHECK(hipStreamSynchronize(t->stream));
+        }
+    });
+}
+
+int p3smi_host_jacobian(const p3smi_topology* t, const double* S, const double* V, int32_t* Jp, int32_t* Jj,
+                        double* Jx, double* R) {
+    return guarded([&] {
+        const p3smi::Topology& h = t->host;
+        std::vector<double> Vm(h.n), Va(h.n);
+        for (int i = 0; i < h.n; ++i) { Vm[i] = std::hypot(V[2 * i], V[2 * i + 1]); Va[i] = std::atan2(V[2 * i + 1], V[2 * i]); }
+        p3smi::host_eval(h, Vm.data(), Va.data(), nullptr, nullptr, S, nullptr, R, Jx);
+        std::copy(h.Jp.begin(), h.Jp.end(), Jp);
+        std::copy(h.Jj.begin(), h.Jj.end(), Jj);
+    });
+}
+
+int p3smi_host_factor_solve(const p3smi_topology* t, const double* Jx, const double* R, double* x, double* pivots) {
+    return guarded([&] {
+        const p3smi::Topology& h = t->host;
+        std::vector<double> fac(h.nnz_lu), y(h.m);
+        p3smi::host_factor(h, Jx, fac.data(), &pivots[0], &pivots[1], &pivots[2]);
+        p3smi::host_solve(h, fac.data(), R, y.data());
+        for (int p = 0; p < h.m; ++p) x[h.perm[p]] = y[p];
+    });
+}
+
+int p3smi_host_maps(const p3smi_topology* t, int32_t* row_p, int32_t* row_q, int32_t* perm) {
+    return guarded([&] {
+        const p3smi::Topology& h = t->host;
+        if (row_p) std::copy(h.row_p.begin(), h.row_p.end(), row_p);
+        if (row_q) std::copy(h.row_q.begin(), h.row_q.end(), row_q);
+        if (perm) std::copy(h.perm.begin(), h.perm.end(), perm);
+    });
+}
+
+int p3smi_batch_create(p3smi_topology* t, int32_t cases, p3smi_batch** out) {
+    return guarded([&] {
+        if (cases <= 0) throw Error(P3SMI_E_ARG, "cases must be > 0");
+        std::unique_ptr<p3smi_batch> b(new p3smi_batch());
+        batch_alloc(b.get(), t, cases);
+        *out = b.release();
+    });
+}
+
+void p3smi_batch_free(p3smi_batch* b) { delete b; }
+
+int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_t v0_per_case) {
+    return guarded([&] {
+        const p3smi::Topology& h = b->topo->host;
+        hipStream_t st = b->topo->stream;
+        check_v0(V0, h.n, v0_per_case ? b->B : 1, "solve_batch: ");
+        b->Ssp.upload(S, (size_t)2 * h.n * b->B, st);
+        b->s_stride = b->B;
+        b->V0.upload(V0, (size_t)2 * h.n * (v0_per_case ? b->B : 1), st);
+        b->v0_stride = v0_per_case ? b->B : 0;
+        b->per_case_y = false;
+        b->has_pin = false;
+        HIPCHECK(hipStreamSynchronize(st));
+        b->uploaded = true;
+    });
+}
+
+int p3smi_batch_solve(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_search, int32_t precision) {
+    return guarded([&] { run_newton(b, max_iter, tol, line_search, precision); });
+}
+
+int p3smi_batch_download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* converged, double* mismatch) {
+    return guarded([&] { download(b, V_out, iterations, converged, mismatch); });
+}
+
+int p3smi_batch_stats(const p3smi_batch* b, p3smi_stats* out) {
+    *out = b->stats;
+    return P3SMI_OK;
+}
+
+int p3smi_batch_set_timing(p3smi_batch* b, int32_t on) {
+    b->timing = on != 0;
+    return P3SMI_OK;
+}
+
+// Algorithmic bytes per case and pass: every array a kernel must touch, touched once.
+int p3smi_batch_bytes_per_case(const p3smi_batch* b, double* out4) {
+    const p3smi::Topology& h = b->topo->host;
+    const double n = h.n, m = h.m, nnzJ = h.nnzJ, lu = (double)h.nnz_lu;
+    // eval: Vm, cs, sn (24 n) + S (16 n) + J write (8 nnzJ) + R write (8 m) [+ per-case Y 16 nnzY]
+    out4[0] = 24 * n + (b->s_stride ? 16 * n : 0) + 8 * nnzJ + 8 * m + (b->per_case_y ? 16.0 * h.nnzY : 0.0);
+    // lu: J read (8 nnzJ) + factor write (8 nnz_lu) + R read (8 m) + x write/read (16 m)
+    out4[1] = 8 * nnzJ + 8 * lu + 8 * m + 16 * m;
+    // step: x read (8 m) + Vm, Va read/write + cs/sn write + base write (8 n * 8)
+    out4[2] = 8 * m + 64 * n;
+    out4[3] = 64;
+    return P3SMI_OK;
+}
+
+static void solve_chunked(p3smi_topology* t, int T, bool per_case_y, bool pin_given, bool per_case_s,
+                          const std::function<void(p3smi_batch*, int, int)>& upload, int max_iter, double tol,
+                          int line_search, int precision, double* V_out, int32_t* iters, uint8_t* conv,
+                          double* mismatch) {
+    const p3smi::Topology& h = t->host;
+    ensure_device(t);
+    const int cs = chunk_cases(t, T, per_case_y, pin_given, per_case_s);
+    for (int c0 = 0; c0 < T; c0 += cs) {
+        const int nc = std::min(cs, T - c0);
+        p3smi_batch* b = cached_batch(t, nc);
+        upload(b, c0, nc);
+        b->uploaded = true;
+        run_newton(b, max_iter, tol, line_search, precision);
+        std::vector<double> Vc(V_out && nc != T ? (size_t)2 * h.n * nc : 0);
+        download(b, V_out ? (nc == T ? V_out : Vc.data()) : nullptr, iters ? iters + c0 : nullptr,
+                 conv ? conv + c0 : nullptr, mismatch ? mismatch + c0 : nullptr);
+        if (V_out && nc != T)
+            for (int i = 0; i < h.n; ++i)
+                std::memcpy(V_out + 2 * ((size_t)i * T + c0), Vc.data() + 2 * (size_t)i * nc, (size_t)nc * 16);
+    }
+}
+
+int p3smi_solve_batch(p3smi_topology* t, const double* S, const double* V0, int32_t v0_per_case, int32_t T,
+                      int32_t max_iter, double tol, int32_t line_search, int32_t precision, double* V_out,
+                      int32_t* iterations, uint8_t* converged, double* mismatch) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(t->mu);
+        const p3smi::Topology& h = t->host;
+        if (T <= 0) throw Error(P3SMI_E_ARG, "solve_batch: T (time steps) must be > 0");
+        if (max_iter <= 0) throw Error(P3SMI_E_ARG, "solve_batch: max_iter must be > 0");
+        if (!(tol > 0.0)) throw Error(P3SMI_E_ARG, "solve_batch: tol must be > 0");
+        check_v0(V0, h.n, v0_per_case ? T : 1, "solve_batch: ");
+        auto up = [&](p3smi_batch* b, int c0, int nc) {
+            hipStream_t st = t->stream;
+            b->Ssp.alloc((size_t)2 * h.n * nc);
+            upload_cols(b->Ssp.p, S, h.n, T, c0, nc, 16, st);
+            b->s_stride = nc;
+            if (v0_per_case) {
+                b->V0.alloc((size_t)2 * h.n * nc);
+                upload_cols(b->V0.p, V0, h.n, T, c0, nc, 16, st);
+                b->v0_stride = nc;
+            } else {
+                b->V0.upload(V0, (size_t)2 * h.n, st);
+                b->v0_stride = 0;
+            }
+            b->per_case_y = false;
+            b->has_pin = false;
+        };
+        solve_chunked(t, T, false, false, true, up, max_iter, tol, line_search, precision, V_out, iterations, converged,
+                      mismatch);
+    });
+}
+
+static void contingency_common(p3smi_topology* t, const double* S, const double* V0, const uint8_t* pin, int L,
+                               const std::function<void(p3smi_batch*, int, int)>& upload_y, int max_iter, double tol,
+                               int line_search, int precision, double* V_out, int32_t* iters, uint8_t* conv,
+                               double* mismatch) {
+    const p3smi::Topology& h = t->host;
+    if (L <= 0) throw Error(P3SMI_E_ARG, "solve_batch_contingency: L (cases) must be > 0");
+    if (max_iter <= 0) throw Error(P3SMI_E_ARG, "solve_batch_contingency: max_iter must be > 0");
+    if (!(tol > 0.0)) throw Error(P3SMI_E_ARG, "solve_batch_contingency: tol must be > 0");
+    auto up = [&](p3smi_batch* b, int c0, int nc) {
+        hipStream_t st = t->stream;
+        b->Ssp.upload(S, (size_t)2 * h.n, st);
+        b->s_stride = 0;
+        b->V0.alloc((size_t)2 * h.n * nc);
+        upload_cols(b->V0.p, V0, h.n, L, c0, nc, 16, st);
+        b->v0_stride = nc;
+        b->Yg.alloc((size_t)h.nnzY * b->S);
+        b->Yb.alloc((size_t)h.nnzY * b->S);
+        upload_y(b, c0, nc);
+        b->per_case_y = true;
+        b->has_pin = pin != nullptr;
+        if (pin) {
+            // pin (n, L) -> device (n, S): zero the padding columns
+            b->pin.alloc((size_t)h.n * b->S);
+            HIPCHECK(hipMemsetAsync(b->pin.p, 0, (size_t)h.n * b->S, st));
+            HIPCHECK(hipMemcpy2DAsync(b->pin.p, b->S, pin + c0, L, nc, h.n, hipMemcpyHostToDevice, st));
+        }
+    };
+    solve_chunked(t, L, true, pin != nullptr, false, up, max_iter, tol, line_search, precision, V_out, iters, conv,
+                  mismatch);
+}
+
+int p3smi_solve_batch_contingency(p3smi_topology* t, const double* Yx_mag, const double* Yx_ang, const double* S,
+                                  const double* V0, const uint8_t* pin, int32_t L, int32_t max_iter, double tol,
+                                  int32_t line_search, int32_t precision, double* V_out, int32_t* iterations,
+                                  uint8_t* converged, double* mismatch) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(t->mu);
+        const p3smi::Topology& h = t->host;
+        auto upy = [&](p3smi_batch* b, int c0, int nc) {
+            hipStream_t st = t->stream;
+            DevBuf<double> mag, ang;
+            mag.alloc((size_t)h.nnzY * nc);
+            ang.alloc((size_t)h.nnzY * nc);
+            upload_cols(mag.p, Yx_mag, h.nnzY, L, c0, nc, 8, st);
+            upload_cols(ang.p, Yx_ang, h.nnzY, L, c0, nc, 8, st);
+            hipLaunchKernelGGL(k_y_polar, grid_items(h.nnzY, b->S), dim3(256), 0, st, mag.p, ang.p, b->Yg.p, b->Yb.p,
+                               h.nnzY, nc, nc, b->S);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipStreamSynchronize(st));
+        };
+        contingency_common(t, S, V0, pin, L, upy, max_iter, tol, line_search, precision, V_out, iterations, converged,
+                           mismatch);
+    });
+}
+
+int p3smi_solve_batch_contingency_stamp(p3smi_topology* t, const int32_t* stamp_k, const int32_t* stamp_c,
+                                        const double* stamp_val, int64_t n_stamp, const double* S, const double* V0,
+                                        const uint8_t* pin, int32_t L, int32_t max_iter, double tol,
+                                        int32_t line_search, int32_t precision, double* V_out, int32_t* iterations,
+                                        uint8_t* converged, double* mismatch) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(t->mu);
+        const p3smi::Topology& h = t->host;
+        for (int64_t e = 0; e < n_stamp; ++e)
+            if (stamp_k[e] < 0 || stamp_k[e] >= h.nnzY || stamp_c[e] < 0 || stamp_c[e] >= L)
+                throw Error(P3SMI_E_ARG, "solve_batch_contingency_stamp: stamp index out of range");
+        DevBuf<int> dk, dc;
+        DevBuf<double> dv;
+        ensure_device(t);
+        dk.upload(stamp_k, (size_t)n_stamp, t->stream);
+        dc.upload(stamp_c, (size_t)n_stamp, t->stream);
+        dv.upload(stamp_val, (size_t)2 * n_stamp, t->stream);
+        auto upy = [&](p3smi_batch* b, int c0, int nc) {
+            hipStream_t st = t->stream;
+            hipLaunchKernelGGL(k_y_base, grid_items(h.nnzY, b->S), dim3(256), 0, st, t->dev.G.p, t->dev.B.p, b->Yg.p,
+                               b->Yb.p, h.nnzY, b->S);
+            if (n_stamp > 0)
+                hipLaunchKernelGGL(k_y_stamp, dim3((unsigned)((n_stamp + 255) / 256)), dim3(256), 0, st, dk.p, dc.p,
+                                   dv.p, b->Yg.p, b->Yb.p, (long long)n_stamp, c0, nc, b->S);
+            HIPCHECK(hipGetLastError());
+        };
+        contingency_common(t, S, V0, pin, L, upy, max_iter, tol, line_search, precision, V_out, iterations, converged,
+                           mismatch);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,525 @@
+// plan.cpp -- host-side topology, Jacobian pattern, ordering, symbolic LU and schedule.
+// See plan.hpp for the layout conventions.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <queue>
+#include <stdexcept>
+
+namespace p3smi {
+
+namespace {
+
+// ------------------------------------------------------------------ minimum degree
+// Greedy minimum degree on the explicit elimination graph of a symmetric pattern
+// (adjacency without the diagonal). Eliminating v turns its live neighbours into a
+// clique. Degree ties go to the lowest index (deterministic order).
+std::vector<int> minimum_degree(int m, std::vector<std::vector<int>> adj) {
+    std::vector<char> dead(m, 0);
+    std::vector<int> stamp(m, -1);
+    using Key = std::pair<int, int>;  // (degree, node)
+    std::priority_queue<Key, std::vector<Key>, std::greater<Key>> pq;
+    for (int i = 0; i < m; ++i) pq.push({(int)adj[i].size(), i});
+    std::vector<int> order;
+    order.reserve(m);
+    int tick = 0;
+    while ((int)order.size() < m) {
+        Key top = pq.top();
+        pq.pop();
+        int v = top.second;
+        if (dead[v] || top.first != (int)adj[v].size()) continue;  // stale entry
+        dead[v] = 1;
+        order.push_back(v);
+        std::vector<int> nb;
+        nb.swap(adj[v]);
+        for (int u : nb) {
+            if (dead[u]) continue;
+            ++tick;
+            auto& au = adj[u];
+            size_t w = 0;
+            for (size_t r = 0; r < au.size(); ++r) {
+                int x = au[r];
+                if (dead[x]) continue;
+                au[w++] = x;
+                stamp[x] = tick;
+            }
+            au.resize(w);
+            stamp[u] = tick;
+            for (int x : nb)
+                if (!dead[x] && stamp[x] != tick) {
+                    stamp[x] = tick;
+                    au.push_back(x);
+                }
+            pq.push({(int)au.size(), u});
+        }
+    }
+    return order;
+}
+
+// Longest-processing-time dealing: tasks (weights w) of one level go, heaviest first, to
+// the currently least-loaded of `ns` lists. `load` carries over between calls so that two
+// task kinds of one level share one balance. Returns the slot of each task.
+std::vector<int> lpt_deal(const std::vector<int64_t>& w, int ns, std::vector<int64_t>& load) {
+    std::vector<int> idx(w.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return w[a] > w[b]; });
+    using Key = std::pair<int64_t, int>;
+    std::priority_queue<Key, std::vector<Key>, std::greater<Key>> pq;
+    for (int s = 0; s < ns; ++s) pq.push({load[s], s});
+    std::vector<int> slot(w.size());
+    for (int i : idx) {
+        Key k = pq.top();
+        pq.pop();
+        slot[i] = k.second;
+        k.first += w[i];
+        load[k.second] = k.first;
+        pq.push(k);
+    }
+    return slot;
+}
+
+// Group items by (level, slot): returns ptr (nlev*ns + 1) and the item order.
+std::vector<int> group_by_slot(const std::vector<int>& level, const std::vector<int>& slot, int nlev, int ns,
+                               std::vector<int>& ptr) {
+    ptr.assign((size_t)nlev * ns + 1, 0);
+    for (size_t i = 0; i < level.size(); ++i) ptr[(size_t)level[i] * ns + slot[i] + 1]++;
+    for (size_t q = 1; q < ptr.size(); ++q) ptr[q] += ptr[q - 1];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1), order(level.size());
+    for (size_t i = 0; i < level.size(); ++i) order[fill[(size_t)level[i] * ns + slot[i]]++] = (int)i;
+    return order;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ topology
+static void build_polar(Topology& t, const int* Yp, const int* Yj, const double* Yx, int n, int nnzY, const int* pv,
+                        int npv, const int* pq, int npq) {
+    if (n <= 0 || nnzY <= 0) throw std::runtime_error("empty Ybus");
+    if (Yp[0] != 0 || Yp[n] != nnzY) throw std::runtime_error("Ybus CSR indptr does not match nnz");
+    t.n = n; t.nnzY = nnzY; t.npv = npv; t.npq = npq; t.npvpq = npv + npq; t.m = npv + 2 * npq;
+    t.Yp.assign(Yp, Yp + n + 1);
+    t.Yj.assign(Yj, Yj + nnzY);
+    for (int i = 0; i < n; ++i) {
+        if (Yp[i + 1] < Yp[i]) throw std::runtime_error("Ybus CSR indptr is not monotone");
+        for (int k = Yp[i]; k < Yp[i + 1]; ++k) {
+            if (Yj[k] < 0 || Yj[k] >= n) throw std::runtime_error("Ybus column index out of range");
+            if (k > Yp[i] && Yj[k] <= Yj[k - 1]) throw std::runtime_error("Ybus columns must be sorted and unique per row");
+        }
+    }
+    t.G.resize(nnzY); t.B.resize(nnzY);
+    for (int k = 0; k < nnzY; ++k) { t.G[k] = Yx[2 * k]; t.B[k] = Yx[2 * k + 1]; }
+    t.Ydiag.assign(n, -1);
+    for (int i = 0; i < n; ++i)
+        for (int k = Yp[i]; k < Yp[i + 1]; ++k)
+            if (Yj[k] == i) { t.Ydiag[i] = k; break; }
+    t.pv.assign(pv, pv + npv);
+    t.pq.assign(pq, pq + npq);
+    t.pvpq = t.pv;
+    t.pvpq.insert(t.pvpq.end(), t.pq.begin(), t.pq.end());
+    t.row_p.assign(n, -1);
+    t.row_q.assign(n, -1);
+    for (int r = 0; r < t.npvpq; ++r) {
+        int b = t.pvpq[r];
+        if (b < 0 || b >= n) throw std::runtime_error("pv/pq bus index out of range");
+        if (t.row_p[b] != -1) throw std::runtime_error("a bus appears twice in pv/pq");
+        t.row_p[b] = r;
+    }
+    for (int r = 0; r < npq; ++r) t.row_q[t.pq[r]] = t.npvpq + r;
+    for (int i = 0; i < n; ++i)
+        if (t.row_p[i] >= 0 && t.Ydiag[i] < 0) throw std::runtime_error("a pv/pq bus has no Ybus diagonal entry");
+}
+
+// Jacobian pattern, CSR by equation row. Row r < npvpq: P equation of bus pvpq[r];
+// row r >= npvpq: Q equation of bus pq[r - npvpq]. Column c < npvpq: Va of pvpq[c];
+// column c >= npvpq: Vm of pq[c - npvpq]. J(r, c) is structural iff Ybus(bus(r), bus(c))
+// is (p3s/cpp/nr_klu.cpp:218-229).
+static void build_jacobian_pattern(Topology& t) {
+    std::vector<std::vector<std::pair<int, int>>> rows(t.m);  // (column, 4k + block)
+    for (int i = 0; i < t.n; ++i) {
+        const int rp = t.row_p[i], rq = t.row_q[i];
+        if (rp < 0 && rq < 0) continue;
+        for (int k = t.Yp[i]; k < t.Yp[i + 1]; ++k) {
+            const int j = t.Yj[k];
+            const int cva = t.row_p[j], cvm = t.row_q[j];
+            if (rp >= 0 && cva >= 0) rows[rp].push_back({cva, 4 * k + 0});
+            if (rp >= 0 && cvm >= 0) rows[rp].push_back({cvm, 4 * k + 1});
+            if (rq >= 0 && cva >= 0) rows[rq].push_back({cva, 4 * k + 2});
+            if (rq >= 0 && cvm >= 0) rows[rq].push_back({cvm, 4 * k + 3});
+        }
+    }
+    t.Jp.assign(t.m + 1, 0);
+    for (int r = 0; r < t.m; ++r) {
+        std::sort(rows[r].begin(), rows[r].end());
+        t.Jp[r + 1] = t.Jp[r] + (int)rows[r].size();
+    }
+    t.nnzJ = t.Jp[t.m];
+    t.Jj.resize(t.nnzJ); t.Jk.resize(t.nnzJ); t.Jb.resize(t.nnzJ);
+    t.posJ.assign(4 * (size_t)t.nnzY, -1);
+    for (int r = 0; r < t.m; ++r)
+        for (size_t q = 0; q < rows[r].size(); ++q) {
+            const int e = t.Jp[r] + (int)q;
+            t.Jj[e] = rows[r][q].first;
+            t.Jk[e] = rows[r][q].second >> 2;
+            t.Jb[e] = rows[r][q].second & 3;
+            t.posJ[rows[r][q].second] = e;
+        }
+}
+
+// Symmetric static-pivot LU: order, elimination tree, strict column patterns S_p,
+// positions, operand pairs. The Jacobian pattern is structurally symmetric with a full
+// diagonal, so the pattern of U(p,:) is S_p as well.
+static void build_lu(Topology& t, int ordering) {
+    const int m = t.m;
+    std::vector<std::vector<int>> adj(m);
+    for (int r = 0; r < m; ++r)
+        for (int e = t.Jp[r]; e < t.Jp[r + 1]; ++e)
+            if (t.Jj[e] != r) adj[r].push_back(t.Jj[e]);
+    // symmetrise (the pattern already is; this guards malformed inputs)
+    for (int r = 0; r < m; ++r)
+        for (int c : adj[r]) if (!std::binary_search(t.Jj.begin() + t.Jp[c], t.Jj.begin() + t.Jp[c + 1], r))
+            throw std::runtime_error("Jacobian pattern is not structurally symmetric (Ybus pattern must be symmetric)");
+    t.ordering = ordering;
+    if (ordering == 2) {
+        t.perm.resize(m);
+        std::iota(t.perm.begin(), t.perm.end(), 0);
+    } else {
+        t.perm = minimum_degree(m, adj);
+    }
+    t.pinv.assign(m, -1);
+    for (int p = 0; p < m; ++p) t.pinv[t.perm[p]] = p;
+
+    // symbolic factorisation on the elimination tree: S_p = (higher neighbours of p)
+    // united with S_c \ {p} of every child c.
+    t.parent.assign(m, -1);
+    std::vector<int> head(m, -1), next(m, -1), mark(m, -1);
+    t.Lp.assign(m + 1, 0);
+    t.Li.clear();
+    for (int p = 0; p < m; ++p) {
+        const size_t start = t.Li.size();
+        mark[p] = p;
+        for (int c : adj[t.perm[p]]) {
+            int i = t.pinv[c];
+            if (i > p && mark[i] != p) { mark[i] = p; t.Li.push_back(i); }
+        }
+        for (int ch = head[p]; ch != -1; ch = next[ch])
+            for (int q = t.Lp[ch]; q < t.Lp[ch + 1]; ++q) {
+                int i = t.Li[q];
+                if (i > p && mark[i] != p) { mark[i] = p; t.Li.push_back(i); }
+            }
+        std::sort(t.Li.begin() + start, t.Li.end());
+        t.Lp[p + 1] = (int)t.Li.size();
+        if (t.Li.size() > start) {
+            const int par = t.Li[start];
+            t.parent[p] = par;
+            next[p] = head[par];
+            head[par] = p;
+        }
+    }
+    t.height.assign(m, 0);
+    for (int p = 0; p < m; ++p)
+        if (t.parent[p] >= 0) t.height[t.parent[p]] = std::max(t.height[t.parent[p]], t.height[p] + 1);
+    t.depth.assign(m, 0);
+    for (int p = m - 1; p >= 0; --p)
+        if (t.parent[p] >= 0) t.depth[p] = t.depth[t.parent[p]] + 1;
+
+    // positions
+    t.base.assign(m + 1, 0);
+    for (int p = 0; p < m; ++p) {
+        int64_t nxt = (int64_t)t.base[p] + 2 * (int64_t)(t.Lp[p + 1] - t.Lp[p]) + 1;
+        if (nxt >= std::numeric_limits<int>::max()) throw std::runtime_error("factor pattern too large for int32 positions");
+        t.base[p + 1] = (int)nxt;
+    }
+    t.nnz_lu = t.base[m];
+    // position of (a, b) in pivot space: diagonal, U(a, b) for a < b, L(a, b) for a > b
+    auto pos = [&](int a, int b) -> int {
+        if (a == b) return t.base[a];
+        const int lo = std::min(a, b), hi = std::max(a, b);
+        const int* beg = t.Li.data() + t.Lp[lo];
+        const int* end = t.Li.data() + t.Lp[lo + 1];
+        const int* it = std::lower_bound(beg, end, hi);
+        if (it == end || *it != hi) throw std::runtime_error("internal: entry outside the factor pattern");
+        const int r = (int)(it - beg);
+        const int cnt = t.Lp[lo + 1] - t.Lp[lo];
+        return a < b ? t.base[lo] + 1 + r : t.base[lo] + 1 + cnt + r;
+    };
+    t.apos.resize(t.nnzJ);
+    for (int r = 0; r < m; ++r)
+        for (int e = t.Jp[r]; e < t.Jp[r + 1]; ++e) t.apos[e] = pos(t.pinv[r], t.pinv[t.Jj[e]]);
+    t.xva.assign(t.n, -1);
+    t.xvm.assign(t.n, -1);
+    for (int i = 0; i < t.n; ++i) {
+        if (t.row_p[i] >= 0) t.xva[i] = t.pinv[t.row_p[i]];
+        if (t.row_q[i] >= 0) t.xvm[i] = t.pinv[t.row_q[i]];
+    }
+
+    // operand pairs, grouped by target position: pivot t contributes L(a,t) U(t,b) to
+    // every entry (a, b) with a, b in S_t
+    std::vector<int64_t> cnt(t.nnz_lu + 1, 0);
+    int64_t npairs = 0;
+    for (int p = 0; p < m; ++p) {
+        const int c = t.Lp[p + 1] - t.Lp[p];
+        npairs += (int64_t)c * c;
+    }
+    if (npairs >= std::numeric_limits<int>::max()) throw std::runtime_error("too many LU operand pairs for int32");
+    t.n_pairs = npairs;
+    std::vector<int> tgt((size_t)npairs), opa((size_t)npairs), opb((size_t)npairs);
+    int64_t q = 0;
+    for (int p = 0; p < m; ++p) {
+        const int c = t.Lp[p + 1] - t.Lp[p];
+        for (int ia = 0; ia < c; ++ia) {
+            const int a = t.Li[t.Lp[p] + ia];
+            for (int ib = 0; ib < c; ++ib) {
+                const int b = t.Li[t.Lp[p] + ib];
+                tgt[q] = pos(a, b);
+                opa[q] = t.base[p] + 1 + c + ia;  // L(a, p)
+                opb[q] = t.base[p] + 1 + ib;      // U(p, b)
+                ++q;
+            }
+        }
+    }
+    for (int64_t e = 0; e < npairs; ++e) cnt[tgt[e] + 1]++;
+    for (int64_t e = 0; e < t.nnz_lu; ++e) cnt[e + 1] += cnt[e];
+    t.pa.resize((size_t)npairs);
+    t.pb.resize((size_t)npairs);
+    {
+        std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+        // pivots ascending within a target: the order a left-looking sweep would use
+        for (int64_t e = 0; e < npairs; ++e) {
+            const int64_t d = fill[tgt[e]]++;
+            t.pa[d] = opa[e];
+            t.pb[d] = opb[e];
+        }
+    }
+    // forward-substitution rows: L(i, j) entries of row i
+    t.Fp.assign(m + 1, 0);
+    for (int p = 0; p < m; ++p)
+        for (int q2 = t.Lp[p]; q2 < t.Lp[p + 1]; ++q2) t.Fp[t.Li[q2] + 1]++;
+    for (int i = 0; i < m; ++i) t.Fp[i + 1] += t.Fp[i];
+    t.Fpos.resize(t.Fp[m]);
+    t.Fcol.resize(t.Fp[m]);
+    {
+        std::vector<int> fill(t.Fp.begin(), t.Fp.end() - 1);
+        for (int p = 0; p < m; ++p) {
+            const int c = t.Lp[p + 1] - t.Lp[p];
+            for (int r = 0; r < c; ++r) {
+                const int i = t.Li[t.Lp[p] + r];
+                const int d = fill[i]++;
+                t.Fpos[d] = t.base[p] + 1 + c + r;
+                t.Fcol[d] = p;
+            }
+        }
+    }
+    t.prange.assign(t.nnz_lu + 1, 0);
+    for (int64_t e = 0; e <= t.nnz_lu; ++e) t.prange[e] = (int)cnt[e];
+}
+
+void build_schedule(Topology& t, int nslots) {
+    if (nslots < 1) throw std::runtime_error("nslots must be >= 1");
+    const int m = t.m;
+    const std::vector<int>& prange = t.prange;
+    std::vector<int> asrc(t.nnz_lu, -1);
+    for (int e = 0; e < t.nnzJ; ++e) asrc[t.apos[e]] = e;
+
+    Schedule s;
+    s.nslots = nslots;
+    s.nlev = 0;
+    for (int p = 0; p < m; ++p) s.nlev = std::max(s.nlev, t.height[p] + 1);
+    s.nblev = 0;
+    for (int p = 0; p < m; ++p) s.nblev = std::max(s.nblev, t.depth[p] + 1);
+
+    // items per level: A = diagonal + U entries + forward rows (one balance), B = L entries
+    std::vector<std::vector<int>> levA(s.nlev), levB(s.nlev), levF(s.nlev), levW(s.nblev);
+    for (int p = 0; p < m; ++p) {
+        const int h = t.height[p];
+        const int c = t.Lp[p + 1] - t.Lp[p];
+        levA[h].push_back(t.base[p]);
+        for (int r = 0; r < c; ++r) levA[h].push_back(t.base[p] + 1 + r);
+        for (int r = 0; r < c; ++r) levB[h].push_back(t.base[p] + 1 + c + r);
+        levF[h].push_back(p);
+        levW[t.depth[p]].push_back(p);
+    }
+    std::vector<int> lA, sA, lB, sB, lF, sF, lW, sW;
+    std::vector<int> itemsA, itemsB, itemsF, itemsW;
+    for (int l = 0; l < s.nlev; ++l) {
+        std::vector<int64_t> load(nslots, 0);
+        std::vector<int64_t> w;
+        for (int d : levA[l]) w.push_back(2 + (prange[d + 1] - prange[d]));
+        for (int p : levF[l]) w.push_back(2 + (t.Fp[p + 1] - t.Fp[p]));
+        std::vector<int> slot = lpt_deal(w, nslots, load);
+        for (size_t i = 0; i < levA[l].size(); ++i) { itemsA.push_back(levA[l][i]); lA.push_back(l); sA.push_back(slot[i]); }
+        for (size_t i = 0; i < levF[l].size(); ++i) {
+            itemsF.push_back(levF[l][i]); lF.push_back(l); sF.push_back(slot[levA[l].size() + i]);
+        }
+        std::vector<int64_t> loadB(nslots, 0), wb;
+        for (int d : levB[l]) wb.push_back(2 + (prange[d + 1] - prange[d]));
+        std::vector<int> slotB = lpt_deal(wb, nslots, loadB);
+        for (size_t i = 0; i < levB[l].size(); ++i) { itemsB.push_back(levB[l][i]); lB.push_back(l); sB.push_back(slotB[i]); }
+    }
+    for (int l = 0; l < s.nblev; ++l) {
+        std::vector<int64_t> load(nslots, 0), w;
+        for (int p : levW[l]) w.push_back(2 + (t.Lp[p + 1] - t.Lp[p]));
+        std::vector<int> slot = lpt_deal(w, nslots, load);
+        for (size_t i = 0; i < levW[l].size(); ++i) { itemsW.push_back(levW[l][i]); lW.push_back(l); sW.push_back(slot[i]); }
+    }
+    // pivot of each factor position (for L entries: the diagonal they are scaled by)
+    std::vector<int> diag_of(t.nnz_lu, -1);
+    std::vector<char> is_diag(t.nnz_lu, 0);
+    for (int p = 0; p < m; ++p) {
+        const int c = t.Lp[p + 1] - t.Lp[p];
+        is_diag[t.base[p]] = 1;
+        for (int r = 0; r < c; ++r) diag_of[t.base[p] + 1 + c + r] = t.base[p];
+    }
+    auto ordA = group_by_slot(lA, sA, s.nlev, nslots, s.ptrA);
+    for (int i : ordA) {
+        const int d = itemsA[i];
+        s.taskA.push_back({d, asrc[d], prange[d], prange[d + 1]});
+        s.isdiagA.push_back((unsigned char)is_diag[d]);
+    }
+    auto ordB = group_by_slot(lB, sB, s.nlev, nslots, s.ptrB);
+    for (int i : ordB) {
+        const int d = itemsB[i];
+        s.taskB.push_back({d, asrc[d], prange[d], prange[d + 1]});
+        s.diagB.push_back(diag_of[d]);
+    }
+    auto ordF = group_by_slot(lF, sF, s.nlev, nslots, s.ptrF);
+    for (int i : ordF) {
+        const int p = itemsF[i];
+        s.fwd.push_back({p, t.perm[p], t.Fp[p], t.Fp[p + 1]});
+    }
+    auto ordW = group_by_slot(lW, sW, s.nblev, nslots, s.ptrW);
+    for (int i : ordW) {
+        const int p = itemsW[i];
+        s.bwd.push_back({p, t.base[p], t.Lp[p], t.Lp[p + 1]});
+    }
+    t.sched = std::move(s);
+}
+
+// ------------------------------------------------------------------ host numerics
+// Jacobian and residual of one system at (Vm, Va), rectangular form: with the unit phasor
+// (cos Va, sin Va) of every bus, Ym cos(delta) = G cu + B su and Ym sin(delta) = G su - B cu
+// where cu = cos(Va_i - Va_k), su = sin(Va_i - Va_k) -- the same quantities as
+// p3s/cpp/nr_klu.cpp:344-378 without a sincos per nonzero.
+void host_eval(const Topology& t, const double* Vm, const double* Va, const double* G, const double* B,
+               const double* Ssp, const unsigned char* pin, double* R, double* J) {
+    if (!G) G = t.G.data();
+    if (!B) B = t.B.data();
+    std::vector<double> cs(t.n), sn(t.n);
+    for (int i = 0; i < t.n; ++i) { cs[i] = std::cos(Va[i]); sn[i] = std::sin(Va[i]); }
+    for (int i = 0; i < t.n; ++i) {
+        const double vi = Vm[i], ci = cs[i], si = sn[i];
+        const bool pinned = pin && pin[i];
+        double p = 0, q = 0, dPVa = 0, dQVa = 0, dPVm = 0, dQVm = 0;
+        for (int k = t.Yp[i]; k < t.Yp[i + 1]; ++k) {
+            const int j = t.Yj[k];
+            const double g = G[k], b = B[k];
+            if (j != i) {
+                const double vk = Vm[j];
+                const double cu = ci * cs[j] + si * sn[j];
+                const double su = si * cs[j] - ci * sn[j];
+                const double gc = g * cu + b * su, gs = g * su - b * cu;
+                const double a = vi * gc * vk, bt = vi * gs * vk;
+                p += a; q += bt;
+                dPVa -= bt; dQVa += a; dPVm += vk * gc; dQVm += vk * gs;
+                if (J) {
+                    const double vals[4] = {bt, vi * gc, -a, vi * gs};
+                    for (int blk = 0; blk < 4; ++blk) {
+                        int e = t.posJ[4 * (size_t)k + blk];
+                        if (e >= 0) J[e] = pinned ? 0.0 : vals[blk];
+                    }
+                }
+            } else {
+                p += vi * vi * g; q -= vi * vi * b;
+                dPVm += 2.0 * vi * g; dQVm -= 2.0 * vi * b;
+            }
+        }
+        const int d = t.Ydiag[i];
+        if (J && d >= 0) {
+            const double vals[4] = {pinned ? 1.0 : dPVa, pinned ? 0.0 : dPVm, pinned ? 0.0 : dQVa, pinned ? 1.0 : dQVm};
+            for (int blk = 0; blk < 4; ++blk) {
+                int e = t.posJ[4 * (size_t)d + blk];
+                if (e >= 0) J[e] = vals[blk];
+            }
+        }
+        if (t.row_p[i] >= 0) R[t.row_p[i]] = pinned ? 0.0 : Ssp[2 * i] - p;
+        if (t.row_q[i] >= 0) R[t.row_q[i]] = pinned ? 0.0 : Ssp[2 * i + 1] - q;
+    }
+}
+
+void host_factor(const Topology& t, const double* J, double* fac, double* pmin, double* pmax, double* psum) {
+    const Schedule& s = t.sched;
+    double lo = std::numeric_limits<double>::infinity(), hi = 0.0, sum = 0.0;
+    auto run = [&](const Task& tk) {
+        double acc = tk.asrc >= 0 ? J[tk.asrc] : 0.0;
+        for (int e = tk.pbeg; e < tk.pend; ++e) acc -= fac[t.pa[e]] * fac[t.pb[e]];
+        return acc;
+    };
+    for (int l = 0; l < s.nlev; ++l) {
+        for (int q = s.ptrA[(size_t)l * s.nslots]; q < s.ptrA[(size_t)(l + 1) * s.nslots]; ++q) {
+            const Task& tk = s.taskA[q];
+            double acc = run(tk);
+            if (s.isdiagA[q]) {
+                const double a = std::fabs(acc);
+                lo = std::min(lo, a); hi = std::max(hi, a); sum += a;
+                acc = 1.0 / acc;
+            }
+            fac[tk.dst] = acc;
+        }
+        for (int q = s.ptrB[(size_t)l * s.nslots]; q < s.ptrB[(size_t)(l + 1) * s.nslots]; ++q) {
+            const Task& tk = s.taskB[q];
+            fac[tk.dst] = run(tk) * fac[s.diagB[q]];
+        }
+    }
+    *pmin = lo; *pmax = hi; *psum = sum;
+}
+
+void host_solve(const Topology& t, const double* fac, const double* R, double* x) {
+    const Schedule& s = t.sched;
+    for (int l = 0; l < s.nlev; ++l)
+        for (int q = s.ptrF[(size_t)l * s.nslots]; q < s.ptrF[(size_t)(l + 1) * s.nslots]; ++q) {
+            const FwdRow& f = s.fwd[q];
+            double acc = R[f.rsrc];
+            for (int e = f.ebeg; e < f.eend; ++e) acc -= fac[t.Fpos[e]] * x[t.Fcol[e]];
+            x[f.row] = acc;
+        }
+    for (int l = 0; l < s.nblev; ++l)
+        for (int q = s.ptrW[(size_t)l * s.nslots]; q < s.ptrW[(size_t)(l + 1) * s.nslots]; ++q) {
+            const BwdRow& b = s.bwd[q];
+            double acc = x[b.row];
+            for (int e = b.lbeg; e < b.lend; ++e) acc -= fac[b.base + 1 + (e - b.lbeg)] * x[t.Li[e]];
+            x[b.row] = acc * fac[b.base];
+        }
+}
+
+// ------------------------------------------------------------------ entry points
+static double reference_pivot_ratio(const Topology& t) {
+    std::vector<double> Vm(t.n, 1.0), Va(t.n, 0.0), S(2 * (size_t)t.n, 0.0), R(t.m), J(t.nnzJ), fac(t.nnz_lu);
+    host_eval(t, Vm.data(), Va.data(), nullptr, nullptr, S.data(), nullptr, R.data(), J.data());
+    double lo, hi, sum;
+    host_factor(t, J.data(), fac.data(), &lo, &hi, &sum);
+    if (!std::isfinite(sum) || !(lo > 0.0) || !(hi > 0.0))
+        throw std::runtime_error(
+            "static-pivot LU: the Jacobian has a zero or non-finite pivot at flat start; the frozen pivot order "
+            "cannot be used for this grid");
+    return lo / hi;
+}
+
+void build_topology(Topology& t, const int* Yp, const int* Yj, const double* Yx, int n, int nnzY, const int* pv,
+                    int npv, const int* pq, int npq, int ordering, int nslots) {
+    build_polar(t, Yp, Yj, Yx, n, nnzY, pv, npv, pq, npq);
+    if (t.m == 0) throw std::runtime_error("no pv/pq buses: nothing to solve");
+    build_jacobian_pattern(t);
+    build_lu(t, ordering);
+    build_schedule(t, nslots);
+    t.rcond0 = reference_pivot_ratio(t);
+}
+
+void update_values(Topology& t, const double* Yx) {
+    for (int k = 0; k < t.nnzY; ++k) { t.G[k] = Yx[2 * k]; t.B[k] = Yx[2 * k + 1]; }
+    t.rcond0 = reference_pivot_ratio(t);
+}
+
+}  // namespace p3smi

@@ -1,0 +1,105 @@
+// plan.hpp -- host-side, once-per-topology setup of the batched polar Newton engine.
+//
+// Everything here depends only on the Ybus sparsity pattern and the pv/pq split, and is
+// built once per topology on the host (the reference does the same setup in
+// p3s/cpp/nr_klu.cpp:732-759 and p3s/cuda/polar_topology.py, lean_lu_plan.py):
+//
+//   * the polar topology: Ybus CSR, diagonal index, the P-row / Q-row of every bus;
+//   * the Jacobian pattern (CSR by equation) and the map from each Ybus nonzero's four
+//     derivative blocks to Jacobian entries (block ids 0 dP/dVa, 1 dP/dVm, 2 dQ/dVa,
+//     3 dQ/dVm, as p3s/cpp/nr_klu.cpp:104-108);
+//   * a symmetric fill-reducing order (minimum degree on the Jacobian graph), the
+//     elimination tree and the static-pivot factor pattern;
+//   * the level schedule the GPU executes: every factor entry is one task (a dot product
+//     over precomputed operand pairs), tasks of one elimination-tree height run
+//     concurrently, dealt to `nslots` task lists by longest-processing-time first.
+//
+// Factor layout (positions, per pivot p in elimination order, S_p = strict column
+// pattern of L(:,p), rows ascending):
+//     base[p]                      1 / U(p,p)
+//     base[p] + 1 + r              U(p, S_p[r])
+//     base[p] + 1 + |S_p| + r      L(S_p[r], p)      (unit lower triangle, scaled)
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace p3smi {
+
+struct Task {      // one factor entry
+    int dst;       // factor position written
+    int asrc;      // Jacobian entry it starts from, -1 = fill (starts from 0)
+    int pbeg, pend;  // operand pairs [pbeg, pend)
+};
+
+struct FwdRow {    // forward substitution row i: y_i = R[rsrc] - sum L(i,j) y_j
+    int row, rsrc, ebeg, eend;
+};
+
+struct BwdRow {    // backward substitution row i: x_i = (y_i - sum U(i,j) x_j) / U(i,i)
+    int row, base, lbeg, lend;
+};
+
+struct Schedule {
+    int nslots = 0;
+    int nlev = 0;   // factor / forward levels (elimination-tree heights)
+    int nblev = 0;  // backward levels (elimination-tree depths)
+    // per (level, slot): [ptr[l*nslots+s], ptr[l*nslots+s+1])
+    std::vector<int> ptrA, ptrB, ptrF, ptrW;
+    std::vector<Task> taskA;  // diagonal and U entries (diagonal marked by dst == base)
+    std::vector<Task> taskB;  // L entries, scaled by 1/U(p,p)
+    std::vector<int> diagB;   // factor position of the inverted pivot of each taskB
+    std::vector<unsigned char> isdiagA;  // taskA entry is a pivot
+    std::vector<FwdRow> fwd;
+    std::vector<BwdRow> bwd;
+};
+
+struct Topology {
+    // ---- polar topology
+    int n = 0, m = 0, npv = 0, npq = 0, npvpq = 0, nnzY = 0, nnzJ = 0;
+    std::vector<int> Yp, Yj, Ydiag;
+    std::vector<double> G, B;             // Ybus = G + jB (rectangular)
+    std::vector<int> pv, pq, pvpq;
+    std::vector<int> row_p, row_q;        // bus -> Jacobian row of its P / Q equation, -1 = none
+    // ---- Jacobian pattern, CSR by equation, columns ascending
+    std::vector<int> Jp, Jj, Jk, Jb;      // entry -> (column, Ybus nonzero, block)
+    std::vector<int> posJ;                // (4 * nnzY) Jacobian entry of block b of nonzero k, -1 = none
+    // ---- static-pivot LU plan
+    std::vector<int> perm, pinv;          // perm[p] = Jacobian row/column eliminated p-th
+    std::vector<int> parent, height, depth;
+    std::vector<int> Lp, Li;              // S_p in pivot space, rows ascending
+    std::vector<int> base;                // (m + 1)
+    std::vector<int> apos;                // Jacobian entry -> factor position
+    std::vector<int> xva, xvm;            // bus -> pivot index of its Va / Vm variable, -1 = none
+    std::vector<int> pa, pb;              // operand pairs: L(a,t) * U(t,b)
+    std::vector<int> prange;              // (nnz_lu + 1) pairs of factor position d: [prange[d], prange[d+1])
+    std::vector<int> Fp, Fpos, Fcol;      // row i of L: entries [Fp[i], Fp[i+1]) = (position, column)
+    int64_t nnz_lu = 0, n_pairs = 0;
+    Schedule sched;
+    double rcond0 = 0.0;                  // min/max |pivot| of the intact grid at flat start
+    int ordering = 0;
+};
+
+// Build everything. ordering: 0 = minimum degree, 2 = natural. Throws std::runtime_error.
+void build_topology(Topology& t, const int* Yp, const int* Yj, const double* Yx, int n, int nnzY, const int* pv,
+                    int npv, const int* pq, int npq, int ordering, int nslots);
+
+// Refresh the admittance values (same pattern) and the reference pivot ratio.
+void update_values(Topology& t, const double* Yx);
+
+// (Re)deal the level schedule to `nslots` task lists.
+void build_schedule(Topology& t, int nslots);
+
+// Host execution of the plan for ONE system (checks and rcond0): Jacobian values at
+// (Vm, Va) with admittance (G, B) [nullptr = the topology's], pin mask [nullptr = none],
+// residual R = Sspec - Scalc in Jacobian row order, J in entry order.
+void host_eval(const Topology& t, const double* Vm, const double* Va, const double* G, const double* B,
+               const double* Ssp /* complex (n) */, const unsigned char* pin, double* R, double* J);
+// Factor J (entry order) into fac (nnz_lu) following the schedule; returns min|pivot|
+// and max|pivot| (pmin > 0 and finite on success).
+void host_factor(const Topology& t, const double* J, double* fac, double* pmin, double* pmax, double* psum);
+// Solve with the factors: x (pivot space, m) from R (Jacobian row order, m).
+void host_solve(const Topology& t, const double* fac, const double* R, double* x);
+
+}  // namespace p3smi

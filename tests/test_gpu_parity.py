@@ -1,0 +1,80 @@
+"""GPU == CPU oracle on the same inputs: identical convergence flags and iteration counts,
+max |dVm| <= 1e-9 p.u. and max |dVa| <= 1e-9 rad (the reference's own GPU test demands
+the same of its GPU path against nr_klu, tests/cuda/test_lean_lu.py:126-147)."""
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSolver
+from p3smi import grids, nr_klu
+from p3smi.timeseries import dc_initial_voltage
+
+pytestmark = pytest.mark.gpu
+
+TOL_V = 1e-9
+
+
+def _compare(gpu, ref, tag=""):
+    assert np.array_equal(gpu["converged"], ref["converged"]), f"{tag}: converged flags differ"
+    assert np.array_equal(gpu["iterations"], ref["iterations"]), (
+        f"{tag}: iterations differ at {np.nonzero(gpu['iterations'] != ref['iterations'])[0][:10]}")
+    ok = ref["converged"]
+    dvm = np.abs(np.abs(gpu["V"][:, ok]) - np.abs(ref["V"][:, ok])).max(initial=0.0)
+    dva = np.abs(np.angle(gpu["V"][:, ok]) - np.angle(ref["V"][:, ok])).max(initial=0.0)
+    assert dvm <= TOL_V, f"{tag}: max|dVm| = {dvm:.3e}"
+    assert dva <= TOL_V, f"{tag}: max|dVa| = {dva:.3e}"
+    return dvm, dva
+
+
+def test_pf200_timeseries(gpu):
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    S = grids.timeseries_scaled(g, 96, seed=1)
+    V0 = g.flat_start()
+    r = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    _compare(r, ref, "pf200")
+    assert r["converged"].all()
+    assert (r["mismatch"] < 1e-8).all()
+
+
+def test_ts2000_subset_dc_start(gpu):
+    g = grids.config_grid("ts2000", 0)
+    Yp, Yj, Yx = g.csr()
+    S = grids.timeseries_daily(g, 8760, seed=1)[:, ::17].copy()  # 516 hours spread over the year
+    V0 = dc_initial_voltage(g)
+    r = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    _compare(r, ref, "ts2000")
+
+
+def test_line_search_from_scrambled_starts(gpu):
+    """Starts far from the solution make the full step overshoot: backtracking decisions
+    must match the oracle (tests/cuda/test_lean_lu.py:150-174)."""
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    rng = np.random.default_rng(7)
+    T = 64
+    S = np.repeat(g.sbus()[:, None], T, axis=1) * rng.uniform(1.0, 3.0, T)[None, :]
+    V0 = g.flat_start()[:, None] * np.exp(1j * rng.normal(0, 0.5, (g.n, T))) * rng.uniform(0.8, 1.2, (g.n, T))
+    for ls in (True, False):
+        r = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8, line_search=ls)
+        ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8, line_search=ls)
+        _compare(r, ref, f"scrambled ls={ls}")
+
+
+def test_nan_column_fails_alone(gpu):
+    """A NaN injection in one column fails that column only (tests/cuda/test_lean_lu.py:212-221)."""
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    S = grids.timeseries_scaled(g, 70, seed=3)
+    S[5, 13] = np.nan
+    V0 = g.flat_start()
+    r = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    assert not r["converged"][13]
+    assert r["converged"][np.arange(70) != 13].all()
+    assert r["iterations"][13] == ref["iterations"][13] == 0
+    keep = np.arange(70) != 13
+    _compare({k: (v[:, keep] if k == "V" else v[keep]) for k, v in r.items()},
+             {k: (v[:, keep] if k == "V" else v[keep]) for k, v in ref.items()}, "nan column")

@@ -1,0 +1,132 @@
+"""Host-side checks of libp3smi.so that need no GPU: the library loads and exports every
+symbol include/p3smi.h declares, the Jacobian pattern and values agree with the oracle's,
+the frozen static-pivot plan factors and solves like a dense/sparse reference solve, and
+bad inputs are rejected with the reference's messages."""
+
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle.oracle import OracleSolver
+from p3smi import _lib, grids, nr_klu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    with open(os.path.join(ROOT, "include", "p3smi.h")) as fh:
+        decl = set(re.findall(r"\b(p3smi_\w+)\s*\(", fh.read()))
+    assert decl, "no declarations parsed"
+    assert decl == set(_lib.EXPORTS)
+    for name in decl:
+        assert hasattr(L, name), name
+
+
+def _random_V(g, seed):
+    rng = np.random.default_rng(seed)
+    return g.flat_start() * (1 + 0.03 * rng.standard_normal(g.n)) * np.exp(1j * 0.1 * rng.standard_normal(g.n))
+
+
+@pytest.mark.parametrize("name", ["pf200", "ts2000"])
+def test_jacobian_equals_oracle(name):
+    g = grids.config_grid(name, 0)
+    Yp, Yj, Yx = g.csr()
+    V = _random_V(g, 1)
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    d = s.debug_J_at(g.sbus(), V)
+    J = sp.csr_matrix((d["Jx"], d["Jj"], d["Jp"]), shape=(d["m"], d["m"]))
+    o = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).debug_J_at(g.sbus(), V)
+    Jo = sp.csc_matrix((o["Jx"], o["Ji"], o["Jp"]), shape=(o["m"], o["m"]))
+    assert (J != 0).nnz == (Jo != 0).nnz
+    diff = abs(J - Jo).max()
+    assert diff <= 1e-12 * abs(Jo).max()
+    np.testing.assert_allclose(d["F"], o["F"], rtol=0, atol=1e-12 * max(1.0, np.abs(o["F"]).max()))
+
+
+@pytest.mark.parametrize("name", ["pf200", "ts2000"])
+def test_static_pivot_plan_solves_like_spsolve(name):
+    g = grids.config_grid(name, 0)
+    Yp, Yj, Yx = g.csr()
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    for seed in range(3):
+        V = _random_V(g, seed)
+        d = s.debug_J_at(g.sbus(), V)
+        J = sp.csr_matrix((d["Jx"], d["Jj"], d["Jp"]), shape=(d["m"], d["m"])).tocsc()
+        R = -d["F"]
+        x, piv = s.host_factor_solve(d["Jx"], R)
+        xr = spla.spsolve(J, R)
+        assert np.abs(J @ x - R).max() <= 1e-10 * max(1.0, np.abs(R).max())
+        assert np.abs(x - xr).max() <= 1e-9 * max(1.0, np.abs(xr).max())
+        assert np.isfinite(piv).all() and piv[0] > 0
+
+
+def test_plan_statistics_are_consistent():
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    info = s.info
+    assert info["m"] == len(g.pv) + 2 * len(g.pq)
+    o = OracleSolver(Yp, Yj, Yx, g.pv, g.pq)
+    assert info["nnz_j"] == o.nnzJ
+    # fill of a symmetric static-pivot LU: nnz(L+U) >= nnz(J), every level non-empty
+    assert info["nnz_lu"] >= info["nnz_j"]
+    assert 0 < info["n_levels"] <= info["m"]
+    assert 0 < info["rcond0"] <= 1.0
+    # same pivot ratio as the oracle's frozen plan on the intact grid at flat start
+    assert abs(info["rcond0"] - o.rcond0) <= 1e-9 * o.rcond0
+    row_p, row_q, perm = s.maps()
+    assert sorted(perm.tolist()) == list(range(info["m"]))
+    assert (row_p[g.ref] == -1).all() and (row_q[g.pv] == -1).all()
+
+
+def test_natural_ordering_gives_more_fill():
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    md = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq, ordering=0).info
+    nat = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq, ordering=2).info
+    assert md["nnz_lu"] < nat["nnz_lu"]
+
+
+def test_invalid_inputs_are_rejected():
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    bad = Yj.copy()
+    bad[[0, 1]] = bad[[1, 0]]
+    with pytest.raises(RuntimeError, match="sorted"):
+        nr_klu.Solver(Yp, bad, Yx, g.pv, g.pq)
+    with pytest.raises(RuntimeError, match="twice"):
+        nr_klu.Solver(Yp, Yj, Yx, g.pv, np.r_[g.pq, g.pv[:1]])
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    with pytest.raises(RuntimeError, match="update_Y: nnz mismatch"):
+        s.update_Y(Yx[:-1])
+    with pytest.raises(RuntimeError, match="solve_batch: Sbus must be 2-D"):
+        s.solve_batch(g.sbus(), g.flat_start())
+    with pytest.raises(RuntimeError, match="solve_batch: V0 \\(n,T\\) shape mismatch"):
+        s.solve_batch(np.zeros((g.n, 3), complex), np.ones((g.n, 2), complex))
+    with pytest.raises(ValueError, match="precision"):
+        s.solve_batch(np.zeros((g.n, 3), complex), g.flat_start(), precision="half")
+
+
+def test_update_y_refreshes_reference_pivot_ratio():
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    r0 = s.info["rcond0"]
+    s.update_Y(Yx * 1.5)
+    assert abs(s.info["rcond0"] - r0) <= 1e-12  # scaling Y leaves the pivot ratio unchanged
+    s.update_Y(Yx)
+    assert s.info["rcond0"] == r0
+
+
+@pytest.mark.skipif(_lib.device_count() > 0, reason="checks the no-GPU error path")
+def test_solve_without_device_fails_loudly():
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        s.solve_batch(np.repeat(g.sbus()[:, None], 2, axis=1), g.flat_start())
