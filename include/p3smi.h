@@ -46,6 +46,8 @@ typedef struct {
 typedef struct {
     int32_t cases, newton_passes, line_search_passes;
     int32_t failed_factorizations;
+    int32_t guard_trips; /* factorizations whose pivot ratio fell below 1e-3 x the flat-start one */
+    int32_t reserved;
     int64_t single_factorizations, double_factorizations, refinement_steps;
     double ms_eval, ms_lu, ms_step, ms_decide, ms_other;
     int32_t n_eval, n_lu, n_step, n_decide;

@@ -52,7 +52,11 @@
 #define LS_BETA 0.5
 #define LS_MAX_TRIALS 10
 #define ARMIJO_C 1e-4
-#define REUSE_RCOND_RATIO 1e-3
+#define REUSE_RCOND_RATIO_DEFAULT 1e-3
+/* pivot-ratio guard of the frozen plan (tests may change it, as the reference's tests
+ * monkeypatch _RCOND_RATIO, tests/cuda/test_lean_lu.py:224-244) */
+static double REUSE_RCOND_RATIO = REUSE_RCOND_RATIO_DEFAULT;
+void oracle_set_lean_guard(double ratio) { REUSE_RCOND_RATIO = ratio; }
 #define PIVOT_TOL 1e-3
 
 static int o_isfinite(double x) {
@@ -715,6 +719,13 @@ static void lean_solve(const leanplan* pl, const double* LU, double* b, double* 
 /* ================================================================== Newton */
 typedef struct { int iterations, converged; } nres;
 
+/* diagnostics: how often the frozen-pivot plan was rejected / a column was re-run */
+static long long g_lean_rejections = 0, g_column_reruns = 0, g_backtracks = 0;
+long long oracle_lean_rejections(void) { return g_lean_rejections; }
+long long oracle_column_reruns(void) { return g_column_reruns; }
+long long oracle_backtracks(void) { return g_backtracks; }
+void oracle_reset_counters(void) { g_lean_rejections = 0; g_column_reruns = 0; g_backtracks = 0; }
+
 /* p3s/cpp/nr_klu.cpp:569-711 */
 static nres run_newton(const otopo* topo, ostate* st, int max_iter, double tol, int line_search) {
     const int npvpq = topo->npvpq, npq = topo->npq, m = topo->m;
@@ -734,6 +745,8 @@ static nres run_newton(const otopo* topo, ostate* st, int max_iter, double tol, 
                 lean_solve(&topo->lean, st->LU, rhs, st->lu_y);
                 lean_done = 1;
             } else {
+#pragma omp atomic
+                g_lean_rejections++;
                 st->use_lean = 0;
                 gp_free(&st->fac);
                 st->factored = 0;
@@ -775,6 +788,8 @@ static nres run_newton(const otopo* topo, ostate* st, int max_iter, double tol, 
             nrm = nrm_full;
             continue;
         }
+#pragma omp atomic
+        g_backtracks++;
         double alpha = 1.0, best_alpha = 1.0, best_nrm = nrm_full;
         for (int trial = 1; trial <= LS_MAX_TRIALS; ++trial) {
             alpha *= LS_BETA;
@@ -808,6 +823,8 @@ static nres run_column(const otopo* topo, ostate* s, init_fn init, const void* c
     s->factored = 0;
     nres r = run_newton(topo, s, max_iter, tol, line_search);
     if (r.converged || !reused) return r;
+#pragma omp atomic
+    g_column_reruns++;
     gp_free(&s->fac);
     s->factored = 0;
     s->use_lean = 0;

@@ -57,6 +57,10 @@ def lib():
     L.oracle_gp_solve.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_int, _f64p]
     L.oracle_min_degree.argtypes = [C.c_int, _i32p, _i32p, _i32p]
     L.oracle_openmp.restype = C.c_int
+    L.oracle_lean_rejections.restype = C.c_longlong
+    L.oracle_column_reruns.restype = C.c_longlong
+    L.oracle_backtracks.restype = C.c_longlong
+    L.oracle_set_lean_guard.argtypes = [C.c_double]
     _lib = L
     return L
 
@@ -209,7 +213,21 @@ def min_degree(A_csc) -> np.ndarray:
     return perm
 
 
-openmp = None
+def counters() -> dict:
+    """Frozen-plan rejections and column re-runs since the last reset_counters()."""
+    L = lib()
+    return {"lean_rejections": int(L.oracle_lean_rejections()), "column_reruns": int(L.oracle_column_reruns()),
+            "backtracks": int(L.oracle_backtracks())}
+
+
+def set_lean_guard(ratio: float = 1e-3) -> None:
+    """Pivot-ratio bound below which the frozen plan is rejected (default 1e-3)."""
+    lib().oracle_set_lean_guard(float(ratio))
+
+
+def reset_counters() -> None:
+    lib().oracle_reset_counters()
+
 
 
 def _openmp() -> bool:

@@ -67,7 +67,7 @@ class Info(C.Structure):
 class Stats(C.Structure):
     _fields_ = [
         ("cases", C.c_int32), ("newton_passes", C.c_int32), ("line_search_passes", C.c_int32),
-        ("failed_factorizations", C.c_int32),
+        ("failed_factorizations", C.c_int32), ("guard_trips", C.c_int32), ("reserved", C.c_int32),
         ("single_factorizations", C.c_int64), ("double_factorizations", C.c_int64), ("refinement_steps", C.c_int64),
         ("ms_eval", C.c_double), ("ms_lu", C.c_double), ("ms_step", C.c_double), ("ms_decide", C.c_double),
         ("ms_other", C.c_double),
