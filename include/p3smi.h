@@ -39,6 +39,7 @@ typedef struct {
     int64_t nnz_lu, n_pairs;
     int32_t n_levels, n_blevels, n_slots, ordering;
     double rcond0;
+    int32_t lu_g, lu_ns, lu_few_g, lu_few_ns, lu_few_max, reserved;
 } p3smi_info;
 
 /* Per-case/aggregate counters of the last solve (the reference's last_stats,
@@ -70,6 +71,11 @@ int p3smi_topology_create(const int32_t* Yp, const int32_t* Yj, const double* Yx
                           int32_t n_slots, p3smi_topology** out);
 void p3smi_topology_free(p3smi_topology* t);
 int p3smi_topology_info(const p3smi_topology* t, p3smi_info* out);
+/* LU launch shape (no counterpart in the reference; set by the autotuner): a workgroup of
+ * 64*ns threads factors g consecutive cases (g in {1, 8, 16, 32, 64}, ns in {4, 8, 16});
+ * (few_g, few_ns) is used instead while at most few_max cases are still iterating. */
+int p3smi_topology_set_lu_shape(p3smi_topology* t, int32_t g, int32_t ns, int32_t few_g, int32_t few_ns,
+                                int32_t few_max);
 /* Refresh Ybus values on the same pattern: nr_klu.Solver.update_Y (p3s/cpp/nr_klu.cpp:999-1011). */
 int p3smi_update_y(p3smi_topology* t, const double* Yx, int32_t nnz);
 

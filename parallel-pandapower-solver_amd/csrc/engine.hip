@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -230,8 +231,36 @@ __global__ __launch_bounds__(64 * EVAL_WAVES) void k_eval(
     }
 }
 
-// Static-pivot LU + substitution for the ACTIVE cases of 64 consecutive cases (lanes);
-// the NS wavefronts of the workgroup are NS task lists. Level l of the elimination tree:
+// acc - sum_e fac[pa_e] * fac[pb_e] over the operand pairs [e, e1) of one factor entry.
+// Four pairs per trip: eight independent loads in flight before the first use.
+__device__ __forceinline__ double dot_pairs(const int2* __restrict__ pairs, int e, int e1,
+                                            const double* __restrict__ fac, int S, int c, double acc) {
+    double acc1 = 0.0;
+    for (; e + 4 <= e1; e += 4) {
+        const int2 p0 = pairs[e], p1 = pairs[e + 1], p2 = pairs[e + 2], p3 = pairs[e + 3];
+        const double a0 = fac[at(p0.x, S, c)], b0 = fac[at(p0.y, S, c)];
+        const double a1 = fac[at(p1.x, S, c)], b1 = fac[at(p1.y, S, c)];
+        const double a2 = fac[at(p2.x, S, c)], b2 = fac[at(p2.y, S, c)];
+        const double a3 = fac[at(p3.x, S, c)], b3 = fac[at(p3.y, S, c)];
+        acc = fma(-a0, b0, acc);
+        acc1 = fma(-a1, b1, acc1);
+        acc = fma(-a2, b2, acc);
+        acc1 = fma(-a3, b3, acc1);
+    }
+    for (; e < e1; ++e) {
+        const int2 p = pairs[e];
+        acc = fma(-fac[at(p.x, S, c)], fac[at(p.y, S, c)], acc);
+    }
+    return acc + acc1;
+}
+
+// Static-pivot LU + substitution. A workgroup owns G consecutive cases and runs
+// TL = 64 * NS / G task lists over them: thread t works on case t % G of the group and on
+// task list t / G. G = 64 makes each wavefront one task list over 64 cases (every index
+// wave-uniform, every value access one 512-byte line); G = 16 keeps 128-byte lines and
+// gives four task lists per wavefront; G = 1 puts a whole workgroup on one case (the
+// fastest single factorization, for the last few cases of a batch).
+// Level l of the elimination tree:
 //   sub-level A: pivots and U entries   acc = a - sum L(i,t) U(t,k); pivot -> 1/acc
 //                forward rows            y_i = R[perm i] - sum L(i,j) y_j
 //   barrier
@@ -243,7 +272,7 @@ __global__ __launch_bounds__(64 * EVAL_WAVES) void k_eval(
 // the rest of the solve to a pivoting factorization) is counted in guard_trips and the
 // static-pivot step is kept: the step is the same solution of J dx = R, and the oracle's
 // runs with and without that fallback agree to roundoff on every converging case.
-template <int NS>
+template <int G, int NS>
 __global__ __launch_bounds__(64 * NS) void k_lu(
     const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
     const int4* __restrict__ taskB, const int* __restrict__ diagB, const int2* __restrict__ pairs,
@@ -252,27 +281,24 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
     const double* __restrict__ Jx, double* __restrict__ fac, const double* __restrict__ R, double* __restrict__ x,
     int* __restrict__ status, int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips,
     double rcond_min, int nlev, int nblev, int S, int it) {
-    __shared__ double smin[NS][64], smax[NS][64], ssum[NS][64];
-    __shared__ int sbad[64];
-    const int lane = threadIdx.x & 63;
-    const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int c = blockIdx.x * 64 + lane;
+    constexpr int TL = 64 * NS / G;
+    __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
+    __shared__ int sbad[G];
+    const int cl = threadIdx.x % G;
+    const int slot = G == 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G)) : (int)(threadIdx.x / G);
+    const int c = blockIdx.x * G + cl;
     bool act = status[c] == ST_ACTIVE;
     if (!__syncthreads_or(act)) return;
 
     double pmin = DBL_MAX, pmax = 0.0, psum = 0.0;
     for (int l = 0; l < nlev; ++l) {
         if (act) {
-            const int q0 = ptrA[l * NS + slot], q1 = ptrA[l * NS + slot + 1];
+            const int q0 = ptrA[l * TL + slot], q1 = ptrA[l * TL + slot + 1];
             for (int q = q0; q < q1; ++q) {
                 const int4 tk = taskA[q];
                 const bool diag = tk.y < -1;
                 const int src = diag ? -tk.y - 2 : tk.y;
-                double acc = src >= 0 ? Jx[at(src, S, c)] : 0.0;
-                for (int e = tk.z; e < tk.w; ++e) {
-                    const int2 pp = pairs[e];
-                    acc = fma(-fac[at(pp.x, S, c)], fac[at(pp.y, S, c)], acc);
-                }
+                double acc = dot_pairs(pairs, tk.z, tk.w, fac, S, c, src >= 0 ? Jx[at(src, S, c)] : 0.0);
                 if (diag) {
                     const double ab = fabs(acc);
                     pmin = fmin(pmin, ab);
@@ -282,43 +308,47 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
                 }
                 fac[at(tk.x, S, c)] = acc;
             }
-            const int f0 = ptrF[l * NS + slot], f1 = ptrF[l * NS + slot + 1];
+            const int f0 = ptrF[l * TL + slot], f1 = ptrF[l * TL + slot + 1];
             for (int q = f0; q < f1; ++q) {
                 const int4 fr = fwd[q];
-                double acc = R[at(fr.y, S, c)];
-                for (int e = fr.z; e < fr.w; ++e) {
+                double acc = R[at(fr.y, S, c)], acc1 = 0.0;
+                int e = fr.z;
+                for (; e + 2 <= fr.w; e += 2) {
+                    const int2 f0e = fent[e], f1e = fent[e + 1];
+                    const double a0 = fac[at(f0e.x, S, c)], y0 = x[at(f0e.y, S, c)];
+                    const double a1 = fac[at(f1e.x, S, c)], y1 = x[at(f1e.y, S, c)];
+                    acc = fma(-a0, y0, acc);
+                    acc1 = fma(-a1, y1, acc1);
+                }
+                if (e < fr.w) {
                     const int2 fe = fent[e];
                     acc = fma(-fac[at(fe.x, S, c)], x[at(fe.y, S, c)], acc);
                 }
-                x[at(fr.x, S, c)] = acc;
+                x[at(fr.x, S, c)] = acc + acc1;
             }
         }
         __syncthreads();
         if (act) {
-            const int q0 = ptrB[l * NS + slot], q1 = ptrB[l * NS + slot + 1];
+            const int q0 = ptrB[l * TL + slot], q1 = ptrB[l * TL + slot + 1];
             for (int q = q0; q < q1; ++q) {
                 const int4 tk = taskB[q];
-                double acc = tk.y >= 0 ? Jx[at(tk.y, S, c)] : 0.0;
-                for (int e = tk.z; e < tk.w; ++e) {
-                    const int2 pp = pairs[e];
-                    acc = fma(-fac[at(pp.x, S, c)], fac[at(pp.y, S, c)], acc);
-                }
+                const double acc = dot_pairs(pairs, tk.z, tk.w, fac, S, c, tk.y >= 0 ? Jx[at(tk.y, S, c)] : 0.0);
                 fac[at(tk.x, S, c)] = acc * fac[at(diagB[q], S, c)];
             }
         }
         __syncthreads();
     }
-    smin[slot][lane] = pmin;
-    smax[slot][lane] = pmax;
-    ssum[slot][lane] = psum;
+    smin[slot][cl] = pmin;
+    smax[slot][cl] = pmax;
+    ssum[slot][cl] = psum;
     __syncthreads();
     if (slot == 0) {
         int bad = 0;
         if (act) {
-            for (int s = 1; s < NS; ++s) {
-                pmin = fmin(pmin, smin[s][lane]);
-                pmax = fmax(pmax, smax[s][lane]);
-                psum += ssum[s][lane];
+            for (int s2 = 1; s2 < TL; ++s2) {
+                pmin = fmin(pmin, smin[s2][cl]);
+                pmax = fmax(pmax, smax[s2][cl]);
+                psum += ssum[s2][cl];
             }
             pstat[2 * (size_t)c] = pmin;
             pstat[2 * (size_t)c + 1] = pmax;
@@ -330,19 +360,26 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
                 atomicAdd(guard_trips, 1);
             }
         }
-        sbad[lane] = bad;
+        sbad[cl] = bad;
     }
     __syncthreads();
-    if (sbad[lane]) act = false;
+    if (sbad[cl]) act = false;
     for (int l = 0; l < nblev; ++l) {
         if (act) {
-            const int q0 = ptrW[l * NS + slot], q1 = ptrW[l * NS + slot + 1];
+            const int q0 = ptrW[l * TL + slot], q1 = ptrW[l * TL + slot + 1];
             for (int q = q0; q < q1; ++q) {
                 const int4 br = bwd[q];
-                double acc = x[at(br.x, S, c)];
+                double acc = x[at(br.x, S, c)], acc1 = 0.0;
                 const int pos0 = br.y + 1 - br.z;
-                for (int e = br.z; e < br.w; ++e) acc = fma(-fac[at(pos0 + e, S, c)], x[at(Li[e], S, c)], acc);
-                x[at(br.x, S, c)] = acc * fac[at(br.y, S, c)];
+                int e = br.z;
+                for (; e + 2 <= br.w; e += 2) {
+                    const double u0 = fac[at(pos0 + e, S, c)], x0 = x[at(Li[e], S, c)];
+                    const double u1 = fac[at(pos0 + e + 1, S, c)], x1 = x[at(Li[e + 1], S, c)];
+                    acc = fma(-u0, x0, acc);
+                    acc1 = fma(-u1, x1, acc1);
+                }
+                if (e < br.w) acc = fma(-fac[at(pos0 + e, S, c)], x[at(Li[e], S, c)], acc);
+                x[at(br.x, S, c)] = (acc + acc1) * fac[at(br.y, S, c)];
             }
         }
         __syncthreads();
@@ -489,13 +526,24 @@ struct DevBuf {
     }
 };
 
+struct DevSched {  // one level schedule dealt to TL task lists
+    int TL = 0, nlev = 0, nblev = 0;
+    DevBuf<int> ptrA, ptrB, ptrF, ptrW, diagB;
+    DevBuf<int4> taskA, taskB, fwd, bwd;
+};
+
 struct DevTopo {
     DevBuf<int> Yp, Yj, Ydiag, row_p, row_q, posJ, xva, xvm;
     DevBuf<double> G, B;
-    DevBuf<int> ptrA, ptrB, ptrF, ptrW, diagB, Li;
-    DevBuf<int4> taskA, taskB, fwd, bwd;
+    DevBuf<int> Li;
     DevBuf<int2> pairs, fent;
+    std::map<int, std::unique_ptr<DevSched>> sched;
     bool ready = false;
+};
+
+struct LuConfig {
+    int G, NS;
+    int tl() const { return 64 * NS / G; }
 };
 
 }  // namespace
@@ -503,6 +551,9 @@ struct DevTopo {
 struct p3smi_topology {
     p3smi::Topology host;
     DevTopo dev;
+    LuConfig lu_batch{16, 8};     // LU launch shape while many cases are active
+    LuConfig lu_few{1, 8};        // ... and for the last few
+    int lu_few_max = 256;         // "few": at most this many active cases
     hipStream_t stream = nullptr;
     std::unique_ptr<p3smi_batch> cached;  // workspace of the one-call entry points
     std::mutex mu;
@@ -549,7 +600,6 @@ void ensure_device(p3smi_topology* t) {
         throw Error(P3SMI_E_DEVICE, "no HIP device available (the MI355X engine needs a GPU)");
     if (!t->stream) HIPCHECK(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
     const p3smi::Topology& h = t->host;
-    const p3smi::Schedule& s = h.sched;
     hipStream_t st = t->stream;
     DevTopo& d = t->dev;
     d.Yp.upload(h.Yp.data(), h.Yp.size(), st);
@@ -562,12 +612,32 @@ void ensure_device(p3smi_topology* t) {
     d.posJ.upload(h.posJ.data(), h.posJ.size(), st);
     d.xva.upload(h.xva.data(), h.xva.size(), st);
     d.xvm.upload(h.xvm.data(), h.xvm.size(), st);
-    d.ptrA.upload(s.ptrA.data(), s.ptrA.size(), st);
-    d.ptrB.upload(s.ptrB.data(), s.ptrB.size(), st);
-    d.ptrF.upload(s.ptrF.data(), s.ptrF.size(), st);
-    d.ptrW.upload(s.ptrW.data(), s.ptrW.size(), st);
-    d.diagB.upload(s.diagB.data(), s.diagB.size(), st);
     d.Li.upload(h.Li.data(), h.Li.size(), st);
+    std::vector<int2> pr(h.pa.size()), fe(h.Fpos.size());
+    for (size_t i = 0; i < pr.size(); ++i) pr[i] = make_int2(h.pa[i], h.pb[i]);
+    for (size_t i = 0; i < fe.size(); ++i) fe[i] = make_int2(h.Fpos[i], h.Fcol[i]);
+    d.pairs.upload(pr.data(), pr.size(), st);
+    d.fent.upload(fe.data(), fe.size(), st);
+    HIPCHECK(hipStreamSynchronize(st));
+    d.ready = true;
+}
+
+// Level schedule for TL task lists, built and uploaded on first use.
+const DevSched& get_sched(p3smi_topology* t, int TL) {
+    auto it = t->dev.sched.find(TL);
+    if (it != t->dev.sched.end()) return *it->second;
+    const p3smi::Topology& h = t->host;
+    const p3smi::Schedule s = p3smi::make_schedule(h, TL);
+    std::unique_ptr<DevSched> d(new DevSched());
+    hipStream_t st = t->stream;
+    d->TL = TL;
+    d->nlev = s.nlev;
+    d->nblev = s.nblev;
+    d->ptrA.upload(s.ptrA.data(), s.ptrA.size(), st);
+    d->ptrB.upload(s.ptrB.data(), s.ptrB.size(), st);
+    d->ptrF.upload(s.ptrF.data(), s.ptrF.size(), st);
+    d->ptrW.upload(s.ptrW.data(), s.ptrW.size(), st);
+    d->diagB.upload(s.diagB.data(), s.diagB.size(), st);
     std::vector<int4> ta(s.taskA.size()), tb(s.taskB.size()), fw(s.fwd.size()), bw(s.bwd.size());
     for (size_t i = 0; i < ta.size(); ++i) {
         const auto& k = s.taskA[i];
@@ -581,17 +651,12 @@ void ensure_device(p3smi_topology* t) {
     for (size_t i = 0; i < tb.size(); ++i) tb[i] = make_int4(s.taskB[i].dst, s.taskB[i].asrc, s.taskB[i].pbeg, s.taskB[i].pend);
     for (size_t i = 0; i < fw.size(); ++i) fw[i] = make_int4(s.fwd[i].row, s.fwd[i].rsrc, s.fwd[i].ebeg, s.fwd[i].eend);
     for (size_t i = 0; i < bw.size(); ++i) bw[i] = make_int4(s.bwd[i].row, s.bwd[i].base, s.bwd[i].lbeg, s.bwd[i].lend);
-    std::vector<int2> pr(h.pa.size()), fe(h.Fpos.size());
-    for (size_t i = 0; i < pr.size(); ++i) pr[i] = make_int2(h.pa[i], h.pb[i]);
-    for (size_t i = 0; i < fe.size(); ++i) fe[i] = make_int2(h.Fpos[i], h.Fcol[i]);
-    d.taskA.upload(ta.data(), ta.size(), st);
-    d.taskB.upload(tb.data(), tb.size(), st);
-    d.fwd.upload(fw.data(), fw.size(), st);
-    d.bwd.upload(bw.data(), bw.size(), st);
-    d.pairs.upload(pr.data(), pr.size(), st);
-    d.fent.upload(fe.data(), fe.size(), st);
+    d->taskA.upload(ta.data(), ta.size(), st);
+    d->taskB.upload(tb.data(), tb.size(), st);
+    d->fwd.upload(fw.data(), fw.size(), st);
+    d->bwd.upload(bw.data(), bw.size(), st);
     HIPCHECK(hipStreamSynchronize(st));
-    d.ready = true;
+    return *(t->dev.sched[TL] = std::move(d));
 }
 
 int eval_nparts(int n) { return (n + EVAL_WAVES * EVAL_BUSES_PER_WAVE - 1) / (EVAL_WAVES * EVAL_BUSES_PER_WAVE); }
@@ -680,29 +745,35 @@ void launch_eval(p3smi_batch* b, int run_mask, int j_mask) {
     HIPCHECK(hipGetLastError());
 }
 
-void launch_lu(p3smi_batch* b, int it) {
+template <int G, int NS>
+void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, int it) {
     const p3smi::Topology& h = b->topo->host;
     const DevTopo& d = b->topo->dev;
-    const int ns = h.sched.nslots;
-    KTimer tm(b, 1);
-    dim3 grid((unsigned)(b->S / 64));
     const double rcond_min = REUSE_RCOND_RATIO * h.rcond0;
-    auto go = [&](auto kern, int nsl) {
-        hipLaunchKernelGGL(kern, grid, dim3(64 * nsl), 0, b->topo->stream, d.ptrA.p, d.taskA.p, d.ptrB.p, d.taskB.p,
-                           d.diagB.p, d.pairs.p, d.ptrF.p, d.fwd.p, d.fent.p, d.ptrW.p, d.bwd.p, d.Li.p, b->Jx.p,
-                           b->fac.p, b->R.p, b->x.p, b->status.p, b->iters.p, b->pstat.p, b->guard.p, rcond_min,
-                           h.sched.nlev,
-                           h.sched.nblev, b->S, it);
-    };
-    switch (ns) {
-        case 1: go(k_lu<1>, 1); break;
-        case 2: go(k_lu<2>, 2); break;
-        case 4: go(k_lu<4>, 4); break;
-        case 8: go(k_lu<8>, 8); break;
-        case 16: go(k_lu<16>, 16); break;
-        default: throw Error(P3SMI_E_ARG, "n_slots must be 1, 2, 4, 8 or 16");
-    }
-    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL((k_lu<G, NS>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0, b->topo->stream, ds.ptrA.p,
+                       ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.diagB.p, d.pairs.p, ds.ptrF.p, ds.fwd.p, d.fent.p,
+                       ds.ptrW.p, ds.bwd.p, d.Li.p, b->Jx.p, b->fac.p, b->R.p, b->x.p, b->status.p, b->iters.p,
+                       b->pstat.p, b->guard.p, rcond_min, ds.nlev, ds.nblev, b->S, it);
+}
+
+bool valid_lu_config(LuConfig c) {
+    const bool g = c.G == 64 || c.G == 32 || c.G == 16 || c.G == 8 || c.G == 1;
+    const bool n = c.NS == 4 || c.NS == 8 || c.NS == 16;
+    return g && n;
+}
+
+void launch_lu(p3smi_batch* b, int it, int n_active) {
+    p3smi_topology* t = b->topo;
+    const LuConfig cfg = n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
+    const DevSched& ds = get_sched(t, cfg.tl());
+    KTimer tm(b, 1);
+#define P3SMI_LU_CASE(GG, NN) \
+    if (cfg.G == GG && cfg.NS == NN) { launch_lu_cfg<GG, NN>(b, ds, it); HIPCHECK(hipGetLastError()); return; }
+#define P3SMI_LU_G(GG) P3SMI_LU_CASE(GG, 4) P3SMI_LU_CASE(GG, 8) P3SMI_LU_CASE(GG, 16)
+    P3SMI_LU_G(64) P3SMI_LU_G(32) P3SMI_LU_G(16) P3SMI_LU_G(8) P3SMI_LU_G(1)
+#undef P3SMI_LU_G
+#undef P3SMI_LU_CASE
+    throw Error(P3SMI_E_ARG, "unsupported LU launch shape");
 }
 
 void launch_step(p3smi_batch* b, int mask, int full) {
@@ -755,7 +826,7 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
         ++it;
         b->stats.newton_passes++;
         b->stats.double_factorizations += cnt[ST_ACTIVE];
-        launch_lu(b, it);
+        launch_lu(b, it, cnt[ST_ACTIVE]);
         launch_step(b, A, 1);
         launch_eval(b, A, A);
         cnt = launch_decide(b, tol, line_search, 1, it, max_iter);
@@ -909,6 +980,22 @@ int p3smi_topology_info(const p3smi_topology* t, p3smi_info* o) {
         o->n = h.n; o->m = h.m; o->npv = h.npv; o->npq = h.npq; o->nnz_y = h.nnzY; o->nnz_j = h.nnzJ;
         o->nnz_lu = h.nnz_lu; o->n_pairs = h.n_pairs; o->n_levels = h.sched.nlev; o->n_blevels = h.sched.nblev;
         o->n_slots = h.sched.nslots; o->ordering = h.ordering; o->rcond0 = h.rcond0;
+        o->lu_g = t->lu_batch.G; o->lu_ns = t->lu_batch.NS; o->lu_few_g = t->lu_few.G; o->lu_few_ns = t->lu_few.NS;
+        o->lu_few_max = t->lu_few_max; o->reserved = 0;
+    });
+}
+
+int p3smi_topology_set_lu_shape(p3smi_topology* t, int32_t g, int32_t ns, int32_t few_g, int32_t few_ns,
+                                int32_t few_max) {
+    return guarded([&] {
+        const LuConfig a{g, ns}, f{few_g, few_ns};
+        if (!valid_lu_config(a) || !valid_lu_config(f))
+            throw Error(P3SMI_E_ARG, "LU launch shape: g must be 1, 8, 16, 32 or 64 and ns 4, 8 or 16");
+        if (few_max < 0) throw Error(P3SMI_E_ARG, "LU launch shape: few_max must be >= 0");
+        std::lock_guard<std::mutex> lk(t->mu);
+        t->lu_batch = a;
+        t->lu_few = f;
+        t->lu_few_max = few_max;
     });
 }
 

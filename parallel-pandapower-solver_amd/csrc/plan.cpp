@@ -317,7 +317,9 @@ static void build_lu(Topology& t, int ordering) {
     for (int64_t e = 0; e <= t.nnz_lu; ++e) t.prange[e] = (int)cnt[e];
 }
 
-void build_schedule(Topology& t, int nslots) {
+void build_schedule(Topology& t, int nslots) { t.sched = make_schedule(t, nslots); }
+
+Schedule make_schedule(const Topology& t, int nslots) {
     if (nslots < 1) throw std::runtime_error("nslots must be >= 1");
     const int m = t.m;
     const std::vector<int>& prange = t.prange;
@@ -395,7 +397,7 @@ void build_schedule(Topology& t, int nslots) {
         const int p = itemsW[i];
         s.bwd.push_back({p, t.base[p], t.Lp[p], t.Lp[p + 1]});
     }
-    t.sched = std::move(s);
+    return s;
 }
 
 // ------------------------------------------------------------------ host numerics

@@ -88,8 +88,9 @@ void build_topology(Topology& t, const int* Yp, const int* Yj, const double* Yx,
 // Refresh the admittance values (same pattern) and the reference pivot ratio.
 void update_values(Topology& t, const double* Yx);
 
-// (Re)deal the level schedule to `nslots` task lists.
+// (Re)deal the level schedule to `nslots` task lists (t.sched), or build another one.
 void build_schedule(Topology& t, int nslots);
+Schedule make_schedule(const Topology& t, int nslots);
 
 // Host execution of the plan for ONE system (checks and rcond0): Jacobian values at
 // (Vm, Va) with admittance (G, B) [nullptr = the topology's], pin mask [nullptr = none],

@@ -61,6 +61,8 @@ class Info(C.Structure):
         ("nnz_y", C.c_int32), ("nnz_j", C.c_int32), ("nnz_lu", C.c_int64), ("n_pairs", C.c_int64),
         ("n_levels", C.c_int32), ("n_blevels", C.c_int32), ("n_slots", C.c_int32), ("ordering", C.c_int32),
         ("rcond0", C.c_double),
+        ("lu_g", C.c_int32), ("lu_ns", C.c_int32), ("lu_few_g", C.c_int32), ("lu_few_ns", C.c_int32),
+        ("lu_few_max", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -79,6 +81,7 @@ class Stats(C.Structure):
 EXPORTS = [
     "p3smi_last_error", "p3smi_version", "p3smi_device_count",
     "p3smi_topology_create", "p3smi_topology_free", "p3smi_topology_info", "p3smi_update_y",
+    "p3smi_topology_set_lu_shape",
     "p3smi_host_jacobian", "p3smi_host_factor_solve", "p3smi_host_maps",
     "p3smi_solve_batch", "p3smi_solve_batch_contingency", "p3smi_solve_batch_contingency_stamp",
     "p3smi_batch_create", "p3smi_batch_free", "p3smi_batch_upload", "p3smi_batch_solve", "p3smi_batch_download",
@@ -107,6 +110,7 @@ def lib():
     L.p3smi_topology_free.restype = None
     L.p3smi_topology_info.argtypes = [P, P]
     L.p3smi_update_y.argtypes = [P, P, _i32]
+    L.p3smi_topology_set_lu_shape.argtypes = [P, _i32, _i32, _i32, _i32, _i32]
     L.p3smi_host_jacobian.argtypes = [P, P, P, P, P, P, P]
     L.p3smi_host_factor_solve.argtypes = [P, P, P, P, P]
     L.p3smi_host_maps.argtypes = [P, P, P, P]

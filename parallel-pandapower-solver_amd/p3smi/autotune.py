@@ -1,0 +1,99 @@
+"""Pick the LU launch shape per grid-size bucket by timing candidates on the device.
+
+The batched LU kernel has two shape parameters: g, the cases one workgroup factors
+together, and ns, its wavefronts (so 64 * ns / g task lists per case). Large g keeps
+memory accesses in long contiguous lines; small g spreads one factorization over more
+task lists and finishes it sooner. The best pair depends on the factor pattern and on how
+many cases are still iterating, so the tuner times every candidate on a sample of the
+real batch and caches the winner per size bucket (<= 500, <= 5k, <= 20k, > 20k buses).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+
+BATCH_CANDIDATES = [(64, 8), (64, 16), (32, 8), (32, 16), (16, 8), (16, 16), (8, 8), (8, 16)]
+FEW_CANDIDATES = [(1, 8), (1, 16), (8, 8)]
+BUCKETS = [(500, "le500"), (5000, "le5k"), (20000, "le20k"), (10**9, "gt20k")]
+CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
+
+
+def bucket(n_bus: int) -> str:
+    for lim, name in BUCKETS:
+        if n_bus <= lim:
+            return name
+    return BUCKETS[-1][1]
+
+
+def _time_solve(solver, bh, reps: int) -> float:
+    L = _lib.lib()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _lib.check(L.p3smi_batch_solve(bh, 20, 1e-8, 1, 0))
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def tune(solver, S, V0, reps: int = 2, log=None) -> dict:
+    """Time the candidate shapes on (S, V0) and set the fastest on ``solver``."""
+    L = _lib.lib()
+    S = np.ascontiguousarray(S, dtype=np.complex128)
+    V0 = np.ascontiguousarray(V0, dtype=np.complex128)
+    bh = C.c_void_p()
+    _lib.check(L.p3smi_batch_create(solver._h, S.shape[1], C.byref(bh)))
+    try:
+        _lib.check(L.p3smi_batch_upload(bh, _lib.ptr(S), _lib.ptr(V0), int(V0.ndim == 2)))
+        results = {}
+        for g, ns in BATCH_CANDIDATES:
+            solver.set_lu_shape(g, ns, 1, 8, 0)  # batch shape for every pass
+            _time_solve(solver, bh, 1)
+            results[(g, ns)] = _time_solve(solver, bh, reps)
+            if log:
+                log(f"[autotune] batch g={g} ns={ns}: {1e3 * results[(g, ns)]:.2f} ms")
+        bg, bns = min(results, key=results.get)
+        few = {}
+        for fg, fns in FEW_CANDIDATES:
+            for fmax in (64, 256, 1024):
+                solver.set_lu_shape(bg, bns, fg, fns, fmax)
+                _time_solve(solver, bh, 1)
+                few[(fg, fns, fmax)] = _time_solve(solver, bh, reps)
+                if log:
+                    log(f"[autotune] few g={fg} ns={fns} max={fmax}: {1e3 * few[(fg, fns, fmax)]:.2f} ms")
+        best_few = min(few, key=few.get)
+        if few[best_few] >= results[(bg, bns)]:
+            best_few = (1, 8, 0)
+        fg, fns, fmax = best_few
+        solver.set_lu_shape(bg, bns, fg, fns, fmax)
+        return {"g": bg, "ns": bns, "few_g": fg, "few_ns": fns, "few_max": fmax,
+                "ms": 1e3 * min(results[(bg, bns)], few.get(best_few, results[(bg, bns)]))}
+    finally:
+        L.p3smi_batch_free(bh)
+
+
+def load_cache() -> dict:
+    if os.path.exists(CACHE):
+        with open(CACHE) as fh:
+            return json.load(fh)
+    return {}
+
+
+def save_cache(entries: dict) -> None:
+    with open(CACHE, "w") as fh:
+        json.dump(entries, fh, indent=1, sort_keys=True)
+
+
+def apply_cached(solver) -> bool:
+    """Set the cached shape of the solver's size bucket, if there is one."""
+    e = load_cache().get(bucket(solver.n))
+    if not e:
+        return False
+    solver.set_lu_shape(e["g"], e["ns"], e["few_g"], e["few_ns"], e["few_max"])
+    return True

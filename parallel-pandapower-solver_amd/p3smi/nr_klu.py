@@ -75,6 +75,12 @@ class Solver:
         check(lib().p3smi_topology_info(self._h, C.byref(inf)))
         return {f: getattr(inf, f) for f, _ in inf._fields_}
 
+    def set_lu_shape(self, g: int, ns: int, few_g: int = 1, few_ns: int = 8, few_max: int = 256):
+        """LU launch shape: g cases per workgroup of 64*ns threads; (few_g, few_ns) while at
+        most few_max cases are still iterating."""
+        check(lib().p3smi_topology_set_lu_shape(self._h, int(g), int(ns), int(few_g), int(few_ns), int(few_max)))
+        self.info = self._info()
+
     @property
     def lean_active(self) -> bool:
         """True: every factorization runs on the frozen static-pivot plan."""
