@@ -79,9 +79,14 @@ def main() -> int:
     Yp, Yj, Yx = g.csr()
     T = S.shape[1]
     log(f"[bench] workload {args.config}: n={g.n} cases={T}")
+    from p3smi import autotune
+
     t0 = time.perf_counter()
     solver = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
     t_setup = time.perf_counter() - t0
+    if not autotune.apply_cached(solver):
+        log("[bench] no cached LU shape for this size bucket: tuning (not timed)")
+        autotune.tune(solver, S, V0, reps=1, log=log)
     info = solver.info
     log(f"[bench] plan: m={info['m']} nnzJ={info['nnz_j']} nnzLU={info['nnz_lu']} pairs={info['n_pairs']} "
         f"levels={info['n_levels']} ({t_setup:.2f}s)")
@@ -168,7 +173,9 @@ def main() -> int:
         "dtype": "f64" if args.precision == "double" else "f32-lu+f64",
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {desc}", "n_bus": g.n, "cases": T, "precision": args.precision,
-                   "nnz_lu": int(info["nnz_lu"]), "n_levels": int(info["n_levels"])},
+                   "nnz_lu": int(info["nnz_lu"]), "n_levels": int(info["n_levels"]),
+                   "lu_shape": [int(info["lu_g"]), int(info["lu_ns"]), int(info["lu_few_g"]), int(info["lu_few_ns"]),
+                                int(info["lu_few_max"])]},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(times, 50)), 3),
                        "p99": round(1e3 * float(np.percentile(times, 99)), 3)},
         "converged": n_conv,
