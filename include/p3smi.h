@@ -116,6 +116,30 @@ int p3smi_solve_batch_contingency_stamp(p3smi_topology* t, const int32_t* stamp_
                                         int32_t line_search, int32_t precision, double* V_out, int32_t* iterations,
                                         uint8_t* converged, double* mismatch);
 
+/* ---- N-1 outage sets ------------------------------------------------------------------
+ * Replaces ContingencyCaseGenerator + solve_contingencies_cpp
+ * (p3s/contingency/case_generator.py:488-566, p3s/contingency/solver_cpp.py:64-156).
+ * Branch k joins buses from[k], to[k]; ypos (nbr x 4) are the Ybus CSR positions of its
+ * (ff, ft, tf, tt) stamp entries and yval (nbr x 4 complex) the stamp values. It couples
+ * its ends (is an edge of the island graph) iff its ft or tf stamp is non-zero. Case c
+ * removes the branches gbr[gptr[c] .. gptr[c+1]). ref (nref): slack buses; gen_z (n):
+ * generator short-circuit |Z| per bus, +inf where a bus has no eligible generator;
+ * reslack != 0 gives an island without a slack its lowest-|Z| generator as reference. */
+typedef struct p3smi_outages p3smi_outages;
+int p3smi_outages_create(p3smi_topology* t, int32_t nbr, const int32_t* from, const int32_t* to, const int32_t* ypos,
+                         const double* yval, int32_t n_cases, const int32_t* gptr, const int32_t* gbr, int32_t nref,
+                         const int32_t* ref, const double* gen_z, int32_t reslack, p3smi_outages** out);
+void p3smi_outages_free(p3smi_outages* o);
+/* Islands of every case on the device (union-find): served (n, L) bus has a reference,
+ * ref_bus (n, L) the reference bus of its island or -1, pinned (L) number of re-slacked
+ * references (_components_after_outage + _resolve_references, case_generator.py:355-430). */
+int p3smi_outages_islands(p3smi_outages* o, uint8_t* served, int32_t* ref_bus, int32_t* pinned);
+/* Solve every case: S (n) complex injections, V_start (n) complex start voltage. V_out
+ * (n, L) complex with NaN at unserved buses; served (n, L) may be NULL. */
+int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start, int32_t max_iter, double tol,
+                        int32_t line_search, int32_t precision, double* V_out, int32_t* iterations,
+                        uint8_t* converged, double* mismatch, uint8_t* served);
+
 /* ---- resident batches (inputs uploaded once, solved many times) ---------------------- */
 int p3smi_batch_create(p3smi_topology* t, int32_t cases, p3smi_batch** out);
 void p3smi_batch_free(p3smi_batch* b);

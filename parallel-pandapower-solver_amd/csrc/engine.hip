@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -485,15 +486,158 @@ __global__ void k_decide(const double* __restrict__ part, int nparts, double* __
 }
 
 // polar state -> complex V (n, B), the layout of the numpy result.
+// polar state -> complex V (n, B), the layout of the numpy result; NaN at buses a served
+// mask marks unserved (p3s/contingency/solver_cpp.py:147-150).
 __global__ void k_finalize(const double* __restrict__ Vm, const double* __restrict__ cs,
-                           const double* __restrict__ sn, double* __restrict__ V, int n, int S, int B) {
+                           const double* __restrict__ sn, const unsigned char* __restrict__ served,
+                           double* __restrict__ V, int n, int S, int B) {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const int bus = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (bus >= n || c >= B) return;
     const size_t o = at(bus, S, c);
     const double vm = Vm[o];
-    V[2 * ((size_t)bus * B + c)] = vm * cs[o];
-    V[2 * ((size_t)bus * B + c) + 1] = vm * sn[o];
+    double re = vm * cs[o], im = vm * sn[o];
+    if (served && !served[o]) re = im = __longlong_as_double(0x7ff8000000000000ll);
+    V[2 * ((size_t)bus * B + c)] = re;
+    V[2 * ((size_t)bus * B + c) + 1] = im;
+}
+
+
+// ===================================================================== N-1 islands
+// Union-find over the live, non-outaged branches of one case per workgroup. A root is
+// hooked under the smaller of the two roots, so every non-root points to a smaller bus
+// index and the final root of an island is its smallest bus. The parent array lives in
+// global memory; every access to it is an agent-scope atomic (CAS hooking and the reads
+// that race with it go to L2, no stale L1 line is ever read), and every find walks a
+// strictly decreasing chain of bus indices, so it ends.
+__device__ __forceinline__ int uf_ld(int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void uf_st(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ int uf_find(int* parent, int x) {
+    for (;;) {
+        const int p = uf_ld(&parent[x]);
+        if (p == x) return x;
+        const int gp = uf_ld(&parent[p]);
+        if (gp != p) uf_st(&parent[x], gp);  // path halving: gp < p < x
+        x = gp;
+    }
+}
+
+__device__ void uf_unite(int* parent, int a, int b) {
+    for (;;) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a < b) { const int t = a; a = b; b = t; }
+        if (atomicCAS(&parent[a], a, b) == a) return;  // else a was hooked meanwhile: retry
+    }
+}
+
+constexpr int MAX_GROUP = 256;   // branches one outage case may remove
+constexpr unsigned long long INF_BITS = 0x7ff0000000000000ull;
+
+// Islands and solver inputs of cases [c0, c0 + gridDim.x) (p3s/contingency/case_generator.py:
+// 355-430, p3s/contingency/solver_cpp.py:119-134). Column cl = blockIdx.x of the outputs
+// (stride S): served, ref bus (or -1), pin (re-slack references and unserved buses) and the
+// start voltage (|V| + 0j at a re-slack reference, 1 + 0j at unserved buses).
+__global__ __launch_bounds__(256) void k_islands(
+    int n, int nbr, const int* __restrict__ f, const int* __restrict__ t, const unsigned char* __restrict__ live,
+    const int* __restrict__ gptr, const int* __restrict__ gbr, int nref, const int* __restrict__ ref, int ngen,
+    const int* __restrict__ gbus, const double* __restrict__ gz, int reslack, int c0, int* __restrict__ scratch,
+    unsigned long long* __restrict__ zscratch, const double* __restrict__ Vstart, unsigned char* __restrict__ served,
+    int* __restrict__ refbus, unsigned char* __restrict__ pin, double* __restrict__ V0, int* __restrict__ npinned,
+    int S) {
+    __shared__ int gm[MAX_GROUP];
+    const int cl = blockIdx.x;
+    const int cs = c0 + cl;
+    const int g0 = gptr[cs], gsz = gptr[cs + 1] - g0;
+    for (int q = threadIdx.x; q < gsz; q += blockDim.x) gm[q] = gbr[g0 + q];
+    int* parent = scratch + (size_t)cl * 2 * n;
+    int* cref = parent + n;
+    for (int v = threadIdx.x; v < n; v += blockDim.x) {
+        uf_st(&parent[v], v);
+        uf_st(&cref[v], INT_MAX);
+    }
+    __threadfence();
+    __syncthreads();
+    for (int e = threadIdx.x; e < nbr; e += blockDim.x) {
+        if (!live[e]) continue;
+        bool out = false;
+        for (int q = 0; q < gsz; ++q) out |= gm[q] == e;
+        if (!out) uf_unite(parent, f[e], t[e]);
+    }
+    __threadfence();
+    __syncthreads();
+    for (int v = threadIdx.x; v < n; v += blockDim.x) uf_st(&parent[v], uf_find(parent, v));
+    __threadfence();
+    __syncthreads();
+    // the lowest-index slack bus of each island
+    for (int i = threadIdx.x; i < nref; i += blockDim.x) atomicMin(&cref[uf_ld(&parent[ref[i]])], ref[i]);
+    __threadfence();
+    __syncthreads();
+    unsigned long long* zb = zscratch ? zscratch + (size_t)cl * n : nullptr;
+    if (reslack && ngen > 0) {
+        // islands without a slack: the lowest generator |Z|, then the lowest bus index among
+        // the generators within np.isclose of it (rtol 1e-5, atol 1e-8)
+        for (int v = threadIdx.x; v < n; v += blockDim.x) zb[v] = INF_BITS;
+        __threadfence();
+        __syncthreads();
+        for (int i = threadIdx.x; i < ngen; i += blockDim.x) {
+            const int r = uf_ld(&parent[gbus[i]]);
+            if (uf_ld(&cref[r]) == INT_MAX) atomicMin(&zb[r], (unsigned long long)__double_as_longlong(gz[i]));
+        }
+        __threadfence();
+        __syncthreads();
+        for (int i = threadIdx.x; i < ngen; i += blockDim.x) {
+            const int r = uf_ld(&parent[gbus[i]]);
+            const unsigned long long zr = __hip_atomic_load(&zb[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (zr == INF_BITS) continue;
+            const double best = __longlong_as_double((long long)zr);
+            if (fabs(gz[i] - best) <= 1e-8 + 1e-5 * fabs(best)) atomicMin(&cref[r], gbus[i]);
+        }
+        __threadfence();
+        __syncthreads();
+    }
+    for (int v = threadIdx.x; v < n; v += blockDim.x) {
+        const int r = uf_ld(&parent[v]);
+        const int cr = uf_ld(&cref[r]);
+        const bool srv = cr != INT_MAX;
+        const bool reslacked = srv && zb != nullptr && reslack &&
+                               __hip_atomic_load(&zb[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != INF_BITS;
+        const size_t o = (size_t)v * S + cl;
+        served[o] = srv;
+        if (refbus) refbus[o] = srv ? cr : -1;
+        const bool pinned_ref = reslacked && cr == v;
+        if (pinned_ref) atomicAdd(&npinned[cl], 1);
+        if (pin) pin[o] = (!srv || pinned_ref) ? 1 : 0;
+        if (V0) {
+            double re = Vstart[2 * v], im = Vstart[2 * v + 1];
+            if (!srv) { re = 1.0; im = 0.0; }
+            else if (pinned_ref) { re = hypot(re, im); im = 0.0; }
+            V0[2 * o] = re;
+            V0[2 * o + 1] = im;
+        }
+    }
+}
+
+// Per-case Ybus = base - the stamps of the case's outaged branches, subtracted in group
+// order (p3s/contingency/case_generator.py:319-325). One thread per case.
+__global__ void k_case_y(const int* __restrict__ gptr, const int* __restrict__ gbr, const int* __restrict__ ypos,
+                         const double* __restrict__ yval, double* __restrict__ G, double* __restrict__ Bv, int c0,
+                         int nc, int S) {
+    const int cl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cl >= nc) return;
+    const int cs = c0 + cl;
+    for (int q = gptr[cs]; q < gptr[cs + 1]; ++q) {
+        const int k = gbr[q];
+        for (int j = 0; j < 4; ++j) {
+            const size_t o = at(ypos[4 * k + j], S, cl);
+            G[o] -= yval[2 * (4 * k + j)];
+            Bv[o] -= yval[2 * (4 * k + j) + 1];
+        }
+    }
 }
 
 // ===================================================================== host side
@@ -570,8 +714,9 @@ struct p3smi_batch {
     DevBuf<double> Ssp, V0, Vout;       // inputs / output in the caller's layouts
     DevBuf<double> Yg, Yb;              // per-case Ybus (N-1)
     DevBuf<unsigned char> pin;          // per-case pin mask (N-1)
+    DevBuf<unsigned char> served;       // per-case served mask (N-1 outage sets)
     int s_stride = 0, v0_stride = 0;
-    bool per_case_y = false, has_pin = false, uploaded = false;
+    bool per_case_y = false, has_pin = false, uploaded = false, mask_unserved = false;
     int* h_counts = nullptr;            // pinned host mirror of counts
     bool timing = false;
     p3smi_stats stats{};
@@ -584,6 +729,16 @@ struct p3smi_batch {
             (void)hipEventDestroy(e.second.second);
         }
     }
+};
+
+struct p3smi_outages {
+    p3smi_topology* topo = nullptr;
+    int nbr = 0, ncases = 0, nref = 0, ngen = 0, reslack = 0;
+    DevBuf<int> f, t, gptr, gbr, ref, gbus, ypos;
+    DevBuf<unsigned char> live;
+    DevBuf<double> gz, yval;
+    DevBuf<int> scratch, npinned;
+    DevBuf<unsigned long long> zscratch;
 };
 
 p3smi_topology::~p3smi_topology() {
@@ -852,8 +1007,8 @@ void download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* conve
     const p3smi::Topology& h = b->topo->host;
     hipStream_t st = b->topo->stream;
     b->Vout.alloc((size_t)2 * h.n * b->B);
-    hipLaunchKernelGGL(k_finalize, grid_items(h.n, b->S), dim3(256), 0, st, b->Vm.p, b->cs.p, b->sn.p, b->Vout.p, h.n,
-                       b->S, b->B);
+    hipLaunchKernelGGL(k_finalize, grid_items(h.n, b->S), dim3(256), 0, st, b->Vm.p, b->cs.p, b->sn.p,
+                       b->mask_unserved ? b->served.p : nullptr, b->Vout.p, h.n, b->S, b->B);
     HIPCHECK(hipGetLastError());
     std::vector<int> status(b->B), it(b->B);
     if (V_out) HIPCHECK(hipMemcpyAsync(V_out, b->Vout.p, (size_t)2 * h.n * b->B * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1065,6 +1220,7 @@ int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_
         b->v0_stride = v0_per_case ? b->B : 0;
         b->per_case_y = false;
         b->has_pin = false;
+        b->mask_unserved = false;
         HIPCHECK(hipStreamSynchronize(st));
         b->uploaded = true;
     });
@@ -1149,6 +1305,7 @@ int p3smi_solve_batch(p3smi_topology* t, const double* S, const double* V0, int3
             }
             b->per_case_y = false;
             b->has_pin = false;
+            b->mask_unserved = false;
         };
         solve_chunked(t, T, false, false, true, up, max_iter, tol, line_search, precision, V_out, iterations, converged,
                       mismatch);
@@ -1175,6 +1332,7 @@ static void contingency_common(p3smi_topology* t, const double* S, const double*
         upload_y(b, c0, nc);
         b->per_case_y = true;
         b->has_pin = pin != nullptr;
+        b->mask_unserved = false;
         if (pin) {
             // pin (n, L) -> device (n, S): zero the padding columns
             b->pin.alloc((size_t)h.n * b->S);
@@ -1238,6 +1396,164 @@ int p3smi_solve_batch_contingency_stamp(p3smi_topology* t, const int32_t* stamp_
         };
         contingency_common(t, S, V0, pin, L, upy, max_iter, tol, line_search, precision, V_out, iterations, converged,
                            mismatch);
+    });
+}
+
+}  // extern "C"
+
+// ---- N-1 outage sets ---------------------------------------------------------------
+namespace {
+
+constexpr int ISLAND_CHUNK = 2048;  // cases per islands launch (bounds the scratch)
+
+// Islands + solver inputs of cases [c0, c0 + nc) into column-strided (stride S) outputs.
+void run_islands(p3smi_outages* o, int c0, int nc, int S, const double* dVstart, unsigned char* served,
+                 int* refbus, unsigned char* pin, double* V0, int* npinned) {
+    p3smi_topology* t = o->topo;
+    const int n = t->host.n;
+    hipStream_t st = t->stream;
+    for (int a = 0; a < nc; a += ISLAND_CHUNK) {
+        const int k = std::min(ISLAND_CHUNK, nc - a);
+        o->scratch.alloc((size_t)ISLAND_CHUNK * 2 * n);
+        if (o->reslack && o->ngen) o->zscratch.alloc((size_t)ISLAND_CHUNK * n);
+        hipLaunchKernelGGL(k_islands, dim3((unsigned)k), dim3(256), 0, st, n, o->nbr, o->f.p, o->t.p, o->live.p,
+                           o->gptr.p, o->gbr.p, o->nref, o->ref.p, o->ngen, o->gbus.p, o->gz.p, o->reslack, c0 + a,
+                           o->scratch.p, (o->reslack && o->ngen) ? o->zscratch.p : nullptr, dVstart, served + a,
+                           refbus ? refbus + a : nullptr, pin ? pin + a : nullptr, V0 ? V0 + 2 * (size_t)a : nullptr,
+                           npinned + a, S);
+        HIPCHECK(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int p3smi_outages_create(p3smi_topology* t, int32_t nbr, const int32_t* from, const int32_t* to, const int32_t* ypos,
+                         const double* yval, int32_t n_cases, const int32_t* gptr, const int32_t* gbr, int32_t nref,
+                         const int32_t* ref, const double* gen_z, int32_t reslack, p3smi_outages** out) {
+    return guarded([&] {
+        const p3smi::Topology& h = t->host;
+        if (nbr < 0 || n_cases < 0 || nref < 0) throw Error(P3SMI_E_ARG, "outages: negative size");
+        for (int k = 0; k < nbr; ++k) {
+            if (from[k] < 0 || from[k] >= h.n || to[k] < 0 || to[k] >= h.n)
+                throw Error(P3SMI_E_ARG, "outages: branch end out of range");
+            for (int j = 0; j < 4; ++j)
+                if (ypos[4 * k + j] < 0 || ypos[4 * k + j] >= h.nnzY)
+                    throw Error(P3SMI_E_ARG, "outages: stamp position out of range");
+        }
+        if (gptr[0] != 0) throw Error(P3SMI_E_ARG, "outages: gptr[0] must be 0");
+        for (int c = 0; c < n_cases; ++c) {
+            if (gptr[c + 1] < gptr[c]) throw Error(P3SMI_E_ARG, "outages: gptr not monotone");
+            if (gptr[c + 1] - gptr[c] > MAX_GROUP)
+                throw Error(P3SMI_E_ARG, "outages: an outage case removes more than 256 branches");
+        }
+        for (int q = 0; q < (n_cases ? gptr[n_cases] : 0); ++q)
+            if (gbr[q] < 0 || gbr[q] >= nbr) throw Error(P3SMI_E_ARG, "outages: branch index out of range");
+        for (int i = 0; i < nref; ++i)
+            if (ref[i] < 0 || ref[i] >= h.n) throw Error(P3SMI_E_ARG, "outages: slack bus out of range");
+        std::unique_ptr<p3smi_outages> o(new p3smi_outages());
+        o->topo = t;
+        o->nbr = nbr; o->ncases = n_cases; o->nref = nref; o->reslack = reslack ? 1 : 0;
+        ensure_device(t);
+        hipStream_t st = t->stream;
+        std::vector<unsigned char> live(nbr);
+        for (int k = 0; k < nbr; ++k)
+            live[k] = (yval[2 * (4 * k + 1)] != 0.0 || yval[2 * (4 * k + 1) + 1] != 0.0 ||
+                       yval[2 * (4 * k + 2)] != 0.0 || yval[2 * (4 * k + 2) + 1] != 0.0);
+        std::vector<int> gb;
+        std::vector<double> gzv;
+        if (gen_z)
+            for (int i = 0; i < h.n; ++i)
+                if (std::isfinite(gen_z[i])) {
+                    if (gen_z[i] < 0) throw Error(P3SMI_E_ARG, "outages: generator |Z| must be >= 0");
+                    gb.push_back(i);
+                    gzv.push_back(gen_z[i]);
+                }
+        o->ngen = (int)gb.size();
+        o->f.upload(from, nbr, st);
+        o->t.upload(to, nbr, st);
+        o->live.upload(live.data(), live.size(), st);
+        o->ypos.upload(ypos, 4 * (size_t)nbr, st);
+        o->yval.upload(yval, 8 * (size_t)nbr, st);
+        o->gptr.upload(gptr, (size_t)n_cases + 1, st);
+        o->gbr.upload(gbr, (size_t)(n_cases ? gptr[n_cases] : 0), st);
+        o->ref.upload(ref, nref, st);
+        o->gbus.upload(gb.data(), gb.size(), st);
+        o->gz.upload(gzv.data(), gzv.size(), st);
+        HIPCHECK(hipStreamSynchronize(st));
+        *out = o.release();
+    });
+}
+
+void p3smi_outages_free(p3smi_outages* o) { delete o; }
+
+int p3smi_outages_islands(p3smi_outages* o, uint8_t* served, int32_t* ref_bus, int32_t* pinned) {
+    return guarded([&] {
+        p3smi_topology* t = o->topo;
+        std::lock_guard<std::mutex> lk(t->mu);
+        const int n = t->host.n, L = o->ncases;
+        if (L == 0) return;
+        hipStream_t st = t->stream;
+        DevBuf<unsigned char> dsrv;
+        DevBuf<int> dref, dnp;
+        DevBuf<double> dv;
+        dsrv.alloc((size_t)n * L);
+        dref.alloc((size_t)n * L);
+        dnp.alloc(L);
+        dv.alloc(2 * (size_t)n);
+        HIPCHECK(hipMemsetAsync(dnp.p, 0, sizeof(int) * L, st));
+        HIPCHECK(hipMemsetAsync(dv.p, 0, sizeof(double) * 2 * n, st));
+        run_islands(o, 0, L, L, dv.p, dsrv.p, dref.p, nullptr, nullptr, dnp.p);
+        if (served) HIPCHECK(hipMemcpyAsync(served, dsrv.p, (size_t)n * L, hipMemcpyDeviceToHost, st));
+        if (ref_bus) HIPCHECK(hipMemcpyAsync(ref_bus, dref.p, (size_t)n * L * sizeof(int), hipMemcpyDeviceToHost, st));
+        if (pinned) HIPCHECK(hipMemcpyAsync(pinned, dnp.p, (size_t)L * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    });
+}
+
+int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start, int32_t max_iter, double tol,
+                        int32_t line_search, int32_t precision, double* V_out, int32_t* iterations,
+                        uint8_t* converged, double* mismatch, uint8_t* served) {
+    return guarded([&] {
+        p3smi_topology* t = o->topo;
+        std::lock_guard<std::mutex> lk(t->mu);
+        const p3smi::Topology& h = t->host;
+        const int L = o->ncases;
+        if (L <= 0) throw Error(P3SMI_E_ARG, "outages: no cases");
+        if (max_iter <= 0) throw Error(P3SMI_E_ARG, "solve_batch_contingency: max_iter must be > 0");
+        if (!(tol > 0.0)) throw Error(P3SMI_E_ARG, "solve_batch_contingency: tol must be > 0");
+        check_v0(V_start, h.n, 1, "solve_batch_contingency: ");
+        DevBuf<double> dvs;
+        dvs.upload(V_start, 2 * (size_t)h.n, t->stream);
+        auto up = [&](p3smi_batch* b, int c0, int nc) {
+            hipStream_t st = t->stream;
+            b->Ssp.upload(S, 2 * (size_t)h.n, st);
+            b->s_stride = 0;
+            b->V0.alloc(2 * (size_t)h.n * b->S);
+            b->v0_stride = b->S;
+            b->pin.alloc((size_t)h.n * b->S);
+            b->served.alloc((size_t)h.n * b->S);
+            HIPCHECK(hipMemsetAsync(b->pin.p, 0, (size_t)h.n * b->S, st));
+            o->npinned.alloc(b->S);
+            HIPCHECK(hipMemsetAsync(o->npinned.p, 0, sizeof(int) * b->S, st));
+            run_islands(o, c0, nc, b->S, dvs.p, b->served.p, nullptr, b->pin.p, b->V0.p, o->npinned.p);
+            b->Yg.alloc((size_t)h.nnzY * b->S);
+            b->Yb.alloc((size_t)h.nnzY * b->S);
+            hipLaunchKernelGGL(k_y_base, grid_items(h.nnzY, b->S), dim3(256), 0, st, t->dev.G.p, t->dev.B.p, b->Yg.p,
+                               b->Yb.p, h.nnzY, b->S);
+            hipLaunchKernelGGL(k_case_y, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, o->gptr.p, o->gbr.p,
+                               o->ypos.p, o->yval.p, b->Yg.p, b->Yb.p, c0, nc, b->S);
+            HIPCHECK(hipGetLastError());
+            b->per_case_y = true;
+            b->has_pin = true;
+            b->mask_unserved = true;
+            if (served) {
+                HIPCHECK(hipMemcpy2DAsync(served + c0, L, b->served.p, b->S, nc, h.n, hipMemcpyDeviceToHost, st));
+            }
+        };
+        solve_chunked(t, L, true, true, false, up, max_iter, tol, line_search, precision, V_out, iterations, converged,
+                      mismatch);
     });
 }
 

@@ -86,6 +86,7 @@ EXPORTS = [
     "p3smi_solve_batch", "p3smi_solve_batch_contingency", "p3smi_solve_batch_contingency_stamp",
     "p3smi_batch_create", "p3smi_batch_free", "p3smi_batch_upload", "p3smi_batch_solve", "p3smi_batch_download",
     "p3smi_batch_stats", "p3smi_batch_set_timing", "p3smi_batch_bytes_per_case",
+    "p3smi_outages_create", "p3smi_outages_free", "p3smi_outages_islands", "p3smi_outages_solve",
 ]
 
 _lib = None
@@ -127,6 +128,11 @@ def lib():
     L.p3smi_batch_stats.argtypes = [P, P]
     L.p3smi_batch_set_timing.argtypes = [P, _i32]
     L.p3smi_batch_bytes_per_case.argtypes = [P, P]
+    L.p3smi_outages_create.argtypes = [P, _i32, P, P, P, P, _i32, P, P, _i32, P, P, _i32, P]
+    L.p3smi_outages_free.argtypes = [P]
+    L.p3smi_outages_free.restype = None
+    L.p3smi_outages_islands.argtypes = [P, P, P, P]
+    L.p3smi_outages_solve.argtypes = [P, P, P, _i32, _f64, _i32, _i32, P, P, P, P, P]
     _lib = L
     return L
 

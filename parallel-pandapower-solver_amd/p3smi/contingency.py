@@ -1,0 +1,167 @@
+"""N-1 contingency analysis on the device: outage cases, islands, reference buses, solves.
+
+The host side restates the input preparation of the reference's case generator
+(p3s/contingency/case_generator.py) for branch lists given directly in p.u.:
+
+  * every branch's 2x2 stamp and the four Ybus CSR positions it lands on
+    (``_index_branches``, :270-317);
+  * live branches: a branch couples its ends iff its Y_ft or Y_tf stamp is non-zero (:316);
+  * one outage case per group of branches; ``single_branch_outages`` gives the N-1 set.
+
+Everything per case then runs on the GPU (libp3smi.so): the island labels by union-find,
+the reference of every island (lowest-index slack, else with re-slacking the lowest-|Z|
+generator, ties within np.isclose to the lowest bus index), the served mask, the pin mask
+and start voltage (p3s/contingency/solver_cpp.py:119-134), the per-case Ybus (base minus
+the outaged stamps) and the Newton solve. Voltages come back NaN at unserved buses.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ptr
+from .nr_klu import Solver
+
+
+@dataclass
+class ContingencyResultTable:
+    """Per-case results (p3s/contingency/solver_cpp.py:47-61)."""
+
+    groups: list
+    V: np.ndarray  # (n, L) complex, NaN at unserved buses
+    served: np.ndarray  # (n, L) bool
+    converged: np.ndarray  # (L,) bool
+    iterations: np.ndarray  # (L,) int
+    mismatch: np.ndarray  # (L,) final infinity-norm mismatch (FP64, on the device)
+
+    @property
+    def vm(self) -> np.ndarray:
+        return np.abs(self.V)
+
+    @property
+    def va(self) -> np.ndarray:
+        return np.degrees(np.angle(self.V))
+
+
+def branch_positions(Yp, Yj, f, t) -> np.ndarray:
+    """(nbr, 4) Ybus CSR data positions of (f,f), (f,t), (t,f), (t,t) for every branch."""
+    Yp = np.asarray(Yp, dtype=np.int64)
+    Yj = np.asarray(Yj, dtype=np.int64)
+    n = Yp.shape[0] - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(Yp))
+    keys = rows * n + Yj  # ascending: rows ascending, columns sorted within a row
+    f = np.asarray(f, dtype=np.int64)
+    t = np.asarray(t, dtype=np.int64)
+    want = np.stack([f * n + f, f * n + t, t * n + f, t * n + t], axis=1)
+    pos = np.searchsorted(keys, want)
+    if (pos >= keys.shape[0]).any() or (keys[np.minimum(pos, keys.shape[0] - 1)] != want).any():
+        raise ValueError("a branch stamp entry is not a structural nonzero of Ybus")
+    return pos.astype(np.int32)
+
+
+def single_branch_outages(nbr: int) -> list:
+    """The N-1 set: one case per branch."""
+    return [np.array([k], dtype=np.int32) for k in range(nbr)]
+
+
+class ContingencySet:
+    """Outage cases of one topology, resident on the device."""
+
+    def __init__(self, solver: Solver, f, t, yff, yft, ytf, ytt, groups, ref, gen_z=None, reslack: bool = False,
+                 names=None):
+        self.solver = solver
+        self.n = solver.n
+        self.f = np.ascontiguousarray(f, dtype=np.int32)
+        self.t = np.ascontiguousarray(t, dtype=np.int32)
+        self.nbr = int(self.f.shape[0])
+        self.ypos = np.ascontiguousarray(branch_positions(solver.Yp, solver.Yj, self.f, self.t))
+        self.yval = np.ascontiguousarray(np.stack([yff, yft, ytf, ytt], axis=1).astype(np.complex128))
+        self.live = (self.yval[:, 1] != 0) | (self.yval[:, 2] != 0)
+        self.groups = [np.asarray(g, dtype=np.int32).ravel() for g in groups]
+        self.names = list(names) if names is not None else list(range(len(self.groups)))
+        self.L = len(self.groups)
+        gptr = np.zeros(self.L + 1, dtype=np.int32)
+        gptr[1:] = np.cumsum([g.size for g in self.groups])
+        gbr = np.concatenate(self.groups).astype(np.int32) if self.L else np.zeros(0, np.int32)
+        self.ref = np.ascontiguousarray(ref, dtype=np.int32).ravel()
+        gz = None if gen_z is None else np.ascontiguousarray(gen_z, dtype=np.float64)
+        if gz is not None and gz.shape != (self.n,):
+            raise ValueError("gen_z must be (n_bus,) with +inf where a bus has no eligible generator")
+        self.gen_z = gz
+        self.reslack = bool(reslack)
+        h = C.c_void_p()
+        check(lib().p3smi_outages_create(solver._h, self.nbr, ptr(self.f), ptr(self.t), ptr(self.ypos),
+                                         ptr(self.yval), self.L, ptr(gptr), ptr(gbr if gbr.size else np.zeros(1, np.int32)),
+                                         int(self.ref.size), ptr(self.ref), ptr(gz), int(self.reslack), C.byref(h)),
+              "contingency: ")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().p3smi_outages_free(h)
+            self._h = None
+
+    def islands(self):
+        """served (n, L) bool, ref_bus (n, L) int32 (-1 = none), re-slacked references (L,)."""
+        served = np.empty((self.n, self.L), dtype=np.uint8)
+        refb = np.empty((self.n, self.L), dtype=np.int32)
+        pinned = np.empty(self.L, dtype=np.int32)
+        check(lib().p3smi_outages_islands(self._h, ptr(served), ptr(refb), ptr(pinned)))
+        return served.astype(bool), refb, pinned
+
+    def case_ybus(self, c: int) -> np.ndarray:
+        """Ybus values of case c (host, for checks): base minus the outaged stamps."""
+        Yx = self.solver.Yx.copy()
+        for k in self.groups[c]:
+            for q in range(4):
+                Yx[self.ypos[k, q]] -= self.yval[k, q]
+        return Yx
+
+    def solve(self, Sbus, V_start, max_iter: int = 30, tol: float = 1e-8, line_search: bool = True,
+              precision: str = "double") -> ContingencyResultTable:
+        from .nr_klu import _prec
+
+        S = np.ascontiguousarray(Sbus, dtype=np.complex128).ravel()
+        V0 = np.ascontiguousarray(V_start, dtype=np.complex128).ravel()
+        if S.shape != (self.n,) or V0.shape != (self.n,):
+            raise RuntimeError("solve_batch_contingency: Sbus and V_start must be (n_bus,)")
+        V = np.empty((self.n, self.L), dtype=np.complex128)
+        it = np.empty(self.L, dtype=np.int32)
+        cv = np.empty(self.L, dtype=np.uint8)
+        mis = np.empty(self.L, dtype=np.float64)
+        served = np.empty((self.n, self.L), dtype=np.uint8)
+        check(lib().p3smi_outages_solve(self._h, ptr(S), ptr(V0), int(max_iter), float(tol), int(bool(line_search)),
+                                        _prec(precision), ptr(V), ptr(it), ptr(cv), ptr(mis), ptr(served)))
+        return ContingencyResultTable(groups=self.names, V=V, served=served.astype(bool), converged=cv.astype(bool),
+                                      iterations=it, mismatch=mis)
+
+
+def from_grid(grid, groups=None, reslack: bool = False, solver: Solver | None = None) -> ContingencySet:
+    """Outage set of a synthetic grid (default: every single-branch outage)."""
+    if solver is None:
+        Yp, Yj, Yx = grid.csr()
+        solver = Solver(Yp, Yj, Yx, grid.pv, grid.pq)
+    yff, yft, ytf, ytt = grid.branch_stamps()
+    if groups is None:
+        groups = single_branch_outages(grid.nbr)
+    return ContingencySet(solver, grid.f, grid.t, yff, yft, ytf, ytt, groups, grid.ref, grid.gen_z, reslack)
+
+
+def solve_contingencies(grid, reslack: bool = False, tol: float = 1e-8, max_iter: int = 30, init: str = "dc",
+                        precision: str = "double", groups=None) -> ContingencyResultTable:
+    """Every outage case of the grid on the device (p3s/contingency/solver_cpp.py:64-156)."""
+    from .timeseries import dc_initial_voltage
+
+    cs = from_grid(grid, groups=groups, reslack=reslack)
+    if init == "dc":
+        v_start = dc_initial_voltage(grid)
+    elif init == "flat":
+        v_start = grid.flat_start()
+    else:
+        raise ValueError(f"init must be 'dc' or 'flat', got {init!r}")
+    return cs.solve(grid.sbus(), v_start, max_iter=max_iter, tol=tol, precision=precision)
