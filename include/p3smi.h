@@ -50,8 +50,8 @@ typedef struct {
     int32_t guard_trips; /* factorizations whose pivot ratio fell below 1e-3 x the flat-start one */
     int32_t reserved;
     int64_t single_factorizations, double_factorizations, refinement_steps;
-    double ms_eval, ms_lu, ms_step, ms_decide, ms_other;
-    int32_t n_eval, n_lu, n_step, n_decide;
+    double ms_eval, ms_lu, ms_step, ms_decide, ms_other, ms_refine;
+    int32_t n_eval, n_lu, n_step, n_decide, n_refine, reserved2;
 } p3smi_stats;
 
 /* ---- library ------------------------------------------------------------------------ */
@@ -72,10 +72,14 @@ int p3smi_topology_create(const int32_t* Yp, const int32_t* Yj, const double* Yx
 void p3smi_topology_free(p3smi_topology* t);
 int p3smi_topology_info(const p3smi_topology* t, p3smi_info* out);
 /* LU launch shape (no counterpart in the reference; set by the autotuner): a workgroup of
- * 64*ns threads factors g consecutive cases (g in {1, 8, 16, 32, 64}, ns in {4, 8, 16});
+ * 64*ns threads factors g consecutive cases (g in {1, 16, 32, 64}, ns in {8, 16});
  * (few_g, few_ns) is used instead while at most few_max cases are still iterating. */
 int p3smi_topology_set_lu_shape(p3smi_topology* t, int32_t g, int32_t ns, int32_t few_g, int32_t few_ns,
                                 int32_t few_max);
+/* Mixed precision (no counterpart in the reference): refinement of the FP32-LU step stops
+ * once the correction is below rtol x the step, at most kmax steps, then falls back to an
+ * FP64 factorization of that case. Defaults 1e-12 and 6. */
+int p3smi_topology_set_mixed(p3smi_topology* t, double rtol, int32_t kmax);
 /* Refresh Ybus values on the same pattern: nr_klu.Solver.update_Y (p3s/cpp/nr_klu.cpp:999-1011). */
 int p3smi_update_y(p3smi_topology* t, const double* Yx, int32_t nnz);
 

@@ -234,15 +234,16 @@ __global__ __launch_bounds__(64 * EVAL_WAVES) void k_eval(
 
 // acc - sum_e fac[pa_e] * fac[pb_e] over the operand pairs [e, e1) of one factor entry.
 // Four pairs per trip: eight independent loads in flight before the first use.
-__device__ __forceinline__ double dot_pairs(const int2* __restrict__ pairs, int e, int e1,
-                                            const double* __restrict__ fac, int S, int c, double acc) {
-    double acc1 = 0.0;
+template <typename real>
+__device__ __forceinline__ real dot_pairs(const int2* __restrict__ pairs, int e, int e1, const real* __restrict__ fac,
+                                          int S, int c, real acc) {
+    real acc1 = 0;
     for (; e + 4 <= e1; e += 4) {
         const int2 p0 = pairs[e], p1 = pairs[e + 1], p2 = pairs[e + 2], p3 = pairs[e + 3];
-        const double a0 = fac[at(p0.x, S, c)], b0 = fac[at(p0.y, S, c)];
-        const double a1 = fac[at(p1.x, S, c)], b1 = fac[at(p1.y, S, c)];
-        const double a2 = fac[at(p2.x, S, c)], b2 = fac[at(p2.y, S, c)];
-        const double a3 = fac[at(p3.x, S, c)], b3 = fac[at(p3.y, S, c)];
+        const real a0 = fac[at(p0.x, S, c)], b0 = fac[at(p0.y, S, c)];
+        const real a1 = fac[at(p1.x, S, c)], b1 = fac[at(p1.y, S, c)];
+        const real a2 = fac[at(p2.x, S, c)], b2 = fac[at(p2.y, S, c)];
+        const real a3 = fac[at(p3.x, S, c)], b3 = fac[at(p3.y, S, c)];
         acc = fma(-a0, b0, acc);
         acc1 = fma(-a1, b1, acc1);
         acc = fma(-a2, b2, acc);
@@ -273,98 +274,136 @@ __device__ __forceinline__ double dot_pairs(const int2* __restrict__ pairs, int 
 // the rest of the solve to a pivoting factorization) is counted in guard_trips and the
 // static-pivot step is kept: the step is the same solution of J dx = R, and the oracle's
 // runs with and without that fallback agree to roundoff on every converging case.
-template <int G, int NS>
+// Which cases a launch works on (besides being ACTIVE): every one (SEL_ALL, FP64 path
+// without mixed precision), those whose vote is FP64 (SEL_F64), FP32 (SEL_F32), or FP32 and
+// still refining (SEL_REFINE).
+enum : int { SEL_ALL = 0, SEL_F64 = 1, SEL_F32 = 2, SEL_REFINE = 3 };
+
+__device__ __forceinline__ bool selected(int sel, int c, const int* status, const int* vote, const int* rf) {
+    if (status[c] != ST_ACTIVE) return false;
+    switch (sel) {
+        case SEL_F64: return vote[c] == 0;
+        case SEL_F32: return vote[c] == 1;
+        case SEL_REFINE: return vote[c] == 1 && rf[c] == 1;
+        default: return true;
+    }
+}
+
+// FACTOR = true: factorize (real = double in place of the Jacobian's copy, real = float in
+// the single-precision factor array) and solve J x = rhs. FACTOR = false: solve only, with
+// the factors of an earlier launch (the refinement solves). Substitutions accumulate in
+// FP64 whatever the factor precision.
+// Pivot failures: FP64 -- a zero or non-finite pivot fails the case, a ratio under the guard
+// is counted; FP32 -- either hands the case to the FP64 factorization of the same iteration
+// (vote = 0), as the reference's single-precision kernel does
+// (p3s/cuda/nr_polar_kernels.cu:477-482).
+template <int G, int NS, typename real, bool FACTOR>
 __global__ __launch_bounds__(64 * NS) void k_lu(
     const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
     const int4* __restrict__ taskB, const int* __restrict__ diagB, const int2* __restrict__ pairs,
     const int* __restrict__ ptrF, const int4* __restrict__ fwd, const int2* __restrict__ fent,
     const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int* __restrict__ Li,
-    const double* __restrict__ Jx, double* __restrict__ fac, const double* __restrict__ R, double* __restrict__ x,
+    const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
     int* __restrict__ status, int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips,
-    double rcond_min, int nlev, int nblev, int S, int it) {
+    int* __restrict__ vote, const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S,
+    int it) {
+    constexpr bool SINGLE = sizeof(real) == 4;
     constexpr int TL = 64 * NS / G;
     __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
     __shared__ int sbad[G];
     const int cl = threadIdx.x % G;
     const int slot = G == 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G)) : (int)(threadIdx.x / G);
     const int c = blockIdx.x * G + cl;
-    bool act = status[c] == ST_ACTIVE;
+    bool act = selected(sel, c, status, vote, rf);
     if (!__syncthreads_or(act)) return;
 
     double pmin = DBL_MAX, pmax = 0.0, psum = 0.0;
     for (int l = 0; l < nlev; ++l) {
         if (act) {
-            const int q0 = ptrA[l * TL + slot], q1 = ptrA[l * TL + slot + 1];
-            for (int q = q0; q < q1; ++q) {
-                const int4 tk = taskA[q];
-                const bool diag = tk.y < -1;
-                const int src = diag ? -tk.y - 2 : tk.y;
-                double acc = dot_pairs(pairs, tk.z, tk.w, fac, S, c, src >= 0 ? Jx[at(src, S, c)] : 0.0);
-                if (diag) {
-                    const double ab = fabs(acc);
-                    pmin = fmin(pmin, ab);
-                    pmax = fmax(pmax, ab);
-                    psum += ab;
-                    acc = 1.0 / acc;
+            if (FACTOR) {
+                const int q0 = ptrA[l * TL + slot], q1 = ptrA[l * TL + slot + 1];
+                for (int q = q0; q < q1; ++q) {
+                    const int4 tk = taskA[q];
+                    const bool diag = tk.y < -1;
+                    const int src = diag ? -tk.y - 2 : tk.y;
+                    real acc = dot_pairs<real>(pairs, tk.z, tk.w, fac, S, c, src >= 0 ? (real)Jx[at(src, S, c)] : (real)0);
+                    if (diag) {
+                        const double ab = fabs((double)acc);
+                        pmin = fmin(pmin, ab);
+                        pmax = fmax(pmax, ab);
+                        psum += ab;
+                        acc = (real)1 / acc;
+                    }
+                    fac[at(tk.x, S, c)] = acc;
                 }
-                fac[at(tk.x, S, c)] = acc;
             }
             const int f0 = ptrF[l * TL + slot], f1 = ptrF[l * TL + slot + 1];
             for (int q = f0; q < f1; ++q) {
                 const int4 fr = fwd[q];
-                double acc = R[at(fr.y, S, c)], acc1 = 0.0;
+                double acc = rhs[at(fr.y, S, c)], acc1 = 0.0;
                 int e = fr.z;
                 for (; e + 2 <= fr.w; e += 2) {
                     const int2 f0e = fent[e], f1e = fent[e + 1];
-                    const double a0 = fac[at(f0e.x, S, c)], y0 = x[at(f0e.y, S, c)];
-                    const double a1 = fac[at(f1e.x, S, c)], y1 = x[at(f1e.y, S, c)];
+                    const double a0 = (double)fac[at(f0e.x, S, c)], y0 = x[at(f0e.y, S, c)];
+                    const double a1 = (double)fac[at(f1e.x, S, c)], y1 = x[at(f1e.y, S, c)];
                     acc = fma(-a0, y0, acc);
                     acc1 = fma(-a1, y1, acc1);
                 }
                 if (e < fr.w) {
                     const int2 fe = fent[e];
-                    acc = fma(-fac[at(fe.x, S, c)], x[at(fe.y, S, c)], acc);
+                    acc = fma(-(double)fac[at(fe.x, S, c)], x[at(fe.y, S, c)], acc);
                 }
                 x[at(fr.x, S, c)] = acc + acc1;
             }
         }
         __syncthreads();
-        if (act) {
-            const int q0 = ptrB[l * TL + slot], q1 = ptrB[l * TL + slot + 1];
-            for (int q = q0; q < q1; ++q) {
-                const int4 tk = taskB[q];
-                const double acc = dot_pairs(pairs, tk.z, tk.w, fac, S, c, tk.y >= 0 ? Jx[at(tk.y, S, c)] : 0.0);
-                fac[at(tk.x, S, c)] = acc * fac[at(diagB[q], S, c)];
+        if (FACTOR) {
+            if (act) {
+                const int q0 = ptrB[l * TL + slot], q1 = ptrB[l * TL + slot + 1];
+                for (int q = q0; q < q1; ++q) {
+                    const int4 tk = taskB[q];
+                    const real acc =
+                        dot_pairs<real>(pairs, tk.z, tk.w, fac, S, c, tk.y >= 0 ? (real)Jx[at(tk.y, S, c)] : (real)0);
+                    fac[at(tk.x, S, c)] = acc * fac[at(diagB[q], S, c)];
+                }
             }
+            __syncthreads();
+        }
+    }
+    if (FACTOR) {
+        smin[slot][cl] = pmin;
+        smax[slot][cl] = pmax;
+        ssum[slot][cl] = psum;
+        __syncthreads();
+        if (slot == 0) {
+            int bad = 0;
+            if (act) {
+                for (int s2 = 1; s2 < TL; ++s2) {
+                    pmin = fmin(pmin, smin[s2][cl]);
+                    pmax = fmax(pmax, smax[s2][cl]);
+                    psum += ssum[s2][cl];
+                }
+                pstat[2 * (size_t)c] = pmin;
+                pstat[2 * (size_t)c + 1] = pmax;
+                const bool broken = !(psum < DBL_MAX && pmin > 0.0);
+                if (SINGLE) {
+                    if (broken || pmin < rcond_min * pmax) {
+                        bad = 1;
+                        vote[c] = 0;
+                    }
+                } else if (broken) {  // zero or non-finite pivot: no step
+                    bad = 1;
+                    status[c] = ST_FAILED;
+                    iters[c] = it;
+                } else if (pmin < rcond_min * pmax) {
+                    atomicAdd(guard_trips, 1);
+                }
+            }
+            sbad[cl] = bad;
         }
         __syncthreads();
+        if (sbad[cl]) act = false;
     }
-    smin[slot][cl] = pmin;
-    smax[slot][cl] = pmax;
-    ssum[slot][cl] = psum;
-    __syncthreads();
-    if (slot == 0) {
-        int bad = 0;
-        if (act) {
-            for (int s2 = 1; s2 < TL; ++s2) {
-                pmin = fmin(pmin, smin[s2][cl]);
-                pmax = fmax(pmax, smax[s2][cl]);
-                psum += ssum[s2][cl];
-            }
-            pstat[2 * (size_t)c] = pmin;
-            pstat[2 * (size_t)c + 1] = pmax;
-            if (!(psum < DBL_MAX && pmin > 0.0)) {  // zero or non-finite pivot: no step
-                bad = 1;
-                status[c] = ST_FAILED;
-                iters[c] = it;
-            } else if (pmin < rcond_min * pmax) {
-                atomicAdd(guard_trips, 1);
-            }
-        }
-        sbad[cl] = bad;
-    }
-    __syncthreads();
-    if (sbad[cl]) act = false;
     for (int l = 0; l < nblev; ++l) {
         if (act) {
             const int q0 = ptrW[l * TL + slot], q1 = ptrW[l * TL + slot + 1];
@@ -374,17 +413,116 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
                 const int pos0 = br.y + 1 - br.z;
                 int e = br.z;
                 for (; e + 2 <= br.w; e += 2) {
-                    const double u0 = fac[at(pos0 + e, S, c)], x0 = x[at(Li[e], S, c)];
-                    const double u1 = fac[at(pos0 + e + 1, S, c)], x1 = x[at(Li[e + 1], S, c)];
+                    const double u0 = (double)fac[at(pos0 + e, S, c)], x0 = x[at(Li[e], S, c)];
+                    const double u1 = (double)fac[at(pos0 + e + 1, S, c)], x1 = x[at(Li[e + 1], S, c)];
                     acc = fma(-u0, x0, acc);
                     acc1 = fma(-u1, x1, acc1);
                 }
-                if (e < br.w) acc = fma(-fac[at(pos0 + e, S, c)], x[at(Li[e], S, c)], acc);
-                x[at(br.x, S, c)] = (acc + acc1) * fac[at(br.y, S, c)];
+                if (e < br.w) acc = fma(-(double)fac[at(pos0 + e, S, c)], x[at(Li[e], S, c)], acc);
+                x[at(br.x, S, c)] = (acc + acc1) * (double)fac[at(br.y, S, c)];
             }
         }
         __syncthreads();
     }
+}
+
+// Refinement residual r = R - J x in FP64 (J in equation-row CSR order, x in pivot order:
+// xcol[e] = pivot index of the column of entry e), one wavefront per row and 64 cases.
+__global__ __launch_bounds__(256) void k_resid(const int* __restrict__ Jp, const int* __restrict__ xcol,
+                                               const double* __restrict__ Jx, const double* __restrict__ R,
+                                               const double* __restrict__ x, double* __restrict__ r,
+                                               const int* __restrict__ status, const int* __restrict__ vote,
+                                               const int* __restrict__ rf, int m, int S) {
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int row = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (row >= m || !selected(SEL_REFINE, c, status, vote, rf)) return;
+    double acc = R[at(row, S, c)], acc1 = 0.0;
+    const int e1 = Jp[row + 1];
+    int e = Jp[row];
+    for (; e + 2 <= e1; e += 2) {
+        acc = fma(-Jx[at(e, S, c)], x[at(xcol[e], S, c)], acc);
+        acc1 = fma(-Jx[at(e + 1, S, c)], x[at(xcol[e + 1], S, c)], acc1);
+    }
+    if (e < e1) acc = fma(-Jx[at(e, S, c)], x[at(xcol[e], S, c)], acc);
+    r[at(row, S, c)] = acc + acc1;
+}
+
+// x += d for the refining cases; per workgroup (64 pivots x 64 cases) the largest |d| and
+// |x| of each case go to part[2 * (blockIdx.x * S + c) + {0, 1}].
+__global__ __launch_bounds__(256) void k_refine_update(double* __restrict__ x, const double* __restrict__ d,
+                                                       double* __restrict__ part, const int* __restrict__ status,
+                                                       const int* __restrict__ vote, const int* __restrict__ rf,
+                                                       int m, int S) {
+    __shared__ double sd[4][64], sx[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = blockIdx.y * 64 + lane;
+    const bool on = selected(SEL_REFINE, c, status, vote, rf);
+    double dm = 0.0, xm = 0.0;
+    if (on) {
+        const int p1 = min(m, (int)(blockIdx.x + 1) * 64);
+        for (int p = blockIdx.x * 64 + wave; p < p1; p += 4) {
+            const size_t o = at(p, S, c);
+            const double dv = d[o], xv = x[o] + dv;
+            x[o] = xv;
+            dm = nan_max(dm, fabs(dv));
+            xm = nan_max(xm, fabs(xv));
+        }
+    }
+    sd[wave][lane] = dm;
+    sx[wave][lane] = xm;
+    __syncthreads();
+    if (wave == 0 && on) {
+        for (int v = 1; v < 4; ++v) {
+            dm = nan_max(dm, sd[v][lane]);
+            xm = nan_max(xm, sx[v][lane]);
+        }
+        part[2 * at(blockIdx.x, S, c)] = dm;
+        part[2 * at(blockIdx.x, S, c) + 1] = xm;
+    }
+}
+
+// Per refining case: stop when |d| <= rtol |x|; fall back to FP64 (vote = 0) when a step
+// did not at least halve |d| (refinement stalls: cond(J) times the FP32 rounding is too big)
+// or when the last allowed step still misses rtol. counts[0] = cases still refining.
+__global__ void k_refine_decide(const double* __restrict__ part, int nparts, double* __restrict__ prevd,
+                                const int* __restrict__ status, int* __restrict__ vote, int* __restrict__ rf,
+                                int* __restrict__ counts, double rtol, int last, int S) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < S && selected(SEL_REFINE, c, status, vote, rf)) {
+        double dm = 0.0, xm = 0.0;
+        for (int b = 0; b < nparts; ++b) {
+            dm = nan_max(dm, part[2 * at(b, S, c)]);
+            xm = nan_max(xm, part[2 * at(b, S, c) + 1]);
+        }
+        if (dm <= rtol * xm) {
+            rf[c] = 0;
+        } else if (!(dm <= 0.5 * prevd[c]) || last) {
+            rf[c] = 0;
+            vote[c] = 0;
+        } else {
+            prevd[c] = dm;
+            atomicAdd(&cnt, 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && cnt) atomicAdd(&counts[0], cnt);
+}
+
+// Start of a mixed-precision iteration: refining = FP32 vote, no previous step.
+__global__ void k_refine_init(const int* __restrict__ status, const int* __restrict__ vote, int* __restrict__ rf,
+                              double* __restrict__ prevd, int S) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= S) return;
+    rf[c] = (status[c] == ST_ACTIVE && vote[c] == 1) ? 1 : 0;
+    prevd[c] = DBL_MAX;
+}
+
+__global__ void k_fill_int(int* p, int v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // V = V0 + alpha * dx for the cases whose status is in mask. full: alpha = 1 and the
@@ -677,7 +815,7 @@ struct DevSched {  // one level schedule dealt to TL task lists
 };
 
 struct DevTopo {
-    DevBuf<int> Yp, Yj, Ydiag, row_p, row_q, posJ, xva, xvm;
+    DevBuf<int> Yp, Yj, Ydiag, row_p, row_q, posJ, xva, xvm, Jp, xcol;
     DevBuf<double> G, B;
     DevBuf<int> Li;
     DevBuf<int2> pairs, fent;
@@ -698,6 +836,8 @@ struct p3smi_topology {
     LuConfig lu_batch{16, 8};     // LU launch shape while many cases are active
     LuConfig lu_few{1, 8};        // ... and for the last few
     int lu_few_max = 256;         // "few": at most this many active cases
+    double mixed_rtol = 1e-12;    // refinement stops once |dx correction| <= rtol |dx|
+    int mixed_kmax = 6;           // refinement steps before the FP64 fallback
     hipStream_t stream = nullptr;
     std::unique_ptr<p3smi_batch> cached;  // workspace of the one-call entry points
     std::mutex mu;
@@ -709,6 +849,11 @@ struct p3smi_batch {
     int B = 0, S = 0;
     int nparts = 0;
     DevBuf<double> Vm, Va, cs, sn, Vm0, Va0, Jx, fac, R, x;
+    DevBuf<float> fac32;                // FP32 factors (mixed precision)
+    DevBuf<double> rr, dvec, prevd, rpart;  // refinement residual, correction, norms
+    DevBuf<int> vote, rf;               // per case: FP32 (1) / FP64 (0); still refining
+    int nparts_r = 0;
+    bool mixed_ready = false;
     DevBuf<double> nrm, alpha, best_alpha, best_nrm, part, pstat;
     DevBuf<int> status, iters, trial, counts, guard;
     DevBuf<double> Ssp, V0, Vout;       // inputs / output in the caller's layouts
@@ -768,6 +913,10 @@ void ensure_device(p3smi_topology* t) {
     d.xva.upload(h.xva.data(), h.xva.size(), st);
     d.xvm.upload(h.xvm.data(), h.xvm.size(), st);
     d.Li.upload(h.Li.data(), h.Li.size(), st);
+    d.Jp.upload(h.Jp.data(), h.Jp.size(), st);
+    std::vector<int> xcol(h.nnzJ);
+    for (int e = 0; e < h.nnzJ; ++e) xcol[e] = h.pinv[h.Jj[e]];
+    d.xcol.upload(xcol.data(), xcol.size(), st);
     std::vector<int2> pr(h.pa.size()), fe(h.Fpos.size());
     for (size_t i = 0; i < pr.size(); ++i) pr[i] = make_int2(h.pa[i], h.pb[i]);
     for (size_t i = 0; i < fe.size(); ++i) fe[i] = make_int2(h.Fpos[i], h.Fcol[i]);
@@ -835,6 +984,9 @@ void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     for (auto* v : {&b->status, &b->iters, &b->trial}) v->alloc(S);
     b->counts.alloc(NSTAT);
     b->guard.alloc(1);
+    b->vote.alloc(S);
+    b->rf.alloc(S);
+    b->mixed_ready = false;
     if (!b->h_counts) HIPCHECK(hipHostMalloc((void**)&b->h_counts, NSTAT * sizeof(int), hipHostMallocDefault));
 }
 
@@ -872,6 +1024,7 @@ void collect_timing(p3smi_batch* b) {
             case 1: b->stats.ms_lu += ms; b->stats.n_lu++; break;
             case 2: b->stats.ms_step += ms; b->stats.n_step++; break;
             case 3: b->stats.ms_decide += ms; b->stats.n_decide++; break;
+            case 5: b->stats.ms_refine += ms; b->stats.n_refine++; break;
             default: b->stats.ms_other += ms; break;
         }
     }
@@ -900,35 +1053,105 @@ void launch_eval(p3smi_batch* b, int run_mask, int j_mask) {
     HIPCHECK(hipGetLastError());
 }
 
-template <int G, int NS>
-void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, int it) {
+template <int G, int NS, typename real, bool FACTOR>
+void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, int it, real* fac, const double* rhs, double* xout, int sel) {
     const p3smi::Topology& h = b->topo->host;
     const DevTopo& d = b->topo->dev;
     const double rcond_min = REUSE_RCOND_RATIO * h.rcond0;
-    hipLaunchKernelGGL((k_lu<G, NS>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0, b->topo->stream, ds.ptrA.p,
-                       ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.diagB.p, d.pairs.p, ds.ptrF.p, ds.fwd.p, d.fent.p,
-                       ds.ptrW.p, ds.bwd.p, d.Li.p, b->Jx.p, b->fac.p, b->R.p, b->x.p, b->status.p, b->iters.p,
-                       b->pstat.p, b->guard.p, rcond_min, ds.nlev, ds.nblev, b->S, it);
+    hipLaunchKernelGGL((k_lu<G, NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0, b->topo->stream,
+                       ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.diagB.p, d.pairs.p, ds.ptrF.p, ds.fwd.p,
+                       d.fent.p, ds.ptrW.p, ds.bwd.p, d.Li.p, b->Jx.p, fac, rhs, xout, b->status.p, b->iters.p,
+                       b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min, ds.nlev, ds.nblev, b->S, it);
 }
 
 bool valid_lu_config(LuConfig c) {
-    const bool g = c.G == 64 || c.G == 32 || c.G == 16 || c.G == 8 || c.G == 1;
-    const bool n = c.NS == 4 || c.NS == 8 || c.NS == 16;
+    const bool g = c.G == 64 || c.G == 32 || c.G == 16 || c.G == 1;
+    const bool n = c.NS == 8 || c.NS == 16;
     return g && n;
 }
 
-void launch_lu(p3smi_batch* b, int it, int n_active) {
+// One LU launch over the selected cases; n_active picks the batch or the few-cases shape.
+template <typename real, bool FACTOR>
+void launch_lu(p3smi_batch* b, int it, int n_active, real* fac, const double* rhs, double* xout, int sel) {
     p3smi_topology* t = b->topo;
     const LuConfig cfg = n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
     const DevSched& ds = get_sched(t, cfg.tl());
     KTimer tm(b, 1);
-#define P3SMI_LU_CASE(GG, NN) \
-    if (cfg.G == GG && cfg.NS == NN) { launch_lu_cfg<GG, NN>(b, ds, it); HIPCHECK(hipGetLastError()); return; }
-#define P3SMI_LU_G(GG) P3SMI_LU_CASE(GG, 4) P3SMI_LU_CASE(GG, 8) P3SMI_LU_CASE(GG, 16)
-    P3SMI_LU_G(64) P3SMI_LU_G(32) P3SMI_LU_G(16) P3SMI_LU_G(8) P3SMI_LU_G(1)
+#define P3SMI_LU_CASE(GG, NN)                                                                \
+    if (cfg.G == GG && cfg.NS == NN) {                                                       \
+        launch_lu_cfg<GG, NN, real, FACTOR>(b, ds, it, fac, rhs, xout, sel);                 \
+        HIPCHECK(hipGetLastError());                                                         \
+        return;                                                                              \
+    }
+#define P3SMI_LU_G(GG) P3SMI_LU_CASE(GG, 8) P3SMI_LU_CASE(GG, 16)
+    P3SMI_LU_G(64) P3SMI_LU_G(32) P3SMI_LU_G(16) P3SMI_LU_G(1)
 #undef P3SMI_LU_G
 #undef P3SMI_LU_CASE
     throw Error(P3SMI_E_ARG, "unsupported LU launch shape");
+}
+
+void ensure_mixed(p3smi_batch* b) {
+    if (b->mixed_ready) return;
+    const p3smi::Topology& h = b->topo->host;
+    const size_t S = b->S;
+    b->fac32.alloc((size_t)h.nnz_lu * S);
+    b->rr.alloc((size_t)h.m * S);
+    b->dvec.alloc((size_t)h.m * S);
+    b->prevd.alloc(S);
+    b->nparts_r = (h.m + 63) / 64;
+    b->rpart.alloc(2 * (size_t)b->nparts_r * S);
+    b->mixed_ready = true;
+}
+
+const int* read_counts(p3smi_batch* b) {
+    hipStream_t st = b->topo->stream;
+    HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return b->h_counts;
+}
+
+// Newton step of the ACTIVE cases. FP64: one factor+solve launch. Mixed: FP32 factors and
+// solve, then refinement x += LU32^-1 (R - J x) with the residual in FP64 until the
+// correction is below rtol |x|; cases whose FP32 pivots fail or whose refinement stalls
+// are factored in FP64 in the same iteration.
+void linear_solve(p3smi_batch* b, int it, int n_active, int precision) {
+    p3smi_topology* t = b->topo;
+    const p3smi::Topology& h = t->host;
+    hipStream_t st = t->stream;
+    if (precision != P3SMI_PREC_MIXED) {
+        b->stats.double_factorizations += n_active;
+        launch_lu<double, true>(b, it, n_active, b->fac.p, b->R.p, b->x.p, SEL_ALL);
+        return;
+    }
+    ensure_mixed(b);
+    const unsigned gS = (unsigned)((b->S + 255) / 256);
+    {
+        KTimer tm(b, 4);
+        hipLaunchKernelGGL(k_refine_init, dim3(gS), dim3(256), 0, st, b->status.p, b->vote.p, b->rf.p, b->prevd.p, b->S);
+    }
+    launch_lu<float, true>(b, it, n_active, b->fac32.p, b->R.p, b->x.p, SEL_F32);
+    for (int k = 1; k <= t->mixed_kmax; ++k) {
+        {
+            KTimer tm(b, 5);
+            hipLaunchKernelGGL(k_resid, dim3((unsigned)(b->S / 64), (unsigned)((h.m + 3) / 4)), dim3(256), 0, st,
+                               t->dev.Jp.p, t->dev.xcol.p, b->Jx.p, b->R.p, b->x.p, b->rr.p, b->status.p, b->vote.p,
+                               b->rf.p, h.m, b->S);
+        }
+        launch_lu<float, false>(b, it, n_active, b->fac32.p, b->rr.p, b->dvec.p, SEL_REFINE);
+        {
+            KTimer tm(b, 5);
+            hipLaunchKernelGGL(k_refine_update, dim3((unsigned)b->nparts_r, (unsigned)(b->S / 64)), dim3(256), 0, st,
+                               b->x.p, b->dvec.p, b->rpart.p, b->status.p, b->vote.p, b->rf.p, h.m, b->S);
+            hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->counts.p, NSTAT);
+            hipLaunchKernelGGL(k_refine_decide, dim3(gS), dim3(256), 0, st, b->rpart.p, b->nparts_r, b->prevd.p,
+                               b->status.p, b->vote.p, b->rf.p, b->counts.p, t->mixed_rtol, (int)(k == t->mixed_kmax),
+                               b->S);
+            HIPCHECK(hipGetLastError());
+        }
+        b->stats.refinement_steps += 1;
+        if (read_counts(b)[0] == 0) break;
+    }
+    launch_lu<double, true>(b, it, n_active, b->fac.p, b->R.p, b->x.p, SEL_F64);
 }
 
 void launch_step(p3smi_batch* b, int mask, int full) {
@@ -969,6 +1192,8 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
         hipLaunchKernelGGL(k_status_init, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p, b->B,
                            b->S);
         hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 1);
+        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->vote.p,
+                           precision == P3SMI_PREC_MIXED ? 1 : 0, b->S);
         hipLaunchKernelGGL(k_init, grid_items(h.n, b->S), dim3(256), 0, st, b->V0.p, b->v0_stride, b->Vm.p, b->Va.p,
                            b->cs.p, b->sn.p, h.n, b->S, b->B);
         HIPCHECK(hipGetLastError());
@@ -980,8 +1205,7 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
     while (it < max_iter && cnt[ST_ACTIVE] > 0) {
         ++it;
         b->stats.newton_passes++;
-        b->stats.double_factorizations += cnt[ST_ACTIVE];
-        launch_lu(b, it, cnt[ST_ACTIVE]);
+        linear_solve(b, it, cnt[ST_ACTIVE], precision);
         launch_step(b, A, 1);
         launch_eval(b, A, A);
         cnt = launch_decide(b, tol, line_search, 1, it, max_iter);
@@ -1140,12 +1364,21 @@ int p3smi_topology_info(const p3smi_topology* t, p3smi_info* o) {
     });
 }
 
+int p3smi_topology_set_mixed(p3smi_topology* t, double rtol, int32_t kmax) {
+    return guarded([&] {
+        if (!(rtol > 0.0) || kmax < 1) throw Error(P3SMI_E_ARG, "mixed precision: rtol must be > 0 and kmax >= 1");
+        std::lock_guard<std::mutex> lk(t->mu);
+        t->mixed_rtol = rtol;
+        t->mixed_kmax = kmax;
+    });
+}
+
 int p3smi_topology_set_lu_shape(p3smi_topology* t, int32_t g, int32_t ns, int32_t few_g, int32_t few_ns,
                                 int32_t few_max) {
     return guarded([&] {
         const LuConfig a{g, ns}, f{few_g, few_ns};
         if (!valid_lu_config(a) || !valid_lu_config(f))
-            throw Error(P3SMI_E_ARG, "LU launch shape: g must be 1, 8, 16, 32 or 64 and ns 4, 8 or 16");
+            throw Error(P3SMI_E_ARG, "LU launch shape: g must be 1, 16, 32 or 64 and ns 8 or 16");
         if (few_max < 0) throw Error(P3SMI_E_ARG, "LU launch shape: few_max must be >= 0");
         std::lock_guard<std::mutex> lk(t->mu);
         t->lu_batch = a;

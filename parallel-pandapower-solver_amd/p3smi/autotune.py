@@ -19,8 +19,8 @@ import numpy as np
 
 from . import _lib
 
-BATCH_CANDIDATES = [(64, 8), (64, 16), (32, 8), (32, 16), (16, 8), (16, 16), (8, 8), (8, 16)]
-FEW_CANDIDATES = [(1, 8), (1, 16), (8, 8)]
+BATCH_CANDIDATES = [(64, 8), (64, 16), (32, 8), (32, 16), (16, 8), (16, 16)]
+FEW_CANDIDATES = [(1, 8), (1, 16), (16, 8)]
 BUCKETS = [(500, "le500"), (5000, "le5k"), (20000, "le20k"), (10**9, "gt20k")]
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
 

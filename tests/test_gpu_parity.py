@@ -82,3 +82,34 @@ def test_nan_column_fails_alone(gpu):
     keep = np.arange(70) != 13
     _compare({k: (v[:, keep] if k == "V" else v[keep]) for k, v in r.items()},
              {k: (v[:, keep] if k == "V" else v[keep]) for k, v in ref.items()}, "nan column")
+
+
+@pytest.mark.parametrize("name", ["pf200", "ts2000"])
+def test_mixed_precision_matches_fp64_oracle(gpu, name):
+    """FP32 factors + FP64 refinement: same flags and iterations as the FP64 oracle and
+    |dV| <= 1e-9 (the refined step is FP64-accurate)."""
+    g = grids.config_grid(name, 0)
+    Yp, Yj, Yx = g.csr()
+    if name == "pf200":
+        S = grids.timeseries_scaled(g, 96, seed=1)
+        V0 = g.flat_start()
+    else:
+        S = grids.timeseries_daily(g, 8760, seed=1)[:, ::23].copy()
+        V0 = dc_initial_voltage(g)
+    r = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8, precision="mixed")
+    ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    _compare(r, ref, f"{name} mixed")
+
+
+def test_mixed_precision_perturbed_starts(gpu):
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    rng = np.random.default_rng(7)
+    T = 256
+    k = rng.uniform(1.0, 2.5, T)
+    S = g.p_gen[:, None] * k[None, :] - (g.p_load + 1j * g.q_load)[:, None] * k[None, :]
+    V0 = np.repeat(g.flat_start()[:, None], T, axis=1)
+    V0[g.pq, :] *= rng.uniform(0.6, 1.4, T)[None, :]
+    r = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8, precision="mixed")
+    ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    _compare(r, ref, "perturbed mixed")
