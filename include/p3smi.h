@@ -158,6 +158,24 @@ int p3smi_batch_set_timing(p3smi_batch* b, int32_t on);
 /* Algorithmic bytes of one pass of each kernel for one case (eval, lu, step, decide). */
 int p3smi_batch_bytes_per_case(const p3smi_batch* b, double* out4);
 
+/* ---- continuous batching (no counterpart in the reference) ---------------------------
+ * A resident batch used as a pool of slots: cases are loaded into free slots while other
+ * slots iterate, every pass advances all loaded cases by one Newton iteration (cases loaded
+ * since the last pass get their start point evaluated first), finished slots are fetched
+ * and freed. Each slot iterates exactly as it would alone: no arithmetic mixes slots. */
+int p3smi_batch_slots_init(p3smi_batch* b);
+/* k cases into slots[0..k): S, V0 (n, k) complex. */
+int p3smi_batch_load(p3smi_batch* b, int32_t k, const int32_t* slots, const double* S, const double* V0);
+/* One pass; counts (8, may be NULL) = slots per status after it:
+ * [active, converged, failed, backtrack, needj, stopped, idle, new]. */
+int p3smi_batch_pass(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_search, int32_t precision,
+                     int32_t* counts);
+/* Slots whose case has finished (converged, failed or stopped), n of them. */
+int p3smi_batch_finished(p3smi_batch* b, int32_t* slots, int32_t* n);
+/* Results of k finished slots (V (n, k) complex, iterations, converged, mismatch); frees them. */
+int p3smi_batch_fetch(p3smi_batch* b, int32_t k, const int32_t* slots, double* V, int32_t* iterations,
+                      uint8_t* converged, double* mismatch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,9 +19,10 @@
 //   k_step      V = V0 + alpha * dx (full Newton step or line-search trial)
 //   k_decide    convergence test and Armijo bookkeeping per case
 //
-// STATUS per case: ACTIVE (iterating), CONVERGED, FAILED (non-finite mismatch or rejected
-// pivots), BACKTRACK (line-search trial), NEEDJ (trial accepted, Jacobian to rebuild),
-// STOPPED (iteration limit), IDLE (padding).
+// STATUS per case: NEW (loaded, start point not yet evaluated), ACTIVE (iterating),
+// CONVERGED, FAILED (non-finite mismatch or rejected pivots), BACKTRACK (line-search trial),
+// NEEDJ (trial accepted, Jacobian to rebuild), STOPPED (iteration limit), IDLE (free slot).
+// Iterations are counted per case, so cases may join a running batch at any pass.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,7 +44,9 @@
 
 namespace {
 
-enum : int { ST_ACTIVE = 0, ST_CONVERGED = 1, ST_FAILED = 2, ST_BACKTRACK = 3, ST_NEEDJ = 4, ST_STOPPED = 5, ST_IDLE = 6 };
+enum : int {
+    ST_ACTIVE = 0, ST_CONVERGED = 1, ST_FAILED = 2, ST_BACKTRACK = 3, ST_NEEDJ = 4, ST_STOPPED = 5, ST_IDLE = 6, ST_NEW = 7
+};
 constexpr int NSTAT = 8;
 constexpr double ARMIJO_C = 1e-4;
 constexpr int LS_MAX_TRIALS = 10;
@@ -72,7 +75,7 @@ __device__ __forceinline__ double nan_max(double w, double a) { return (a > w ||
 
 __global__ void k_status_init(int* status, int B, int S) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < S) status[c] = c < B ? ST_ACTIVE : ST_IDLE;
+    if (c < S) status[c] = c < B ? ST_NEW : ST_IDLE;
 }
 
 __global__ void k_clear_int(int* p, int n) {
@@ -80,18 +83,16 @@ __global__ void k_clear_int(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// start voltage -> polar state. V0: (n) shared complex (v0_stride 0) or (n, B) complex.
+// start voltage -> polar state of the NEW cases. V0: (n) shared complex (v0_stride 0) or
+// (n, v0_stride) complex, column c.
 __global__ void k_init(const double* __restrict__ V0, int v0_stride, double* __restrict__ Vm,
-                       double* __restrict__ Va, double* __restrict__ cs, double* __restrict__ sn, int n, int S,
-                       int B) {
+                       double* __restrict__ Va, double* __restrict__ cs, double* __restrict__ sn,
+                       const int* __restrict__ status, int* __restrict__ vote, int vote_new, int n, int S) {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const int bus = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (bus >= n) return;
+    if (bus >= n || status[c] != ST_NEW) return;
+    if (bus == 0) vote[c] = vote_new;
     const size_t o = at(bus, S, c);
-    if (c >= B) {
-        Vm[o] = 1.0; Va[o] = 0.0; cs[o] = 1.0; sn[o] = 0.0;
-        return;
-    }
     const size_t v = 2 * (v0_stride ? (size_t)bus * v0_stride + c : (size_t)bus);
     const double re = V0[v], im = V0[v + 1];
     const double va = atan2(im, re);
@@ -391,10 +392,10 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
                         bad = 1;
                         vote[c] = 0;
                     }
-                } else if (broken) {  // zero or non-finite pivot: no step
+                } else if (broken) {  // zero or non-finite pivot: no step (in this iteration)
                     bad = 1;
                     status[c] = ST_FAILED;
-                    iters[c] = it;
+                    iters[c] += 1;
                 } else if (pmin < rcond_min * pmax) {
                     atomicAdd(guard_trips, 1);
                 }
@@ -557,8 +558,8 @@ __global__ void k_step(const double* __restrict__ x, const int* __restrict__ xva
 }
 
 // Per-case Newton bookkeeping after an eval pass (p3s/cpp/nr_klu.cpp:590-709):
-//   phase 0  start point: converged if ||F|| < tol, failed if non-finite
-//   phase 1  full step of iteration `it`: Armijo test, else start backtracking
+//   phase 0  start point of a NEW case: converged if ||F|| < tol, failed if non-finite
+//   phase 1  full step of the case's next iteration: Armijo test, else start backtracking
 //   phase 2  backtracking trial: accept, halve, or after 10 trials take the best
 //   phase 3  Jacobian rebuilt at the accepted point
 // The loop of the reference ends after the step of iteration max_iter without a further
@@ -574,13 +575,13 @@ __global__ void k_decide(const double* __restrict__ part, int nparts, double* __
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < S) {
         int st = status[c];
-        const int want = phase == 2 ? ST_BACKTRACK : phase == 3 ? ST_NEEDJ : ST_ACTIVE;
+        const int want = phase == 0 ? ST_NEW : phase == 2 ? ST_BACKTRACK : phase == 3 ? ST_NEEDJ : ST_ACTIVE;
         if (st == want) {
             double nw = 0.0;
             for (int b = 0; b < nparts; ++b) nw = nan_max(nw, part[at(b, S, c)]);
             if (!(nw < DBL_MAX)) nw = DBL_MAX;
             auto classify = [&](double v) {
-                if (it >= max_iter && phase != 0) return (int)ST_STOPPED;
+                if (phase != 0 && iters[c] >= max_iter) return (int)ST_STOPPED;
                 return v < tol ? (int)ST_CONVERGED : (v >= DBL_MAX ? (int)ST_FAILED : (int)ST_ACTIVE);
             };
             if (phase == 0) {
@@ -588,7 +589,7 @@ __global__ void k_decide(const double* __restrict__ part, int nparts, double* __
                 iters[c] = 0;
                 st = classify(nw);
             } else if (phase == 1) {
-                iters[c] = it;
+                iters[c] += 1;
                 if (!line_search || nw <= (1.0 - ARMIJO_C) * nrm[c]) {
                     nrm[c] = nw;
                     st = classify(nw);
@@ -640,6 +641,50 @@ __global__ void k_finalize(const double* __restrict__ Vm, const double* __restri
     V[2 * ((size_t)bus * B + c) + 1] = im;
 }
 
+
+
+// ===================================================================== slots (serving)
+// Copy k staged cases (S and V0, (n, k) complex) into slot columns and mark them NEW.
+__global__ void k_load_slots(const double* __restrict__ Sst, const double* __restrict__ Vst,
+                             const int* __restrict__ slots, double* __restrict__ Ssp, double* __restrict__ V0,
+                             int* __restrict__ status, int n, int k, int S) {
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= k || bus >= n) return;
+    const int c = slots[j];
+    const size_t src = 2 * ((size_t)bus * k + j), dst = 2 * ((size_t)bus * S + c);
+    Ssp[dst] = Sst[src];
+    Ssp[dst + 1] = Sst[src + 1];
+    V0[dst] = Vst[src];
+    V0[dst + 1] = Vst[src + 1];
+    if (bus == 0) status[c] = ST_NEW;
+}
+
+// Results of k slots into (n, k) complex V, iterations, flags, final mismatch.
+__global__ void k_fetch_slots(const int* __restrict__ slots, const double* __restrict__ Vm,
+                              const double* __restrict__ cs, const double* __restrict__ sn,
+                              const int* __restrict__ status, const int* __restrict__ iters,
+                              const double* __restrict__ nrm, double* __restrict__ V, int* __restrict__ it_out,
+                              unsigned char* __restrict__ cv_out, double* __restrict__ mis_out, int n, int k, int S) {
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= k || bus >= n) return;
+    const int c = slots[j];
+    const size_t o = at(bus, S, c);
+    const double vm = Vm[o];
+    V[2 * ((size_t)bus * k + j)] = vm * cs[o];
+    V[2 * ((size_t)bus * k + j) + 1] = vm * sn[o];
+    if (bus == 0) {
+        it_out[j] = iters[c];
+        cv_out[j] = status[c] == ST_CONVERGED;
+        mis_out[j] = nrm[c];
+    }
+}
+
+__global__ void k_release_slots(const int* __restrict__ slots, int* __restrict__ status, int k) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < k) status[slots[j]] = ST_IDLE;
+}
 
 // ===================================================================== N-1 islands
 // Union-find over the live, non-outaged branches of one case per workgroup. A root is
@@ -863,6 +908,11 @@ struct p3smi_batch {
     int s_stride = 0, v0_stride = 0;
     bool per_case_y = false, has_pin = false, uploaded = false, mask_unserved = false;
     int* h_counts = nullptr;            // pinned host mirror of counts
+    bool pending_new = false;           // slots loaded since the last pass
+    DevBuf<double> st_S, st_V, st_out;  // staging of slot loads / fetches
+    DevBuf<int> st_slots, st_it;
+    DevBuf<unsigned char> st_cv;
+    DevBuf<double> st_mis;
     bool timing = false;
     p3smi_stats stats{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
@@ -1177,13 +1227,55 @@ const int* launch_decide(p3smi_batch* b, double tol, int ls, int phase, int it, 
     return b->h_counts;
 }
 
-void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int precision) {
-    if (!b->uploaded) throw Error(P3SMI_E_ARG, "batch has no inputs (upload first)");
+void check_solve_args(int max_iter, double tol, int precision) {
     if (max_iter <= 0) throw Error(P3SMI_E_ARG, "max_iter must be > 0");
     if (!(tol > 0.0)) throw Error(P3SMI_E_ARG, "tol must be > 0");
     if (precision != P3SMI_PREC_DOUBLE && precision != P3SMI_PREC_MIXED)
         throw Error(P3SMI_E_ARG, "precision must be 0 (double) or 1 (mixed)");
+}
+
+// Start point of the NEW cases: polar state, mismatch and Jacobian, classification.
+const int* start_new(p3smi_batch* b, double tol, int line_search, int precision, int max_iter) {
     const p3smi::Topology& h = b->topo->host;
+    {
+        KTimer tm(b, 4);
+        hipLaunchKernelGGL(k_init, grid_items(h.n, b->S), dim3(256), 0, b->topo->stream, b->V0.p, b->v0_stride,
+                           b->Vm.p, b->Va.p, b->cs.p, b->sn.p, b->status.p, b->vote.p,
+                           precision == P3SMI_PREC_MIXED ? 1 : 0, h.n, b->S);
+        HIPCHECK(hipGetLastError());
+    }
+    launch_eval(b, 1 << ST_NEW, 1 << ST_NEW);
+    return launch_decide(b, tol, line_search, 0, 0, max_iter);
+}
+
+// One Newton iteration of every ACTIVE case (p3s/cpp/nr_klu.cpp:595-708): linear solve,
+// full step scored by a full evaluation, backtracking passes while any case backtracks,
+// Jacobian rebuilt at accepted trial points.
+const int* newton_iteration(p3smi_batch* b, int n_active, double tol, int line_search, int precision, int max_iter,
+                            int it) {
+    const int A = 1 << ST_ACTIVE, BT = 1 << ST_BACKTRACK, NJ = 1 << ST_NEEDJ;
+    b->stats.newton_passes++;
+    linear_solve(b, it, n_active, precision);
+    launch_step(b, A, 1);
+    launch_eval(b, A, A);
+    const int* cnt = launch_decide(b, tol, line_search, 1, it, max_iter);
+    while (cnt[ST_BACKTRACK] > 0) {
+        b->stats.line_search_passes++;
+        launch_step(b, BT, 0);
+        launch_eval(b, BT, 0);
+        cnt = launch_decide(b, tol, line_search, 2, it, max_iter);
+    }
+    if (cnt[ST_NEEDJ] > 0) {
+        launch_step(b, NJ, 0);
+        launch_eval(b, NJ, NJ);
+        cnt = launch_decide(b, tol, line_search, 3, it, max_iter);
+    }
+    return cnt;
+}
+
+void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int precision) {
+    if (!b->uploaded) throw Error(P3SMI_E_ARG, "batch has no inputs (upload first)");
+    check_solve_args(max_iter, tol, precision);
     hipStream_t st = b->topo->stream;
     std::memset(&b->stats, 0, sizeof b->stats);
     b->stats.cases = b->B;
@@ -1192,34 +1284,14 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
         hipLaunchKernelGGL(k_status_init, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p, b->B,
                            b->S);
         hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 1);
-        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->vote.p,
-                           precision == P3SMI_PREC_MIXED ? 1 : 0, b->S);
-        hipLaunchKernelGGL(k_init, grid_items(h.n, b->S), dim3(256), 0, st, b->V0.p, b->v0_stride, b->Vm.p, b->Va.p,
-                           b->cs.p, b->sn.p, h.n, b->S, b->B);
+        hipLaunchKernelGGL(k_clear_int, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->iters.p, b->S);
         HIPCHECK(hipGetLastError());
     }
-    const int A = 1 << ST_ACTIVE, BT = 1 << ST_BACKTRACK, NJ = 1 << ST_NEEDJ;
-    launch_eval(b, A, A);
-    const int* cnt = launch_decide(b, tol, line_search, 0, 0, max_iter);
+    const int* cnt = start_new(b, tol, line_search, precision, max_iter);
     int it = 0;
     while (it < max_iter && cnt[ST_ACTIVE] > 0) {
         ++it;
-        b->stats.newton_passes++;
-        linear_solve(b, it, cnt[ST_ACTIVE], precision);
-        launch_step(b, A, 1);
-        launch_eval(b, A, A);
-        cnt = launch_decide(b, tol, line_search, 1, it, max_iter);
-        while (cnt[ST_BACKTRACK] > 0) {
-            b->stats.line_search_passes++;
-            launch_step(b, BT, 0);
-            launch_eval(b, BT, 0);
-            cnt = launch_decide(b, tol, line_search, 2, it, max_iter);
-        }
-        if (cnt[ST_NEEDJ] > 0) {
-            launch_step(b, NJ, 0);
-            launch_eval(b, NJ, NJ);
-            cnt = launch_decide(b, tol, line_search, 3, it, max_iter);
-        }
+        cnt = newton_iteration(b, cnt[ST_ACTIVE], tol, line_search, precision, max_iter, it);
     }
     b->stats.failed_factorizations = cnt[ST_FAILED];
     HIPCHECK(hipMemcpyAsync(&b->stats.guard_trips, b->guard.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1787,6 +1859,110 @@ int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start
         };
         solve_chunked(t, L, true, true, false, up, max_iter, tol, line_search, precision, V_out, iterations, converged,
                       mismatch);
+    });
+}
+
+}  // extern "C"
+
+// ---- continuous batching: a resident batch used as a pool of slots ------------------
+extern "C" {
+
+int p3smi_batch_slots_init(p3smi_batch* b) {
+    return guarded([&] {
+        const p3smi::Topology& h = b->topo->host;
+        hipStream_t st = b->topo->stream;
+        b->Ssp.alloc(2 * (size_t)h.n * b->S);
+        b->V0.alloc(2 * (size_t)h.n * b->S);
+        b->s_stride = b->S;
+        b->v0_stride = b->S;
+        b->per_case_y = false;
+        b->has_pin = false;
+        b->mask_unserved = false;
+        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p, ST_IDLE,
+                           b->S);
+        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 1);
+        HIPCHECK(hipGetLastError());
+        std::memset(b->h_counts, 0, NSTAT * sizeof(int));
+        b->h_counts[ST_IDLE] = b->S;
+        HIPCHECK(hipStreamSynchronize(st));
+        b->uploaded = true;
+        b->pending_new = false;
+    });
+}
+
+int p3smi_batch_load(p3smi_batch* b, int32_t k, const int32_t* slots, const double* S, const double* V0) {
+    return guarded([&] {
+        const p3smi::Topology& h = b->topo->host;
+        hipStream_t st = b->topo->stream;
+        if (k <= 0) return;
+        for (int j = 0; j < k; ++j)
+            if (slots[j] < 0 || slots[j] >= b->B) throw Error(P3SMI_E_ARG, "load: slot out of range");
+        check_v0(V0, h.n, k, "solve_batch: ");
+        b->st_S.upload(S, 2 * (size_t)h.n * k, st);
+        b->st_V.upload(V0, 2 * (size_t)h.n * k, st);
+        b->st_slots.upload(slots, k, st);
+        hipLaunchKernelGGL(k_load_slots, dim3((unsigned)((k + 63) / 64), (unsigned)((h.n + 3) / 4)), dim3(256), 0, st,
+                           b->st_S.p, b->st_V.p, b->st_slots.p, b->Ssp.p, b->V0.p, b->status.p, h.n, k, b->S);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(st));
+        b->pending_new = true;
+    });
+}
+
+int p3smi_batch_pass(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_search, int32_t precision,
+                     int32_t* counts) {
+    return guarded([&] {
+        if (!b->uploaded) throw Error(P3SMI_E_ARG, "pass: batch slots not initialised");
+        check_solve_args(max_iter, tol, precision);
+        const int* cnt = b->h_counts;
+        if (b->pending_new) {
+            cnt = start_new(b, tol, line_search, precision, max_iter);
+            b->pending_new = false;
+        }
+        if (cnt[ST_ACTIVE] > 0) cnt = newton_iteration(b, cnt[ST_ACTIVE], tol, line_search, precision, max_iter, 0);
+        if (counts) std::copy(cnt, cnt + NSTAT, counts);
+        collect_timing(b);
+    });
+}
+
+int p3smi_batch_finished(p3smi_batch* b, int32_t* slots, int32_t* n) {
+    return guarded([&] {
+        std::vector<int> status(b->B);
+        HIPCHECK(hipMemcpyAsync(status.data(), b->status.p, b->B * sizeof(int), hipMemcpyDeviceToHost, b->topo->stream));
+        HIPCHECK(hipStreamSynchronize(b->topo->stream));
+        int k = 0;
+        for (int c = 0; c < b->B; ++c)
+            if (status[c] == ST_CONVERGED || status[c] == ST_FAILED || status[c] == ST_STOPPED) slots[k++] = c;
+        *n = k;
+    });
+}
+
+int p3smi_batch_fetch(p3smi_batch* b, int32_t k, const int32_t* slots, double* V, int32_t* iterations,
+                      uint8_t* converged, double* mismatch) {
+    return guarded([&] {
+        const p3smi::Topology& h = b->topo->host;
+        hipStream_t st = b->topo->stream;
+        if (k <= 0) return;
+        for (int j = 0; j < k; ++j)
+            if (slots[j] < 0 || slots[j] >= b->B) throw Error(P3SMI_E_ARG, "fetch: slot out of range");
+        b->st_slots.upload(slots, k, st);
+        b->st_out.alloc(2 * (size_t)h.n * k);
+        b->st_it.alloc(k);
+        b->st_cv.alloc(k);
+        b->st_mis.alloc(k);
+        hipLaunchKernelGGL(k_fetch_slots, dim3((unsigned)((k + 63) / 64), (unsigned)((h.n + 3) / 4)), dim3(256), 0, st,
+                           b->st_slots.p, b->Vm.p, b->cs.p, b->sn.p, b->status.p, b->iters.p, b->nrm.p, b->st_out.p,
+                           b->st_it.p, b->st_cv.p, b->st_mis.p, h.n, k, b->S);
+        hipLaunchKernelGGL(k_release_slots, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, st, b->st_slots.p,
+                           b->status.p, k);
+        HIPCHECK(hipGetLastError());
+        if (V) HIPCHECK(hipMemcpyAsync(V, b->st_out.p, 2 * (size_t)h.n * k * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (iterations) HIPCHECK(hipMemcpyAsync(iterations, b->st_it.p, k * sizeof(int), hipMemcpyDeviceToHost, st));
+        std::vector<unsigned char> cv(k);
+        HIPCHECK(hipMemcpyAsync(cv.data(), b->st_cv.p, k, hipMemcpyDeviceToHost, st));
+        if (mismatch) HIPCHECK(hipMemcpyAsync(mismatch, b->st_mis.p, k * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (converged) std::copy(cv.begin(), cv.end(), converged);
     });
 }
 

@@ -88,6 +88,7 @@ EXPORTS = [
     "p3smi_batch_create", "p3smi_batch_free", "p3smi_batch_upload", "p3smi_batch_solve", "p3smi_batch_download",
     "p3smi_batch_stats", "p3smi_batch_set_timing", "p3smi_batch_bytes_per_case",
     "p3smi_outages_create", "p3smi_outages_free", "p3smi_outages_islands", "p3smi_outages_solve",
+    "p3smi_batch_slots_init", "p3smi_batch_load", "p3smi_batch_pass", "p3smi_batch_finished", "p3smi_batch_fetch",
 ]
 
 _lib = None
@@ -135,6 +136,11 @@ def lib():
     L.p3smi_outages_free.restype = None
     L.p3smi_outages_islands.argtypes = [P, P, P, P]
     L.p3smi_outages_solve.argtypes = [P, P, P, _i32, _f64, _i32, _i32, P, P, P, P, P]
+    L.p3smi_batch_slots_init.argtypes = [P]
+    L.p3smi_batch_load.argtypes = [P, _i32, P, P, P]
+    L.p3smi_batch_pass.argtypes = [P, _i32, _f64, _i32, _i32, P]
+    L.p3smi_batch_finished.argtypes = [P, P, P]
+    L.p3smi_batch_fetch.argtypes = [P, _i32, P, P, P, P, P]
     _lib = L
     return L
 

@@ -1,0 +1,55 @@
+"""Continuous batching: requests of two topologies arrive while others are on the GPU, are
+admitted into free slots, and each result is bit-identical to solving that request alone."""
+
+import time
+
+import numpy as np
+import pytest
+
+from p3smi import grids, nr_klu
+from p3smi.serving import Server
+from p3smi.timeseries import dc_initial_voltage
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_results_equal_alone(gpu):
+    g1 = grids.config_grid("pf200", 0)
+    g2 = grids.config_grid("ts2000", 0)
+    topo = {}
+    for name, g in (("pf200", g1), ("ts2000", g2)):
+        Yp, Yj, Yx = g.csr()
+        topo[name] = (g, Yp, Yj, Yx)
+    S1 = grids.timeseries_scaled(g1, 600, seed=4)
+    S2 = grids.timeseries_daily(g2, 800, seed=4)
+    V1, V2 = g1.flat_start(), dc_initial_voltage(g2)
+    rng = np.random.default_rng(0)
+    srv = Server(max_slots=256, mem_budget=4 << 30)
+    reqs = []
+    try:
+        c1 = c2 = 0
+        for wave in range(4):
+            for _ in range(3):
+                b = int(rng.integers(1, 90))
+                g, Yp, Yj, Yx = topo["pf200"]
+                reqs.append(("pf200", c1, b, srv.submit(Yp, Yj, Yx, g.pv, g.pq, S1[:, c1:c1 + b], V1)))
+                c1 += b
+                b = int(rng.integers(1, 120))
+                g, Yp, Yj, Yx = topo["ts2000"]
+                reqs.append(("ts2000", c2, b, srv.submit(Yp, Yj, Yx, g.pv, g.pq, S2[:, c2:c2 + b], V2)))
+                c2 += b
+            time.sleep(0.02)  # the next wave arrives while these iterate
+        srv.wait_all([r[3] for r in reqs], timeout=120)
+        st = srv.stats()
+        assert st["requests"] == len(reqs)
+    finally:
+        srv.close()
+    solvers = {k: nr_klu.Solver(v[1], v[2], v[3], v[0].pv, v[0].pq) for k, v in topo.items()}
+    for name, c0, b, r in reqs:
+        S = S1 if name == "pf200" else S2
+        V0 = V1 if name == "pf200" else V2
+        alone = solvers[name].solve_batch(S[:, c0:c0 + b], V0, max_iter=20, tol=1e-8)
+        res = r.result()
+        np.testing.assert_array_equal(res["converged"], alone["converged"])
+        np.testing.assert_array_equal(res["iterations"], alone["iterations"])
+        np.testing.assert_array_equal(res["V"], alone["V"])
