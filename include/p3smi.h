@@ -39,7 +39,7 @@ typedef struct {
     int64_t nnz_lu, n_pairs;
     int32_t n_levels, n_blevels, n_slots, ordering;
     double rcond0;
-    int32_t lu_g, lu_ns, lu_few_g, lu_few_ns, lu_few_max, reserved;
+    int32_t lu_g, lu_ns, lu_few_g, lu_few_ns, lu_few_max, lu_group;
 } p3smi_info;
 
 /* Per-case/aggregate counters of the last solve (the reference's last_stats,
@@ -73,9 +73,10 @@ void p3smi_topology_free(p3smi_topology* t);
 int p3smi_topology_info(const p3smi_topology* t, p3smi_info* out);
 /* LU launch shape (no counterpart in the reference; set by the autotuner): a workgroup of
  * 64*ns threads factors g consecutive cases (g in {1, 16, 32, 64}, ns in {8, 16});
- * (few_g, few_ns) is used instead while at most few_max cases are still iterating. */
+ * (few_g, few_ns) is used instead while at most few_max cases are still iterating; group > 0
+ * deals runs of up to `group` entries of one pivot's U row / L column to one task list. */
 int p3smi_topology_set_lu_shape(p3smi_topology* t, int32_t g, int32_t ns, int32_t few_g, int32_t few_ns,
-                                int32_t few_max);
+                                int32_t few_max, int32_t group);
 /* Mixed precision (no counterpart in the reference): refinement of the FP32-LU step stops
  * once the correction is below rtol x the step, at most kmax steps, then falls back to an
  * FP64 factorization of that case. Defaults 1e-12 and 6. */

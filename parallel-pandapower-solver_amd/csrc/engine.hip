@@ -869,8 +869,9 @@ struct DevTopo {
 };
 
 struct LuConfig {
-    int G, NS;
+    int G, NS, grp;
     int tl() const { return 64 * NS / G; }
+    int key() const { return tl() * 1024 + grp; }
 };
 
 }  // namespace
@@ -878,8 +879,8 @@ struct LuConfig {
 struct p3smi_topology {
     p3smi::Topology host;
     DevTopo dev;
-    LuConfig lu_batch{16, 8};     // LU launch shape while many cases are active
-    LuConfig lu_few{1, 8};        // ... and for the last few
+    LuConfig lu_batch{64, 16, 0};  // LU launch shape while many cases are active
+    LuConfig lu_few{1, 16, 0};     // ... and for the last few
     int lu_few_max = 256;         // "few": at most this many active cases
     double mixed_rtol = 1e-12;    // refinement stops once |dx correction| <= rtol |dx|
     int mixed_kmax = 6;           // refinement steps before the FP64 fallback
@@ -976,12 +977,14 @@ void ensure_device(p3smi_topology* t) {
     d.ready = true;
 }
 
-// Level schedule for TL task lists, built and uploaded on first use.
-const DevSched& get_sched(p3smi_topology* t, int TL) {
-    auto it = t->dev.sched.find(TL);
+// Level schedule for a launch shape (TL task lists, dealing group), built and uploaded on
+// first use.
+const DevSched& get_sched(p3smi_topology* t, LuConfig cfg) {
+    const int TL = cfg.tl();
+    auto it = t->dev.sched.find(cfg.key());
     if (it != t->dev.sched.end()) return *it->second;
     const p3smi::Topology& h = t->host;
-    const p3smi::Schedule s = p3smi::make_schedule(h, TL);
+    const p3smi::Schedule s = p3smi::make_schedule(h, TL, cfg.grp);
     std::unique_ptr<DevSched> d(new DevSched());
     hipStream_t st = t->stream;
     d->TL = TL;
@@ -1010,7 +1013,7 @@ const DevSched& get_sched(p3smi_topology* t, int TL) {
     d->fwd.upload(fw.data(), fw.size(), st);
     d->bwd.upload(bw.data(), bw.size(), st);
     HIPCHECK(hipStreamSynchronize(st));
-    return *(t->dev.sched[TL] = std::move(d));
+    return *(t->dev.sched[cfg.key()] = std::move(d));
 }
 
 int eval_nparts(int n) { return (n + EVAL_WAVES * EVAL_BUSES_PER_WAVE - 1) / (EVAL_WAVES * EVAL_BUSES_PER_WAVE); }
@@ -1125,7 +1128,7 @@ template <typename real, bool FACTOR>
 void launch_lu(p3smi_batch* b, int it, int n_active, real* fac, const double* rhs, double* xout, int sel) {
     p3smi_topology* t = b->topo;
     const LuConfig cfg = n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
-    const DevSched& ds = get_sched(t, cfg.tl());
+    const DevSched& ds = get_sched(t, cfg);
     KTimer tm(b, 1);
 #define P3SMI_LU_CASE(GG, NN)                                                                \
     if (cfg.G == GG && cfg.NS == NN) {                                                       \
@@ -1432,7 +1435,7 @@ int p3smi_topology_info(const p3smi_topology* t, p3smi_info* o) {
         o->nnz_lu = h.nnz_lu; o->n_pairs = h.n_pairs; o->n_levels = h.sched.nlev; o->n_blevels = h.sched.nblev;
         o->n_slots = h.sched.nslots; o->ordering = h.ordering; o->rcond0 = h.rcond0;
         o->lu_g = t->lu_batch.G; o->lu_ns = t->lu_batch.NS; o->lu_few_g = t->lu_few.G; o->lu_few_ns = t->lu_few.NS;
-        o->lu_few_max = t->lu_few_max; o->reserved = 0;
+        o->lu_few_max = t->lu_few_max; o->lu_group = t->lu_batch.grp;
     });
 }
 
@@ -1446,9 +1449,10 @@ int p3smi_topology_set_mixed(p3smi_topology* t, double rtol, int32_t kmax) {
 }
 
 int p3smi_topology_set_lu_shape(p3smi_topology* t, int32_t g, int32_t ns, int32_t few_g, int32_t few_ns,
-                                int32_t few_max) {
+                                int32_t few_max, int32_t group) {
     return guarded([&] {
-        const LuConfig a{g, ns}, f{few_g, few_ns};
+        if (group < 0 || group > 4096) throw Error(P3SMI_E_ARG, "LU launch shape: group must be in [0, 4096]");
+        const LuConfig a{g, ns, group}, f{few_g, few_ns, group};
         if (!valid_lu_config(a) || !valid_lu_config(f))
             throw Error(P3SMI_E_ARG, "LU launch shape: g must be 1, 16, 32 or 64 and ns 8 or 16");
         if (few_max < 0) throw Error(P3SMI_E_ARG, "LU launch shape: few_max must be >= 0");

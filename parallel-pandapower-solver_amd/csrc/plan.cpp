@@ -317,9 +317,9 @@ static void build_lu(Topology& t, int ordering) {
     for (int64_t e = 0; e <= t.nnz_lu; ++e) t.prange[e] = (int)cnt[e];
 }
 
-void build_schedule(Topology& t, int nslots) { t.sched = make_schedule(t, nslots); }
+void build_schedule(Topology& t, int nslots) { t.sched = make_schedule(t, nslots, 0); }
 
-Schedule make_schedule(const Topology& t, int nslots) {
+Schedule make_schedule(const Topology& t, int nslots, int group) {
     if (nslots < 1) throw std::runtime_error("nslots must be >= 1");
     const int m = t.m;
     const std::vector<int>& prange = t.prange;
@@ -346,20 +346,46 @@ Schedule make_schedule(const Topology& t, int nslots) {
     }
     std::vector<int> lA, sA, lB, sB, lF, sF, lW, sW;
     std::vector<int> itemsA, itemsB, itemsF, itemsW;
+    // dealing units: single entries (group 0) or runs of up to `group` consecutive entries
+    // of one pivot's U row / L column, which share one operand of every pair (L(p,t) along a
+    // U row, U(t,p) down an L column) and so reuse it from cache within one task list
+    auto units_of = [&](const std::vector<int>& items, std::vector<std::vector<int>>& units) {
+        units.clear();
+        for (size_t i = 0; i < items.size(); ++i) {
+            const int d = items[i];
+            const bool cont = group > 0 && !units.empty() && (int)units.back().size() < group &&
+                              units.back().back() + 1 == d;
+            if (cont) units.back().push_back(d);
+            else units.push_back({d});
+        }
+    };
+    std::vector<std::vector<int>> ua, ub;
     for (int l = 0; l < s.nlev; ++l) {
+        units_of(levA[l], ua);
         std::vector<int64_t> load(nslots, 0);
         std::vector<int64_t> w;
-        for (int d : levA[l]) w.push_back(2 + (prange[d + 1] - prange[d]));
+        for (auto& u : ua) {
+            int64_t wt = 0;
+            for (int d : u) wt += 2 + (prange[d + 1] - prange[d]);
+            w.push_back(wt);
+        }
         for (int p : levF[l]) w.push_back(2 + (t.Fp[p + 1] - t.Fp[p]));
         std::vector<int> slot = lpt_deal(w, nslots, load);
-        for (size_t i = 0; i < levA[l].size(); ++i) { itemsA.push_back(levA[l][i]); lA.push_back(l); sA.push_back(slot[i]); }
+        for (size_t i = 0; i < ua.size(); ++i)
+            for (int d : ua[i]) { itemsA.push_back(d); lA.push_back(l); sA.push_back(slot[i]); }
         for (size_t i = 0; i < levF[l].size(); ++i) {
-            itemsF.push_back(levF[l][i]); lF.push_back(l); sF.push_back(slot[levA[l].size() + i]);
+            itemsF.push_back(levF[l][i]); lF.push_back(l); sF.push_back(slot[ua.size() + i]);
         }
+        units_of(levB[l], ub);
         std::vector<int64_t> loadB(nslots, 0), wb;
-        for (int d : levB[l]) wb.push_back(2 + (prange[d + 1] - prange[d]));
+        for (auto& u : ub) {
+            int64_t wt = 0;
+            for (int d : u) wt += 2 + (prange[d + 1] - prange[d]);
+            wb.push_back(wt);
+        }
         std::vector<int> slotB = lpt_deal(wb, nslots, loadB);
-        for (size_t i = 0; i < levB[l].size(); ++i) { itemsB.push_back(levB[l][i]); lB.push_back(l); sB.push_back(slotB[i]); }
+        for (size_t i = 0; i < ub.size(); ++i)
+            for (int d : ub[i]) { itemsB.push_back(d); lB.push_back(l); sB.push_back(slotB[i]); }
     }
     for (int l = 0; l < s.nblev; ++l) {
         std::vector<int64_t> load(nslots, 0), w;

@@ -90,7 +90,8 @@ void update_values(Topology& t, const double* Yx);
 
 // (Re)deal the level schedule to `nslots` task lists (t.sched), or build another one.
 void build_schedule(Topology& t, int nslots);
-Schedule make_schedule(const Topology& t, int nslots);
+// group > 0: deal runs of up to `group` entries of one pivot's U row / L column together.
+Schedule make_schedule(const Topology& t, int nslots, int group);
 
 // Host execution of the plan for ONE system (checks and rcond0): Jacobian values at
 // (Vm, Va) with admittance (G, B) [nullptr = the topology's], pin mask [nullptr = none],

@@ -62,7 +62,7 @@ class Info(C.Structure):
         ("n_levels", C.c_int32), ("n_blevels", C.c_int32), ("n_slots", C.c_int32), ("ordering", C.c_int32),
         ("rcond0", C.c_double),
         ("lu_g", C.c_int32), ("lu_ns", C.c_int32), ("lu_few_g", C.c_int32), ("lu_few_ns", C.c_int32),
-        ("lu_few_max", C.c_int32), ("reserved", C.c_int32),
+        ("lu_few_max", C.c_int32), ("lu_group", C.c_int32),
     ]
 
 
@@ -113,7 +113,7 @@ def lib():
     L.p3smi_topology_free.restype = None
     L.p3smi_topology_info.argtypes = [P, P]
     L.p3smi_update_y.argtypes = [P, P, _i32]
-    L.p3smi_topology_set_lu_shape.argtypes = [P, _i32, _i32, _i32, _i32, _i32]
+    L.p3smi_topology_set_lu_shape.argtypes = [P, _i32, _i32, _i32, _i32, _i32, _i32]
     L.p3smi_topology_set_mixed.argtypes = [P, _f64, _i32]
     L.p3smi_host_jacobian.argtypes = [P, P, P, P, P, P, P]
     L.p3smi_host_factor_solve.argtypes = [P, P, P, P, P]

@@ -21,6 +21,7 @@ from . import _lib
 
 BATCH_CANDIDATES = [(64, 8), (64, 16), (32, 8), (32, 16), (16, 8), (16, 16)]
 FEW_CANDIDATES = [(1, 8), (1, 16), (16, 8)]
+GROUP_CANDIDATES = [0, 4, 16, 64]
 BUCKETS = [(500, "le500"), (5000, "le5k"), (20000, "le20k"), (10**9, "gt20k")]
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
 
@@ -53,27 +54,28 @@ def tune(solver, S, V0, reps: int = 2, log=None) -> dict:
         _lib.check(L.p3smi_batch_upload(bh, _lib.ptr(S), _lib.ptr(V0), int(V0.ndim == 2)))
         results = {}
         for g, ns in BATCH_CANDIDATES:
-            solver.set_lu_shape(g, ns, 1, 8, 0)  # batch shape for every pass
-            _time_solve(solver, bh, 1)
-            results[(g, ns)] = _time_solve(solver, bh, reps)
-            if log:
-                log(f"[autotune] batch g={g} ns={ns}: {1e3 * results[(g, ns)]:.2f} ms")
-        bg, bns = min(results, key=results.get)
+            for grp in GROUP_CANDIDATES:
+                solver.set_lu_shape(g, ns, 1, 16, 0, grp)  # batch shape for every pass
+                _time_solve(solver, bh, 1)
+                results[(g, ns, grp)] = _time_solve(solver, bh, reps)
+                if log:
+                    log(f"[autotune] batch g={g} ns={ns} group={grp}: {1e3 * results[(g, ns, grp)]:.2f} ms")
+        bg, bns, bgrp = min(results, key=results.get)
         few = {}
         for fg, fns in FEW_CANDIDATES:
             for fmax in (64, 256, 1024):
-                solver.set_lu_shape(bg, bns, fg, fns, fmax)
+                solver.set_lu_shape(bg, bns, fg, fns, fmax, bgrp)
                 _time_solve(solver, bh, 1)
                 few[(fg, fns, fmax)] = _time_solve(solver, bh, reps)
                 if log:
                     log(f"[autotune] few g={fg} ns={fns} max={fmax}: {1e3 * few[(fg, fns, fmax)]:.2f} ms")
         best_few = min(few, key=few.get)
-        if few[best_few] >= results[(bg, bns)]:
-            best_few = (1, 8, 0)
+        if few[best_few] >= results[(bg, bns, bgrp)]:
+            best_few = (1, 16, 0)
         fg, fns, fmax = best_few
-        solver.set_lu_shape(bg, bns, fg, fns, fmax)
-        return {"g": bg, "ns": bns, "few_g": fg, "few_ns": fns, "few_max": fmax,
-                "ms": 1e3 * min(results[(bg, bns)], few.get(best_few, results[(bg, bns)]))}
+        solver.set_lu_shape(bg, bns, fg, fns, fmax, bgrp)
+        return {"g": bg, "ns": bns, "group": bgrp, "few_g": fg, "few_ns": fns, "few_max": fmax,
+                "ms": 1e3 * min(results[(bg, bns, bgrp)], few.get(best_few, results[(bg, bns, bgrp)]))}
     finally:
         L.p3smi_batch_free(bh)
 
@@ -95,5 +97,5 @@ def apply_cached(solver) -> bool:
     e = load_cache().get(bucket(solver.n))
     if not e:
         return False
-    solver.set_lu_shape(e["g"], e["ns"], e["few_g"], e["few_ns"], e["few_max"])
+    solver.set_lu_shape(e["g"], e["ns"], e["few_g"], e["few_ns"], e["few_max"], e.get("group", 0))
     return True

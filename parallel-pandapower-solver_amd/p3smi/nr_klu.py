@@ -75,10 +75,11 @@ class Solver:
         check(lib().p3smi_topology_info(self._h, C.byref(inf)))
         return {f: getattr(inf, f) for f, _ in inf._fields_}
 
-    def set_lu_shape(self, g: int, ns: int, few_g: int = 1, few_ns: int = 8, few_max: int = 256):
+    def set_lu_shape(self, g: int, ns: int, few_g: int = 1, few_ns: int = 16, few_max: int = 256, group: int = 0):
         """LU launch shape: g cases per workgroup of 64*ns threads; (few_g, few_ns) while at
-        most few_max cases are still iterating."""
-        check(lib().p3smi_topology_set_lu_shape(self._h, int(g), int(ns), int(few_g), int(few_ns), int(few_max)))
+        most few_max cases are still iterating; group > 0 deals runs of a pivot's entries together."""
+        check(lib().p3smi_topology_set_lu_shape(self._h, int(g), int(ns), int(few_g), int(few_ns), int(few_max),
+                                                int(group)))
         self.info = self._info()
 
     @property
