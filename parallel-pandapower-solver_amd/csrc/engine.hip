@@ -303,7 +303,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
     const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
     const int4* __restrict__ taskB, const int* __restrict__ diagB, const int2* __restrict__ pairs,
     const int* __restrict__ ptrF, const int4* __restrict__ fwd, const int2* __restrict__ fent,
-    const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int* __restrict__ Li,
+    const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int2* __restrict__ bent,
     const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
     int* __restrict__ status, int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips,
     int* __restrict__ vote, const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S,
@@ -411,18 +411,206 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
             for (int q = q0; q < q1; ++q) {
                 const int4 br = bwd[q];
                 double acc = x[at(br.x, S, c)], acc1 = 0.0;
-                const int pos0 = br.y + 1 - br.z;
                 int e = br.z;
                 for (; e + 2 <= br.w; e += 2) {
-                    const double u0 = (double)fac[at(pos0 + e, S, c)], x0 = x[at(Li[e], S, c)];
-                    const double u1 = (double)fac[at(pos0 + e + 1, S, c)], x1 = x[at(Li[e + 1], S, c)];
+                    const int2 b0 = bent[e], b1 = bent[e + 1];
+                    const double u0 = (double)fac[at(b0.x, S, c)], x0 = x[at(b0.y, S, c)];
+                    const double u1 = (double)fac[at(b1.x, S, c)], x1 = x[at(b1.y, S, c)];
                     acc = fma(-u0, x0, acc);
                     acc1 = fma(-u1, x1, acc1);
                 }
-                if (e < br.w) acc = fma(-(double)fac[at(pos0 + e, S, c)], x[at(Li[e], S, c)], acc);
+                if (e < br.w) {
+                    const int2 be = bent[e];
+                    acc = fma(-(double)fac[at(be.x, S, c)], x[at(be.y, S, c)], acc);
+                }
                 x[at(br.x, S, c)] = (acc + acc1) * (double)fac[at(br.y, S, c)];
             }
         }
+        __syncthreads();
+    }
+}
+
+// ---- G = 64: operand streams, software-pipelined -------------------------------------
+// With 64 cases per workgroup a task list is one wavefront and its control flow is the
+// same in every lane. The host lays the operand pairs of a task list out back to back in
+// execution order (one stream per list and sub-level), so the wavefront can walk the
+// stream in chunks of K pairs with the loads of the next chunk in flight while it
+// multiplies the current one, across task boundaries: a task's end only stores a result.
+// The start values of a task (its Jacobian entry, pivot inverse, right-hand side) are
+// loaded one task ahead and added at its end. The memory-level parallelism of a wavefront
+// is thus 2K operand pairs whatever the task lengths, where the task-at-a-time loop of
+// k_lu waits for every task's first loads.
+constexpr int LU_K = 8;
+
+template <typename TA, typename TB>
+__device__ __forceinline__ void load_chunk(const int2* __restrict__ st, int e, int eend, const TA* __restrict__ A,
+                                           const TB* __restrict__ Bv, int S, int c, TA (&a)[LU_K], TB (&b)[LU_K]) {
+#pragma unroll
+    for (int u = 0; u < LU_K; ++u) {
+        const int2 p = st[min(e + u, eend - 1)];  // past the end: repeat the last pair (unused)
+        a[u] = A[at(p.x, S, c)];
+        b[u] = Bv[at(p.y, S, c)];
+    }
+}
+
+// Tasks [q0, q1) of one list; task q = (w0, w1, pbeg, pend) with pend(q) == pbeg(q + 1).
+// start(q, task) -> In issues the task's start loads; finish(task, In, sum) consumes
+// sum = -sum_pairs a*b in type ACC.
+template <typename ACC, typename In, typename TA, typename TB, typename Start, typename Finish>
+__device__ __forceinline__ void seg_stream(const int4* __restrict__ tasks, int q0, int q1,
+                                           const int2* __restrict__ st, const TA* __restrict__ A,
+                                           const TB* __restrict__ Bv, int S, int c, Start start, Finish finish) {
+    if (q0 >= q1) return;
+    int q = q0;
+    int4 tk = tasks[q];
+    In in = start(q, tk);
+    int4 tkn = tk;
+    In inn = in;
+    if (q + 1 < q1) {
+        tkn = tasks[q + 1];
+        inn = start(q + 1, tkn);
+    }
+    const int eend = tasks[q1 - 1].w;
+    ACC acc = 0;
+    auto advance = [&]() {
+        finish(tk, in, acc);
+        acc = 0;
+        ++q;
+        if (q < q1) {
+            tk = tkn;
+            in = inn;
+            if (q + 1 < q1) {
+                tkn = tasks[q + 1];
+                inn = start(q + 1, tkn);
+            }
+        }
+    };
+    int e = tk.z;
+    TA a[LU_K];
+    TB b[LU_K];
+    if (e < eend) load_chunk(st, e, eend, A, Bv, S, c, a, b);
+    for (; e < eend; e += LU_K) {
+        TA a2[LU_K];
+        TB b2[LU_K];
+        if (e + LU_K < eend) load_chunk(st, e + LU_K, eend, A, Bv, S, c, a2, b2);
+#pragma unroll
+        for (int u = 0; u < LU_K; ++u) {
+            const int pos = e + u;
+            while (q < q1 && pos == tk.w) advance();
+            if (pos < eend) acc = fma(-(ACC)a[u], (ACC)b[u], acc);
+        }
+#pragma unroll
+        for (int u = 0; u < LU_K; ++u) {
+            a[u] = a2[u];
+            b[u] = b2[u];
+        }
+    }
+    while (q < q1) advance();
+}
+
+struct In1 { double v; };
+struct In2 { double v, d; };
+
+// Same contract and results (up to summation order) as k_lu<64, NS, real, FACTOR>. All
+// lanes of a wavefront belong to one task list, so every loop bound below is uniform.
+template <int NS, typename real, bool FACTOR>
+__global__ __launch_bounds__(64 * NS) void k_lu_stream(
+    const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
+    const int4* __restrict__ taskB, const int* __restrict__ diagB, const int2* __restrict__ pairs,
+    const int* __restrict__ ptrF, const int4* __restrict__ fwd, const int2* __restrict__ fent,
+    const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int2* __restrict__ bent,
+    const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
+    int* __restrict__ status, int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips,
+    int* __restrict__ vote, const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S,
+    int it) {
+    constexpr bool SINGLE = sizeof(real) == 4;
+    __shared__ double smin[NS][64], smax[NS][64], ssum[NS][64];
+    __shared__ int sbad[64];
+    const int lane = threadIdx.x & 63;
+    const int slot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int c = blockIdx.x * 64 + lane;
+    bool act = selected(sel, c, status, vote, rf);
+    if (!__syncthreads_or(act)) return;
+
+    double pmin = DBL_MAX, pmax = 0.0, psum = 0.0;
+    // sub-level A: pivots (y = -src - 2) and U entries (y = Jacobian entry or -1 for fill)
+    auto startA = [&](int, int4 t) -> In1 {
+        const int src = t.y < -1 ? -t.y - 2 : t.y;
+        return {src >= 0 ? Jx[at(src, S, c)] : 0.0};
+    };
+    auto finishA = [&](int4 t, In1 in, real sum) {
+        real v = (real)in.v + sum;
+        if (t.y < -1) {
+            const double ab = fabs((double)v);
+            pmin = fmin(pmin, ab);
+            pmax = fmax(pmax, ab);
+            psum += ab;
+            v = (real)1 / v;
+        }
+        fac[at(t.x, S, c)] = v;
+    };
+    // sub-level B: L entries, scaled by the pivot inverse diagB[q]
+    auto startB = [&](int q, int4 t) -> In2 {
+        return {t.y >= 0 ? Jx[at(t.y, S, c)] : 0.0, (double)fac[at(diagB[q], S, c)]};
+    };
+    auto finishB = [&](int4 t, In2 in, real sum) { fac[at(t.x, S, c)] = ((real)in.v + sum) * (real)in.d; };
+    // forward rows (row, right-hand side index): y = rhs - sum L y
+    auto startF = [&](int, int4 t) -> In1 { return {rhs[at(t.y, S, c)]}; };
+    auto finishF = [&](int4 t, In1 in, double sum) { x[at(t.x, S, c)] = in.v + sum; };
+    // backward rows (row, pivot position): x = (y - sum U x) / U(p,p)
+    auto startW = [&](int, int4 t) -> In2 { return {x[at(t.x, S, c)], (double)fac[at(t.y, S, c)]}; };
+    auto finishW = [&](int4 t, In2 in, double sum) { x[at(t.x, S, c)] = (in.v + sum) * in.d; };
+
+    for (int l = 0; l < nlev; ++l) {
+        const int i0 = l * NS + slot;
+        if (act) {
+            if (FACTOR)
+                seg_stream<real, In1>(taskA, ptrA[i0], ptrA[i0 + 1], pairs, fac, fac, S, c, startA, finishA);
+            seg_stream<double, In1>(fwd, ptrF[i0], ptrF[i0 + 1], fent, fac, x, S, c, startF, finishF);
+        }
+        __syncthreads();
+        if (FACTOR) {
+            if (act) seg_stream<real, In2>(taskB, ptrB[i0], ptrB[i0 + 1], pairs, fac, fac, S, c, startB, finishB);
+            __syncthreads();
+        }
+    }
+    if (FACTOR) {
+        smin[slot][lane] = pmin;
+        smax[slot][lane] = pmax;
+        ssum[slot][lane] = psum;
+        __syncthreads();
+        if (slot == 0) {
+            int bad = 0;
+            if (act) {
+                for (int s2 = 1; s2 < NS; ++s2) {
+                    pmin = fmin(pmin, smin[s2][lane]);
+                    pmax = fmax(pmax, smax[s2][lane]);
+                    psum += ssum[s2][lane];
+                }
+                pstat[2 * (size_t)c] = pmin;
+                pstat[2 * (size_t)c + 1] = pmax;
+                const bool broken = !(psum < DBL_MAX && pmin > 0.0);
+                if (SINGLE) {
+                    if (broken || pmin < rcond_min * pmax) {
+                        bad = 1;
+                        vote[c] = 0;
+                    }
+                } else if (broken) {
+                    bad = 1;
+                    status[c] = ST_FAILED;
+                    iters[c] += 1;
+                } else if (pmin < rcond_min * pmax) {
+                    atomicAdd(guard_trips, 1);
+                }
+            }
+            sbad[lane] = bad;
+        }
+        __syncthreads();
+        if (sbad[lane]) act = false;
+    }
+    for (int l = 0; l < nblev; ++l) {
+        const int i0 = l * NS + slot;
+        if (act) seg_stream<double, In2>(bwd, ptrW[i0], ptrW[i0 + 1], bent, fac, x, S, c, startW, finishW);
         __syncthreads();
     }
 }
@@ -857,13 +1045,12 @@ struct DevSched {  // one level schedule dealt to TL task lists
     int TL = 0, nlev = 0, nblev = 0;
     DevBuf<int> ptrA, ptrB, ptrF, ptrW, diagB;
     DevBuf<int4> taskA, taskB, fwd, bwd;
+    DevBuf<int2> pairs, fent, bent;  // operand streams in task-list order
 };
 
 struct DevTopo {
     DevBuf<int> Yp, Yj, Ydiag, row_p, row_q, posJ, xva, xvm, Jp, xcol;
     DevBuf<double> G, B;
-    DevBuf<int> Li;
-    DevBuf<int2> pairs, fent;
     std::map<int, std::unique_ptr<DevSched>> sched;
     bool ready = false;
 };
@@ -963,16 +1150,10 @@ void ensure_device(p3smi_topology* t) {
     d.posJ.upload(h.posJ.data(), h.posJ.size(), st);
     d.xva.upload(h.xva.data(), h.xva.size(), st);
     d.xvm.upload(h.xvm.data(), h.xvm.size(), st);
-    d.Li.upload(h.Li.data(), h.Li.size(), st);
     d.Jp.upload(h.Jp.data(), h.Jp.size(), st);
     std::vector<int> xcol(h.nnzJ);
     for (int e = 0; e < h.nnzJ; ++e) xcol[e] = h.pinv[h.Jj[e]];
     d.xcol.upload(xcol.data(), xcol.size(), st);
-    std::vector<int2> pr(h.pa.size()), fe(h.Fpos.size());
-    for (size_t i = 0; i < pr.size(); ++i) pr[i] = make_int2(h.pa[i], h.pb[i]);
-    for (size_t i = 0; i < fe.size(); ++i) fe[i] = make_int2(h.Fpos[i], h.Fcol[i]);
-    d.pairs.upload(pr.data(), pr.size(), st);
-    d.fent.upload(fe.data(), fe.size(), st);
     HIPCHECK(hipStreamSynchronize(st));
     d.ready = true;
 }
@@ -995,7 +1176,16 @@ const DevSched& get_sched(p3smi_topology* t, LuConfig cfg) {
     d->ptrF.upload(s.ptrF.data(), s.ptrF.size(), st);
     d->ptrW.upload(s.ptrW.data(), s.ptrW.size(), st);
     d->diagB.upload(s.diagB.data(), s.diagB.size(), st);
+    // Operand streams: the entries every task reads, laid out in execution order (tasks of
+    // one list and sub-level back to back), so that a task's range ends where the next begins.
     std::vector<int4> ta(s.taskA.size()), tb(s.taskB.size()), fw(s.fwd.size()), bw(s.bwd.size());
+    std::vector<int2> pr, fe, be;
+    pr.reserve((size_t)h.n_pairs);
+    auto put_pairs = [&](const p3smi::Task& k) {
+        const int b0 = (int)pr.size();
+        for (int e = k.pbeg; e < k.pend; ++e) pr.push_back(make_int2(h.pa[e], h.pb[e]));
+        return std::make_pair(b0, (int)pr.size());
+    };
     for (size_t i = 0; i < ta.size(); ++i) {
         const auto& k = s.taskA[i];
         int src = k.asrc;
@@ -1003,11 +1193,31 @@ const DevSched& get_sched(p3smi_topology* t, LuConfig cfg) {
             if (src < 0) throw Error(P3SMI_E_INTERNAL, "pivot without a Jacobian entry");
             src = -src - 2;
         }
-        ta[i] = make_int4(k.dst, src, k.pbeg, k.pend);
+        const auto r = put_pairs(k);
+        ta[i] = make_int4(k.dst, src, r.first, r.second);
     }
-    for (size_t i = 0; i < tb.size(); ++i) tb[i] = make_int4(s.taskB[i].dst, s.taskB[i].asrc, s.taskB[i].pbeg, s.taskB[i].pend);
-    for (size_t i = 0; i < fw.size(); ++i) fw[i] = make_int4(s.fwd[i].row, s.fwd[i].rsrc, s.fwd[i].ebeg, s.fwd[i].eend);
-    for (size_t i = 0; i < bw.size(); ++i) bw[i] = make_int4(s.bwd[i].row, s.bwd[i].base, s.bwd[i].lbeg, s.bwd[i].lend);
+    for (size_t i = 0; i < tb.size(); ++i) {
+        const auto r = put_pairs(s.taskB[i]);
+        tb[i] = make_int4(s.taskB[i].dst, s.taskB[i].asrc, r.first, r.second);
+    }
+    for (size_t i = 0; i < fw.size(); ++i) {
+        const auto& f = s.fwd[i];
+        const int b0 = (int)fe.size();
+        for (int e = f.ebeg; e < f.eend; ++e) fe.push_back(make_int2(h.Fpos[e], h.Fcol[e]));
+        fw[i] = make_int4(f.row, f.rsrc, b0, (int)fe.size());
+    }
+    for (size_t i = 0; i < bw.size(); ++i) {
+        const auto& w = s.bwd[i];
+        const int b0 = (int)be.size();
+        for (int e = w.lbeg; e < w.lend; ++e) be.push_back(make_int2(w.base + 1 + (e - w.lbeg), h.Li[e]));
+        bw[i] = make_int4(w.row, w.base, b0, (int)be.size());
+    }
+    if (pr.empty()) pr.push_back(make_int2(0, 0));  // keep every stream a valid pointer
+    if (fe.empty()) fe.push_back(make_int2(0, 0));
+    if (be.empty()) be.push_back(make_int2(0, 0));
+    d->pairs.upload(pr.data(), pr.size(), st);
+    d->fent.upload(fe.data(), fe.size(), st);
+    d->bent.upload(be.data(), be.size(), st);
     d->taskA.upload(ta.data(), ta.size(), st);
     d->taskB.upload(tb.data(), tb.size(), st);
     d->fwd.upload(fw.data(), fw.size(), st);
@@ -1111,10 +1321,14 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, int it, real* fac, const 
     const p3smi::Topology& h = b->topo->host;
     const DevTopo& d = b->topo->dev;
     const double rcond_min = REUSE_RCOND_RATIO * h.rcond0;
-    hipLaunchKernelGGL((k_lu<G, NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0, b->topo->stream,
-                       ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.diagB.p, d.pairs.p, ds.ptrF.p, ds.fwd.p,
-                       d.fent.p, ds.ptrW.p, ds.bwd.p, d.Li.p, b->Jx.p, fac, rhs, xout, b->status.p, b->iters.p,
-                       b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min, ds.nlev, ds.nblev, b->S, it);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)(b->S / G)), dim3(64 * NS), 0, b->topo->stream, ds.ptrA.p,
+                           ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.diagB.p, ds.pairs.p, ds.ptrF.p, ds.fwd.p, ds.fent.p,
+                           ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout, b->status.p, b->iters.p,
+                           b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min, ds.nlev, ds.nblev, b->S, it);
+    };
+    if constexpr (G == 64) go(k_lu_stream<NS, real, FACTOR>);
+    else go(k_lu<G, NS, real, FACTOR>);
 }
 
 bool valid_lu_config(LuConfig c) {
