@@ -1,15 +1,24 @@
 #!/usr/bin/env python3
 """Throughput benchmark: batched polar Newton power flow on one MI355X.
 
-Workload (default ``ts2000``): the 2,000-bus synthetic grid (spanning tree + 600 chords,
-200 PV buses, one slack), 8,760 hourly load cases (daily sinusoid x N(1, 0.05) per load,
-PV output x U(0, 1)), DC-power-flow start, tol 1e-8 p.u., max 20 iterations, FP64.
+Workloads (``--config``):
+
+* ``ts2000`` (default): the 2,000-bus synthetic grid (spanning tree + 600 chords, 200 PV
+  buses, one slack), 8,760 hourly load cases (daily sinusoid x N(1, 0.05) per load, PV
+  output x U(0, 1)), DC-power-flow start, tol 1e-8 p.u., max 20 iterations.
+* ``pf200``: the 200-bus grid, 96 load-scaled cases from a flat start.
+* ``n1_10k``: the 10,000-bus grid (spanning tree + 3,500 chords, 800 PV buses), every
+  single-branch outage (13,499 cases) with re-slacking of islands, DC start. One step is
+  the whole N-1 solve on the device: islands (union-find), per-case Ybus, Newton. The
+  other precision is timed too (mixed vs FP64).
 
 A step is one complete batched solve of all cases from the start voltage, with the
 inputs already resident in device memory. The JSON line reports converged cases/s over
 the timed steps, p50/p99 batch latency, the roofline position of the dominant kernel
-(HIP events on the library's stream, algorithmic bytes per launch) and the CPU oracle
-timed on the host cores over a bounded sample of the same workload.
+(HIP events on the library's stream, algorithmic bytes per launch; HBM traffic per launch
+from the committed rocprofv3 counter summary when one matches) and the CPU oracle timed on
+the host cores over a bounded sample of the same workload, with the per-case error of the
+GPU results against it.
 """
 
 from __future__ import annotations
@@ -18,6 +27,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -29,13 +39,107 @@ for p in (os.path.join(ROOT, "parallel-pandapower-solver_amd"), ROOT):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "converged PF cases/sec at 1e-8 p.u. mismatch; p50/p99 solve latency"
+MAX_ITER, TOL = 20, 1e-8
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_summary.md")
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def workload(name: str):
+def measured_traffic(kernel: str, workload: str, precision: str):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated) of a kernel on a workload, from
+    the lines 'traffic kernel=K workload=W precision=P bytes=B' of the committed summary."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    pat = re.compile(r"traffic kernel=(\S+) workload=(\S+) precision=(\S+) bytes=([0-9.eE+]+)")
+    with open(TRAFFIC_FILE) as fh:
+        for line in fh:
+            m = pat.search(line)
+            if m and m.group(1) == kernel and m.group(2) == workload and m.group(3) == precision:
+                return float(m.group(4))
+    return None
+
+
+def stats_acc():
+    return {"ms_eval": 0.0, "ms_lu": 0.0, "ms_step": 0.0, "ms_decide": 0.0, "ms_other": 0.0, "ms_refine": 0.0,
+            "n_eval": 0, "n_lu": 0, "n_step": 0, "n_decide": 0, "n_refine": 0, "double_factorizations": 0,
+            "single_factorizations": 0, "refinement_steps": 0, "newton_passes": 0, "line_search_passes": 0}
+
+
+def add_stats(acc, L, bh):
+    from p3smi import _lib
+
+    st = _lib.Stats()
+    L.p3smi_batch_stats(bh, C.byref(st))
+    for key in acc:
+        acc[key] += getattr(st, key)
+
+
+def roofline(acc, bpc, steps, workload, precision):
+    """Dominant kernel's achieved algorithmic GB/s over its event-timed launches."""
+    kern = "lu" if acc["ms_lu"] >= acc["ms_eval"] else "eval"
+    # one Jacobian evaluation and one factorization per case-iteration; in mixed precision the
+    # LU launches also carry the refinement solves, which the byte count leaves out
+    facts = acc["double_factorizations"] + acc["single_factorizations"]
+    bytes_total = facts * (bpc[1] if kern == "lu" else bpc[0])
+    ms = acc[f"ms_{kern}"]
+    n_launch = acc[f"n_{kern}"]
+    achieved = bytes_total / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    traffic = measured_traffic(f"k_{kern}", workload, precision)
+    return {
+        "kernel": f"k_{kern}",
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "avg_launch_ms": round(ms / max(n_launch, 1), 4),
+        "bytes_per_case": float(bpc[1] if kern == "lu" else bpc[0]),
+        "kernel_ms_per_step": {k: round(acc[f"ms_{k}"] / steps, 3)
+                               for k in ("eval", "lu", "refine", "step", "decide", "other")},
+    }
+
+
+def base_out(args, value, times, t_total, precision, config):
+    return {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "cases/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == "double" else "f32-lu+f64",
+        "data": "synthetic",
+        "config": config,
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(times, 50)), 3),
+                       "p99": round(1e3 * float(np.percentile(times, 99)), 3)},
+    }
+
+
+def lu_shape(info):
+    return [int(info["lu_g"]), int(info["lu_ns"]), int(info["lu_few_g"]), int(info["lu_few_ns"]),
+            int(info["lu_few_max"])]
+
+
+def dv_max(Vg, Vr, ok):
+    """max |dVm|, |dVa| over the finite entries of the columns ok."""
+    a, b = Vg[:, ok], Vr[:, ok]
+    fin = np.isfinite(a) & np.isfinite(b)
+    if not fin.any():
+        return 0.0, 0.0
+    return (float(np.abs(np.abs(a[fin]) - np.abs(b[fin])).max()),
+            float(np.abs(np.angle(a[fin]) - np.angle(b[fin])).max()))
+
+
+# ====================================================================== time series
+def timeseries_workload(name: str):
     from p3smi import grids
     from p3smi.timeseries import dc_initial_voltage
 
@@ -44,43 +148,56 @@ def workload(name: str):
         S = grids.timeseries_daily(g, 8760, seed=1)
         V0 = dc_initial_voltage(g)
         desc = "2000-bus synthetic grid, 8760 hourly cases, DC start, tol 1e-8, max_iter 20"
-    elif name == "pf200":
+    else:
         g = grids.config_grid("pf200", 0)
         S = grids.timeseries_scaled(g, 96, seed=1)
         V0 = g.flat_start()
         desc = "200-bus synthetic grid, 96 load-scaled cases, flat start, tol 1e-8, max_iter 20"
-    else:
-        raise SystemExit(f"unknown workload {name!r}")
     return g, S, V0, desc
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="ts2000")
-    ap.add_argument("--precision", default="double", choices=["double", "mixed"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample duration")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported (use --gpus 1)",
-              file=sys.stderr)
-        return 2
-
-    from p3smi import _lib, nr_klu
+def cpu_timeseries(args, g, S_c, V0_c, V, it, conv, out):
+    from oracle.oracle import OracleSolver
     from p3smi.timeseries import usable_cpus
 
-    if _lib.device_count() < 1:
-        print("bench.py: no HIP device visible", file=sys.stderr)
-        return 1
-    g, S, V0, desc = workload(args.config)
+    Yp, Yj, Yx = g.csr()
+    T = S_c.shape[1]
+    cores = usable_cpus()
+    ora = OracleSolver(Yp, Yj, Yx, g.pv, g.pq)
+    v0 = (lambda k: V0_c) if V0_c.ndim == 1 else (lambda k: V0_c[:, :k])
+    probe = min(T, 256)
+    t0 = time.perf_counter()
+    ora.solve_batch(S_c[:, :probe], v0(probe), MAX_ITER, TOL, 0)
+    rate = probe / max(time.perf_counter() - t0, 1e-6)
+    ns = int(min(T, max(probe, rate * args.cpu_seconds)))
+    log(f"[bench] cpu oracle sample: {ns} cases on {cores} threads")
+    Ss = np.ascontiguousarray(S_c[:, :ns])
+    t0 = time.perf_counter()
+    ref = ora.solve_batch(Ss, v0(ns), MAX_ITER, TOL, 0)
+    t_cpu = time.perf_counter() - t0
+    # one thread, a smaller sample (a few seconds)
+    n1 = int(min(ns, max(8, rate / max(cores, 1) * 3.0)))
+    t0 = time.perf_counter()
+    r1 = ora.solve_batch(np.ascontiguousarray(S_c[:, :n1]), v0(n1), MAX_ITER, TOL, 1)
+    t_one = time.perf_counter() - t0
+    out["cpu_baseline"] = {
+        "value": round(int(ref["converged"].sum()) / t_cpu, 1), "unit": "cases/s", "cores": cores, "kind": "port",
+        "sample": f"first {ns} of the {T} cases, solve call only, one pass, {cores} host threads",
+        "single_thread": {"value": round(int(r1["converged"].sum()) / t_one, 1), "cases": n1}}
+    okm = ref["converged"] & conv[:ns]
+    dvm, dva = dv_max(V[:, :ns], ref["V"], okm)
+    out["parity"] = {"cases": ns, "max_dVm": dvm, "max_dVa": dva,
+                     "converged_equal": bool(np.array_equal(ref["converged"], conv[:ns])),
+                     "iterations_equal": bool(np.array_equal(ref["iterations"], it[:ns]))}
+
+
+def run_timeseries(args) -> dict:
+    from p3smi import _lib, autotune, nr_klu
+
+    g, S, V0, desc = timeseries_workload(args.config)
     Yp, Yj, Yx = g.csr()
     T = S.shape[1]
     log(f"[bench] workload {args.config}: n={g.n} cases={T}")
-    from p3smi import autotune
-
     t0 = time.perf_counter()
     solver = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
     t_setup = time.perf_counter() - t0
@@ -100,25 +217,20 @@ def main() -> int:
     _lib.check(L.p3smi_batch_upload(bh, _lib.ptr(S_c), _lib.ptr(V0_c), int(V0_c.ndim == 2)))
 
     def solve():
-        _lib.check(L.p3smi_batch_solve(bh, 20, 1e-8, 1, prec))
+        _lib.check(L.p3smi_batch_solve(bh, MAX_ITER, TOL, 1, prec))
 
     for w in range(args.warmup):
         solve()
         log(f"[bench] warmup {w + 1}/{args.warmup} done")
-
     _lib.check(L.p3smi_batch_set_timing(bh, 1))
-    st = _lib.Stats()
-    acc = {"ms_eval": 0.0, "ms_lu": 0.0, "ms_step": 0.0, "ms_decide": 0.0, "ms_other": 0.0, "n_eval": 0, "n_lu": 0,
-           "n_step": 0, "n_decide": 0, "double_factorizations": 0, "newton_passes": 0, "line_search_passes": 0}
+    acc = stats_acc()
     times = []
     t_all0 = time.perf_counter()
     for k in range(args.steps):
         ts = time.perf_counter()
         solve()
         times.append(time.perf_counter() - ts)
-        L.p3smi_batch_stats(bh, C.byref(st))
-        for key in acc:
-            acc[key] += getattr(st, key)
+        add_stats(acc, L, bh)
         log(f"[bench] step {k + 1}/{args.steps}: {1e3 * times[-1]:.1f} ms")
     t_total = time.perf_counter() - t_all0
 
@@ -129,86 +241,175 @@ def main() -> int:
     _lib.check(L.p3smi_batch_download(bh, _lib.ptr(V), _lib.ptr(it), _lib.ptr(cv), _lib.ptr(mis)))
     conv = cv.astype(bool)
     n_conv = int(conv.sum())
-    value = n_conv * args.steps / t_total
-
     bpc = np.empty(4, dtype=np.float64)
     L.p3smi_batch_bytes_per_case(bh, _lib.ptr(bpc))
-    # dominant kernel: the one with the largest total time
-    kern = max(("lu", "eval"), key=lambda k: acc[f"ms_{k}"])
-    if kern == "lu":
-        cases_processed = acc["double_factorizations"]
-        bytes_total = cases_processed * bpc[1]
-        n_launch = acc["n_lu"]
-    else:
-        # eval passes touch the cases of their status mask; approximate with active cases per pass
-        cases_processed = acc["double_factorizations"] + T * args.steps
-        bytes_total = cases_processed * bpc[0]
-        n_launch = acc["n_eval"]
-    ms = acc[f"ms_{kern}"]
-    achieved = bytes_total / (ms / 1e3) / 1e9 if ms > 0 else 0.0
-    roofline = {
-        "kernel": f"k_{kern}",
-        "bound": "hbm",
-        "achieved": round(achieved, 2),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "avg_launch_ms": round(ms / max(n_launch, 1), 4),
-        "bytes_per_case": float(bpc[1] if kern == "lu" else bpc[0]),
-        "kernel_ms_per_step": {k: round(acc[f"ms_{k}"] / args.steps, 3) for k in ("eval", "lu", "step", "decide", "other")},
-    }
-
-    out = {
-        "metric": "converged PF cases/sec at 1e-8 p.u. mismatch; p50/p99 solve latency",
-        "value": round(value, 1),
-        "unit": "cases/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_total / args.steps, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "double" else "f32-lu+f64",
-        "data": "synthetic",
-        "config": {"workload": f"{args.config}: {desc}", "n_bus": g.n, "cases": T, "precision": args.precision,
-                   "nnz_lu": int(info["nnz_lu"]), "n_levels": int(info["n_levels"]),
-                   "lu_shape": [int(info["lu_g"]), int(info["lu_ns"]), int(info["lu_few_g"]), int(info["lu_few_ns"]),
-                                int(info["lu_few_max"])]},
-        "latency_ms": {"p50": round(1e3 * float(np.percentile(times, 50)), 3),
-                       "p99": round(1e3 * float(np.percentile(times, 99)), 3)},
-        "converged": n_conv,
-        "iterations_hist": np.bincount(it, minlength=1).tolist(),
-        "newton_passes_per_step": acc["newton_passes"] / args.steps,
-        "roofline": roofline,
-    }
-
+    config = {"workload": f"{args.config}: {desc}", "n_bus": g.n, "cases": T, "precision": args.precision,
+              "nnz_lu": int(info["nnz_lu"]), "n_levels": int(info["n_levels"]), "lu_shape": lu_shape(info)}
+    out = base_out(args, n_conv * args.steps / t_total, times, t_total, args.precision, config)
+    out.update({"converged": n_conv, "iterations_hist": np.bincount(it, minlength=1).tolist(),
+                "newton_passes_per_step": acc["newton_passes"] / args.steps,
+                "roofline": roofline(acc, bpc, args.steps, args.config, args.precision)})
     if not args.no_cpu:
-        from oracle.oracle import OracleSolver
-
-        cores = usable_cpus()
-        ora = OracleSolver(Yp, Yj, Yx, g.pv, g.pq)
-        probe = min(T, 256)
-        t0 = time.perf_counter()
-        ora.solve_batch(S_c[:, :probe], V0_c if V0_c.ndim == 1 else V0_c[:, :probe], 20, 1e-8, 0)
-        rate = probe / max(time.perf_counter() - t0, 1e-6)
-        ns = int(min(T, max(probe, rate * args.cpu_seconds)))
-        log(f"[bench] cpu oracle sample: {ns} cases on {cores} threads")
-        Ss = np.ascontiguousarray(S_c[:, :ns])
-        t0 = time.perf_counter()
-        ref = ora.solve_batch(Ss, V0_c if V0_c.ndim == 1 else V0_c[:, :ns], 20, 1e-8, 0)
-        t_cpu = time.perf_counter() - t0
-        cpu_value = int(ref["converged"].sum()) / t_cpu
-        out["cpu_baseline"] = {"value": round(cpu_value, 1), "unit": "cases/s", "cores": cores, "kind": "port",
-                               "sample": f"first {ns} of the {T} cases, one pass, all {cores} host threads"}
-        okm = ref["converged"] & conv[:ns]
-        dvm = float(np.abs(np.abs(V[:, :ns][:, okm]) - np.abs(ref["V"][:, okm])).max(initial=0.0))
-        dva = float(np.abs(np.angle(V[:, :ns][:, okm]) - np.angle(ref["V"][:, okm])).max(initial=0.0))
-        out["parity"] = {"cases": ns, "max_dVm": dvm, "max_dVa": dva,
-                         "converged_equal": bool(np.array_equal(ref["converged"], conv[:ns])),
-                         "iterations_equal": bool(np.array_equal(ref["iterations"], it[:ns]))}
+        cpu_timeseries(args, g, S_c, V0_c, V, it, conv, out)
     L.p3smi_batch_free(bh)
+    return out
+
+
+# ====================================================================== N-1
+def run_n1(args) -> dict:
+    from p3smi import _lib, autotune, contingency, grids, nr_klu
+    from p3smi.timeseries import dc_initial_voltage
+
+    g = grids.config_grid("n1_10k", 0)
+    Yp, Yj, Yx = g.csr()
+    t0 = time.perf_counter()
+    solver = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    autotune.apply_cached(solver)
+    cs = contingency.from_grid(g, reslack=True, solver=solver)
+    t_setup = time.perf_counter() - t0
+    info = solver.info
+    Lc = cs.L
+    log(f"[bench] workload n1_10k: n={g.n} outages={Lc} nnzLU={info['nnz_lu']} levels={info['n_levels']} "
+        f"({t_setup:.2f}s)")
+    S = np.ascontiguousarray(g.sbus(), dtype=np.complex128)
+    Vs = np.ascontiguousarray(dc_initial_voltage(g), dtype=np.complex128)
+    L = _lib.lib()
+    it = np.empty(Lc, dtype=np.int32)
+    cv = np.empty(Lc, dtype=np.uint8)
+    mis = np.empty(Lc, dtype=np.float64)
+
+    def solve(prec, V=None, served=None):
+        _lib.check(L.p3smi_outages_solve(cs._h, _lib.ptr(S), _lib.ptr(Vs), MAX_ITER, TOL, 1, prec,
+                                         _lib.ptr(V) if V is not None else None, _lib.ptr(it), _lib.ptr(cv),
+                                         _lib.ptr(mis), _lib.ptr(served) if served is not None else None))
+
+    main = args.precision
+    other = "double" if main == "mixed" else "mixed"
+    code = {"double": 0, "mixed": 1}
+    for w in range(args.warmup):
+        solve(code[main])
+        solve(code[other])
+        log(f"[bench] warmup {w + 1}/{args.warmup} done")
+    bh = C.c_void_p()
+    _lib.check(L.p3smi_topology_last_batch(solver._h, C.byref(bh)))
+    _lib.check(L.p3smi_batch_set_timing(bh, 1))
+
+    def timed(prec):
+        acc = stats_acc()
+        times = []
+        t_all0 = time.perf_counter()
+        for k in range(args.steps):
+            ts = time.perf_counter()
+            solve(code[prec])
+            times.append(time.perf_counter() - ts)
+            add_stats(acc, L, bh)
+            log(f"[bench] {prec} step {k + 1}/{args.steps}: {1e3 * times[-1]:.1f} ms")
+        return acc, times, time.perf_counter() - t_all0, int(cv.astype(bool).sum()), it.copy()
+
+    acc, times, t_total, n_conv, it_main = timed(main)
+    acc2, times2, t_total2, n_conv2, it_other = timed(other)
+    bpc = np.empty(4, dtype=np.float64)
+    L.p3smi_batch_bytes_per_case(bh, _lib.ptr(bpc))
+    _lib.check(L.p3smi_batch_set_timing(bh, 0))
+
+    # untimed: the full result table of the main precision (V, served) for the checks
+    V = np.empty((g.n, Lc), dtype=np.complex128)
+    served = np.empty((g.n, Lc), dtype=np.uint8)
+    solve(code[main], V, served)
+    conv = cv.astype(bool)
+    config = {"workload": f"n1_10k: 10000-bus synthetic grid, every single-branch outage ({Lc} cases), re-slack, "
+                          f"DC start, tol 1e-8, max_iter 20",
+              "n_bus": g.n, "cases": Lc, "precision": main, "nnz_lu": int(info["nnz_lu"]),
+              "n_levels": int(info["n_levels"]), "lu_shape": lu_shape(info)}
+    out = base_out(args, n_conv * args.steps / t_total, times, t_total, main, config)
+    value2 = n_conv2 * args.steps / t_total2
+    out.update({
+        "converged": n_conv,
+        "iterations_hist": np.bincount(it_main, minlength=1).tolist(),
+        "cases_with_unserved_buses": int((~served.astype(bool)).any(axis=0).sum()),
+        "newton_passes_per_step": acc["newton_passes"] / args.steps,
+        "roofline": roofline(acc, bpc, args.steps, "n1_10k", main),
+        "compare": {"precision": other, "value": round(value2, 1), "ms_per_step": round(1e3 * t_total2 / args.steps, 3),
+                    "converged": n_conv2, "iterations_equal": bool(np.array_equal(it_main, it_other)),
+                    "kernel_ms_per_step": {k: round(acc2[f"ms_{k}"] / args.steps, 3)
+                                           for k in ("eval", "lu", "refine", "step", "decide", "other")}},
+        "mixed_over_fp64": round((out["value"] / value2) if main == "mixed" else (value2 / out["value"]), 3),
+    })
+    if not args.no_cpu:
+        cpu_n1(args, g, cs, Vs, V, it_main, conv, served.astype(bool), out)
+    return out
+
+
+def cpu_n1(args, g, cs, Vs, V, it, conv, served, out):
+    """Oracle on the first k outage cases: islands and per-case Ybus restated on the host, then
+    the FP64 restatement of solve_batch_contingency timed on all host threads."""
+    from oracle import islands
+    from oracle.oracle import OracleSolver
+    from p3smi.timeseries import usable_cpus
+
+    Yp, Yj, Yx = g.csr()
+    cores = usable_cpus()
+    ora = OracleSolver(Yp, Yj, Yx, g.pv, g.pq)
+    gen_bus = np.nonzero(np.isfinite(g.gen_z))[0]
+
+    def inputs(c0, k):
+        groups = cs.groups[c0:c0 + k]
+        srv, _, pinned = islands.cases(g.n, g.f, g.t, cs.live, groups, g.ref, gen_bus, g.gen_z[gen_bus], True)
+        V0, pin = islands.solver_inputs(Vs, srv, pinned)
+        Yc = np.stack([islands.case_ybus(cs.solver.Yx, cs.ypos, cs.yval, gr) for gr in groups], axis=1)
+        return (np.ascontiguousarray(np.abs(Yc)), np.ascontiguousarray(np.angle(Yc)), np.ascontiguousarray(V0),
+                np.ascontiguousarray(pin), srv)
+
+    S = g.sbus()
+    probe = min(cs.L, 2 * cores)
+    mag, ang, V0, pin, _ = inputs(0, probe)
+    t0 = time.perf_counter()
+    ora.solve_batch_contingency(mag, ang, S, V0, pin, MAX_ITER, TOL, 0)
+    rate = probe / max(time.perf_counter() - t0, 1e-6)
+    ns = int(min(cs.L, 2048, max(probe, rate * args.cpu_seconds)))
+    log(f"[bench] cpu oracle sample: {ns} outage cases on {cores} threads")
+    mag, ang, V0, pin, srv = inputs(0, ns)
+    t0 = time.perf_counter()
+    ref = ora.solve_batch_contingency(mag, ang, S, V0, pin, MAX_ITER, TOL, 0)
+    t_cpu = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": round(int(ref["converged"].sum()) / t_cpu, 1), "unit": "cases/s", "cores": cores,
+                           "kind": "port",
+                           "sample": f"first {ns} of the {cs.L} outage cases, solve_batch_contingency call only, "
+                                     f"{cores} host threads"}
+    Vr = ref["V"].copy()
+    Vr[~srv] = np.nan
+    okm = ref["converged"] & conv[:ns]
+    dvm, dva = dv_max(V[:, :ns], Vr, okm)
+    out["parity"] = {"cases": ns, "max_dVm": dvm, "max_dVa": dva,
+                     "served_equal": bool(np.array_equal(srv, served[:, :ns])),
+                     "converged_equal": bool(np.array_equal(ref["converged"], conv[:ns])),
+                     "iterations_equal": bool(np.array_equal(ref["iterations"], it[:ns]))}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="ts2000", choices=["ts2000", "pf200", "n1_10k"])
+    ap.add_argument("--precision", default=None, choices=["double", "mixed"],
+                    help="default: double (time series), mixed (n1_10k)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample duration")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported (use --gpus 1)",
+              file=sys.stderr)
+        return 2
+    if args.precision is None:
+        args.precision = "mixed" if args.config == "n1_10k" else "double"
+
+    from p3smi import _lib
+
+    if _lib.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        return 1
+    out = run_n1(args) if args.config == "n1_10k" else run_timeseries(args)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -153,6 +153,14 @@ int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_
 /* Run the whole Newton solve of every case on the device from the uploaded inputs. */
 int p3smi_batch_solve(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_search, int32_t precision);
 int p3smi_batch_download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* converged, double* mismatch);
+/* The resident batch the one-call entry points (solve_batch*, outages_solve) of this
+ * topology ran on last: owned by the topology, valid until it is freed or a one-call solve
+ * of another case count replaces it. For timing (batch_set_timing) and statistics. */
+int p3smi_topology_last_batch(p3smi_topology* t, p3smi_batch** out);
+/* min |V| and max |V| of every case of the last solve (cases), over the buses a served
+ * mask keeps: the voltage-band test of NewtonPowerflowCpp.calculate
+ * (p3s/NewtonPowerflowCpp.py:553-584), computed on the device. */
+int p3smi_batch_vrange(p3smi_batch* b, double* vmin, double* vmax);
 int p3smi_batch_stats(const p3smi_batch* b, p3smi_stats* out);
 /* 1 = record HIP events around every kernel of later solves (per-kernel times in stats). */
 int p3smi_batch_set_timing(p3smi_batch* b, int32_t on);

@@ -233,28 +233,30 @@ __global__ __launch_bounds__(64 * EVAL_WAVES) void k_eval(
     }
 }
 
-// acc - sum_e fac[pa_e] * fac[pb_e] over the operand pairs [e, e1) of one factor entry.
-// Four pairs per trip: eight independent loads in flight before the first use.
-template <typename real>
-__device__ __forceinline__ real dot_pairs(const int2* __restrict__ pairs, int e, int e1, const real* __restrict__ fac,
-                                          int S, int c, real acc) {
-    real acc1 = 0;
+// Summation order shared by every LU kernel, so that a case's factors and step are the same
+// bits whichever launch shape (cases per workgroup, task lists) solved it: one accumulator
+// runs over the operand pairs of an entry in stream order from zero, s = fma(-a, b, s), and
+// the start value (Jacobian entry, right-hand side) is added once at the end.
+// dot_pairs returns that s over the pairs [e, e1); four pairs per trip keep eight loads
+// in flight before the first use.
+template <typename ACC, typename TA, typename TB>
+__device__ __forceinline__ ACC dot_pairs(const int2* __restrict__ pairs, int e, int e1, const TA* __restrict__ A,
+                                         const TB* __restrict__ Bv, int S, int c) {
+    ACC s = 0;
     for (; e + 4 <= e1; e += 4) {
         const int2 p0 = pairs[e], p1 = pairs[e + 1], p2 = pairs[e + 2], p3 = pairs[e + 3];
-        const real a0 = fac[at(p0.x, S, c)], b0 = fac[at(p0.y, S, c)];
-        const real a1 = fac[at(p1.x, S, c)], b1 = fac[at(p1.y, S, c)];
-        const real a2 = fac[at(p2.x, S, c)], b2 = fac[at(p2.y, S, c)];
-        const real a3 = fac[at(p3.x, S, c)], b3 = fac[at(p3.y, S, c)];
-        acc = fma(-a0, b0, acc);
-        acc1 = fma(-a1, b1, acc1);
-        acc = fma(-a2, b2, acc);
-        acc1 = fma(-a3, b3, acc1);
+        const TA a0 = A[at(p0.x, S, c)], a1 = A[at(p1.x, S, c)], a2 = A[at(p2.x, S, c)], a3 = A[at(p3.x, S, c)];
+        const TB b0 = Bv[at(p0.y, S, c)], b1 = Bv[at(p1.y, S, c)], b2 = Bv[at(p2.y, S, c)], b3 = Bv[at(p3.y, S, c)];
+        s = fma(-(ACC)a0, (ACC)b0, s);
+        s = fma(-(ACC)a1, (ACC)b1, s);
+        s = fma(-(ACC)a2, (ACC)b2, s);
+        s = fma(-(ACC)a3, (ACC)b3, s);
     }
     for (; e < e1; ++e) {
         const int2 p = pairs[e];
-        acc = fma(-fac[at(p.x, S, c)], fac[at(p.y, S, c)], acc);
+        s = fma(-(ACC)A[at(p.x, S, c)], (ACC)Bv[at(p.y, S, c)], s);
     }
-    return acc + acc1;
+    return s;
 }
 
 // Static-pivot LU + substitution. A workgroup owns G consecutive cases and runs
@@ -327,7 +329,8 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
                     const int4 tk = taskA[q];
                     const bool diag = tk.y < -1;
                     const int src = diag ? -tk.y - 2 : tk.y;
-                    real acc = dot_pairs<real>(pairs, tk.z, tk.w, fac, S, c, src >= 0 ? (real)Jx[at(src, S, c)] : (real)0);
+                    real acc = (src >= 0 ? (real)Jx[at(src, S, c)] : (real)0) +
+                               dot_pairs<real>(pairs, tk.z, tk.w, fac, fac, S, c);
                     if (diag) {
                         const double ab = fabs((double)acc);
                         pmin = fmin(pmin, ab);
@@ -341,20 +344,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
             const int f0 = ptrF[l * TL + slot], f1 = ptrF[l * TL + slot + 1];
             for (int q = f0; q < f1; ++q) {
                 const int4 fr = fwd[q];
-                double acc = rhs[at(fr.y, S, c)], acc1 = 0.0;
-                int e = fr.z;
-                for (; e + 2 <= fr.w; e += 2) {
-                    const int2 f0e = fent[e], f1e = fent[e + 1];
-                    const double a0 = (double)fac[at(f0e.x, S, c)], y0 = x[at(f0e.y, S, c)];
-                    const double a1 = (double)fac[at(f1e.x, S, c)], y1 = x[at(f1e.y, S, c)];
-                    acc = fma(-a0, y0, acc);
-                    acc1 = fma(-a1, y1, acc1);
-                }
-                if (e < fr.w) {
-                    const int2 fe = fent[e];
-                    acc = fma(-(double)fac[at(fe.x, S, c)], x[at(fe.y, S, c)], acc);
-                }
-                x[at(fr.x, S, c)] = acc + acc1;
+                x[at(fr.x, S, c)] = rhs[at(fr.y, S, c)] + dot_pairs<double>(fent, fr.z, fr.w, fac, x, S, c);
             }
         }
         __syncthreads();
@@ -364,7 +354,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
                 for (int q = q0; q < q1; ++q) {
                     const int4 tk = taskB[q];
                     const real acc =
-                        dot_pairs<real>(pairs, tk.z, tk.w, fac, S, c, tk.y >= 0 ? (real)Jx[at(tk.y, S, c)] : (real)0);
+                        (tk.y >= 0 ? (real)Jx[at(tk.y, S, c)] : (real)0) + dot_pairs<real>(pairs, tk.z, tk.w, fac, fac, S, c);
                     fac[at(tk.x, S, c)] = acc * fac[at(diagB[q], S, c)];
                 }
             }
@@ -410,20 +400,8 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
             const int q0 = ptrW[l * TL + slot], q1 = ptrW[l * TL + slot + 1];
             for (int q = q0; q < q1; ++q) {
                 const int4 br = bwd[q];
-                double acc = x[at(br.x, S, c)], acc1 = 0.0;
-                int e = br.z;
-                for (; e + 2 <= br.w; e += 2) {
-                    const int2 b0 = bent[e], b1 = bent[e + 1];
-                    const double u0 = (double)fac[at(b0.x, S, c)], x0 = x[at(b0.y, S, c)];
-                    const double u1 = (double)fac[at(b1.x, S, c)], x1 = x[at(b1.y, S, c)];
-                    acc = fma(-u0, x0, acc);
-                    acc1 = fma(-u1, x1, acc1);
-                }
-                if (e < br.w) {
-                    const int2 be = bent[e];
-                    acc = fma(-(double)fac[at(be.x, S, c)], x[at(be.y, S, c)], acc);
-                }
-                x[at(br.x, S, c)] = (acc + acc1) * (double)fac[at(br.y, S, c)];
+                const double acc = x[at(br.x, S, c)] + dot_pairs<double>(bent, br.z, br.w, fac, x, S, c);
+                x[at(br.x, S, c)] = acc * (double)fac[at(br.y, S, c)];
             }
         }
         __syncthreads();
@@ -830,6 +808,26 @@ __global__ void k_finalize(const double* __restrict__ Vm, const double* __restri
 }
 
 
+
+// Voltage band of every case: min and max |V| over its buses, skipping buses a served mask
+// marks unserved. NewtonPowerflowCpp.calculate accepts a converged solve only if every
+// |V| lies in [0.5, 1.5] p.u. (p3s/NewtonPowerflowCpp.py:553-584). One thread per case,
+// reads coalesced across cases.
+__global__ void k_vrange(const double* __restrict__ Vm, const unsigned char* __restrict__ served,
+                         double* __restrict__ vmin, double* __restrict__ vmax, int n, int S, int B) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= B) return;
+    double lo = DBL_MAX, hi = -DBL_MAX;
+    for (int i = 0; i < n; ++i) {
+        const size_t o = at(i, S, c);
+        if (served && !served[o]) continue;
+        const double v = fabs(Vm[o]);
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+    vmin[c] = lo;
+    vmax[c] = hi;
+}
 
 // ===================================================================== slots (serving)
 // Copy k staged cases (S and V0, (n, k) complex) into slot columns and mark them NEW.
@@ -1755,6 +1753,36 @@ int p3smi_batch_solve(p3smi_batch* b, int32_t max_iter, double tol, int32_t line
 
 int p3smi_batch_download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* converged, double* mismatch) {
     return guarded([&] { download(b, V_out, iterations, converged, mismatch); });
+}
+
+int p3smi_topology_last_batch(p3smi_topology* t, p3smi_batch** out) {
+    if (!t || !out) {
+        g_err = "topology_last_batch: null argument";
+        return P3SMI_E_ARG;
+    }
+    *out = t->cached.get();
+    if (!*out) {
+        g_err = "topology_last_batch: no one-call solve has run on this topology";
+        return P3SMI_E_ARG;
+    }
+    return P3SMI_OK;
+}
+
+int p3smi_batch_vrange(p3smi_batch* b, double* vmin, double* vmax) {
+    return guarded([&] {
+        if (!b || !vmin || !vmax) throw Error(P3SMI_E_ARG, "batch_vrange: null argument");
+        const p3smi::Topology& h = b->topo->host;
+        hipStream_t st = b->topo->stream;
+        DevBuf<double> lo, hi;
+        lo.alloc(b->B);
+        hi.alloc(b->B);
+        hipLaunchKernelGGL(k_vrange, dim3((unsigned)((b->B + 255) / 256)), dim3(256), 0, st, b->Vm.p,
+                           b->mask_unserved ? b->served.p : nullptr, lo.p, hi.p, h.n, b->S, b->B);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(vmin, lo.p, b->B * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(vmax, hi.p, b->B * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    });
 }
 
 int p3smi_batch_stats(const p3smi_batch* b, p3smi_stats* out) {
