@@ -201,7 +201,7 @@ def run_timeseries(args) -> dict:
     t0 = time.perf_counter()
     solver = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
     t_setup = time.perf_counter() - t0
-    if not autotune.apply_cached(solver):
+    if not autotune.apply_cached(solver, T):
         log("[bench] no cached LU shape for this size bucket: tuning (not timed)")
         autotune.tune(solver, S, V0, reps=1, log=log)
     info = solver.info
@@ -264,7 +264,7 @@ def run_n1(args) -> dict:
     Yp, Yj, Yx = g.csr()
     t0 = time.perf_counter()
     solver = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
-    autotune.apply_cached(solver)
+    autotune.apply_cached(solver, g.nbr)
     cs = contingency.from_grid(g, reslack=True, solver=solver)
     t_setup = time.perf_counter() - t0
     info = solver.info
@@ -286,6 +286,8 @@ def run_n1(args) -> dict:
     main = args.precision
     other = "double" if main == "mixed" else "mixed"
     code = {"double": 0, "mixed": 1}
+    # the first solve allocates the resident batch whose kernel timers the steps read
+    args.warmup = max(1, args.warmup)
     for w in range(args.warmup):
         solve(code[main])
         solve(code[other])
@@ -386,6 +388,25 @@ def cpu_n1(args, g, cs, Vs, V, it, conv, served, out):
                      "iterations_equal": bool(np.array_equal(ref["iterations"], it[:ns]))}
 
 
+def calibrate_copy() -> int:
+    """One device-to-device copy of 1 GiB, three times: the reference reading for the HBM
+    traffic counters (FETCH_SIZE, WRITE_SIZE per dispatch against 1 GiB read and written)."""
+    hip = C.CDLL("libamdhip64.so")
+    n = 1 << 30
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        print("bench.py: hipMalloc failed", file=sys.stderr)
+        return 1
+    ok = hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        ok = ok and hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration": "device copy", "bytes": n, "repeats": 3, "ok": bool(ok)}), flush=True)
+    return 0 if ok else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,7 +417,11 @@ def main() -> int:
                     help="default: double (time series), mixed (n1_10k)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample duration")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (reference for the HBM traffic counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported (use --gpus 1)",
               file=sys.stderr)

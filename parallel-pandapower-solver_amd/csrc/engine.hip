@@ -489,9 +489,13 @@ __device__ __forceinline__ void seg_stream(const int4* __restrict__ tasks, int q
 struct In1 { double v; };
 struct In2 { double v, d; };
 
-// Same contract and results (up to summation order) as k_lu<64, NS, real, FACTOR>. All
-// lanes of a wavefront belong to one task list, so every loop bound below is uniform.
-template <int NS, typename real, bool FACTOR>
+// Same contract and results (bit for bit: same summation order) as k_lu<G, NS, real,
+// FACTOR>. A workgroup owns G consecutive cases and TL = 64 * NS / G task lists; lane t works
+// on case t % G and list t / G. With G = 64 a wavefront is one list and every loop bound
+// is uniform (scalar); with G = 32 or 16 the two or four lists of a wavefront walk their
+// streams side by side (divergent only where their task boundaries differ), which gives a
+// batch of few cases more lists per case and more workgroups for the compute units.
+template <int G, int NS, typename real, bool FACTOR>
 __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
     const int4* __restrict__ taskB, const int* __restrict__ diagB, const int2* __restrict__ pairs,
@@ -502,11 +506,12 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     int* __restrict__ vote, const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S,
     int it) {
     constexpr bool SINGLE = sizeof(real) == 4;
-    __shared__ double smin[NS][64], smax[NS][64], ssum[NS][64];
-    __shared__ int sbad[64];
-    const int lane = threadIdx.x & 63;
-    const int slot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int c = blockIdx.x * 64 + lane;
+    constexpr int TL = 64 * NS / G;
+    __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
+    __shared__ int sbad[G];
+    const int lane = threadIdx.x % G;
+    const int slot = G == 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G)) : (int)(threadIdx.x / G);
+    const int c = blockIdx.x * G + lane;
     bool act = selected(sel, c, status, vote, rf);
     if (!__syncthreads_or(act)) return;
 
@@ -540,7 +545,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     auto finishW = [&](int4 t, In2 in, double sum) { x[at(t.x, S, c)] = (in.v + sum) * in.d; };
 
     for (int l = 0; l < nlev; ++l) {
-        const int i0 = l * NS + slot;
+        const int i0 = l * TL + slot;
         if (act) {
             if (FACTOR)
                 seg_stream<real, In1>(taskA, ptrA[i0], ptrA[i0 + 1], pairs, fac, fac, S, c, startA, finishA);
@@ -560,7 +565,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
         if (slot == 0) {
             int bad = 0;
             if (act) {
-                for (int s2 = 1; s2 < NS; ++s2) {
+                for (int s2 = 1; s2 < TL; ++s2) {
                     pmin = fmin(pmin, smin[s2][lane]);
                     pmax = fmax(pmax, smax[s2][lane]);
                     psum += ssum[s2][lane];
@@ -587,7 +592,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
         if (sbad[lane]) act = false;
     }
     for (int l = 0; l < nblev; ++l) {
-        const int i0 = l * NS + slot;
+        const int i0 = l * TL + slot;
         if (act) seg_stream<double, In2>(bwd, ptrW[i0], ptrW[i0 + 1], bent, fac, x, S, c, startW, finishW);
         __syncthreads();
     }
@@ -1325,7 +1330,7 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, int it, real* fac, const 
                            ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout, b->status.p, b->iters.p,
                            b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min, ds.nlev, ds.nblev, b->S, it);
     };
-    if constexpr (G == 64) go(k_lu_stream<NS, real, FACTOR>);
+    if constexpr (G >= 16) go(k_lu_stream<G, NS, real, FACTOR>);
     else go(k_lu<G, NS, real, FACTOR>);
 }
 
@@ -1389,6 +1394,7 @@ void linear_solve(p3smi_batch* b, int it, int n_active, int precision) {
         return;
     }
     ensure_mixed(b);
+    b->stats.single_factorizations += n_active;  // cases whose FP32 pivots fail are refactored in FP64 below
     const unsigned gS = (unsigned)((b->S + 255) / 256);
     {
         KTimer tm(b, 4);

@@ -1,11 +1,14 @@
-"""Pick the LU launch shape per grid-size bucket by timing candidates on the device.
+"""Pick the LU launch shape per grid-size bucket and batch-size class by timing candidates
+on the device.
 
 The batched LU kernel has two shape parameters: g, the cases one workgroup factors
 together, and ns, its wavefronts (so 64 * ns / g task lists per case). Large g keeps
 memory accesses in long contiguous lines; small g spreads one factorization over more
 task lists and finishes it sooner. The best pair depends on the factor pattern and on how
-many cases are still iterating, so the tuner times every candidate on a sample of the
-real batch and caches the winner per size bucket (<= 500, <= 5k, <= 20k, > 20k buses).
+many cases there are: a batch of fewer than 8,192 cases gives g = 64 fewer than 128
+workgroups for the 256 compute units, where more task lists per case pay. So the tuner
+times every candidate on the real batch and caches the winner per size bucket (<= 500,
+<= 5k, <= 20k, > 20k buses) and case class ("few" < 8,192 cases <= "many").
 """
 
 from __future__ import annotations
@@ -26,11 +29,18 @@ BUCKETS = [(500, "le500"), (5000, "le5k"), (20000, "le20k"), (10**9, "gt20k")]
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
 
 
+MANY_CASES = 8192
+
+
 def bucket(n_bus: int) -> str:
     for lim, name in BUCKETS:
         if n_bus <= lim:
             return name
     return BUCKETS[-1][1]
+
+
+def key(n_bus: int, cases: int) -> str:
+    return f"{bucket(n_bus)}:{'many' if cases >= MANY_CASES else 'few'}"
 
 
 def _time_solve(solver, bh, reps: int) -> float:
@@ -92,9 +102,12 @@ def save_cache(entries: dict) -> None:
         json.dump(entries, fh, indent=1, sort_keys=True)
 
 
-def apply_cached(solver) -> bool:
-    """Set the cached shape of the solver's size bucket, if there is one."""
-    e = load_cache().get(bucket(solver.n))
+def apply_cached(solver, cases: int = 1) -> bool:
+    """Set the cached shape for the solver's size bucket and a batch of `cases` cases (the
+    other case class of the bucket when only that one is cached), if there is one."""
+    cache = load_cache()
+    b = bucket(solver.n)
+    e = cache.get(key(solver.n, cases)) or cache.get(f"{b}:few") or cache.get(f"{b}:many")
     if not e:
         return False
     solver.set_lu_shape(e["g"], e["ns"], e["few_g"], e["few_ns"], e["few_max"], e.get("group", 0))

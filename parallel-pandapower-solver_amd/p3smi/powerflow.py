@@ -93,8 +93,11 @@ class PowerFlow:
     # ------------------------------------------------------------------ many cases
     def _device_solve(self, S, V0, v0_per_case, max_iterations, tolerance, line_search):
         """Resident-batch solve: V (n, B), iterations, converged, mismatch, vmin, vmax."""
+        from . import autotune
+
         L = _lib.lib()
         B = S.shape[1]
+        autotune.apply_cached(self.solver, B)
         bh = C.c_void_p()
         _lib.check(L.p3smi_batch_create(self.solver._h, B, C.byref(bh)))
         try:

@@ -242,7 +242,7 @@ class Server:
             solver = Solver(Yp, Yj, Yx, pv, pq)
             from . import autotune
 
-            autotune.apply_cached(solver)
+            autotune.apply_cached(solver, self.max_slots)
             self.solvers[key] = solver
         per = case_footprint(solver.info)
         want = self.max_slots * per

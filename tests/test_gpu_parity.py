@@ -113,3 +113,30 @@ def test_mixed_precision_perturbed_starts(gpu):
     r = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8, precision="mixed")
     ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
     _compare(r, ref, "perturbed mixed")
+
+
+SHAPES = [(64, 16), (64, 8), (32, 16), (32, 8), (16, 16), (16, 8), (1, 16), (1, 8)]
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_every_lu_shape_gives_the_same_bits(gpu, precision):
+    """Every launch shape sums each factor entry in one order, so a case's result does not
+    depend on the shape that solved it (serving relies on this); the results also meet the
+    oracle bar."""
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    S = grids.timeseries_scaled(g, 192, seed=3)
+    V0 = g.flat_start()
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    first = None
+    for gg, ns in SHAPES:
+        s.set_lu_shape(gg, ns, 1, 16, 0)  # few_max 0: every launch uses the batch shape
+        r = s.solve_batch(S, V0, max_iter=20, tol=1e-8, precision=precision)
+        if first is None:
+            first = r
+            ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+            _compare(r, ref, f"shape {gg}x{ns}")
+            continue
+        assert np.array_equal(r["iterations"], first["iterations"]), (gg, ns)
+        assert np.array_equal(r["V"].view(np.float64), first["V"].view(np.float64)), (gg, ns)
+        assert np.array_equal(r["mismatch"], first["mismatch"]), (gg, ns)
