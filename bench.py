@@ -11,6 +11,12 @@ Workloads (``--config``):
   single-branch outage (13,499 cases) with re-slacking of islands, DC start. One step is
   the whole N-1 solve on the device: islands (union-find), per-case Ybus, Newton. The
   other precision is timed too (mixed vs FP64).
+* ``multi48``: 48 grids of 100..20,000 buses with 4,096 cases in all, submitted at once to
+  the continuous-batching server (one slot pool per grid, pools overlapping on the device),
+  both precisions, against the plain FP64 one-grid-at-a-time loop.
+* ``serve64``: Poisson arrivals (``--rate``, ``--requests``) of time-series and N-1 jobs
+  over 64 cached topologies of 100..50,000 buses; p50/p99 request latency and throughput
+  against the plain FP64 one-request-at-a-time loop.
 
 A step is one complete batched solve of all cases from the start voltage, with the
 inputs already resident in device memory. The JSON line reports converged cases/s over
@@ -388,6 +394,193 @@ def cpu_n1(args, g, cs, Vs, V, it, conv, served, out):
                      "iterations_equal": bool(np.array_equal(ref["iterations"], it[:ns]))}
 
 
+# ====================================================================== many grids
+def _counts(rng, k, total, lo, hi):
+    """k batch sizes, log-uniform in [lo, hi], rescaled to sum to `total` (each kept in range)."""
+    c = np.exp(rng.uniform(np.log(lo), np.log(hi), k))
+    c = np.clip(np.round(c * total / c.sum()), lo, hi).astype(int)
+    while c.sum() != total:
+        d = total - c.sum()
+        i = int(np.argmax(c)) if d < 0 else int(np.argmin(c))
+        c[i] = int(np.clip(c[i] + np.sign(d) * min(abs(d), hi), lo, hi))
+    return c
+
+
+def _run_requests(srv, jobs, arrivals=None):
+    """Submit jobs (at their arrival offsets, seconds, if given), wait for all. Returns the
+    requests, the wall time from the first submission to the last completion."""
+    reqs = []
+    t0 = time.perf_counter()
+    for k, job in enumerate(jobs):
+        if arrivals is not None:
+            dt = t0 + arrivals[k] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+        if job["kind"] == "ts":
+            reqs.append(srv.submit(*job["topo"], job["S"], job["V0"]))
+        else:
+            reqs.append(srv.submit_contingency(job["key"], job["cases"], job["S"], job["V0"]))
+    for r in reqs:
+        r.wait()
+    return reqs, max(r.t_done for r in reqs) - t0
+
+
+def _latency(reqs):
+    lat = np.array([r.latency for r in reqs])
+    return {"p50": round(1e3 * float(np.percentile(lat, 50)), 3), "p99": round(1e3 * float(np.percentile(lat, 99)), 3)}
+
+
+def _sequential(jobs, solvers, n1sets, arrivals=None):
+    """The plain FP64 loop: one request at a time on the same GPU, in arrival order. Latency
+    = queueing (if arrivals are given) + service, with the measured service times."""
+    from p3smi import autotune
+
+    t_free = 0.0
+    lat, conv, cases = [], 0, 0
+    t0 = time.perf_counter()
+    for k, job in enumerate(jobs):
+        ts = time.perf_counter()
+        if job["kind"] == "ts":
+            sol = solvers[job["gid"]]
+            autotune.apply_cached(sol, job["S"].shape[1])
+            r = sol.solve_batch(job["S"], job["V0"], max_iter=MAX_ITER, tol=TOL)
+            c = r["converged"]
+        else:
+            from p3smi.contingency import ContingencySet
+
+            full = n1sets[job["gid"]]
+            sub = ContingencySet(full.solver, full.f, full.t, *full.yval.T, [full.groups[i] for i in job["cases"]],
+                                 full.ref, full.gen_z, full.reslack)
+            c = sub.solve(job["S"], job["V0"], max_iter=MAX_ITER, tol=TOL).converged
+        svc = time.perf_counter() - ts
+        arr = arrivals[k] if arrivals is not None else 0.0
+        start = max(arr, t_free)
+        t_free = start + svc
+        lat.append(t_free - arr)
+        conv += int(c.sum())
+        cases += int(c.size)
+    wall = time.perf_counter() - t0 if arrivals is None else t_free - (arrivals[0] if len(arrivals) else 0.0)
+    return {"value": round(conv / wall, 1), "wall_s": round(wall, 3), "converged": conv, "cases": cases,
+            "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3)}}
+
+
+def run_multi48(args) -> dict:
+    """48 grids, sizes log-uniform in 100..20,000 buses, 4,096 load-scaled cases in batches
+    of 1..256 per grid, all submitted at once to the continuous-batching server: every grid
+    gets a slot pool sized to its batch, the pools iterate concurrently on their own streams."""
+    from p3smi import grids, nr_klu
+    from p3smi.serving import Server
+    from p3smi.timeseries import dc_initial_voltage
+
+    rng = np.random.default_rng(4)
+    fam = grids.grid_family(48, 100, 20000, seed=4)
+    counts = _counts(rng, 48, 4096, 1, 256)
+    jobs, solvers = [], []
+    for k, (g, b) in enumerate(zip(fam, counts)):
+        Yp, Yj, Yx = g.csr()
+        jobs.append({"kind": "ts", "gid": k, "topo": (Yp, Yj, Yx, g.pv, g.pq),
+                     "S": np.ascontiguousarray(grids.timeseries_scaled(g, int(b), seed=100 + k, sigma=0.1)),
+                     "V0": np.ascontiguousarray(dc_initial_voltage(g))})
+        solvers.append(nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq))
+    log(f"[bench] multi48: sizes {min(g.n for g in fam)}..{max(g.n for g in fam)} buses, {int(counts.sum())} cases")
+    res = {}
+    for prec in (args.precision, "mixed" if args.precision == "double" else "double"):
+        srv = Server(tol=TOL, max_iter=MAX_ITER, precision=prec, max_slots=256)
+        for w in range(args.warmup):
+            _run_requests(srv, jobs)
+            log(f"[bench] {prec} warmup {w + 1}/{args.warmup} done")
+        walls, lats, conv = [], [], 0
+        for k in range(args.steps):
+            reqs, wall = _run_requests(srv, jobs)
+            walls.append(wall)
+            lats += [r.latency for r in reqs]
+            conv = sum(int(r.converged.sum()) for r in reqs)
+            log(f"[bench] {prec} step {k + 1}/{args.steps}: {1e3 * wall:.1f} ms")
+        srv.close()
+        res[prec] = {"value": conv * len(walls) / sum(walls), "walls": walls, "conv": conv,
+                     "lat": {"p50": round(1e3 * float(np.percentile(lats, 50)), 3),
+                             "p99": round(1e3 * float(np.percentile(lats, 99)), 3)}}
+    seq = _sequential(jobs, solvers, None)
+    main, other = res[args.precision], res["mixed" if args.precision == "double" else "double"]
+    config = {"workload": "multi48: 48 synthetic grids, sizes log-uniform 100-20000 buses, 4096 load-scaled cases "
+                          "(batches of 1-256 per grid, N(1, 0.1) load scaling), DC start, all submitted at once to "
+                          "the continuous-batching server", "grids": 48, "cases": int(counts.sum()),
+              "precision": args.precision}
+    out = base_out(args, main["value"], main["walls"], sum(main["walls"]), args.precision, config)
+    out["latency_ms"] = main["lat"]
+    out.update({"converged": main["conv"],
+                "compare": {"precision": "mixed" if args.precision == "double" else "double",
+                            "value": round(other["value"], 1), "latency_ms": other["lat"]},
+                "sequential_fp64_one_at_a_time": seq})
+    return out
+
+
+def run_serve64(args) -> dict:
+    """Poisson arrivals of (grid, batch) jobs over 64 cached topologies (100..50,000 buses,
+    log-uniform), batch sizes log-uniform 1..1,024, 70 % time-series and 30 % N-1 subsets, at
+    `--rate` requests/s; against the plain FP64 one-request-at-a-time loop."""
+    from p3smi import contingency, grids, nr_klu
+    from p3smi.serving import Server, topology_key
+    from p3smi.timeseries import dc_initial_voltage
+
+    rng = np.random.default_rng(5)
+    fam = grids.grid_family(64, 100, 50000, seed=5)
+    solvers, starts, topos = [], [], []
+    for g in fam:
+        Yp, Yj, Yx = g.csr()
+        topos.append((Yp, Yj, Yx, g.pv, g.pq))
+        solvers.append(nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq))
+        starts.append(np.ascontiguousarray(dc_initial_voltage(g)))
+    nreq = args.requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, nreq))
+    arrivals -= arrivals[0]
+    jobs = []
+    for k in range(nreq):
+        gid = int(rng.integers(0, len(fam)))
+        g = fam[gid]
+        b = int(np.clip(np.round(np.exp(rng.uniform(0.0, np.log(1024)))), 1, 1024))
+        if rng.random() < 0.3:
+            b = min(b, g.nbr)
+            jobs.append({"kind": "n1", "gid": gid, "cases": np.sort(rng.choice(g.nbr, b, replace=False)),
+                         "S": np.ascontiguousarray(g.sbus()), "V0": starts[gid]})
+        else:
+            jobs.append({"kind": "ts", "gid": gid, "topo": topos[gid], "V0": starts[gid],
+                         "S": np.ascontiguousarray(grids.timeseries_scaled(g, b, seed=1000 + k, sigma=0.1))})
+    total = sum(j["S"].shape[1] if j["kind"] == "ts" else j["cases"].size for j in jobs)
+    log(f"[bench] serve64: {nreq} requests at {args.rate} req/s, {total} cases, "
+        f"{sum(j['kind'] == 'n1' for j in jobs)} N-1 subsets")
+    srv = Server(tol=TOL, max_iter=MAX_ITER, precision=args.precision, max_slots=1024)
+    n1sets = {}
+    for gid, g in enumerate(fam):
+        key = srv.register_outages(*topos[gid], g.f, g.t, g.branch_stamps(),
+                                   contingency.single_branch_outages(g.nbr), g.ref, g.gen_z, True)
+        n1sets[gid] = srv.outage_sets[key]
+    for j in jobs:
+        if j["kind"] == "n1":
+            j["key"] = topology_key(*topos[j["gid"]])
+    # warm the topology cache: one small request per grid (plans, schedules, pools)
+    warm = [{"kind": "ts", "gid": i, "topo": topos[i], "V0": starts[i],
+             "S": np.ascontiguousarray(grids.timeseries_scaled(fam[i], 8, seed=7, sigma=0.1))} for i in range(len(fam))]
+    _run_requests(srv, warm)
+    log("[bench] topology cache warm")
+    reqs, wall = _run_requests(srv, jobs, arrivals)
+    conv = sum(int(r.converged.sum()) for r in reqs)
+    srv.close()
+    log(f"[bench] served: {1e3 * wall:.1f} ms")
+    seq = _sequential(jobs, solvers, n1sets, arrivals)
+    config = {"workload": f"serve64: Poisson arrivals at {args.rate} req/s, {nreq} requests over 64 cached synthetic "
+                          "topologies (100-50000 buses, log-uniform), batches log-uniform 1-1024 cases, 70% time "
+                          "series / 30% N-1 subsets, DC start", "requests": nreq, "cases": int(total),
+              "precision": args.precision}
+    out = base_out(args, conv / wall, [r.latency for r in reqs], wall * args.steps, args.precision, config)
+    out["latency_ms"] = _latency(reqs)
+    out["steps"] = 1
+    out["ms_per_step"] = round(1e3 * wall, 3)
+    out.update({"converged": conv, "sequential_fp64_one_at_a_time": seq})
+    return out
+
+
 def calibrate_copy() -> int:
     """One device-to-device copy of 1 GiB, three times: the reference reading for the HBM
     traffic counters (FETCH_SIZE, WRITE_SIZE per dispatch against 1 GiB read and written)."""
@@ -412,7 +605,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="ts2000", choices=["ts2000", "pf200", "n1_10k"])
+    ap.add_argument("--config", default="ts2000", choices=["ts2000", "pf200", "n1_10k", "multi48", "serve64"])
+    ap.add_argument("--rate", type=float, default=200.0, help="serve64: mean arrival rate (requests/s)")
+    ap.add_argument("--requests", type=int, default=300, help="serve64: number of requests")
     ap.add_argument("--precision", default=None, choices=["double", "mixed"],
                     help="default: double (time series), mixed (n1_10k)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
@@ -434,7 +629,8 @@ def main() -> int:
     if _lib.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
-    out = run_n1(args) if args.config == "n1_10k" else run_timeseries(args)
+    run = {"n1_10k": run_n1, "multi48": run_multi48, "serve64": run_serve64}.get(args.config, run_timeseries)
+    out = run(args)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -153,6 +153,9 @@ int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_
 /* Run the whole Newton solve of every case on the device from the uploaded inputs. */
 int p3smi_batch_solve(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_search, int32_t precision);
 int p3smi_batch_download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* converged, double* mismatch);
+/* Free the device workspace the one-call entry points keep between calls (it is
+ * allocated again, at the size needed, by the next one-call solve). */
+int p3smi_topology_release_cache(p3smi_topology* t);
 /* The resident batch the one-call entry points (solve_batch*, outages_solve) of this
  * topology ran on last: owned by the topology, valid until it is freed or a one-call solve
  * of another case count replaces it. For timing (batch_set_timing) and statistics. */

@@ -1761,6 +1761,14 @@ int p3smi_batch_download(p3smi_batch* b, double* V_out, int32_t* iterations, uin
     return guarded([&] { download(b, V_out, iterations, converged, mismatch); });
 }
 
+int p3smi_topology_release_cache(p3smi_topology* t) {
+    return guarded([&] {
+        if (!t) throw Error(P3SMI_E_ARG, "topology_release_cache: null topology");
+        std::lock_guard<std::mutex> lk(t->mu);
+        t->cached.reset();
+    });
+}
+
 int p3smi_topology_last_batch(p3smi_topology* t, p3smi_batch** out) {
     if (!t || !out) {
         g_err = "topology_last_batch: null argument";

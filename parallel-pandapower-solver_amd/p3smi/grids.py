@@ -251,3 +251,24 @@ CONFIGS = {
 def config_grid(name: str, seed: int = 0) -> Grid:
     n, ch, npv, _, ps = CONFIGS[name]
     return make_grid(n, ch, npv, seed=seed, power_scale=ps)
+
+
+def size_scale(n_bus: int) -> float:
+    """Injection scale of a grid of any size: 0.007 up to 10,000 buses, then falling as
+    1/sqrt(n) (the total load grows with n faster than the grid's transfer capacity; at 50,000
+    buses a flat 0.007 lands some cases on a voltage-collapse branch)."""
+    return 0.007 * min(1.0, (10000.0 / max(int(n_bus), 1)) ** 0.5)
+
+
+def sized_grid(n_bus: int, seed: int, pv_frac: float = 0.09, chord_frac: float = 0.3) -> Grid:
+    """A grid of the multi-grid workloads: tree + chord_frac * n chords, pv_frac * n PV buses."""
+    n = int(n_bus)
+    return make_grid(n, int(chord_frac * n), max(1, int(round(pv_frac * n))), seed=seed, power_scale=size_scale(n))
+
+
+def grid_family(count: int, lo: int, hi: int, seed: int) -> list:
+    """`count` grids with sizes log-uniform in [lo, hi] buses (seeded), PV share U(8%, 10%)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).round().astype(int)
+    pv = rng.uniform(0.08, 0.10, count)
+    return [sized_grid(int(n), seed=int(seed * 1000 + k), pv_frac=float(f)) for k, (n, f) in enumerate(zip(sizes, pv))]

@@ -9,12 +9,20 @@ scattered into their requests, and the slots refilled. A request completes when 
 cases have. Each pool is driven by its own host thread on its own HIP stream, so passes of
 different topologies overlap on the device.
 
-Admission is by device memory: a pool's slot count is its share of the memory budget
-(``mem_fraction`` of the free HBM at start) divided by the footprint of one case; a new
-topology that does not fit evicts idle pools first and otherwise waits.
+Admission is by device memory: a pool gets as many slots as its first request has cases
+(a multiple of 64, at most ``max_slots``), within what is left of the memory budget
+(``mem_fraction`` of the free HBM at start) at the footprint of one case. A topology that
+does not fit evicts idle pools first and otherwise waits; an idle pool too small for a new
+request is rebuilt larger.
 
-No arithmetic mixes slots, so each case's result is bit-identical to solving it alone.
-Reported per request: submit-to-completion latency (p50/p99 over requests in ``stats``).
+N-1 requests (``submit_contingency``: a subset of branch-outage cases of a topology) run on
+that topology's resident outage set (islands, per-case Ybus and Newton on the device), on a
+solver handle and stream of their own with one worker thread per topology, so they overlap
+with the pools' passes and with the N-1 work of other topologies.
+
+No arithmetic mixes slots, and every LU launch shape sums in one order, so each case's
+result is bit-identical to solving it alone. Reported per request: submit-to-completion
+latency (p50/p99 over requests in ``stats``).
 """
 
 from __future__ import annotations
@@ -47,14 +55,25 @@ def case_footprint(info: dict) -> int:
     return int(8 * (6 * n + nnzj + nnzlu + 2 * m) + 16 * 2 * n + 8 * ((n + 63) // 64) + 128)
 
 
+def _round64(k: int) -> int:
+    return (int(k) + 63) // 64 * 64
+
+
 class Request:
-    def __init__(self, key, S, V0):
+    def __init__(self, key, S, V0, cases=None):
         self.key = key
-        self.S = np.ascontiguousarray(S, dtype=np.complex128)
-        self.B = int(self.S.shape[1])
+        if cases is None:  # time series: S (n, B), V0 (n,) or (n, B)
+            self.S = np.ascontiguousarray(S, dtype=np.complex128)
+            self.B = int(self.S.shape[1])
+            V0 = np.ascontiguousarray(V0, dtype=np.complex128)
+            self.V0 = np.repeat(V0[:, None], self.B, axis=1) if V0.ndim == 1 else V0
+        else:  # N-1 subset: S (n,), V0 (n,) start, outage case indices
+            self.S = np.ascontiguousarray(S, dtype=np.complex128).ravel()
+            self.V0 = np.ascontiguousarray(V0, dtype=np.complex128).ravel()
+            self.cases = np.ascontiguousarray(cases, dtype=np.int64).ravel()
+            self.B = int(self.cases.size)
         n = self.S.shape[0]
-        V0 = np.ascontiguousarray(V0, dtype=np.complex128)
-        self.V0 = np.repeat(V0[:, None], self.B, axis=1) if V0.ndim == 1 else V0
+        self.served = None
         self.V = np.empty((n, self.B), dtype=np.complex128)
         self.iterations = np.empty(self.B, dtype=np.int32)
         self.converged = np.empty(self.B, dtype=bool)
@@ -77,7 +96,10 @@ class Request:
 
     def result(self) -> dict:
         self.wait()
-        return {"V": self.V, "iterations": self.iterations, "converged": self.converged, "mismatch": self.mismatch}
+        out = {"V": self.V, "iterations": self.iterations, "converged": self.converged, "mismatch": self.mismatch}
+        if self.served is not None:
+            out["served"] = self.served
+        return out
 
 
 class _Pool:
@@ -201,6 +223,7 @@ class Server:
 
     def __init__(self, tol: float = 1e-8, max_iter: int = 20, precision: str = "double", line_search: bool = True,
                  mem_fraction: float = 0.6, max_slots: int = 4096, mem_budget: int | None = None):
+        self.precision = precision
         self.tol = float(tol)
         self.max_iter = int(max_iter)
         self.prec = _prec(precision)
@@ -220,46 +243,120 @@ class Server:
         self.lock = threading.Lock()
         self.finished: list = []
         self.pending: deque = deque()  # requests waiting for memory
+        self.outage_sets: dict = {}  # topology key -> resident N-1 outage set
+        self.n1_queues: dict = {}  # topology key -> deque of N-1 requests
+        self.n1_threads: dict = {}
+        self.n1_cv = threading.Condition()
+        self.n1_stop = False
 
     # ------------------------------------------------------------------ API
     def submit(self, Yp, Yj, Yx, pv, pq, S, V0) -> Request:
         key = topology_key(Yp, Yj, Yx, pv, pq)
         req = Request(key, S, V0)
-        with self.lock:
-            pool = self.pools.get(key)
-            if pool is None:
-                pool = self._make_pool(key, Yp, Yj, Yx, pv, pq)
-            if pool is None:
-                self.pending.append((req, (Yp, Yj, Yx, pv, pq)))
-                return req
-            self.pools.move_to_end(key)
-        pool.enqueue(req)
+        self._route(req, (Yp, Yj, Yx, pv, pq))
         return req
 
-    def _make_pool(self, key, Yp, Yj, Yx, pv, pq):
+    def _route(self, req, topo):
+        want = min(self.max_slots, _round64(req.B))
+        with self.lock:
+            pool = self.pools.get(req.key)
+            if pool is not None and pool.capacity < want and pool.idle():
+                self._evict(req.key)  # rebuilt below with room for this request
+                pool = None
+            if pool is None:
+                pool = self._make_pool(req.key, *topo, want)
+            if pool is None:
+                self.pending.append((req, topo))
+                return
+            self.pools.move_to_end(req.key)
+        pool.enqueue(req)
+
+    def _solver(self, key, Yp, Yj, Yx, pv, pq):
         solver = self.solvers.get(key)
         if solver is None:
             solver = Solver(Yp, Yj, Yx, pv, pq)
-            from . import autotune
-
-            autotune.apply_cached(solver, self.max_slots)
             self.solvers[key] = solver
+        return solver
+
+    def _make_pool(self, key, Yp, Yj, Yx, pv, pq, want: int):
+        solver = self._solver(key, Yp, Yj, Yx, pv, pq)
         per = case_footprint(solver.info)
-        want = self.max_slots * per
-        while self.used + min(want, 64 * per) > self.budget:
+        while self.used + min(want, 64) * per > self.budget:
             victim = next((k for k, p in self.pools.items() if p.idle()), None)
             if victim is None:
                 return None
             self._evict(victim)
-        slots = int(min(self.max_slots, (self.budget - self.used) // per))
-        slots = max(64, slots // 64 * 64) if slots >= 64 else slots
+        slots = int(min(want, (self.budget - self.used) // per))
+        slots = slots // 64 * 64 if slots >= 64 else slots
         if slots <= 0:
             return None
+        from . import autotune
+
+        autotune.apply_cached(solver, slots)
         pool = _Pool(self, key, solver, slots)
         pool.bytes = slots * per
         self.used += pool.bytes
         self.pools[key] = pool
         return pool
+
+    # ------------------------------------------------------------------ N-1 requests
+    def register_outages(self, Yp, Yj, Yx, pv, pq, f, t, stamps, groups, ref, gen_z=None, reslack=False) -> str:
+        """Make the outage set of a topology resident: branches f -> t with stamps (yff, yft,
+        ytf, ytt), outage groups (branch index lists), slack buses, generator |Z| per bus.
+        Returns the topology key that submit_contingency takes."""
+        from .contingency import ContingencySet
+
+        key = topology_key(Yp, Yj, Yx, pv, pq)
+        with self.lock:
+            if key not in self.outage_sets:
+                own = Solver(Yp, Yj, Yx, pv, pq)  # own handle and stream: runs beside the pool
+                self.outage_sets[key] = ContingencySet(own, f, t, *stamps, groups, ref, gen_z, reslack)
+                self.n1_queues[key] = deque()
+                th = threading.Thread(target=self._n1_run, args=(key,), daemon=True)
+                self.n1_threads[key] = th
+                th.start()
+        return key
+
+    def submit_contingency(self, key: str, cases, S, V_start) -> Request:
+        """Solve the outage cases `cases` (indices into the registered groups) of a topology."""
+        if key not in self.outage_sets:
+            raise KeyError("submit_contingency: register_outages first")
+        req = Request(key, S, V_start, cases=cases)
+        with self.n1_cv:
+            self.n1_queues[key].append(req)
+            self.n1_cv.notify_all()
+        return req
+
+    def _n1_run(self, key):
+        from .contingency import ContingencySet
+
+        queue = self.n1_queues[key]
+        while True:
+            with self.n1_cv:
+                while not self.n1_stop and not queue:
+                    self.n1_cv.wait(0.05)
+                if self.n1_stop:
+                    return
+                req = queue.popleft()
+            full = self.outage_sets[req.key]
+            try:
+                sub = ContingencySet(full.solver, full.f, full.t, *full.yval.T, [full.groups[c] for c in req.cases],
+                                     full.ref, full.gen_z, full.reslack)
+                r = sub.solve(req.S, req.V0, max_iter=self.max_iter, tol=self.tol, line_search=self.line_search,
+                              precision=self.precision)
+                req.V[:] = r.V
+                req.iterations[:] = r.iterations
+                req.converged[:] = r.converged
+                req.mismatch[:] = r.mismatch
+                req.served = r.served
+            except Exception as e:  # a bad request fails alone
+                req.error = e
+            # the solve's workspace is not in the memory budget: give it back at once
+            _lib.lib().p3smi_topology_release_cache(full.solver._h)
+            req.remaining = 0
+            req.t_done = time.perf_counter()
+            req._done.set()
+            self._finished(req)
 
     def _evict(self, key):
         pool = self.pools.pop(key)
@@ -272,12 +369,7 @@ class Server:
             waiting = list(self.pending)
             self.pending.clear()
         for req2, topo in waiting:  # retry admission of requests that waited for memory
-            with self.lock:
-                pool = self.pools.get(req2.key) or self._make_pool(req2.key, *topo)
-                if pool is None:
-                    self.pending.append((req2, topo))
-                    continue
-            pool.enqueue(req2)
+            self._route(req2, topo)
 
     def wait_all(self, requests, timeout=None):
         t_end = None if timeout is None else time.perf_counter() + timeout
@@ -296,7 +388,14 @@ class Server:
                 "pools": len(self.pools), "mem_used": self.used, "mem_budget": self.budget}
 
     def close(self):
+        with self.n1_cv:
+            self.n1_stop = True
+            self.n1_cv.notify_all()
+        for th in self.n1_threads.values():
+            th.join()
+        self.n1_threads.clear()
         with self.lock:
             keys = list(self.pools)
         for k in keys:
             self._evict(k)
+        self.outage_sets.clear()

@@ -53,3 +53,41 @@ def test_served_results_equal_alone(gpu):
         np.testing.assert_array_equal(res["converged"], alone["converged"])
         np.testing.assert_array_equal(res["iterations"], alone["iterations"])
         np.testing.assert_array_equal(res["V"], alone["V"])
+
+
+def test_n1_requests_and_pool_growth(gpu):
+    """N-1 subsets served beside time-series requests equal the same subsets solved alone;
+    a pool too small for a later request is rebuilt with room for it."""
+    from p3smi import contingency
+
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    V0 = g.flat_start()
+    S = grids.timeseries_scaled(g, 400, seed=6)
+    srv = Server(max_slots=512, mem_budget=2 << 30)
+    try:
+        key = srv.register_outages(Yp, Yj, Yx, g.pv, g.pq, g.f, g.t, g.branch_stamps(),
+                                   contingency.single_branch_outages(g.nbr), g.ref, g.gen_z, True)
+        small = srv.submit(Yp, Yj, Yx, g.pv, g.pq, S[:, :10], V0)
+        small.wait(60)
+        cap0 = srv.pools[key].capacity
+        subsets = [np.arange(0, 40), np.arange(100, 103), np.arange(150, g.nbr)]
+        n1 = [srv.submit_contingency(key, c, g.sbus(), V0) for c in subsets]
+        big = srv.submit(Yp, Yj, Yx, g.pv, g.pq, S[:, 10:400], V0)
+        srv.wait_all(n1 + [big], timeout=120)
+        cap1 = srv.pools[key].capacity
+    finally:
+        srv.close()
+    assert cap0 == 64 and cap1 == 448, (cap0, cap1)
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    for req, cols in ((small, slice(0, 10)), (big, slice(10, 400))):
+        alone = s.solve_batch(S[:, cols], V0, max_iter=20, tol=1e-8)
+        np.testing.assert_array_equal(req.result()["V"], alone["V"])
+        np.testing.assert_array_equal(req.result()["iterations"], alone["iterations"])
+    for req, c in zip(n1, subsets):
+        cs = contingency.from_grid(g, groups=[np.array([k], np.int32) for k in c], reslack=True)
+        alone = cs.solve(g.sbus(), V0, max_iter=20, tol=1e-8)
+        res = req.result()
+        np.testing.assert_array_equal(res["served"], alone.served)
+        np.testing.assert_array_equal(res["iterations"], alone.iterations)
+        np.testing.assert_array_equal(res["V"].view(np.float64), alone.V.view(np.float64))
