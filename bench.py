@@ -239,6 +239,16 @@ def run_timeseries(args) -> dict:
         add_stats(acc, L, bh)
         log(f"[bench] step {k + 1}/{args.steps}: {1e3 * times[-1]:.1f} ms")
     t_total = time.perf_counter() - t_all0
+    # the same steps with each Newton iteration replayed as a captured hipGraph (timers off)
+    _lib.check(L.p3smi_batch_set_timing(bh, 0))
+    _lib.check(L.p3smi_batch_set_graphs(bh, 1))
+    solve()
+    tg0 = time.perf_counter()
+    for k in range(args.steps):
+        solve()
+    t_graph = time.perf_counter() - tg0
+    _lib.check(L.p3smi_batch_set_graphs(bh, 0))
+    log(f"[bench] graph replay: {1e3 * t_graph / args.steps:.1f} ms per step")
 
     V = np.empty((g.n, T), dtype=np.complex128)
     it = np.empty(T, dtype=np.int32)
@@ -254,6 +264,8 @@ def run_timeseries(args) -> dict:
     out = base_out(args, n_conv * args.steps / t_total, times, t_total, args.precision, config)
     out.update({"converged": n_conv, "iterations_hist": np.bincount(it, minlength=1).tolist(),
                 "newton_passes_per_step": acc["newton_passes"] / args.steps,
+                "graph_replay": {"ms_per_step": round(1e3 * t_graph / args.steps, 3),
+                                 "value": round(n_conv * args.steps / t_graph, 1)} if prec == 0 else None,
                 "roofline": roofline(acc, bpc, args.steps, args.config, args.precision)})
     if not args.no_cpu:
         cpu_timeseries(args, g, S_c, V0_c, V, it, conv, out)

@@ -165,6 +165,12 @@ int p3smi_topology_last_batch(p3smi_topology* t, p3smi_batch** out);
  * (p3s/NewtonPowerflowCpp.py:553-584), computed on the device. */
 int p3smi_batch_vrange(p3smi_batch* b, double* vmin, double* vmax);
 int p3smi_batch_stats(const p3smi_batch* b, p3smi_stats* out);
+/* 1 = replay each FP64 Newton iteration as a captured hipGraph (one launch and one
+ * synchronisation per phase instead of one launch per kernel); used while kernel timing
+ * is off. No counterpart in the reference (its GPU path launches kernel by kernel). */
+int p3smi_batch_set_graphs(p3smi_batch* b, int32_t on);
+/* The same for the one-call entry points of a topology (solve_batch*, outages_solve). */
+int p3smi_topology_set_graphs(p3smi_topology* t, int32_t on);
 /* 1 = record HIP events around every kernel of later solves (per-kernel times in stats). */
 int p3smi_batch_set_timing(p3smi_batch* b, int32_t on);
 /* Algorithmic bytes of one pass of each kernel for one case (eval, lu, step, decide). */

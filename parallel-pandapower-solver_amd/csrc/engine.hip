@@ -418,13 +418,16 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
 // loaded one task ahead and added at its end. The memory-level parallelism of a wavefront
 // is thus 2K operand pairs whatever the task lengths, where the task-at-a-time loop of
 // k_lu waits for every task's first loads.
-constexpr int LU_K = 8;
-
+// Chunk length: 8 pairs when an operand is FP64, 16 when both are FP32 (the same registers
+// and the same bytes in flight per wavefront).
 template <typename TA, typename TB>
+constexpr int lu_chunk() { return sizeof(TA) + sizeof(TB) <= 8 ? 16 : 8; }
+
+template <int K, typename TA, typename TB>
 __device__ __forceinline__ void load_chunk(const int2* __restrict__ st, int e, int eend, const TA* __restrict__ A,
-                                           const TB* __restrict__ Bv, int S, int c, TA (&a)[LU_K], TB (&b)[LU_K]) {
+                                           const TB* __restrict__ Bv, int S, int c, TA (&a)[K], TB (&b)[K]) {
 #pragma unroll
-    for (int u = 0; u < LU_K; ++u) {
+    for (int u = 0; u < K; ++u) {
         const int2 p = st[min(e + u, eend - 1)];  // past the end: repeat the last pair (unused)
         a[u] = A[at(p.x, S, c)];
         b[u] = Bv[at(p.y, S, c)];
@@ -463,22 +466,23 @@ __device__ __forceinline__ void seg_stream(const int4* __restrict__ tasks, int q
             }
         }
     };
+    constexpr int K = lu_chunk<TA, TB>();
     int e = tk.z;
-    TA a[LU_K];
-    TB b[LU_K];
-    if (e < eend) load_chunk(st, e, eend, A, Bv, S, c, a, b);
-    for (; e < eend; e += LU_K) {
-        TA a2[LU_K];
-        TB b2[LU_K];
-        if (e + LU_K < eend) load_chunk(st, e + LU_K, eend, A, Bv, S, c, a2, b2);
+    TA a[K];
+    TB b[K];
+    if (e < eend) load_chunk<K>(st, e, eend, A, Bv, S, c, a, b);
+    for (; e < eend; e += K) {
+        TA a2[K];
+        TB b2[K];
+        if (e + K < eend) load_chunk<K>(st, e + K, eend, A, Bv, S, c, a2, b2);
 #pragma unroll
-        for (int u = 0; u < LU_K; ++u) {
+        for (int u = 0; u < K; ++u) {
             const int pos = e + u;
             while (q < q1 && pos == tk.w) advance();
             if (pos < eend) acc = fma(-(ACC)a[u], (ACC)b[u], acc);
         }
 #pragma unroll
-        for (int u = 0; u < LU_K; ++u) {
+        for (int u = 0; u < K; ++u) {
             a[u] = a2[u];
             b[u] = b2[u];
         }
@@ -1072,6 +1076,7 @@ struct p3smi_topology {
     LuConfig lu_batch{64, 16, 0};  // LU launch shape while many cases are active
     LuConfig lu_few{1, 16, 0};     // ... and for the last few
     int lu_few_max = 256;         // "few": at most this many active cases
+    bool graphs = false;          // one-call solves replay captured Newton iterations
     double mixed_rtol = 1e-12;    // refinement stops once |dx correction| <= rtol |dx|
     int mixed_kmax = 6;           // refinement steps before the FP64 fallback
     hipStream_t stream = nullptr;
@@ -1105,10 +1110,20 @@ struct p3smi_batch {
     DevBuf<unsigned char> st_cv;
     DevBuf<double> st_mis;
     bool timing = false;
+    bool graphs = false;                // replay captured Newton iterations (FP64, timers off)
+    std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gcache;
     p3smi_stats stats{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
     size_t ev_used = 0;
+    void drop_graphs() {
+        for (auto& g : gcache) {
+            (void)hipGraphExecDestroy(g.second.second);
+            (void)hipGraphDestroy(g.second.first);
+        }
+        gcache.clear();
+    }
     ~p3smi_batch() {
+        drop_graphs();
         if (h_counts) (void)hipHostFree(h_counts);
         for (auto& e : ev) {
             (void)hipEventDestroy(e.second.first);
@@ -1469,12 +1484,97 @@ const int* start_new(p3smi_batch* b, double tol, int line_search, int precision,
     return launch_decide(b, tol, line_search, 0, 0, max_iter);
 }
 
+// ---- hipGraph replay of Newton iterations ---------------------------------------------
+// The kernels of one iteration depend only on device state (status words steer them), so an
+// iteration is recorded once per (LU launch shape, solve parameters, buffers) and replayed
+// with one graph launch and one synchronisation per phase: [LU, full step, evaluation,
+// decide, counts to the host], then [trial step, evaluation, decide, counts] while cases
+// backtrack, then [step, evaluation with Jacobian, decide, counts] at accepted trial points.
+// The status counters are cleared by a kernel (no memset node is recorded); schedules are
+// uploaded before recording. Kernel timers are not recorded: graphs run with timing off.
+void record_decide(p3smi_batch* b, double tol, int ls, int phase, int max_iter) {
+    hipStream_t st = b->topo->stream;
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->counts.p, NSTAT);
+    hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->part.p, b->nparts,
+                       b->nrm.p, b->alpha.p, b->best_alpha.p, b->best_nrm.p, b->trial.p, b->status.p, b->iters.p,
+                       b->counts.p, tol, ls, phase, 0, max_iter, b->S);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
+}
+
+template <typename F>
+const int* replay(p3smi_batch* b, const std::string& key, F&& record) {
+    hipStream_t st = b->topo->stream;
+    auto it = b->gcache.find(key);
+    if (it == b->gcache.end()) {
+        if (b->gcache.size() >= 32) b->drop_graphs();  // inputs re-allocated many times: start over
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            record();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        HIPCHECK(hipStreamEndCapture(st, &g));
+        if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            throw Error(P3SMI_E_DEVICE, "hipGraphInstantiate failed");
+        }
+        it = b->gcache.emplace(key, std::make_pair(g, x)).first;
+    }
+    HIPCHECK(hipGraphLaunch(it->second.second, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return b->h_counts;
+}
+
+const int* newton_iteration_graph(p3smi_batch* b, int n_active, double tol, int line_search, int max_iter) {
+    p3smi_topology* t = b->topo;
+    const int A = 1 << ST_ACTIVE, BT = 1 << ST_BACKTRACK, NJ = 1 << ST_NEEDJ;
+    const LuConfig cfg = n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
+    (void)get_sched(t, cfg);  // upload before recording
+    char key[320];
+    std::snprintf(key, sizeof key, "%d/%d/%d/%a/%d/%p/%d/%p/%p/%p/%d/%d", cfg.key(), cfg.G, cfg.NS, tol, max_iter,
+                  (const void*)b->Ssp.p, b->s_stride, (const void*)b->Yg.p, (const void*)b->Yb.p,
+                  (const void*)b->pin.p, (int)b->per_case_y, (int)b->has_pin);
+    const std::string base(key);
+    b->stats.newton_passes++;
+    b->stats.double_factorizations += n_active;
+    const int* cnt = replay(b, "it" + base + "/" + std::to_string(line_search), [&] {
+        launch_lu<double, true>(b, 0, n_active, b->fac.p, b->R.p, b->x.p, SEL_ALL);
+        launch_step(b, A, 1);
+        launch_eval(b, A, A);
+        record_decide(b, tol, line_search, 1, max_iter);
+    });
+    while (cnt[ST_BACKTRACK] > 0) {
+        b->stats.line_search_passes++;
+        cnt = replay(b, "bt" + base, [&] {
+            launch_step(b, BT, 0);
+            launch_eval(b, BT, 0);
+            record_decide(b, tol, line_search, 2, max_iter);
+        });
+    }
+    if (cnt[ST_NEEDJ] > 0) {
+        cnt = replay(b, "nj" + base, [&] {
+            launch_step(b, NJ, 0);
+            launch_eval(b, NJ, NJ);
+            record_decide(b, tol, line_search, 3, max_iter);
+        });
+    }
+    return cnt;
+}
+
 // One Newton iteration of every ACTIVE case (p3s/cpp/nr_klu.cpp:595-708): linear solve,
 // full step scored by a full evaluation, backtracking passes while any case backtracks,
 // Jacobian rebuilt at accepted trial points.
 const int* newton_iteration(p3smi_batch* b, int n_active, double tol, int line_search, int precision, int max_iter,
                             int it) {
     const int A = 1 << ST_ACTIVE, BT = 1 << ST_BACKTRACK, NJ = 1 << ST_NEEDJ;
+    if (b->graphs && !b->timing && precision == P3SMI_PREC_DOUBLE)
+        return newton_iteration_graph(b, n_active, tol, line_search, max_iter);
     b->stats.newton_passes++;
     linear_solve(b, it, n_active, precision);
     launch_step(b, A, 1);
@@ -1593,6 +1693,8 @@ p3smi_batch* cached_batch(p3smi_topology* t, int cases) {
         t->cached.reset(new p3smi_batch());
         batch_alloc(t->cached.get(), t, cases);
     }
+    if (t->cached->graphs && !t->graphs) t->cached->drop_graphs();
+    t->cached->graphs = t->graphs;
     return t->cached.get();
 }
 
@@ -1801,6 +1903,24 @@ int p3smi_batch_vrange(p3smi_batch* b, double* vmin, double* vmax) {
 
 int p3smi_batch_stats(const p3smi_batch* b, p3smi_stats* out) {
     *out = b->stats;
+    return P3SMI_OK;
+}
+
+int p3smi_topology_set_graphs(p3smi_topology* t, int32_t on) {
+    return guarded([&] {
+        if (!t) throw Error(P3SMI_E_ARG, "topology_set_graphs: null topology");
+        std::lock_guard<std::mutex> lk(t->mu);
+        t->graphs = on != 0;
+    });
+}
+
+int p3smi_batch_set_graphs(p3smi_batch* b, int32_t on) {
+    if (!b) {
+        g_err = "batch_set_graphs: null batch";
+        return P3SMI_E_ARG;
+    }
+    b->graphs = on != 0;
+    if (!b->graphs) b->drop_graphs();
     return P3SMI_OK;
 }
 

@@ -82,6 +82,10 @@ class Solver:
                                                 int(group)))
         self.info = self._info()
 
+    def set_graphs(self, on: bool = True):
+        """Replay each FP64 Newton iteration of later solves as a captured hipGraph."""
+        check(lib().p3smi_topology_set_graphs(self._h, int(bool(on))))
+
     @property
     def lean_active(self) -> bool:
         """True: every factorization runs on the frozen static-pivot plan."""

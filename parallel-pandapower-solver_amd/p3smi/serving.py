@@ -113,6 +113,7 @@ class _Pool:
         self.bh = C.c_void_p()
         _lib.check(self.L.p3smi_batch_create(solver._h, slots, C.byref(self.bh)))
         _lib.check(self.L.p3smi_batch_slots_init(self.bh))
+        _lib.check(self.L.p3smi_batch_set_graphs(self.bh, 1))  # FP64 passes replay captured graphs
         self.free = deque(range(slots))
         self.owner = {}  # slot -> (request, column)
         self.queue = deque()  # (request, next column)
