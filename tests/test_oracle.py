@@ -222,3 +222,73 @@ def test_minimum_degree_is_a_permutation_and_reduces_fill():
         return int(lu.L.nnz + lu.U.nnz)
 
     assert fill(p) < fill(np.arange(n * n))
+
+
+@pytest.mark.parametrize("tag,max_iter,ls", [("ls", 20, True), ("ls_lim", 5, True), ("nols", 20, False),
+                                              ("nols_lim", 5, False)])
+def test_damped_newton_matches_independent_dense_restatement(tag, max_iter, ls):
+    """Golden vectors from a dense numpy Newton that restates the reference's damping rules
+    (tests/golden/make_golden.py:newton_armijo_dense, p3s/cpp/nr_klu.cpp:588-708) on 48
+    perturbed starts: Armijo backtracking (12 cases), the iteration limit (max_iter 5) and,
+    without line search, divergence. Flags and iteration counts are exact; voltages of the
+    converged cases and of the backtracking cases stopped at the limit agree to roundoff
+    (diverging full-step trajectories amplify roundoff and are not compared)."""
+    g = GOLD
+    Y = sp.csr_matrix(g["geo30_Y"])
+    Y.sort_indices()
+    o = OracleSolver(Y.indptr, Y.indices, Y.data, g["geo30_pv"], g["geo30_pq"])
+    r = o.solve_batch(g["damped_S"], g["damped_V0"], max_iter=max_iter, tol=1e-8, line_search=ls)
+    cv, it, V = g[f"damped_{tag}_cv"], g[f"damped_{tag}_it"], g[f"damped_{tag}_V"]
+    np.testing.assert_array_equal(r["converged"], cv)
+    np.testing.assert_array_equal(r["iterations"], it)
+    keep = cv if not ls else np.ones_like(cv)
+    assert np.abs(r["V"][:, keep] - V[:, keep]).max() < 1e-12
+
+
+def test_best_trial_commit_matches_independent_dense_restatement():
+    """Loads past the transfer limit: the line search finds no sufficient decrease and commits
+    its least-bad trial (p3s/cpp/nr_klu.cpp:688-697). After 6 iterations (best-trial commits
+    in most cases) the state equals the dense restatement's to roundoff; after 20, flags and
+    iteration counts (the limit) are exact."""
+    g = GOLD
+    Y = sp.csr_matrix(g["geo30_Y"])
+    Y.sort_indices()
+    o = OracleSolver(Y.indptr, Y.indices, Y.data, g["geo30_pv"], g["geo30_pq"])
+    r6 = o.solve_batch(g["overload_S"], g["geo30_V0"], max_iter=6, tol=1e-8, line_search=True)
+    np.testing.assert_array_equal(r6["converged"], g["overload6_cv"])
+    np.testing.assert_array_equal(r6["iterations"], g["overload6_it"])
+    assert np.abs(r6["V"] - g["overload6_V"]).max() < 1e-10
+    r20 = o.solve_batch(g["overload_S"], g["geo30_V0"], max_iter=20, tol=1e-8, line_search=True)
+    np.testing.assert_array_equal(r20["converged"], g["overload20_cv"])
+    np.testing.assert_array_equal(r20["iterations"], g["overload20_it"])
+
+
+def test_rerun_on_pivoting_factorization_equals_pivoting_solve():
+    """run_column (p3s/cpp/nr_klu.cpp:844-857): a column that does not converge on the frozen
+    pivot order is solved again from scratch on a pivoting factorization. So every
+    non-converged column must equal, bit for bit, the solve in which the frozen plan is
+    always rejected (every factorization pivoting from the first iteration)."""
+    om = oracle
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    rng = np.random.default_rng(7)
+    T = 128
+    k = rng.uniform(1.0, 2.5, T)
+    S = g.p_gen[:, None] * k[None, :] - (g.p_load + 1j * g.q_load)[:, None] * k[None, :]
+    V0 = np.repeat(g.flat_start()[:, None], T, axis=1)
+    V0[g.pq, :] *= rng.uniform(0.6, 1.4, T)[None, :]
+    o = OracleSolver(Yp, Yj, Yx, g.pv, g.pq)
+    try:
+        om.reset_counters()
+        r0 = o.solve_batch(S, V0, max_iter=20, tol=1e-8)
+        reruns = om.counters()["column_reruns"]
+        om.set_lean_guard(1e300)  # the frozen plan never passes: pivoting factorizations only
+        r1 = o.solve_batch(S, V0, max_iter=20, tol=1e-8)
+    finally:
+        om.set_lean_guard(1e-3)
+    bad = ~r0["converged"]
+    assert reruns == int(bad.sum()) > 0
+    np.testing.assert_array_equal(r1["converged"][bad], r0["converged"][bad])
+    np.testing.assert_array_equal(r1["iterations"][bad], r0["iterations"][bad])
+    np.testing.assert_array_equal(np.ascontiguousarray(r1["V"][:, bad]).view(np.float64),
+                                  np.ascontiguousarray(r0["V"][:, bad]).view(np.float64))
