@@ -267,6 +267,30 @@ def run_timeseries(args) -> dict:
                 "graph_replay": {"ms_per_step": round(1e3 * t_graph / args.steps, 3),
                                  "value": round(n_conv * args.steps / t_graph, 1)} if prec == 0 else None,
                 "roofline": roofline(acc, bpc, args.steps, args.config, args.precision)})
+    # the drop-in boundary takes host arrays: one solve_batch call with S and V0 from host
+    # memory and V back to it (PCIe transfers and layout changes included), for reference
+    solver.solve_batch(S_c, V0_c, max_iter=MAX_ITER, tol=TOL, precision=args.precision)
+    th = time.perf_counter()
+    rh = solver.solve_batch(S_c, V0_c, max_iter=MAX_ITER, tol=TOL, precision=args.precision)
+    t_host = time.perf_counter() - th
+    out["one_call_host_arrays"] = {"ms": round(1e3 * t_host, 3),
+                                   "value": round(int(rh["converged"].sum()) / t_host, 1)}
+    # the step before the path: injections of every case built on the device from element
+    # series (loads p, q at PQ buses, generators p at PV buses), written into the batch
+    from p3smi.timeseries import SbusBuilder
+
+    rng = np.random.default_rng(9)
+    pq, pv = g.pq, g.pv
+    series = [g.p_load[pq][:, None] * rng.uniform(0.7, 1.3, (pq.size, T)),
+              g.q_load[pq][:, None] * rng.uniform(0.7, 1.3, (pq.size, T)),
+              g.p_gen[pv][:, None] * rng.uniform(0.0, 1.0, (pv.size, T))]
+    sb = SbusBuilder(g.n, g.sbus(), [{"bus": pq, "coef": -np.ones(pq.size), "base": g.p_load[pq], "imag": False},
+                                     {"bus": pq, "coef": -np.ones(pq.size), "base": g.q_load[pq], "imag": True},
+                                     {"bus": pv, "coef": np.ones(pv.size), "base": g.p_gen[pv], "imag": False}])
+    sb.build(bh, series)
+    ts = time.perf_counter()
+    sb.build(bh, series)
+    out["sbus_build_device_ms"] = round(1e3 * (time.perf_counter() - ts), 3)
     if not args.no_cpu:
         cpu_timeseries(args, g, S_c, V0_c, V, it, conv, out)
     L.p3smi_batch_free(bh)

@@ -148,8 +148,19 @@ int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start
 /* ---- resident batches (inputs uploaded once, solved many times) ---------------------- */
 int p3smi_batch_create(p3smi_topology* t, int32_t cases, p3smi_batch** out);
 void p3smi_batch_free(p3smi_batch* b);
-/* Upload time-series inputs: S (n, cases) complex, V0 (n,) or (n, cases) complex. */
+/* Upload time-series inputs: S (n, cases) complex, V0 (n,) or (n, cases) complex. S = NULL
+ * keeps injections built on the device by p3smi_batch_build_sbus. */
 int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_t v0_per_case);
+/* The time-series injections built on the device, replacing the numba builder of
+ * build_sbus_matrix (p3s/timeseries.py:36-141): S[i, t] = s0[i] + sum_k coef[k] *
+ * (X[row[k], t] - base[k]) over the contributions k of bus i, [ptr[i], ptr[i+1]), in order,
+ * on the imaginary part where imag[k]. s0 (n) complex: base injections; X (nrows, cases)
+ * float64: the element series stacked; row/coef/base/imag (nk). The result is the batch's
+ * S; then p3smi_batch_upload with S = NULL sets V0 and keeps it. */
+int p3smi_batch_build_sbus(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk, const int32_t* row,
+                           const double* coef, const double* base, const uint8_t* imag, int32_t nrows, const double* X);
+/* S (n, cases) complex of the batch, as built or uploaded. */
+int p3smi_batch_download_sbus(p3smi_batch* b, double* S_out);
 /* Run the whole Newton solve of every case on the device from the uploaded inputs. */
 int p3smi_batch_solve(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_search, int32_t precision);
 int p3smi_batch_download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* converged, double* mismatch);

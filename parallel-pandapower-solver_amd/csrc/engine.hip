@@ -12,12 +12,18 @@
 // (Ybus column, factor position, operand pair) is wave-uniform and comes through the
 // scalar unit once per wavefront.
 //
-// KERNELS (one pass each; the host loop below sequences them):
-//   k_eval      mismatch R = Sspec - Scalc, its infinity norm, and (masked) the Jacobian
-//   k_lu        static-pivot LU of the Jacobian + forward/backward substitution, one
-//               workgroup per 64 cases, level-synchronous over the elimination tree
-//   k_step      V = V0 + alpha * dx (full Newton step or line-search trial)
-//   k_decide    convergence test and Armijo bookkeeping per case
+// KERNELS (one pass each; the host loop below sequences them, kernel by kernel or as
+// replays of captured hipGraphs):
+//   k_eval        mismatch R = Sspec - Scalc, its infinity norm, and (masked) the Jacobian
+//   k_lu_stream   static-pivot LU of the Jacobian + forward/backward substitution, a
+//                 workgroup per 64, 32 or 16 cases, level-synchronous over the elimination
+//                 tree, each task list walking a software-pipelined operand stream
+//   k_lu          the same with one case per workgroup (small batches, last few cases)
+//   k_step        V = V0 + alpha * dx (full Newton step or line-search trial)
+//   k_decide      convergence test and Armijo bookkeeping per case
+//   k_resid, k_refine_*   FP64 residual and update of the mixed-precision refinement
+//   k_islands, k_case_y   N-1: islands, references, pins, per-case Ybus
+//   k_vrange      min / max |V| per case (voltage-band acceptance)
 //
 // STATUS per case: NEW (loaded, start point not yet evaluated), ACTIVE (iterating),
 // CONVERGED, FAILED (non-finite mismatch or rejected pivots), BACKTRACK (line-search trial),
@@ -818,6 +824,33 @@ __global__ void k_finalize(const double* __restrict__ Vm, const double* __restri
 
 
 
+// Time-series injections on the device (p3s/timeseries.py:36-141): for bus i and case t,
+// S[i, t] = s0[i] + sum_k coef[k] * (X[row[k], t] - base[k]) over the contributions k of
+// bus i, [ptr[i], ptr[i + 1]), in their order, added to the imaginary part where imag[k].
+// X stacks the element series (rows, T). One thread per (bus, case): the case index runs
+// along the thread x dimension, so X rows and S rows are read and written in contiguous
+// lines. Products and sums are rounded one by one (no contraction), as the reference's
+// loop rounds them.
+__global__ void k_sbus(const double* __restrict__ s0, const int* __restrict__ ptr, const int* __restrict__ row,
+                       const double* __restrict__ coef, const double* __restrict__ base,
+                       const unsigned char* __restrict__ imag, const double* __restrict__ X, double* __restrict__ out,
+                       int n, int T) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    for (int i = blockIdx.y; i < n; i += gridDim.y) {
+        double re = s0[2 * i], im = s0[2 * i + 1];
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+            const double d = X[(size_t)row[k] * T + t] - base[k];
+            const double v = coef[k] * d;
+            if (imag[k]) im = im + v;
+            else re = re + v;
+        }
+        out[2 * ((size_t)i * T + t)] = re;
+        out[2 * ((size_t)i * T + t) + 1] = im;
+    }
+}
+
 // Voltage band of every case: min and max |V| over its buses, skipping buses a served mask
 // marks unserved. NewtonPowerflowCpp.calculate accepts a converged solve only if every
 // |V| lies in [0.5, 1.5] p.u. (p3s/NewtonPowerflowCpp.py:553-584). One thread per case,
@@ -1110,6 +1143,7 @@ struct p3smi_batch {
     DevBuf<unsigned char> st_cv;
     DevBuf<double> st_mis;
     bool timing = false;
+    bool sbus_built = false;            // Ssp holds injections built on the device (k_sbus)
     bool graphs = false;                // replay captured Newton iterations (FP64, timers off)
     std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gcache;
     p3smi_stats stats{};
@@ -1843,7 +1877,9 @@ int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_
         const p3smi::Topology& h = b->topo->host;
         hipStream_t st = b->topo->stream;
         check_v0(V0, h.n, v0_per_case ? b->B : 1, "solve_batch: ");
-        b->Ssp.upload(S, (size_t)2 * h.n * b->B, st);
+        if (S) b->Ssp.upload(S, (size_t)2 * h.n * b->B, st);
+        else if (!b->sbus_built) throw Error(P3SMI_E_ARG, "batch_upload: S is NULL and no Sbus was built on the device");
+        b->sbus_built = false;
         b->s_stride = b->B;
         b->V0.upload(V0, (size_t)2 * h.n * (v0_per_case ? b->B : 1), st);
         b->v0_stride = v0_per_case ? b->B : 0;
@@ -1861,6 +1897,49 @@ int p3smi_batch_solve(p3smi_batch* b, int32_t max_iter, double tol, int32_t line
 
 int p3smi_batch_download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* converged, double* mismatch) {
     return guarded([&] { download(b, V_out, iterations, converged, mismatch); });
+}
+
+int p3smi_batch_build_sbus(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk, const int32_t* row,
+                           const double* coef, const double* base, const uint8_t* imag, int32_t nrows, const double* X) {
+    return guarded([&] {
+        if (!b || !s0 || !ptr || (nk > 0 && (!row || !coef || !base || !imag)) || (nrows > 0 && !X))
+            throw Error(P3SMI_E_ARG, "build_sbus: null argument");
+        const p3smi::Topology& h = b->topo->host;
+        const int n = h.n, T = b->B;
+        if (ptr[0] != 0 || ptr[n] != nk) throw Error(P3SMI_E_ARG, "build_sbus: ptr must start at 0 and end at nk");
+        for (int i = 0; i < n; ++i)
+            if (ptr[i + 1] < ptr[i]) throw Error(P3SMI_E_ARG, "build_sbus: ptr is not monotone");
+        for (int k = 0; k < nk; ++k)
+            if (row[k] < 0 || row[k] >= nrows) throw Error(P3SMI_E_ARG, "build_sbus: row index out of range");
+        hipStream_t st = b->topo->stream;
+        DevBuf<double> ds0, dcoef, dbase, dX;
+        DevBuf<int> dptr, drow;
+        DevBuf<unsigned char> dimag;
+        ds0.upload(s0, 2 * (size_t)n, st);
+        dptr.upload(ptr, (size_t)n + 1, st);
+        drow.upload(row, (size_t)nk, st);
+        dcoef.upload(coef, (size_t)nk, st);
+        dbase.upload(base, (size_t)nk, st);
+        dimag.upload(imag, (size_t)nk, st);
+        dX.upload(X, (size_t)nrows * T, st);
+        b->Ssp.alloc((size_t)2 * n * T);
+        const dim3 grid((unsigned)((T + 255) / 256), (unsigned)std::min(n, 65535));
+        hipLaunchKernelGGL(k_sbus, grid, dim3(256), 0, st, ds0.p, dptr.p, drow.p, dcoef.p, dbase.p, dimag.p, dX.p,
+                           b->Ssp.p, n, T);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(st));
+        b->sbus_built = true;
+    });
+}
+
+int p3smi_batch_download_sbus(p3smi_batch* b, double* S_out) {
+    return guarded([&] {
+        const p3smi::Topology& h = b->topo->host;
+        if (!b->sbus_built && b->s_stride != b->B) throw Error(P3SMI_E_ARG, "download_sbus: the batch has no (n, B) Sbus");
+        HIPCHECK(hipMemcpyAsync(S_out, b->Ssp.p, (size_t)2 * h.n * b->B * sizeof(double), hipMemcpyDeviceToHost,
+                                b->topo->stream));
+        HIPCHECK(hipStreamSynchronize(b->topo->stream));
+    });
 }
 
 int p3smi_topology_release_cache(p3smi_topology* t) {

@@ -67,3 +67,66 @@ def usable_cpus() -> int:
     if aff is not None:
         return len(aff(0))
     return os.cpu_count() or 1
+
+
+class SbusBuilder:
+    """Per-bus, per-case injections from element time series, built on the device.
+
+    The reference's ``build_sbus_matrix`` (p3s/timeseries.py:36-141) starts from the base
+    injection s0 and adds, for every time-series row of a load / sgen / gen variable, the
+    delta coef * (value - base) to the element's bus (loads -1/sn, generation +1/sn, times
+    scaling and in-service; q rows to the imaginary part). Its contribution list is ordered
+    bus-major with the element order kept within a bus; this class forms the same list from
+    arrays and hands it, with the stacked series, to ``p3smi_batch_build_sbus``, which writes
+    S (n, T) straight into a resident batch (no host copy of the (n, T) matrix).
+
+    sources: list of dicts, one per series (element, variable): ``bus`` (rows,) int, ``coef``
+    (rows,), ``base`` (rows,), ``imag`` bool; rows with bus < 0 (unsupplied) are dropped.
+    """
+
+    def __init__(self, n_bus: int, s0, sources):
+        self.n = int(n_bus)
+        self.s0 = np.ascontiguousarray(s0, dtype=np.complex128)
+        if self.s0.shape != (self.n,):
+            raise ValueError("SbusBuilder: s0 must be (n_bus,)")
+        bus, row, coef, base, imag = [], [], [], [], []
+        self.offsets = []
+        off = 0
+        for src in sources:
+            b = np.asarray(src["bus"], dtype=np.int64)
+            keep = np.nonzero(b >= 0)[0]
+            if np.any(b[keep] >= self.n):
+                raise ValueError("SbusBuilder: element bus out of range")
+            bus.append(b[keep])
+            row.append(off + keep)
+            coef.append(np.asarray(src["coef"], dtype=np.float64)[keep])
+            base.append(np.asarray(src["base"], dtype=np.float64)[keep])
+            imag.append(np.full(keep.size, bool(src["imag"])))
+            self.offsets.append(off)
+            off += b.size
+        self.nrows = off
+        bus_a = np.concatenate(bus) if bus else np.zeros(0, np.int64)
+        order = np.argsort(bus_a, kind="stable")  # bus-major, element order kept within a bus
+        self.ptr = np.zeros(self.n + 1, dtype=np.int32)
+        self.ptr[1:] = np.cumsum(np.bincount(bus_a, minlength=self.n))
+        self.row = np.ascontiguousarray(np.concatenate(row)[order], dtype=np.int32) if bus else np.zeros(0, np.int32)
+        self.coef = np.ascontiguousarray(np.concatenate(coef)[order]) if bus else np.zeros(0)
+        self.base = np.ascontiguousarray(np.concatenate(base)[order]) if bus else np.zeros(0)
+        self.imag = np.ascontiguousarray(np.concatenate(imag)[order], dtype=np.uint8) if bus else np.zeros(0, np.uint8)
+
+    def stack(self, series) -> np.ndarray:
+        """The series (one (rows, T) array per source, in source order) as one (rows, T) array."""
+        X = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in series], axis=0))
+        if X.shape[0] != self.nrows:
+            raise ValueError("SbusBuilder: series rows do not match the sources")
+        return X
+
+    def build(self, batch_handle, series) -> None:
+        """S of the batch (n, cases) from the series, on the device."""
+        from . import _lib
+
+        X = self.stack(series)
+        L = _lib.lib()
+        _lib.check(L.p3smi_batch_build_sbus(batch_handle, _lib.ptr(self.s0), _lib.ptr(self.ptr), int(self.row.size),
+                                            _lib.ptr(self.row), _lib.ptr(self.coef), _lib.ptr(self.base),
+                                            _lib.ptr(self.imag), int(self.nrows), _lib.ptr(X)))
