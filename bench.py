@@ -356,11 +356,30 @@ def run_n1(args) -> dict:
     L.p3smi_batch_bytes_per_case(bh, _lib.ptr(bpc))
     _lib.check(L.p3smi_batch_set_timing(bh, 0))
 
+    # ratings for the post-processing screen: 125 % of each branch's loading in the intact
+    # grid (an outage set with one empty group, solved and post-processed on the device)
+    vn = np.full(g.nbr, 20.0)
+    base = contingency.from_grid(g, groups=[np.zeros(0, np.int32)], reslack=True, solver=solver)
+    base.solve(S, Vs, max_iter=MAX_ITER, tol=TOL)
+    bl = contingency.find_overloads(base, 1e300, vn, vn, np.ones(g.nbr), cap=0).max_loading_percent
+    rating = np.where(np.isfinite(bl) & (bl > 0), bl / 100.0 * 1.25, 1.0)
+    del base
     # untimed: the full result table of the main precision (V, served) for the checks
     V = np.empty((g.n, Lc), dtype=np.complex128)
     served = np.empty((g.n, Lc), dtype=np.uint8)
     solve(code[main], V, served)
     conv = cv.astype(bool)
+    # N-1 post-processing on the resident results: loadings of every (branch, case), the
+    # overload list (above 125 % of the intact loading) with each branch's worst case, and
+    # the voltage-band violations
+    tp = time.perf_counter()
+    rep = contingency.find_overloads(cs, 100.0, vn, vn, rating, cap=1 << 22)
+    t_over = time.perf_counter() - tp
+    tp = time.perf_counter()
+    vv = contingency.find_voltage_violations(cs, 0.95, 1.05, cap=1 << 22)
+    t_vv = time.perf_counter() - tp
+    log(f"[bench] post-processing: {len(rep)} overloads in {1e3 * t_over:.1f} ms, {len(vv)} voltage violations "
+        f"in {1e3 * t_vv:.1f} ms")
     config = {"workload": f"n1_10k: 10000-bus synthetic grid, every single-branch outage ({Lc} cases), re-slack, "
                           f"DC start, tol 1e-8, max_iter 20",
               "n_bus": g.n, "cases": Lc, "precision": main, "nnz_lu": int(info["nnz_lu"]),
@@ -378,6 +397,9 @@ def run_n1(args) -> dict:
                     "kernel_ms_per_step": {k: round(acc2[f"ms_{k}"] / args.steps, 3)
                                            for k in ("eval", "lu", "refine", "step", "decide", "other")}},
         "mixed_over_fp64": round((out["value"] / value2) if main == "mixed" else (value2 / out["value"]), 3),
+        "post_processing": {"branch_case_pairs": int(g.nbr) * int(Lc), "overloads": len(rep),
+                            "overload_screen_ms": round(1e3 * t_over, 3), "voltage_violations": len(vv),
+                            "voltage_screen_ms": round(1e3 * t_vv, 3)},
     })
     if not args.no_cpu:
         cpu_n1(args, g, cs, Vs, V, it_main, conv, served.astype(bool), out)

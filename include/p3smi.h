@@ -145,6 +145,36 @@ int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start
                         int32_t line_search, int32_t precision, double* V_out, int32_t* iterations,
                         uint8_t* converged, double* mismatch, uint8_t* served);
 
+/* ---- N-1 post-processing on the device, from the last p3smi_outages_solve of the set
+ * (its results stay resident when the set was solved in one chunk) ---------------------
+ * Branch results (compute_branch_loading + find_overloads, p3s/contingency/
+ * find_overloads.py:128-420): per branch k and case c, S_from = conj(yff V_f + yft V_t) V_f
+ * sn_mva, S_to likewise, I = |S| / (vn |V| sqrt 3) in kA, loading = max(I_from, I_to) /
+ * rating * 100 (rating = max_i_ka) for a line, max(I vn sqrt 3) / rating * 100 (rating =
+ * sn_mva) for a transformer (trafo[k] != 0); 0 on the case's outaged branches, NaN on
+ * branches touching an unserved bus and in non-converged cases (there the reactive powers
+ * are 0, as the reference's NaN assignment into complex arrays leaves them). Outputs: the (nbr, L)
+ * tables asked for in `table` (NULL members are skipped), per branch the largest loading
+ * over the cases and the first case reaching it (NaN / -1 when none is defined), and the
+ * (branch, case, loading, i_ka) of every loading above `threshold` percent: up to `cap`
+ * entries, in no particular order, *n_over = their total number. */
+typedef struct p3smi_branch_table {
+    double *p_from, *q_from, *p_to, *q_to; /* MW, Mvar */
+    double *i_from, *i_to, *i_ka;          /* kA; i_ka = max(i_from, i_to) */
+    double* loading;                       /* percent */
+} p3smi_branch_table;
+int p3smi_outages_branch_flows(p3smi_outages* o, const double* vn_from_kv, const double* vn_to_kv, const double* rating,
+                               const uint8_t* trafo, double sn_mva, double threshold, p3smi_branch_table* table,
+                               double* max_loading, int32_t* worst_case, int32_t cap, int32_t* ov_branch,
+                               int32_t* ov_case, double* ov_loading, double* ov_i_ka, int32_t* n_over);
+/* Voltage-band violations (find_voltage_violations, p3s/contingency/voltage_band.py:103-182):
+ * |V| < vm_min - tol (kind 0, "low") or > vm_max + tol (kind 1, "high") at served, in-service
+ * buses of converged cases; NaN bounds never flag. Up to `cap` entries (bus, case, |V|,
+ * kind), in no particular order; *n_viol = their total number. */
+int p3smi_outages_voltage_violations(p3smi_outages* o, const double* vm_min, const double* vm_max,
+                                     const uint8_t* in_service, double tol, int32_t cap, int32_t* bus, int32_t* case_,
+                                     double* vm, uint8_t* kind, int32_t* n_viol);
+
 /* ---- resident batches (inputs uploaded once, solved many times) ---------------------- */
 int p3smi_batch_create(p3smi_topology* t, int32_t cases, p3smi_batch** out);
 void p3smi_batch_free(p3smi_batch* b);

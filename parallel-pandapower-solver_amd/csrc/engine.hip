@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -73,6 +74,7 @@ struct Error : std::runtime_error {
     } while (0)
 
 thread_local std::string g_err;
+std::atomic<unsigned long long> g_solve_ids{0};
 
 // ===================================================================== device code
 __device__ __forceinline__ size_t at(int item, int S, int c) { return (size_t)item * (size_t)S + (size_t)c; }
@@ -871,6 +873,169 @@ __global__ void k_vrange(const double* __restrict__ Vm, const unsigned char* __r
     vmax[c] = hi;
 }
 
+// ===================================================================== N-1 post-processing
+// Branch results of every outage case from the resident solve state, as
+// compute_branch_loading does them on the (n, L) voltage table
+// (p3s/contingency/find_overloads.py:128-262): with V = Vm cos + j Vm sin at each end and
+// the intact branch stamps, S_from = conj(yff V_f + yft V_t) V_f sn, S_to = conj(ytf V_f +
+// ytt V_t) V_t sn; I = |S| / (vn |V| sqrt 3); loading = max(I_from, I_to) / rating * 100
+// for a line, max(|S_from|-equivalent, |S_to|-equivalent) / rating * 100 for a
+// transformer. A branch touching an unserved bus is NaN (its voltages are), a branch of the
+// case's own outage group is 0 (it carries nothing), a non-converged case is NaN (reactive
+// powers 0, as the reference's complex NaN assignment leaves them): in that order of
+// precedence, as the reference applies them. Products and sums are rounded one by one in
+// the reference's order. Thread per (case, branch), cases along x.
+__device__ __forceinline__ void cmul(double ar, double ai, double br, double bi, double& cr, double& ci) {
+#pragma clang fp contract(off)
+    cr = ar * br - ai * bi;
+    ci = ar * bi + ai * br;
+}
+
+__global__ void k_branch_flows(const int* __restrict__ fb, const int* __restrict__ tb, const double* __restrict__ yval,
+                               const int* __restrict__ gptr, const int* __restrict__ gbr, const double* __restrict__ Vm,
+                               const double* __restrict__ cs, const double* __restrict__ sn,
+                               const unsigned char* __restrict__ served, const int* __restrict__ status,
+                               const double* __restrict__ vnf, const double* __restrict__ vnt,
+                               const double* __restrict__ rating, const unsigned char* __restrict__ trafo, double sn_mva,
+                               int nbr, int L, int S, double* __restrict__ pf, double* __restrict__ qf,
+                               double* __restrict__ pt, double* __restrict__ qt, double* __restrict__ i_from,
+                               double* __restrict__ i_to, double* __restrict__ i_ka, double* __restrict__ loading) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= L) return;
+    const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    const double sqrt3 = 1.7320508075688772;  // the double nearest sqrt(3)
+    for (int k = blockIdx.y; k < nbr; k += gridDim.y) {
+        const int f = fb[k], t = tb[k];
+        const size_t of = at(f, S, c), ot = at(t, S, c);
+        double vfr = Vm[of] * cs[of], vfi = Vm[of] * sn[of];
+        double vtr = Vm[ot] * cs[ot], vti = Vm[ot] * sn[ot];
+        if (!served[of]) vfr = vfi = nan;
+        if (!served[ot]) vtr = vti = nan;
+        const double* y = yval + 8 * (size_t)k;  // ff, ft, tf, tt (re, im)
+        // Yf V and Yt V, terms in ascending bus order (the CSR order of the sparse product)
+        double a1r, a1i, a2r, a2i, b1r, b1i, b2r, b2i;
+        cmul(y[0], y[1], vfr, vfi, a1r, a1i);  // yff V_f
+        cmul(y[2], y[3], vtr, vti, a2r, a2i);  // yft V_t
+        cmul(y[4], y[5], vfr, vfi, b1r, b1i);  // ytf V_f
+        cmul(y[6], y[7], vtr, vti, b2r, b2i);  // ytt V_t
+        double ifr, ifi, itr, iti;
+        if (f < t) {
+            ifr = a1r + a2r; ifi = a1i + a2i;
+            itr = b1r + b2r; iti = b1i + b2i;
+        } else {
+            ifr = a2r + a1r; ifi = a2i + a1i;
+            itr = b2r + b1r; iti = b2i + b1i;
+        }
+        double sfr, sfi, str_, sti;
+        cmul(ifr, -ifi, vfr, vfi, sfr, sfi);  // conj(I_f) V_f
+        cmul(itr, -iti, vtr, vti, str_, sti);
+        sfr = sfr * sn_mva; sfi = sfi * sn_mva;
+        str_ = str_ * sn_mva; sti = sti * sn_mva;
+        const double vmf = hypot(vfr, vfi), vmt = hypot(vtr, vti);
+        double cif = hypot(sfr, sfi) / (vnf[k] * vmf * sqrt3);
+        double cit = hypot(str_, sti) / (vnt[k] * vmt * sqrt3);
+        double imax = (cif != cif || cit != cit) ? nan : fmax(cif, cit);
+        double ld;
+        if (trafo[k]) {
+            const double a = cif * vnf[k] * sqrt3, b = cit * vnt[k] * sqrt3;
+            ld = (a != a || b != b) ? nan : fmax(a, b);
+            ld = ld / rating[k] * 100.0;
+        } else {
+            ld = imax / rating[k] * 100.0;
+        }
+        bool out = false;
+        for (int q = gptr[c]; q < gptr[c + 1]; ++q) out = out || gbr[q] == k;
+        if (out) sfr = sfi = str_ = sti = cif = cit = imax = ld = 0.0;
+        if (status[c] != ST_CONVERGED) {
+            // the reference assigns NaN into its complex power arrays, which stores NaN + 0j:
+            // a non-converged case reports NaN active and 0 reactive power
+            // (p3s/contingency/find_overloads.py:249-252)
+            sfr = str_ = cif = cit = imax = ld = nan;
+            sfi = sti = 0.0;
+        }
+        const size_t o = (size_t)k * L + c;
+        if (pf) { pf[o] = sfr; qf[o] = sfi; pt[o] = str_; qt[o] = sti; }
+        if (i_from) { i_from[o] = cif; i_to[o] = cit; }
+        i_ka[o] = imax;
+        loading[o] = ld;
+    }
+}
+
+// Per branch: the largest loading over the cases and the first case reaching it (NaN and
+// -1 when every case is NaN), as nanmax / nanargmax do it; and every (branch, case) above
+// the threshold into a list (atomic append, order arbitrary: the host sorts it).
+__global__ __launch_bounds__(256) void k_branch_worst(const double* __restrict__ loading, int nbr, int L,
+                                                      double* __restrict__ best, int* __restrict__ arg) {
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    const int k = blockIdx.x;
+    double v = -DBL_MAX;
+    int idx = INT_MAX;
+    for (int c = threadIdx.x; c < L; c += blockDim.x) {
+        const double x = loading[(size_t)k * L + c];
+        if (x == x && (x > v || (x == v && c < idx))) { v = x; idx = c; }
+    }
+    sv[threadIdx.x] = v;
+    si[threadIdx.x] = idx;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double ov = sv[threadIdx.x + w];
+            const int oi = si[threadIdx.x + w];
+            if (ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                sv[threadIdx.x] = ov;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const bool none = si[0] == INT_MAX;
+        best[k] = none ? __longlong_as_double(0x7ff8000000000000ll) : sv[0];
+        arg[k] = none ? -1 : si[0];
+    }
+}
+
+__global__ void k_over_list(const double* __restrict__ loading, const double* __restrict__ i_ka, int nbr, int L,
+                            double threshold, int cap, int* __restrict__ count, int* __restrict__ ob,
+                            int* __restrict__ oc, double* __restrict__ ol, double* __restrict__ oi) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= L) return;
+    for (int k = blockIdx.y; k < nbr; k += gridDim.y) {
+        const size_t o = (size_t)k * L + c;
+        const double x = loading[o];
+        if (x > threshold) {  // NaN never compares true
+            const int j = atomicAdd(count, 1);
+            if (j < cap) { ob[j] = k; oc[j] = c; ol[j] = x; oi[j] = i_ka[o]; }
+        }
+    }
+}
+
+// Bus voltage-band violations (p3s/contingency/voltage_band.py:103-182): |V| below
+// vm_min - tol (kind 0) or above vm_max + tol (kind 1) at served, in-service buses of
+// converged cases; a NaN bound never flags.
+__global__ void k_vviol(const double* __restrict__ Vm, const double* __restrict__ cs, const double* __restrict__ sn,
+                        const unsigned char* __restrict__ served, const int* __restrict__ status,
+                        const double* __restrict__ vmin, const double* __restrict__ vmax,
+                        const unsigned char* __restrict__ in_service, double tol, int n, int L, int S, int cap,
+                        int* __restrict__ count, int* __restrict__ ob, int* __restrict__ oc, double* __restrict__ ov,
+                        unsigned char* __restrict__ okind) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= L || status[c] != ST_CONVERGED) return;
+    for (int i = blockIdx.y; i < n; i += gridDim.y) {
+        const size_t o = at(i, S, c);
+        if (!served[o] || !in_service[i]) continue;
+        const double vm = hypot(Vm[o] * cs[o], Vm[o] * sn[o]);
+        const bool low = vm < vmin[i] - tol, high = vm > vmax[i] + tol;
+        if (low || high) {
+            const int j = atomicAdd(count, 1);
+            if (j < cap) { ob[j] = i; oc[j] = c; ov[j] = vm; okind[j] = high ? 1 : 0; }
+        }
+    }
+}
+
 // ===================================================================== slots (serving)
 // Copy k staged cases (S and V0, (n, k) complex) into slot columns and mark them NEW.
 __global__ void k_load_slots(const double* __restrict__ Sst, const double* __restrict__ Vst,
@@ -1143,6 +1308,7 @@ struct p3smi_batch {
     DevBuf<unsigned char> st_cv;
     DevBuf<double> st_mis;
     bool timing = false;
+    unsigned long long solve_id = 0;    // stamp of the last run_newton on this batch
     bool sbus_built = false;            // Ssp holds injections built on the device (k_sbus)
     bool graphs = false;                // replay captured Newton iterations (FP64, timers off)
     std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gcache;
@@ -1174,6 +1340,9 @@ struct p3smi_outages {
     DevBuf<double> gz, yval;
     DevBuf<int> scratch, npinned;
     DevBuf<unsigned long long> zscratch;
+    // the batch and solve stamp holding this set's last results on the device (one chunk)
+    const p3smi_batch* res_batch = nullptr;
+    unsigned long long res_id = 0;
 };
 
 p3smi_topology::~p3smi_topology() {
@@ -1633,6 +1802,7 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
     check_solve_args(max_iter, tol, precision);
     hipStream_t st = b->topo->stream;
     std::memset(&b->stats, 0, sizeof b->stats);
+    b->solve_id = ++g_solve_ids;
     b->stats.cases = b->B;
     {
         KTimer tm(b, 4);
@@ -2022,10 +2192,11 @@ int p3smi_batch_bytes_per_case(const p3smi_batch* b, double* out4) {
     return P3SMI_OK;
 }
 
-static void solve_chunked(p3smi_topology* t, int T, bool per_case_y, bool pin_given, bool per_case_s,
-                          const std::function<void(p3smi_batch*, int, int)>& upload, int max_iter, double tol,
-                          int line_search, int precision, double* V_out, int32_t* iters, uint8_t* conv,
-                          double* mismatch) {
+// Returns the number of chunks the cases were solved in.
+static int solve_chunked(p3smi_topology* t, int T, bool per_case_y, bool pin_given, bool per_case_s,
+                         const std::function<void(p3smi_batch*, int, int)>& upload, int max_iter, double tol,
+                         int line_search, int precision, double* V_out, int32_t* iters, uint8_t* conv,
+                         double* mismatch) {
     const p3smi::Topology& h = t->host;
     ensure_device(t);
     const int cs = chunk_cases(t, T, per_case_y, pin_given, per_case_s);
@@ -2042,6 +2213,7 @@ static void solve_chunked(p3smi_topology* t, int T, bool per_case_y, bool pin_gi
             for (int i = 0; i < h.n; ++i)
                 std::memcpy(V_out + 2 * ((size_t)i * T + c0), Vc.data() + 2 * (size_t)i * nc, (size_t)nc * 16);
     }
+    return (T + cs - 1) / cs;
 }
 
 int p3smi_solve_batch(p3smi_topology* t, const double* S, const double* V0, int32_t v0_per_case, int32_t T,
@@ -2316,8 +2488,140 @@ int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start
                 HIPCHECK(hipMemcpy2DAsync(served + c0, L, b->served.p, b->S, nc, h.n, hipMemcpyDeviceToHost, st));
             }
         };
-        solve_chunked(t, L, true, true, false, up, max_iter, tol, line_search, precision, V_out, iterations, converged,
-                      mismatch);
+        o->res_batch = nullptr;
+        const int chunks = solve_chunked(t, L, true, true, false, up, max_iter, tol, line_search, precision, V_out,
+                                         iterations, converged, mismatch);
+        if (chunks == 1) {  // the whole set's results stay on the device for post-processing
+            o->res_batch = t->cached.get();
+            o->res_id = t->cached->solve_id;
+        }
+    });
+}
+
+const p3smi_batch* resident_results(const p3smi_outages* o) {
+    const p3smi_batch* b = o->res_batch;
+    if (!b || b != o->topo->cached.get() || b->solve_id != o->res_id)
+        throw Error(P3SMI_E_ARG, "outages: no resident results (solve the set in one chunk first; a later one-call "
+                                 "solve on the topology replaces them)");
+    return b;
+}
+
+int p3smi_outages_branch_flows(p3smi_outages* o, const double* vn_from_kv, const double* vn_to_kv, const double* rating,
+                               const uint8_t* trafo, double sn_mva, double threshold, p3smi_branch_table* table,
+                               double* max_loading, int32_t* worst_case, int32_t cap, int32_t* ov_branch,
+                               int32_t* ov_case, double* ov_loading, double* ov_i_ka, int32_t* n_over) {
+    return guarded([&] {
+        if (!o || !vn_from_kv || !vn_to_kv || !rating || !trafo) throw Error(P3SMI_E_ARG, "branch_flows: null argument");
+        if (cap < 0 || (cap > 0 && (!ov_branch || !ov_case || !ov_loading || !ov_i_ka)))
+            throw Error(P3SMI_E_ARG, "branch_flows: violation list needs its four outputs");
+        std::lock_guard<std::mutex> lk(o->topo->mu);
+        const p3smi_batch* b = resident_results(o);
+        hipStream_t st = o->topo->stream;
+        const int nbr = o->nbr, L = o->ncases;
+        const size_t NL = (size_t)nbr * L;
+        DevBuf<double> vnf, vnt, rat, ld, ik, pfb, qfb, ptb, qtb, ifb, itb, best;
+        DevBuf<unsigned char> tr;
+        DevBuf<int> arg, cnt, ob, oc;
+        DevBuf<double> ol, oi;
+        vnf.upload(vn_from_kv, nbr, st);
+        vnt.upload(vn_to_kv, nbr, st);
+        rat.upload(rating, nbr, st);
+        tr.upload(trafo, nbr, st);
+        ld.alloc(NL);
+        ik.alloc(NL);
+        const bool full = table && table->p_from && table->q_from && table->p_to && table->q_to;
+        const bool cur = table && table->i_from && table->i_to;
+        if (full) { pfb.alloc(NL); qfb.alloc(NL); ptb.alloc(NL); qtb.alloc(NL); }
+        if (cur) { ifb.alloc(NL); itb.alloc(NL); }
+        const dim3 grid((unsigned)((L + 255) / 256), (unsigned)std::max(1, std::min(nbr, 65535)));
+        if (nbr > 0)
+            hipLaunchKernelGGL(k_branch_flows, grid, dim3(256), 0, st, o->f.p, o->t.p, o->yval.p, o->gptr.p, o->gbr.p,
+                               b->Vm.p, b->cs.p, b->sn.p, b->served.p, b->status.p, vnf.p, vnt.p, rat.p, tr.p, sn_mva,
+                               nbr, L, b->S, full ? pfb.p : nullptr, full ? qfb.p : nullptr, full ? ptb.p : nullptr,
+                               full ? qtb.p : nullptr, cur ? ifb.p : nullptr, cur ? itb.p : nullptr, ik.p, ld.p);
+        HIPCHECK(hipGetLastError());
+        if (max_loading || worst_case) {
+            best.alloc(nbr);
+            arg.alloc(nbr);
+            if (nbr > 0)
+                hipLaunchKernelGGL(k_branch_worst, dim3((unsigned)nbr), dim3(256), 0, st, ld.p, nbr, L, best.p, arg.p);
+            HIPCHECK(hipGetLastError());
+            if (max_loading) HIPCHECK(hipMemcpyAsync(max_loading, best.p, nbr * sizeof(double), hipMemcpyDeviceToHost, st));
+            if (worst_case) HIPCHECK(hipMemcpyAsync(worst_case, arg.p, nbr * sizeof(int), hipMemcpyDeviceToHost, st));
+        }
+        if (n_over) {
+            cnt.alloc(1);
+            HIPCHECK(hipMemsetAsync(cnt.p, 0, sizeof(int), st));
+            ob.alloc(std::max(cap, 1)); oc.alloc(std::max(cap, 1)); ol.alloc(std::max(cap, 1)); oi.alloc(std::max(cap, 1));
+            if (nbr > 0)
+                hipLaunchKernelGGL(k_over_list, grid, dim3(256), 0, st, ld.p, ik.p, nbr, L, threshold, cap, cnt.p, ob.p,
+                                   oc.p, ol.p, oi.p);
+            HIPCHECK(hipGetLastError());
+            int total = 0;
+            HIPCHECK(hipMemcpyAsync(&total, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            const int k = std::min(total, cap);
+            if (k > 0) {
+                HIPCHECK(hipMemcpyAsync(ov_branch, ob.p, k * sizeof(int), hipMemcpyDeviceToHost, st));
+                HIPCHECK(hipMemcpyAsync(ov_case, oc.p, k * sizeof(int), hipMemcpyDeviceToHost, st));
+                HIPCHECK(hipMemcpyAsync(ov_loading, ol.p, k * sizeof(double), hipMemcpyDeviceToHost, st));
+                HIPCHECK(hipMemcpyAsync(ov_i_ka, oi.p, k * sizeof(double), hipMemcpyDeviceToHost, st));
+            }
+            *n_over = total;
+        }
+        if (table) {
+            auto down = [&](double* dst, const DevBuf<double>& src) {
+                if (dst && src.p) HIPCHECK(hipMemcpyAsync(dst, src.p, NL * sizeof(double), hipMemcpyDeviceToHost, st));
+            };
+            down(table->loading, ld);
+            down(table->i_ka, ik);
+            if (full) { down(table->p_from, pfb); down(table->q_from, qfb); down(table->p_to, ptb); down(table->q_to, qtb); }
+            if (cur) { down(table->i_from, ifb); down(table->i_to, itb); }
+        }
+        HIPCHECK(hipStreamSynchronize(st));
+    });
+}
+
+int p3smi_outages_voltage_violations(p3smi_outages* o, const double* vm_min, const double* vm_max,
+                                     const uint8_t* in_service, double tol, int32_t cap, int32_t* bus, int32_t* case_,
+                                     double* vm, uint8_t* kind, int32_t* n_viol) {
+    return guarded([&] {
+        if (!o || !vm_min || !vm_max || !in_service || !n_viol) throw Error(P3SMI_E_ARG, "voltage_violations: null argument");
+        if (!(tol >= 0.0) || !std::isfinite(tol)) throw Error(P3SMI_E_ARG, "voltage_violations: tol must be finite and >= 0");
+        if (cap < 0 || (cap > 0 && (!bus || !case_ || !vm || !kind)))
+            throw Error(P3SMI_E_ARG, "voltage_violations: the list needs its four outputs");
+        std::lock_guard<std::mutex> lk(o->topo->mu);
+        const p3smi::Topology& h = o->topo->host;
+        for (int i = 0; i < h.n; ++i)
+            if (vm_min[i] > vm_max[i]) throw Error(P3SMI_E_ARG, "voltage_violations: vm_min_pu > vm_max_pu at bus " + std::to_string(i));
+        const p3smi_batch* b = resident_results(o);
+        hipStream_t st = o->topo->stream;
+        const int L = o->ncases;
+        DevBuf<double> lo, hi, ov;
+        DevBuf<unsigned char> ins, ok;
+        DevBuf<int> cnt, ob, oc;
+        lo.upload(vm_min, h.n, st);
+        hi.upload(vm_max, h.n, st);
+        ins.upload(in_service, h.n, st);
+        cnt.alloc(1);
+        HIPCHECK(hipMemsetAsync(cnt.p, 0, sizeof(int), st));
+        ob.alloc(std::max(cap, 1)); oc.alloc(std::max(cap, 1)); ov.alloc(std::max(cap, 1)); ok.alloc(std::max(cap, 1));
+        const dim3 grid((unsigned)((L + 255) / 256), (unsigned)std::min(h.n, 65535));
+        hipLaunchKernelGGL(k_vviol, grid, dim3(256), 0, st, b->Vm.p, b->cs.p, b->sn.p, b->served.p, b->status.p, lo.p,
+                           hi.p, ins.p, tol, h.n, L, b->S, cap, cnt.p, ob.p, oc.p, ov.p, ok.p);
+        HIPCHECK(hipGetLastError());
+        int total = 0;
+        HIPCHECK(hipMemcpyAsync(&total, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        const int k = std::min(total, cap);
+        if (k > 0) {
+            HIPCHECK(hipMemcpyAsync(bus, ob.p, k * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipMemcpyAsync(case_, oc.p, k * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipMemcpyAsync(vm, ov.p, k * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipMemcpyAsync(kind, ok.p, k, hipMemcpyDeviceToHost, st));
+        }
+        HIPCHECK(hipStreamSynchronize(st));
+        *n_viol = total;
     });
 }
 

@@ -165,3 +165,164 @@ def solve_contingencies(grid, reslack: bool = False, tol: float = 1e-8, max_iter
     else:
         raise ValueError(f"init must be 'dc' or 'flat', got {init!r}")
     return cs.solve(grid.sbus(), v_start, max_iter=max_iter, tol=tol, precision=precision)
+
+
+# ---------------------------------------------------------------------- post-processing
+class _BranchTable(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in ("p_from", "q_from", "p_to", "q_to", "i_from", "i_to", "i_ka", "loading")]
+
+
+@dataclass
+class BranchLoadingTable:
+    """(n_branch, L) branch results (p3s/contingency/find_overloads.py:48-100). NaN: the case
+    did not converge or the branch touches an unserved bus; 0: the branch is the outage."""
+
+    groups: list
+    p_from_mw: np.ndarray
+    q_from_mvar: np.ndarray
+    p_to_mw: np.ndarray
+    q_to_mvar: np.ndarray
+    i_from_ka: np.ndarray
+    i_to_ka: np.ndarray
+    i_ka: np.ndarray
+    loading_percent: np.ndarray
+    max_loading_percent: np.ndarray  # (n_branch,) nanmax over the cases
+    worst_case: np.ndarray  # (n_branch,) first case of the maximum, -1 if all NaN
+
+
+@dataclass
+class OverloadReport:
+    """Every (branch, case) loaded above the threshold, worst first (ties: branch, case)."""
+
+    threshold_percent: float
+    branch: np.ndarray
+    case: np.ndarray
+    group: np.ndarray
+    loading_percent: np.ndarray
+    i_ka: np.ndarray
+    rating: np.ndarray
+    max_loading_percent: np.ndarray
+    worst_case: np.ndarray
+
+    def __len__(self) -> int:
+        return int(self.branch.size)
+
+
+@dataclass
+class VoltageViolationReport:
+    """Every (bus, case) outside its band, ordered as p3s/contingency/voltage_band.py:164-182."""
+
+    bus: np.ndarray
+    case: np.ndarray
+    group: np.ndarray
+    kind: np.ndarray  # "low" / "high"
+    vm_pu: np.ndarray
+    limit_pu: np.ndarray
+    deviation_pu: np.ndarray
+
+    def __len__(self) -> int:
+        return int(self.bus.size)
+
+
+def _per_bus(v, n, name):
+    a = np.asarray(v, dtype=np.float64)
+    if a.ndim == 0:
+        return np.full(n, float(a))
+    if a.shape != (n,):
+        raise ValueError(f"{name} must be a scalar or have shape ({n},)")
+    return np.ascontiguousarray(a)
+
+
+def _branch_args(cs, vn_from_kv, vn_to_kv, rating, trafo):
+    nb = cs.nbr
+    vf = _per_bus(vn_from_kv, nb, "vn_from_kv")
+    vt = vf if vn_to_kv is None else _per_bus(vn_to_kv, nb, "vn_to_kv")
+    rt = _per_bus(rating, nb, "rating")
+    tr = np.zeros(nb, np.uint8) if trafo is None else np.ascontiguousarray(np.asarray(trafo, dtype=bool), dtype=np.uint8)
+    if tr.shape != (nb,):
+        raise ValueError(f"trafo must have shape ({nb},)")
+    return vf, vt, rt, tr
+
+
+def _branch_flows(cs, vn_from_kv, vn_to_kv, rating, trafo, sn_mva, threshold, table, cap):
+    vf, vt, rt, tr = _branch_args(cs, vn_from_kv, vn_to_kv, rating, trafo)
+    best = np.empty(cs.nbr, np.float64)
+    arg = np.empty(cs.nbr, np.int32)
+    ob = np.empty(max(cap, 1), np.int32)
+    oc = np.empty(max(cap, 1), np.int32)
+    ol = np.empty(max(cap, 1), np.float64)
+    oi = np.empty(max(cap, 1), np.float64)
+    n = C.c_int32(0)
+    check(lib().p3smi_outages_branch_flows(cs._h, ptr(vf), ptr(vt), ptr(rt), ptr(tr), float(sn_mva), float(threshold),
+                                           C.byref(table) if table is not None else None, ptr(best), ptr(arg),
+                                           int(cap), ptr(ob), ptr(oc), ptr(ol), ptr(oi), C.byref(n)),
+          "contingency: ")
+    return best, arg, int(n.value), ob, oc, ol, oi, rt
+
+
+def branch_loading(cs, vn_from_kv, vn_to_kv=None, rating=1.0, trafo=None, sn_mva: float = 1.0) -> BranchLoadingTable:
+    """compute_branch_loading (p3s/contingency/find_overloads.py:128-262) of the set's last
+    solve, on the device; the (n_branch, L) tables come back to the host."""
+    shape = (cs.nbr, cs.L)
+    arrs = {f: np.empty(shape, np.float64) for f, _ in _BranchTable._fields_}
+    tab = _BranchTable(*[arrs[f].ctypes.data for f, _ in _BranchTable._fields_])
+    best, arg, *_ = _branch_flows(cs, vn_from_kv, vn_to_kv, rating, trafo, sn_mva, np.inf, tab, 0)
+    return BranchLoadingTable(groups=cs.names, p_from_mw=arrs["p_from"], q_from_mvar=arrs["q_from"],
+                              p_to_mw=arrs["p_to"], q_to_mvar=arrs["q_to"], i_from_ka=arrs["i_from"],
+                              i_to_ka=arrs["i_to"], i_ka=arrs["i_ka"], loading_percent=arrs["loading"],
+                              max_loading_percent=best, worst_case=arg.astype(np.int64))
+
+
+def find_overloads(cs, threshold_percent: float, vn_from_kv, vn_to_kv=None, rating=1.0, trafo=None,
+                   sn_mva: float = 1.0, cap: int = 1 << 20) -> OverloadReport:
+    """find_overloads (p3s/contingency/find_overloads.py:322-420) of the set's last solve: only
+    the violations and the per-branch worst case leave the device."""
+    if not np.isfinite(threshold_percent):
+        raise ValueError(f"threshold_percent must be finite, got {threshold_percent!r}")
+    best, arg, n, ob, oc, ol, oi, rt = _branch_flows(cs, vn_from_kv, vn_to_kv, rating, trafo, sn_mva,
+                                                     threshold_percent, None, cap)
+    if n > cap:
+        raise RuntimeError(f"find_overloads: {n} violations exceed cap={cap}; raise cap or the threshold")
+    ob, oc, ol, oi = ob[:n], oc[:n], ol[:n], oi[:n]
+    order = np.lexsort((oc, ob, -ol))  # worst first, ties by branch then case
+    groups = np.asarray(cs.names, dtype=object)
+    return OverloadReport(threshold_percent=float(threshold_percent), branch=ob[order].astype(np.int64),
+                          case=oc[order].astype(np.int64), group=groups[oc[order]], loading_percent=ol[order],
+                          i_ka=oi[order], rating=rt[ob[order]], max_loading_percent=best,
+                          worst_case=arg.astype(np.int64))
+
+
+def find_voltage_violations(cs, vm_min_pu, vm_max_pu, in_service=None, tol_pu: float = 1e-9,
+                            cap: int = 1 << 20) -> VoltageViolationReport:
+    """find_voltage_violations (p3s/contingency/voltage_band.py:103-182) of the set's last solve."""
+    if not np.isfinite(tol_pu) or tol_pu < 0:
+        raise ValueError(f"tol_pu must be finite and >= 0, got {tol_pu!r}")
+    n_bus = cs.n
+    lo = _per_bus(vm_min_pu, n_bus, "vm_min_pu")
+    hi = _per_bus(vm_max_pu, n_bus, "vm_max_pu")
+    if np.isnan(lo).all() and np.isnan(hi).all():
+        raise ValueError("no voltage band: pass vm_min_pu / vm_max_pu")
+    ins = np.ones(n_bus, np.uint8) if in_service is None else np.ascontiguousarray(in_service, dtype=np.uint8)
+    ob = np.empty(max(cap, 1), np.int32)
+    oc = np.empty(max(cap, 1), np.int32)
+    ov = np.empty(max(cap, 1), np.float64)
+    ok = np.empty(max(cap, 1), np.uint8)
+    n = C.c_int32(0)
+    check(lib().p3smi_outages_voltage_violations(cs._h, ptr(lo), ptr(hi), ptr(ins), float(tol_pu), int(cap), ptr(ob),
+                                                 ptr(oc), ptr(ov), ptr(ok), C.byref(n)), "contingency: ")
+    n = int(n.value)
+    if n > cap:
+        raise RuntimeError(f"find_voltage_violations: {n} violations exceed cap={cap}")
+    ob, oc, ov, ok = ob[:n], oc[:n], ov[:n], ok[:n]
+    # the reference enumerates the low violations, then the high ones, each in (bus, case)
+    # order, and sorts them stably by deviation, worst first
+    pre = np.lexsort((oc, ob, ok))
+    ob, oc, ov, ok = ob[pre], oc[pre], ov[pre], ok[pre]
+    limit = np.where(ok == 1, hi[ob], lo[ob])
+    dev = np.abs(ov - limit)
+    order = np.argsort(dev, kind="stable")[::-1]
+    groups = np.asarray(cs.names, dtype=object)
+    kind = np.where(ok == 1, "high", "low").astype(object)
+    return VoltageViolationReport(bus=ob[order].astype(np.int64), case=oc[order].astype(np.int64),
+                                  group=groups[oc[order]], kind=kind[order], vm_pu=ov[order], limit_pu=limit[order],
+                                  deviation_pu=dev[order])
