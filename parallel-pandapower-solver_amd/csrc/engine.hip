@@ -316,8 +316,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
     const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int2* __restrict__ bent,
     const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
     int* __restrict__ status, int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips,
-    int* __restrict__ vote, const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S,
-    int it) {
+    int* __restrict__ vote, const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S) {
     constexpr bool SINGLE = sizeof(real) == 4;
     constexpr int TL = 64 * NS / G;
     __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
@@ -515,8 +514,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int2* __restrict__ bent,
     const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
     int* __restrict__ status, int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips,
-    int* __restrict__ vote, const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S,
-    int it) {
+    int* __restrict__ vote, const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S) {
     constexpr bool SINGLE = sizeof(real) == 4;
     constexpr int TL = 64 * NS / G;
     __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
@@ -750,8 +748,7 @@ __global__ void k_step(const double* __restrict__ x, const int* __restrict__ xva
 __global__ void k_decide(const double* __restrict__ part, int nparts, double* __restrict__ nrm,
                          double* __restrict__ alpha, double* __restrict__ best_alpha, double* __restrict__ best_nrm,
                          int* __restrict__ trial, int* __restrict__ status, int* __restrict__ iters,
-                         int* __restrict__ counts, double tol, int line_search, int phase, int it, int max_iter,
-                         int S) {
+                         int* __restrict__ counts, double tol, int line_search, int phase, int max_iter, int S) {
     __shared__ int cnt[NSTAT];
     if (threadIdx.x < NSTAT) cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -965,7 +962,7 @@ __global__ void k_branch_flows(const int* __restrict__ fb, const int* __restrict
 // Per branch: the largest loading over the cases and the first case reaching it (NaN and
 // -1 when every case is NaN), as nanmax / nanargmax do it; and every (branch, case) above
 // the threshold into a list (atomic append, order arbitrary: the host sorts it).
-__global__ __launch_bounds__(256) void k_branch_worst(const double* __restrict__ loading, int nbr, int L,
+__global__ __launch_bounds__(256) void k_branch_worst(const double* __restrict__ loading, int L,
                                                       double* __restrict__ best, int* __restrict__ arg) {
     __shared__ double sv[256];
     __shared__ int si[256];
@@ -1538,15 +1535,14 @@ void launch_eval(p3smi_batch* b, int run_mask, int j_mask) {
 }
 
 template <int G, int NS, typename real, bool FACTOR>
-void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, int it, real* fac, const double* rhs, double* xout, int sel) {
+void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* rhs, double* xout, int sel) {
     const p3smi::Topology& h = b->topo->host;
-    const DevTopo& d = b->topo->dev;
     const double rcond_min = REUSE_RCOND_RATIO * h.rcond0;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3((unsigned)(b->S / G)), dim3(64 * NS), 0, b->topo->stream, ds.ptrA.p,
                            ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.diagB.p, ds.pairs.p, ds.ptrF.p, ds.fwd.p, ds.fent.p,
                            ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout, b->status.p, b->iters.p,
-                           b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min, ds.nlev, ds.nblev, b->S, it);
+                           b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min, ds.nlev, ds.nblev, b->S);
     };
     if constexpr (G >= 16) go(k_lu_stream<G, NS, real, FACTOR>);
     else go(k_lu<G, NS, real, FACTOR>);
@@ -1560,14 +1556,14 @@ bool valid_lu_config(LuConfig c) {
 
 // One LU launch over the selected cases; n_active picks the batch or the few-cases shape.
 template <typename real, bool FACTOR>
-void launch_lu(p3smi_batch* b, int it, int n_active, real* fac, const double* rhs, double* xout, int sel) {
+void launch_lu(p3smi_batch* b, int n_active, real* fac, const double* rhs, double* xout, int sel) {
     p3smi_topology* t = b->topo;
     const LuConfig cfg = n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
     const DevSched& ds = get_sched(t, cfg);
     KTimer tm(b, 1);
 #define P3SMI_LU_CASE(GG, NN)                                                                \
     if (cfg.G == GG && cfg.NS == NN) {                                                       \
-        launch_lu_cfg<GG, NN, real, FACTOR>(b, ds, it, fac, rhs, xout, sel);                 \
+        launch_lu_cfg<GG, NN, real, FACTOR>(b, ds, fac, rhs, xout, sel);                     \
         HIPCHECK(hipGetLastError());                                                         \
         return;                                                                              \
     }
@@ -1602,13 +1598,13 @@ const int* read_counts(p3smi_batch* b) {
 // solve, then refinement x += LU32^-1 (R - J x) with the residual in FP64 until the
 // correction is below rtol |x|; cases whose FP32 pivots fail or whose refinement stalls
 // are factored in FP64 in the same iteration.
-void linear_solve(p3smi_batch* b, int it, int n_active, int precision) {
+void linear_solve(p3smi_batch* b, int n_active, int precision) {
     p3smi_topology* t = b->topo;
     const p3smi::Topology& h = t->host;
     hipStream_t st = t->stream;
     if (precision != P3SMI_PREC_MIXED) {
         b->stats.double_factorizations += n_active;
-        launch_lu<double, true>(b, it, n_active, b->fac.p, b->R.p, b->x.p, SEL_ALL);
+        launch_lu<double, true>(b, n_active, b->fac.p, b->R.p, b->x.p, SEL_ALL);
         return;
     }
     ensure_mixed(b);
@@ -1618,7 +1614,7 @@ void linear_solve(p3smi_batch* b, int it, int n_active, int precision) {
         KTimer tm(b, 4);
         hipLaunchKernelGGL(k_refine_init, dim3(gS), dim3(256), 0, st, b->status.p, b->vote.p, b->rf.p, b->prevd.p, b->S);
     }
-    launch_lu<float, true>(b, it, n_active, b->fac32.p, b->R.p, b->x.p, SEL_F32);
+    launch_lu<float, true>(b, n_active, b->fac32.p, b->R.p, b->x.p, SEL_F32);
     for (int k = 1; k <= t->mixed_kmax; ++k) {
         {
             KTimer tm(b, 5);
@@ -1626,7 +1622,7 @@ void linear_solve(p3smi_batch* b, int it, int n_active, int precision) {
                                t->dev.Jp.p, t->dev.xcol.p, b->Jx.p, b->R.p, b->x.p, b->rr.p, b->status.p, b->vote.p,
                                b->rf.p, h.m, b->S);
         }
-        launch_lu<float, false>(b, it, n_active, b->fac32.p, b->rr.p, b->dvec.p, SEL_REFINE);
+        launch_lu<float, false>(b, n_active, b->fac32.p, b->rr.p, b->dvec.p, SEL_REFINE);
         {
             KTimer tm(b, 5);
             hipLaunchKernelGGL(k_refine_update, dim3((unsigned)b->nparts_r, (unsigned)(b->S / 64)), dim3(256), 0, st,
@@ -1640,7 +1636,7 @@ void linear_solve(p3smi_batch* b, int it, int n_active, int precision) {
         b->stats.refinement_steps += 1;
         if (read_counts(b)[0] == 0) break;
     }
-    launch_lu<double, true>(b, it, n_active, b->fac.p, b->R.p, b->x.p, SEL_F64);
+    launch_lu<double, true>(b, n_active, b->fac.p, b->R.p, b->x.p, SEL_F64);
 }
 
 void launch_step(p3smi_batch* b, int mask, int full) {
@@ -1653,13 +1649,13 @@ void launch_step(p3smi_batch* b, int mask, int full) {
     HIPCHECK(hipGetLastError());
 }
 
-const int* launch_decide(p3smi_batch* b, double tol, int ls, int phase, int it, int max_iter) {
+const int* launch_decide(p3smi_batch* b, double tol, int ls, int phase, int max_iter) {
     KTimer tm(b, 3);
     hipStream_t st = b->topo->stream;
     hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->counts.p, NSTAT);
     hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->part.p, b->nparts,
                        b->nrm.p, b->alpha.p, b->best_alpha.p, b->best_nrm.p, b->trial.p, b->status.p, b->iters.p,
-                       b->counts.p, tol, ls, phase, it, max_iter, b->S);
+                       b->counts.p, tol, ls, phase, max_iter, b->S);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -1684,7 +1680,7 @@ const int* start_new(p3smi_batch* b, double tol, int line_search, int precision,
         HIPCHECK(hipGetLastError());
     }
     launch_eval(b, 1 << ST_NEW, 1 << ST_NEW);
-    return launch_decide(b, tol, line_search, 0, 0, max_iter);
+    return launch_decide(b, tol, line_search, 0, max_iter);
 }
 
 // ---- hipGraph replay of Newton iterations ---------------------------------------------
@@ -1700,7 +1696,7 @@ void record_decide(p3smi_batch* b, double tol, int ls, int phase, int max_iter) 
     hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->counts.p, NSTAT);
     hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->part.p, b->nparts,
                        b->nrm.p, b->alpha.p, b->best_alpha.p, b->best_nrm.p, b->trial.p, b->status.p, b->iters.p,
-                       b->counts.p, tol, ls, phase, 0, max_iter, b->S);
+                       b->counts.p, tol, ls, phase, max_iter, b->S);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
 }
@@ -1747,7 +1743,7 @@ const int* newton_iteration_graph(p3smi_batch* b, int n_active, double tol, int 
     b->stats.newton_passes++;
     b->stats.double_factorizations += n_active;
     const int* cnt = replay(b, "it" + base + "/" + std::to_string(line_search), [&] {
-        launch_lu<double, true>(b, 0, n_active, b->fac.p, b->R.p, b->x.p, SEL_ALL);
+        launch_lu<double, true>(b, n_active, b->fac.p, b->R.p, b->x.p, SEL_ALL);
         launch_step(b, A, 1);
         launch_eval(b, A, A);
         record_decide(b, tol, line_search, 1, max_iter);
@@ -1773,26 +1769,25 @@ const int* newton_iteration_graph(p3smi_batch* b, int n_active, double tol, int 
 // One Newton iteration of every ACTIVE case (p3s/cpp/nr_klu.cpp:595-708): linear solve,
 // full step scored by a full evaluation, backtracking passes while any case backtracks,
 // Jacobian rebuilt at accepted trial points.
-const int* newton_iteration(p3smi_batch* b, int n_active, double tol, int line_search, int precision, int max_iter,
-                            int it) {
+const int* newton_iteration(p3smi_batch* b, int n_active, double tol, int line_search, int precision, int max_iter) {
     const int A = 1 << ST_ACTIVE, BT = 1 << ST_BACKTRACK, NJ = 1 << ST_NEEDJ;
     if (b->graphs && !b->timing && precision == P3SMI_PREC_DOUBLE)
         return newton_iteration_graph(b, n_active, tol, line_search, max_iter);
     b->stats.newton_passes++;
-    linear_solve(b, it, n_active, precision);
+    linear_solve(b, n_active, precision);
     launch_step(b, A, 1);
     launch_eval(b, A, A);
-    const int* cnt = launch_decide(b, tol, line_search, 1, it, max_iter);
+    const int* cnt = launch_decide(b, tol, line_search, 1, max_iter);
     while (cnt[ST_BACKTRACK] > 0) {
         b->stats.line_search_passes++;
         launch_step(b, BT, 0);
         launch_eval(b, BT, 0);
-        cnt = launch_decide(b, tol, line_search, 2, it, max_iter);
+        cnt = launch_decide(b, tol, line_search, 2, max_iter);
     }
     if (cnt[ST_NEEDJ] > 0) {
         launch_step(b, NJ, 0);
         launch_eval(b, NJ, NJ);
-        cnt = launch_decide(b, tol, line_search, 3, it, max_iter);
+        cnt = launch_decide(b, tol, line_search, 3, max_iter);
     }
     return cnt;
 }
@@ -1816,7 +1811,7 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
     int it = 0;
     while (it < max_iter && cnt[ST_ACTIVE] > 0) {
         ++it;
-        cnt = newton_iteration(b, cnt[ST_ACTIVE], tol, line_search, precision, max_iter, it);
+        cnt = newton_iteration(b, cnt[ST_ACTIVE], tol, line_search, precision, max_iter);
     }
     b->stats.failed_factorizations = cnt[ST_FAILED];
     HIPCHECK(hipMemcpyAsync(&b->stats.guard_trips, b->guard.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1863,12 +1858,6 @@ void upload_cols(void* dst, const void* src, size_t rows, size_t ld, size_t c0, 
                  hipStream_t st) {
     HIPCHECK(hipMemcpy2DAsync(dst, cols * elem, (const char*)src + c0 * elem, ld * elem, cols * elem, rows,
                               hipMemcpyHostToDevice, st));
-}
-
-void download_cols(void* dst, const void* src, size_t rows, size_t ld, size_t c0, size_t cols, size_t elem,
-                   hipStream_t st) {
-    HIPCHECK(hipMemcpy2DAsync((char*)dst + c0 * elem, ld * elem, src, cols * elem, cols * elem, rows,
-                              hipMemcpyDeviceToHost, st));
 }
 
 size_t bytes_per_case(const p3smi::Topology& h, bool per_case_y, bool pin, bool per_case_s) {
@@ -2184,8 +2173,10 @@ int p3smi_batch_bytes_per_case(const p3smi_batch* b, double* out4) {
     const double n = h.n, m = h.m, nnzJ = h.nnzJ, lu = (double)h.nnz_lu;
     // eval: Vm, cs, sn (24 n) + S (16 n) + J write (8 nnzJ) + R write (8 m) [+ per-case Y 16 nnzY]
     out4[0] = 24 * n + (b->s_stride ? 16 * n : 0) + 8 * nnzJ + 8 * m + (b->per_case_y ? 16.0 * h.nnzY : 0.0);
-    // lu: J read (8 nnzJ) + factor write (8 nnz_lu) + R read (8 m) + x write/read (16 m)
-    out4[1] = 8 * nnzJ + 8 * lu + 8 * m + 16 * m;
+    // lu: J read (8 nnzJ) + factor write (8 nnz_lu) + the substitutions' read of every factor
+    // entry (8 nnz_lu: a 64-case group's factors, nnz_lu * 512 B, do not stay on chip between
+    // the factor and solve phases) + R read (8 m) + x write/read (16 m)
+    out4[1] = 8 * nnzJ + 16 * lu + 8 * m + 16 * m;
     // step: x read (8 m) + Vm, Va read/write + cs/sn write + base write (8 n * 8)
     out4[2] = 8 * m + 64 * n;
     out4[3] = 64;
@@ -2544,7 +2535,7 @@ int p3smi_outages_branch_flows(p3smi_outages* o, const double* vn_from_kv, const
             best.alloc(nbr);
             arg.alloc(nbr);
             if (nbr > 0)
-                hipLaunchKernelGGL(k_branch_worst, dim3((unsigned)nbr), dim3(256), 0, st, ld.p, nbr, L, best.p, arg.p);
+                hipLaunchKernelGGL(k_branch_worst, dim3((unsigned)nbr), dim3(256), 0, st, ld.p, L, best.p, arg.p);
             HIPCHECK(hipGetLastError());
             if (max_loading) HIPCHECK(hipMemcpyAsync(max_loading, best.p, nbr * sizeof(double), hipMemcpyDeviceToHost, st));
             if (worst_case) HIPCHECK(hipMemcpyAsync(worst_case, arg.p, nbr * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -2682,7 +2673,7 @@ int p3smi_batch_pass(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_
             cnt = start_new(b, tol, line_search, precision, max_iter);
             b->pending_new = false;
         }
-        if (cnt[ST_ACTIVE] > 0) cnt = newton_iteration(b, cnt[ST_ACTIVE], tol, line_search, precision, max_iter, 0);
+        if (cnt[ST_ACTIVE] > 0) cnt = newton_iteration(b, cnt[ST_ACTIVE], tol, line_search, precision, max_iter);
         if (counts) std::copy(cnt, cnt + NSTAT, counts);
         collect_timing(b);
     });
