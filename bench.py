@@ -121,7 +121,7 @@ def base_out(args, value, times, t_total, precision, config):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if precision == "double" else "f32-lu+f64",
+        "dtype": {"double": "f64", "mixed": "f32-lu+f64", "mixed_switch": "f32/f64-lu"}[precision],
         "data": "synthetic",
         "config": config,
         "latency_ms": {"p50": round(1e3 * float(np.percentile(times, 50)), 3),
@@ -215,7 +215,7 @@ def run_timeseries(args) -> dict:
         f"levels={info['n_levels']} ({t_setup:.2f}s)")
 
     L = _lib.lib()
-    prec = {"double": 0, "mixed": 1}[args.precision]
+    prec = {"double": 0, "mixed": 1, "mixed_switch": 2}[args.precision]
     bh = C.c_void_p()
     _lib.check(L.p3smi_batch_create(solver._h, T, C.byref(bh)))
     S_c = np.ascontiguousarray(S, dtype=np.complex128)
@@ -326,8 +326,8 @@ def run_n1(args) -> dict:
                                          _lib.ptr(mis), _lib.ptr(served) if served is not None else None))
 
     main = args.precision
-    other = "double" if main == "mixed" else "mixed"
-    code = {"double": 0, "mixed": 1}
+    other = "double" if main != "double" else "mixed"
+    code = {"double": 0, "mixed": 1, "mixed_switch": 2}
     # the first solve allocates the resident batch whose kernel timers the steps read
     args.warmup = max(1, args.warmup)
     for w in range(args.warmup):
@@ -352,6 +352,9 @@ def run_n1(args) -> dict:
 
     acc, times, t_total, n_conv, it_main = timed(main)
     acc2, times2, t_total2, n_conv2, it_other = timed(other)
+    third = ({"double", "mixed", "mixed_switch"} - {main, other}).pop()
+    solve(code[third])  # warm-up of the third precision
+    acc3, _, t_total3, n_conv3, it_third = timed(third)
     bpc = np.empty(4, dtype=np.float64)
     L.p3smi_batch_bytes_per_case(bh, _lib.ptr(bpc))
     _lib.check(L.p3smi_batch_set_timing(bh, 0))
@@ -396,6 +399,11 @@ def run_n1(args) -> dict:
                     "converged": n_conv2, "iterations_equal": bool(np.array_equal(it_main, it_other)),
                     "kernel_ms_per_step": {k: round(acc2[f"ms_{k}"] / args.steps, 3)
                                            for k in ("eval", "lu", "refine", "step", "decide", "other")}},
+        "compare_2": {"precision": third, "value": round(n_conv3 * args.steps / t_total3, 1),
+                      "ms_per_step": round(1e3 * t_total3 / args.steps, 3), "converged": n_conv3,
+                      "iterations_equal_fraction_vs_main": round(float(np.mean(it_third == it_main)), 4),
+                      "kernel_ms_per_step": {k: round(acc3[f"ms_{k}"] / args.steps, 3)
+                                             for k in ("eval", "lu", "refine", "step", "decide", "other")}},
         "mixed_over_fp64": round((out["value"] / value2) if main == "mixed" else (value2 / out["value"]), 3),
         "post_processing": {"branch_case_pairs": int(g.nbr) * int(Lc), "overloads": len(rep),
                             "overload_screen_ms": round(1e3 * t_over, 3), "voltage_violations": len(vv),
@@ -666,7 +674,7 @@ def main() -> int:
     ap.add_argument("--config", default="ts2000", choices=["ts2000", "pf200", "n1_10k", "multi48", "serve64"])
     ap.add_argument("--rate", type=float, default=200.0, help="serve64: mean arrival rate (requests/s)")
     ap.add_argument("--requests", type=int, default=300, help="serve64: number of requests")
-    ap.add_argument("--precision", default=None, choices=["double", "mixed"],
+    ap.add_argument("--precision", default=None, choices=["double", "mixed", "mixed_switch"],
                     help="default: double (time series), mixed (n1_10k)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample duration")

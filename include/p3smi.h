@@ -30,6 +30,12 @@ extern "C" {
 /* precision of the Newton-step linear solve */
 #define P3SMI_PREC_DOUBLE 0 /* FP64 static-pivot LU: the iteration nr_klu runs */
 #define P3SMI_PREC_MIXED 1  /* FP32 LU + FP64 iterative refinement, FP64 LU fallback per case */
+/* The reference GPU solver's mixed mode (p3s/cuda/nr_polar_kernels.cu:209-288): FP32 LU (no
+ * refinement) while the case's mismatch is above the switching value (1e-3 p.u. unless
+ * set), FP64 below it, FP64 for good once an FP32 step fails to halve the mismatch or the
+ * line search cuts a step, or FP32 pivots fail (that iteration retried in FP64). An inexact
+ * Newton: iteration counts may differ from the FP64 iteration. */
+#define P3SMI_PREC_MIXED_SWITCH 2
 
 typedef struct p3smi_topology p3smi_topology; /* one grid topology, resident on the device */
 typedef struct p3smi_batch p3smi_batch;       /* device workspace for a batch of cases */
@@ -71,6 +77,9 @@ int p3smi_topology_create(const int32_t* Yp, const int32_t* Yj, const double* Yx
                           int32_t n_slots, p3smi_topology** out);
 void p3smi_topology_free(p3smi_topology* t);
 int p3smi_topology_info(const p3smi_topology* t, p3smi_info* out);
+/* Switching mismatch of P3SMI_PREC_MIXED_SWITCH (p.u., default 1e-3 as the reference's
+ * mixed_switch, p3s/cuda/nr_polar_solver.py:323). */
+int p3smi_topology_set_mixed_switch(p3smi_topology* t, double mismatch);
 /* LU launch shape (no counterpart in the reference; set by the autotuner): a workgroup of
  * 64*ns threads factors g consecutive cases (g in {1, 16, 32, 64}, ns in {8, 16});
  * (few_g, few_ns) is used instead while at most few_max cases are still iterating; group > 0

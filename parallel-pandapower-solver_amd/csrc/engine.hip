@@ -286,14 +286,15 @@ __device__ __forceinline__ ACC dot_pairs(const int2* __restrict__ pairs, int e, 
 // static-pivot step is kept: the step is the same solution of J dx = R, and the oracle's
 // runs with and without that fallback agree to roundoff on every converging case.
 // Which cases a launch works on (besides being ACTIVE): every one (SEL_ALL, FP64 path
-// without mixed precision), those whose vote is FP64 (SEL_F64), FP32 (SEL_F32), or FP32 and
+// without mixed precision), those whose vote is not FP32 (SEL_F64: vote 0, or -1 = FP64 from now
+// on), FP32 (SEL_F32: vote 1), or FP32 and
 // still refining (SEL_REFINE).
 enum : int { SEL_ALL = 0, SEL_F64 = 1, SEL_F32 = 2, SEL_REFINE = 3 };
 
 __device__ __forceinline__ bool selected(int sel, int c, const int* status, const int* vote, const int* rf) {
     if (status[c] != ST_ACTIVE) return false;
     switch (sel) {
-        case SEL_F64: return vote[c] == 0;
+        case SEL_F64: return vote[c] != 1;
         case SEL_F32: return vote[c] == 1;
         case SEL_REFINE: return vote[c] == 1 && rf[c] == 1;
         default: return true;
@@ -387,7 +388,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
                 if (SINGLE) {
                     if (broken || pmin < rcond_min * pmax) {
                         bad = 1;
-                        vote[c] = 0;
+                        vote[c] = -1;
                     }
                 } else if (broken) {  // zero or non-finite pivot: no step (in this iteration)
                     bad = 1;
@@ -586,7 +587,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
                 if (SINGLE) {
                     if (broken || pmin < rcond_min * pmax) {
                         bad = 1;
-                        vote[c] = 0;
+                        vote[c] = -1;
                     }
                 } else if (broken) {
                     bad = 1;
@@ -683,7 +684,7 @@ __global__ void k_refine_decide(const double* __restrict__ part, int nparts, dou
             rf[c] = 0;
         } else if (!(dm <= 0.5 * prevd[c]) || last) {
             rf[c] = 0;
-            vote[c] = 0;
+            vote[c] = -1;
         } else {
             prevd[c] = dm;
             atomicAdd(&cnt, 1);
@@ -745,10 +746,20 @@ __global__ void k_step(const double* __restrict__ x, const int* __restrict__ xva
 //   phase 3  Jacobian rebuilt at the accepted point
 // The loop of the reference ends after the step of iteration max_iter without a further
 // convergence test, so such a case ends STOPPED (not converged).
+// Precision vote of the switching mixed mode (p3s/cuda/nr_polar_kernels.cu:209-275): FP32
+// factors while the mismatch is above `sw`, FP64 below it, and FP64 from then on (-1) once an
+// FP32 step failed to halve the mismatch or the line search had to cut a step.
+__device__ __forceinline__ void vote_switch(int* vote, int c, double old_nrm, double nw, double sw) {
+    if (vote[c] < 0) return;
+    if (vote[c] == 1 && !(nw <= 0.5 * old_nrm)) vote[c] = -1;
+    else vote[c] = nw > sw ? 1 : 0;
+}
+
 __global__ void k_decide(const double* __restrict__ part, int nparts, double* __restrict__ nrm,
                          double* __restrict__ alpha, double* __restrict__ best_alpha, double* __restrict__ best_nrm,
                          int* __restrict__ trial, int* __restrict__ status, int* __restrict__ iters,
-                         int* __restrict__ counts, double tol, int line_search, int phase, int max_iter, int S) {
+                         int* __restrict__ counts, int* __restrict__ vote, double tol, int line_search, int phase,
+                         int max_iter, double sw, int S) {
     __shared__ int cnt[NSTAT];
     if (threadIdx.x < NSTAT) cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -768,9 +779,11 @@ __global__ void k_decide(const double* __restrict__ part, int nparts, double* __
                 nrm[c] = nw;
                 iters[c] = 0;
                 st = classify(nw);
+                if (sw > 0.0) vote[c] = nw > sw ? 1 : 0;
             } else if (phase == 1) {
                 iters[c] += 1;
                 if (!line_search || nw <= (1.0 - ARMIJO_C) * nrm[c]) {
+                    if (sw > 0.0) vote_switch(vote, c, nrm[c], nw, sw);
                     nrm[c] = nw;
                     st = classify(nw);
                 } else {
@@ -779,6 +792,7 @@ __global__ void k_decide(const double* __restrict__ part, int nparts, double* __
                     alpha[c] = 0.5;
                     trial[c] = 1;
                     st = ST_BACKTRACK;
+                    if (sw > 0.0) vote[c] = -1;
                 }
             } else if (phase == 2) {
                 const double a = alpha[c];
@@ -1273,6 +1287,7 @@ struct p3smi_topology {
     int lu_few_max = 256;         // "few": at most this many active cases
     bool graphs = false;          // one-call solves replay captured Newton iterations
     double mixed_rtol = 1e-12;    // refinement stops once |dx correction| <= rtol |dx|
+    double mixed_switch = 1e-3;   // switching mode: FP32 factors while the mismatch is above this
     int mixed_kmax = 6;           // refinement steps before the FP64 fallback
     hipStream_t stream = nullptr;
     std::unique_ptr<p3smi_batch> cached;  // workspace of the one-call entry points
@@ -1305,6 +1320,7 @@ struct p3smi_batch {
     DevBuf<unsigned char> st_cv;
     DevBuf<double> st_mis;
     bool timing = false;
+    int precision = 0;                  // precision of the solve in progress
     unsigned long long solve_id = 0;    // stamp of the last run_newton on this batch
     bool sbus_built = false;            // Ssp holds injections built on the device (k_sbus)
     bool graphs = false;                // replay captured Newton iterations (FP64, timers off)
@@ -1602,9 +1618,18 @@ void linear_solve(p3smi_batch* b, int n_active, int precision) {
     p3smi_topology* t = b->topo;
     const p3smi::Topology& h = t->host;
     hipStream_t st = t->stream;
-    if (precision != P3SMI_PREC_MIXED) {
+    if (precision == P3SMI_PREC_DOUBLE) {
         b->stats.double_factorizations += n_active;
         launch_lu<double, true>(b, n_active, b->fac.p, b->R.p, b->x.p, SEL_ALL);
+        return;
+    }
+    if (precision == P3SMI_PREC_MIXED_SWITCH) {
+        // FP32 factors and solve for the cases voting FP32, no refinement; FP64 for the others
+        // and for those whose FP32 pivots failed (vote -1), in the same iteration
+        ensure_mixed(b);
+        b->stats.single_factorizations += n_active;  // launched over every active case
+        launch_lu<float, true>(b, n_active, b->fac32.p, b->R.p, b->x.p, SEL_F32);
+        launch_lu<double, true>(b, n_active, b->fac.p, b->R.p, b->x.p, SEL_F64);
         return;
     }
     ensure_mixed(b);
@@ -1652,10 +1677,11 @@ void launch_step(p3smi_batch* b, int mask, int full) {
 const int* launch_decide(p3smi_batch* b, double tol, int ls, int phase, int max_iter) {
     KTimer tm(b, 3);
     hipStream_t st = b->topo->stream;
+    const double sw = b->precision == P3SMI_PREC_MIXED_SWITCH ? b->topo->mixed_switch : 0.0;
     hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->counts.p, NSTAT);
     hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->part.p, b->nparts,
                        b->nrm.p, b->alpha.p, b->best_alpha.p, b->best_nrm.p, b->trial.p, b->status.p, b->iters.p,
-                       b->counts.p, tol, ls, phase, max_iter, b->S);
+                       b->counts.p, b->vote.p, tol, ls, phase, max_iter, sw, b->S);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -1665,8 +1691,8 @@ const int* launch_decide(p3smi_batch* b, double tol, int ls, int phase, int max_
 void check_solve_args(int max_iter, double tol, int precision) {
     if (max_iter <= 0) throw Error(P3SMI_E_ARG, "max_iter must be > 0");
     if (!(tol > 0.0)) throw Error(P3SMI_E_ARG, "tol must be > 0");
-    if (precision != P3SMI_PREC_DOUBLE && precision != P3SMI_PREC_MIXED)
-        throw Error(P3SMI_E_ARG, "precision must be 0 (double) or 1 (mixed)");
+    if (precision != P3SMI_PREC_DOUBLE && precision != P3SMI_PREC_MIXED && precision != P3SMI_PREC_MIXED_SWITCH)
+        throw Error(P3SMI_E_ARG, "precision must be 0 (double), 1 (mixed) or 2 (mixed_switch)");
 }
 
 // Start point of the NEW cases: polar state, mismatch and Jacobian, classification.
@@ -1676,7 +1702,7 @@ const int* start_new(p3smi_batch* b, double tol, int line_search, int precision,
         KTimer tm(b, 4);
         hipLaunchKernelGGL(k_init, grid_items(h.n, b->S), dim3(256), 0, b->topo->stream, b->V0.p, b->v0_stride,
                            b->Vm.p, b->Va.p, b->cs.p, b->sn.p, b->status.p, b->vote.p,
-                           precision == P3SMI_PREC_MIXED ? 1 : 0, h.n, b->S);
+                           precision == P3SMI_PREC_DOUBLE ? 0 : 1, h.n, b->S);
         HIPCHECK(hipGetLastError());
     }
     launch_eval(b, 1 << ST_NEW, 1 << ST_NEW);
@@ -1696,7 +1722,7 @@ void record_decide(p3smi_batch* b, double tol, int ls, int phase, int max_iter) 
     hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->counts.p, NSTAT);
     hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->part.p, b->nparts,
                        b->nrm.p, b->alpha.p, b->best_alpha.p, b->best_nrm.p, b->trial.p, b->status.p, b->iters.p,
-                       b->counts.p, tol, ls, phase, max_iter, b->S);
+                       b->counts.p, b->vote.p, tol, ls, phase, max_iter, 0.0, b->S);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
 }
@@ -1797,6 +1823,7 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
     check_solve_args(max_iter, tol, precision);
     hipStream_t st = b->topo->stream;
     std::memset(&b->stats, 0, sizeof b->stats);
+    b->precision = precision;
     b->solve_id = ++g_solve_ids;
     b->stats.cases = b->B;
     {
@@ -1958,6 +1985,15 @@ int p3smi_topology_set_mixed(p3smi_topology* t, double rtol, int32_t kmax) {
         std::lock_guard<std::mutex> lk(t->mu);
         t->mixed_rtol = rtol;
         t->mixed_kmax = kmax;
+    });
+}
+
+int p3smi_topology_set_mixed_switch(p3smi_topology* t, double mismatch) {
+    return guarded([&] {
+        if (!(mismatch > 0.0) || !std::isfinite(mismatch))
+            throw Error(P3SMI_E_ARG, "mixed_switch: the switching mismatch must be finite and > 0");
+        std::lock_guard<std::mutex> lk(t->mu);
+        t->mixed_switch = mismatch;
     });
 }
 
@@ -2668,6 +2704,7 @@ int p3smi_batch_pass(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_
     return guarded([&] {
         if (!b->uploaded) throw Error(P3SMI_E_ARG, "pass: batch slots not initialised");
         check_solve_args(max_iter, tol, precision);
+        b->precision = precision;
         const int* cnt = b->h_counts;
         if (b->pending_new) {
             cnt = start_new(b, tol, line_search, precision, max_iter);

@@ -12,7 +12,10 @@ Every batch call runs the whole Newton iteration of all cases on the GPU through
 libp3smi.so. Results add ``mismatch``: each case's final infinity-norm mismatch,
 evaluated in FP64 on the device. ``n_threads`` is accepted for signature compatibility
 (the work is not spread over CPU threads). ``precision`` selects the linear solver:
-"double" (FP64 static-pivot LU, the default) or "mixed".
+"double" (FP64 static-pivot LU, the default); "mixed" (FP32 LU refined in FP64 to the FP64
+step, per-case FP64 fallback: the FP64 iteration, counts included); "mixed_switch" (the
+reference GPU solver's rule: FP32 LU without refinement while the mismatch is above 1e-3,
+FP64 after -- an inexact Newton whose iteration counts may differ).
 """
 
 from __future__ import annotations
@@ -27,7 +30,7 @@ from ._lib import P3smiError, check, lib, ptr
 
 openmp = False  # the batch is spread over the GPU, not over OpenMP threads
 
-_PREC = {"double": 0, "mixed": 1}
+_PREC = {"double": 0, "mixed": 1, "mixed_switch": 2}
 
 
 def _c128(a, shape_err: str | None = None) -> np.ndarray:
@@ -38,7 +41,7 @@ def _prec(precision) -> int:
     try:
         return _PREC[precision]
     except KeyError:
-        raise ValueError(f"precision must be 'double' or 'mixed', got {precision!r}") from None
+        raise ValueError(f"precision must be 'double', 'mixed' or 'mixed_switch', got {precision!r}") from None
 
 
 class Solver:
