@@ -80,6 +80,11 @@ int p3smi_topology_info(const p3smi_topology* t, p3smi_info* out);
 /* Switching mismatch of P3SMI_PREC_MIXED_SWITCH (p.u., default 1e-3 as the reference's
  * mixed_switch, p3s/cuda/nr_polar_solver.py:323). */
 int p3smi_topology_set_mixed_switch(p3smi_topology* t, double mismatch);
+/* Largest number of cases the one-call entry points solve at once: the reference GPU
+ * solver's max_chunk (p3s/cuda/nr_polar_solver.py:311, 619-637). 0, the default: as many as
+ * 60% of the free HBM holds; else a positive multiple of 64. Larger calls are split into
+ * chunks of nearly equal size, whole multiples of 64, solved one after the other. */
+int p3smi_topology_set_max_chunk(p3smi_topology* t, int32_t cases);
 /* LU launch shape (no counterpart in the reference; set by the autotuner): a workgroup of
  * 64*ns threads factors g consecutive cases (g in {1, 16, 32, 64}, ns in {8, 16});
  * (few_g, few_ns) is used instead while at most few_max cases are still iterating; group > 0

@@ -286,8 +286,8 @@ __device__ __forceinline__ ACC dot_pairs(const int2* __restrict__ pairs, int e, 
 // static-pivot step is kept: the step is the same solution of J dx = R, and the oracle's
 // runs with and without that fallback agree to roundoff on every converging case.
 // Which cases a launch works on (besides being ACTIVE): every one (SEL_ALL, FP64 path
-// without mixed precision), those whose vote is not FP32 (SEL_F64: vote 0, or -1 = FP64 from now
-// on), FP32 (SEL_F32: vote 1), or FP32 and
+// without mixed precision), those whose vote is not FP32 (SEL_F64: vote 0, -1 = FP64 from now
+// on, or 2 = FP32 voter promoted with its group), FP32 (SEL_F32: vote 1), or FP32 and
 // still refining (SEL_REFINE).
 enum : int { SEL_ALL = 0, SEL_F64 = 1, SEL_F32 = 2, SEL_REFINE = 3 };
 
@@ -301,14 +301,37 @@ __device__ __forceinline__ bool selected(int sel, int c, const int* status, cons
     }
 }
 
+// Switching mixed mode, before its two LU launches. A wave factors 64 aligned cases, so it
+// costs its full time in a launch as soon as one of them is selected: with FP32 and FP64
+// votes mixed inside a wave, the wave would run in both launches. Here the unit is the wave:
+// when any active case of a 64-case group needs FP64, the group's FP32 voters take this
+// iteration's step in FP64 as well (vote 2: FP64 now, a fresh vote after the step), so every
+// group runs in one launch only. The groups are fixed by case index, not by launch shape.
+// Counts the FP32 and FP64 factorizations into fcount[1] and fcount[2].
+__global__ __launch_bounds__(256) void k_group_vote(const int* __restrict__ status, int* __restrict__ vote,
+                                                    int* __restrict__ fcount, int S) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;  // blockDim 256: each wave is one group
+    const bool act = c < S && status[c] == ST_ACTIVE;
+    const int v = act ? vote[c] : 1;
+    const bool any64 = __ballot(act && v != 1) != 0ull;
+    if (act && v == 1 && any64) vote[c] = 2;
+    const int n_act = __popcll(__ballot(act));
+    const int n32 = any64 ? 0 : n_act;
+    if ((threadIdx.x & 63) == 0 && n_act) {
+        if (n32) atomicAdd(&fcount[1], n32);
+        if (n_act - n32) atomicAdd(&fcount[2], n_act - n32);
+    }
+}
+
 // FACTOR = true: factorize (real = double in place of the Jacobian's copy, real = float in
 // the single-precision factor array) and solve J x = rhs. FACTOR = false: solve only, with
 // the factors of an earlier launch (the refinement solves). Substitutions accumulate in
 // FP64 whatever the factor precision.
 // Pivot failures: FP64 -- a zero or non-finite pivot fails the case, a ratio under the guard
-// is counted; FP32 -- either hands the case to the FP64 factorization of the same iteration
-// (vote = 0), as the reference's single-precision kernel does
-// (p3s/cuda/nr_polar_kernels.cu:477-482).
+// is counted in guard_trips[0]; FP32 -- either hands the case to the FP64 factorization of
+// the same iteration (vote = -1), as the reference's single-precision kernel does
+// (p3s/cuda/nr_polar_kernels.cu:477-482), counted in guard_trips[2] (the FP64
+// factorizations, see k_group_vote).
 template <int G, int NS, typename real, bool FACTOR>
 __global__ __launch_bounds__(64 * NS) void k_lu(
     const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
@@ -389,6 +412,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
                     if (broken || pmin < rcond_min * pmax) {
                         bad = 1;
                         vote[c] = -1;
+                        atomicAdd(guard_trips + 2, 1);  // refactored in FP64 below
                     }
                 } else if (broken) {  // zero or non-finite pivot: no step (in this iteration)
                     bad = 1;
@@ -588,6 +612,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
                     if (broken || pmin < rcond_min * pmax) {
                         bad = 1;
                         vote[c] = -1;
+                        atomicAdd(guard_trips + 2, 1);  // refactored in FP64 below
                     }
                 } else if (broken) {
                     bad = 1;
@@ -1289,6 +1314,7 @@ struct p3smi_topology {
     double mixed_rtol = 1e-12;    // refinement stops once |dx correction| <= rtol |dx|
     double mixed_switch = 1e-3;   // switching mode: FP32 factors while the mismatch is above this
     int mixed_kmax = 6;           // refinement steps before the FP64 fallback
+    int max_chunk = 0;            // one-call entry points: cases per chunk at most (0: sized to free HBM)
     hipStream_t stream = nullptr;
     std::unique_ptr<p3smi_batch> cached;  // workspace of the one-call entry points
     std::mutex mu;
@@ -1480,7 +1506,7 @@ void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     b->pstat.alloc(2 * S);
     for (auto* v : {&b->status, &b->iters, &b->trial}) v->alloc(S);
     b->counts.alloc(NSTAT);
-    b->guard.alloc(1);
+    b->guard.alloc(3);  // guard trips, FP32 and FP64 factorizations of the switching mode
     b->vote.alloc(S);
     b->rf.alloc(S);
     b->mixed_ready = false;
@@ -1624,10 +1650,16 @@ void linear_solve(p3smi_batch* b, int n_active, int precision) {
         return;
     }
     if (precision == P3SMI_PREC_MIXED_SWITCH) {
-        // FP32 factors and solve for the cases voting FP32, no refinement; FP64 for the others
-        // and for those whose FP32 pivots failed (vote -1), in the same iteration
+        // FP32 factors and solve for the 64-case groups voting FP32 alone, no refinement; FP64
+        // for the others and for the cases whose FP32 pivots failed (vote -1), in the same
+        // iteration. The factorization counts are kept on the device (guard[1], guard[2]).
         ensure_mixed(b);
-        b->stats.single_factorizations += n_active;  // launched over every active case
+        {
+            KTimer tm(b, 4);
+            hipLaunchKernelGGL(k_group_vote, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p,
+                               b->vote.p, b->guard.p, b->S);
+            HIPCHECK(hipGetLastError());
+        }
         launch_lu<float, true>(b, n_active, b->fac32.p, b->R.p, b->x.p, SEL_F32);
         launch_lu<double, true>(b, n_active, b->fac.p, b->R.p, b->x.p, SEL_F64);
         return;
@@ -1830,7 +1862,7 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
         KTimer tm(b, 4);
         hipLaunchKernelGGL(k_status_init, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p, b->B,
                            b->S);
-        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 1);
+        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 3);
         hipLaunchKernelGGL(k_clear_int, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->iters.p, b->S);
         HIPCHECK(hipGetLastError());
     }
@@ -1841,8 +1873,14 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
         cnt = newton_iteration(b, cnt[ST_ACTIVE], tol, line_search, precision, max_iter);
     }
     b->stats.failed_factorizations = cnt[ST_FAILED];
-    HIPCHECK(hipMemcpyAsync(&b->stats.guard_trips, b->guard.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    int g[3];
+    HIPCHECK(hipMemcpyAsync(g, b->guard.p, sizeof g, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    b->stats.guard_trips = g[0];
+    if (precision == P3SMI_PREC_MIXED_SWITCH) {
+        b->stats.single_factorizations = g[1];
+        b->stats.double_factorizations = g[2];
+    }
     collect_timing(b);
 }
 
@@ -1904,8 +1942,14 @@ int chunk_cases(p3smi_topology* t, int total, bool per_case_y, bool pin, bool pe
     size_t cap = (size_t)(0.6 * (double)freeb) / per;
     if (t->cached) cap += (size_t)t->cached->S;  // its memory is reusable
     if (cap < 64) cap = 64;
+    if (t->max_chunk > 0 && (size_t)t->max_chunk < cap) cap = (size_t)t->max_chunk;
     if ((size_t)total <= cap) return total;
-    return (int)(cap / 64 * 64);
+    // whole waves of cases, and chunks of (nearly) equal size: a small last chunk cannot fill
+    // the GPU yet costs a full chunk's iterations (as p3s/cuda/nr_polar_solver.py:630-637)
+    cap = cap / 64 * 64;
+    const size_t n_chunks = ((size_t)total + cap - 1) / cap;
+    const size_t even = ((size_t)total + n_chunks - 1) / n_chunks;
+    return (int)std::min(cap, (even + 63) / 64 * 64);
 }
 
 p3smi_batch* cached_batch(p3smi_topology* t, int cases) {
@@ -1994,6 +2038,16 @@ int p3smi_topology_set_mixed_switch(p3smi_topology* t, double mismatch) {
             throw Error(P3SMI_E_ARG, "mixed_switch: the switching mismatch must be finite and > 0");
         std::lock_guard<std::mutex> lk(t->mu);
         t->mixed_switch = mismatch;
+    });
+}
+
+int p3smi_topology_set_max_chunk(p3smi_topology* t, int32_t cases) {
+    return guarded([&] {
+        if (!t) throw Error(P3SMI_E_ARG, "set_max_chunk: null topology");
+        if (cases != 0 && (cases < 64 || cases % 64))
+            throw Error(P3SMI_E_ARG, "set_max_chunk: cases must be 0 or a positive multiple of 64");
+        std::lock_guard<std::mutex> lk(t->mu);
+        t->max_chunk = cases;
     });
 }
 
@@ -2220,6 +2274,28 @@ int p3smi_batch_bytes_per_case(const p3smi_batch* b, double* out4) {
 }
 
 // Returns the number of chunks the cases were solved in.
+static void add_stats(p3smi_stats& a, const p3smi_stats& s) {
+    a.cases += s.cases;
+    a.newton_passes += s.newton_passes;
+    a.line_search_passes += s.line_search_passes;
+    a.failed_factorizations += s.failed_factorizations;
+    a.guard_trips += s.guard_trips;
+    a.single_factorizations += s.single_factorizations;
+    a.double_factorizations += s.double_factorizations;
+    a.refinement_steps += s.refinement_steps;
+    a.ms_eval += s.ms_eval;
+    a.ms_lu += s.ms_lu;
+    a.ms_step += s.ms_step;
+    a.ms_decide += s.ms_decide;
+    a.ms_other += s.ms_other;
+    a.ms_refine += s.ms_refine;
+    a.n_eval += s.n_eval;
+    a.n_lu += s.n_lu;
+    a.n_step += s.n_step;
+    a.n_decide += s.n_decide;
+    a.n_refine += s.n_refine;
+}
+
 static int solve_chunked(p3smi_topology* t, int T, bool per_case_y, bool pin_given, bool per_case_s,
                          const std::function<void(p3smi_batch*, int, int)>& upload, int max_iter, double tol,
                          int line_search, int precision, double* V_out, int32_t* iters, uint8_t* conv,
@@ -2227,12 +2303,15 @@ static int solve_chunked(p3smi_topology* t, int T, bool per_case_y, bool pin_giv
     const p3smi::Topology& h = t->host;
     ensure_device(t);
     const int cs = chunk_cases(t, T, per_case_y, pin_given, per_case_s);
+    p3smi_stats total{};  // the counters of every chunk, left in the last chunk's batch
+    p3smi_batch* b = nullptr;
     for (int c0 = 0; c0 < T; c0 += cs) {
         const int nc = std::min(cs, T - c0);
-        p3smi_batch* b = cached_batch(t, nc);
+        b = cached_batch(t, nc);
         upload(b, c0, nc);
         b->uploaded = true;
         run_newton(b, max_iter, tol, line_search, precision);
+        add_stats(total, b->stats);
         std::vector<double> Vc(V_out && nc != T ? (size_t)2 * h.n * nc : 0);
         download(b, V_out ? (nc == T ? V_out : Vc.data()) : nullptr, iters ? iters + c0 : nullptr,
                  conv ? conv + c0 : nullptr, mismatch ? mismatch + c0 : nullptr);
@@ -2240,6 +2319,7 @@ static int solve_chunked(p3smi_topology* t, int T, bool per_case_y, bool pin_giv
             for (int i = 0; i < h.n; ++i)
                 std::memcpy(V_out + 2 * ((size_t)i * T + c0), Vc.data() + 2 * (size_t)i * nc, (size_t)nc * 16);
     }
+    b->stats = total;
     return (T + cs - 1) / cs;
 }
 
@@ -2670,7 +2750,7 @@ int p3smi_batch_slots_init(p3smi_batch* b) {
         b->mask_unserved = false;
         hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p, ST_IDLE,
                            b->S);
-        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 1);
+        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 3);
         HIPCHECK(hipGetLastError());
         std::memset(b->h_counts, 0, NSTAT * sizeof(int));
         b->h_counts[ST_IDLE] = b->S;

@@ -82,7 +82,7 @@ class Stats(C.Structure):
 EXPORTS = [
     "p3smi_last_error", "p3smi_version", "p3smi_device_count",
     "p3smi_topology_create", "p3smi_topology_free", "p3smi_topology_info", "p3smi_update_y",
-    "p3smi_topology_set_lu_shape", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_mixed",
+    "p3smi_topology_set_lu_shape", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_max_chunk", "p3smi_topology_set_mixed",
     "p3smi_host_jacobian", "p3smi_host_factor_solve", "p3smi_host_maps",
     "p3smi_solve_batch", "p3smi_solve_batch_contingency", "p3smi_solve_batch_contingency_stamp",
     "p3smi_outages_branch_flows", "p3smi_outages_voltage_violations",
@@ -137,6 +137,7 @@ def lib():
     L.p3smi_batch_download_sbus.argtypes = [P, P]
     L.p3smi_topology_release_cache.argtypes = [P]
     L.p3smi_topology_last_batch.argtypes = [P, P]
+    L.p3smi_topology_set_max_chunk.argtypes = [P, _i32]
     L.p3smi_batch_vrange.argtypes = [P, P, P]
     L.p3smi_batch_stats.argtypes = [P, P]
     L.p3smi_batch_set_graphs.argtypes = [P, C.c_int32]

@@ -73,6 +73,16 @@ class Solver:
             lib().p3smi_topology_free(h)
             self._h = None
 
+    def _fill_stats(self):
+        """Counters of the last one-call solve (the reference's ``last_stats``,
+        p3s/cuda/nr_polar_solver.py:671-681): Newton and line-search passes, FP32 / FP64
+        factorizations, guard trips; summed over the chunks of a split call."""
+        bh = C.c_void_p()
+        check(lib().p3smi_topology_last_batch(self._h, C.byref(bh)))
+        st = _lib.Stats()
+        check(lib().p3smi_batch_stats(bh, C.byref(st)))
+        self.last_stats = {f: getattr(st, f) for f, _ in st._fields_ if not f.startswith("reserved")}
+
     def _info(self) -> dict:
         inf = _lib.Info()
         check(lib().p3smi_topology_info(self._h, C.byref(inf)))
@@ -84,6 +94,12 @@ class Solver:
         check(lib().p3smi_topology_set_lu_shape(self._h, int(g), int(ns), int(few_g), int(few_ns), int(few_max),
                                                 int(group)))
         self.info = self._info()
+
+    def set_max_chunk(self, cases: int = 0):
+        """Solve at most ``cases`` cases at once (a positive multiple of 64; 0: as many as 60%
+        of the free HBM holds); larger calls run as nearly equal chunks one after the other
+        (the reference GPU solver's ``max_chunk``, p3s/cuda/nr_polar_solver.py:311, 619-637)."""
+        check(lib().p3smi_topology_set_max_chunk(self._h, int(cases)))
 
     def set_graphs(self, on: bool = True):
         """Replay each FP64 Newton iteration of later solves as a captured hipGraph."""
@@ -144,6 +160,7 @@ class Solver:
         mis = np.empty(T, dtype=np.float64)
         check(lib().p3smi_solve_batch(self._h, ptr(S), ptr(V0), int(per_col), T, int(max_iter), float(tol),
                                       int(bool(line_search)), _prec(precision), ptr(V), ptr(it), ptr(cv), ptr(mis)))
+        self._fill_stats()
         return {"V": V, "iterations": it, "converged": cv.astype(bool), "mismatch": mis}
 
     def _check_contingency(self, L, Sbus, V0, pin, who="solve_batch_contingency"):
@@ -179,6 +196,7 @@ class Solver:
         check(lib().p3smi_solve_batch_contingency(self._h, ptr(Ym), ptr(Ya), ptr(S), ptr(V0), ptr(pin), L,
                                                   int(max_iter), float(tol), int(bool(line_search)), _prec(precision),
                                                   ptr(V), ptr(it), ptr(cv), ptr(mis)))
+        self._fill_stats()
         return {"V": V, "iterations": it, "converged": cv.astype(bool), "mismatch": mis}
 
     def solve_batch_contingency_cx(self, Yx, Sbus, V0, pin, max_iter: int = 30, tol: float = 1e-8,
@@ -209,6 +227,7 @@ class Solver:
                                                         ptr(pin), L, int(max_iter), float(tol),
                                                         int(bool(line_search)), _prec(precision), ptr(V), ptr(it),
                                                         ptr(cv), ptr(mis)))
+        self._fill_stats()
         return {"V": V, "iterations": it, "converged": cv.astype(bool), "mismatch": mis}
 
     # ---------------------------------------------------------------- host checks

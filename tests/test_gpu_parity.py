@@ -154,7 +154,8 @@ def test_switching_mixed_mode(gpu):
             S, V0 = grids.timeseries_scaled(g, T, seed=1), g.flat_start()
         else:
             S, V0 = grids.timeseries_daily(g, T, seed=1), dc_initial_voltage(g)
-        r = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8, precision="mixed_switch")
+        solver = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+        r = solver.solve_batch(S, V0, max_iter=20, tol=1e-8, precision="mixed_switch")
         ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
         assert np.array_equal(r["converged"], ref["converged"]), name
         ok = ref["converged"]
@@ -162,4 +163,33 @@ def test_switching_mixed_mode(gpu):
         dva = np.abs(np.angle(r["V"][:, ok]) - np.angle(ref["V"][:, ok])).max()
         assert dvm <= TOL_V and dva <= TOL_V, (name, dvm, dva)
         assert (r["mismatch"][ok] < 1e-8).all()
-        print(f"{name}: iterations equal in {np.mean(r['iterations'] == ref['iterations']):.3f} of the cases")
+        # one factorization per case and iteration; FP32 pivot failures add an FP64 one
+        st = solver.last_stats
+        assert st["single_factorizations"] > 0, st
+        assert st["single_factorizations"] + st["double_factorizations"] >= r["iterations"].sum(), st
+        print(f"{name}: iterations equal in {np.mean(r['iterations'] == ref['iterations']):.3f} of the cases, "
+              f"{st['single_factorizations']} FP32 / {st['double_factorizations']} FP64 factorizations")
+
+
+def test_last_stats_count_one_factorization_per_iteration(gpu):
+    """FP64: every active case is factored once per Newton iteration, so the solve's FP64
+    factorization count equals the sum of the iteration counts (also over a call that the
+    workspace limit splits: the counters of every chunk are summed)."""
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    S = grids.timeseries_scaled(g, 300, seed=5)
+    solver = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    r = solver.solve_batch(S, g.flat_start(), max_iter=20, tol=1e-8)
+    st = solver.last_stats
+    assert st["cases"] == 300
+    assert st["double_factorizations"] == r["iterations"].sum(), st
+    assert st["single_factorizations"] == 0
+    # the same call in chunks of at most 128 cases (128, 128, 44): identical bits, summed counters
+    solver.set_max_chunk(128)
+    rc = solver.solve_batch(S, g.flat_start(), max_iter=20, tol=1e-8)
+    solver.set_max_chunk(0)
+    assert np.array_equal(rc["V"].view(np.float64), r["V"].view(np.float64))
+    assert np.array_equal(rc["iterations"], r["iterations"])
+    assert np.array_equal(rc["mismatch"], r["mismatch"])
+    assert solver.last_stats["cases"] == 300
+    assert solver.last_stats["double_factorizations"] == st["double_factorizations"]

@@ -110,6 +110,11 @@ def test_invalid_inputs_are_rejected():
         s.solve_batch(np.zeros((g.n, 3), complex), np.ones((g.n, 2), complex))
     with pytest.raises(ValueError, match="precision"):
         s.solve_batch(np.zeros((g.n, 3), complex), g.flat_start(), precision="half")
+    for bad_chunk in (-64, 32, 100):
+        with pytest.raises(RuntimeError, match="set_max_chunk: cases must be 0 or a positive multiple of 64"):
+            s.set_max_chunk(bad_chunk)
+    s.set_max_chunk(128)
+    s.set_max_chunk(0)
 
 
 def test_update_y_refreshes_reference_pivot_ratio():
