@@ -499,25 +499,38 @@ __device__ __forceinline__ void seg_stream(const int4* __restrict__ tasks, int q
         }
     };
     constexpr int K = lu_chunk<TA, TB>();
-    int e = tk.z;
-    TA a[K];
-    TB b[K];
-    if (e < eend) load_chunk<K>(st, e, eend, A, Bv, S, c, a, b);
-    for (; e < eend; e += K) {
-        TA a2[K];
-        TB b2[K];
-        if (e + K < eend) load_chunk<K>(st, e + K, eend, A, Bv, S, c, a2, b2);
+    auto consume = [&](int e, const TA (&a)[K], const TB (&b)[K]) {
 #pragma unroll
         for (int u = 0; u < K; ++u) {
             const int pos = e + u;
             while (q < q1 && pos == tk.w) advance();
             if (pos < eend) acc = fma(-(ACC)a[u], (ACC)b[u], acc);
         }
-#pragma unroll
-        for (int u = 0; u < K; ++u) {
-            a[u] = a2[u];
-            b[u] = b2[u];
+    };
+    // Two chunk buffers in turn, the loop unrolled by two: while a next chunk exists it is
+    // loaded before the current one is consumed, so it is in flight meanwhile, and no register
+    // is copied while a load into it may still be pending (which would make the wave wait for
+    // every load in flight). No chunk past the end of the stream is loaded.
+    int e = tk.z;
+    if (e < eend) {
+        TA a0[K], a1[K];
+        TB b0[K], b1[K];
+        load_chunk<K>(st, e, eend, A, Bv, S, c, a0, b0);
+        bool last0 = true;  // the chunk in buffer 0 is the last one
+        while (e + K < eend) {
+            load_chunk<K>(st, e + K, eend, A, Bv, S, c, a1, b1);
+            consume(e, a0, b0);
+            e += K;
+            if (e + K >= eend) {
+                last0 = false;
+                break;
+            }
+            load_chunk<K>(st, e + K, eend, A, Bv, S, c, a0, b0);
+            consume(e, a1, b1);
+            e += K;
         }
+        if (last0) consume(e, a0, b0);
+        else consume(e, a1, b1);
     }
     while (q < q1) advance();
 }
