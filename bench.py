@@ -409,9 +409,24 @@ def run_n1(args) -> dict:
                             "overload_screen_ms": round(1e3 * t_over, 3), "voltage_violations": len(vv),
                             "voltage_screen_ms": round(1e3 * t_vv, 3)},
     })
+    out["one_outage_at_a_time_fp64"] = one_at_a_time_n1(cs, S, Vs)
     if not args.no_cpu:
         cpu_n1(args, g, cs, Vs, V, it_main, conv, served.astype(bool), out)
     return out
+
+
+def one_at_a_time_n1(cs, S, Vs, k=32):
+    """The plain FP64 GPU form without batching: the first k outages solved one per call
+    (each a one-case outage set, built outside the timed region)."""
+    from p3smi.contingency import ContingencySet
+
+    subs = [ContingencySet(cs.solver, cs.f, cs.t, *cs.yval.T, [cs.groups[j]], cs.ref, cs.gen_z, cs.reslack)
+            for j in range(k)]
+    subs[0].solve(S, Vs, max_iter=MAX_ITER, tol=TOL)  # warm-up
+    t0 = time.perf_counter()
+    conv = sum(int(sub.solve(S, Vs, max_iter=MAX_ITER, tol=TOL).converged.sum()) for sub in subs)
+    wall = time.perf_counter() - t0
+    return {"sample": k, "value": round(conv / wall, 1), "ms_per_case": round(1e3 * wall / k, 3), "converged": conv}
 
 
 def cpu_n1(args, g, cs, Vs, V, it, conv, served, out):
