@@ -1359,6 +1359,7 @@ struct p3smi_batch {
     DevBuf<unsigned char> st_cv;
     DevBuf<double> st_mis;
     bool timing = false;
+    bool slot_mode = false;             // driven by slots_init / load / pass (continuous batching)
     int precision = 0;                  // precision of the solve in progress
     unsigned long long solve_id = 0;    // stamp of the last run_newton on this batch
     bool sbus_built = false;            // Ssp holds injections built on the device (k_sbus)
@@ -2150,6 +2151,7 @@ int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_
         b->mask_unserved = false;
         HIPCHECK(hipStreamSynchronize(st));
         b->uploaded = true;
+        b->slot_mode = false;
     });
 }
 
@@ -2243,8 +2245,21 @@ int p3smi_batch_vrange(p3smi_batch* b, double* vmin, double* vmax) {
 }
 
 int p3smi_batch_stats(const p3smi_batch* b, p3smi_stats* out) {
-    *out = b->stats;
-    return P3SMI_OK;
+    return guarded([&] {
+        *out = b->stats;
+        if (b->slot_mode) {
+            // passes of a slot batch do not read the device counters back: read them here
+            // (guard trips; FP32 and FP64 factorizations of the switching mode) since slots_init
+            int g[3];
+            HIPCHECK(hipMemcpyAsync(g, b->guard.p, sizeof g, hipMemcpyDeviceToHost, b->topo->stream));
+            HIPCHECK(hipStreamSynchronize(b->topo->stream));
+            out->guard_trips = g[0];
+            if (b->precision == P3SMI_PREC_MIXED_SWITCH) {
+                out->single_factorizations = g[1];
+                out->double_factorizations = g[2];
+            }
+        }
+    });
 }
 
 int p3smi_topology_set_graphs(p3smi_topology* t, int32_t on) {
@@ -2323,6 +2338,7 @@ static int solve_chunked(p3smi_topology* t, int T, bool per_case_y, bool pin_giv
         b = cached_batch(t, nc);
         upload(b, c0, nc);
         b->uploaded = true;
+        b->slot_mode = false;
         run_newton(b, max_iter, tol, line_search, precision);
         add_stats(total, b->stats);
         std::vector<double> Vc(V_out && nc != T ? (size_t)2 * h.n * nc : 0);
@@ -2761,6 +2777,7 @@ int p3smi_batch_slots_init(p3smi_batch* b) {
         b->per_case_y = false;
         b->has_pin = false;
         b->mask_unserved = false;
+        b->slot_mode = true;
         hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p, ST_IDLE,
                            b->S);
         hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 3);

@@ -91,3 +91,48 @@ def test_n1_requests_and_pool_growth(gpu):
         np.testing.assert_array_equal(res["served"], alone.served)
         np.testing.assert_array_equal(res["iterations"], alone.iterations)
         np.testing.assert_array_equal(res["V"].view(np.float64), alone.V.view(np.float64))
+
+
+def test_slot_batch_stats_count_factorizations_in_switching_mode(gpu):
+    """A slot batch (continuous batching) driven pass by pass in the switching mixed mode
+    reports its FP32 and FP64 factorizations, counted on the device: one per active case and
+    Newton pass, plus an FP64 one for every FP32 pivot failure."""
+    import ctypes as C
+
+    from p3smi import _lib
+
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    solver = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    L = _lib.lib()
+    k = 40
+    S = np.ascontiguousarray(grids.timeseries_scaled(g, k, seed=9), dtype=np.complex128)
+    V0 = np.ascontiguousarray(np.repeat(g.flat_start()[:, None], k, axis=1), dtype=np.complex128)
+    bh = C.c_void_p()
+    _lib.check(L.p3smi_batch_create(solver._h, 64, C.byref(bh)))
+    try:
+        _lib.check(L.p3smi_batch_slots_init(bh))
+        slots = np.arange(k, dtype=np.int32)
+        _lib.check(L.p3smi_batch_load(bh, k, _lib.ptr(slots), _lib.ptr(S), _lib.ptr(V0)))
+        counts = np.zeros(8, dtype=np.int32)
+        for _ in range(30):
+            _lib.check(L.p3smi_batch_pass(bh, 20, 1e-8, 1, 2, _lib.ptr(counts)))
+            if counts[0] == 0 and counts[3] == 0 and counts[4] == 0 and counts[7] == 0:
+                break
+        buf = np.zeros(64, dtype=np.int32)
+        nfin = C.c_int32()
+        _lib.check(L.p3smi_batch_finished(bh, _lib.ptr(buf), C.byref(nfin)))
+        assert nfin.value == k
+        V = np.empty((g.n, k), dtype=np.complex128)
+        it = np.empty(k, dtype=np.int32)
+        cv = np.empty(k, dtype=np.uint8)
+        mis = np.empty(k, dtype=np.float64)
+        fin = np.ascontiguousarray(buf[:k])
+        _lib.check(L.p3smi_batch_fetch(bh, k, _lib.ptr(fin), _lib.ptr(V), _lib.ptr(it), _lib.ptr(cv), _lib.ptr(mis)))
+        assert cv.all()
+        st = _lib.Stats()
+        _lib.check(L.p3smi_batch_stats(bh, C.byref(st)))
+        assert st.single_factorizations > 0
+        assert st.single_factorizations + st.double_factorizations >= int(it.sum())
+    finally:
+        L.p3smi_batch_free(bh)
