@@ -647,6 +647,260 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     }
 }
 
+// ---- 2x2 supernode blocks, G = 64 ----------------------------------------------------
+// A block task (3 int4: {kind | nr << 8 | nc << 16, -, sbeg, send}, {tgt0, tgt1, aux0,
+// aux1}, {src00, src01, src10, src11}) updates up to four entries (rows R x columns C of
+// a pivot group's diagonal, U or L block) from one step stream: a step loads L(R, t) and
+// U(t, C) -- the rows at the step's positions and, for a twin row or column, the next
+// position -- and makes four multiply-adds. A block without a second row (column) skips
+// that load (the step's flags are the same in every lane) and ignores the extra sums. The terms of t = p for the second row or
+// column of a pivot pair are added in the finish, after the shared ones, so every entry
+// sums in ascending pivot order and the factors equal those of k_lu bit for bit.
+struct BlkIn { double j[4]; double x[3]; };
+
+// Steps per chunk: four row loads per step, 128 bytes in flight per lane and chunk buffer.
+// With 8 wavefronts per workgroup (256 VGPRs) this fits without spilling; with 16 (128
+// VGPRs) the start values of two tasks do not, so only NS = 8 runs the block schedule.
+template <typename real>
+constexpr int blk_chunk() { return sizeof(real) == 4 ? 16 : 8; }
+
+template <int K, typename real>
+__device__ __forceinline__ void load_steps(const int2* __restrict__ st, int e, int eend, const real* __restrict__ fac,
+                                           int S, int c, real (&a0)[K], real (&a1)[K], real (&b0)[K], real (&b1)[K]) {
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+        const int2 p = st[min(e + u, eend - 1)];  // past the end: repeat the last step (unused)
+        const int pa = p.x & (p3smi::BLK_TWIN - 1), pb = p.y & (p3smi::BLK_TWIN - 1);
+        a0[u] = fac[at(pa, S, c)];
+        b0[u] = fac[at(pb, S, c)];
+        if (p.x & p3smi::BLK_TWIN) a1[u] = fac[at(pa + 1, S, c)];  // the step's flags are uniform
+        if (p.y & p3smi::BLK_TWIN) b1[u] = fac[at(pb + 1, S, c)];
+    }
+}
+
+// Block tasks [q0, q1) of one list (steps back to back); pivot statistics of the diagonal
+// blocks go to (pmin, pmax, psum).
+template <int K, typename real>
+__device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q0, int q1, const int2* __restrict__ st,
+                                           const double* __restrict__ Jx, real* __restrict__ fac, int S, int c,
+                                           double& pmin, double& pmax, double& psum) {
+    if (q0 >= q1) return;
+    using ACC = real;
+    auto start = [&](int q) -> BlkIn {
+        const int4 w0 = tasks[3 * q], w1 = tasks[3 * q + 1], w2 = tasks[3 * q + 2];
+        const int kind = w0.x & 255;
+        BlkIn in;
+        in.j[0] = w2.x >= 0 ? Jx[at(w2.x, S, c)] : 0.0;
+        in.j[1] = w2.y >= 0 ? Jx[at(w2.y, S, c)] : 0.0;
+        in.j[2] = w2.z >= 0 ? Jx[at(w2.z, S, c)] : 0.0;
+        in.j[3] = w2.w >= 0 ? Jx[at(w2.w, S, c)] : 0.0;
+        in.x[0] = in.x[1] = in.x[2] = 0.0;
+        if (kind >= p3smi::BLK_U2) in.x[0] = (double)fac[at(w1.z, S, c)];  // L(p+1,p) or 1/U(p,p)
+        if (kind == p3smi::BLK_L2) {
+            in.x[1] = (double)fac[at(w1.z + 1, S, c)];  // U(p, p+1)
+            in.x[2] = (double)fac[at(w1.w, S, c)];      // 1/U(p+1, p+1)
+        }
+        return in;
+    };
+    auto pivot = [&](real v) -> real {
+        const double ab = fabs((double)v);
+        pmin = fmin(pmin, ab);
+        pmax = fmax(pmax, ab);
+        psum += ab;
+        return (real)1 / v;
+    };
+    auto finish = [&](int q, const BlkIn& in, const ACC (&acc)[4]) {
+        const int4 w0 = tasks[3 * q], w1 = tasks[3 * q + 1];
+        const int kind = w0.x & 255, nr = (w0.x >> 8) & 255, nc = w0.x >> 16;
+        switch (kind) {
+            case p3smi::BLK_DIAG1:
+                fac[at(w1.x, S, c)] = pivot((real)in.j[0] + acc[0]);
+                break;
+            case p3smi::BLK_DIAG2: {
+                const real inv0 = pivot((real)in.j[0] + acc[0]);
+                const real u01 = (real)in.j[1] + acc[1];
+                const real l10 = ((real)in.j[2] + acc[2]) * inv0;
+                fac[at(w1.x, S, c)] = inv0;
+                fac[at(w1.z, S, c)] = u01;
+                fac[at(w1.w, S, c)] = l10;
+                fac[at(w1.y, S, c)] = pivot((real)in.j[3] + fma(-(ACC)l10, (ACC)u01, acc[3]));
+                break;
+            }
+            case p3smi::BLK_U1:
+                fac[at(w1.x, S, c)] = (real)in.j[0] + acc[0];
+                if (nc == 2) fac[at(w1.x + 1, S, c)] = (real)in.j[1] + acc[1];
+                break;
+            case p3smi::BLK_U2: {
+                const real l10 = (real)in.x[0];
+                const real u00 = (real)in.j[0] + acc[0];
+                fac[at(w1.x, S, c)] = u00;
+                fac[at(w1.y, S, c)] = (real)in.j[2] + fma(-(ACC)l10, (ACC)u00, acc[2]);
+                if (nc == 2) {
+                    const real u01 = (real)in.j[1] + acc[1];
+                    fac[at(w1.x + 1, S, c)] = u01;
+                    fac[at(w1.y + 1, S, c)] = (real)in.j[3] + fma(-(ACC)l10, (ACC)u01, acc[3]);
+                }
+                break;
+            }
+            case p3smi::BLK_L1: {
+                const real inv = (real)in.x[0];
+                fac[at(w1.x, S, c)] = ((real)in.j[0] + acc[0]) * inv;
+                if (nr == 2) fac[at(w1.x + 1, S, c)] = ((real)in.j[2] + acc[2]) * inv;
+                break;
+            }
+            default: {  // BLK_L2
+                const real inv0 = (real)in.x[0], u01 = (real)in.x[1], inv1 = (real)in.x[2];
+                const real l00 = ((real)in.j[0] + acc[0]) * inv0;
+                fac[at(w1.x, S, c)] = l00;
+                fac[at(w1.y, S, c)] = ((real)in.j[1] + fma(-(ACC)l00, (ACC)u01, acc[1])) * inv1;
+                if (nr == 2) {
+                    const real l10 = ((real)in.j[2] + acc[2]) * inv0;
+                    fac[at(w1.x + 1, S, c)] = l10;
+                    fac[at(w1.y + 1, S, c)] = ((real)in.j[3] + fma(-(ACC)l10, (ACC)u01, acc[3])) * inv1;
+                }
+                break;
+            }
+        }
+    };
+    int q = q0;
+    int4 tk = tasks[3 * q];
+    BlkIn in = start(q);
+    BlkIn inn = in;
+    if (q + 1 < q1) inn = start(q + 1);
+    const int eend = tasks[3 * (q1 - 1)].w;
+    ACC acc[4] = {0, 0, 0, 0};
+    auto advance = [&]() {
+        finish(q, in, acc);
+        acc[0] = acc[1] = acc[2] = acc[3] = 0;
+        ++q;
+        if (q < q1) {
+            tk = tasks[3 * q];
+            in = inn;
+            if (q + 1 < q1) inn = start(q + 1);
+        }
+    };
+    auto consume = [&](int e, const real (&a0)[K], const real (&a1)[K], const real (&b0)[K], const real (&b1)[K]) {
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            const int pos = e + u;
+            while (q < q1 && pos == tk.w) advance();
+            if (pos < eend) {
+                acc[0] = fma(-(ACC)a0[u], (ACC)b0[u], acc[0]);
+                acc[1] = fma(-(ACC)a0[u], (ACC)b1[u], acc[1]);
+                acc[2] = fma(-(ACC)a1[u], (ACC)b0[u], acc[2]);
+                acc[3] = fma(-(ACC)a1[u], (ACC)b1[u], acc[3]);
+            }
+        }
+    };
+    // two chunk buffers in turn, as in seg_stream
+    int e = tk.z;
+    if (e < eend) {
+        real a0[K], a1[K], b0[K], b1[K], c0[K], c1[K], d0[K], d1[K];
+#pragma unroll
+        for (int u = 0; u < K; ++u) a1[u] = b1[u] = c1[u] = d1[u] = 0;  // second rows absent: stale, unused
+        load_steps<K>(st, e, eend, fac, S, c, a0, a1, b0, b1);
+        bool last0 = true;
+        while (e + K < eend) {
+            load_steps<K>(st, e + K, eend, fac, S, c, c0, c1, d0, d1);
+            consume(e, a0, a1, b0, b1);
+            e += K;
+            if (e + K >= eend) {
+                last0 = false;
+                break;
+            }
+            load_steps<K>(st, e + K, eend, fac, S, c, a0, a1, b0, b1);
+            consume(e, c0, c1, d0, d1);
+            e += K;
+        }
+        if (last0) consume(e, a0, a1, b0, b1);
+        else consume(e, c0, c1, d0, d1);
+    }
+    while (q < q1) advance();
+}
+
+// k_lu_stream with the 2x2-block schedule (G = 64): same contract and, bit for bit, the
+// same factors and solutions. Per level: sub-level A = diagonal blocks and U blocks of
+// single pivots; sub-level B = the U blocks of pivot pairs, the L blocks and the forward
+// rows (which read L(p+1, p), written by the diagonal block of a pair in sub-level A).
+template <int NS, typename real, bool FACTOR>
+__global__ __launch_bounds__(64 * NS) void k_lu_blk(
+    const int* __restrict__ ptrA, const int4* __restrict__ blkA, const int* __restrict__ ptrB,
+    const int4* __restrict__ blkB, const int2* __restrict__ steps, const int* __restrict__ ptrF,
+    const int4* __restrict__ fwd, const int2* __restrict__ fent, const int* __restrict__ ptrW,
+    const int4* __restrict__ bwd, const int2* __restrict__ bent, const double* __restrict__ Jx,
+    real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x, int* __restrict__ status,
+    int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips, int* __restrict__ vote,
+    const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S) {
+    constexpr bool SINGLE = sizeof(real) == 4;
+    constexpr int G = 64, TL = NS;
+    __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
+    __shared__ int sbad[G];
+    const int lane = threadIdx.x % G;
+    const int slot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G));
+    const int c = blockIdx.x * G + lane;
+    bool act = selected(sel, c, status, vote, rf);
+    if (!__syncthreads_or(act)) return;
+
+    double pmin = DBL_MAX, pmax = 0.0, psum = 0.0;
+    auto startF = [&](int, int4 t) -> In1 { return {rhs[at(t.y, S, c)]}; };
+    auto finishF = [&](int4 t, In1 in, double sum) { x[at(t.x, S, c)] = in.v + sum; };
+    auto startW = [&](int, int4 t) -> In2 { return {x[at(t.x, S, c)], (double)fac[at(t.y, S, c)]}; };
+    auto finishW = [&](int4 t, In2 in, double sum) { x[at(t.x, S, c)] = (in.v + sum) * in.d; };
+
+    for (int l = 0; l < nlev; ++l) {
+        const int i0 = l * TL + slot;
+        if (FACTOR) {
+            if (act) blk_stream<blk_chunk<real>(), real>(blkA, ptrA[i0], ptrA[i0 + 1], steps, Jx, fac, S, c, pmin, pmax, psum);
+            __syncthreads();
+        }
+        if (act) {
+            if (FACTOR) blk_stream<blk_chunk<real>(), real>(blkB, ptrB[i0], ptrB[i0 + 1], steps, Jx, fac, S, c, pmin, pmax, psum);
+            seg_stream<double, In1>(fwd, ptrF[i0], ptrF[i0 + 1], fent, fac, x, S, c, startF, finishF);
+        }
+        __syncthreads();
+    }
+    if (FACTOR) {
+        smin[slot][lane] = pmin;
+        smax[slot][lane] = pmax;
+        ssum[slot][lane] = psum;
+        __syncthreads();
+        if (slot == 0) {
+            int bad = 0;
+            if (act) {
+                for (int s2 = 1; s2 < TL; ++s2) {
+                    pmin = fmin(pmin, smin[s2][lane]);
+                    pmax = fmax(pmax, smax[s2][lane]);
+                    psum += ssum[s2][lane];
+                }
+                pstat[2 * (size_t)c] = pmin;
+                pstat[2 * (size_t)c + 1] = pmax;
+                const bool broken = !(psum < DBL_MAX && pmin > 0.0);
+                if (SINGLE) {
+                    if (broken || pmin < rcond_min * pmax) {
+                        bad = 1;
+                        vote[c] = -1;
+                        atomicAdd(guard_trips + 2, 1);  // refactored in FP64 below
+                    }
+                } else if (broken) {
+                    bad = 1;
+                    status[c] = ST_FAILED;
+                    iters[c] += 1;
+                } else if (pmin < rcond_min * pmax) {
+                    atomicAdd(guard_trips, 1);
+                }
+            }
+            sbad[lane] = bad;
+        }
+        __syncthreads();
+        if (sbad[lane]) act = false;
+    }
+    for (int l = 0; l < nblev; ++l) {
+        const int i0 = l * TL + slot;
+        if (act) seg_stream<double, In2>(bwd, ptrW[i0], ptrW[i0 + 1], bent, fac, x, S, c, startW, finishW);
+        __syncthreads();
+    }
+}
+
 // Refinement residual r = R - J x in FP64 (J in equation-row CSR order, x in pivot order:
 // xcol[e] = pivot index of the column of entry e), one wavefront per row and 64 cases.
 __global__ __launch_bounds__(256) void k_resid(const int* __restrict__ Jp, const int* __restrict__ xcol,
@@ -1297,6 +1551,7 @@ struct DevBuf {
 
 struct DevSched {  // one level schedule dealt to TL task lists
     int TL = 0, nlev = 0, nblev = 0;
+    bool blk = false;  // 2x2-block schedule: taskA / taskB hold 3 int4 per block, pairs the steps
     DevBuf<int> ptrA, ptrB, ptrF, ptrW, diagB;
     DevBuf<int4> taskA, taskB, fwd, bwd;
     DevBuf<int2> pairs, fent, bent;  // operand streams in task-list order
@@ -1309,10 +1564,14 @@ struct DevTopo {
     bool ready = false;
 };
 
+// LU launch shape: G cases per workgroup, NS wavefronts; grp = 0: 2x2 supernode blocks
+// when G = 64 and NS = 8 (k_lu_blk), single entries otherwise; grp > 0: runs of up to grp entries of
+// one pivot dealt together (1 = single entries).
 struct LuConfig {
     int G, NS, grp;
     int tl() const { return 64 * NS / G; }
-    int key() const { return tl() * 1024 + grp; }
+    bool blk() const { return G == 64 && NS == 8 && grp == 0; }
+    int key() const { return (tl() * 8192 + grp) * 2 + (blk() ? 1 : 0); }
 };
 
 }  // namespace
@@ -1434,11 +1693,94 @@ void ensure_device(p3smi_topology* t) {
 
 // Level schedule for a launch shape (TL task lists, dealing group), built and uploaded on
 // first use.
+// The 2x2-block schedule on the device. Every position a block task or step names is
+// checked against the factor and Jacobian sizes here, before any kernel reads it.
+std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
+    const p3smi::Topology& h = t->host;
+    const p3smi::BlockSchedule s = p3smi::make_block_schedule(h, TL);
+    const int64_t nlu = h.nnz_lu, nj = h.nnzJ;
+    auto bad = [](const char* what) { throw Error(P3SMI_E_INTERNAL, std::string("block schedule: ") + what); };
+    const int64_t nsteps = (int64_t)s.steps.size() / 2;
+    for (int64_t k = 0; k < nsteps; ++k)
+        for (int w = 0; w < 2; ++w) {
+            const int v = s.steps[2 * k + w];
+            const int64_t pos = v & (p3smi::BLK_TWIN - 1);
+            if (v < 0 || pos + ((unsigned)v >> 30) >= nlu) bad("step position out of range");
+        }
+    auto pack = [&](const std::vector<p3smi::BlockTask>& ts, const std::vector<int>& ptr) {
+        std::vector<int4> out(3 * ts.size());
+        for (size_t i = 0; i < ts.size(); ++i) {
+            const p3smi::BlockTask& k = ts[i];
+            if (k.kind < p3smi::BLK_DIAG1 || k.kind > p3smi::BLK_L2 || k.nr < 1 || k.nr > 2 || k.nc < 1 || k.nc > 2)
+                bad("task shape");
+            if (k.sbeg < 0 || k.send < k.sbeg || k.send > nsteps) bad("step range");
+            const bool two = k.kind == p3smi::BLK_DIAG2 || k.kind == p3smi::BLK_U2 || k.kind == p3smi::BLK_L2;
+            const int ext = (k.kind == p3smi::BLK_U1 || k.kind == p3smi::BLK_U2) ? k.nc - 1
+                            : (k.kind == p3smi::BLK_L1 || k.kind == p3smi::BLK_L2) ? k.nr - 1 : 0;
+            if (k.tgt0 < 0 || k.tgt0 + ext >= nlu) bad("target position");
+            if (two && (k.tgt1 < 0 || k.tgt1 + ext >= nlu)) bad("target position");
+            if (k.kind == p3smi::BLK_DIAG2 && (k.aux0 < 0 || k.aux0 >= nlu || k.aux1 < 0 || k.aux1 >= nlu)) bad("aux");
+            if (k.kind >= p3smi::BLK_U2 && (k.aux0 < 0 || k.aux0 + 1 >= nlu)) bad("aux");
+            if (k.kind == p3smi::BLK_L2 && (k.aux1 < 0 || k.aux1 >= nlu)) bad("aux");
+            for (int j = 0; j < 4; ++j)
+                if (k.src[j] >= nj) bad("Jacobian source");
+            out[3 * i] = make_int4(k.kind | (k.nr << 8) | (k.nc << 16), 0, k.sbeg, k.send);
+            out[3 * i + 1] = make_int4(k.tgt0, k.tgt1, k.aux0, k.aux1);
+            out[3 * i + 2] = make_int4(k.src[0], k.src[1], k.src[2], k.src[3]);
+        }
+        // a list's tasks must have their steps back to back
+        for (size_t r = 0; r + 1 < ptr.size(); ++r)
+            for (int q = ptr[r]; q + 1 < ptr[r + 1]; ++q)
+                if (ts[q].send != ts[q + 1].sbeg) bad("steps not contiguous");
+        if (out.empty()) out.push_back(make_int4(0, 0, 0, 0));
+        return out;
+    };
+    const std::vector<int4> ta = pack(s.taskA, s.ptrA), tb = pack(s.taskB, s.ptrB);
+    std::unique_ptr<DevSched> d(new DevSched());
+    hipStream_t st = t->stream;
+    d->TL = TL;
+    d->nlev = s.nlev;
+    d->nblev = s.nblev;
+    d->blk = true;
+    d->ptrA.upload(s.ptrA.data(), s.ptrA.size(), st);
+    d->ptrB.upload(s.ptrB.data(), s.ptrB.size(), st);
+    d->ptrF.upload(s.ptrF.data(), s.ptrF.size(), st);
+    d->ptrW.upload(s.ptrW.data(), s.ptrW.size(), st);
+    d->diagB.alloc(1);
+    d->taskA.upload(ta.data(), ta.size(), st);
+    d->taskB.upload(tb.data(), tb.size(), st);
+    std::vector<int2> pr(std::max<int64_t>(nsteps, 1), make_int2(0, 0)), fe, be;
+    for (int64_t k = 0; k < nsteps; ++k) pr[k] = make_int2(s.steps[2 * k], s.steps[2 * k + 1]);
+    std::vector<int4> fw(s.fwd.size()), bw(s.bwd.size());
+    for (size_t i = 0; i < fw.size(); ++i) {
+        const auto& f = s.fwd[i];
+        const int b0 = (int)fe.size();
+        for (int e = f.ebeg; e < f.eend; ++e) fe.push_back(make_int2(h.Fpos[e], h.Fcol[e]));
+        fw[i] = make_int4(f.row, f.rsrc, b0, (int)fe.size());
+    }
+    for (size_t i = 0; i < bw.size(); ++i) {
+        const auto& w = s.bwd[i];
+        const int b0 = (int)be.size();
+        for (int e = w.lbeg; e < w.lend; ++e) be.push_back(make_int2(w.base + 1 + (e - w.lbeg), h.Li[e]));
+        bw[i] = make_int4(w.row, w.base, b0, (int)be.size());
+    }
+    if (fe.empty()) fe.push_back(make_int2(0, 0));
+    if (be.empty()) be.push_back(make_int2(0, 0));
+    d->pairs.upload(pr.data(), pr.size(), st);
+    d->fent.upload(fe.data(), fe.size(), st);
+    d->bent.upload(be.data(), be.size(), st);
+    d->fwd.upload(fw.data(), fw.size(), st);
+    d->bwd.upload(bw.data(), bw.size(), st);
+    HIPCHECK(hipStreamSynchronize(st));
+    return d;
+}
+
 const DevSched& get_sched(p3smi_topology* t, LuConfig cfg) {
     const int TL = cfg.tl();
     auto it = t->dev.sched.find(cfg.key());
     if (it != t->dev.sched.end()) return *it->second;
     const p3smi::Topology& h = t->host;
+    if (cfg.blk()) return *(t->dev.sched[cfg.key()] = upload_block_sched(t, TL));
     const p3smi::Schedule s = p3smi::make_schedule(h, TL, cfg.grp);
     std::unique_ptr<DevSched> d(new DevSched());
     hipStream_t st = t->stream;
@@ -1600,6 +1942,17 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* 
                            ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout, b->status.p, b->iters.p,
                            b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min, ds.nlev, ds.nblev, b->S);
     };
+    if constexpr (G == 64 && NS == 8) {
+        if (ds.blk) {
+            hipLaunchKernelGGL((k_lu_blk<NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
+                               b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.pairs.p, ds.ptrF.p,
+                               ds.fwd.p, ds.fent.p, ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout,
+                               b->status.p, b->iters.p, b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min,
+                               ds.nlev, ds.nblev, b->S);
+            return;
+        }
+    }
+    if (ds.blk) throw Error(P3SMI_E_INTERNAL, "2x2-block schedule launched with a shape other than G = 64, NS = 8");
     if constexpr (G >= 16) go(k_lu_stream<G, NS, real, FACTOR>);
     else go(k_lu<G, NS, real, FACTOR>);
 }

@@ -426,6 +426,223 @@ Schedule make_schedule(const Topology& t, int nslots, int group) {
     return s;
 }
 
+BlockSchedule make_block_schedule(const Topology& t, int nslots) {
+    if (nslots < 1) throw std::runtime_error("nslots must be >= 1");
+    if (t.nnz_lu >= BLK_TWIN - 1) throw std::runtime_error("factor pattern too large for the 2x2-block steps");
+    const int m = t.m;
+    auto S_beg = [&](int p) { return t.Li.data() + t.Lp[p]; };
+    auto S_cnt = [&](int p) { return t.Lp[p + 1] - t.Lp[p]; };
+    // rank of i in S_p (must be there)
+    auto rank = [&](int p, int i) {
+        const int* b = S_beg(p);
+        const int* e = b + S_cnt(p);
+        const int* it = std::lower_bound(b, e, i);
+        if (it == e || *it != i) throw std::runtime_error("internal: block entry outside the factor pattern");
+        return (int)(it - b);
+    };
+    auto posU = [&](int p, int j) { return t.base[p] + 1 + rank(p, j); };             // U(p, j), j > p
+    auto posL = [&](int i, int p) { return t.base[p] + 1 + S_cnt(p) + rank(p, i); };  // L(i, p), i > p
+    std::vector<int> asrc(t.nnz_lu, -1);
+    for (int e = 0; e < t.nnzJ; ++e) asrc[t.apos[e]] = e;
+
+    // row patterns of L: rows[i] = pivots t < i with i in S_t, ascending
+    std::vector<std::vector<int>> rows(m);
+    for (int p = 0; p < m; ++p)
+        for (int q = t.Lp[p]; q < t.Lp[p + 1]; ++q) rows[t.Li[q]].push_back(p);
+    // pivot pairs, greedily from the first pivot
+    std::vector<int> gsize(m, 1);  // 2 = first of a pair, 0 = second of a pair
+    BlockSchedule s;
+    s.nslots = nslots;
+    for (int p = 0; p + 1 < m; ++p) {
+        if (gsize[p] != 1 || t.parent[p] != p + 1) continue;
+        const int c0 = S_cnt(p), c1 = S_cnt(p + 1);
+        if (c0 != c1 + 1 || S_beg(p)[0] != p + 1 || !std::equal(S_beg(p) + 1, S_beg(p) + c0, S_beg(p + 1))) continue;
+        const auto& r0 = rows[p];
+        const auto& r1 = rows[p + 1];
+        if (r1.size() != r0.size() + 1 || r1.back() != p || !std::equal(r0.begin(), r0.end(), r1.begin())) continue;
+        gsize[p] = 2;
+        gsize[p + 1] = 0;
+        ++s.npairs;
+    }
+    // groups of a pattern S (ascending): (lead, size)
+    auto groups_of = [&](const int* b, int cnt, std::vector<std::pair<int, int>>& out) {
+        out.clear();
+        for (int k = 0; k < cnt; ++k) {
+            const int i = b[k];
+            if (gsize[i] == 0) throw std::runtime_error("internal: pivot pair split in a column pattern");
+            if (gsize[i] == 2) {
+                if (k + 1 >= cnt || b[k + 1] != i + 1) throw std::runtime_error("internal: pivot pair split in a column pattern");
+                out.push_back({i, 2});
+                ++k;
+            } else {
+                out.push_back({i, 1});
+            }
+        }
+    };
+    s.nlev = 0;
+    for (int p = 0; p < m; ++p) s.nlev = std::max(s.nlev, t.height[p] + 1);
+    s.nblev = 0;
+    for (int p = 0; p < m; ++p) s.nblev = std::max(s.nblev, t.depth[p] + 1);
+
+    // blocks per level and sub-level, each with its shared pivots
+    struct Blk {
+        BlockTask k;
+        int r0, c0, p0;  // lead row, lead column, lead pivot of the owning group
+    };
+    std::vector<std::vector<Blk>> levA(s.nlev), levB(s.nlev);
+    std::vector<std::vector<int>> levF(s.nlev), levW(s.nblev);
+    std::vector<std::pair<int, int>> cg;
+    for (int p0 = 0; p0 < m; ++p0) {
+        if (gsize[p0] == 0) continue;
+        const int sz = gsize[p0] == 2 ? 2 : 1;
+        const int last = p0 + sz - 1;
+        const int lev = t.height[last];
+        auto src = [&](int pos) { return pos >= 0 ? asrc[pos] : -1; };
+        {  // diagonal block
+            Blk b{};
+            b.r0 = b.c0 = b.p0 = p0;
+            b.k.nr = b.k.nc = sz;
+            b.k.tgt0 = t.base[p0];
+            b.k.src[0] = src(t.base[p0]);
+            b.k.src[1] = b.k.src[2] = b.k.src[3] = -1;
+            if (sz == 2) {
+                b.k.kind = BLK_DIAG2;
+                b.k.tgt1 = t.base[p0 + 1];
+                b.k.aux0 = posU(p0, p0 + 1);
+                b.k.aux1 = posL(p0 + 1, p0);
+                b.k.src[1] = src(b.k.aux0);
+                b.k.src[2] = src(b.k.aux1);
+                b.k.src[3] = src(t.base[p0 + 1]);
+                if (b.k.src[3] < 0) throw std::runtime_error("internal: pivot without a Jacobian entry");
+            } else {
+                b.k.kind = BLK_DIAG1;
+                b.k.tgt1 = b.k.aux0 = b.k.aux1 = 0;
+            }
+            if (b.k.src[0] < 0) throw std::runtime_error("internal: pivot without a Jacobian entry");
+            levA[lev].push_back(b);
+        }
+        groups_of(S_beg(last), S_cnt(last), cg);
+        for (auto [g0, gs] : cg) {
+            // U block P x C
+            Blk u{};
+            u.r0 = u.p0 = p0;
+            u.c0 = g0;
+            u.k.nr = sz;
+            u.k.nc = gs;
+            u.k.tgt0 = posU(p0, g0);
+            u.k.src[0] = src(u.k.tgt0);
+            u.k.src[1] = gs == 2 ? src(u.k.tgt0 + 1) : -1;
+            if (sz == 2) {
+                u.k.kind = BLK_U2;
+                u.k.tgt1 = posU(p0 + 1, g0);
+                u.k.aux0 = posL(p0 + 1, p0);
+                u.k.src[2] = src(u.k.tgt1);
+                u.k.src[3] = gs == 2 ? src(u.k.tgt1 + 1) : -1;
+                levB[lev].push_back(u);
+            } else {
+                u.k.kind = BLK_U1;
+                u.k.src[2] = u.k.src[3] = -1;
+                levA[lev].push_back(u);
+            }
+            // L block C x P
+            Blk l{};
+            l.r0 = g0;
+            l.c0 = l.p0 = p0;
+            l.k.nr = gs;
+            l.k.nc = sz;
+            l.k.tgt0 = posL(g0, p0);
+            l.k.aux0 = t.base[p0];
+            l.k.src[0] = src(l.k.tgt0);
+            l.k.src[2] = gs == 2 ? src(l.k.tgt0 + 1) : -1;
+            if (sz == 2) {
+                l.k.kind = BLK_L2;
+                l.k.tgt1 = posL(g0, p0 + 1);
+                l.k.aux1 = t.base[p0 + 1];
+                l.k.src[1] = src(l.k.tgt1);
+                l.k.src[3] = gs == 2 ? src(l.k.tgt1 + 1) : -1;
+            } else {
+                l.k.kind = BLK_L1;
+                l.k.src[1] = l.k.src[3] = -1;
+            }
+            levB[lev].push_back(l);
+        }
+        for (int p = p0; p <= last; ++p) levW[t.depth[p]].push_back(p);
+    }
+    for (int p = 0; p < m; ++p) levF[t.height[p]].push_back(p);
+
+    // shared pivots of a block (R, C) of group P: t < p0 with r0 and c0 in S_t
+    std::vector<int> tl;
+    auto shared = [&](const Blk& b) {
+        tl.clear();
+        const auto& a = rows[b.r0];
+        const auto& c = rows[b.c0];
+        size_t i = 0, j = 0;
+        while (i < a.size() && j < c.size()) {
+            if (a[i] < c[j]) ++i;
+            else if (c[j] < a[i]) ++j;
+            else {
+                if (a[i] < b.p0) tl.push_back(a[i]);
+                ++i;
+                ++j;
+            }
+        }
+    };
+    auto weight = [&](const Blk& b) {  // row loads of the task
+        shared(b);
+        return (int64_t)2 * b.k.nr * b.k.nc + (int64_t)tl.size() * (b.k.nr + b.k.nc);
+    };
+    auto emit = [&](Blk b, std::vector<BlockTask>& out) {
+        shared(b);
+        b.k.sbeg = (int)(s.steps.size() / 2);
+        for (int tp : tl) {
+            const int pa = posL(b.r0, tp), pb = posU(tp, b.c0);
+            s.steps.push_back(pa | (b.k.nr == 2 ? BLK_TWIN : 0));
+            s.steps.push_back(pb | (b.k.nc == 2 ? BLK_TWIN : 0));
+        }
+        b.k.send = (int)(s.steps.size() / 2);
+        out.push_back(b.k);
+    };
+    std::vector<int> lA, sA, lB, sB, lF, sF, lW, sW;
+    std::vector<const Blk*> itA, itB;
+    std::vector<int> itF, itW;
+    for (int l = 0; l < s.nlev; ++l) {
+        std::vector<int64_t> load(nslots, 0), w;
+        for (const Blk& b : levA[l]) w.push_back(weight(b));
+        std::vector<int> slot = lpt_deal(w, nslots, load);
+        for (size_t i = 0; i < levA[l].size(); ++i) { itA.push_back(&levA[l][i]); lA.push_back(l); sA.push_back(slot[i]); }
+        std::vector<int64_t> loadB(nslots, 0), wb;
+        for (const Blk& b : levB[l]) wb.push_back(weight(b));
+        for (int p : levF[l]) wb.push_back(2 + 2 * (int64_t)(t.Fp[p + 1] - t.Fp[p]));
+        std::vector<int> slotB = lpt_deal(wb, nslots, loadB);
+        for (size_t i = 0; i < levB[l].size(); ++i) { itB.push_back(&levB[l][i]); lB.push_back(l); sB.push_back(slotB[i]); }
+        for (size_t i = 0; i < levF[l].size(); ++i) {
+            itF.push_back(levF[l][i]); lF.push_back(l); sF.push_back(slotB[levB[l].size() + i]);
+        }
+    }
+    for (int l = 0; l < s.nblev; ++l) {
+        std::vector<int64_t> load(nslots, 0), w;
+        for (int p : levW[l]) w.push_back(2 + (t.Lp[p + 1] - t.Lp[p]));
+        std::vector<int> slot = lpt_deal(w, nslots, load);
+        for (size_t i = 0; i < levW[l].size(); ++i) { itW.push_back(levW[l][i]); lW.push_back(l); sW.push_back(slot[i]); }
+    }
+    // execution order: per level, sub-level A of every list, then sub-level B; steps follow it
+    std::vector<int> ordA = group_by_slot(lA, sA, s.nlev, nslots, s.ptrA);
+    std::vector<int> ordB = group_by_slot(lB, sB, s.nlev, nslots, s.ptrB);
+    for (int l = 0; l < s.nlev; ++l) {
+        for (int q = s.ptrA[(size_t)l * nslots]; q < s.ptrA[(size_t)(l + 1) * nslots]; ++q) emit(*itA[ordA[q]], s.taskA);
+        for (int q = s.ptrB[(size_t)l * nslots]; q < s.ptrB[(size_t)(l + 1) * nslots]; ++q) emit(*itB[ordB[q]], s.taskB);
+    }
+    for (int i : group_by_slot(lF, sF, s.nlev, nslots, s.ptrF)) {
+        const int p = itF[i];
+        s.fwd.push_back({p, t.perm[p], t.Fp[p], t.Fp[p + 1]});
+    }
+    for (int i : group_by_slot(lW, sW, s.nblev, nslots, s.ptrW)) {
+        const int p = itW[i];
+        s.bwd.push_back({p, t.base[p], t.Lp[p], t.Lp[p + 1]});
+    }
+    return s;
+}
+
 // ------------------------------------------------------------------ host numerics
 // Jacobian and residual of one system at (Vm, Va), rectangular form: with the unit phasor
 // (cos Va, sin Va) of every bus, Ym cos(delta) = G cu + B su and Ym sin(delta) = G su - B cu

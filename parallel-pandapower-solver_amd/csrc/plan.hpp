@@ -55,6 +55,45 @@ struct Schedule {
     std::vector<BwdRow> bwd;
 };
 
+// ---- 2x2 supernode blocks
+// Pivots p, p+1 form a pair when p+1 is p's elimination-tree parent, S_p = {p+1} + S_{p+1},
+// and rows p and p+1 of L have the same pattern left of p: the P and Q unknowns of one PQ
+// bus, which have identical Jacobian patterns, whenever the order puts them side by side.
+// A pivot group P (one pivot or a pair) owns the diagonal block P x P, a U block P x C and an
+// L block C x P for every group C of its column pattern. Every entry of a block (R, C)
+// takes the contributions of the same pivots t < P (R and C are twins), so a step loads
+// L(R, t) and U(t, C) once -- |R| + |C| rows -- for |R| * |C| updates. The contribution of
+// t = p to the second row or column of a pair comes last, so each entry still sums its
+// terms in ascending pivot order, as the one-entry-per-task schedule does.
+enum BlockKind { BLK_DIAG1 = 0, BLK_DIAG2, BLK_U1, BLK_U2, BLK_L1, BLK_L2 };
+
+struct BlockTask {
+    int kind;        // BlockKind
+    int nr, nc;      // rows, columns of the block (1 or 2)
+    int tgt0, tgt1;  // DIAG: 1/U(p,p), 1/U(p+1,p+1); U: U(p,C0), U(p+1,C0); L: L(R0,p), L(R0,p+1)
+    int aux0, aux1;  // DIAG2: U(p,p+1), L(p+1,p); U2: L(p+1,p); L1: 1/U(p,p); L2: 1/U(p,p), 1/U(p+1,p+1)
+    int src[4];      // Jacobian entries of (0,0) (0,1) (1,0) (1,1), -1 = fill or absent
+    int sbeg, send;  // steps [sbeg, send)
+};
+
+// Steps of a block: (position of L(R0, t), position of U(t, C0)) with bit 30 set when the
+// block has a second row (first word) or a second column (second word): the rows at
+// position + 1 belong to the step as well.
+constexpr int BLK_TWIN = 1 << 30;
+
+struct BlockSchedule {
+    int nslots = 0, nlev = 0, nblev = 0;
+    int npairs = 0;  // pivot pairs
+    // per (level, slot): sub-level A = diagonal blocks and U blocks of single pivots;
+    // sub-level B = the other U and L blocks and the forward rows.
+    std::vector<int> ptrA, ptrB, ptrF, ptrW;
+    std::vector<BlockTask> taskA, taskB;  // steps of the tasks back to back in execution order
+    std::vector<int> steps;               // 2 ints per step
+    std::vector<FwdRow> fwd;
+    std::vector<BwdRow> bwd;
+};
+
+
 struct Topology {
     // ---- polar topology
     int n = 0, m = 0, npv = 0, npq = 0, npvpq = 0, nnzY = 0, nnzJ = 0;
@@ -92,6 +131,9 @@ void update_values(Topology& t, const double* Yx);
 void build_schedule(Topology& t, int nslots);
 // group > 0: deal runs of up to `group` entries of one pivot's U row / L column together.
 Schedule make_schedule(const Topology& t, int nslots, int group);
+// The 2x2-block schedule dealt to `nslots` task lists. Throws if the factor is too large
+// for the step encoding (positions below 2^30).
+BlockSchedule make_block_schedule(const Topology& t, int nslots);
 
 // Host execution of the plan for ONE system (checks and rcond0): Jacobian values at
 // (Vm, Va) with admittance (G, B) [nullptr = the topology's], pin mask [nullptr = none],

@@ -115,31 +115,38 @@ def test_mixed_precision_perturbed_starts(gpu):
     _compare(r, ref, "perturbed mixed")
 
 
-SHAPES = [(64, 16), (64, 8), (32, 16), (32, 8), (16, 16), (16, 8), (1, 16), (1, 8)]
+# (cases per workgroup, wavefronts, group): group 0 at 64 x 8 is the 2x2-block schedule,
+# group 1 the one-entry-per-task schedule of the same shape
+SHAPES = [(64, 16, 0), (64, 8, 0), (64, 8, 1), (32, 16, 0), (32, 8, 0), (16, 16, 0), (16, 8, 0), (1, 16, 0),
+          (1, 8, 0)]
 
 
-@pytest.mark.parametrize("precision", ["double", "mixed"])
-def test_every_lu_shape_gives_the_same_bits(gpu, precision):
+@pytest.mark.parametrize("precision", ["double", "mixed", "mixed_switch"])
+@pytest.mark.parametrize("name", ["pf200", "ts2000"])
+def test_every_lu_shape_gives_the_same_bits(gpu, precision, name):
     """Every launch shape sums each factor entry in one order, so a case's result does not
     depend on the shape that solved it (serving relies on this); the results also meet the
     oracle bar."""
-    g = grids.config_grid("pf200", 0)
+    g = grids.config_grid(name, 0)
     Yp, Yj, Yx = g.csr()
-    S = grids.timeseries_scaled(g, 192, seed=3)
-    V0 = g.flat_start()
+    if name == "pf200":
+        S, V0 = grids.timeseries_scaled(g, 192, seed=3), g.flat_start()
+    else:
+        S, V0 = grids.timeseries_daily(g, 128, seed=3), dc_initial_voltage(g)
     s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
     first = None
-    for gg, ns in SHAPES:
-        s.set_lu_shape(gg, ns, 1, 16, 0)  # few_max 0: every launch uses the batch shape
+    for gg, ns, grp in SHAPES:
+        s.set_lu_shape(gg, ns, 1, 16, 0, grp)  # few_max 0: every launch uses the batch shape
         r = s.solve_batch(S, V0, max_iter=20, tol=1e-8, precision=precision)
         if first is None:
             first = r
-            ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
-            _compare(r, ref, f"shape {gg}x{ns}")
+            if precision == "double":
+                ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+                _compare(r, ref, f"shape {gg}x{ns}")
             continue
-        assert np.array_equal(r["iterations"], first["iterations"]), (gg, ns)
-        assert np.array_equal(r["V"].view(np.float64), first["V"].view(np.float64)), (gg, ns)
-        assert np.array_equal(r["mismatch"], first["mismatch"]), (gg, ns)
+        assert np.array_equal(r["iterations"], first["iterations"]), (gg, ns, grp)
+        assert np.array_equal(r["V"].view(np.float64), first["V"].view(np.float64)), (gg, ns, grp)
+        assert np.array_equal(r["mismatch"], first["mismatch"]), (gg, ns, grp)
 
 
 def test_switching_mixed_mode(gpu):
