@@ -347,7 +347,12 @@ __global__ __launch_bounds__(64 * NS) void k_lu(
     __shared__ int sbad[G];
     const int cl = threadIdx.x % G;
     const int slot = G == 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G)) : (int)(threadIdx.x / G);
-    const int c = blockIdx.x * G + cl;
+    // With one case per workgroup, a factor row's 128-byte line holds 16 cases (FP64), and
+    // workgroups are dealt to the 8 XCDs in turn: give XCD x (workgroups x, x + 8, ...) the
+    // consecutive cases [x * n / 8, (x + 1) * n / 8), so the cases sharing a line share an L2.
+    int wg = blockIdx.x;
+    if (G == 1 && (gridDim.x & 7) == 0) wg = (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+    const int c = wg * G + cl;
     bool act = selected(sel, c, status, vote, rf);
     if (!__syncthreads_or(act)) return;
 
