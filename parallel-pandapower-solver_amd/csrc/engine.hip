@@ -655,50 +655,56 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
 // ---- 2x2 supernode blocks, G = 64 ----------------------------------------------------
 // A block task (3 int4: {kind | nr << 8 | nc << 16, -, sbeg, send}, {tgt0, tgt1, aux0,
 // aux1}, {src00, src01, src10, src11}) updates up to four entries (rows R x columns C of
-// a pivot group's diagonal, U or L block) from one step stream: a step loads L(R, t) and
-// U(t, C) -- the rows at the step's positions and, for a twin row or column, the next
-// position -- and makes four multiply-adds. A block without a second row (column) skips
-// that load (the step's flags are the same in every lane) and ignores the extra sums. The terms of t = p for the second row or
+// a pivot group's diagonal, U or L block) from one step stream: a step (int4 of factor
+// positions, the zero row where an entry is absent) loads L(R, t) and U(t, C) and makes four
+// multiply-adds. A block without a second row (column) has -1 there, skips that load (the
+// step is the same in every lane) and ignores the extra sums. The terms of t = p for the second row or
 // column of a pivot pair are added in the finish, after the shared ones, so every entry
 // sums in ascending pivot order and the factors equal those of k_lu bit for bit.
-struct BlkIn { double j[4]; double x[3]; };
+struct BlkJ { double j[4]; };   // Jacobian start values, loaded one task ahead
+struct BlkX { double x[3]; };   // pivot-group values, loaded when the task begins
 
-// Steps per chunk: four row loads per step, 128 bytes in flight per lane and chunk buffer.
-// With 8 wavefronts per workgroup (256 VGPRs) this fits without spilling; with 16 (128
-// VGPRs) the start values of two tasks do not, so only NS = 8 runs the block schedule.
+// Steps per chunk: four row loads per step. The task finish is inlined at every step of a
+// chunk, so the chunk length sets the kernel's size: 4 steps keep it at ~22k instructions
+// (as k_lu_stream) and ran faster than 8 (fewer instruction-cache misses, 0.57% of fetches
+// at 8 steps and three consume sites), and 16 wavefronts fit in 128 VGPRs.
 template <typename real>
-constexpr int blk_chunk() { return sizeof(real) == 4 ? 16 : 8; }
+constexpr int blk_chunk() { return sizeof(real) == 4 ? 8 : 4; }
 
 template <int K, typename real>
-__device__ __forceinline__ void load_steps(const int2* __restrict__ st, int e, int eend, const real* __restrict__ fac,
+__device__ __forceinline__ void load_steps(const int4* __restrict__ st, int e, int eend, const real* __restrict__ fac,
                                            int S, int c, real (&a0)[K], real (&a1)[K], real (&b0)[K], real (&b1)[K]) {
 #pragma unroll
     for (int u = 0; u < K; ++u) {
-        const int2 p = st[min(e + u, eend - 1)];  // past the end: repeat the last step (unused)
-        const int pa = p.x & (p3smi::BLK_TWIN - 1), pb = p.y & (p3smi::BLK_TWIN - 1);
-        a0[u] = fac[at(pa, S, c)];
-        b0[u] = fac[at(pb, S, c)];
-        if (p.x & p3smi::BLK_TWIN) a1[u] = fac[at(pa + 1, S, c)];  // the step's flags are uniform
-        if (p.y & p3smi::BLK_TWIN) b1[u] = fac[at(pb + 1, S, c)];
+        const int4 p = st[min(e + u, eend - 1)];  // past the end: repeat the last step (unused)
+        a0[u] = fac[at(p.x, S, c)];
+        b0[u] = fac[at(p.z, S, c)];
+        if (p.y >= 0) a1[u] = fac[at(p.y, S, c)];  // the step's positions are uniform
+        if (p.w >= 0) b1[u] = fac[at(p.w, S, c)];
     }
 }
 
 // Block tasks [q0, q1) of one list (steps back to back); pivot statistics of the diagonal
 // blocks go to (pmin, pmax, psum).
 template <int K, typename real>
-__device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q0, int q1, const int2* __restrict__ st,
+__device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q0, int q1, const int4* __restrict__ st,
                                            const double* __restrict__ Jx, real* __restrict__ fac, int S, int c,
                                            double& pmin, double& pmax, double& psum) {
     if (q0 >= q1) return;
     using ACC = real;
-    auto start = [&](int q) -> BlkIn {
-        const int4 w0 = tasks[3 * q], w1 = tasks[3 * q + 1], w2 = tasks[3 * q + 2];
-        const int kind = w0.x & 255;
-        BlkIn in;
+    auto startJ = [&](int q) -> BlkJ {
+        const int4 w2 = tasks[3 * q + 2];
+        BlkJ in;
         in.j[0] = w2.x >= 0 ? Jx[at(w2.x, S, c)] : 0.0;
         in.j[1] = w2.y >= 0 ? Jx[at(w2.y, S, c)] : 0.0;
         in.j[2] = w2.z >= 0 ? Jx[at(w2.z, S, c)] : 0.0;
         in.j[3] = w2.w >= 0 ? Jx[at(w2.w, S, c)] : 0.0;
+        return in;
+    };
+    auto startX = [&](int q) -> BlkX {
+        const int4 w0 = tasks[3 * q], w1 = tasks[3 * q + 1];
+        const int kind = w0.x & 255;
+        BlkX in;
         in.x[0] = in.x[1] = in.x[2] = 0.0;
         if (kind >= p3smi::BLK_U2) in.x[0] = (double)fac[at(w1.z, S, c)];  // L(p+1,p) or 1/U(p,p)
         if (kind == p3smi::BLK_L2) {
@@ -714,7 +720,7 @@ __device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q
         psum += ab;
         return (real)1 / v;
     };
-    auto finish = [&](int q, const BlkIn& in, const ACC (&acc)[4]) {
+    auto finish = [&](int q, const BlkJ& in, const BlkX& ix, const ACC (&acc)[4]) {
         const int4 w0 = tasks[3 * q], w1 = tasks[3 * q + 1];
         const int kind = w0.x & 255, nr = (w0.x >> 8) & 255, nc = w0.x >> 16;
         switch (kind) {
@@ -736,7 +742,7 @@ __device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q
                 if (nc == 2) fac[at(w1.x + 1, S, c)] = (real)in.j[1] + acc[1];
                 break;
             case p3smi::BLK_U2: {
-                const real l10 = (real)in.x[0];
+                const real l10 = (real)ix.x[0];
                 const real u00 = (real)in.j[0] + acc[0];
                 fac[at(w1.x, S, c)] = u00;
                 fac[at(w1.y, S, c)] = (real)in.j[2] + fma(-(ACC)l10, (ACC)u00, acc[2]);
@@ -748,13 +754,13 @@ __device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q
                 break;
             }
             case p3smi::BLK_L1: {
-                const real inv = (real)in.x[0];
+                const real inv = (real)ix.x[0];
                 fac[at(w1.x, S, c)] = ((real)in.j[0] + acc[0]) * inv;
                 if (nr == 2) fac[at(w1.x + 1, S, c)] = ((real)in.j[2] + acc[2]) * inv;
                 break;
             }
             default: {  // BLK_L2
-                const real inv0 = (real)in.x[0], u01 = (real)in.x[1], inv1 = (real)in.x[2];
+                const real inv0 = (real)ix.x[0], u01 = (real)ix.x[1], inv1 = (real)ix.x[2];
                 const real l00 = ((real)in.j[0] + acc[0]) * inv0;
                 fac[at(w1.x, S, c)] = l00;
                 fac[at(w1.y, S, c)] = ((real)in.j[1] + fma(-(ACC)l00, (ACC)u01, acc[1])) * inv1;
@@ -769,19 +775,21 @@ __device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q
     };
     int q = q0;
     int4 tk = tasks[3 * q];
-    BlkIn in = start(q);
-    BlkIn inn = in;
-    if (q + 1 < q1) inn = start(q + 1);
+    BlkJ in = startJ(q);
+    BlkX ix = startX(q);
+    BlkJ inn = in;
+    if (q + 1 < q1) inn = startJ(q + 1);
     const int eend = tasks[3 * (q1 - 1)].w;
     ACC acc[4] = {0, 0, 0, 0};
     auto advance = [&]() {
-        finish(q, in, acc);
+        finish(q, in, ix, acc);
         acc[0] = acc[1] = acc[2] = acc[3] = 0;
         ++q;
         if (q < q1) {
             tk = tasks[3 * q];
             in = inn;
-            if (q + 1 < q1) inn = start(q + 1);
+            ix = startX(q);
+            if (q + 1 < q1) inn = startJ(q + 1);
         }
     };
     auto consume = [&](int e, const real (&a0)[K], const real (&a1)[K], const real (&b0)[K], const real (&b1)[K]) {
@@ -797,28 +805,26 @@ __device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q
             }
         }
     };
-    // two chunk buffers in turn, as in seg_stream
+    // Two chunk buffers in turn, as in seg_stream, but with two consume sites instead of three
+    // (the last chunk is consumed inside the loop; a chunk past the end is not loaded): every
+    // site inlines the task finish K times, and the kernel's size decides its instruction-cache
+    // misses.
     int e = tk.z;
     if (e < eend) {
         real a0[K], a1[K], b0[K], b1[K], c0[K], c1[K], d0[K], d1[K];
 #pragma unroll
         for (int u = 0; u < K; ++u) a1[u] = b1[u] = c1[u] = d1[u] = 0;  // second rows absent: stale, unused
         load_steps<K>(st, e, eend, fac, S, c, a0, a1, b0, b1);
-        bool last0 = true;
-        while (e + K < eend) {
-            load_steps<K>(st, e + K, eend, fac, S, c, c0, c1, d0, d1);
+        while (true) {
+            if (e + K < eend) load_steps<K>(st, e + K, eend, fac, S, c, c0, c1, d0, d1);
             consume(e, a0, a1, b0, b1);
             e += K;
-            if (e + K >= eend) {
-                last0 = false;
-                break;
-            }
-            load_steps<K>(st, e + K, eend, fac, S, c, a0, a1, b0, b1);
+            if (e >= eend) break;
+            if (e + K < eend) load_steps<K>(st, e + K, eend, fac, S, c, a0, a1, b0, b1);
             consume(e, c0, c1, d0, d1);
             e += K;
+            if (e >= eend) break;
         }
-        if (last0) consume(e, a0, a1, b0, b1);
-        else consume(e, c0, c1, d0, d1);
     }
     while (q < q1) advance();
 }
@@ -830,7 +836,7 @@ __device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q
 template <int NS, typename real, bool FACTOR>
 __global__ __launch_bounds__(64 * NS) void k_lu_blk(
     const int* __restrict__ ptrA, const int4* __restrict__ blkA, const int* __restrict__ ptrB,
-    const int4* __restrict__ blkB, const int2* __restrict__ steps, const int* __restrict__ ptrF,
+    const int4* __restrict__ blkB, const int4* __restrict__ steps, const int* __restrict__ ptrF,
     const int4* __restrict__ fwd, const int2* __restrict__ fent, const int* __restrict__ ptrW,
     const int4* __restrict__ bwd, const int2* __restrict__ bent, const double* __restrict__ Jx,
     real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x, int* __restrict__ status,
@@ -846,7 +852,13 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
     bool act = selected(sel, c, status, vote, rf);
     if (!__syncthreads_or(act)) return;
 
-    double pmin = DBL_MAX, pmax = 0.0, psum = 0.0;
+    // pivot statistics live in LDS (a diagonal block updates them; registers are scarce here)
+    double& pmin = smin[slot][lane];
+    double& pmax = smax[slot][lane];
+    double& psum = ssum[slot][lane];
+    pmin = DBL_MAX;
+    pmax = 0.0;
+    psum = 0.0;
     auto startF = [&](int, int4 t) -> In1 { return {rhs[at(t.y, S, c)]}; };
     auto finishF = [&](int4 t, In1 in, double sum) { x[at(t.x, S, c)] = in.v + sum; };
     auto startW = [&](int, int4 t) -> In2 { return {x[at(t.x, S, c)], (double)fac[at(t.y, S, c)]}; };
@@ -865,23 +877,21 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
         __syncthreads();
     }
     if (FACTOR) {
-        smin[slot][lane] = pmin;
-        smax[slot][lane] = pmax;
-        ssum[slot][lane] = psum;
         __syncthreads();
         if (slot == 0) {
             int bad = 0;
             if (act) {
+                double lo = smin[0][lane], hi = smax[0][lane], sum = ssum[0][lane];
                 for (int s2 = 1; s2 < TL; ++s2) {
-                    pmin = fmin(pmin, smin[s2][lane]);
-                    pmax = fmax(pmax, smax[s2][lane]);
-                    psum += ssum[s2][lane];
+                    lo = fmin(lo, smin[s2][lane]);
+                    hi = fmax(hi, smax[s2][lane]);
+                    sum += ssum[s2][lane];
                 }
-                pstat[2 * (size_t)c] = pmin;
-                pstat[2 * (size_t)c + 1] = pmax;
-                const bool broken = !(psum < DBL_MAX && pmin > 0.0);
+                pstat[2 * (size_t)c] = lo;
+                pstat[2 * (size_t)c + 1] = hi;
+                const bool broken = !(sum < DBL_MAX && lo > 0.0);
                 if (SINGLE) {
-                    if (broken || pmin < rcond_min * pmax) {
+                    if (broken || lo < rcond_min * hi) {
                         bad = 1;
                         vote[c] = -1;
                         atomicAdd(guard_trips + 2, 1);  // refactored in FP64 below
@@ -890,7 +900,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
                     bad = 1;
                     status[c] = ST_FAILED;
                     iters[c] += 1;
-                } else if (pmin < rcond_min * pmax) {
+                } else if (lo < rcond_min * hi) {
                     atomicAdd(guard_trips, 1);
                 }
             }
@@ -998,6 +1008,12 @@ __global__ void k_refine_init(const int* __restrict__ status, const int* __restr
     if (c >= S) return;
     rf[c] = (status[c] == ST_ACTIVE && vote[c] == 1) ? 1 : 0;
     prevd[c] = DBL_MAX;
+}
+
+template <typename T>
+__global__ void k_zero(T* p, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (T)0;
 }
 
 __global__ void k_fill_int(int* p, int v, int n) {
@@ -1556,9 +1572,10 @@ struct DevBuf {
 
 struct DevSched {  // one level schedule dealt to TL task lists
     int TL = 0, nlev = 0, nblev = 0;
-    bool blk = false;  // 2x2-block schedule: taskA / taskB hold 3 int4 per block, pairs the steps
+    bool blk = false;  // 2x2-block schedule: taskA / taskB hold 3 int4 per block
     DevBuf<int> ptrA, ptrB, ptrF, ptrW, diagB;
     DevBuf<int4> taskA, taskB, fwd, bwd;
+    DevBuf<int4> steps;  // 2x2-block schedule: the block steps
     DevBuf<int2> pairs, fent, bent;  // operand streams in task-list order
 };
 
@@ -1570,12 +1587,12 @@ struct DevTopo {
 };
 
 // LU launch shape: G cases per workgroup, NS wavefronts; grp = 0: 2x2 supernode blocks
-// when G = 64 and NS = 8 (k_lu_blk), single entries otherwise; grp > 0: runs of up to grp entries of
+// when G = 64 (k_lu_blk, for the FP64 factorization), single entries otherwise; grp > 0: runs of up to grp entries of
 // one pivot dealt together (1 = single entries).
 struct LuConfig {
     int G, NS, grp;
     int tl() const { return 64 * NS / G; }
-    bool blk() const { return G == 64 && NS == 8 && grp == 0; }
+    bool blk() const { return G == 64 && grp == 0; }
     int key() const { return (tl() * 8192 + grp) * 2 + (blk() ? 1 : 0); }
 };
 
@@ -1705,12 +1722,13 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
     const p3smi::BlockSchedule s = p3smi::make_block_schedule(h, TL);
     const int64_t nlu = h.nnz_lu, nj = h.nnzJ;
     auto bad = [](const char* what) { throw Error(P3SMI_E_INTERNAL, std::string("block schedule: ") + what); };
-    const int64_t nsteps = (int64_t)s.steps.size() / 2;
+    // steps: positions in [0, nnz_lu] (nnz_lu = the zero row); the second row / column is -1
+    // exactly when the block has one row / column
+    const int64_t nsteps = (int64_t)s.steps.size() / 4;
     for (int64_t k = 0; k < nsteps; ++k)
-        for (int w = 0; w < 2; ++w) {
-            const int v = s.steps[2 * k + w];
-            const int64_t pos = v & (p3smi::BLK_TWIN - 1);
-            if (v < 0 || pos + ((unsigned)v >> 30) >= nlu) bad("step position out of range");
+        for (int w = 0; w < 4; ++w) {
+            const int v = s.steps[4 * k + w];
+            if (v > nlu || v < ((w & 1) ? -1 : 0)) bad("step position out of range");
         }
     auto pack = [&](const std::vector<p3smi::BlockTask>& ts, const std::vector<int>& ptr) {
         std::vector<int4> out(3 * ts.size());
@@ -1719,6 +1737,8 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
             if (k.kind < p3smi::BLK_DIAG1 || k.kind > p3smi::BLK_L2 || k.nr < 1 || k.nr > 2 || k.nc < 1 || k.nc > 2)
                 bad("task shape");
             if (k.sbeg < 0 || k.send < k.sbeg || k.send > nsteps) bad("step range");
+            for (int e = k.sbeg; e < k.send; ++e)
+                if ((s.steps[4 * e + 1] < 0) != (k.nr == 1) || (s.steps[4 * e + 3] < 0) != (k.nc == 1)) bad("step shape");
             const bool two = k.kind == p3smi::BLK_DIAG2 || k.kind == p3smi::BLK_U2 || k.kind == p3smi::BLK_L2;
             const int ext = (k.kind == p3smi::BLK_U1 || k.kind == p3smi::BLK_U2) ? k.nc - 1
                             : (k.kind == p3smi::BLK_L1 || k.kind == p3smi::BLK_L2) ? k.nr - 1 : 0;
@@ -1754,8 +1774,10 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
     d->diagB.alloc(1);
     d->taskA.upload(ta.data(), ta.size(), st);
     d->taskB.upload(tb.data(), tb.size(), st);
-    std::vector<int2> pr(std::max<int64_t>(nsteps, 1), make_int2(0, 0)), fe, be;
-    for (int64_t k = 0; k < nsteps; ++k) pr[k] = make_int2(s.steps[2 * k], s.steps[2 * k + 1]);
+    std::vector<int4> st4(std::max<int64_t>(nsteps, 1), make_int4(0, -1, 0, -1));
+    for (int64_t k = 0; k < nsteps; ++k)
+        st4[k] = make_int4(s.steps[4 * k], s.steps[4 * k + 1], s.steps[4 * k + 2], s.steps[4 * k + 3]);
+    std::vector<int2> fe, be;
     std::vector<int4> fw(s.fwd.size()), bw(s.bwd.size());
     for (size_t i = 0; i < fw.size(); ++i) {
         const auto& f = s.fwd[i];
@@ -1771,7 +1793,8 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
     }
     if (fe.empty()) fe.push_back(make_int2(0, 0));
     if (be.empty()) be.push_back(make_int2(0, 0));
-    d->pairs.upload(pr.data(), pr.size(), st);
+    d->pairs.alloc(1);
+    d->steps.upload(st4.data(), st4.size(), st);
     d->fent.upload(fe.data(), fe.size(), st);
     d->bent.upload(be.data(), be.size(), st);
     d->fwd.upload(fw.data(), fw.size(), st);
@@ -1847,6 +1870,13 @@ const DevSched& get_sched(p3smi_topology* t, LuConfig cfg) {
     return *(t->dev.sched[cfg.key()] = std::move(d));
 }
 
+// Clear n elements with a kernel (no memset node if a caller captures the stream).
+template <typename T>
+void zero_row(T* p, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    HIPCHECK(hipGetLastError());
+}
+
 int eval_nparts(int n) { return (n + EVAL_WAVES * EVAL_BUSES_PER_WAVE - 1) / (EVAL_WAVES * EVAL_BUSES_PER_WAVE); }
 
 void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
@@ -1859,7 +1889,8 @@ void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     b->nparts = eval_nparts(h.n);
     for (auto* v : {&b->Vm, &b->Va, &b->cs, &b->sn, &b->Vm0, &b->Va0}) v->alloc((size_t)h.n * S);
     b->Jx.alloc((size_t)h.nnzJ * S);
-    b->fac.alloc((size_t)h.nnz_lu * S);
+    b->fac.alloc(((size_t)h.nnz_lu + 1) * S);  // + the zero row the 2x2-block steps read
+    zero_row(b->fac.p + (size_t)h.nnz_lu * S, S, t->stream);
     b->R.alloc((size_t)h.m * S);
     b->x.alloc((size_t)h.m * S);
     for (auto* v : {&b->nrm, &b->alpha, &b->best_alpha, &b->best_nrm}) v->alloc(S);
@@ -1947,17 +1978,17 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* 
                            ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout, b->status.p, b->iters.p,
                            b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min, ds.nlev, ds.nblev, b->S);
     };
-    if constexpr (G == 64 && NS == 8) {
+    if constexpr (G == 64 && FACTOR && sizeof(real) == 8) {
         if (ds.blk) {
             hipLaunchKernelGGL((k_lu_blk<NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
-                               b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.pairs.p, ds.ptrF.p,
+                               b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.steps.p, ds.ptrF.p,
                                ds.fwd.p, ds.fent.p, ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout,
                                b->status.p, b->iters.p, b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min,
                                ds.nlev, ds.nblev, b->S);
             return;
         }
     }
-    if (ds.blk) throw Error(P3SMI_E_INTERNAL, "2x2-block schedule launched with a shape other than G = 64, NS = 8");
+    if (ds.blk) throw Error(P3SMI_E_INTERNAL, "2x2-block schedule launched for other than an FP64 factorization at G = 64");
     if constexpr (G >= 16) go(k_lu_stream<G, NS, real, FACTOR>);
     else go(k_lu<G, NS, real, FACTOR>);
 }
@@ -1972,7 +2003,11 @@ bool valid_lu_config(LuConfig c) {
 template <typename real, bool FACTOR>
 void launch_lu(p3smi_batch* b, int n_active, real* fac, const double* rhs, double* xout, int sel) {
     p3smi_topology* t = b->topo;
-    const LuConfig cfg = n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
+    LuConfig cfg = n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
+    // The 2x2 blocks serve the FP64 factorization. Solve-only launches (refinement) have no
+    // blocks, and the FP32 factorization runs faster with 16 wavefronts one entry per task
+    // (2.26 against 2.41 ms on ts2000); the factor layout is the same, so they take that shape.
+    if (cfg.blk() && (!FACTOR || sizeof(real) == 4)) cfg = LuConfig{64, 16, 1};
     const DevSched& ds = get_sched(t, cfg);
     KTimer tm(b, 1);
 #define P3SMI_LU_CASE(GG, NN)                                                                \
@@ -1992,7 +2027,8 @@ void ensure_mixed(p3smi_batch* b) {
     if (b->mixed_ready) return;
     const p3smi::Topology& h = b->topo->host;
     const size_t S = b->S;
-    b->fac32.alloc((size_t)h.nnz_lu * S);
+    b->fac32.alloc(((size_t)h.nnz_lu + 1) * S);  // + the zero row
+    zero_row(b->fac32.p + (size_t)h.nnz_lu * S, S, b->topo->stream);
     b->rr.alloc((size_t)h.m * S);
     b->dvec.alloc((size_t)h.m * S);
     b->prevd.alloc(S);
@@ -2298,7 +2334,7 @@ void upload_cols(void* dst, const void* src, size_t rows, size_t ld, size_t c0, 
 }
 
 size_t bytes_per_case(const p3smi::Topology& h, bool per_case_y, bool pin, bool per_case_s) {
-    size_t b = (size_t)h.n * 6 * 8 + (size_t)h.nnzJ * 8 + (size_t)h.nnz_lu * 8 + (size_t)h.m * 16;
+    size_t b = (size_t)h.n * 6 * 8 + (size_t)h.nnzJ * 8 + ((size_t)h.nnz_lu + 1) * 8 + (size_t)h.m * 16;
     b += (size_t)eval_nparts(h.n) * 8 + 64;
     b += (size_t)h.n * 16 * 2;  // V0 / V out
     if (per_case_s) b += (size_t)h.n * 16;

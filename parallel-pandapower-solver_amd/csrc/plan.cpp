@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <iterator>
 #include <limits>
 #include <numeric>
 #include <queue>
@@ -428,10 +429,12 @@ Schedule make_schedule(const Topology& t, int nslots, int group) {
 
 BlockSchedule make_block_schedule(const Topology& t, int nslots) {
     if (nslots < 1) throw std::runtime_error("nslots must be >= 1");
-    if (t.nnz_lu >= BLK_TWIN - 1) throw std::runtime_error("factor pattern too large for the 2x2-block steps");
+    if (t.nnz_lu >= std::numeric_limits<int>::max()) throw std::runtime_error("factor pattern too large");
     const int m = t.m;
+    const int zero = (int)t.nnz_lu;  // the zero row
     auto S_beg = [&](int p) { return t.Li.data() + t.Lp[p]; };
     auto S_cnt = [&](int p) { return t.Lp[p + 1] - t.Lp[p]; };
+    auto in_S = [&](int p, int i) { return std::binary_search(S_beg(p), S_beg(p) + S_cnt(p), i); };
     // rank of i in S_p (must be there)
     auto rank = [&](int p, int i) {
         const int* b = S_beg(p);
@@ -457,21 +460,17 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
         if (gsize[p] != 1 || t.parent[p] != p + 1) continue;
         const int c0 = S_cnt(p), c1 = S_cnt(p + 1);
         if (c0 != c1 + 1 || S_beg(p)[0] != p + 1 || !std::equal(S_beg(p) + 1, S_beg(p) + c0, S_beg(p + 1))) continue;
-        const auto& r0 = rows[p];
-        const auto& r1 = rows[p + 1];
-        if (r1.size() != r0.size() + 1 || r1.back() != p || !std::equal(r0.begin(), r0.end(), r1.begin())) continue;
         gsize[p] = 2;
         gsize[p + 1] = 0;
         ++s.npairs;
     }
-    // groups of a pattern S (ascending): (lead, size)
+    // groups of a pattern S (ascending): (lead, size); a pair with one member in S counts
+    // as a single pivot there
     auto groups_of = [&](const int* b, int cnt, std::vector<std::pair<int, int>>& out) {
         out.clear();
         for (int k = 0; k < cnt; ++k) {
             const int i = b[k];
-            if (gsize[i] == 0) throw std::runtime_error("internal: pivot pair split in a column pattern");
-            if (gsize[i] == 2) {
-                if (k + 1 >= cnt || b[k + 1] != i + 1) throw std::runtime_error("internal: pivot pair split in a column pattern");
+            if (gsize[i] == 2 && k + 1 < cnt && b[k + 1] == i + 1) {
                 out.push_back({i, 2});
                 ++k;
             } else {
@@ -489,6 +488,8 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
         BlockTask k;
         int r0, c0, p0;  // lead row, lead column, lead pivot of the owning group
     };
+    // the pairs' second pivots never start a block; their own rows and columns are in the
+    // pair's blocks
     std::vector<std::vector<Blk>> levA(s.nlev), levB(s.nlev);
     std::vector<std::vector<int>> levF(s.nlev), levW(s.nblev);
     std::vector<std::pair<int, int>> cg;
@@ -570,18 +571,23 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
     }
     for (int p = 0; p < m; ++p) levF[t.height[p]].push_back(p);
 
-    // shared pivots of a block (R, C) of group P: t < p0 with r0 and c0 in S_t
-    std::vector<int> tl;
+    // pivots of a block (R, C) of group P: t < p0 with a row of R and a column of C in S_t
+    std::vector<int> tl, ra, ca;
+    auto merge = [](const std::vector<int>& x, const std::vector<int>* y, std::vector<int>& out) {
+        out.clear();
+        if (!y) { out = x; return; }
+        std::set_union(x.begin(), x.end(), y->begin(), y->end(), std::back_inserter(out));
+    };
     auto shared = [&](const Blk& b) {
+        merge(rows[b.r0], b.k.nr == 2 ? &rows[b.r0 + 1] : nullptr, ra);
+        merge(rows[b.c0], b.k.nc == 2 ? &rows[b.c0 + 1] : nullptr, ca);
         tl.clear();
-        const auto& a = rows[b.r0];
-        const auto& c = rows[b.c0];
         size_t i = 0, j = 0;
-        while (i < a.size() && j < c.size()) {
-            if (a[i] < c[j]) ++i;
-            else if (c[j] < a[i]) ++j;
+        while (i < ra.size() && j < ca.size()) {
+            if (ra[i] < ca[j]) ++i;
+            else if (ca[j] < ra[i]) ++j;
             else {
-                if (a[i] < b.p0) tl.push_back(a[i]);
+                if (ra[i] < b.p0) tl.push_back(ra[i]);
                 ++i;
                 ++j;
             }
@@ -593,13 +599,14 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
     };
     auto emit = [&](Blk b, std::vector<BlockTask>& out) {
         shared(b);
-        b.k.sbeg = (int)(s.steps.size() / 2);
+        b.k.sbeg = (int)(s.steps.size() / 4);
         for (int tp : tl) {
-            const int pa = posL(b.r0, tp), pb = posU(tp, b.c0);
-            s.steps.push_back(pa | (b.k.nr == 2 ? BLK_TWIN : 0));
-            s.steps.push_back(pb | (b.k.nc == 2 ? BLK_TWIN : 0));
+            s.steps.push_back(in_S(tp, b.r0) ? posL(b.r0, tp) : zero);
+            s.steps.push_back(b.k.nr == 2 ? (in_S(tp, b.r0 + 1) ? posL(b.r0 + 1, tp) : zero) : -1);
+            s.steps.push_back(in_S(tp, b.c0) ? posU(tp, b.c0) : zero);
+            s.steps.push_back(b.k.nc == 2 ? (in_S(tp, b.c0 + 1) ? posU(tp, b.c0 + 1) : zero) : -1);
         }
-        b.k.send = (int)(s.steps.size() / 2);
+        b.k.send = (int)(s.steps.size() / 4);
         out.push_back(b.k);
     };
     std::vector<int> lA, sA, lB, sB, lF, sF, lW, sW;

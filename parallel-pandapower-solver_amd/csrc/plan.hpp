@@ -56,15 +56,17 @@ struct Schedule {
 };
 
 // ---- 2x2 supernode blocks
-// Pivots p, p+1 form a pair when p+1 is p's elimination-tree parent, S_p = {p+1} + S_{p+1},
-// and rows p and p+1 of L have the same pattern left of p: the P and Q unknowns of one PQ
-// bus, which have identical Jacobian patterns, whenever the order puts them side by side.
-// A pivot group P (one pivot or a pair) owns the diagonal block P x P, a U block P x C and an
-// L block C x P for every group C of its column pattern. Every entry of a block (R, C)
-// takes the contributions of the same pivots t < P (R and C are twins), so a step loads
-// L(R, t) and U(t, C) once -- |R| + |C| rows -- for |R| * |C| updates. The contribution of
-// t = p to the second row or column of a pair comes last, so each entry still sums its
-// terms in ascending pivot order, as the one-entry-per-task schedule does.
+// Pivots p, p+1 form a pair when p+1 is p's elimination-tree parent and S_p = {p+1} + S_{p+1}
+// (the P and Q unknowns of one PQ bus whenever the order puts them side by side, and the
+// chain of single pivots at the top of the tree). A pivot group P (one pivot or a pair) owns
+// the diagonal block P x P, a U block P x C and an L block C x P for every group C (a pair,
+// or a single pivot) of its column pattern. A block (R, C) steps through the pivots t < P
+// that contribute to any of its entries, in ascending order; a step loads L(R, t) and
+// U(t, C) once -- |R| + |C| rows -- for |R| * |C| updates. Where one row or column of a pair
+// has no entry at t, the step reads the zero row (position nnz_lu of the factor arena): the
+// term is an exact zero, so every entry sums the same nonzero terms in the same ascending
+// order as the one-entry-per-task schedule (identical bits while the factors are finite).
+// The contribution of t = p to the second row or column of a pair comes last, in the finish.
 enum BlockKind { BLK_DIAG1 = 0, BLK_DIAG2, BLK_U1, BLK_U2, BLK_L1, BLK_L2 };
 
 struct BlockTask {
@@ -76,10 +78,8 @@ struct BlockTask {
     int sbeg, send;  // steps [sbeg, send)
 };
 
-// Steps of a block: (position of L(R0, t), position of U(t, C0)) with bit 30 set when the
-// block has a second row (first word) or a second column (second word): the rows at
-// position + 1 belong to the step as well.
-constexpr int BLK_TWIN = 1 << 30;
+// Steps of a block: positions of L(R0, t), L(R1, t), U(t, C0), U(t, C1); nnz_lu = the zero
+// row; -1 for the second row (column) of a block that has one row (column).
 
 struct BlockSchedule {
     int nslots = 0, nlev = 0, nblev = 0;
@@ -88,7 +88,7 @@ struct BlockSchedule {
     // sub-level B = the other U and L blocks and the forward rows.
     std::vector<int> ptrA, ptrB, ptrF, ptrW;
     std::vector<BlockTask> taskA, taskB;  // steps of the tasks back to back in execution order
-    std::vector<int> steps;               // 2 ints per step
+    std::vector<int> steps;               // 4 ints per step
     std::vector<FwdRow> fwd;
     std::vector<BwdRow> bwd;
 };
@@ -131,8 +131,7 @@ void update_values(Topology& t, const double* Yx);
 void build_schedule(Topology& t, int nslots);
 // group > 0: deal runs of up to `group` entries of one pivot's U row / L column together.
 Schedule make_schedule(const Topology& t, int nslots, int group);
-// The 2x2-block schedule dealt to `nslots` task lists. Throws if the factor is too large
-// for the step encoding (positions below 2^30).
+// The 2x2-block schedule dealt to `nslots` task lists (reads position nnz_lu as zero).
 BlockSchedule make_block_schedule(const Topology& t, int nslots);
 
 // Host execution of the plan for ONE system (checks and rcond0): Jacobian values at
