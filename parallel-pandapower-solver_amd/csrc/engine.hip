@@ -829,15 +829,98 @@ __device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q
     while (q < q1) advance();
 }
 
+// Forward rows of pivot groups (2 int4 per task: {row, nr, sbeg, send}, {rsrc0, rsrc1,
+// lpos, -}; steps int4 {L(row, j), L(row+1, j) or -1, j, -}): a step loads L(R, j) and y(j)
+// once for the group's rows; the pair's term L(row+1, row) y(row) comes last. Chunks of
+// K steps in two buffers, as blk_stream.
+template <int K, typename real>
+__device__ __forceinline__ void fblk_stream(const int4* __restrict__ tasks, int q0, int q1, const int4* __restrict__ st,
+                                            const real* __restrict__ fac, double* __restrict__ x,
+                                            const double* __restrict__ rhs, int S, int c) {
+    if (q0 >= q1) return;
+    struct FIn { double r0, r1, l10; };
+    auto start = [&](int q) -> FIn {
+        const int4 w0 = tasks[2 * q], w1 = tasks[2 * q + 1];
+        FIn in;
+        in.r0 = rhs[at(w1.x, S, c)];
+        in.r1 = w0.y == 2 ? rhs[at(w1.y, S, c)] : 0.0;
+        in.l10 = w0.y == 2 ? (double)fac[at(w1.z, S, c)] : 0.0;
+        return in;
+    };
+    auto finish = [&](int q, const FIn& in, double a0, double a1) {
+        const int4 w0 = tasks[2 * q];
+        const double y0 = in.r0 + a0;
+        x[at(w0.x, S, c)] = y0;
+        if (w0.y == 2) x[at(w0.x + 1, S, c)] = in.r1 + fma(-in.l10, y0, a1);
+    };
+    int q = q0;
+    int4 tk = tasks[2 * q];
+    FIn in = start(q);
+    FIn inn = in;
+    if (q + 1 < q1) inn = start(q + 1);
+    const int eend = tasks[2 * (q1 - 1)].w;
+    double acc0 = 0.0, acc1 = 0.0;
+    auto advance = [&]() {
+        finish(q, in, acc0, acc1);
+        acc0 = acc1 = 0.0;
+        ++q;
+        if (q < q1) {
+            tk = tasks[2 * q];
+            in = inn;
+            if (q + 1 < q1) inn = start(q + 1);
+        }
+    };
+    auto load = [&](int e, real (&a0)[K], real (&a1)[K], double (&y)[K]) {
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            const int4 p = st[min(e + u, eend - 1)];
+            a0[u] = fac[at(p.x, S, c)];
+            if (p.y >= 0) a1[u] = fac[at(p.y, S, c)];
+            y[u] = x[at(p.z, S, c)];
+        }
+    };
+    auto consume = [&](int e, const real (&a0)[K], const real (&a1)[K], const double (&y)[K]) {
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            const int pos = e + u;
+            while (q < q1 && pos == tk.w) advance();
+            if (pos < eend) {
+                acc0 = fma(-(double)a0[u], y[u], acc0);
+                acc1 = fma(-(double)a1[u], y[u], acc1);
+            }
+        }
+    };
+    int e = tk.z;
+    if (e < eend) {
+        real a0[K], a1[K], b0[K], b1[K];
+        double y0[K], y1[K];
+#pragma unroll
+        for (int u = 0; u < K; ++u) a1[u] = b1[u] = 0;  // second rows absent: stale, unused
+        load(e, a0, a1, y0);
+        while (true) {
+            if (e + K < eend) load(e + K, b0, b1, y1);
+            consume(e, a0, a1, y0);
+            e += K;
+            if (e >= eend) break;
+            if (e + K < eend) load(e + K, a0, a1, y0);
+            consume(e, b0, b1, y1);
+            e += K;
+            if (e >= eend) break;
+        }
+    }
+    while (q < q1) advance();
+}
+
 // k_lu_stream with the 2x2-block schedule (G = 64): same contract and, bit for bit, the
 // same factors and solutions. Per level: sub-level A = diagonal blocks and U blocks of
 // single pivots; sub-level B = the U blocks of pivot pairs, the L blocks and the forward
-// rows (which read L(p+1, p), written by the diagonal block of a pair in sub-level A).
+// rows by pivot group (which read L(p+1, p), written by the diagonal block of a pair in
+// sub-level A).
 template <int NS, typename real, bool FACTOR>
 __global__ __launch_bounds__(64 * NS) void k_lu_blk(
     const int* __restrict__ ptrA, const int4* __restrict__ blkA, const int* __restrict__ ptrB,
     const int4* __restrict__ blkB, const int4* __restrict__ steps, const int* __restrict__ ptrF,
-    const int4* __restrict__ fwd, const int2* __restrict__ fent, const int* __restrict__ ptrW,
+    const int4* __restrict__ fwd, const int4* __restrict__ fsteps, const int* __restrict__ ptrW,
     const int4* __restrict__ bwd, const int2* __restrict__ bent, const double* __restrict__ Jx,
     real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x, int* __restrict__ status,
     int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips, int* __restrict__ vote,
@@ -859,8 +942,6 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
     pmin = DBL_MAX;
     pmax = 0.0;
     psum = 0.0;
-    auto startF = [&](int, int4 t) -> In1 { return {rhs[at(t.y, S, c)]}; };
-    auto finishF = [&](int4 t, In1 in, double sum) { x[at(t.x, S, c)] = in.v + sum; };
     auto startW = [&](int, int4 t) -> In2 { return {x[at(t.x, S, c)], (double)fac[at(t.y, S, c)]}; };
     auto finishW = [&](int4 t, In2 in, double sum) { x[at(t.x, S, c)] = (in.v + sum) * in.d; };
 
@@ -872,7 +953,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
         }
         if (act) {
             if (FACTOR) blk_stream<blk_chunk<real>(), real>(blkB, ptrB[i0], ptrB[i0 + 1], steps, Jx, fac, S, c, pmin, pmax, psum);
-            seg_stream<double, In1>(fwd, ptrF[i0], ptrF[i0 + 1], fent, fac, x, S, c, startF, finishF);
+            fblk_stream<4, real>(fwd, ptrF[i0], ptrF[i0 + 1], fsteps, fac, x, rhs, S, c);
         }
         __syncthreads();
     }
@@ -1575,7 +1656,8 @@ struct DevSched {  // one level schedule dealt to TL task lists
     bool blk = false;  // 2x2-block schedule: taskA / taskB hold 3 int4 per block
     DevBuf<int> ptrA, ptrB, ptrF, ptrW, diagB;
     DevBuf<int4> taskA, taskB, fwd, bwd;
-    DevBuf<int4> steps;  // 2x2-block schedule: the block steps
+    DevBuf<int4> steps;   // 2x2-block schedule: the block steps
+    DevBuf<int4> fsteps;  // 2x2-block schedule: the forward-row steps (fwd: 2 int4 per group)
     DevBuf<int2> pairs, fent, bent;  // operand streams in task-list order
 };
 
@@ -1777,25 +1859,42 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
     std::vector<int4> st4(std::max<int64_t>(nsteps, 1), make_int4(0, -1, 0, -1));
     for (int64_t k = 0; k < nsteps; ++k)
         st4[k] = make_int4(s.steps[4 * k], s.steps[4 * k + 1], s.steps[4 * k + 2], s.steps[4 * k + 3]);
-    std::vector<int2> fe, be;
-    std::vector<int4> fw(s.fwd.size()), bw(s.bwd.size());
-    for (size_t i = 0; i < fw.size(); ++i) {
-        const auto& f = s.fwd[i];
-        const int b0 = (int)fe.size();
-        for (int e = f.ebeg; e < f.eend; ++e) fe.push_back(make_int2(h.Fpos[e], h.Fcol[e]));
-        fw[i] = make_int4(f.row, f.rsrc, b0, (int)fe.size());
+    std::vector<int2> be;
+    // forward rows by pivot group: {row, nr, sbeg, send}, {rsrc0, rsrc1, lpos, 0}; steps
+    // {L(row, j), L(row+1, j) or -1, j, 0}
+    const int64_t nfs = (int64_t)s.fsteps.size() / 3;
+    std::vector<int4> fw(2 * std::max<size_t>(s.fwd.size(), 1), make_int4(0, 1, 0, 0));
+    std::vector<int4> fs(std::max<int64_t>(nfs, 1), make_int4(0, -1, 0, 0));
+    for (int64_t k = 0; k < nfs; ++k) {
+        const int la = s.fsteps[3 * k], lb = s.fsteps[3 * k + 1], j = s.fsteps[3 * k + 2];
+        if (la < 0 || la > nlu || lb < -1 || lb > nlu || j < 0 || j >= h.m) bad("forward step out of range");
+        fs[k] = make_int4(la, lb, j, 0);
     }
+    for (size_t i = 0; i < s.fwd.size(); ++i) {
+        const p3smi::FwdBlock& f = s.fwd[i];
+        if (f.nr < 1 || f.nr > 2 || f.row < 0 || f.row + f.nr > h.m || f.rsrc0 < 0 || f.rsrc0 >= h.m) bad("forward row");
+        if (f.nr == 2 && (f.rsrc1 < 0 || f.rsrc1 >= h.m || f.lpos < 0 || f.lpos >= nlu)) bad("forward pair");
+        if (f.sbeg < 0 || f.send < f.sbeg || f.send > nfs) bad("forward step range");
+        for (int e = f.sbeg; e < f.send; ++e)
+            if ((s.fsteps[3 * e + 1] < 0) != (f.nr == 1)) bad("forward step shape");
+        fw[2 * i] = make_int4(f.row, f.nr, f.sbeg, f.send);
+        fw[2 * i + 1] = make_int4(f.rsrc0, f.nr == 2 ? f.rsrc1 : 0, f.nr == 2 ? f.lpos : 0, 0);
+    }
+    for (size_t r = 0; r + 1 < s.ptrF.size(); ++r)
+        for (int q = s.ptrF[r]; q + 1 < s.ptrF[r + 1]; ++q)
+            if (s.fwd[q].send != s.fwd[q + 1].sbeg) bad("forward steps not contiguous");
+    std::vector<int4> bw(s.bwd.size());
     for (size_t i = 0; i < bw.size(); ++i) {
         const auto& w = s.bwd[i];
         const int b0 = (int)be.size();
         for (int e = w.lbeg; e < w.lend; ++e) be.push_back(make_int2(w.base + 1 + (e - w.lbeg), h.Li[e]));
         bw[i] = make_int4(w.row, w.base, b0, (int)be.size());
     }
-    if (fe.empty()) fe.push_back(make_int2(0, 0));
     if (be.empty()) be.push_back(make_int2(0, 0));
     d->pairs.alloc(1);
     d->steps.upload(st4.data(), st4.size(), st);
-    d->fent.upload(fe.data(), fe.size(), st);
+    d->fsteps.upload(fs.data(), fs.size(), st);
+    d->fent.alloc(1);
     d->bent.upload(be.data(), be.size(), st);
     d->fwd.upload(fw.data(), fw.size(), st);
     d->bwd.upload(bw.data(), bw.size(), st);
@@ -1982,7 +2081,7 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* 
         if (ds.blk) {
             hipLaunchKernelGGL((k_lu_blk<NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
                                b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.steps.p, ds.ptrF.p,
-                               ds.fwd.p, ds.fent.p, ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout,
+                               ds.fwd.p, ds.fsteps.p, ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout,
                                b->status.p, b->iters.p, b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min,
                                ds.nlev, ds.nblev, b->S);
             return;

@@ -569,7 +569,8 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
         }
         for (int p = p0; p <= last; ++p) levW[t.depth[p]].push_back(p);
     }
-    for (int p = 0; p < m; ++p) levF[t.height[p]].push_back(p);
+    for (int p0 = 0; p0 < m; ++p0)  // forward rows by pivot group, at the group's level
+        if (gsize[p0] != 0) levF[t.height[p0 + (gsize[p0] == 2 ? 1 : 0)]].push_back(p0);
 
     // pivots of a block (R, C) of group P: t < p0 with a row of R and a column of C in S_t
     std::vector<int> tl, ra, ca;
@@ -619,7 +620,13 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
         for (size_t i = 0; i < levA[l].size(); ++i) { itA.push_back(&levA[l][i]); lA.push_back(l); sA.push_back(slot[i]); }
         std::vector<int64_t> loadB(nslots, 0), wb;
         for (const Blk& b : levB[l]) wb.push_back(weight(b));
-        for (int p : levF[l]) wb.push_back(2 + 2 * (int64_t)(t.Fp[p + 1] - t.Fp[p]));
+        for (int p : levF[l]) {
+            const int nr = gsize[p] == 2 ? 2 : 1;
+            merge(rows[p], nr == 2 ? &rows[p + 1] : nullptr, ra);
+            int64_t nstep = 0;
+            for (int j : ra) nstep += j < p;
+            wb.push_back(2 + nstep * (nr + 1));
+        }
         std::vector<int> slotB = lpt_deal(wb, nslots, loadB);
         for (size_t i = 0; i < levB[l].size(); ++i) { itB.push_back(&levB[l][i]); lB.push_back(l); sB.push_back(slotB[i]); }
         for (size_t i = 0; i < levF[l].size(); ++i) {
@@ -641,7 +648,23 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
     }
     for (int i : group_by_slot(lF, sF, s.nlev, nslots, s.ptrF)) {
         const int p = itF[i];
-        s.fwd.push_back({p, t.perm[p], t.Fp[p], t.Fp[p + 1]});
+        const int nr = gsize[p] == 2 ? 2 : 1;
+        FwdBlock f{};
+        f.row = p;
+        f.nr = nr;
+        f.rsrc0 = t.perm[p];
+        f.rsrc1 = nr == 2 ? t.perm[p + 1] : -1;
+        f.lpos = nr == 2 ? posL(p + 1, p) : -1;
+        f.sbeg = (int)(s.fsteps.size() / 3);
+        merge(rows[p], nr == 2 ? &rows[p + 1] : nullptr, ra);
+        for (int j : ra) {
+            if (j >= p) continue;
+            s.fsteps.push_back(in_S(j, p) ? posL(p, j) : zero);
+            s.fsteps.push_back(nr == 2 ? (in_S(j, p + 1) ? posL(p + 1, j) : zero) : -1);
+            s.fsteps.push_back(j);
+        }
+        f.send = (int)(s.fsteps.size() / 3);
+        s.fwd.push_back(f);
     }
     for (int i : group_by_slot(lW, sW, s.nblev, nslots, s.ptrW)) {
         const int p = itW[i];

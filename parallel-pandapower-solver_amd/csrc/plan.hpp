@@ -81,6 +81,16 @@ struct BlockTask {
 // Steps of a block: positions of L(R0, t), L(R1, t), U(t, C0), U(t, C1); nnz_lu = the zero
 // row; -1 for the second row (column) of a block that has one row (column).
 
+// Forward rows of a pivot group: y(row..row+nr-1) = R - sum_j L(., j) y(j) over the columns j
+// of either row (the zero row where one row has no entry); for a pair, the term
+// L(row+1, row) y(row) comes last, in the finish.
+struct FwdBlock {
+    int row, nr;           // first row (pivot index), 1 or 2 rows
+    int rsrc0, rsrc1;      // right-hand side entries (Jacobian row order)
+    int lpos;              // pair: position of L(row+1, row)
+    int sbeg, send;        // forward steps [sbeg, send)
+};
+
 struct BlockSchedule {
     int nslots = 0, nlev = 0, nblev = 0;
     int npairs = 0;  // pivot pairs
@@ -89,7 +99,8 @@ struct BlockSchedule {
     std::vector<int> ptrA, ptrB, ptrF, ptrW;
     std::vector<BlockTask> taskA, taskB;  // steps of the tasks back to back in execution order
     std::vector<int> steps;               // 4 ints per step
-    std::vector<FwdRow> fwd;
+    std::vector<FwdBlock> fwd;            // forward rows by pivot group
+    std::vector<int> fsteps;              // 3 ints per step: L(row, j), L(row+1, j) (-1: one row), j
     std::vector<BwdRow> bwd;
 };
 
