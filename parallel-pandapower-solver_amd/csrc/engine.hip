@@ -911,6 +911,96 @@ __device__ __forceinline__ void fblk_stream(const int4* __restrict__ tasks, int 
     while (q < q1) advance();
 }
 
+// Backward rows of pivot groups (2 int4 per task: {row, nr, sbeg, send}, {1/U(row,row),
+// 1/U(row+1,row+1), -, -} positions; steps int4 {U(row, j), U(row+1, j) or -1, j, -} over
+// the columns of the group's last row, descending): a step loads x(j) once for the group's
+// rows; for a pair, x(row+1) first, then U(row, row+1) x(row+1) last in row's sum.
+template <int K, typename real>
+__device__ __forceinline__ void bblk_stream(const int4* __restrict__ tasks, int q0, int q1, const int4* __restrict__ st,
+                                            const real* __restrict__ fac, double* __restrict__ x, int S, int c) {
+    if (q0 >= q1) return;
+    struct BIn { double y0, y1, inv0, inv1, u01; };
+    auto start = [&](int q) -> BIn {
+        const int4 w0 = tasks[2 * q], w1 = tasks[2 * q + 1];
+        BIn in;
+        in.y0 = x[at(w0.x, S, c)];
+        in.inv0 = (double)fac[at(w1.x, S, c)];
+        in.y1 = in.inv1 = in.u01 = 0.0;
+        if (w0.y == 2) {
+            in.y1 = x[at(w0.x + 1, S, c)];
+            in.inv1 = (double)fac[at(w1.y, S, c)];
+            in.u01 = (double)fac[at(w1.x + 1, S, c)];
+        }
+        return in;
+    };
+    auto finish = [&](int q, const BIn& in, double a0, double a1) {
+        const int4 w0 = tasks[2 * q];
+        if (w0.y == 2) {
+            const double x1 = (in.y1 + a1) * in.inv1;
+            x[at(w0.x + 1, S, c)] = x1;
+            x[at(w0.x, S, c)] = (in.y0 + fma(-in.u01, x1, a0)) * in.inv0;
+        } else {
+            x[at(w0.x, S, c)] = (in.y0 + a0) * in.inv0;
+        }
+    };
+    int q = q0;
+    int4 tk = tasks[2 * q];
+    BIn in = start(q);
+    BIn inn = in;
+    if (q + 1 < q1) inn = start(q + 1);
+    const int eend = tasks[2 * (q1 - 1)].w;
+    double acc0 = 0.0, acc1 = 0.0;
+    auto advance = [&]() {
+        finish(q, in, acc0, acc1);
+        acc0 = acc1 = 0.0;
+        ++q;
+        if (q < q1) {
+            tk = tasks[2 * q];
+            in = inn;
+            if (q + 1 < q1) inn = start(q + 1);
+        }
+    };
+    auto load = [&](int e, real (&a0)[K], real (&a1)[K], double (&xv)[K]) {
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            const int4 p = st[min(e + u, eend - 1)];
+            a0[u] = fac[at(p.x, S, c)];
+            if (p.y >= 0) a1[u] = fac[at(p.y, S, c)];
+            xv[u] = x[at(p.z, S, c)];
+        }
+    };
+    auto consume = [&](int e, const real (&a0)[K], const real (&a1)[K], const double (&xv)[K]) {
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            const int pos = e + u;
+            while (q < q1 && pos == tk.w) advance();
+            if (pos < eend) {
+                acc0 = fma(-(double)a0[u], xv[u], acc0);
+                acc1 = fma(-(double)a1[u], xv[u], acc1);
+            }
+        }
+    };
+    int e = tk.z;
+    if (e < eend) {
+        real a0[K], a1[K], b0[K], b1[K];
+        double x0[K], x1[K];
+#pragma unroll
+        for (int u = 0; u < K; ++u) a1[u] = b1[u] = 0;  // second rows absent: stale, unused
+        load(e, a0, a1, x0);
+        while (true) {
+            if (e + K < eend) load(e + K, b0, b1, x1);
+            consume(e, a0, a1, x0);
+            e += K;
+            if (e >= eend) break;
+            if (e + K < eend) load(e + K, a0, a1, x0);
+            consume(e, b0, b1, x1);
+            e += K;
+            if (e >= eend) break;
+        }
+    }
+    while (q < q1) advance();
+}
+
 // k_lu_stream with the 2x2-block schedule (G = 64): same contract and, bit for bit, the
 // same factors and solutions. Per level: sub-level A = diagonal blocks and U blocks of
 // single pivots; sub-level B = the U blocks of pivot pairs, the L blocks and the forward
@@ -921,7 +1011,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
     const int* __restrict__ ptrA, const int4* __restrict__ blkA, const int* __restrict__ ptrB,
     const int4* __restrict__ blkB, const int4* __restrict__ steps, const int* __restrict__ ptrF,
     const int4* __restrict__ fwd, const int4* __restrict__ fsteps, const int* __restrict__ ptrW,
-    const int4* __restrict__ bwd, const int2* __restrict__ bent, const double* __restrict__ Jx,
+    const int4* __restrict__ bwd, const int4* __restrict__ bsteps, const double* __restrict__ Jx,
     real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x, int* __restrict__ status,
     int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips, int* __restrict__ vote,
     const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S) {
@@ -942,8 +1032,6 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
     pmin = DBL_MAX;
     pmax = 0.0;
     psum = 0.0;
-    auto startW = [&](int, int4 t) -> In2 { return {x[at(t.x, S, c)], (double)fac[at(t.y, S, c)]}; };
-    auto finishW = [&](int4 t, In2 in, double sum) { x[at(t.x, S, c)] = (in.v + sum) * in.d; };
 
     for (int l = 0; l < nlev; ++l) {
         const int i0 = l * TL + slot;
@@ -992,7 +1080,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
     }
     for (int l = 0; l < nblev; ++l) {
         const int i0 = l * TL + slot;
-        if (act) seg_stream<double, In2>(bwd, ptrW[i0], ptrW[i0 + 1], bent, fac, x, S, c, startW, finishW);
+        if (act) bblk_stream<4, real>(bwd, ptrW[i0], ptrW[i0 + 1], bsteps, fac, x, S, c);
         __syncthreads();
     }
 }
@@ -1658,6 +1746,7 @@ struct DevSched {  // one level schedule dealt to TL task lists
     DevBuf<int4> taskA, taskB, fwd, bwd;
     DevBuf<int4> steps;   // 2x2-block schedule: the block steps
     DevBuf<int4> fsteps;  // 2x2-block schedule: the forward-row steps (fwd: 2 int4 per group)
+    DevBuf<int4> bsteps;  // 2x2-block schedule: the backward-row steps (bwd: 2 int4 per group)
     DevBuf<int2> pairs, fent, bent;  // operand streams in task-list order
 };
 
@@ -1859,7 +1948,6 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
     std::vector<int4> st4(std::max<int64_t>(nsteps, 1), make_int4(0, -1, 0, -1));
     for (int64_t k = 0; k < nsteps; ++k)
         st4[k] = make_int4(s.steps[4 * k], s.steps[4 * k + 1], s.steps[4 * k + 2], s.steps[4 * k + 3]);
-    std::vector<int2> be;
     // forward rows by pivot group: {row, nr, sbeg, send}, {rsrc0, rsrc1, lpos, 0}; steps
     // {L(row, j), L(row+1, j) or -1, j, 0}
     const int64_t nfs = (int64_t)s.fsteps.size() / 3;
@@ -1883,19 +1971,36 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
     for (size_t r = 0; r + 1 < s.ptrF.size(); ++r)
         for (int q = s.ptrF[r]; q + 1 < s.ptrF[r + 1]; ++q)
             if (s.fwd[q].send != s.fwd[q + 1].sbeg) bad("forward steps not contiguous");
-    std::vector<int4> bw(s.bwd.size());
-    for (size_t i = 0; i < bw.size(); ++i) {
-        const auto& w = s.bwd[i];
-        const int b0 = (int)be.size();
-        for (int e = w.lbeg; e < w.lend; ++e) be.push_back(make_int2(w.base + 1 + (e - w.lbeg), h.Li[e]));
-        bw[i] = make_int4(w.row, w.base, b0, (int)be.size());
+    // backward rows by pivot group: {row, nr, sbeg, send}, {base0, base1, 0, 0}; steps
+    // {U(row, j), U(row+1, j) or -1, j, 0}
+    const int64_t nbs = (int64_t)s.bsteps.size() / 3;
+    std::vector<int4> bw(2 * std::max<size_t>(s.bwd.size(), 1), make_int4(0, 1, 0, 0));
+    std::vector<int4> bs4(std::max<int64_t>(nbs, 1), make_int4(0, -1, 0, 0));
+    for (int64_t k = 0; k < nbs; ++k) {
+        const int ua = s.bsteps[3 * k], ub = s.bsteps[3 * k + 1], j = s.bsteps[3 * k + 2];
+        if (ua < 0 || ua >= nlu || ub < -1 || ub >= nlu || j < 0 || j >= h.m) bad("backward step out of range");
+        bs4[k] = make_int4(ua, ub, j, 0);
     }
-    if (be.empty()) be.push_back(make_int2(0, 0));
+    for (size_t i = 0; i < s.bwd.size(); ++i) {
+        const p3smi::BwdBlock& w = s.bwd[i];
+        if (w.nr < 1 || w.nr > 2 || w.row < 0 || w.row + w.nr > h.m || w.base0 < 0 || w.base0 + 1 >= nlu + 1)
+            bad("backward row");
+        if (w.nr == 2 && (w.base1 < 0 || w.base1 >= nlu || w.base0 + 1 >= nlu)) bad("backward pair");
+        if (w.sbeg < 0 || w.send < w.sbeg || w.send > nbs) bad("backward step range");
+        for (int e = w.sbeg; e < w.send; ++e)
+            if ((s.bsteps[3 * e + 1] < 0) != (w.nr == 1)) bad("backward step shape");
+        bw[2 * i] = make_int4(w.row, w.nr, w.sbeg, w.send);
+        bw[2 * i + 1] = make_int4(w.base0, w.nr == 2 ? w.base1 : 0, 0, 0);
+    }
+    for (size_t r = 0; r + 1 < s.ptrW.size(); ++r)
+        for (int q = s.ptrW[r]; q + 1 < s.ptrW[r + 1]; ++q)
+            if (s.bwd[q].send != s.bwd[q + 1].sbeg) bad("backward steps not contiguous");
     d->pairs.alloc(1);
     d->steps.upload(st4.data(), st4.size(), st);
     d->fsteps.upload(fs.data(), fs.size(), st);
+    d->bsteps.upload(bs4.data(), bs4.size(), st);
     d->fent.alloc(1);
-    d->bent.upload(be.data(), be.size(), st);
+    d->bent.alloc(1);
     d->fwd.upload(fw.data(), fw.size(), st);
     d->bwd.upload(bw.data(), bw.size(), st);
     HIPCHECK(hipStreamSynchronize(st));
@@ -1952,7 +2057,9 @@ const DevSched& get_sched(p3smi_topology* t, LuConfig cfg) {
     for (size_t i = 0; i < bw.size(); ++i) {
         const auto& w = s.bwd[i];
         const int b0 = (int)be.size();
-        for (int e = w.lbeg; e < w.lend; ++e) be.push_back(make_int2(w.base + 1 + (e - w.lbeg), h.Li[e]));
+        // columns descending: every backward row sums in this order (the 2x2-block kernel adds a
+        // pair's own term U(p, p+1) x(p+1) last)
+        for (int e = w.lend - 1; e >= w.lbeg; --e) be.push_back(make_int2(w.base + 1 + (e - w.lbeg), h.Li[e]));
         bw[i] = make_int4(w.row, w.base, b0, (int)be.size());
     }
     if (pr.empty()) pr.push_back(make_int2(0, 0));  // keep every stream a valid pointer
@@ -2081,7 +2188,7 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* 
         if (ds.blk) {
             hipLaunchKernelGGL((k_lu_blk<NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
                                b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.steps.p, ds.ptrF.p,
-                               ds.fwd.p, ds.fsteps.p, ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout,
+                               ds.fwd.p, ds.fsteps.p, ds.ptrW.p, ds.bwd.p, ds.bsteps.p, b->Jx.p, fac, rhs, xout,
                                b->status.p, b->iters.p, b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min,
                                ds.nlev, ds.nblev, b->S);
             return;

@@ -567,7 +567,7 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
             }
             levB[lev].push_back(l);
         }
-        for (int p = p0; p <= last; ++p) levW[t.depth[p]].push_back(p);
+        levW[t.depth[last]].push_back(p0);  // backward rows by pivot group, at the group's depth
     }
     for (int p0 = 0; p0 < m; ++p0)  // forward rows by pivot group, at the group's level
         if (gsize[p0] != 0) levF[t.height[p0 + (gsize[p0] == 2 ? 1 : 0)]].push_back(p0);
@@ -635,7 +635,10 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
     }
     for (int l = 0; l < s.nblev; ++l) {
         std::vector<int64_t> load(nslots, 0), w;
-        for (int p : levW[l]) w.push_back(2 + (t.Lp[p + 1] - t.Lp[p]));
+        for (int p : levW[l]) {
+            const int last = p + (gsize[p] == 2 ? 1 : 0);
+            w.push_back(2 + (int64_t)(t.Lp[last + 1] - t.Lp[last]) * (gsize[p] == 2 ? 3 : 2));
+        }
         std::vector<int> slot = lpt_deal(w, nslots, load);
         for (size_t i = 0; i < levW[l].size(); ++i) { itW.push_back(levW[l][i]); lW.push_back(l); sW.push_back(slot[i]); }
     }
@@ -668,7 +671,21 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
     }
     for (int i : group_by_slot(lW, sW, s.nblev, nslots, s.ptrW)) {
         const int p = itW[i];
-        s.bwd.push_back({p, t.base[p], t.Lp[p], t.Lp[p + 1]});
+        const int nr = gsize[p] == 2 ? 2 : 1, last = p + nr - 1;
+        BwdBlock b{};
+        b.row = p;
+        b.nr = nr;
+        b.base0 = t.base[p];
+        b.base1 = nr == 2 ? t.base[p + 1] : -1;
+        b.sbeg = (int)(s.bsteps.size() / 3);
+        for (int q = t.Lp[last + 1] - 1; q >= t.Lp[last]; --q) {  // columns of S_last, descending
+            const int j = t.Li[q];
+            s.bsteps.push_back(posU(p, j));
+            s.bsteps.push_back(nr == 2 ? posU(p + 1, j) : -1);
+            s.bsteps.push_back(j);
+        }
+        b.send = (int)(s.bsteps.size() / 3);
+        s.bwd.push_back(b);
     }
     return s;
 }
@@ -764,7 +781,7 @@ void host_solve(const Topology& t, const double* fac, const double* R, double* x
         for (int q = s.ptrW[(size_t)l * s.nslots]; q < s.ptrW[(size_t)(l + 1) * s.nslots]; ++q) {
             const BwdRow& b = s.bwd[q];
             double acc = x[b.row];
-            for (int e = b.lbeg; e < b.lend; ++e) acc -= fac[b.base + 1 + (e - b.lbeg)] * x[t.Li[e]];
+            for (int e = b.lend - 1; e >= b.lbeg; --e) acc -= fac[b.base + 1 + (e - b.lbeg)] * x[t.Li[e]];
             x[b.row] = acc * fac[b.base];
         }
 }

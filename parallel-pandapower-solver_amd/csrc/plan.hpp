@@ -37,7 +37,7 @@ struct FwdRow {    // forward substitution row i: y_i = R[rsrc] - sum L(i,j) y_j
     int row, rsrc, ebeg, eend;
 };
 
-struct BwdRow {    // backward substitution row i: x_i = (y_i - sum U(i,j) x_j) / U(i,i)
+struct BwdRow {    // backward substitution row i: x_i = (y_i - sum U(i,j) x_j) / U(i,i), j descending
     int row, base, lbeg, lend;
 };
 
@@ -91,6 +91,15 @@ struct FwdBlock {
     int sbeg, send;        // forward steps [sbeg, send)
 };
 
+// Backward rows of a pivot group, columns j in descending order (the order of every
+// backward row): x(row+1) over S_{row+1}, and x(row) over the same columns followed, last,
+// by U(row, row+1) x(row+1).
+struct BwdBlock {
+    int row, nr;           // first row (pivot index), 1 or 2 rows
+    int base0, base1;      // 1/U(row,row) and (pair) 1/U(row+1,row+1) positions
+    int sbeg, send;        // backward steps [sbeg, send)
+};
+
 struct BlockSchedule {
     int nslots = 0, nlev = 0, nblev = 0;
     int npairs = 0;  // pivot pairs
@@ -101,7 +110,8 @@ struct BlockSchedule {
     std::vector<int> steps;               // 4 ints per step
     std::vector<FwdBlock> fwd;            // forward rows by pivot group
     std::vector<int> fsteps;              // 3 ints per step: L(row, j), L(row+1, j) (-1: one row), j
-    std::vector<BwdRow> bwd;
+    std::vector<BwdBlock> bwd;            // backward rows by pivot group
+    std::vector<int> bsteps;              // 3 ints per step: U(row, j), U(row+1, j) (-1: one row), j
 };
 
 

@@ -58,7 +58,14 @@ def test_branch_table_matches_restatement(gpu, solved):
         _close(getattr(tab, ours), ref[theirs], ours, (g, res, cs))
     best, arg = postproc.worst(ref["loading"])
     _close(tab.max_loading_percent, best)
-    assert np.array_equal(tab.worst_case, arg)
+    # the worst case of every branch; where two cases' loadings agree to ~1 ulp the argmax
+    # may fall on either, and then both must be maxima within the table tolerance
+    diff = np.nonzero(tab.worst_case != arg)[0]
+    assert diff.size <= 2, diff
+    for k in diff:
+        assert tab.worst_case[k] >= 0 and arg[k] >= 0, k
+        a_dev, a_ref = ref["loading"][k, tab.worst_case[k]], ref["loading"][k, arg[k]]
+        assert np.isclose(a_dev, a_ref, rtol=1e-12, atol=0), (k, a_dev, a_ref)
     # outaged branches are zero in their own case
     for c, gr in enumerate(cs.groups):
         if res.converged[c]:
