@@ -88,9 +88,9 @@ int p3smi_topology_set_max_chunk(p3smi_topology* t, int32_t cases);
 /* LU launch shape (no counterpart in the reference; set by the autotuner): a workgroup of
  * 64*ns threads factors g consecutive cases (g in {1, 16, 32, 64}, ns in {8, 16});
  * (few_g, few_ns) is used instead while at most few_max cases are still iterating. group 0
- * with g = 64 factors 2x2 blocks of paired pivots (the P and Q unknowns of a PQ bus) as one
- * task in factorizations; group > 0 deals runs of up to `group` entries of one pivot's
- * U row / L column to one task list (1: one entry per task). Every shape gives the same bits. */
+ * with g = 64 treats 2x2 blocks of paired pivots (the P and Q unknowns of a PQ bus) as one
+ * task; group > 0 deals runs of up to `group` entries of one pivot's U row / L column to one
+ * task list (1: one entry per task). Every shape gives the same bits. */
 int p3smi_topology_set_lu_shape(p3smi_topology* t, int32_t g, int32_t ns, int32_t few_g, int32_t few_ns,
                                 int32_t few_max, int32_t group);
 /* Mixed precision (no counterpart in the reference): refinement of the FP32-LU step stops

@@ -1758,7 +1758,7 @@ struct DevTopo {
 };
 
 // LU launch shape: G cases per workgroup, NS wavefronts; grp = 0: 2x2 supernode blocks
-// when G = 64 (k_lu_blk, for factorizations), single entries otherwise; grp > 0: runs of up to grp entries of
+// when G = 64 (k_lu_blk), single entries otherwise; grp > 0: runs of up to grp entries of
 // one pivot dealt together (1 = single entries).
 struct LuConfig {
     int G, NS, grp;
@@ -2184,7 +2184,7 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* 
                            ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout, b->status.p, b->iters.p,
                            b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min, ds.nlev, ds.nblev, b->S);
     };
-    if constexpr (G == 64 && FACTOR) {
+    if constexpr (G == 64) {
         if (ds.blk) {
             hipLaunchKernelGGL((k_lu_blk<NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
                                b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.steps.p, ds.ptrF.p,
@@ -2194,7 +2194,7 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* 
             return;
         }
     }
-    if (ds.blk) throw Error(P3SMI_E_INTERNAL, "2x2-block schedule launched for other than a factorization at G = 64");
+    if (ds.blk) throw Error(P3SMI_E_INTERNAL, "2x2-block schedule launched with G != 64");
     if constexpr (G >= 16) go(k_lu_stream<G, NS, real, FACTOR>);
     else go(k_lu<G, NS, real, FACTOR>);
 }
@@ -2210,9 +2210,6 @@ template <typename real, bool FACTOR>
 void launch_lu(p3smi_batch* b, int n_active, real* fac, const double* rhs, double* xout, int sel) {
     p3smi_topology* t = b->topo;
     LuConfig cfg = n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
-    // The 2x2 blocks serve factorizations. Solve-only launches (refinement) have no blocks;
-    // the factor layout is the same, so they run one entry per task with 16 wavefronts.
-    if (cfg.blk() && !FACTOR) cfg = LuConfig{64, 16, 1};
     const DevSched& ds = get_sched(t, cfg);
     KTimer tm(b, 1);
 #define P3SMI_LU_CASE(GG, NN)                                                                \

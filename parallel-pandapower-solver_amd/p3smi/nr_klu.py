@@ -91,7 +91,7 @@ class Solver:
     def set_lu_shape(self, g: int, ns: int, few_g: int = 1, few_ns: int = 16, few_max: int = 256, group: int = 0):
         """LU launch shape: g cases per workgroup of 64*ns threads; (few_g, few_ns) while at
         most few_max cases are still iterating. group 0 at g = 64 factors 2x2 blocks of paired
-        pivots per task (factorizations); group > 0 deals runs of a pivot's entries together (1: one entry
+        pivots per task; group > 0 deals runs of a pivot's entries together (1: one entry
         per task). Every shape gives the same bits."""
         check(lib().p3smi_topology_set_lu_shape(self._h, int(g), int(ns), int(few_g), int(few_ns), int(few_max),
                                                 int(group)))

@@ -116,7 +116,7 @@ def test_mixed_precision_perturbed_starts(gpu):
 
 
 # (cases per workgroup, wavefronts, group): group 0 at 64 cases is the 2x2-block schedule
-# (factorizations), group 1 the one-entry-per-task schedule of the same shape
+# (factorization and substitutions), group 1 the one-entry-per-task schedule of the same shape
 SHAPES = [(64, 16, 0), (64, 16, 1), (64, 8, 0), (64, 8, 1), (32, 16, 0), (32, 8, 0), (16, 16, 0), (16, 8, 0),
           (1, 16, 0), (1, 8, 0)]
 
