@@ -109,6 +109,16 @@ int p3smi_host_jacobian(const p3smi_topology* t, const double* S, const double* 
 /* Factor J (entry order) with the frozen plan and solve J x = R on the host; x in Jacobian
  * column order. pivots[0..2] = min |pivot|, max |pivot|, sum |pivot|. */
 int p3smi_host_factor_solve(const p3smi_topology* t, const double* Jx, const double* R, double* x, double* pivots);
+/* Host emulation of the device LU arithmetic (no counterpart in the reference; a check of
+ * the schedules): factor J (entry order) and solve J x = R on the one-entry schedule
+ * (blocks = 0) or the 2x2-block schedule (blocks = 1) dealt to nslots task lists, in FP64
+ * or FP32 factors (single = 1), summing as the kernels do; x in Jacobian column order.
+ * pivots[0..1] = min |pivot|, max |pivot|. checks[0] = schedule violations (factor
+ * positions written twice or never, read before written, the zero row written, a
+ * substitution reading an unsolved value; 0 for a valid schedule), [1] pivot pairs,
+ * [2] operand steps. Both schedules give the same x bit for bit. */
+int p3smi_host_lu_emulate(const p3smi_topology* t, int32_t blocks, int32_t nslots, int32_t single, const double* Jx,
+                          const double* R, double* x, double* pivots, int64_t* checks);
 /* Bus -> variable maps and the elimination order (m): xva/xvm (n), perm (m). */
 int p3smi_host_maps(const p3smi_topology* t, int32_t* row_p, int32_t* row_q, int32_t* perm);
 

@@ -2711,6 +2711,18 @@ int p3smi_host_factor_solve(const p3smi_topology* t, const double* Jx, const dou
     });
 }
 
+int p3smi_host_lu_emulate(const p3smi_topology* t, int32_t blocks, int32_t nslots, int32_t single, const double* Jx,
+                          const double* R, double* x, double* pivots, int64_t* checks) {
+    return guarded([&] {
+        if (!t || !Jx || !R || !x || !pivots || !checks) throw Error(P3SMI_E_ARG, "host_lu_emulate: null argument");
+        if (nslots < 1 || nslots > 4096) throw Error(P3SMI_E_ARG, "host_lu_emulate: nslots must be in [1, 4096]");
+        const p3smi::Topology& h = t->host;
+        std::vector<double> y(h.m);
+        p3smi::host_lu_emulate(h, blocks != 0, nslots, single != 0, Jx, R, y.data(), &pivots[0], &pivots[1], checks);
+        for (int p = 0; p < h.m; ++p) x[h.perm[p]] = y[p];
+    });
+}
+
 int p3smi_host_maps(const p3smi_topology* t, int32_t* row_p, int32_t* row_q, int32_t* perm) {
     return guarded([&] {
         const p3smi::Topology& h = t->host;

@@ -786,6 +786,225 @@ void host_solve(const Topology& t, const double* fac, const double* R, double* x
         }
 }
 
+// ------------------------------------------------------------------ LU emulation
+namespace {
+
+template <typename real>
+struct Emu {
+    const Topology& t;
+    std::vector<real> fac;
+    std::vector<char> written;
+    std::vector<double> x;
+    std::vector<char> solved;
+    int64_t bad = 0;
+    double lo = std::numeric_limits<double>::max(), hi = 0.0;
+    explicit Emu(const Topology& tt)
+        : t(tt), fac(tt.nnz_lu + 1, (real)0), written(tt.nnz_lu + 1, 0), x(tt.m, 0.0), solved(tt.m, 0) {
+        written[t.nnz_lu] = 1;  // the zero row is readable
+    }
+    real rd(int pos) {
+        if (pos < 0 || pos > t.nnz_lu || !written[pos]) { ++bad; return (real)0; }
+        return fac[pos];
+    }
+    void wr(int pos, real v) {
+        if (pos < 0 || pos >= t.nnz_lu || written[pos]) { ++bad; return; }
+        written[pos] = 1;
+        fac[pos] = v;
+    }
+    real pivot(real v) {
+        const double ab = std::fabs((double)v);
+        lo = std::min(lo, ab);
+        hi = std::max(hi, ab);
+        return (real)1 / v;
+    }
+    double xv(int i) {
+        if (!solved[i]) ++bad;
+        return x[i];
+    }
+    template <typename F>
+    void lists(const std::vector<int>& ptr, int l, int ns, F f) {  // lists of one sub-level, last first
+        for (int sl = ns - 1; sl >= 0; --sl)
+            for (int q = ptr[(size_t)l * ns + sl]; q < ptr[(size_t)l * ns + sl + 1]; ++q) f(q);
+    }
+    void backward_row(int row, int base, int lbeg, int lend) {
+        double acc = 0.0;
+        for (int e = lend - 1; e >= lbeg; --e) acc = std::fma(-(double)rd(base + 1 + (e - lbeg)), xv(t.Li[e]), acc);
+        x[row] = (x[row] + acc) * (double)rd(base);
+    }
+};
+
+template <typename real>
+void emulate_one(const Topology& t, int ns, const double* J, const double* R, Emu<real>& E, int64_t* checks) {
+    const Schedule s = make_schedule(t, ns, 0);
+    auto dot = [&](const Task& k) {
+        real acc = 0;
+        for (int e = k.pbeg; e < k.pend; ++e) acc = std::fma(-E.rd(t.pa[e]), E.rd(t.pb[e]), acc);
+        return acc;
+    };
+    for (int l = 0; l < s.nlev; ++l) {
+        E.lists(s.ptrA, l, ns, [&](int q) {
+            const Task& k = s.taskA[q];
+            real v = (real)(k.asrc >= 0 ? J[k.asrc] : 0.0) + dot(k);
+            E.wr(k.dst, s.isdiagA[q] ? E.pivot(v) : v);
+        });
+        E.lists(s.ptrB, l, ns, [&](int q) {
+            const Task& k = s.taskB[q];
+            E.wr(k.dst, ((real)(k.asrc >= 0 ? J[k.asrc] : 0.0) + dot(k)) * (real)(double)E.rd(s.diagB[q]));
+        });
+    }
+    for (int l = 0; l < s.nlev; ++l)
+        E.lists(s.ptrF, l, ns, [&](int q) {
+            const FwdRow& f = s.fwd[q];
+            double acc = 0.0;
+            for (int e = f.ebeg; e < f.eend; ++e) acc = std::fma(-(double)E.rd(t.Fpos[e]), E.xv(t.Fcol[e]), acc);
+            E.x[f.row] = R[f.rsrc] + acc;
+            E.solved[f.row] = 1;
+        });
+    std::fill(E.solved.begin(), E.solved.end(), 0);
+    for (int l = 0; l < s.nblev; ++l)
+        E.lists(s.ptrW, l, ns, [&](int q) {
+            const BwdRow& b = s.bwd[q];
+            E.backward_row(b.row, b.base, b.lbeg, b.lend);
+            E.solved[b.row] = 1;
+        });
+    checks[1] = 0;
+    checks[2] = t.n_pairs;
+}
+
+template <typename real>
+void emulate_blocks(const Topology& t, int ns, const double* J, const double* R, Emu<real>& E, int64_t* checks) {
+    const BlockSchedule s = make_block_schedule(t, ns);
+    auto run = [&](const BlockTask& k) {
+        real acc[4] = {0, 0, 0, 0};
+        for (int e = k.sbeg; e < k.send; ++e) {
+            const int* st = &s.steps[4 * (size_t)e];
+            if ((st[1] < 0) != (k.nr == 1) || (st[3] < 0) != (k.nc == 1)) ++E.bad;
+            const real a0 = E.rd(st[0]), a1 = st[1] >= 0 ? E.rd(st[1]) : (real)0;
+            const real b0 = E.rd(st[2]), b1 = st[3] >= 0 ? E.rd(st[3]) : (real)0;
+            acc[0] = std::fma(-a0, b0, acc[0]);
+            acc[1] = std::fma(-a0, b1, acc[1]);
+            acc[2] = std::fma(-a1, b0, acc[2]);
+            acc[3] = std::fma(-a1, b1, acc[3]);
+        }
+        real j[4];
+        for (int i = 0; i < 4; ++i) j[i] = (real)(k.src[i] >= 0 ? J[k.src[i]] : 0.0);
+        switch (k.kind) {
+            case BLK_DIAG1: E.wr(k.tgt0, E.pivot(j[0] + acc[0])); break;
+            case BLK_DIAG2: {
+                const real inv0 = E.pivot(j[0] + acc[0]), u01 = j[1] + acc[1], l10 = (j[2] + acc[2]) * inv0;
+                E.wr(k.tgt0, inv0);
+                E.wr(k.aux0, u01);
+                E.wr(k.aux1, l10);
+                E.wr(k.tgt1, E.pivot(j[3] + std::fma(-l10, u01, acc[3])));
+                break;
+            }
+            case BLK_U1:
+                E.wr(k.tgt0, j[0] + acc[0]);
+                if (k.nc == 2) E.wr(k.tgt0 + 1, j[1] + acc[1]);
+                break;
+            case BLK_U2: {
+                const real l10 = E.rd(k.aux0), u00 = j[0] + acc[0];
+                E.wr(k.tgt0, u00);
+                E.wr(k.tgt1, j[2] + std::fma(-l10, u00, acc[2]));
+                if (k.nc == 2) {
+                    const real u01 = j[1] + acc[1];
+                    E.wr(k.tgt0 + 1, u01);
+                    E.wr(k.tgt1 + 1, j[3] + std::fma(-l10, u01, acc[3]));
+                }
+                break;
+            }
+            case BLK_L1: {
+                const real inv = E.rd(k.aux0);
+                E.wr(k.tgt0, (j[0] + acc[0]) * inv);
+                if (k.nr == 2) E.wr(k.tgt0 + 1, (j[2] + acc[2]) * inv);
+                break;
+            }
+            default: {
+                const real inv0 = E.rd(k.aux0), u01 = E.rd(k.aux0 + 1), inv1 = E.rd(k.aux1);
+                const real l00 = (j[0] + acc[0]) * inv0;
+                E.wr(k.tgt0, l00);
+                E.wr(k.tgt1, (j[1] + std::fma(-l00, u01, acc[1])) * inv1);
+                if (k.nr == 2) {
+                    const real l10 = (j[2] + acc[2]) * inv0;
+                    E.wr(k.tgt0 + 1, l10);
+                    E.wr(k.tgt1 + 1, (j[3] + std::fma(-l10, u01, acc[3])) * inv1);
+                }
+            }
+        }
+    };
+    for (int l = 0; l < s.nlev; ++l) {
+        E.lists(s.ptrA, l, ns, [&](int q) { run(s.taskA[q]); });
+        E.lists(s.ptrB, l, ns, [&](int q) { run(s.taskB[q]); });
+    }
+    for (int l = 0; l < s.nlev; ++l)
+        E.lists(s.ptrF, l, ns, [&](int q) {
+            const FwdBlock& f = s.fwd[q];
+            double a0 = 0.0, a1 = 0.0;
+            for (int e = f.sbeg; e < f.send; ++e) {
+                const int* st = &s.fsteps[3 * (size_t)e];
+                if ((st[1] < 0) != (f.nr == 1)) ++E.bad;
+                const double y = E.xv(st[2]);
+                a0 = std::fma(-(double)E.rd(st[0]), y, a0);
+                if (st[1] >= 0) a1 = std::fma(-(double)E.rd(st[1]), y, a1);
+            }
+            const double y0 = R[f.rsrc0] + a0;
+            E.x[f.row] = y0;
+            E.solved[f.row] = 1;
+            if (f.nr == 2) {
+                E.x[f.row + 1] = R[f.rsrc1] + std::fma(-(double)E.rd(f.lpos), y0, a1);
+                E.solved[f.row + 1] = 1;
+            }
+        });
+    std::fill(E.solved.begin(), E.solved.end(), 0);
+    for (int l = 0; l < s.nblev; ++l)
+        E.lists(s.ptrW, l, ns, [&](int q) {
+            const BwdBlock& b = s.bwd[q];
+            double a0 = 0.0, a1 = 0.0;
+            for (int e = b.sbeg; e < b.send; ++e) {
+                const int* st = &s.bsteps[3 * (size_t)e];
+                if ((st[1] < 0) != (b.nr == 1)) ++E.bad;
+                const double xj = E.xv(st[2]);
+                a0 = std::fma(-(double)E.rd(st[0]), xj, a0);
+                if (st[1] >= 0) a1 = std::fma(-(double)E.rd(st[1]), xj, a1);
+            }
+            if (b.nr == 2) {
+                const double x1 = (E.x[b.row + 1] + a1) * (double)E.rd(b.base1);
+                E.x[b.row + 1] = x1;
+                E.solved[b.row + 1] = 1;
+                E.x[b.row] = (E.x[b.row] + std::fma(-(double)E.rd(b.base0 + 1), x1, a0)) * (double)E.rd(b.base0);
+            } else {
+                E.x[b.row] = (E.x[b.row] + a0) * (double)E.rd(b.base0);
+            }
+            E.solved[b.row] = 1;
+        });
+    checks[1] = s.npairs;
+    checks[2] = (int64_t)(s.steps.size() / 4);
+}
+
+template <typename real>
+void emulate(const Topology& t, bool blocks, int ns, const double* J, const double* R, double* x, double* pmin,
+             double* pmax, int64_t* checks) {
+    Emu<real> E(t);
+    if (blocks) emulate_blocks<real>(t, ns, J, R, E, checks);
+    else emulate_one<real>(t, ns, J, R, E, checks);
+    for (int64_t i = 0; i < t.nnz_lu; ++i) E.bad += !E.written[i];
+    if (E.fac[t.nnz_lu] != (real)0) ++E.bad;
+    for (int i = 0; i < t.m; ++i) E.bad += !E.solved[i];
+    std::copy(E.x.begin(), E.x.end(), x);
+    *pmin = E.lo;
+    *pmax = E.hi;
+    checks[0] = E.bad;
+}
+
+}  // namespace
+
+void host_lu_emulate(const Topology& t, bool blocks, int nslots, bool single, const double* J, const double* R,
+                     double* x, double* pmin, double* pmax, int64_t* checks) {
+    if (nslots < 1) throw std::runtime_error("nslots must be >= 1");
+    if (single) emulate<float>(t, blocks, nslots, J, R, x, pmin, pmax, checks);
+    else emulate<double>(t, blocks, nslots, J, R, x, pmin, pmax, checks);
+}
+
 // ------------------------------------------------------------------ entry points
 static double reference_pivot_ratio(const Topology& t) {
     std::vector<double> Vm(t.n, 1.0), Va(t.n, 0.0), S(2 * (size_t)t.n, 0.0), R(t.m), J(t.nnzJ), fac(t.nnz_lu);

@@ -166,4 +166,14 @@ void host_factor(const Topology& t, const double* J, double* fac, double* pmin, 
 // Solve with the factors: x (pivot space, m) from R (Jacobian row order, m).
 void host_solve(const Topology& t, const double* fac, const double* R, double* x);
 
+// Host emulation of the device LU arithmetic (sums from zero with fused multiply-adds in
+// stream order, start value added at the end, pivots inverted, backward columns descending)
+// on the one-entry schedule (blocks = false) or the 2x2-block schedule, dealt to nslots
+// lists, in FP64 or FP32 (single). Within a sub-level the lists run last to first, so an
+// entry read before the sub-level that writes it is caught. x in pivot space. checks:
+// [0] violations (a factor position written twice or never, read before written; the zero
+// row written; a substitution reading an unsolved value), [1] pivot pairs, [2] steps.
+void host_lu_emulate(const Topology& t, bool blocks, int nslots, bool single, const double* J, const double* R,
+                     double* x, double* pmin, double* pmax, int64_t* checks);
+
 }  // namespace p3smi

@@ -83,7 +83,7 @@ EXPORTS = [
     "p3smi_last_error", "p3smi_version", "p3smi_device_count",
     "p3smi_topology_create", "p3smi_topology_free", "p3smi_topology_info", "p3smi_update_y",
     "p3smi_topology_set_lu_shape", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_max_chunk", "p3smi_topology_set_mixed",
-    "p3smi_host_jacobian", "p3smi_host_factor_solve", "p3smi_host_maps",
+    "p3smi_host_jacobian", "p3smi_host_factor_solve", "p3smi_host_lu_emulate", "p3smi_host_maps",
     "p3smi_solve_batch", "p3smi_solve_batch_contingency", "p3smi_solve_batch_contingency_stamp",
     "p3smi_outages_branch_flows", "p3smi_outages_voltage_violations",
     "p3smi_batch_create", "p3smi_batch_free", "p3smi_batch_build_sbus", "p3smi_batch_download_sbus", "p3smi_batch_upload", "p3smi_batch_solve", "p3smi_batch_download",
@@ -119,6 +119,7 @@ def lib():
     L.p3smi_topology_set_mixed.argtypes = [P, _f64, _i32]
     L.p3smi_host_jacobian.argtypes = [P, P, P, P, P, P, P]
     L.p3smi_host_factor_solve.argtypes = [P, P, P, P, P]
+    L.p3smi_host_lu_emulate.argtypes = [P, _i32, _i32, _i32, P, P, P, P, P]
     L.p3smi_host_maps.argtypes = [P, P, P, P]
     L.p3smi_solve_batch.argtypes = [P, P, P, _i32, _i32, _i32, _f64, _i32, _i32, P, P, P, P]
     L.p3smi_solve_batch_contingency.argtypes = [P, P, P, P, P, P, _i32, _i32, _f64, _i32, _i32, P, P, P, P]

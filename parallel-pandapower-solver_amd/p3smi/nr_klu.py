@@ -252,6 +252,18 @@ class Solver:
                                             ptr(np.ascontiguousarray(R, np.float64)), ptr(x), ptr(piv)))
         return x, piv
 
+    def host_lu_emulate(self, Jx, R, blocks: bool, nslots: int = 16, single: bool = False):
+        """Host emulation of the device LU on the one-entry or the 2x2-block schedule (schedule
+        checks): returns x (Jacobian column order), (min, max) |pivot|, and the check counts
+        (violations, pivot pairs, operand steps)."""
+        x = np.empty(self.info["m"], np.float64)
+        piv = np.empty(2, np.float64)
+        checks = np.zeros(3, np.int64)
+        check(lib().p3smi_host_lu_emulate(self._h, int(blocks), int(nslots), int(single),
+                                          ptr(np.ascontiguousarray(Jx, np.float64)),
+                                          ptr(np.ascontiguousarray(R, np.float64)), ptr(x), ptr(piv), ptr(checks)))
+        return x, piv, checks
+
     def maps(self):
         """(row_p, row_q, perm): Jacobian row of each bus's P / Q equation and the pivot order."""
         row_p = np.empty(self.n, np.int32)

@@ -135,3 +135,41 @@ def test_solve_without_device_fails_loudly():
     s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
     with pytest.raises(RuntimeError, match="no HIP device"):
         s.solve_batch(np.repeat(g.sbus()[:, None], 2, axis=1), g.flat_start())
+
+
+@pytest.mark.parametrize("name", ["pf200", "ts2000"])
+@pytest.mark.parametrize("single", [False, True])
+def test_block_schedule_emulation_equals_one_entry_schedule(name, single):
+    """The 2x2-block schedule, run on the host with the kernels' arithmetic, writes every
+    factor position once, reads nothing before it is written (lists of a sub-level run last
+    to first), and gives the one-entry schedule's solution bit for bit, in FP64 and FP32
+    factors; the FP64 solution solves J x = R like scipy's sparse LU."""
+    g = grids.config_grid(name, 0)
+    Yp, Yj, Yx = g.csr()
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    for seed, nslots in ((0, 16), (1, 1024)):
+        V = _random_V(g, seed)
+        d = s.debug_J_at(g.sbus(), V)
+        R = -d["F"]
+        x1, p1, c1 = s.host_lu_emulate(d["Jx"], R, blocks=False, nslots=nslots, single=single)
+        x2, p2, c2 = s.host_lu_emulate(d["Jx"], R, blocks=True, nslots=nslots, single=single)
+        assert c1[0] == 0 and c2[0] == 0, (c1, c2)
+        assert c2[1] > 0.8 * len(g.pq) and c2[2] < c1[2]  # most PQ twins pair up; fewer steps than pairs
+        assert np.array_equal(x1.view(np.int64), x2.view(np.int64))
+        assert np.array_equal(p1, p2)
+        if not single:
+            J = sp.csr_matrix((d["Jx"], d["Jj"], d["Jp"]), shape=(d["m"], d["m"])).tocsc()
+            xr = spla.spsolve(J, R)
+            assert np.abs(x2 - xr).max() <= 1e-9 * max(1.0, np.abs(xr).max())
+        else:
+            x64, _, _ = s.host_lu_emulate(d["Jx"], R, blocks=True, nslots=nslots, single=False)
+            assert np.abs(x2 - x64).max() <= 1e-3 * max(1.0, np.abs(x64).max())  # FP32 factors, FP64 sums
+
+
+def test_block_emulation_rejects_bad_arguments():
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    d = s.debug_J_at(g.sbus(), g.flat_start())
+    with pytest.raises(RuntimeError, match="nslots"):
+        s.host_lu_emulate(d["Jx"], -d["F"], blocks=True, nslots=0)
