@@ -58,8 +58,10 @@ constexpr int NSTAT = 8;
 constexpr double ARMIJO_C = 1e-4;
 constexpr int LS_MAX_TRIALS = 10;
 constexpr double REUSE_RCOND_RATIO = 1e-3;
-constexpr int EVAL_WAVES = 4;         // wavefronts per k_eval workgroup
-constexpr int EVAL_BUSES_PER_WAVE = 16;
+// k_eval workgroups: 8 wavefronts of 4 buses (32 buses). Against 4 x 16, ts2000's k_eval +
+// k_decide (which reduces the per-workgroup maxima) take 5.01 instead of 5.35 ms per solve.
+constexpr int EVAL_WAVES = 8;
+constexpr int EVAL_BUSES_PER_WAVE = 4;
 
 struct Error : std::runtime_error {
     int code;
