@@ -36,6 +36,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -44,7 +45,10 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include <unistd.h>
 
 #include "p3smi.h"
 #include "plan.hpp"
@@ -1741,6 +1745,185 @@ struct DevBuf {
     }
 };
 
+// ---- host copies of the callers' arrays
+// Copies between pageable host memory and the device run at about half of what the link
+// does from pinned memory (the runtime stages them itself, on one thread), and a download
+// into pages the caller has not touched yet pays their faults on that one thread too. Large
+// transfers of the one-call entry points and of resident batches therefore go through a ring
+// of pinned slots: a few host threads pack (upload) or scatter (download) one slice of rows
+// while the DMA of the neighbouring slice runs.
+
+// Workers that run one function on every worker and return when all are done. One pool per
+// process, created on first use (again in a forked child) and never destroyed; callers from
+// several threads take turns.
+class HostPool {
+  public:
+    explicit HostPool(int n) : n_(n), pid_(getpid()) {
+        for (int i = 1; i < n; ++i) std::thread([this, i] { loop(i); }).detach();
+    }
+    int size() const { return n_; }
+    pid_t pid() const { return pid_; }
+    void run(const std::function<void(int)>& f) {
+        std::lock_guard<std::mutex> turn(run_mu_);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &f;
+            pending_ = n_ - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        f(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+  private:
+    void loop(int i) {
+        unsigned long long seen = 0;
+        for (;;) {
+            const std::function<void(int)>* f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                f = job_;
+            }
+            (*f)(i);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    const int n_;
+    const pid_t pid_;
+    std::mutex run_mu_, mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)>* job_ = nullptr;
+    int pending_ = 0;
+    unsigned long long gen_ = 0;
+};
+
+HostPool& host_pool() {
+    static std::mutex mu;
+    static HostPool* pool = nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!pool || pool->pid() != getpid()) {
+        const unsigned hc = std::thread::hardware_concurrency();
+        pool = new HostPool((int)std::max(1u, std::min(8u, hc / 2)));
+    }
+    return *pool;
+}
+
+// Pinned slots of one batch, allocated on its first large transfer.
+struct Staging {
+    static constexpr int NSLOT = 4;
+    static constexpr size_t SLOT = (size_t)8 << 20;
+    static constexpr size_t MIN_BYTES = (size_t)16 << 20;  // smaller transfers copy directly
+    char* buf[NSLOT] = {};
+    hipEvent_t ev[NSLOT] = {};
+    Staging() = default;
+    Staging(const Staging&) = delete;
+    Staging& operator=(const Staging&) = delete;
+    ~Staging() {
+        for (int s = 0; s < NSLOT; ++s) {
+            if (ev[s]) (void)hipEventSynchronize(ev[s]);
+            if (ev[s]) (void)hipEventDestroy(ev[s]);
+            if (buf[s]) (void)hipHostFree(buf[s]);
+        }
+    }
+    void ensure() {
+        for (int s = 0; s < NSLOT; ++s) {
+            if (!buf[s] && hipHostMalloc((void**)&buf[s], SLOT, hipHostMallocDefault) != hipSuccess) {
+                buf[s] = nullptr;
+                throw Error(P3SMI_E_MEMORY, "pinned staging allocation failed");
+            }
+            if (!ev[s]) HIPCHECK(hipEventCreateWithFlags(&ev[s], hipEventDisableTiming));
+        }
+    }
+};
+
+// Worker w of W copies bytes [split64(nb, w, W), split64(nb, w + 1, W)) of a slice: 64-byte
+// aligned cuts, the last worker ends at nb.
+size_t split64(size_t nb, size_t w, size_t W) { return w >= W ? nb : (nb * w / W) & ~(size_t)63; }
+
+// Bytes [b0, b1) of a slice packed row after row (row = rowb bytes) <-> rows of a strided array
+// (row r at base + r * ldb).
+void pack_range(char* packed, const char* base, size_t ldb, size_t rowb, size_t b0, size_t b1) {
+    while (b0 < b1) {
+        const size_t r = b0 / rowb, off = b0 % rowb, len = std::min(rowb - off, b1 - b0);
+        std::memcpy(packed + b0, base + r * ldb + off, len);
+        b0 += len;
+    }
+}
+void unpack_range(char* base, const char* packed, size_t ldb, size_t rowb, size_t b0, size_t b1) {
+    while (b0 < b1) {
+        const size_t r = b0 / rowb, off = b0 % rowb, len = std::min(rowb - off, b1 - b0);
+        std::memcpy(base + r * ldb + off, packed + b0, len);
+        b0 += len;
+    }
+}
+
+// Column block [c0, c0 + cols) of a (rows, ld) host array of `elem` bytes -> device (rows, cols),
+// queued on `st`.
+void upload_cols(Staging& sg, void* dst, const void* src, size_t rows, size_t ld, size_t c0, size_t cols,
+                 size_t elem, hipStream_t st) {
+    const size_t rowb = cols * elem, ldb = ld * elem;
+    const char* base = (const char*)src + c0 * elem;
+    if (rows * rowb < Staging::MIN_BYTES || rowb > Staging::SLOT) {
+        HIPCHECK(hipMemcpy2DAsync(dst, rowb, base, ldb, rowb, rows, hipMemcpyHostToDevice, st));
+        return;
+    }
+    sg.ensure();
+    HostPool& pool = host_pool();
+    const size_t prow = Staging::SLOT / rowb;  // rows per slice
+    for (size_t r0 = 0, i = 0; r0 < rows; r0 += prow, ++i) {
+        const int s = (int)(i % Staging::NSLOT);
+        HIPCHECK(hipEventSynchronize(sg.ev[s]));  // the slot's previous DMA is done
+        const size_t nb = std::min(prow, rows - r0) * rowb;
+        char* buf = sg.buf[s];
+        const char* sp = base + r0 * ldb;
+        const size_t W = (size_t)pool.size();
+        pool.run([&](int w) { pack_range(buf, sp, ldb, rowb, split64(nb, w, W), split64(nb, w + 1, W)); });
+        HIPCHECK(hipMemcpyAsync((char*)dst + r0 * rowb, buf, nb, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipEventRecord(sg.ev[s], st));
+    }
+}
+
+// Device (rows, cols) -> column block [c0, c0 + cols) of a (rows, ld) host array. Waits for the
+// work queued on `st` before it.
+void download_cols(Staging& sg, void* dst, const void* src, size_t rows, size_t ld, size_t c0, size_t cols,
+                   size_t elem, hipStream_t st) {
+    const size_t rowb = cols * elem, ldb = ld * elem;
+    char* base = (char*)dst + c0 * elem;
+    if (rows * rowb < Staging::MIN_BYTES || rowb > Staging::SLOT) {
+        HIPCHECK(hipMemcpy2DAsync(base, ldb, src, rowb, rowb, rows, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        return;
+    }
+    sg.ensure();
+    HostPool& pool = host_pool();
+    const size_t prow = Staging::SLOT / rowb;
+    const size_t nslice = (rows + prow - 1) / prow;
+    auto issue = [&](size_t i) {
+        const int s = (int)(i % Staging::NSLOT);
+        const size_t r0 = i * prow, nb = std::min(prow, rows - r0) * rowb;
+        HIPCHECK(hipMemcpyAsync(sg.buf[s], (const char*)src + r0 * rowb, nb, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipEventRecord(sg.ev[s], st));
+    };
+    for (int s = 0; s < Staging::NSLOT; ++s) HIPCHECK(hipEventSynchronize(sg.ev[s]));  // earlier uses
+    for (size_t i = 0; i < std::min(nslice, (size_t)Staging::NSLOT); ++i) issue(i);
+    for (size_t i = 0; i < nslice; ++i) {
+        const int s = (int)(i % Staging::NSLOT);
+        HIPCHECK(hipEventSynchronize(sg.ev[s]));
+        const size_t r0 = i * prow, nb = std::min(prow, rows - r0) * rowb;
+        const char* buf = sg.buf[s];
+        char* dp = base + r0 * ldb;
+        const size_t W = (size_t)pool.size();
+        pool.run([&](int w) { unpack_range(dp, buf, ldb, rowb, split64(nb, w, W), split64(nb, w + 1, W)); });
+        if (i + Staging::NSLOT < nslice) issue(i + Staging::NSLOT);
+    }
+}
+
 struct DevSched {  // one level schedule dealt to TL task lists
     int TL = 0, nlev = 0, nblev = 0;
     bool blk = false;  // 2x2-block schedule: taskA / taskB hold 3 int4 per block
@@ -1817,6 +2000,7 @@ struct p3smi_batch {
     int precision = 0;                  // precision of the solve in progress
     unsigned long long solve_id = 0;    // stamp of the last run_newton on this batch
     bool sbus_built = false;            // Ssp holds injections built on the device (k_sbus)
+    Staging stage;                      // pinned slots of large host transfers
     bool graphs = false;                // replay captured Newton iterations (FP64, timers off)
     std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gcache;
     p3smi_stats stats{};
@@ -2496,7 +2680,9 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
     collect_timing(b);
 }
 
-void download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* converged, double* mismatch) {
+// V goes to columns [c0, c0 + B) of the caller's (n, ld) array.
+void download(p3smi_batch* b, double* V_out, size_t ld, size_t c0, int32_t* iterations, uint8_t* converged,
+              double* mismatch) {
     const p3smi::Topology& h = b->topo->host;
     hipStream_t st = b->topo->stream;
     b->Vout.alloc((size_t)2 * h.n * b->B);
@@ -2504,10 +2690,10 @@ void download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* conve
                        b->mask_unserved ? b->served.p : nullptr, b->Vout.p, h.n, b->S, b->B);
     HIPCHECK(hipGetLastError());
     std::vector<int> status(b->B), it(b->B);
-    if (V_out) HIPCHECK(hipMemcpyAsync(V_out, b->Vout.p, (size_t)2 * h.n * b->B * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(status.data(), b->status.p, b->B * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(it.data(), b->iters.p, b->B * sizeof(int), hipMemcpyDeviceToHost, st));
     if (mismatch) HIPCHECK(hipMemcpyAsync(mismatch, b->nrm.p, b->B * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (V_out) download_cols(b->stage, V_out, b->Vout.p, h.n, ld, c0, b->B, 16, st);
     HIPCHECK(hipStreamSynchronize(st));
     for (int c = 0; c < b->B; ++c) {
         if (iterations) iterations[c] = it[c];
@@ -2527,14 +2713,6 @@ void check_v0(const double* V0, size_t rows, size_t cols, const char* who) {
                 throw Error(P3SMI_E_ARG, std::string(who) + "invalid voltage magnitude " + std::to_string(vm) +
                                              " at bus " + std::to_string(i));
         }
-}
-
-// Upload a column block [c0, c0 + cols) of a (rows, ld) host array of `elem` bytes into a
-// device (rows, cols) array.
-void upload_cols(void* dst, const void* src, size_t rows, size_t ld, size_t c0, size_t cols, size_t elem,
-                 hipStream_t st) {
-    HIPCHECK(hipMemcpy2DAsync(dst, cols * elem, (const char*)src + c0 * elem, ld * elem, cols * elem, rows,
-                              hipMemcpyHostToDevice, st));
 }
 
 size_t bytes_per_case(const p3smi::Topology& h, bool per_case_y, bool pin, bool per_case_s) {
@@ -2750,11 +2928,19 @@ int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_
         const p3smi::Topology& h = b->topo->host;
         hipStream_t st = b->topo->stream;
         check_v0(V0, h.n, v0_per_case ? b->B : 1, "solve_batch: ");
-        if (S) b->Ssp.upload(S, (size_t)2 * h.n * b->B, st);
+        if (S) {
+            b->Ssp.alloc((size_t)2 * h.n * b->B);
+            upload_cols(b->stage, b->Ssp.p, S, h.n, b->B, 0, b->B, 16, st);
+        }
         else if (!b->sbus_built) throw Error(P3SMI_E_ARG, "batch_upload: S is NULL and no Sbus was built on the device");
         b->sbus_built = false;
         b->s_stride = b->B;
-        b->V0.upload(V0, (size_t)2 * h.n * (v0_per_case ? b->B : 1), st);
+        if (v0_per_case) {
+            b->V0.alloc((size_t)2 * h.n * b->B);
+            upload_cols(b->stage, b->V0.p, V0, h.n, b->B, 0, b->B, 16, st);
+        } else {
+            b->V0.upload(V0, (size_t)2 * h.n, st);
+        }
         b->v0_stride = v0_per_case ? b->B : 0;
         b->per_case_y = false;
         b->has_pin = false;
@@ -2770,7 +2956,7 @@ int p3smi_batch_solve(p3smi_batch* b, int32_t max_iter, double tol, int32_t line
 }
 
 int p3smi_batch_download(p3smi_batch* b, double* V_out, int32_t* iterations, uint8_t* converged, double* mismatch) {
-    return guarded([&] { download(b, V_out, iterations, converged, mismatch); });
+    return guarded([&] { download(b, V_out, (size_t)b->B, 0, iterations, converged, mismatch); });
 }
 
 int p3smi_batch_build_sbus(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk, const int32_t* row,
@@ -2951,12 +3137,8 @@ static int solve_chunked(p3smi_topology* t, int T, bool per_case_y, bool pin_giv
         b->slot_mode = false;
         run_newton(b, max_iter, tol, line_search, precision);
         add_stats(total, b->stats);
-        std::vector<double> Vc(V_out && nc != T ? (size_t)2 * h.n * nc : 0);
-        download(b, V_out ? (nc == T ? V_out : Vc.data()) : nullptr, iters ? iters + c0 : nullptr,
-                 conv ? conv + c0 : nullptr, mismatch ? mismatch + c0 : nullptr);
-        if (V_out && nc != T)
-            for (int i = 0; i < h.n; ++i)
-                std::memcpy(V_out + 2 * ((size_t)i * T + c0), Vc.data() + 2 * (size_t)i * nc, (size_t)nc * 16);
+        download(b, V_out, (size_t)T, (size_t)c0, iters ? iters + c0 : nullptr, conv ? conv + c0 : nullptr,
+                 mismatch ? mismatch + c0 : nullptr);
     }
     b->stats = total;
     return (T + cs - 1) / cs;
@@ -2975,11 +3157,11 @@ int p3smi_solve_batch(p3smi_topology* t, const double* S, const double* V0, int3
         auto up = [&](p3smi_batch* b, int c0, int nc) {
             hipStream_t st = t->stream;
             b->Ssp.alloc((size_t)2 * h.n * nc);
-            upload_cols(b->Ssp.p, S, h.n, T, c0, nc, 16, st);
+            upload_cols(b->stage, b->Ssp.p, S, h.n, T, c0, nc, 16, st);
             b->s_stride = nc;
             if (v0_per_case) {
                 b->V0.alloc((size_t)2 * h.n * nc);
-                upload_cols(b->V0.p, V0, h.n, T, c0, nc, 16, st);
+                upload_cols(b->stage, b->V0.p, V0, h.n, T, c0, nc, 16, st);
                 b->v0_stride = nc;
             } else {
                 b->V0.upload(V0, (size_t)2 * h.n, st);
@@ -3007,7 +3189,7 @@ static void contingency_common(p3smi_topology* t, const double* S, const double*
         b->Ssp.upload(S, (size_t)2 * h.n, st);
         b->s_stride = 0;
         b->V0.alloc((size_t)2 * h.n * nc);
-        upload_cols(b->V0.p, V0, h.n, L, c0, nc, 16, st);
+        upload_cols(b->stage, b->V0.p, V0, h.n, L, c0, nc, 16, st);
         b->v0_stride = nc;
         b->Yg.alloc((size_t)h.nnzY * b->S);
         b->Yb.alloc((size_t)h.nnzY * b->S);
@@ -3038,8 +3220,8 @@ int p3smi_solve_batch_contingency(p3smi_topology* t, const double* Yx_mag, const
             DevBuf<double> mag, ang;
             mag.alloc((size_t)h.nnzY * nc);
             ang.alloc((size_t)h.nnzY * nc);
-            upload_cols(mag.p, Yx_mag, h.nnzY, L, c0, nc, 8, st);
-            upload_cols(ang.p, Yx_ang, h.nnzY, L, c0, nc, 8, st);
+            upload_cols(b->stage, mag.p, Yx_mag, h.nnzY, L, c0, nc, 8, st);
+            upload_cols(b->stage, ang.p, Yx_ang, h.nnzY, L, c0, nc, 8, st);
             hipLaunchKernelGGL(k_y_polar, grid_items(h.nnzY, b->S), dim3(256), 0, st, mag.p, ang.p, b->Yg.p, b->Yb.p,
                                h.nnzY, nc, nc, b->S);
             HIPCHECK(hipGetLastError());
