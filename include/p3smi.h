@@ -215,6 +215,11 @@ int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_
  * S; then p3smi_batch_upload with S = NULL sets V0 and keeps it. */
 int p3smi_batch_build_sbus(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk, const int32_t* row,
                            const double* coef, const double* base, const uint8_t* imag, int32_t nrows, const double* X);
+/* The same with X given as nblk blocks of rows, stacked in order: block k is a (blk_rows[k],
+ * cases) float64 array at blk[k] (one per element series; no host copy stacks them). */
+int p3smi_batch_build_sbus_blocks(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk,
+                                  const int32_t* row, const double* coef, const double* base, const uint8_t* imag,
+                                  int32_t nblk, const int32_t* blk_rows, const double* const* blk);
 /* S (n, cases) complex of the batch, as built or uploaded. */
 int p3smi_batch_download_sbus(p3smi_batch* b, double* S_out);
 /* Run the whole Newton solve of every case on the device from the uploaded inputs. */

@@ -2001,6 +2001,7 @@ struct p3smi_batch {
     unsigned long long solve_id = 0;    // stamp of the last run_newton on this batch
     bool sbus_built = false;            // Ssp holds injections built on the device (k_sbus)
     Staging stage;                      // pinned slots of large host transfers
+    DevBuf<double> sbX;                 // element series of the last device Sbus build
     bool graphs = false;                // replay captured Newton iterations (FP64, timers off)
     std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gcache;
     p3smi_stats stats{};
@@ -2959,37 +2960,63 @@ int p3smi_batch_download(p3smi_batch* b, double* V_out, int32_t* iterations, uin
     return guarded([&] { download(b, V_out, (size_t)b->B, 0, iterations, converged, mismatch); });
 }
 
+static void build_sbus(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk, const int32_t* row,
+                       const double* coef, const double* base, const uint8_t* imag, int32_t nblk,
+                       const int32_t* blk_rows, const double* const* blk) {
+    if (!b || !s0 || !ptr || (nk > 0 && (!row || !coef || !base || !imag)) || nblk < 0 || (nblk > 0 && !blk_rows))
+        throw Error(P3SMI_E_ARG, "build_sbus: null argument");
+    int64_t nrows = 0;
+    for (int k = 0; k < nblk; ++k) {
+        if (blk_rows[k] < 0) throw Error(P3SMI_E_ARG, "build_sbus: negative block row count");
+        if (blk_rows[k] > 0 && (!blk || !blk[k])) throw Error(P3SMI_E_ARG, "build_sbus: null argument");
+        nrows += blk_rows[k];
+    }
+    if (nrows > INT_MAX) throw Error(P3SMI_E_ARG, "build_sbus: too many series rows");
+    const p3smi::Topology& h = b->topo->host;
+    const int n = h.n, T = b->B;
+    if (ptr[0] != 0 || ptr[n] != nk) throw Error(P3SMI_E_ARG, "build_sbus: ptr must start at 0 and end at nk");
+    for (int i = 0; i < n; ++i)
+        if (ptr[i + 1] < ptr[i]) throw Error(P3SMI_E_ARG, "build_sbus: ptr is not monotone");
+    for (int k = 0; k < nk; ++k)
+        if (row[k] < 0 || row[k] >= nrows) throw Error(P3SMI_E_ARG, "build_sbus: row index out of range");
+    hipStream_t st = b->topo->stream;
+    DevBuf<double> ds0, dcoef, dbase;
+    DevBuf<int> dptr, drow;
+    DevBuf<unsigned char> dimag;
+    ds0.upload(s0, 2 * (size_t)n, st);
+    dptr.upload(ptr, (size_t)n + 1, st);
+    drow.upload(row, (size_t)nk, st);
+    dcoef.upload(coef, (size_t)nk, st);
+    dbase.upload(base, (size_t)nk, st);
+    dimag.upload(imag, (size_t)nk, st);
+    // the series, block after block, into the batch's (nrows, cases) array
+    if (b->sbX.n < (size_t)std::max<int64_t>(nrows, 1) * T) b->sbX.alloc((size_t)std::max<int64_t>(nrows, 1) * T);
+    for (int k = 0, r0 = 0; k < nblk; r0 += blk_rows[k], ++k)
+        if (blk_rows[k] > 0)
+            upload_cols(b->stage, b->sbX.p + (size_t)r0 * T, blk[k], (size_t)blk_rows[k], (size_t)T, 0, (size_t)T, 8,
+                        st);
+    b->Ssp.alloc((size_t)2 * n * T);
+    const dim3 grid((unsigned)((T + 255) / 256), (unsigned)std::min(n, 65535));
+    hipLaunchKernelGGL(k_sbus, grid, dim3(256), 0, st, ds0.p, dptr.p, drow.p, dcoef.p, dbase.p, dimag.p, b->sbX.p,
+                       b->Ssp.p, n, T);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    b->sbus_built = true;
+}
+
 int p3smi_batch_build_sbus(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk, const int32_t* row,
                            const double* coef, const double* base, const uint8_t* imag, int32_t nrows, const double* X) {
     return guarded([&] {
-        if (!b || !s0 || !ptr || (nk > 0 && (!row || !coef || !base || !imag)) || (nrows > 0 && !X))
-            throw Error(P3SMI_E_ARG, "build_sbus: null argument");
-        const p3smi::Topology& h = b->topo->host;
-        const int n = h.n, T = b->B;
-        if (ptr[0] != 0 || ptr[n] != nk) throw Error(P3SMI_E_ARG, "build_sbus: ptr must start at 0 and end at nk");
-        for (int i = 0; i < n; ++i)
-            if (ptr[i + 1] < ptr[i]) throw Error(P3SMI_E_ARG, "build_sbus: ptr is not monotone");
-        for (int k = 0; k < nk; ++k)
-            if (row[k] < 0 || row[k] >= nrows) throw Error(P3SMI_E_ARG, "build_sbus: row index out of range");
-        hipStream_t st = b->topo->stream;
-        DevBuf<double> ds0, dcoef, dbase, dX;
-        DevBuf<int> dptr, drow;
-        DevBuf<unsigned char> dimag;
-        ds0.upload(s0, 2 * (size_t)n, st);
-        dptr.upload(ptr, (size_t)n + 1, st);
-        drow.upload(row, (size_t)nk, st);
-        dcoef.upload(coef, (size_t)nk, st);
-        dbase.upload(base, (size_t)nk, st);
-        dimag.upload(imag, (size_t)nk, st);
-        dX.upload(X, (size_t)nrows * T, st);
-        b->Ssp.alloc((size_t)2 * n * T);
-        const dim3 grid((unsigned)((T + 255) / 256), (unsigned)std::min(n, 65535));
-        hipLaunchKernelGGL(k_sbus, grid, dim3(256), 0, st, ds0.p, dptr.p, drow.p, dcoef.p, dbase.p, dimag.p, dX.p,
-                           b->Ssp.p, n, T);
-        HIPCHECK(hipGetLastError());
-        HIPCHECK(hipStreamSynchronize(st));
-        b->sbus_built = true;
+        if (nrows < 0) throw Error(P3SMI_E_ARG, "build_sbus: nrows must be >= 0");
+        if (nrows > 0 && !X) throw Error(P3SMI_E_ARG, "build_sbus: null argument");
+        build_sbus(b, s0, ptr, nk, row, coef, base, imag, 1, &nrows, &X);
     });
+}
+
+int p3smi_batch_build_sbus_blocks(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk,
+                                  const int32_t* row, const double* coef, const double* base, const uint8_t* imag,
+                                  int32_t nblk, const int32_t* blk_rows, const double* const* blk) {
+    return guarded([&] { build_sbus(b, s0, ptr, nk, row, coef, base, imag, nblk, blk_rows, blk); });
 }
 
 int p3smi_batch_download_sbus(p3smi_batch* b, double* S_out) {

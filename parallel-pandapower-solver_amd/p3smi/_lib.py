@@ -86,7 +86,7 @@ EXPORTS = [
     "p3smi_host_jacobian", "p3smi_host_factor_solve", "p3smi_host_lu_emulate", "p3smi_host_maps",
     "p3smi_solve_batch", "p3smi_solve_batch_contingency", "p3smi_solve_batch_contingency_stamp",
     "p3smi_outages_branch_flows", "p3smi_outages_voltage_violations",
-    "p3smi_batch_create", "p3smi_batch_free", "p3smi_batch_build_sbus", "p3smi_batch_download_sbus", "p3smi_batch_upload", "p3smi_batch_solve", "p3smi_batch_download",
+    "p3smi_batch_create", "p3smi_batch_free", "p3smi_batch_build_sbus", "p3smi_batch_build_sbus_blocks", "p3smi_batch_download_sbus", "p3smi_batch_upload", "p3smi_batch_solve", "p3smi_batch_download",
     "p3smi_topology_release_cache", "p3smi_topology_last_batch", "p3smi_batch_vrange", "p3smi_batch_stats", "p3smi_batch_set_graphs", "p3smi_topology_set_graphs", "p3smi_batch_set_timing", "p3smi_batch_bytes_per_case",
     "p3smi_outages_create", "p3smi_outages_free", "p3smi_outages_islands", "p3smi_outages_solve",
     "p3smi_batch_slots_init", "p3smi_batch_load", "p3smi_batch_pass", "p3smi_batch_finished", "p3smi_batch_fetch",
@@ -135,6 +135,7 @@ def lib():
                                              P]
     L.p3smi_outages_voltage_violations.argtypes = [P, P, P, P, C.c_double, C.c_int32, P, P, P, P, P]
     L.p3smi_batch_build_sbus.argtypes = [P, P, P, C.c_int32, P, P, P, P, C.c_int32, P]
+    L.p3smi_batch_build_sbus_blocks.argtypes = [P, P, P, C.c_int32, P, P, P, P, C.c_int32, P, P]
     L.p3smi_batch_download_sbus.argtypes = [P, P]
     L.p3smi_topology_release_cache.argtypes = [P]
     L.p3smi_topology_last_batch.argtypes = [P, P]

@@ -121,12 +121,29 @@ class SbusBuilder:
             raise ValueError("SbusBuilder: series rows do not match the sources")
         return X
 
+    def blocks(self, series) -> list:
+        """The series as float64 C-contiguous (rows, T) arrays (copies only where the caller's
+        are not), rows adding up to the sources' rows."""
+        X = [np.ascontiguousarray(x, dtype=np.float64) for x in series]
+        if any(x.ndim != 2 for x in X) or sum(x.shape[0] for x in X) != self.nrows:
+            raise ValueError("SbusBuilder: series rows do not match the sources")
+        if len({x.shape[1] for x in X}) > 1:
+            raise ValueError("SbusBuilder: series differ in length")
+        return X
+
     def build(self, batch_handle, series) -> None:
-        """S of the batch (n, cases) from the series, on the device."""
+        """S of the batch (n, cases) from the series, on the device. Every series holds as
+        many columns as the batch has cases. The series go to the device one by one (no
+        stacked host copy)."""
+        import ctypes as C
+
         from . import _lib
 
-        X = self.stack(series)
+        X = self.blocks(series)
+        rows = np.array([x.shape[0] for x in X], dtype=np.int32)
+        blk = (C.c_void_p * max(len(X), 1))(*[x.ctypes.data for x in X])
         L = _lib.lib()
-        _lib.check(L.p3smi_batch_build_sbus(batch_handle, _lib.ptr(self.s0), _lib.ptr(self.ptr), int(self.row.size),
-                                            _lib.ptr(self.row), _lib.ptr(self.coef), _lib.ptr(self.base),
-                                            _lib.ptr(self.imag), int(self.nrows), _lib.ptr(X)))
+        _lib.check(L.p3smi_batch_build_sbus_blocks(batch_handle, _lib.ptr(self.s0), _lib.ptr(self.ptr),
+                                                   int(self.row.size), _lib.ptr(self.row), _lib.ptr(self.coef),
+                                                   _lib.ptr(self.base), _lib.ptr(self.imag), len(X), _lib.ptr(rows),
+                                                   C.cast(blk, C.c_void_p)))

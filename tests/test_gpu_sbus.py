@@ -64,3 +64,34 @@ def test_device_sbus_equals_restatement_and_solves_alike(gpu):
     assert cv.astype(bool).all()
     assert np.array_equal(it, ref["iterations"])
     assert np.array_equal(V.view(np.float64), ref["V"].view(np.float64))
+
+
+def test_device_sbus_large_series_both_entry_points(gpu):
+    """4096 cases: the load series (a row per PQ bus, above 16 MB each) go through the pinned
+    slots. The block entry point (series one by one) and the stacked-array entry point write
+    the same S as the restatement."""
+    g = grids.config_grid("ts2000", 0)
+    Yp, Yj, Yx = g.csr()
+    T = 4096
+    s0 = g.sbus()
+    sources, series = _sources(g, T, 6)
+    host = build_sbus(g.n, s0, sources, series)
+    sb = SbusBuilder(g.n, s0, sources)
+    solver = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    L = _lib.lib()
+    bh = C.c_void_p()
+    _lib.check(L.p3smi_batch_create(solver._h, T, C.byref(bh)))
+    try:
+        S_dev = np.empty((g.n, T), dtype=np.complex128)
+        sb.build(bh, series)
+        _lib.check(L.p3smi_batch_download_sbus(bh, _lib.ptr(S_dev)))
+        assert np.array_equal(S_dev.view(np.float64), host.view(np.float64))
+        X = sb.stack(series)
+        _lib.check(L.p3smi_batch_build_sbus(bh, _lib.ptr(sb.s0), _lib.ptr(sb.ptr), int(sb.row.size), _lib.ptr(sb.row),
+                                            _lib.ptr(sb.coef), _lib.ptr(sb.base), _lib.ptr(sb.imag), int(sb.nrows),
+                                            _lib.ptr(X)))
+        S_dev[...] = 0
+        _lib.check(L.p3smi_batch_download_sbus(bh, _lib.ptr(S_dev)))
+        assert np.array_equal(S_dev.view(np.float64), host.view(np.float64))
+    finally:
+        L.p3smi_batch_free(bh)

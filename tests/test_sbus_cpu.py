@@ -81,3 +81,11 @@ def test_builder_rejects_bad_inputs():
     sb = SbusBuilder(3, np.zeros(3, complex), [{"bus": [1, 2], "coef": [1.0, 1.0], "base": [0.0, 0.0], "imag": False}])
     with pytest.raises(ValueError):
         sb.stack([np.zeros((3, 4))])
+    with pytest.raises(ValueError):
+        sb.blocks([np.zeros((1, 4)), np.zeros((2, 4))])  # 3 rows for 2 elements
+    with pytest.raises(ValueError):
+        sb.blocks([np.zeros((1, 4)), np.zeros((1, 5))])  # series of different lengths
+    with pytest.raises(ValueError):
+        sb.blocks([np.zeros(8)])
+    X = sb.blocks([np.arange(4.0)[None, :], np.ones((1, 4), np.float32)])
+    assert [x.dtype for x in X] == [np.float64, np.float64] and all(x.flags.c_contiguous for x in X)
