@@ -3023,8 +3023,7 @@ int p3smi_batch_download_sbus(p3smi_batch* b, double* S_out) {
     return guarded([&] {
         const p3smi::Topology& h = b->topo->host;
         if (!b->sbus_built && b->s_stride != b->B) throw Error(P3SMI_E_ARG, "download_sbus: the batch has no (n, B) Sbus");
-        HIPCHECK(hipMemcpyAsync(S_out, b->Ssp.p, (size_t)2 * h.n * b->B * sizeof(double), hipMemcpyDeviceToHost,
-                                b->topo->stream));
+        download_cols(b->stage, S_out, b->Ssp.p, h.n, b->B, 0, b->B, 16, b->topo->stream);
         HIPCHECK(hipStreamSynchronize(b->topo->stream));
     });
 }
