@@ -95,3 +95,40 @@ def test_device_sbus_large_series_both_entry_points(gpu):
         assert np.array_equal(S_dev.view(np.float64), host.view(np.float64))
     finally:
         L.p3smi_batch_free(bh)
+
+
+def test_device_sbus_blocks_rejects_bad_arguments(gpu):
+    """The block entry point checks its blocks before any copy: a negative row count, a missing
+    block pointer, and contributions indexing past the rows the blocks add up to."""
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    T = 8
+    sources, series = _sources(g, T, 7)
+    sb = SbusBuilder(g.n, g.sbus(), sources)
+    X = sb.blocks(series)
+    solver = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    L = _lib.lib()
+    bh = C.c_void_p()
+    _lib.check(L.p3smi_batch_create(solver._h, T, C.byref(bh)))
+
+    def call(rows, ptrs):
+        rows = np.asarray(rows, dtype=np.int32)
+        blk = (C.c_void_p * max(len(ptrs), 1))(*ptrs)
+        return L.p3smi_batch_build_sbus_blocks(bh, _lib.ptr(sb.s0), _lib.ptr(sb.ptr), int(sb.row.size),
+                                               _lib.ptr(sb.row), _lib.ptr(sb.coef), _lib.ptr(sb.base),
+                                               _lib.ptr(sb.imag), len(rows), _lib.ptr(rows), C.cast(blk, C.c_void_p))
+
+    try:
+        good_rows = [x.shape[0] for x in X]
+        good_ptrs = [x.ctypes.data for x in X]
+        assert call(good_rows, good_ptrs) == 0
+        bad = list(good_rows)
+        bad[0] = -1
+        assert call(bad, good_ptrs) != 0
+        assert "negative" in L.p3smi_last_error().decode()
+        assert call(good_rows, [good_ptrs[0], None, good_ptrs[2]]) != 0
+        assert "null" in L.p3smi_last_error().decode()
+        assert call(good_rows[:2], good_ptrs[:2]) != 0  # the generator rows are missing
+        assert "row index out of range" in L.p3smi_last_error().decode()
+    finally:
+        L.p3smi_batch_free(bh)
