@@ -65,6 +65,9 @@ const char* p3smi_last_error(void);
 int p3smi_version(void);
 /* Number of visible HIP devices (0 without a GPU); never initialises a context. */
 int p3smi_device_count(int32_t* count);
+/* Free and total device memory in bytes (no counterpart in the reference; the serving
+ * layer sizes its admission budget from it). Fails without a device. */
+int p3smi_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 
 /* ---- topology (replaces nr_klu.Solver.__init__, p3s/cpp/nr_klu.cpp:966-994, and
  *      PolarNewtonSolverCUDA.__init__, p3s/cuda/nr_polar_solver.py:282-376) ---------------
@@ -216,10 +219,11 @@ int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_
 int p3smi_batch_build_sbus(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk, const int32_t* row,
                            const double* coef, const double* base, const uint8_t* imag, int32_t nrows, const double* X);
 /* The same with X given as nblk blocks of rows, stacked in order: block k is a (blk_rows[k],
- * cases) float64 array at blk[k] (one per element series; no host copy stacks them). */
+ * cols) float64 array at blk[k] (one per element series; no host copy stacks them). cols must
+ * equal the batch's case count (the call fails otherwise, before any copy). */
 int p3smi_batch_build_sbus_blocks(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk,
                                   const int32_t* row, const double* coef, const double* base, const uint8_t* imag,
-                                  int32_t nblk, const int32_t* blk_rows, const double* const* blk);
+                                  int32_t cols, int32_t nblk, const int32_t* blk_rows, const double* const* blk);
 /* S (n, cases) complex of the batch, as built or uploaded. */
 int p3smi_batch_download_sbus(p3smi_batch* b, double* S_out);
 /* Run the whole Newton solve of every case on the device from the uploaded inputs. */

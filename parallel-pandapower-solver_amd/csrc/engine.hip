@@ -2785,6 +2785,19 @@ int p3smi_device_count(int32_t* count) {
     return P3SMI_OK;
 }
 
+int p3smi_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+    return guarded([&] {
+        if (!free_bytes || !total_bytes) throw Error(P3SMI_E_ARG, "device_memory: null argument");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw Error(P3SMI_E_DEVICE, "device_memory: no HIP device available");
+        size_t f = 0, t = 0;
+        HIPCHECK(hipMemGetInfo(&f, &t));
+        *free_bytes = (int64_t)f;
+        *total_bytes = (int64_t)t;
+    });
+}
+
 int p3smi_topology_create(const int32_t* Yp, const int32_t* Yj, const double* Yx, int32_t n, int32_t nnz,
                           const int32_t* pv, int32_t npv, const int32_t* pq, int32_t npq, int32_t ordering,
                           int32_t n_slots, p3smi_topology** out) {
@@ -3015,8 +3028,15 @@ int p3smi_batch_build_sbus(p3smi_batch* b, const double* s0, const int32_t* ptr,
 
 int p3smi_batch_build_sbus_blocks(p3smi_batch* b, const double* s0, const int32_t* ptr, int32_t nk,
                                   const int32_t* row, const double* coef, const double* base, const uint8_t* imag,
-                                  int32_t nblk, const int32_t* blk_rows, const double* const* blk) {
-    return guarded([&] { build_sbus(b, s0, ptr, nk, row, coef, base, imag, nblk, blk_rows, blk); });
+                                  int32_t cols, int32_t nblk, const int32_t* blk_rows, const double* const* blk) {
+    return guarded([&] {
+        if (!b) throw Error(P3SMI_E_ARG, "build_sbus: null argument");
+        // every block is read as (rows, cols) with row stride cols: the width must be the batch's
+        if (cols != b->B)
+            throw Error(P3SMI_E_ARG, "build_sbus: the series have " + std::to_string(cols) + " columns, the batch " +
+                                         std::to_string(b->B) + " cases");
+        build_sbus(b, s0, ptr, nk, row, coef, base, imag, nblk, blk_rows, blk);
+    });
 }
 
 int p3smi_batch_download_sbus(p3smi_batch* b, double* S_out) {

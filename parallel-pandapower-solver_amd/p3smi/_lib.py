@@ -80,7 +80,7 @@ class Stats(C.Structure):
 
 # every symbol include/p3smi.h declares
 EXPORTS = [
-    "p3smi_last_error", "p3smi_version", "p3smi_device_count",
+    "p3smi_last_error", "p3smi_version", "p3smi_device_count", "p3smi_device_memory",
     "p3smi_topology_create", "p3smi_topology_free", "p3smi_topology_info", "p3smi_update_y",
     "p3smi_topology_set_lu_shape", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_max_chunk", "p3smi_topology_set_mixed",
     "p3smi_host_jacobian", "p3smi_host_factor_solve", "p3smi_host_lu_emulate", "p3smi_host_maps",
@@ -109,6 +109,7 @@ def lib():
     L.p3smi_last_error.restype = C.c_char_p
     L.p3smi_version.restype = C.c_int
     L.p3smi_device_count.argtypes = [P]
+    L.p3smi_device_memory.argtypes = [P, P]
     L.p3smi_topology_create.argtypes = [P, P, P, _i32, _i32, P, _i32, P, _i32, _i32, _i32, P]
     L.p3smi_topology_free.argtypes = [P]
     L.p3smi_topology_free.restype = None
@@ -135,7 +136,7 @@ def lib():
                                              P]
     L.p3smi_outages_voltage_violations.argtypes = [P, P, P, P, C.c_double, C.c_int32, P, P, P, P, P]
     L.p3smi_batch_build_sbus.argtypes = [P, P, P, C.c_int32, P, P, P, P, C.c_int32, P]
-    L.p3smi_batch_build_sbus_blocks.argtypes = [P, P, P, C.c_int32, P, P, P, P, C.c_int32, P, P]
+    L.p3smi_batch_build_sbus_blocks.argtypes = [P, P, P, C.c_int32, P, P, P, P, C.c_int32, C.c_int32, P, P]
     L.p3smi_batch_download_sbus.argtypes = [P, P]
     L.p3smi_topology_release_cache.argtypes = [P]
     L.p3smi_topology_last_batch.argtypes = [P, P]
@@ -174,3 +175,10 @@ def device_count() -> int:
     n = C.c_int32(0)
     lib().p3smi_device_count(C.byref(n))
     return int(n.value)
+
+
+def device_memory() -> tuple:
+    """(free, total) device memory in bytes; raises without a device."""
+    f, t = C.c_int64(0), C.c_int64(0)
+    check(lib().p3smi_device_memory(C.byref(f), C.byref(t)), "device_memory: ")
+    return int(f.value), int(t.value)

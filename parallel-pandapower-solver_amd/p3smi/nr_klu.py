@@ -233,8 +233,9 @@ class Solver:
         return {"V": V, "iterations": it, "converged": cv.astype(bool), "mismatch": mis}
 
     # ---------------------------------------------------------------- host checks
-    def debug_J_at(self, Sbus, V0):
-        """Jacobian (CSR by equation row) and mismatch F = Scalc - Sspec at V0, on the host."""
+    def jacobian_rows_at(self, Sbus, V0):
+        """Jacobian in this engine's own layout (CSR by equation row, columns ascending) and
+        the mismatch F = Scalc - Sspec at V0, evaluated on the host."""
         m, nnzJ = self.info["m"], self.info["nnz_j"]
         Jp = np.empty(m + 1, np.int32)
         Jj = np.empty(nnzJ, np.int32)
@@ -243,6 +244,26 @@ class Solver:
         check(lib().p3smi_host_jacobian(self._h, ptr(_c128(Sbus).ravel()), ptr(_c128(V0).ravel()), ptr(Jp), ptr(Jj),
                                         ptr(Jx), ptr(R)))
         return {"Jp": Jp, "Jj": Jj, "Jx": Jx, "m": m, "F": -R}
+
+    def debug_J_at(self, Sbus, V0):
+        """The Jacobian at V0 as the reference's ``debug_J_at`` returns it
+        (p3s/cpp/nr_klu.cpp:1284-1304): CSC ``{Jp (m+1), Ji (nnz), Jx (nnz), m}`` in the order
+        of ``build_J_pattern`` (p3s/cpp/nr_klu.cpp:230-285). Column c < npvpq is dVa of bus
+        pvpq[c], the others dVm of bus pq[c - npvpq]; row r < npvpq is P of pvpq[r], the others
+        Q of pq[r - npvpq]; within a column the P rows come first, then the Q rows, each in
+        ascending bus order (the Ybus column order). ``F`` = Scalc - Sspec is added."""
+        r = self.jacobian_rows_at(Sbus, V0)
+        m = r["m"]
+        npvpq = int(self.pv.size + self.pq.size)
+        bus_of_row = np.concatenate([self.pv, self.pq, self.pq]).astype(np.int64)  # pvpq = [pv; pq]
+        rows = np.repeat(np.arange(m, dtype=np.int64), np.diff(r["Jp"]))
+        cols = r["Jj"].astype(np.int64)
+        # sort entries by (column, Q after P, bus of the row)
+        order = np.lexsort((bus_of_row[rows], rows >= npvpq, cols))
+        Jp = np.zeros(m + 1, dtype=np.int32)
+        Jp[1:] = np.cumsum(np.bincount(cols, minlength=m))
+        return {"Jp": Jp, "Ji": np.ascontiguousarray(rows[order], dtype=np.int32),
+                "Jx": np.ascontiguousarray(r["Jx"][order]), "m": m, "F": r["F"]}
 
     def host_factor_solve(self, Jx, R):
         """Factor J with the frozen plan and solve J x = R on the host (plan checks)."""
@@ -288,7 +309,8 @@ def solve_single(Yp, Yj, Yx, Sbus, V0, pv, pq, max_iter: int = 30, tol: float = 
 
 
 def debug_J(Yp, Yj, Yx, Sbus, V0, pv, pq):
-    """Jacobian at V0 (p3s/cpp/nr_klu.cpp:1332-1342), in this engine's CSR row order."""
+    """Jacobian at V0 as CSC ``{Jp, Ji, Jx, m}`` in the reference's row and column order
+    (p3s/cpp/nr_klu.cpp:1332-1342)."""
     return Solver(Yp, Yj, Yx, pv, pq).debug_J_at(Sbus, V0)
 
 

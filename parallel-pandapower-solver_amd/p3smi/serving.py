@@ -122,16 +122,21 @@ class _Pool:
         self.thread = threading.Thread(target=self._run, daemon=True)
         self.stop = False
         self.busy = False
+        self.dead = None  # the exception that ended the worker, if one did
         self.thread.start()
 
-    def enqueue(self, req):
+    def enqueue(self, req) -> bool:
+        """Queue a request; False if the pool's worker has died (the caller routes it anew)."""
         with self.cv:
+            if self.dead is not None:
+                return False
             self.queue.append([req, 0])
             self.cv.notify()
+        return True
 
     def idle(self) -> bool:
         with self.cv:
-            return not self.queue and not self.owner and not self.busy
+            return self.dead is not None or (not self.queue and not self.owner and not self.busy)
 
     def _admit(self):
         """Move queued cases into free slots (one load call per request chunk)."""
@@ -204,11 +209,34 @@ class _Pool:
                 if self.stop:
                     return
                 self.busy = True
-            self._admit()
-            _lib.check(self.L.p3smi_batch_pass(self.bh, srv.max_iter, srv.tol, int(srv.line_search), srv.prec,
-                                               _lib.ptr(counts)))
-            self._harvest()
+            try:
+                self._admit()
+                _lib.check(self.L.p3smi_batch_pass(self.bh, srv.max_iter, srv.tol, int(srv.line_search), srv.prec,
+                                                   _lib.ptr(counts)))
+                self._harvest()
+            except Exception as e:  # the pool cannot go on: fail everything it holds
+                self._fail_all(e)
+                return
             self.last_used = time.perf_counter()
+
+    def _fail_all(self, err):
+        """End every request the pool holds (queued or in slots) with `err` and mark the pool
+        dead: the server evicts it and routes later requests of its topology to a new pool."""
+        with self.cv:
+            self.dead = err
+            self.busy = False
+            reqs = {id(item[0]): item[0] for item in self.queue}
+            reqs.update({id(r): r for r, _ in self.owner.values()})
+            self.queue.clear()
+            self.owner.clear()
+        now = time.perf_counter()
+        for req in reqs.values():
+            if req._done.is_set():
+                continue
+            req.error = err
+            req.t_done = now
+            req._done.set()
+            self.server._finished(req)
 
     def close(self):
         with self.cv:
@@ -217,6 +245,9 @@ class _Pool:
         self.thread.join()
         self.L.p3smi_batch_free(self.bh)
         self.bh = None
+        # requests still held (a pool closed while busy) end with an error, not a hang
+        if self.queue or self.owner:
+            self._fail_all(RuntimeError("serving: pool closed"))
 
 
 class Server:
@@ -231,12 +262,8 @@ class Server:
         self.line_search = bool(line_search)
         self.max_slots = int(max_slots)
         if mem_budget is None:
-            free = C.c_size_t(0)
-            total = C.c_size_t(0)
-            hip = C.CDLL("libamdhip64.so")
-            if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
-                raise RuntimeError("serving: hipMemGetInfo failed (no device?)")
-            mem_budget = int(free.value * mem_fraction)
+            free, _total = _lib.device_memory()
+            mem_budget = int(free * mem_fraction)
         self.budget = int(mem_budget)
         self.used = 0
         self.pools: OrderedDict[str, _Pool] = OrderedDict()
@@ -259,18 +286,20 @@ class Server:
 
     def _route(self, req, topo):
         want = min(self.max_slots, _round64(req.B))
-        with self.lock:
-            pool = self.pools.get(req.key)
-            if pool is not None and pool.capacity < want and pool.idle():
-                self._evict(req.key)  # rebuilt below with room for this request
-                pool = None
-            if pool is None:
-                pool = self._make_pool(req.key, *topo, want)
-            if pool is None:
-                self.pending.append((req, topo))
+        while True:
+            with self.lock:
+                pool = self.pools.get(req.key)
+                if pool is not None and (pool.dead is not None or (pool.capacity < want and pool.idle())):
+                    self._evict(req.key)  # a dead pool, or one too small: rebuilt below
+                    pool = None
+                if pool is None:
+                    pool = self._make_pool(req.key, *topo, want)
+                if pool is None:
+                    self.pending.append((req, topo))
+                    return
+                self.pools.move_to_end(req.key)
+            if pool.enqueue(req):
                 return
-            self.pools.move_to_end(req.key)
-        pool.enqueue(req)
 
     def _solver(self, key, Yp, Yj, Yx, pv, pq):
         solver = self.solvers.get(key)

@@ -140,10 +140,12 @@ class SbusBuilder:
         from . import _lib
 
         X = self.blocks(series)
+        cols = X[0].shape[1] if X else 0
         rows = np.array([x.shape[0] for x in X], dtype=np.int32)
         blk = (C.c_void_p * max(len(X), 1))(*[x.ctypes.data for x in X])
         L = _lib.lib()
         _lib.check(L.p3smi_batch_build_sbus_blocks(batch_handle, _lib.ptr(self.s0), _lib.ptr(self.ptr),
                                                    int(self.row.size), _lib.ptr(self.row), _lib.ptr(self.coef),
-                                                   _lib.ptr(self.base), _lib.ptr(self.imag), len(X), _lib.ptr(rows),
+                                                   _lib.ptr(self.base), _lib.ptr(self.imag), int(cols), len(X),
+                                                   _lib.ptr(rows),
                                                    C.cast(blk, C.c_void_p)))

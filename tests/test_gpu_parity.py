@@ -151,9 +151,11 @@ def test_every_lu_shape_gives_the_same_bits(gpu, precision, name):
 
 def test_switching_mixed_mode(gpu):
     """The reference GPU solver's mixed rule (p3s/cuda/nr_polar_kernels.cu:209-288): FP32 steps
-    while the mismatch is above 1e-3 p.u., FP64 after. An inexact Newton: the converged
-    voltages meet the 1e-9 bar against the FP64 oracle and the flags agree; iteration counts
-    may differ (their agreement is reported, the bar is on flags and voltages)."""
+    while the mismatch is above 1e-3 p.u., FP64 after. The converged voltages meet the 1e-9 bar
+    against the FP64 oracle, and flags and iteration counts are exact: far from the solution
+    an FP32 step's relative error (~cond(J) x 6e-8) is far below the Newton contraction, and
+    the last steps, below 1e-3, are FP64 (the reference only guarantees +1 iteration,
+    tests/cuda/test_lean_lu.py:201; here equality holds on these sets and is required)."""
     for name, T in (("pf200", 96), ("ts2000", 512)):
         g = grids.config_grid(name, 0)
         Yp, Yj, Yx = g.csr()
@@ -165,6 +167,8 @@ def test_switching_mixed_mode(gpu):
         r = solver.solve_batch(S, V0, max_iter=20, tol=1e-8, precision="mixed_switch")
         ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
         assert np.array_equal(r["converged"], ref["converged"]), name
+        assert np.array_equal(r["iterations"], ref["iterations"]), (
+            name, np.nonzero(r["iterations"] != ref["iterations"])[0][:10])
         ok = ref["converged"]
         dvm = np.abs(np.abs(r["V"][:, ok]) - np.abs(ref["V"][:, ok])).max()
         dva = np.abs(np.angle(r["V"][:, ok]) - np.angle(ref["V"][:, ok])).max()
@@ -174,8 +178,7 @@ def test_switching_mixed_mode(gpu):
         st = solver.last_stats
         assert st["single_factorizations"] > 0, st
         assert st["single_factorizations"] + st["double_factorizations"] >= r["iterations"].sum(), st
-        print(f"{name}: iterations equal in {np.mean(r['iterations'] == ref['iterations']):.3f} of the cases, "
-              f"{st['single_factorizations']} FP32 / {st['double_factorizations']} FP64 factorizations")
+        print(f"{name}: {st['single_factorizations']} FP32 / {st['double_factorizations']} FP64 factorizations")
 
 
 def test_last_stats_count_one_factorization_per_iteration(gpu):
@@ -200,3 +203,28 @@ def test_last_stats_count_one_factorization_per_iteration(gpu):
     assert np.array_equal(rc["mismatch"], r["mismatch"])
     assert solver.last_stats["cases"] == 300
     assert solver.last_stats["double_factorizations"] == st["double_factorizations"]
+
+
+def test_update_Y_equals_a_rebuilt_solver(gpu):
+    """Solver.update_Y (p3s/cpp/nr_klu.cpp:999-1011) refreshes the admittances on the device
+    without a new symbolic set-up: solving afterwards gives the bits a solver built from the
+    new values gives (tests/cuda/test_lean_lu.py:270-284), and a wrong nnz is refused."""
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    S = grids.timeseries_scaled(g, 64, seed=2)
+    V0 = g.flat_start()
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    first = s.solve_batch(S, V0, max_iter=20, tol=1e-8)  # device buffers resident before the update
+    rng = np.random.default_rng(3)
+    Yx2 = Yx * (1.0 + 0.03 * rng.uniform(-1.0, 1.0, Yx.shape))  # same pattern, new values
+    s.update_Y(Yx2)
+    upd = s.solve_batch(S, V0, max_iter=20, tol=1e-8)
+    fresh = nr_klu.Solver(Yp, Yj, Yx2, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    np.testing.assert_array_equal(upd["iterations"], fresh["iterations"])
+    np.testing.assert_array_equal(upd["converged"], fresh["converged"])
+    np.testing.assert_array_equal(upd["V"].view(np.float64), fresh["V"].view(np.float64))
+    assert not np.array_equal(upd["V"], first["V"])
+    ref = OracleSolver(Yp, Yj, Yx2, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    _compare(upd, ref, "update_Y")
+    with pytest.raises(Exception, match="nnz"):
+        s.update_Y(Yx2[:-1])

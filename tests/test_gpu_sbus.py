@@ -111,12 +111,13 @@ def test_device_sbus_blocks_rejects_bad_arguments(gpu):
     bh = C.c_void_p()
     _lib.check(L.p3smi_batch_create(solver._h, T, C.byref(bh)))
 
-    def call(rows, ptrs):
+    def call(rows, ptrs, cols=T):
         rows = np.asarray(rows, dtype=np.int32)
         blk = (C.c_void_p * max(len(ptrs), 1))(*ptrs)
         return L.p3smi_batch_build_sbus_blocks(bh, _lib.ptr(sb.s0), _lib.ptr(sb.ptr), int(sb.row.size),
                                                _lib.ptr(sb.row), _lib.ptr(sb.coef), _lib.ptr(sb.base),
-                                               _lib.ptr(sb.imag), len(rows), _lib.ptr(rows), C.cast(blk, C.c_void_p))
+                                               _lib.ptr(sb.imag), int(cols), len(rows), _lib.ptr(rows),
+                                               C.cast(blk, C.c_void_p))
 
     try:
         good_rows = [x.shape[0] for x in X]
@@ -130,5 +131,12 @@ def test_device_sbus_blocks_rejects_bad_arguments(gpu):
         assert "null" in L.p3smi_last_error().decode()
         assert call(good_rows[:2], good_ptrs[:2]) != 0  # the generator rows are missing
         assert "row index out of range" in L.p3smi_last_error().decode()
+        # series narrower or wider than the batch: rejected before any copy
+        for cols in (T - 1, T + 1):
+            assert call(good_rows, good_ptrs, cols) != 0
+            assert "columns" in L.p3smi_last_error().decode()
+        narrow = [np.ascontiguousarray(x[:, :T - 2]) for x in X]
+        with pytest.raises(Exception, match="columns"):
+            sb.build(bh, narrow)
     finally:
         L.p3smi_batch_free(bh)

@@ -136,3 +136,43 @@ def test_slot_batch_stats_count_factorizations_in_switching_mode(gpu):
         assert st.single_factorizations + st.double_factorizations >= int(it.sum())
     finally:
         L.p3smi_batch_free(bh)
+
+
+def test_worker_failure_fails_its_requests_and_the_pool_is_rebuilt(gpu):
+    """A pass that fails inside a pool's worker thread ends every request the pool holds with
+    the error (no request waits forever); the next request of that topology gets a new pool."""
+    from p3smi import _lib
+
+    g = grids.config_grid("pf200", 0)
+    Yp, Yj, Yx = g.csr()
+    S = grids.timeseries_scaled(g, 40, seed=6)
+    V0 = g.flat_start()
+    srv = Server(max_slots=64, mem_budget=1 << 30)
+    try:
+        ok = srv.submit(Yp, Yj, Yx, g.pv, g.pq, S[:, :8], V0)
+        srv.wait_all([ok], timeout=60)
+        pool = next(iter(srv.pools.values()))
+
+        class Failing:
+            def __init__(self, L):
+                self._L = L
+
+            def __getattr__(self, name):
+                if name == "p3smi_batch_pass":
+                    return lambda *a: 1  # P3SMI_E_ARG
+                return getattr(self._L, name)
+
+        pool.L = Failing(pool.L)
+        bad = [srv.submit(Yp, Yj, Yx, g.pv, g.pq, S[:, 8 + 4 * k:12 + 4 * k], V0) for k in range(3)]
+        for r in bad:
+            assert r._done.wait(60), "a request of the failed pool never finished"
+            with pytest.raises(_lib.P3smiError):
+                r.wait()
+        pool.L = pool.L._L
+        again = srv.submit(Yp, Yj, Yx, g.pv, g.pq, S[:, 20:40], V0)
+        srv.wait_all([again], timeout=60)
+        assert next(iter(srv.pools.values())) is not pool
+        alone = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S[:, 20:40], V0, max_iter=20, tol=1e-8)
+        np.testing.assert_array_equal(again.result()["V"], alone["V"])
+    finally:
+        srv.close()

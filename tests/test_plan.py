@@ -38,7 +38,7 @@ def test_jacobian_equals_oracle(name):
     Yp, Yj, Yx = g.csr()
     V = _random_V(g, 1)
     s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
-    d = s.debug_J_at(g.sbus(), V)
+    d = s.jacobian_rows_at(g.sbus(), V)
     J = sp.csr_matrix((d["Jx"], d["Jj"], d["Jp"]), shape=(d["m"], d["m"]))
     o = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).debug_J_at(g.sbus(), V)
     Jo = sp.csc_matrix((o["Jx"], o["Ji"], o["Jp"]), shape=(o["m"], o["m"]))
@@ -49,13 +49,35 @@ def test_jacobian_equals_oracle(name):
 
 
 @pytest.mark.parametrize("name", ["pf200", "ts2000"])
+def test_debug_J_has_the_reference_csc_layout(name):
+    """debug_J returns the reference's CSC dict {Jp, Ji, Jx, m} (p3s/cpp/nr_klu.cpp:1284-1304):
+    the same column pointers and row indices, entry for entry, as the oracle's restatement of
+    build_J_pattern (p3s/cpp/nr_klu.cpp:230-285), and the same values; a caller building
+    csc_matrix((Jx, Ji, Jp)) gets the Jacobian (tests/cuda/test_lean_lu.py:90-100)."""
+    g = grids.config_grid(name, 0)
+    Yp, Yj, Yx = g.csr()
+    V = _random_V(g, 2)
+    d = nr_klu.debug_J(Yp, Yj, Yx, g.sbus(), V, g.pv, g.pq)
+    assert set(d) >= {"Jp", "Ji", "Jx", "m"}
+    o = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).debug_J_at(g.sbus(), V)
+    assert d["m"] == o["m"]
+    np.testing.assert_array_equal(d["Jp"], o["Jp"])
+    np.testing.assert_array_equal(d["Ji"], o["Ji"])
+    np.testing.assert_allclose(d["Jx"], o["Jx"], rtol=0, atol=1e-12 * np.abs(o["Jx"]).max())
+    J = sp.csc_matrix((d["Jx"], d["Ji"], d["Jp"]), shape=(d["m"], d["m"]))
+    Jr = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq).jacobian_rows_at(g.sbus(), V)
+    Jc = sp.csr_matrix((Jr["Jx"], Jr["Jj"], Jr["Jp"]), shape=J.shape)
+    assert abs(J - Jc).max() == 0.0
+
+
+@pytest.mark.parametrize("name", ["pf200", "ts2000"])
 def test_static_pivot_plan_solves_like_spsolve(name):
     g = grids.config_grid(name, 0)
     Yp, Yj, Yx = g.csr()
     s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
     for seed in range(3):
         V = _random_V(g, seed)
-        d = s.debug_J_at(g.sbus(), V)
+        d = s.jacobian_rows_at(g.sbus(), V)
         J = sp.csr_matrix((d["Jx"], d["Jj"], d["Jp"]), shape=(d["m"], d["m"])).tocsc()
         R = -d["F"]
         x, piv = s.host_factor_solve(d["Jx"], R)
@@ -149,7 +171,7 @@ def test_block_schedule_emulation_equals_one_entry_schedule(name, single):
     s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
     for seed, nslots in ((0, 16), (1, 1024)):
         V = _random_V(g, seed)
-        d = s.debug_J_at(g.sbus(), V)
+        d = s.jacobian_rows_at(g.sbus(), V)
         R = -d["F"]
         x1, p1, c1 = s.host_lu_emulate(d["Jx"], R, blocks=False, nslots=nslots, single=single)
         x2, p2, c2 = s.host_lu_emulate(d["Jx"], R, blocks=True, nslots=nslots, single=single)
@@ -170,6 +192,6 @@ def test_block_emulation_rejects_bad_arguments():
     g = grids.config_grid("pf200", 0)
     Yp, Yj, Yx = g.csr()
     s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
-    d = s.debug_J_at(g.sbus(), g.flat_start())
+    d = s.jacobian_rows_at(g.sbus(), g.flat_start())
     with pytest.raises(RuntimeError, match="nslots"):
         s.host_lu_emulate(d["Jx"], -d["F"], blocks=True, nslots=0)
