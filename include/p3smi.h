@@ -168,6 +168,17 @@ void p3smi_outages_free(p3smi_outages* o);
  * ref_bus (n, L) the reference bus of its island or -1, pinned (L) number of re-slacked
  * references (_components_after_outage + _resolve_references, case_generator.py:355-430). */
 int p3smi_outages_islands(p3smi_outages* o, uint8_t* served, int32_t* ref_bus, int32_t* pinned);
+/* p3smi_outages_islands without allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. served (n, L), ref_bus (n, L, may be NULL) and pinned (L) are
+ * DEVICE pointers the caller owns; the results equal p3smi_outages_islands' bit for bit. The
+ * union-find scratch is a 64 MiB array resident on each GPU for the life of the process
+ * (2n ints, plus n 8-byte words with re-slacking, per case; cases run in chunks that fit): a
+ * grid needing more than 64 MiB for one case (above ~4 million buses) is rejected with an
+ * error. Calls share the buffer: this library serialises its own calls, and calls that
+ * overlap on one GPU from elsewhere (another copy of the library) would race. The call
+ * returns after its kernels have finished on the topology's stream. */
+int p3smi_outages_islands_noalloc(p3smi_outages* o, uint8_t* served, int32_t* ref_bus, int32_t* pinned);
 /* Solve every case: S (n) complex injections, V_start (n) complex start voltage. V_out
  * (n, L) complex with NaN at unserved buses; served (n, L) may be NULL. */
 int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start, int32_t max_iter, double tol,

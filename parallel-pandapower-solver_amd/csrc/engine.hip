@@ -1620,7 +1620,7 @@ constexpr unsigned long long INF_BITS = 0x7ff0000000000000ull;
 __global__ __launch_bounds__(256) void k_islands(
     int n, int nbr, const int* __restrict__ f, const int* __restrict__ t, const unsigned char* __restrict__ live,
     const int* __restrict__ gptr, const int* __restrict__ gbr, int nref, const int* __restrict__ ref, int ngen,
-    const int* __restrict__ gbus, const double* __restrict__ gz, int reslack, int c0, int* __restrict__ scratch,
+    const int* __restrict__ gbus, const double* __restrict__ gz, int reslack, int c0, int* __restrict__ ufbuf,
     unsigned long long* __restrict__ zscratch, const double* __restrict__ Vstart, unsigned char* __restrict__ served,
     int* __restrict__ refbus, unsigned char* __restrict__ pin, double* __restrict__ V0, int* __restrict__ npinned,
     int S) {
@@ -1629,7 +1629,7 @@ __global__ __launch_bounds__(256) void k_islands(
     const int cs = c0 + cl;
     const int g0 = gptr[cs], gsz = gptr[cs + 1] - g0;
     for (int q = threadIdx.x; q < gsz; q += blockDim.x) gm[q] = gbr[g0 + q];
-    int* parent = scratch + (size_t)cl * 2 * n;
+    int* parent = ufbuf + (size_t)cl * 2 * n;
     int* cref = parent + n;
     for (int v = threadIdx.x; v < n; v += blockDim.x) {
         uf_st(&parent[v], v);
@@ -3316,24 +3316,67 @@ namespace {
 
 constexpr int ISLAND_CHUNK = 2048;  // cases per islands launch (bounds the scratch)
 
-// Islands + solver inputs of cases [c0, c0 + nc) into column-strided (stride S) outputs.
-void run_islands(p3smi_outages* o, int c0, int nc, int S, const double* dVstart, unsigned char* served,
-                 int* refbus, unsigned char* pin, double* V0, int* npinned) {
+bool islands_need_z(const p3smi_outages* o) { return o->reslack && o->ngen; }
+
+// Islands + solver inputs of cases [c0, c0 + nc) into column-strided (stride S) outputs, in
+// launches of at most `chunk` cases. uf holds chunk * 2n ints (union-find parents and island
+// references), z (with re-slacking) chunk * n words of generator |Z|.
+void launch_islands(p3smi_outages* o, int c0, int nc, int S, int chunk, int* uf, unsigned long long* z,
+                    const double* dVstart, unsigned char* served, int* refbus, unsigned char* pin, double* V0,
+                    int* npinned) {
     p3smi_topology* t = o->topo;
     const int n = t->host.n;
     hipStream_t st = t->stream;
-    for (int a = 0; a < nc; a += ISLAND_CHUNK) {
-        const int k = std::min(ISLAND_CHUNK, nc - a);
-        o->scratch.alloc((size_t)ISLAND_CHUNK * 2 * n);
-        if (o->reslack && o->ngen) o->zscratch.alloc((size_t)ISLAND_CHUNK * n);
+    for (int a = 0; a < nc; a += chunk) {
+        const int k = std::min(chunk, nc - a);
         hipLaunchKernelGGL(k_islands, dim3((unsigned)k), dim3(256), 0, st, n, o->nbr, o->f.p, o->t.p, o->live.p,
                            o->gptr.p, o->gbr.p, o->nref, o->ref.p, o->ngen, o->gbus.p, o->gz.p, o->reslack, c0 + a,
-                           o->scratch.p, (o->reslack && o->ngen) ? o->zscratch.p : nullptr, dVstart, served + a,
-                           refbus ? refbus + a : nullptr, pin ? pin + a : nullptr, V0 ? V0 + 2 * (size_t)a : nullptr,
-                           npinned + a, S);
+                           uf, islands_need_z(o) ? z : nullptr, dVstart, served + a, refbus ? refbus + a : nullptr,
+                           pin ? pin + a : nullptr, V0 ? V0 + 2 * (size_t)a : nullptr, npinned + a, S);
         HIPCHECK(hipGetLastError());
     }
 }
+
+void run_islands(p3smi_outages* o, int c0, int nc, int S, const double* dVstart, unsigned char* served,
+                 int* refbus, unsigned char* pin, double* V0, int* npinned) {
+    const int n = o->topo->host.n;
+    o->scratch.alloc((size_t)ISLAND_CHUNK * 2 * n);
+    if (islands_need_z(o)) o->zscratch.alloc((size_t)ISLAND_CHUNK * n);
+    launch_islands(o, c0, nc, S, ISLAND_CHUNK, o->scratch.p, o->zscratch.p, dVstart, served, refbus, pin, V0,
+                   npinned);
+}
+
+// ---- islands without allocation ----------------------------------------------------
+// The union-find scratch of p3smi_outages_islands_noalloc lives in this array, one copy per
+// GPU for the life of the process. Its device address comes from hipGetSymbolAddress (once
+// per device, with that device current); host code never passes the array's name itself.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;  // held for a whole no-allocation call: calls in this process never overlap
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of the current device's copy of `scratch` (caller holds g_noalloc_mu).
+unsigned char* noalloc_scratch() {
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) throw Error(P3SMI_E_DEVICE, "noalloc: device index out of range");
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(P3SMI_E_DEVICE, std::string("noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_noalloc_ptr[dev] = p;
+    }
+    return (unsigned char*)g_noalloc_ptr[dev];
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of scratch `k` cases of n buses need: parents and references, then (re-slacking) the
+// |Z| words at a 256-byte boundary.
+size_t noalloc_island_bytes(size_t k, size_t n, bool z) { return align256(k * 2 * n * sizeof(int)) + (z ? k * n * 8 : 0); }
 
 }  // namespace
 
@@ -3418,6 +3461,37 @@ int p3smi_outages_islands(p3smi_outages* o, uint8_t* served, int32_t* ref_bus, i
         if (served) HIPCHECK(hipMemcpyAsync(served, dsrv.p, (size_t)n * L, hipMemcpyDeviceToHost, st));
         if (ref_bus) HIPCHECK(hipMemcpyAsync(ref_bus, dref.p, (size_t)n * L * sizeof(int), hipMemcpyDeviceToHost, st));
         if (pinned) HIPCHECK(hipMemcpyAsync(pinned, dnp.p, (size_t)L * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    });
+}
+
+int p3smi_outages_islands_noalloc(p3smi_outages* o, uint8_t* served, int32_t* ref_bus, int32_t* pinned) {
+    return guarded([&] {
+        if (!o || !served || !pinned) throw Error(P3SMI_E_ARG, "islands_noalloc: null argument");
+        p3smi_topology* t = o->topo;
+        std::lock_guard<std::mutex> lk(t->mu);
+        std::lock_guard<std::mutex> lk2(g_noalloc_mu);
+        if (!t->dev.ready || !t->stream) throw Error(P3SMI_E_ARG, "islands_noalloc: the topology is not resident");
+        const size_t n = (size_t)t->host.n;
+        const int L = o->ncases;
+        if (L == 0) return;
+        const bool z = islands_need_z(o);
+        if (noalloc_island_bytes(1, n, z) > NOALLOC_SCRATCH_BYTES)
+            throw Error(P3SMI_E_ARG, "islands_noalloc: a grid of " + std::to_string(n) + " buses needs " +
+                                         std::to_string(noalloc_island_bytes(1, n, z)) +
+                                         " bytes of scratch per case, above the " +
+                                         std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte no-allocation buffer");
+        size_t chunk = NOALLOC_SCRATCH_BYTES / (n * (z ? 16 : 8));
+        while (chunk > 1 && noalloc_island_bytes(chunk, n, z) > NOALLOC_SCRATCH_BYTES) --chunk;
+        chunk = std::min(chunk, (size_t)L);
+        unsigned char* base = noalloc_scratch();
+        int* uf = (int*)base;
+        unsigned long long* zb = z ? (unsigned long long*)(base + align256(chunk * 2 * n * sizeof(int))) : nullptr;
+        hipStream_t st = t->stream;
+        // the re-slack counters are summed with atomics: cleared by a kernel, not a memset
+        hipLaunchKernelGGL(k_clear_int, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, st, pinned, L);
+        HIPCHECK(hipGetLastError());
+        launch_islands(o, 0, L, L, (int)chunk, uf, zb, nullptr, served, ref_bus, nullptr, nullptr, pinned);
         HIPCHECK(hipStreamSynchronize(st));
     });
 }

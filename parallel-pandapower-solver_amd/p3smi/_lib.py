@@ -88,7 +88,7 @@ EXPORTS = [
     "p3smi_outages_branch_flows", "p3smi_outages_voltage_violations",
     "p3smi_batch_create", "p3smi_batch_free", "p3smi_batch_build_sbus", "p3smi_batch_build_sbus_blocks", "p3smi_batch_download_sbus", "p3smi_batch_upload", "p3smi_batch_solve", "p3smi_batch_download",
     "p3smi_topology_release_cache", "p3smi_topology_last_batch", "p3smi_batch_vrange", "p3smi_batch_stats", "p3smi_batch_set_graphs", "p3smi_topology_set_graphs", "p3smi_batch_set_timing", "p3smi_batch_bytes_per_case",
-    "p3smi_outages_create", "p3smi_outages_free", "p3smi_outages_islands", "p3smi_outages_solve",
+    "p3smi_outages_create", "p3smi_outages_free", "p3smi_outages_islands", "p3smi_outages_islands_noalloc", "p3smi_outages_solve",
     "p3smi_batch_slots_init", "p3smi_batch_load", "p3smi_batch_pass", "p3smi_batch_finished", "p3smi_batch_fetch",
 ]
 
@@ -151,6 +151,7 @@ def lib():
     L.p3smi_outages_free.argtypes = [P]
     L.p3smi_outages_free.restype = None
     L.p3smi_outages_islands.argtypes = [P, P, P, P]
+    L.p3smi_outages_islands_noalloc.argtypes = [P, P, P, P]
     L.p3smi_outages_solve.argtypes = [P, P, P, _i32, _f64, _i32, _i32, P, P, P, P, P]
     L.p3smi_batch_slots_init.argtypes = [P]
     L.p3smi_batch_load.argtypes = [P, _i32, P, P, P]
