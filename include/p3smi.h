@@ -46,6 +46,7 @@ typedef struct {
     int32_t n_levels, n_blevels, n_slots, ordering;
     double rcond0;
     int32_t lu_g, lu_ns, lu_few_g, lu_few_ns, lu_few_max, lu_group;
+    int32_t eval_rows; /* Ybus rows per tile of the mismatch/Jacobian evaluation, at most */
 } p3smi_info;
 
 /* Per-case/aggregate counters of the last solve (the reference's last_stats,
@@ -96,6 +97,10 @@ int p3smi_topology_set_max_chunk(p3smi_topology* t, int32_t cases);
  * task list (1: one entry per task). Every shape gives the same bits. */
 int p3smi_topology_set_lu_shape(p3smi_topology* t, int32_t g, int32_t ns, int32_t few_g, int32_t few_ns,
                                 int32_t few_max, int32_t group);
+/* Row tiles of the mismatch/Jacobian evaluation (no counterpart in the reference; set by the
+ * autotuner): a workgroup evaluates consecutive Ybus rows, at most `rows` (1..1024) of them
+ * and at most 768 nonzeros, staged in LDS. Default 32. Results do not depend on it. */
+int p3smi_topology_set_eval_tile(p3smi_topology* t, int32_t rows);
 /* Mixed precision (no counterpart in the reference): refinement of the FP32-LU step stops
  * once the correction is below rtol x the step, at most kmax steps, then falls back to an
  * FP64 factorization of that case. Defaults 1e-12 and 6. */

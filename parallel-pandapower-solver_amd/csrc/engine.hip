@@ -1,34 +1,38 @@
 // engine.hip -- batched polar Newton-Raphson power flow on one MI355X (gfx950).
 //
 // The Newton iteration is the one of the reference's nr_klu (p3s/cpp/nr_klu.cpp:569-711):
-// polar mismatch and Jacobian, static-pivot LU on a frozen pattern with a pivot-ratio
-// guard, Armijo backtracking (beta 0.5, at most 10 halvings, c = 1e-4, best trial taken
-// when none passes). Every case of a batch iterates on its own; the device keeps one
-// status word per case and every kernel skips the cases it has nothing to do for.
+// mismatch and Jacobian in polar variables, static-pivot LU on a frozen pattern with a
+// pivot-ratio guard, Armijo backtracking (beta 0.5, at most 10 halvings, c = 1e-4, best trial
+// taken when none passes). Every case of a batch iterates on its own.
 //
-// LAYOUT. A per-case quantity q of item i (bus, Jacobian entry, factor position, ...) is
-// stored at [i * S + c], S = number of cases rounded up to 64: one wavefront covers 64
-// cases of one item, so every value load is one contiguous 512-byte line and every index
-// (Ybus column, factor position, operand pair) is wave-uniform and comes through the
-// scalar unit once per wavefront.
+// LAYOUT. A per-case quantity q of item i (bus, Jacobian entry, factor position, ...) is stored
+// at [i * S + c], S = number of cases rounded up to 64: a wavefront covers 64 cases of one item,
+// so every value load is one contiguous 512-byte line and every index (Ybus column, factor
+// position, operand pair) is the same in all lanes.
 //
-// KERNELS (one pass each; the host loop below sequences them, kernel by kernel or as
-// replays of captured hipGraphs):
-//   k_eval        mismatch R = Sspec - Scalc, its infinity norm, and (masked) the Jacobian
-//   k_lu_stream   static-pivot LU of the Jacobian + forward/backward substitution, a
-//                 workgroup per 64, 32 or 16 cases, level-synchronous over the elimination
-//                 tree, each task list walking a software-pipelined operand stream
-//   k_lu          the same with one case per workgroup (small batches, last few cases)
-//   k_step        V = V0 + alpha * dx (full Newton step or line-search trial)
-//   k_decide      convergence test and Armijo bookkeeping per case
-//   k_resid, k_refine_*   FP64 residual and update of the mixed-precision refinement
-//   k_islands, k_case_y   N-1: islands, references, pins, per-case Ybus
-//   k_vrange      min / max |V| per case (voltage-band acceptance)
+// STATE. The voltage of bus i is kept both rectangular (Ve + j Vf, what the mismatch reads) and
+// polar (Vm, Va, what the Newton step updates), with the base point (Vm0, Va0) of the step in
+// progress. The mismatch is the current-injection form S = V conj(Y V); the Jacobian's magnitude
+// columns are scaled by |V| (derivatives in ln |V|), so the step solves for dVm / Vm.
 //
-// STATUS per case: NEW (loaded, start point not yet evaluated), ACTIVE (iterating),
-// CONVERGED, FAILED (non-finite mismatch or rejected pivots), BACKTRACK (line-search trial),
-// NEEDJ (trial accepted, Jacobian to rebuild), STOPPED (iteration limit), IDLE (free slot).
-// Iterations are counted per case, so cases may join a running batch at any pass.
+// CONTROL. One 32-bit word per case (ctl) holds its stage, line-search trial count, precision
+// class and iteration count. The host runs passes; a pass is
+//     [LU of the cases in STEP] -> k_begin / k_advance -> k_evaluate -> k_control -> counters
+// and every kernel works only on the cases whose stage concerns it, so a case's backtracking
+// trial, another's next Newton iteration and a newly loaded case's start point share one pass.
+//
+// KERNELS:
+//   k_begin       start voltage -> state of the cases in START
+//   k_advance     V = base + alpha dx of the cases in STEP (alpha 1, base saved), TRIAL, COMMIT
+//   k_evaluate    mismatch R = Sspec - V conj(I), the Jacobian (except for TRIAL cases) and the
+//                 largest |R| of each (row tile, case); row tiles balanced by Ybus nonzeros,
+//                 their CSR segment staged in LDS
+//   k_control     per-case Newton bookkeeping (convergence, Armijo, iteration limit, precision
+//                 vote) and the stage counters the host reads back
+//   k_lu_blk, k_lu_stream   static-pivot LU + forward/backward substitution (see below)
+//   k_resid, k_refine_*     FP64 residual and update of the mixed-precision refinement
+//   k_islands, k_admittance_*   N-1: islands, references, pins, per-case Ybus
+//   k_emit        state -> complex V (n, B), NaN at unserved buses
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -55,17 +59,56 @@
 
 namespace {
 
-enum : int {
-    ST_ACTIVE = 0, ST_CONVERGED = 1, ST_FAILED = 2, ST_BACKTRACK = 3, ST_NEEDJ = 4, ST_STOPPED = 5, ST_IDLE = 6, ST_NEW = 7
+// ---- the per-case control word
+//   bits  0..3   stage
+//   bits  4..7   backtracking trials taken in the current step (0..10)
+//   bits  8..9   precision class of the next factorization (mixed modes)
+//   bit   10     refining: the case's FP32 step is still being refined (mixed mode)
+//   bits 16..31  Newton iterations so far (linear solves)
+enum : unsigned {
+    SG_IDLE = 0,    // free slot, or a padding column
+    SG_START = 1,   // start voltage loaded, start point not evaluated yet
+    SG_STEP = 2,    // evaluated and iterating: next a linear solve and the full step
+    SG_TRIAL = 3,   // line search: move to base + alpha dx, mismatch only
+    SG_COMMIT = 4,  // line search over: move to base + alpha dx, full evaluation
+    SG_CONV = 5,    // converged
+    SG_FAIL = 6,    // non-finite mismatch, or no usable factorization
+    SG_STOP = 7     // iteration limit reached
 };
+// precision classes: FP64 this time; FP32; FP64 for good (after a slow or cut FP32 step or
+// failed FP32 pivots); FP64 this once (an FP32 voter whose 64-case group takes an FP64 step)
+enum : unsigned { PR_F64 = 0, PR_F32 = 1, PR_F64_ALWAYS = 2, PR_F64_ONCE = 3 };
+constexpr unsigned CTL_REFINING = 1u << 10;
+constexpr int MAX_ITER_LIMIT = 65535;  // the iteration count has 16 bits
+
+__host__ __device__ constexpr unsigned stage_of(unsigned w) { return w & 15u; }
+__host__ __device__ constexpr unsigned trials_of(unsigned w) { return (w >> 4) & 15u; }
+__host__ __device__ constexpr unsigned prec_of(unsigned w) { return (w >> 8) & 3u; }
+__host__ __device__ constexpr int iters_of(unsigned w) { return (int)(w >> 16); }
+__host__ __device__ constexpr unsigned set_stage(unsigned w, unsigned s) { return (w & ~15u) | s; }
+__host__ __device__ constexpr unsigned set_trials(unsigned w, unsigned t) { return (w & ~(15u << 4)) | (t << 4); }
+__host__ __device__ constexpr unsigned set_prec(unsigned w, unsigned p) { return (w & ~(3u << 8)) | (p << 8); }
+__host__ __device__ constexpr unsigned set_iters(unsigned w, int it) { return (w & 0xFFFFu) | ((unsigned)it << 16); }
+__host__ __device__ constexpr bool evaluated_stage(unsigned s) { return s >= SG_START && s <= SG_COMMIT; }
+
+// Stage counters as the C ABI reports them (p3smi_batch_pass): active, converged, failed,
+// backtrack, needj, stopped, idle, new.
 constexpr int NSTAT = 8;
+enum : int { CNT_ACTIVE = 0, CNT_CONV, CNT_FAIL, CNT_TRIAL, CNT_COMMIT, CNT_STOP, CNT_IDLE, CNT_START };
+__host__ __device__ constexpr int counter_of(unsigned s) {
+    return s == SG_START ? CNT_START : s == SG_STEP ? CNT_ACTIVE : s == SG_TRIAL ? CNT_TRIAL
+         : s == SG_COMMIT ? CNT_COMMIT : s == SG_CONV ? CNT_CONV : s == SG_FAIL ? CNT_FAIL
+         : s == SG_STOP ? CNT_STOP : CNT_IDLE;
+}
+
 constexpr double ARMIJO_C = 1e-4;
 constexpr int LS_MAX_TRIALS = 10;
 constexpr double REUSE_RCOND_RATIO = 1e-3;
-// k_eval workgroups: 8 wavefronts of 4 buses (32 buses). Against 4 x 16, ts2000's k_eval +
-// k_decide (which reduces the per-workgroup maxima) take 5.01 instead of 5.35 ms per solve.
-constexpr int EVAL_WAVES = 8;
-constexpr int EVAL_BUSES_PER_WAVE = 4;
+// k_evaluate: workgroups of 4 wavefronts over one row tile; a tile's CSR segment (column,
+// Jacobian positions, shared admittance) is staged in LDS when it has at most EV_NZ nonzeros
+// (the host cuts tiles to fit, except around a single row with more).
+constexpr int EV_WAVES = 4;
+constexpr int EV_NZ = 768;
 
 struct Error : std::runtime_error {
     int code;
@@ -85,11 +128,13 @@ std::atomic<unsigned long long> g_solve_ids{0};
 // ===================================================================== device code
 __device__ __forceinline__ size_t at(int item, int S, int c) { return (size_t)item * (size_t)S + (size_t)c; }
 
+// largest of w and a, a NaN once seen kept
 __device__ __forceinline__ double nan_max(double w, double a) { return (a > w || a != a) ? a : w; }
 
-__global__ void k_status_init(int* status, int B, int S) {
+// Cases [0, B) start (START, no iterations, precision class p0), padding columns [B, S) idle.
+__global__ void k_ctl_reset(unsigned* __restrict__ ctl, int B, int S, unsigned p0) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < S) status[c] = c < B ? ST_NEW : ST_IDLE;
+    if (c < S) ctl[c] = c < B ? set_prec(SG_START, p0) : SG_IDLE;
 }
 
 __global__ void k_clear_int(int* p, int n) {
@@ -97,153 +142,201 @@ __global__ void k_clear_int(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// start voltage -> polar state of the NEW cases. V0: (n) shared complex (v0_stride 0) or
-// (n, v0_stride) complex, column c.
-__global__ void k_init(const double* __restrict__ V0, int v0_stride, double* __restrict__ Vm,
-                       double* __restrict__ Va, double* __restrict__ cs, double* __restrict__ sn,
-                       const int* __restrict__ status, int* __restrict__ vote, int vote_new, int n, int S) {
+// Start voltages of the cases in START: V0 (n) shared (v0_stride 0) or (n, v0_stride) complex,
+// column c. The rectangular state is V0 itself; the polar one its modulus and argument. Sets the
+// case's precision class p0 (one thread per case writes it).
+__global__ __launch_bounds__(256) void k_begin(const double* __restrict__ V0, int v0_stride,
+                                               unsigned* __restrict__ ctl, unsigned p0, double* __restrict__ Vm,
+                                               double* __restrict__ Va, double* __restrict__ Ve,
+                                               double* __restrict__ Vf, int n, int S) {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int bus = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (bus >= n || status[c] != ST_NEW) return;
-    if (bus == 0) vote[c] = vote_new;
+    const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (bus >= n) return;
+    const unsigned w = ctl[c];
+    if (stage_of(w) != SG_START) return;
+    const size_t src = 2 * (v0_stride ? (size_t)bus * v0_stride + c : (size_t)bus);
+    const double re = V0[src], im = V0[src + 1];
     const size_t o = at(bus, S, c);
-    const size_t v = 2 * (v0_stride ? (size_t)bus * v0_stride + c : (size_t)bus);
-    const double re = V0[v], im = V0[v + 1];
-    const double va = atan2(im, re);
-    double s, co;
-    sincos(va, &s, &co);
+    Ve[o] = re;
+    Vf[o] = im;
     Vm[o] = hypot(re, im);
-    Va[o] = va;
-    cs[o] = co;
-    sn[o] = s;
+    Va[o] = atan2(im, re);
+    if (bus == 0) ctl[c] = set_prec(w, p0);  // the stage stays START: other threads may read it
 }
 
-// Per-case Ybus from magnitude/angle (nnz, L) -> G, B at [k * S + c].
-__global__ void k_y_polar(const double* __restrict__ mag, const double* __restrict__ ang, double* __restrict__ G,
-                          double* __restrict__ Bv, int nnz, int L, int ld, int S) {
+// Per-case admittance G + jB at [k * S + c] of columns [0, nc): mode 0 copies the shared base
+// values (G0, B0 indexed by k), mode 1 converts magnitude/angle (nnz, ld) host-layout arrays.
+// One thread per (nonzero, case), cases along x.
+__global__ __launch_bounds__(256) void k_admittance_fill(int mode, const double* __restrict__ p0,
+                                                         const double* __restrict__ p1, int ld, int nc,
+                                                         double* __restrict__ G, double* __restrict__ Bv, int nnz,
+                                                         int S) {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int k = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (k >= nnz || c >= L) return;
-    const double m = mag[(size_t)k * ld + c], a = ang[(size_t)k * ld + c];
-    double s, co;
-    sincos(a, &s, &co);
-    G[at(k, S, c)] = m * co;
-    Bv[at(k, S, c)] = m * s;
+    const int k = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (k >= nnz || c >= S) return;
+    const size_t o = at(k, S, c);
+    if (mode == 0 || c >= nc) {
+        G[o] = mode == 0 ? p0[k] : 0.0;
+        Bv[o] = mode == 0 ? p1[k] : 0.0;
+    } else {
+        const double mag = p0[(size_t)k * ld + c], ang = p1[(size_t)k * ld + c];
+        double s, co;
+        sincos(ang, &s, &co);
+        G[o] = mag * co;
+        Bv[o] = mag * s;
+    }
 }
 
-// Per-case Ybus = base - stamp: fill with the base, then subtract the COO stamp entries.
-__global__ void k_y_base(const double* __restrict__ G0, const double* __restrict__ B0, double* __restrict__ G,
-                         double* __restrict__ Bv, int nnz, int S) {
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int k = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (k >= nnz) return;
-    G[at(k, S, c)] = G0[k];
-    Bv[at(k, S, c)] = B0[k];
+// Per-case admittance deltas: case c (column c - c0 of the batch) subtracts its entries
+// [dptr[c], dptr[c + 1]) = (Ybus position, complex value) in list order -- an outage case's
+// branch stamps in group order (p3s/contingency/case_generator.py:319-325). One thread per case.
+__global__ void k_admittance_delta(const int* __restrict__ dptr, const int* __restrict__ dpos,
+                                   const double* __restrict__ dval, int c0, int nc, double* __restrict__ G,
+                                   double* __restrict__ Bv, int S) {
+    const int cl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cl >= nc) return;
+    for (int q = dptr[c0 + cl], q1 = dptr[c0 + cl + 1]; q < q1; ++q) {
+        const size_t o = at(dpos[q], S, cl);
+        G[o] -= dval[2 * (size_t)q];
+        Bv[o] -= dval[2 * (size_t)q + 1];
+    }
 }
 
-__global__ void k_y_stamp(const int* __restrict__ sk, const int* __restrict__ sc, const double* __restrict__ sv,
-                          double* __restrict__ G, double* __restrict__ Bv, long long nst, int c0, int nc, int S) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nst) return;
-    const int c = sc[e] - c0;
-    if (c < 0 || c >= nc) return;
-    G[at(sk[e], S, c)] -= sv[2 * e];
-    Bv[at(sk[e], S, c)] -= sv[2 * e + 1];
-}
+// Mismatch, Jacobian and per-tile infinity norm of every case that takes an evaluation this
+// pass (stages START, STEP, TRIAL, COMMIT; TRIAL without the Jacobian).
+//
+// Bus i's current injection I_i = sum_k Y_ik V_k is accumulated over its Ybus row, and
+//   S_i = V_i conj(I_i),   R = Sspec - S at the P row (pv, pq buses) and Q row (pq buses).
+// With w_ik = V_i conj(Y_ik V_k) the derivatives (angle columns, and magnitude columns in
+// ln |V|, i.e. multiplied by |V_k|) are, off the diagonal,
+//   dP_i/dVa_k =  Im w_ik   dQ_i/dVa_k = -Re w_ik   dP_i/dln|V_k| = Re w_ik   dQ_i/dln|V_k| = Im w_ik
+// and on it, with m2 = |V_i|^2 and Y_ii = g + jb,
+//   dP_i/dVa_i = -Q_i - m2 b   dQ_i/dVa_i = P_i - m2 g   dP_i/dln|V_i| = P_i + m2 g
+//   dQ_i/dln|V_i| = Q_i - m2 b
+// (dS/dVa = j diag(V) conj(diag(I) - Y diag(V)), |V| dS/d|V| = diag(V) conj(Y diag(V)) +
+// diag(conj(I)) diag(V) in matrix form; the polar derivatives of p3s/cpp/nr_klu.cpp:288-378
+// times the column scale). A pinned bus (N-1 island reference, unserved bus) has identity rows
+// and zero mismatch (p3s/cpp/nr_klu.cpp:398-423).
+//
+// A workgroup takes one row tile [tile_ptr[t], tile_ptr[t + 1]) and 64 cases; wavefront w walks
+// the tile's rows w, w + 4, ...; the tile's column indices, Jacobian positions and (shared
+// Ybus) admittances are read from LDS. part[t * S + c] = largest |R| of the tile's rows (NaN
+// kept).
+struct EvalArgs {
+    const int* tile_ptr;
+    const int* Yp;
+    const int* Yj;
+    const int* Ydiag;
+    const int4* posJ;
+    const double* G;   // (nnz) shared, or (nnz, S) per case
+    const double* B;
+    const double* Ve;
+    const double* Vf;
+    const double* Ssp;  // complex (n) shared (s_stride 0) or (n, s_stride)
+    int s_stride;
+    const int* row_p;
+    const int* row_q;
+    const unsigned char* pin;  // (n, S) or null
+    const unsigned* ctl;
+    double* J;
+    double* R;
+    double* part;
+    int n, S;
+};
 
-// Mismatch, its infinity norm and (for statuses in j_mask) the Jacobian. A wavefront walks
-// one Ybus row for 64 cases. With the unit phasor (cs, sn) of every bus,
-//   Ym cos(delta) = G cu + B su,  Ym sin(delta) = G su - B cu,
-//   cu = cos(Va_i - Va_k) = ci ck + si sk,  su = sin(Va_i - Va_k) = si ck - ci sk,
-// the polar derivatives of p3s/cpp/nr_klu.cpp:288-378 need no trigonometry per nonzero.
-// A pinned bus gets identity rows and zero mismatch (p3s/cpp/nr_klu.cpp:398-423).
-// part[blockIdx.x * S + c]: the largest |R| of the workgroup's buses (NaN kept).
-template <bool PER_CASE_Y, bool HAS_PIN>
-__global__ __launch_bounds__(64 * EVAL_WAVES) void k_eval(
-    const int* __restrict__ Yp, const int* __restrict__ Yj, const int* __restrict__ Ydiag,
-    const double* __restrict__ G, const double* __restrict__ Bv, const double* __restrict__ Vm,
-    const double* __restrict__ cs, const double* __restrict__ sn, const double* __restrict__ Ssp, int s_stride,
-    const int* __restrict__ row_p, const int* __restrict__ row_q, const unsigned char* __restrict__ pin,
-    const int* __restrict__ status, int run_mask, int j_mask, const int* __restrict__ posJ, double* __restrict__ Jx,
-    double* __restrict__ R, double* __restrict__ part, int n, int S) {
-    __shared__ double sw[EVAL_WAVES][64];
+template <bool CASE_Y, bool PIN>
+__global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
+    __shared__ int s_col[EV_NZ];
+    __shared__ int4 s_pos[EV_NZ];
+    __shared__ double2 s_y[CASE_Y ? 1 : EV_NZ];
+    __shared__ double s_worst[EV_WAVES][64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int S = a.S;
     const int c = blockIdx.y * 64 + lane;
-    const int st = status[c];
-    const bool run = (run_mask >> st) & 1;
-    const bool wj = (j_mask >> st) & 1;
+    const int r0 = a.tile_ptr[blockIdx.x], r1 = a.tile_ptr[blockIdx.x + 1];
+    const int k0 = a.Yp[r0], k1 = a.Yp[r1];
+    const bool staged = k1 - k0 <= EV_NZ;
+    if (staged) {
+        for (int k = k0 + (int)threadIdx.x; k < k1; k += 64 * EV_WAVES) {
+            s_col[k - k0] = a.Yj[k];
+            s_pos[k - k0] = a.posJ[k];
+            if (!CASE_Y) s_y[k - k0] = make_double2(a.G[k], a.B[k]);
+        }
+    }
+    __syncthreads();
+    const unsigned sg = stage_of(a.ctl[c]);
+    const bool run = evaluated_stage(sg);
+    const bool jac = run && sg != SG_TRIAL;
     double worst = 0.0;
-    const int b0 = blockIdx.x * (EVAL_WAVES * EVAL_BUSES_PER_WAVE);
-    const int b1 = min(n, b0 + EVAL_WAVES * EVAL_BUSES_PER_WAVE);
     if (run) {
-        for (int row = b0 + wave; row < b1; row += EVAL_WAVES) {
+        for (int row = r0 + wave; row < r1; row += EV_WAVES) {
             const size_t oi = at(row, S, c);
-            const double vi = Vm[oi], ci = cs[oi], si = sn[oi];
-            const bool pinned = HAS_PIN && pin[oi];
-            double p = 0.0, q = 0.0, dPVa = 0.0, dQVa = 0.0, dPVm = 0.0, dQVm = 0.0;
-            const int k1 = Yp[row + 1];
-            for (int k = Yp[row]; k < k1; ++k) {
-                const int col = Yj[k];
-                const double g = PER_CASE_Y ? G[at(k, S, c)] : G[k];
-                const double b = PER_CASE_Y ? Bv[at(k, S, c)] : Bv[k];
-                if (col != row) {
-                    const size_t ok = at(col, S, c);
-                    const double vk = Vm[ok], ck = cs[ok], sk = sn[ok];
-                    const double cu = ci * ck + si * sk;
-                    const double su = si * ck - ci * sk;
-                    const double gc = g * cu + b * su;
-                    const double gs = g * su - b * cu;
-                    const double vgc = vi * gc, vgs = vi * gs;
-                    const double a = vgc * vk, bt = vgs * vk;
-                    p += a;
-                    q += bt;
-                    dPVa -= bt;
-                    dQVa += a;
-                    dPVm += vk * gc;
-                    dQVm += vk * gs;
-                    if (wj) {
-                        const int4 pj = reinterpret_cast<const int4*>(posJ)[k];
-                        if (pj.x >= 0) Jx[at(pj.x, S, c)] = pinned ? 0.0 : bt;
-                        if (pj.y >= 0) Jx[at(pj.y, S, c)] = pinned ? 0.0 : vgc;
-                        if (pj.z >= 0) Jx[at(pj.z, S, c)] = pinned ? 0.0 : -a;
-                        if (pj.w >= 0) Jx[at(pj.w, S, c)] = pinned ? 0.0 : vgs;
-                    }
+            const double ei = a.Ve[oi], fi = a.Vf[oi];
+            const bool pinned = PIN && a.pin[oi];
+            double ir = 0.0, ii = 0.0, gd = 0.0, bd = 0.0;
+            const int q1 = a.Yp[row + 1] - k0;
+            for (int q = a.Yp[row] - k0; q < q1; ++q) {
+                const int col = staged ? s_col[q] : a.Yj[k0 + q];
+                double g, b;
+                if (CASE_Y) {
+                    g = a.G[at(k0 + q, S, c)];
+                    b = a.B[at(k0 + q, S, c)];
+                } else if (staged) {
+                    g = s_y[q].x;
+                    b = s_y[q].y;
                 } else {
-                    p += vi * vi * g;
-                    q -= vi * vi * b;
-                    dPVm += 2.0 * vi * g;
-                    dQVm -= 2.0 * vi * b;
+                    g = a.G[k0 + q];
+                    b = a.B[k0 + q];
+                }
+                const size_t ok = at(col, S, c);
+                const double ek = a.Ve[ok], fk = a.Vf[ok];
+                const double tr = g * ek - b * fk, ti = g * fk + b * ek;  // Y_ik V_k
+                ir += tr;
+                ii += ti;
+                if (col == row) {
+                    gd = g;
+                    bd = b;
+                } else if (jac) {
+                    const double wr = ei * tr + fi * ti, wi = fi * tr - ei * ti;  // V_i conj(Y_ik V_k)
+                    const int4 pj = staged ? s_pos[q] : a.posJ[k0 + q];
+                    if (pj.x >= 0) a.J[at(pj.x, S, c)] = pinned ? 0.0 : wi;
+                    if (pj.y >= 0) a.J[at(pj.y, S, c)] = pinned ? 0.0 : wr;
+                    if (pj.z >= 0) a.J[at(pj.z, S, c)] = pinned ? 0.0 : -wr;
+                    if (pj.w >= 0) a.J[at(pj.w, S, c)] = pinned ? 0.0 : wi;
                 }
             }
-            const int d = Ydiag[row];
-            if (wj && d >= 0) {
-                const int4 pj = reinterpret_cast<const int4*>(posJ)[d];
-                if (pj.x >= 0) Jx[at(pj.x, S, c)] = pinned ? 1.0 : dPVa;
-                if (pj.y >= 0) Jx[at(pj.y, S, c)] = pinned ? 0.0 : dPVm;
-                if (pj.z >= 0) Jx[at(pj.z, S, c)] = pinned ? 0.0 : dQVa;
-                if (pj.w >= 0) Jx[at(pj.w, S, c)] = pinned ? 1.0 : dQVm;
+            const double P = ei * ir + fi * ii, Q = fi * ir - ei * ii;  // V_i conj(I_i)
+            if (jac) {
+                const int d = a.Ydiag[row];
+                if (d >= 0) {
+                    const int4 pj = (staged && d >= k0 && d < k1) ? s_pos[d - k0] : a.posJ[d];
+                    const double m2 = ei * ei + fi * fi;
+                    if (pj.x >= 0) a.J[at(pj.x, S, c)] = pinned ? 1.0 : -Q - m2 * bd;
+                    if (pj.y >= 0) a.J[at(pj.y, S, c)] = pinned ? 0.0 : P + m2 * gd;
+                    if (pj.z >= 0) a.J[at(pj.z, S, c)] = pinned ? 0.0 : P - m2 * gd;
+                    if (pj.w >= 0) a.J[at(pj.w, S, c)] = pinned ? 1.0 : Q - m2 * bd;
+                }
             }
-            const size_t so = 2 * (s_stride ? (size_t)row * s_stride + c : (size_t)row);
-            const int rp = row_p[row], rq = row_q[row];
+            const size_t so = 2 * (a.s_stride ? (size_t)row * a.s_stride + c : (size_t)row);
+            const int rp = a.row_p[row], rq = a.row_q[row];
             if (rp >= 0) {
-                const double r = pinned ? 0.0 : Ssp[so] - p;
-                R[at(rp, S, c)] = r;
+                const double r = pinned ? 0.0 : a.Ssp[so] - P;
+                a.R[at(rp, S, c)] = r;
                 worst = nan_max(worst, fabs(r));
             }
             if (rq >= 0) {
-                const double r = pinned ? 0.0 : Ssp[so + 1] - q;
-                R[at(rq, S, c)] = r;
+                const double r = pinned ? 0.0 : a.Ssp[so + 1] - Q;
+                a.R[at(rq, S, c)] = r;
                 worst = nan_max(worst, fabs(r));
             }
         }
     }
-    sw[wave][lane] = worst;
+    s_worst[wave][lane] = worst;
     __syncthreads();
     if (wave == 0 && run) {
-        double w = sw[0][lane];
-        for (int v = 1; v < EVAL_WAVES; ++v) w = nan_max(w, sw[v][lane]);
-        part[at(blockIdx.x, S, c)] = w;
+        for (int v = 1; v < EV_WAVES; ++v) worst = nan_max(worst, s_worst[v][lane]);
+        a.part[at(blockIdx.x, S, c)] = worst;
     }
 }
 
@@ -251,76 +344,49 @@ __global__ __launch_bounds__(64 * EVAL_WAVES) void k_eval(
 // bits whichever launch shape (cases per workgroup, task lists) solved it: one accumulator
 // runs over the operand pairs of an entry in stream order from zero, s = fma(-a, b, s), and
 // the start value (Jacobian entry, right-hand side) is added once at the end.
-// dot_pairs returns that s over the pairs [e, e1); four pairs per trip keep eight loads
-// in flight before the first use.
-template <typename ACC, typename TA, typename TB>
-__device__ __forceinline__ ACC dot_pairs(const int2* __restrict__ pairs, int e, int e1, const TA* __restrict__ A,
-                                         const TB* __restrict__ Bv, int S, int c) {
-    ACC s = 0;
-    for (; e + 4 <= e1; e += 4) {
-        const int2 p0 = pairs[e], p1 = pairs[e + 1], p2 = pairs[e + 2], p3 = pairs[e + 3];
-        const TA a0 = A[at(p0.x, S, c)], a1 = A[at(p1.x, S, c)], a2 = A[at(p2.x, S, c)], a3 = A[at(p3.x, S, c)];
-        const TB b0 = Bv[at(p0.y, S, c)], b1 = Bv[at(p1.y, S, c)], b2 = Bv[at(p2.y, S, c)], b3 = Bv[at(p3.y, S, c)];
-        s = fma(-(ACC)a0, (ACC)b0, s);
-        s = fma(-(ACC)a1, (ACC)b1, s);
-        s = fma(-(ACC)a2, (ACC)b2, s);
-        s = fma(-(ACC)a3, (ACC)b3, s);
-    }
-    for (; e < e1; ++e) {
-        const int2 p = pairs[e];
-        s = fma(-(ACC)A[at(p.x, S, c)], (ACC)Bv[at(p.y, S, c)], s);
-    }
-    return s;
-}
-
-// Static-pivot LU + substitution. A workgroup owns G consecutive cases and runs
-// TL = 64 * NS / G task lists over them: thread t works on case t % G of the group and on
-// task list t / G. G = 64 makes each wavefront one task list over 64 cases (every index
-// wave-uniform, every value access one 512-byte line); G = 16 keeps 128-byte lines and
-// gives four task lists per wavefront; G = 1 puts a whole workgroup on one case (the
-// fastest single factorization, for the last few cases of a batch).
-// Level l of the elimination tree:
+// (Each LU kernel below keeps this order; k_lu_blk's 2x2 blocks add a pair's own term last.)
+//
+// Static-pivot LU + substitution of the cases in STEP. A workgroup owns G consecutive cases and
+// runs TL = 64 * NS / G task lists over them. Level l of the elimination tree:
 //   sub-level A: pivots and U entries   acc = a - sum L(i,t) U(t,k); pivot -> 1/acc
 //                forward rows            y_i = R[perm i] - sum L(i,j) y_j
 //   barrier
 //   sub-level B: L entries               acc = (a - sum L(i,t) U(t,k)) / U(k,k)
 //   barrier
-// then backward rows by depth: x_i = (y_i - sum U(i,j) x_j) / U(i,i).
-// Pivot check (p3s/cpp/lean_lu.hpp:146-177, p3s/cpp/nr_klu.cpp:610-620): a zero or
-// non-finite pivot fails the case. A pivot ratio below rcond_min (where the reference hands
-// the rest of the solve to a pivoting factorization) is counted in guard_trips and the
-// static-pivot step is kept: the step is the same solution of J dx = R, and the oracle's
-// runs with and without that fallback agree to roundoff on every converging case.
-// Which cases a launch works on (besides being ACTIVE): every one (SEL_ALL, FP64 path
-// without mixed precision), those whose vote is not FP32 (SEL_F64: vote 0, -1 = FP64 from now
-// on, or 2 = FP32 voter promoted with its group), FP32 (SEL_F32: vote 1), or FP32 and
-// still refining (SEL_REFINE).
+// then backward rows by depth: x_i = (y_i - sum U(i,j) x_j) / U(i,i), columns descending.
+// FACTOR = true: factorize (real = double in the FP64 factor array, float in the FP32 one) and
+// solve J x = rhs; FACTOR = false: solve only, with the factors of an earlier launch (the
+// refinement solves). Substitutions accumulate in FP64 whatever the factor precision.
+
+// Which cases an LU launch works on (besides being in STEP): every one (SEL_ALL, the FP64
+// solve), the FP64 classes (SEL_F64), FP32 (SEL_F32), or FP32 and still refining (SEL_REFINE).
 enum : int { SEL_ALL = 0, SEL_F64 = 1, SEL_F32 = 2, SEL_REFINE = 3 };
 
-__device__ __forceinline__ bool selected(int sel, int c, const int* status, const int* vote, const int* rf) {
-    if (status[c] != ST_ACTIVE) return false;
+__device__ __forceinline__ bool selected(int sel, unsigned w) {
+    if (stage_of(w) != SG_STEP) return false;
+    const unsigned p = prec_of(w);
     switch (sel) {
-        case SEL_F64: return vote[c] != 1;
-        case SEL_F32: return vote[c] == 1;
-        case SEL_REFINE: return vote[c] == 1 && rf[c] == 1;
+        case SEL_F64: return p != PR_F32;
+        case SEL_F32: return p == PR_F32;
+        case SEL_REFINE: return p == PR_F32 && (w & CTL_REFINING) != 0u;
         default: return true;
     }
 }
 
 // Switching mixed mode, before its two LU launches. A wave factors 64 aligned cases, so it
 // costs its full time in a launch as soon as one of them is selected: with FP32 and FP64
-// votes mixed inside a wave, the wave would run in both launches. Here the unit is the wave:
-// when any active case of a 64-case group needs FP64, the group's FP32 voters take this
-// iteration's step in FP64 as well (vote 2: FP64 now, a fresh vote after the step), so every
-// group runs in one launch only. The groups are fixed by case index, not by launch shape.
-// Counts the FP32 and FP64 factorizations into fcount[1] and fcount[2].
-__global__ __launch_bounds__(256) void k_group_vote(const int* __restrict__ status, int* __restrict__ vote,
-                                                    int* __restrict__ fcount, int S) {
+// classes mixed inside a wave, the wave would run in both launches. Here the unit is the wave:
+// when any case of a 64-case group in STEP needs FP64, the group's FP32 cases take this step in
+// FP64 as well (PR_F64_ONCE: a fresh vote after the step), so every group runs in one launch.
+// The groups are fixed by case index, not by launch shape. Counts the FP32 and FP64
+// factorizations into fcount[1] and fcount[2].
+__global__ __launch_bounds__(256) void k_group_vote(unsigned* __restrict__ ctl, int* __restrict__ fcount, int S) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;  // blockDim 256: each wave is one group
-    const bool act = c < S && status[c] == ST_ACTIVE;
-    const int v = act ? vote[c] : 1;
-    const bool any64 = __ballot(act && v != 1) != 0ull;
-    if (act && v == 1 && any64) vote[c] = 2;
+    const unsigned w = c < S ? ctl[c] : SG_IDLE;
+    const bool act = stage_of(w) == SG_STEP;
+    const bool f32 = act && prec_of(w) == PR_F32;
+    const bool any64 = __ballot(act && !f32) != 0ull;
+    if (f32 && any64) ctl[c] = set_prec(w, PR_F64_ONCE);
     const int n_act = __popcll(__ballot(act));
     const int n32 = any64 ? 0 : n_act;
     if ((threadIdx.x & 63) == 0 && n_act) {
@@ -329,138 +395,43 @@ __global__ __launch_bounds__(256) void k_group_vote(const int* __restrict__ stat
     }
 }
 
-// FACTOR = true: factorize (real = double in place of the Jacobian's copy, real = float in
-// the single-precision factor array) and solve J x = rhs. FACTOR = false: solve only, with
-// the factors of an earlier launch (the refinement solves). Substitutions accumulate in
-// FP64 whatever the factor precision.
-// Pivot failures: FP64 -- a zero or non-finite pivot fails the case, a ratio under the guard
-// is counted in guard_trips[0]; FP32 -- either hands the case to the FP64 factorization of
-// the same iteration (vote = -1), as the reference's single-precision kernel does
-// (p3s/cuda/nr_polar_kernels.cu:477-482), counted in guard_trips[2] (the FP64
-// factorizations, see k_group_vote).
-template <int G, int NS, typename real, bool FACTOR>
-__global__ __launch_bounds__(64 * NS) void k_lu(
-    const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
-    const int4* __restrict__ taskB, const int* __restrict__ diagB, const int2* __restrict__ pairs,
-    const int* __restrict__ ptrF, const int4* __restrict__ fwd, const int2* __restrict__ fent,
-    const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int2* __restrict__ bent,
-    const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
-    int* __restrict__ status, int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips,
-    int* __restrict__ vote, const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S) {
-    constexpr bool SINGLE = sizeof(real) == 4;
-    constexpr int TL = 64 * NS / G;
-    __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
-    __shared__ int sbad[G];
-    const int cl = threadIdx.x % G;
-    const int slot = G == 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G)) : (int)(threadIdx.x / G);
-    // With one case per workgroup, a factor row's 128-byte line holds 16 cases (FP64), and
-    // workgroups are dealt to the 8 XCDs in turn: give XCD x (workgroups x, x + 8, ...) the
-    // consecutive cases [x * n / 8, (x + 1) * n / 8), so the cases sharing a line share an L2.
-    int wg = blockIdx.x;
-    if (G == 1 && (gridDim.x & 7) == 0) wg = (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
-    const int c = wg * G + cl;
-    bool act = selected(sel, c, status, vote, rf);
-    if (!__syncthreads_or(act)) return;
-
-    double pmin = DBL_MAX, pmax = 0.0, psum = 0.0;
-    for (int l = 0; l < nlev; ++l) {
-        if (act) {
-            if (FACTOR) {
-                const int q0 = ptrA[l * TL + slot], q1 = ptrA[l * TL + slot + 1];
-                for (int q = q0; q < q1; ++q) {
-                    const int4 tk = taskA[q];
-                    const bool diag = tk.y < -1;
-                    const int src = diag ? -tk.y - 2 : tk.y;
-                    real acc = (src >= 0 ? (real)Jx[at(src, S, c)] : (real)0) +
-                               dot_pairs<real>(pairs, tk.z, tk.w, fac, fac, S, c);
-                    if (diag) {
-                        const double ab = fabs((double)acc);
-                        pmin = fmin(pmin, ab);
-                        pmax = fmax(pmax, ab);
-                        psum += ab;
-                        acc = (real)1 / acc;
-                    }
-                    fac[at(tk.x, S, c)] = acc;
-                }
-            }
-            const int f0 = ptrF[l * TL + slot], f1 = ptrF[l * TL + slot + 1];
-            for (int q = f0; q < f1; ++q) {
-                const int4 fr = fwd[q];
-                x[at(fr.x, S, c)] = rhs[at(fr.y, S, c)] + dot_pairs<double>(fent, fr.z, fr.w, fac, x, S, c);
-            }
+// Pivot check at the end of a factorization (p3s/cpp/lean_lu.hpp:146-177, p3s/cpp/nr_klu.cpp:
+// 610-620), by the one thread that owns case c's statistics. FP32: a zero or non-finite pivot,
+// or a pivot ratio under the guard, moves the case to FP64 for good; it is factored in FP64 in
+// the same iteration (guard_trips[2] counts these). FP64: a zero or non-finite pivot fails the
+// case (its iteration counted); a ratio under the guard keeps the static-pivot step
+// (guard_trips[0]). Returns true when the case takes no step from these factors.
+template <bool SINGLE>
+__device__ __forceinline__ bool pivot_verdict(unsigned* ctl, int c, double lo, double hi, double sum,
+                                              double rcond_min, int* guard_trips) {
+    const bool broken = !(sum < DBL_MAX && lo > 0.0);
+    const unsigned w = ctl[c];
+    if (SINGLE) {
+        if (broken || lo < rcond_min * hi) {
+            ctl[c] = set_prec(w, PR_F64_ALWAYS);
+            atomicAdd(guard_trips + 2, 1);
+            return true;
         }
-        __syncthreads();
-        if (FACTOR) {
-            if (act) {
-                const int q0 = ptrB[l * TL + slot], q1 = ptrB[l * TL + slot + 1];
-                for (int q = q0; q < q1; ++q) {
-                    const int4 tk = taskB[q];
-                    const real acc =
-                        (tk.y >= 0 ? (real)Jx[at(tk.y, S, c)] : (real)0) + dot_pairs<real>(pairs, tk.z, tk.w, fac, fac, S, c);
-                    fac[at(tk.x, S, c)] = acc * fac[at(diagB[q], S, c)];
-                }
-            }
-            __syncthreads();
-        }
+        return false;
     }
-    if (FACTOR) {
-        smin[slot][cl] = pmin;
-        smax[slot][cl] = pmax;
-        ssum[slot][cl] = psum;
-        __syncthreads();
-        if (slot == 0) {
-            int bad = 0;
-            if (act) {
-                for (int s2 = 1; s2 < TL; ++s2) {
-                    pmin = fmin(pmin, smin[s2][cl]);
-                    pmax = fmax(pmax, smax[s2][cl]);
-                    psum += ssum[s2][cl];
-                }
-                pstat[2 * (size_t)c] = pmin;
-                pstat[2 * (size_t)c + 1] = pmax;
-                const bool broken = !(psum < DBL_MAX && pmin > 0.0);
-                if (SINGLE) {
-                    if (broken || pmin < rcond_min * pmax) {
-                        bad = 1;
-                        vote[c] = -1;
-                        atomicAdd(guard_trips + 2, 1);  // refactored in FP64 below
-                    }
-                } else if (broken) {  // zero or non-finite pivot: no step (in this iteration)
-                    bad = 1;
-                    status[c] = ST_FAILED;
-                    iters[c] += 1;
-                } else if (pmin < rcond_min * pmax) {
-                    atomicAdd(guard_trips, 1);
-                }
-            }
-            sbad[cl] = bad;
-        }
-        __syncthreads();
-        if (sbad[cl]) act = false;
+    if (broken) {
+        ctl[c] = set_iters(set_stage(w, SG_FAIL), iters_of(w) + 1);
+        return true;
     }
-    for (int l = 0; l < nblev; ++l) {
-        if (act) {
-            const int q0 = ptrW[l * TL + slot], q1 = ptrW[l * TL + slot + 1];
-            for (int q = q0; q < q1; ++q) {
-                const int4 br = bwd[q];
-                const double acc = x[at(br.x, S, c)] + dot_pairs<double>(bent, br.z, br.w, fac, x, S, c);
-                x[at(br.x, S, c)] = acc * (double)fac[at(br.y, S, c)];
-            }
-        }
-        __syncthreads();
-    }
+    if (lo < rcond_min * hi) atomicAdd(guard_trips, 1);
+    return false;
 }
 
-// ---- G = 64: operand streams, software-pipelined -------------------------------------
+// ---- operand streams, software-pipelined ----------------------------------------------
 // With 64 cases per workgroup a task list is one wavefront and its control flow is the
-// same in every lane. The host lays the operand pairs of a task list out back to back in
+// same in every lane (with fewer cases per workgroup, each lane group walks its own list). The host lays the operand pairs of a task list out back to back in
 // execution order (one stream per list and sub-level), so the wavefront can walk the
 // stream in chunks of K pairs with the loads of the next chunk in flight while it
 // multiplies the current one, across task boundaries: a task's end only stores a result.
 // The start values of a task (its Jacobian entry, pivot inverse, right-hand side) are
 // loaded one task ahead and added at its end. The memory-level parallelism of a wavefront
-// is thus 2K operand pairs whatever the task lengths, where the task-at-a-time loop of
-// k_lu waits for every task's first loads.
+// is thus 2K operand pairs whatever the task lengths, where a task-at-a-time loop would
+// wait for every task's first loads.
 // Chunk length: 8 pairs when an operand is FP64, 16 when both are FP32 (the same registers
 // and the same bytes in flight per wavefront).
 template <typename TA, typename TB>
@@ -480,7 +451,7 @@ __device__ __forceinline__ void load_chunk(const int2* __restrict__ st, int e, i
 // Tasks [q0, q1) of one list; task q = (w0, w1, pbeg, pend) with pend(q) == pbeg(q + 1).
 // start(q, task) -> In issues the task's start loads; finish(task, In, sum) consumes
 // sum = -sum_pairs a*b in type ACC.
-template <typename ACC, typename In, typename TA, typename TB, typename Start, typename Finish>
+template <int K, typename ACC, typename In, typename TA, typename TB, typename Start, typename Finish>
 __device__ __forceinline__ void seg_stream(const int4* __restrict__ tasks, int q0, int q1,
                                            const int2* __restrict__ st, const TA* __restrict__ A,
                                            const TB* __restrict__ Bv, int S, int c, Start start, Finish finish) {
@@ -509,7 +480,6 @@ __device__ __forceinline__ void seg_stream(const int4* __restrict__ tasks, int q
             }
         }
     };
-    constexpr int K = lu_chunk<TA, TB>();
     auto consume = [&](int e, const TA (&a)[K], const TB (&b)[K]) {
 #pragma unroll
         for (int u = 0; u < K; ++u) {
@@ -549,9 +519,8 @@ __device__ __forceinline__ void seg_stream(const int4* __restrict__ tasks, int q
 struct In1 { double v; };
 struct In2 { double v, d; };
 
-// Same contract and results (bit for bit: same summation order) as k_lu<G, NS, real,
-// FACTOR>. A workgroup owns G consecutive cases and TL = 64 * NS / G task lists; lane t works
-// on case t % G and list t / G. With G = 64 a wavefront is one list and every loop bound
+// The LU with one-entry tasks. A workgroup owns G consecutive cases (G = 64, 32, 16 or 1) and
+// TL = 64 * NS / G task lists; lane t works on case t % G and list t / G. With G = 64 a wavefront is one list and every loop bound
 // is uniform (scalar); with G = 32 or 16 the two or four lists of a wavefront walk their
 // streams side by side (divergent only where their task boundaries differ), which gives a
 // batch of few cases more lists per case and more workgroups for the compute units.
@@ -562,16 +531,27 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     const int* __restrict__ ptrF, const int4* __restrict__ fwd, const int2* __restrict__ fent,
     const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int2* __restrict__ bent,
     const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
-    int* __restrict__ status, int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips,
-    int* __restrict__ vote, const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S) {
+    unsigned* __restrict__ ctl, double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min,
+    int nlev, int nblev, int S) {
     constexpr bool SINGLE = sizeof(real) == 4;
     constexpr int TL = 64 * NS / G;
+    // fewer than 64 cases per workgroup: lists differ within a wavefront, so their indices and
+    // addresses sit in vector registers -- shorter chunks keep 16 wavefronts within 128
+    // registers (a quarter with one case per workgroup, half for FP32 factors otherwise)
+    constexpr int KDIV = G == 1 ? 4 : (G < 64 && SINGLE ? 2 : 1);
+    constexpr int K = lu_chunk<real, real>() / KDIV;
+    constexpr int KF = lu_chunk<real, double>() / KDIV;
     __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
     __shared__ int sbad[G];
     const int lane = threadIdx.x % G;
     const int slot = G == 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G)) : (int)(threadIdx.x / G);
-    const int c = blockIdx.x * G + lane;
-    bool act = selected(sel, c, status, vote, rf);
+    // With one case per workgroup, a factor row's 128-byte line holds 16 cases (FP64), and
+    // workgroups are dealt to the 8 XCDs in turn: XCD x takes the consecutive cases
+    // [x * n / 8, (x + 1) * n / 8) (workgroups x, x + 8, ...), so the cases of a line share an L2.
+    int wg = blockIdx.x;
+    if (G == 1 && (gridDim.x & 7) == 0) wg = (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+    const int c = wg * G + lane;
+    bool act = selected(sel, ctl[c]);
     if (!__syncthreads_or(act)) return;
 
     double pmin = DBL_MAX, pmax = 0.0, psum = 0.0;
@@ -607,12 +587,12 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
         const int i0 = l * TL + slot;
         if (act) {
             if (FACTOR)
-                seg_stream<real, In1>(taskA, ptrA[i0], ptrA[i0 + 1], pairs, fac, fac, S, c, startA, finishA);
-            seg_stream<double, In1>(fwd, ptrF[i0], ptrF[i0 + 1], fent, fac, x, S, c, startF, finishF);
+                seg_stream<K, real, In1>(taskA, ptrA[i0], ptrA[i0 + 1], pairs, fac, fac, S, c, startA, finishA);
+            seg_stream<KF, double, In1>(fwd, ptrF[i0], ptrF[i0 + 1], fent, fac, x, S, c, startF, finishF);
         }
         __syncthreads();
         if (FACTOR) {
-            if (act) seg_stream<real, In2>(taskB, ptrB[i0], ptrB[i0 + 1], pairs, fac, fac, S, c, startB, finishB);
+            if (act) seg_stream<K, real, In2>(taskB, ptrB[i0], ptrB[i0 + 1], pairs, fac, fac, S, c, startB, finishB);
             __syncthreads();
         }
     }
@@ -631,20 +611,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
                 }
                 pstat[2 * (size_t)c] = pmin;
                 pstat[2 * (size_t)c + 1] = pmax;
-                const bool broken = !(psum < DBL_MAX && pmin > 0.0);
-                if (SINGLE) {
-                    if (broken || pmin < rcond_min * pmax) {
-                        bad = 1;
-                        vote[c] = -1;
-                        atomicAdd(guard_trips + 2, 1);  // refactored in FP64 below
-                    }
-                } else if (broken) {
-                    bad = 1;
-                    status[c] = ST_FAILED;
-                    iters[c] += 1;
-                } else if (pmin < rcond_min * pmax) {
-                    atomicAdd(guard_trips, 1);
-                }
+                bad = pivot_verdict<SINGLE>(ctl, c, pmin, pmax, psum, rcond_min, guard_trips) ? 1 : 0;
             }
             sbad[lane] = bad;
         }
@@ -653,7 +620,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     }
     for (int l = 0; l < nblev; ++l) {
         const int i0 = l * TL + slot;
-        if (act) seg_stream<double, In2>(bwd, ptrW[i0], ptrW[i0 + 1], bent, fac, x, S, c, startW, finishW);
+        if (act) seg_stream<KF, double, In2>(bwd, ptrW[i0], ptrW[i0 + 1], bent, fac, x, S, c, startW, finishW);
         __syncthreads();
     }
 }
@@ -1018,9 +985,8 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
     const int4* __restrict__ blkB, const int4* __restrict__ steps, const int* __restrict__ ptrF,
     const int4* __restrict__ fwd, const int4* __restrict__ fsteps, const int* __restrict__ ptrW,
     const int4* __restrict__ bwd, const int4* __restrict__ bsteps, const double* __restrict__ Jx,
-    real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x, int* __restrict__ status,
-    int* __restrict__ iters, double* __restrict__ pstat, int* __restrict__ guard_trips, int* __restrict__ vote,
-    const int* __restrict__ rf, int sel, double rcond_min, int nlev, int nblev, int S) {
+    real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x, unsigned* __restrict__ ctl,
+    double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min, int nlev, int nblev, int S) {
     constexpr bool SINGLE = sizeof(real) == 4;
     constexpr int G = 64, TL = NS;
     __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
@@ -1028,7 +994,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
     const int lane = threadIdx.x % G;
     const int slot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G));
     const int c = blockIdx.x * G + lane;
-    bool act = selected(sel, c, status, vote, rf);
+    bool act = selected(sel, ctl[c]);
     if (!__syncthreads_or(act)) return;
 
     // pivot statistics live in LDS (a diagonal block updates them; registers are scarce here)
@@ -1064,20 +1030,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
                 }
                 pstat[2 * (size_t)c] = lo;
                 pstat[2 * (size_t)c + 1] = hi;
-                const bool broken = !(sum < DBL_MAX && lo > 0.0);
-                if (SINGLE) {
-                    if (broken || lo < rcond_min * hi) {
-                        bad = 1;
-                        vote[c] = -1;
-                        atomicAdd(guard_trips + 2, 1);  // refactored in FP64 below
-                    }
-                } else if (broken) {
-                    bad = 1;
-                    status[c] = ST_FAILED;
-                    iters[c] += 1;
-                } else if (lo < rcond_min * hi) {
-                    atomicAdd(guard_trips, 1);
-                }
+                bad = pivot_verdict<SINGLE>(ctl, c, lo, hi, sum, rcond_min, guard_trips) ? 1 : 0;
             }
             sbad[lane] = bad;
         }
@@ -1096,11 +1049,10 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
 __global__ __launch_bounds__(256) void k_resid(const int* __restrict__ Jp, const int* __restrict__ xcol,
                                                const double* __restrict__ Jx, const double* __restrict__ R,
                                                const double* __restrict__ x, double* __restrict__ r,
-                                               const int* __restrict__ status, const int* __restrict__ vote,
-                                               const int* __restrict__ rf, int m, int S) {
+                                               const unsigned* __restrict__ ctl, int m, int S) {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const int row = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (row >= m || !selected(SEL_REFINE, c, status, vote, rf)) return;
+    if (row >= m || !selected(SEL_REFINE, ctl[c])) return;
     double acc = R[at(row, S, c)], acc1 = 0.0;
     const int e1 = Jp[row + 1];
     int e = Jp[row];
@@ -1115,13 +1067,12 @@ __global__ __launch_bounds__(256) void k_resid(const int* __restrict__ Jp, const
 // x += d for the refining cases; per workgroup (64 pivots x 64 cases) the largest |d| and
 // |x| of each case go to part[2 * (blockIdx.x * S + c) + {0, 1}].
 __global__ __launch_bounds__(256) void k_refine_update(double* __restrict__ x, const double* __restrict__ d,
-                                                       double* __restrict__ part, const int* __restrict__ status,
-                                                       const int* __restrict__ vote, const int* __restrict__ rf,
+                                                       double* __restrict__ part, const unsigned* __restrict__ ctl,
                                                        int m, int S) {
     __shared__ double sd[4][64], sx[4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = blockIdx.y * 64 + lane;
-    const bool on = selected(SEL_REFINE, c, status, vote, rf);
+    const bool on = selected(SEL_REFINE, ctl[c]);
     double dm = 0.0, xm = 0.0;
     if (on) {
         const int p1 = min(m, (int)(blockIdx.x + 1) * 64);
@@ -1146,27 +1097,26 @@ __global__ __launch_bounds__(256) void k_refine_update(double* __restrict__ x, c
     }
 }
 
-// Per refining case: stop when |d| <= rtol |x|; fall back to FP64 (vote = 0) when a step
-// did not at least halve |d| (refinement stalls: cond(J) times the FP32 rounding is too big)
-// or when the last allowed step still misses rtol. counts[0] = cases still refining.
+// Per refining case: stop when |d| <= rtol |x|; fall back to FP64 for good when a step did not
+// at least halve |d| (refinement stalls: cond(J) times the FP32 rounding is too big) or when the
+// last allowed step still misses rtol. counts[0] = cases still refining.
 __global__ void k_refine_decide(const double* __restrict__ part, int nparts, double* __restrict__ prevd,
-                                const int* __restrict__ status, int* __restrict__ vote, int* __restrict__ rf,
-                                int* __restrict__ counts, double rtol, int last, int S) {
+                                unsigned* __restrict__ ctl, int* __restrict__ counts, double rtol, int last, int S) {
     __shared__ int cnt;
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < S && selected(SEL_REFINE, c, status, vote, rf)) {
+    const unsigned w = c < S ? ctl[c] : SG_IDLE;
+    if (selected(SEL_REFINE, w)) {
         double dm = 0.0, xm = 0.0;
         for (int b = 0; b < nparts; ++b) {
             dm = nan_max(dm, part[2 * at(b, S, c)]);
             xm = nan_max(xm, part[2 * at(b, S, c) + 1]);
         }
         if (dm <= rtol * xm) {
-            rf[c] = 0;
+            ctl[c] = w & ~CTL_REFINING;
         } else if (!(dm <= 0.5 * prevd[c]) || last) {
-            rf[c] = 0;
-            vote[c] = -1;
+            ctl[c] = set_prec(w & ~CTL_REFINING, PR_F64_ALWAYS);
         } else {
             prevd[c] = dm;
             atomicAdd(&cnt, 1);
@@ -1176,12 +1126,13 @@ __global__ void k_refine_decide(const double* __restrict__ part, int nparts, dou
     if (threadIdx.x == 0 && cnt) atomicAdd(&counts[0], cnt);
 }
 
-// Start of a mixed-precision iteration: refining = FP32 vote, no previous step.
-__global__ void k_refine_init(const int* __restrict__ status, const int* __restrict__ vote, int* __restrict__ rf,
-                              double* __restrict__ prevd, int S) {
+// Start of a mixed-precision iteration: the FP32 cases in STEP refine, no previous correction.
+__global__ void k_refine_init(unsigned* __restrict__ ctl, double* __restrict__ prevd, int S) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= S) return;
-    rf[c] = (status[c] == ST_ACTIVE && vote[c] == 1) ? 1 : 0;
+    const unsigned w = ctl[c];
+    const bool on = stage_of(w) == SG_STEP && prec_of(w) == PR_F32;
+    ctl[c] = on ? (w | CTL_REFINING) : (w & ~CTL_REFINING);
     prevd[c] = DBL_MAX;
 }
 
@@ -1191,138 +1142,176 @@ __global__ void k_zero(T* p, size_t n) {
     if (i < n) p[i] = (T)0;
 }
 
-__global__ void k_fill_int(int* p, int v, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-// V = V0 + alpha * dx for the cases whose status is in mask. full: alpha = 1 and the
-// current point is saved as the base V0 of the step (p3s/cpp/nr_klu.cpp:665-669).
-__global__ void k_step(const double* __restrict__ x, const int* __restrict__ xva, const int* __restrict__ xvm,
-                       const int* __restrict__ status, int mask, int full, const double* __restrict__ alpha,
-                       double* __restrict__ Vm, double* __restrict__ Va, double* __restrict__ cs,
-                       double* __restrict__ sn, double* __restrict__ Vm0, double* __restrict__ Va0, int n, int S) {
+// Newton move of the cases this pass advances (p3s/cpp/nr_klu.cpp:665-700): STEP takes the full
+// step (alpha 1) from the current point and saves that point as the step's base; TRIAL and
+// COMMIT move to base + alpha dx. The solve's x is in pivot order: x[xva[bus]] the angle step,
+// x[xvm[bus]] the relative magnitude step dVm / Vm (magnitude columns are in ln |V|), so
+// Vm = Vm0 (1 + alpha x). One thread per (bus, case); the slack has neither and stays.
+__global__ __launch_bounds__(256) void k_advance(const double* __restrict__ x, const int* __restrict__ xva,
+                                                 const int* __restrict__ xvm, const unsigned* __restrict__ ctl,
+                                                 const double* __restrict__ alpha, double* __restrict__ Vm,
+                                                 double* __restrict__ Va, double* __restrict__ Ve,
+                                                 double* __restrict__ Vf, double* __restrict__ Vm0,
+                                                 double* __restrict__ Va0, int n, int S) {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int bus = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (bus >= n) return;
-    if (!((mask >> status[c]) & 1)) return;
-    const double a = full ? 1.0 : alpha[c];
+    const int iva = xva[bus], ivm = xvm[bus];
+    if (iva < 0 && ivm < 0) return;
+    const unsigned sg = stage_of(ctl[c]);
+    if (sg != SG_STEP && sg != SG_TRIAL && sg != SG_COMMIT) return;
     const size_t o = at(bus, S, c);
-    const int iva = xva[bus];
-    if (iva >= 0) {
-        double va0;
-        if (full) { va0 = Va[o]; Va0[o] = va0; } else va0 = Va0[o];
-        const double va = va0 + a * x[at(iva, S, c)];
-        double s, co;
-        sincos(va, &s, &co);
-        Va[o] = va;
-        cs[o] = co;
-        sn[o] = s;
+    double va, vm, step;
+    if (sg == SG_STEP) {
+        va = Va[o];
+        vm = Vm[o];
+        Va0[o] = va;
+        Vm0[o] = vm;
+        step = 1.0;
+    } else {
+        va = Va0[o];
+        vm = Vm0[o];
+        step = alpha[c];
     }
-    const int ivm = xvm[bus];
-    if (ivm >= 0) {
-        double vm0;
-        if (full) { vm0 = Vm[o]; Vm0[o] = vm0; } else vm0 = Vm0[o];
-        Vm[o] = vm0 + a * x[at(ivm, S, c)];
-    }
+    if (iva >= 0) va += step * x[at(iva, S, c)];
+    if (ivm >= 0) vm = fma(vm, step * x[at(ivm, S, c)], vm);
+    double s, co;
+    sincos(va, &s, &co);
+    Va[o] = va;
+    Vm[o] = vm;
+    Ve[o] = vm * co;
+    Vf[o] = vm * s;
 }
 
-// Per-case Newton bookkeeping after an eval pass (p3s/cpp/nr_klu.cpp:590-709):
-//   phase 0  start point of a NEW case: converged if ||F|| < tol, failed if non-finite
-//   phase 1  full step of the case's next iteration: Armijo test, else start backtracking
-//   phase 2  backtracking trial: accept, halve, or after 10 trials take the best
-//   phase 3  Jacobian rebuilt at the accepted point
-// The loop of the reference ends after the step of iteration max_iter without a further
-// convergence test, so such a case ends STOPPED (not converged).
-// Precision vote of the switching mixed mode (p3s/cuda/nr_polar_kernels.cu:209-275): FP32
-// factors while the mismatch is above `sw`, FP64 below it, and FP64 from then on (-1) once an
-// FP32 step failed to halve the mismatch or the line search had to cut a step.
-__device__ __forceinline__ void vote_switch(int* vote, int c, double old_nrm, double nw, double sw) {
-    if (vote[c] < 0) return;
-    if (vote[c] == 1 && !(nw <= 0.5 * old_nrm)) vote[c] = -1;
-    else vote[c] = nw > sw ? 1 : 0;
+// Precision vote of the switching mixed mode after an accepted step (the reference GPU
+// solver's rule, p3s/cuda/nr_polar_kernels.cu:209-275): FP32 while the mismatch is above `sw`,
+// FP64 below it, FP64 for good once an FP32 step failed to halve the mismatch. (A cut step
+// sets FP64 for good where the line search starts.)
+__device__ __forceinline__ unsigned switch_vote(unsigned p, double before, double after, double sw) {
+    if (p == PR_F64_ALWAYS) return p;
+    if (p == PR_F32 && !(after <= 0.5 * before)) return PR_F64_ALWAYS;
+    return after > sw ? PR_F32 : PR_F64;
 }
 
-__global__ void k_decide(const double* __restrict__ part, int nparts, double* __restrict__ nrm,
-                         double* __restrict__ alpha, double* __restrict__ best_alpha, double* __restrict__ best_nrm,
-                         int* __restrict__ trial, int* __restrict__ status, int* __restrict__ iters,
-                         int* __restrict__ counts, int* __restrict__ vote, double tol, int line_search, int phase,
-                         int max_iter, double sw, int S) {
-    __shared__ int cnt[NSTAT];
-    if (threadIdx.x < NSTAT) cnt[threadIdx.x] = 0;
+// Newton bookkeeping after k_evaluate, one thread per case, by the stage the case was evaluated
+// in (p3s/cpp/nr_klu.cpp:590-709):
+//   START   the start point: converged if ||R|| < tol, failed if non-finite, else iterating
+//   STEP    the full step of iteration it + 1: accepted if ||R|| <= (1 - c) ||R_0|| (always
+//           without line search), else the line search starts at alpha 1/2
+//   TRIAL   accepted if ||R|| <= (1 - c alpha) ||R_0||, else alpha halves; after 10 trials the
+//           best one is taken
+//   COMMIT  the Jacobian rebuilt at the accepted point: converged, failed or iterating
+// nr_klu leaves its loop after the step of iteration max_iter without a further convergence
+// test, so such a case ends STOP. A non-finite norm counts as DBL_MAX, so no comparison
+// against it passes. The stage counters of the launch are summed over its workgroups; the last
+// workgroup to finish copies them to counts_out (what the host reads) and clears the sums for
+// the next launch, so no clearing kernel or memset is needed.
+struct CtlArgs {
+    const double* part;  // (nparts, S) largest |R| per row tile
+    int nparts;
+    unsigned* ctl;
+    double* nrm;         // mismatch norm at the accepted point
+    double* alpha;
+    double* best_alpha;
+    double* best_nrm;
+    int* tally;          // NSTAT sums + 1 finish counter, zero between launches
+    int* counts_out;     // NSTAT
+    double tol, sw;      // sw > 0: switching mixed mode
+    int line_search, max_iter, S;
+};
+
+__global__ __launch_bounds__(256) void k_control(CtlArgs a) {
+    __shared__ int s_cnt[NSTAT];
+    __shared__ int s_last;
+    if (threadIdx.x < NSTAT) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < S) {
-        int st = status[c];
-        const int want = phase == 0 ? ST_NEW : phase == 2 ? ST_BACKTRACK : phase == 3 ? ST_NEEDJ : ST_ACTIVE;
-        if (st == want) {
+    if (c < a.S) {
+        unsigned w = a.ctl[c];
+        unsigned sg = stage_of(w);
+        if (evaluated_stage(sg)) {
             double nw = 0.0;
-            for (int b = 0; b < nparts; ++b) nw = nan_max(nw, part[at(b, S, c)]);
+            for (int t = 0; t < a.nparts; ++t) nw = nan_max(nw, a.part[at(t, a.S, c)]);
             if (!(nw < DBL_MAX)) nw = DBL_MAX;
-            auto classify = [&](double v) {
-                if (phase != 0 && iters[c] >= max_iter) return (int)ST_STOPPED;
-                return v < tol ? (int)ST_CONVERGED : (v >= DBL_MAX ? (int)ST_FAILED : (int)ST_ACTIVE);
+            int it = iters_of(w);
+            unsigned prec = prec_of(w);
+            const double before = a.nrm[c];
+            auto settle = [&](double v, bool stepped) -> unsigned {
+                if (stepped && it >= a.max_iter) return SG_STOP;
+                if (v < a.tol) return SG_CONV;
+                return v >= DBL_MAX ? SG_FAIL : SG_STEP;
             };
-            if (phase == 0) {
-                nrm[c] = nw;
-                iters[c] = 0;
-                st = classify(nw);
-                if (sw > 0.0) vote[c] = nw > sw ? 1 : 0;
-            } else if (phase == 1) {
-                iters[c] += 1;
-                if (!line_search || nw <= (1.0 - ARMIJO_C) * nrm[c]) {
-                    if (sw > 0.0) vote_switch(vote, c, nrm[c], nw, sw);
-                    nrm[c] = nw;
-                    st = classify(nw);
+            if (sg == SG_START) {
+                it = 0;
+                a.nrm[c] = nw;
+                sg = settle(nw, false);
+                if (a.sw > 0.0) prec = nw > a.sw ? PR_F32 : PR_F64;
+            } else if (sg == SG_STEP) {
+                it += 1;
+                if (!a.line_search || nw <= (1.0 - ARMIJO_C) * before) {
+                    if (a.sw > 0.0) prec = switch_vote(prec, before, nw, a.sw);
+                    a.nrm[c] = nw;
+                    sg = settle(nw, true);
                 } else {
-                    best_alpha[c] = 1.0;
-                    best_nrm[c] = nw;
-                    alpha[c] = 0.5;
-                    trial[c] = 1;
-                    st = ST_BACKTRACK;
-                    if (sw > 0.0) vote[c] = -1;
+                    a.best_alpha[c] = 1.0;
+                    a.best_nrm[c] = nw;
+                    a.alpha[c] = 0.5;
+                    w = set_trials(w, 1);
+                    sg = SG_TRIAL;
+                    if (a.sw > 0.0) prec = PR_F64_ALWAYS;
                 }
-            } else if (phase == 2) {
-                const double a = alpha[c];
-                if (nw < best_nrm[c]) { best_nrm[c] = nw; best_alpha[c] = a; }
-                if (nw <= (1.0 - ARMIJO_C * a) * nrm[c]) {
-                    st = ST_NEEDJ;
-                } else if (trial[c] >= LS_MAX_TRIALS) {
-                    alpha[c] = best_alpha[c];
-                    st = ST_NEEDJ;
+            } else if (sg == SG_TRIAL) {
+                const double al = a.alpha[c];
+                if (nw < a.best_nrm[c]) {
+                    a.best_nrm[c] = nw;
+                    a.best_alpha[c] = al;
+                }
+                const unsigned tr = trials_of(w);
+                if (nw <= (1.0 - ARMIJO_C * al) * before) {
+                    sg = SG_COMMIT;
+                } else if ((int)tr >= LS_MAX_TRIALS) {
+                    a.alpha[c] = a.best_alpha[c];
+                    sg = SG_COMMIT;
                 } else {
-                    trial[c] += 1;
-                    alpha[c] = 0.5 * a;
+                    w = set_trials(w, tr + 1);
+                    a.alpha[c] = 0.5 * al;
                 }
-            } else {
-                nrm[c] = nw;
-                st = classify(nw);
+            } else {  // SG_COMMIT
+                a.nrm[c] = nw;
+                sg = settle(nw, true);
+                w = set_trials(w, 0);
             }
-            status[c] = st;
+            a.ctl[c] = set_iters(set_prec(set_stage(w, sg), prec), it);
         }
-        atomicAdd(&cnt[st], 1);
+        atomicAdd(&s_cnt[counter_of(sg)], 1);
     }
     __syncthreads();
-    if (threadIdx.x < NSTAT && cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], cnt[threadIdx.x]);
+    if (threadIdx.x < NSTAT && s_cnt[threadIdx.x]) atomicAdd(&a.tally[threadIdx.x], s_cnt[threadIdx.x]);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&a.tally[NSTAT], 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        if (threadIdx.x < NSTAT) a.counts_out[threadIdx.x] = atomicExch(&a.tally[threadIdx.x], 0);
+        if (threadIdx.x == 0) atomicExch(&a.tally[NSTAT], 0);
+    }
 }
 
-// polar state -> complex V (n, B), the layout of the numpy result.
-// polar state -> complex V (n, B), the layout of the numpy result; NaN at buses a served
+// Complex V (n, B) of the cases' state, the layout of the numpy result; NaN at buses a served
 // mask marks unserved (p3s/contingency/solver_cpp.py:147-150).
-__global__ void k_finalize(const double* __restrict__ Vm, const double* __restrict__ cs,
-                           const double* __restrict__ sn, const unsigned char* __restrict__ served,
-                           double* __restrict__ V, int n, int S, int B) {
+__global__ __launch_bounds__(256) void k_emit(const double* __restrict__ Ve, const double* __restrict__ Vf,
+                                              const unsigned char* __restrict__ served, double* __restrict__ V, int n,
+                                              int S, int B) {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int bus = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (bus >= n || c >= B) return;
     const size_t o = at(bus, S, c);
-    const double vm = Vm[o];
-    double re = vm * cs[o], im = vm * sn[o];
+    double re = Ve[o], im = Vf[o];
     if (served && !served[o]) re = im = __longlong_as_double(0x7ff8000000000000ll);
     V[2 * ((size_t)bus * B + c)] = re;
     V[2 * ((size_t)bus * B + c) + 1] = im;
 }
-
 
 
 // Time-series injections on the device (p3s/timeseries.py:36-141): for bus i and case t,
@@ -1353,18 +1342,19 @@ __global__ void k_sbus(const double* __restrict__ s0, const int* __restrict__ pt
 }
 
 // Voltage band of every case: min and max |V| over its buses, skipping buses a served mask
-// marks unserved. NewtonPowerflowCpp.calculate accepts a converged solve only if every
-// |V| lies in [0.5, 1.5] p.u. (p3s/NewtonPowerflowCpp.py:553-584). One thread per case,
-// reads coalesced across cases.
-__global__ void k_vrange(const double* __restrict__ Vm, const unsigned char* __restrict__ served,
-                         double* __restrict__ vmin, double* __restrict__ vmax, int n, int S, int B) {
+// marks unserved. NewtonPowerflowCpp.calculate accepts a converged solve only if every |V| lies
+// in [0.5, 1.5] p.u. (p3s/NewtonPowerflowCpp.py:553-584); |V| is that of the returned voltage.
+// One thread per case, reads coalesced across cases.
+__global__ void k_vrange(const double* __restrict__ Ve, const double* __restrict__ Vf,
+                         const unsigned char* __restrict__ served, double* __restrict__ vmin,
+                         double* __restrict__ vmax, int n, int S, int B) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= B) return;
     double lo = DBL_MAX, hi = -DBL_MAX;
     for (int i = 0; i < n; ++i) {
         const size_t o = at(i, S, c);
         if (served && !served[o]) continue;
-        const double v = fabs(Vm[o]);
+        const double v = hypot(Ve[o], Vf[o]);
         lo = fmin(lo, v);
         hi = fmax(hi, v);
     }
@@ -1391,9 +1381,9 @@ __device__ __forceinline__ void cmul(double ar, double ai, double br, double bi,
 }
 
 __global__ void k_branch_flows(const int* __restrict__ fb, const int* __restrict__ tb, const double* __restrict__ yval,
-                               const int* __restrict__ gptr, const int* __restrict__ gbr, const double* __restrict__ Vm,
-                               const double* __restrict__ cs, const double* __restrict__ sn,
-                               const unsigned char* __restrict__ served, const int* __restrict__ status,
+                               const int* __restrict__ gptr, const int* __restrict__ gbr, const double* __restrict__ Ve,
+                               const double* __restrict__ Vf, const unsigned char* __restrict__ served,
+                               const unsigned* __restrict__ ctl,
                                const double* __restrict__ vnf, const double* __restrict__ vnt,
                                const double* __restrict__ rating, const unsigned char* __restrict__ trafo, double sn_mva,
                                int nbr, int L, int S, double* __restrict__ pf, double* __restrict__ qf,
@@ -1407,8 +1397,8 @@ __global__ void k_branch_flows(const int* __restrict__ fb, const int* __restrict
     for (int k = blockIdx.y; k < nbr; k += gridDim.y) {
         const int f = fb[k], t = tb[k];
         const size_t of = at(f, S, c), ot = at(t, S, c);
-        double vfr = Vm[of] * cs[of], vfi = Vm[of] * sn[of];
-        double vtr = Vm[ot] * cs[ot], vti = Vm[ot] * sn[ot];
+        double vfr = Ve[of], vfi = Vf[of];
+        double vtr = Ve[ot], vti = Vf[ot];
         if (!served[of]) vfr = vfi = nan;
         if (!served[ot]) vtr = vti = nan;
         const double* y = yval + 8 * (size_t)k;  // ff, ft, tf, tt (re, im)
@@ -1446,7 +1436,7 @@ __global__ void k_branch_flows(const int* __restrict__ fb, const int* __restrict
         bool out = false;
         for (int q = gptr[c]; q < gptr[c + 1]; ++q) out = out || gbr[q] == k;
         if (out) sfr = sfi = str_ = sti = cif = cit = imax = ld = 0.0;
-        if (status[c] != ST_CONVERGED) {
+        if (stage_of(ctl[c]) != SG_CONV) {
             // the reference assigns NaN into its complex power arrays, which stores NaN + 0j:
             // a non-converged case reports NaN active and 0 reactive power
             // (p3s/contingency/find_overloads.py:249-252)
@@ -1514,19 +1504,19 @@ __global__ void k_over_list(const double* __restrict__ loading, const double* __
 // Bus voltage-band violations (p3s/contingency/voltage_band.py:103-182): |V| below
 // vm_min - tol (kind 0) or above vm_max + tol (kind 1) at served, in-service buses of
 // converged cases; a NaN bound never flags.
-__global__ void k_vviol(const double* __restrict__ Vm, const double* __restrict__ cs, const double* __restrict__ sn,
-                        const unsigned char* __restrict__ served, const int* __restrict__ status,
+__global__ void k_vviol(const double* __restrict__ Ve, const double* __restrict__ Vf,
+                        const unsigned char* __restrict__ served, const unsigned* __restrict__ ctl,
                         const double* __restrict__ vmin, const double* __restrict__ vmax,
                         const unsigned char* __restrict__ in_service, double tol, int n, int L, int S, int cap,
                         int* __restrict__ count, int* __restrict__ ob, int* __restrict__ oc, double* __restrict__ ov,
                         unsigned char* __restrict__ okind) {
 #pragma clang fp contract(off)
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= L || status[c] != ST_CONVERGED) return;
+    if (c >= L || stage_of(ctl[c]) != SG_CONV) return;
     for (int i = blockIdx.y; i < n; i += gridDim.y) {
         const size_t o = at(i, S, c);
         if (!served[o] || !in_service[i]) continue;
-        const double vm = hypot(Vm[o] * cs[o], Vm[o] * sn[o]);
+        const double vm = hypot(Ve[o], Vf[o]);
         const bool low = vm < vmin[i] - tol, high = vm > vmax[i] + tol;
         if (low || high) {
             const int j = atomicAdd(count, 1);
@@ -1536,10 +1526,11 @@ __global__ void k_vviol(const double* __restrict__ Vm, const double* __restrict_
 }
 
 // ===================================================================== slots (serving)
-// Copy k staged cases (S and V0, (n, k) complex) into slot columns and mark them NEW.
-__global__ void k_load_slots(const double* __restrict__ Sst, const double* __restrict__ Vst,
-                             const int* __restrict__ slots, double* __restrict__ Ssp, double* __restrict__ V0,
-                             int* __restrict__ status, int n, int k, int S) {
+// Copy k staged cases (S and V0, (n, k) complex) into slot columns and mark them START.
+__global__ __launch_bounds__(256) void k_load_slots(const double* __restrict__ Sst, const double* __restrict__ Vst,
+                                                    const int* __restrict__ slots, double* __restrict__ Ssp,
+                                                    double* __restrict__ V0, unsigned* __restrict__ ctl, int n, int k,
+                                                    int S) {
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (j >= k || bus >= n) return;
@@ -1549,33 +1540,33 @@ __global__ void k_load_slots(const double* __restrict__ Sst, const double* __res
     Ssp[dst + 1] = Sst[src + 1];
     V0[dst] = Vst[src];
     V0[dst + 1] = Vst[src + 1];
-    if (bus == 0) status[c] = ST_NEW;
+    if (bus == 0) ctl[c] = SG_START;
 }
 
 // Results of k slots into (n, k) complex V, iterations, flags, final mismatch.
-__global__ void k_fetch_slots(const int* __restrict__ slots, const double* __restrict__ Vm,
-                              const double* __restrict__ cs, const double* __restrict__ sn,
-                              const int* __restrict__ status, const int* __restrict__ iters,
-                              const double* __restrict__ nrm, double* __restrict__ V, int* __restrict__ it_out,
-                              unsigned char* __restrict__ cv_out, double* __restrict__ mis_out, int n, int k, int S) {
+__global__ __launch_bounds__(256) void k_fetch_slots(const int* __restrict__ slots, const double* __restrict__ Ve,
+                                                     const double* __restrict__ Vf, const unsigned* __restrict__ ctl,
+                                                     const double* __restrict__ nrm, double* __restrict__ V,
+                                                     int* __restrict__ it_out, unsigned char* __restrict__ cv_out,
+                                                     double* __restrict__ mis_out, int n, int k, int S) {
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (j >= k || bus >= n) return;
     const int c = slots[j];
     const size_t o = at(bus, S, c);
-    const double vm = Vm[o];
-    V[2 * ((size_t)bus * k + j)] = vm * cs[o];
-    V[2 * ((size_t)bus * k + j) + 1] = vm * sn[o];
+    V[2 * ((size_t)bus * k + j)] = Ve[o];
+    V[2 * ((size_t)bus * k + j) + 1] = Vf[o];
     if (bus == 0) {
-        it_out[j] = iters[c];
-        cv_out[j] = status[c] == ST_CONVERGED;
+        const unsigned w = ctl[c];
+        it_out[j] = iters_of(w);
+        cv_out[j] = stage_of(w) == SG_CONV;
         mis_out[j] = nrm[c];
     }
 }
 
-__global__ void k_release_slots(const int* __restrict__ slots, int* __restrict__ status, int k) {
+__global__ void k_release_slots(const int* __restrict__ slots, unsigned* __restrict__ ctl, int k) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < k) status[slots[j]] = ST_IDLE;
+    if (j < k) ctl[slots[j]] = SG_IDLE;
 }
 
 // ===================================================================== N-1 islands
@@ -1693,24 +1684,6 @@ __global__ __launch_bounds__(256) void k_islands(
             else if (pinned_ref) { re = hypot(re, im); im = 0.0; }
             V0[2 * o] = re;
             V0[2 * o + 1] = im;
-        }
-    }
-}
-
-// Per-case Ybus = base - the stamps of the case's outaged branches, subtracted in group
-// order (p3s/contingency/case_generator.py:319-325). One thread per case.
-__global__ void k_case_y(const int* __restrict__ gptr, const int* __restrict__ gbr, const int* __restrict__ ypos,
-                         const double* __restrict__ yval, double* __restrict__ G, double* __restrict__ Bv, int c0,
-                         int nc, int S) {
-    const int cl = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cl >= nc) return;
-    const int cs = c0 + cl;
-    for (int q = gptr[cs]; q < gptr[cs + 1]; ++q) {
-        const int k = gbr[q];
-        for (int j = 0; j < 4; ++j) {
-            const size_t o = at(ypos[4 * k + j], S, cl);
-            G[o] -= yval[2 * (4 * k + j)];
-            Bv[o] -= yval[2 * (4 * k + j) + 1];
         }
     }
 }
@@ -1938,6 +1911,8 @@ struct DevSched {  // one level schedule dealt to TL task lists
 struct DevTopo {
     DevBuf<int> Yp, Yj, Ydiag, row_p, row_q, posJ, xva, xvm, Jp, xcol;
     DevBuf<double> G, B;
+    DevBuf<int> tiles;    // k_evaluate row tiles: first row of each, then n
+    int ntiles = 0, tile_rows = 0;
     std::map<int, std::unique_ptr<DevSched>> sched;
     bool ready = false;
 };
@@ -1965,6 +1940,7 @@ struct p3smi_topology {
     double mixed_switch = 1e-3;   // switching mode: FP32 factors while the mismatch is above this
     int mixed_kmax = 6;           // refinement steps before the FP64 fallback
     int max_chunk = 0;            // one-call entry points: cases per chunk at most (0: sized to free HBM)
+    int eval_rows = 32;           // k_evaluate: rows per tile at most (the autotuner may change it)
     hipStream_t stream = nullptr;
     std::unique_ptr<p3smi_batch> cached;  // workspace of the one-call entry points
     std::mutex mu;
@@ -1974,23 +1950,23 @@ struct p3smi_topology {
 struct p3smi_batch {
     p3smi_topology* topo = nullptr;
     int B = 0, S = 0;
-    int nparts = 0;
-    DevBuf<double> Vm, Va, cs, sn, Vm0, Va0, Jx, fac, R, x;
+    DevBuf<double> Vm, Va, Ve, Vf, Vm0, Va0, Jx, fac, R, x;  // polar, rectangular and base state
     DevBuf<float> fac32;                // FP32 factors (mixed precision)
     DevBuf<double> rr, dvec, prevd, rpart;  // refinement residual, correction, norms
-    DevBuf<int> vote, rf;               // per case: FP32 (1) / FP64 (0); still refining
     int nparts_r = 0;
     bool mixed_ready = false;
     DevBuf<double> nrm, alpha, best_alpha, best_nrm, part, pstat;
-    DevBuf<int> status, iters, trial, counts, guard;
+    DevBuf<unsigned> ctl;               // per-case control word
+    DevBuf<int> tally, counts, guard, rcount;  // k_control sums + finish counter, its result; guards; refining
     DevBuf<double> Ssp, V0, Vout;       // inputs / output in the caller's layouts
     DevBuf<double> Yg, Yb;              // per-case Ybus (N-1)
     DevBuf<unsigned char> pin;          // per-case pin mask (N-1)
     DevBuf<unsigned char> served;       // per-case served mask (N-1 outage sets)
     int s_stride = 0, v0_stride = 0;
     bool per_case_y = false, has_pin = false, uploaded = false, mask_unserved = false;
-    int* h_counts = nullptr;            // pinned host mirror of counts
-    bool pending_new = false;           // slots loaded since the last pass
+    int* h_counts = nullptr;            // pinned host mirror of counts (stage counters, C ABI order)
+    int* h_rcount = nullptr;            // pinned host mirror of rcount
+    bool pending_start = false;         // cases in START since the last pass
     DevBuf<double> st_S, st_V, st_out;  // staging of slot loads / fetches
     DevBuf<int> st_slots, st_it;
     DevBuf<unsigned char> st_cv;
@@ -2017,6 +1993,7 @@ struct p3smi_batch {
     ~p3smi_batch() {
         drop_graphs();
         if (h_counts) (void)hipHostFree(h_counts);
+        if (h_rcount) (void)hipHostFree(h_rcount);
         for (auto& e : ev) {
             (void)hipEventDestroy(e.second.first);
             (void)hipEventDestroy(e.second.second);
@@ -2030,6 +2007,8 @@ struct p3smi_outages {
     DevBuf<int> f, t, gptr, gbr, ref, gbus, ypos;
     DevBuf<unsigned char> live;
     DevBuf<double> gz, yval;
+    DevBuf<int> dptr, dpos;  // per case, the Ybus deltas of its outaged branches in group order
+    DevBuf<double> dval;
     DevBuf<int> scratch, npinned;
     DevBuf<unsigned long long> zscratch;
     // the batch and solve stamp holding this set's last results on the device (one chunk)
@@ -2070,6 +2049,8 @@ void ensure_device(p3smi_topology* t) {
     HIPCHECK(hipStreamSynchronize(st));
     d.ready = true;
 }
+
+void ensure_tiles(p3smi_topology* t);
 
 // Level schedule for a launch shape (TL task lists, dealing group), built and uploaded on
 // first use.
@@ -2270,35 +2251,69 @@ void zero_row(T* p, size_t n, hipStream_t st) {
     HIPCHECK(hipGetLastError());
 }
 
-int eval_nparts(int n) { return (n + EVAL_WAVES * EVAL_BUSES_PER_WAVE - 1) / (EVAL_WAVES * EVAL_BUSES_PER_WAVE); }
+// Row tiles of k_evaluate: consecutive rows, at most `rows` of them and at most EV_NZ Ybus
+// nonzeros, so that a tile's CSR segment fits the kernel's LDS (a single row with more
+// nonzeros is a tile of its own and read from global memory).
+std::vector<int> eval_tiles(const p3smi::Topology& h, int rows) {
+    std::vector<int> ptr{0};
+    int r0 = 0;
+    for (int r = 0; r < h.n; ++r)
+        if (r > r0 && (r + 1 - r0 > rows || h.Yp[r + 1] - h.Yp[r0] > EV_NZ)) {
+            ptr.push_back(r);
+            r0 = r;
+        }
+    ptr.push_back(h.n);
+    return ptr;
+}
+
+void ensure_tiles(p3smi_topology* t) {
+    DevTopo& d = t->dev;
+    if (d.ntiles > 0 && d.tile_rows == t->eval_rows) return;
+    const std::vector<int> ptr = eval_tiles(t->host, t->eval_rows);
+    d.tiles.upload(ptr.data(), ptr.size(), t->stream);
+    HIPCHECK(hipStreamSynchronize(t->stream));
+    d.ntiles = (int)ptr.size() - 1;
+    d.tile_rows = t->eval_rows;
+}
+
+// rows per tile the evaluation may use at most (its partial maxima are sized for the most tiles)
+int max_tiles(const p3smi::Topology& h) { return (int)eval_tiles(h, 1).size() - 1; }
 
 void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     ensure_device(t);
+    ensure_tiles(t);
     const p3smi::Topology& h = t->host;
     b->topo = t;
     b->B = cases;
     b->S = (cases + 63) / 64 * 64;
     const size_t S = b->S;
-    b->nparts = eval_nparts(h.n);
-    for (auto* v : {&b->Vm, &b->Va, &b->cs, &b->sn, &b->Vm0, &b->Va0}) v->alloc((size_t)h.n * S);
+    for (auto* v : {&b->Vm, &b->Va, &b->Ve, &b->Vf, &b->Vm0, &b->Va0}) v->alloc((size_t)h.n * S);
     b->Jx.alloc((size_t)h.nnzJ * S);
     b->fac.alloc(((size_t)h.nnz_lu + 1) * S);  // + the zero row the 2x2-block steps read
     zero_row(b->fac.p + (size_t)h.nnz_lu * S, S, t->stream);
     b->R.alloc((size_t)h.m * S);
     b->x.alloc((size_t)h.m * S);
     for (auto* v : {&b->nrm, &b->alpha, &b->best_alpha, &b->best_nrm}) v->alloc(S);
-    b->part.alloc((size_t)b->nparts * S);
+    b->part.alloc((size_t)t->dev.ntiles * S);
     b->pstat.alloc(2 * S);
-    for (auto* v : {&b->status, &b->iters, &b->trial}) v->alloc(S);
+    b->ctl.alloc(S);
+    b->tally.alloc(NSTAT + 1);
     b->counts.alloc(NSTAT);
     b->guard.alloc(3);  // guard trips, FP32 and FP64 factorizations of the switching mode
-    b->vote.alloc(S);
-    b->rf.alloc(S);
+    b->rcount.alloc(1);
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, t->stream, b->tally.p, NSTAT + 1);
+    hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, t->stream, b->ctl.p, 0,
+                       (int)S, 0u);
+    HIPCHECK(hipGetLastError());
     b->mixed_ready = false;
     if (!b->h_counts) HIPCHECK(hipHostMalloc((void**)&b->h_counts, NSTAT * sizeof(int), hipHostMallocDefault));
+    if (!b->h_rcount) HIPCHECK(hipHostMalloc((void**)&b->h_rcount, sizeof(int), hipHostMallocDefault));
+    std::memset(b->h_counts, 0, NSTAT * sizeof(int));
+    b->h_counts[CNT_IDLE] = (int)S;
+    HIPCHECK(hipStreamSynchronize(t->stream));
 }
 
-// kernel timing (optional): kinds 0 eval, 1 lu, 2 step, 3 decide, 4 other
+// kernel timing (optional): kinds 0 eval, 1 lu, 2 step, 3 decide, 4 other, 5 refine
 struct KTimer {
     p3smi_batch* b;
     int kind;
@@ -2341,23 +2356,39 @@ void collect_timing(p3smi_batch* b) {
 
 dim3 grid_items(int items, int S) { return dim3((unsigned)(S / 64), (unsigned)((items + 3) / 4)); }
 
-void launch_eval(p3smi_batch* b, int run_mask, int j_mask) {
-    const p3smi::Topology& h = b->topo->host;
-    const DevTopo& d = b->topo->dev;
+void launch_evaluate(p3smi_batch* b) {
+    p3smi_topology* t = b->topo;
+    const p3smi::Topology& h = t->host;
+    const DevTopo& d = t->dev;
+    if (b->part.n < (size_t)d.ntiles * b->S) b->part.alloc((size_t)d.ntiles * b->S);
     KTimer tm(b, 0);
-    dim3 grid((unsigned)b->nparts, (unsigned)(b->S / 64));
-    const double* G = b->per_case_y ? b->Yg.p : d.G.p;
-    const double* Bv = b->per_case_y ? b->Yb.p : d.B.p;
-    auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, grid, dim3(64 * EVAL_WAVES), 0, b->topo->stream, d.Yp.p, d.Yj.p, d.Ydiag.p, G, Bv,
-                           b->Vm.p, b->cs.p, b->sn.p, b->Ssp.p, b->s_stride, d.row_p.p, d.row_q.p,
-                           b->has_pin ? b->pin.p : nullptr, b->status.p, run_mask, j_mask, d.posJ.p, b->Jx.p, b->R.p,
-                           b->part.p, h.n, b->S);
-    };
-    if (b->per_case_y && b->has_pin) go(k_eval<true, true>);
-    else if (b->per_case_y) go(k_eval<true, false>);
-    else if (b->has_pin) go(k_eval<false, true>);
-    else go(k_eval<false, false>);
+    EvalArgs a;
+    a.tile_ptr = d.tiles.p;
+    a.Yp = d.Yp.p;
+    a.Yj = d.Yj.p;
+    a.Ydiag = d.Ydiag.p;
+    a.posJ = reinterpret_cast<const int4*>(d.posJ.p);
+    a.G = b->per_case_y ? b->Yg.p : d.G.p;
+    a.B = b->per_case_y ? b->Yb.p : d.B.p;
+    a.Ve = b->Ve.p;
+    a.Vf = b->Vf.p;
+    a.Ssp = b->Ssp.p;
+    a.s_stride = b->s_stride;
+    a.row_p = d.row_p.p;
+    a.row_q = d.row_q.p;
+    a.pin = b->has_pin ? b->pin.p : nullptr;
+    a.ctl = b->ctl.p;
+    a.J = b->Jx.p;
+    a.R = b->R.p;
+    a.part = b->part.p;
+    a.n = h.n;
+    a.S = b->S;
+    const dim3 grid((unsigned)d.ntiles, (unsigned)(b->S / 64));
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(64 * EV_WAVES), 0, t->stream, a); };
+    if (b->per_case_y && b->has_pin) go(k_evaluate<true, true>);
+    else if (b->per_case_y) go(k_evaluate<true, false>);
+    else if (b->has_pin) go(k_evaluate<false, true>);
+    else go(k_evaluate<false, false>);
     HIPCHECK(hipGetLastError());
 }
 
@@ -2365,25 +2396,20 @@ template <int G, int NS, typename real, bool FACTOR>
 void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* rhs, double* xout, int sel) {
     const p3smi::Topology& h = b->topo->host;
     const double rcond_min = REUSE_RCOND_RATIO * h.rcond0;
-    auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3((unsigned)(b->S / G)), dim3(64 * NS), 0, b->topo->stream, ds.ptrA.p,
-                           ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.diagB.p, ds.pairs.p, ds.ptrF.p, ds.fwd.p, ds.fent.p,
-                           ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout, b->status.p, b->iters.p,
-                           b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min, ds.nlev, ds.nblev, b->S);
-    };
     if constexpr (G == 64) {
         if (ds.blk) {
             hipLaunchKernelGGL((k_lu_blk<NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
                                b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.steps.p, ds.ptrF.p,
                                ds.fwd.p, ds.fsteps.p, ds.ptrW.p, ds.bwd.p, ds.bsteps.p, b->Jx.p, fac, rhs, xout,
-                               b->status.p, b->iters.p, b->pstat.p, b->guard.p, b->vote.p, b->rf.p, sel, rcond_min,
-                               ds.nlev, ds.nblev, b->S);
+                               b->ctl.p, b->pstat.p, b->guard.p, sel, rcond_min, ds.nlev, ds.nblev, b->S);
             return;
         }
     }
     if (ds.blk) throw Error(P3SMI_E_INTERNAL, "2x2-block schedule launched with G != 64");
-    if constexpr (G >= 16) go(k_lu_stream<G, NS, real, FACTOR>);
-    else go(k_lu<G, NS, real, FACTOR>);
+    hipLaunchKernelGGL((k_lu_stream<G, NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
+                       b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.diagB.p, ds.pairs.p,
+                       ds.ptrF.p, ds.fwd.p, ds.fent.p, ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout,
+                       b->ctl.p, b->pstat.p, b->guard.p, sel, rcond_min, ds.nlev, ds.nblev, b->S);
 }
 
 bool valid_lu_config(LuConfig c) {
@@ -2392,11 +2418,15 @@ bool valid_lu_config(LuConfig c) {
     return g && n;
 }
 
+LuConfig lu_config_for(const p3smi_topology* t, int n_active) {
+    return n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
+}
+
 // One LU launch over the selected cases; n_active picks the batch or the few-cases shape.
 template <typename real, bool FACTOR>
 void launch_lu(p3smi_batch* b, int n_active, real* fac, const double* rhs, double* xout, int sel) {
     p3smi_topology* t = b->topo;
-    LuConfig cfg = n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
+    const LuConfig cfg = lu_config_for(t, n_active);
     const DevSched& ds = get_sched(t, cfg);
     KTimer tm(b, 1);
 #define P3SMI_LU_CASE(GG, NN)                                                                \
@@ -2426,17 +2456,10 @@ void ensure_mixed(p3smi_batch* b) {
     b->mixed_ready = true;
 }
 
-const int* read_counts(p3smi_batch* b) {
-    hipStream_t st = b->topo->stream;
-    HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    return b->h_counts;
-}
-
-// Newton step of the ACTIVE cases. FP64: one factor+solve launch. Mixed: FP32 factors and
-// solve, then refinement x += LU32^-1 (R - J x) with the residual in FP64 until the
-// correction is below rtol |x|; cases whose FP32 pivots fail or whose refinement stalls
-// are factored in FP64 in the same iteration.
+// Newton step of the cases in STEP. FP64: one factor+solve launch. Mixed: FP32 factors and
+// solve, then refinement x += LU32^-1 (R - J x) with the residual in FP64 until the correction
+// is below rtol |x|; cases whose FP32 pivots fail or whose refinement stalls are factored in
+// FP64 in the same iteration.
 void linear_solve(p3smi_batch* b, int n_active, int precision) {
     p3smi_topology* t = b->topo;
     const p3smi::Topology& h = t->host;
@@ -2447,14 +2470,14 @@ void linear_solve(p3smi_batch* b, int n_active, int precision) {
         return;
     }
     if (precision == P3SMI_PREC_MIXED_SWITCH) {
-        // FP32 factors and solve for the 64-case groups voting FP32 alone, no refinement; FP64
-        // for the others and for the cases whose FP32 pivots failed (vote -1), in the same
-        // iteration. The factorization counts are kept on the device (guard[1], guard[2]).
+        // FP32 factors and solve for the 64-case groups of FP32 cases alone, no refinement; FP64
+        // for the others and for the cases whose FP32 pivots failed, in the same iteration. The
+        // factorization counts are kept on the device (guard[1], guard[2]).
         ensure_mixed(b);
         {
             KTimer tm(b, 4);
-            hipLaunchKernelGGL(k_group_vote, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p,
-                               b->vote.p, b->guard.p, b->S);
+            hipLaunchKernelGGL(k_group_vote, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->ctl.p,
+                               b->guard.p, b->S);
             HIPCHECK(hipGetLastError());
         }
         launch_lu<float, true>(b, n_active, b->fac32.p, b->R.p, b->x.p, SEL_F32);
@@ -2466,98 +2489,101 @@ void linear_solve(p3smi_batch* b, int n_active, int precision) {
     const unsigned gS = (unsigned)((b->S + 255) / 256);
     {
         KTimer tm(b, 4);
-        hipLaunchKernelGGL(k_refine_init, dim3(gS), dim3(256), 0, st, b->status.p, b->vote.p, b->rf.p, b->prevd.p, b->S);
+        hipLaunchKernelGGL(k_refine_init, dim3(gS), dim3(256), 0, st, b->ctl.p, b->prevd.p, b->S);
     }
     launch_lu<float, true>(b, n_active, b->fac32.p, b->R.p, b->x.p, SEL_F32);
     for (int k = 1; k <= t->mixed_kmax; ++k) {
         {
             KTimer tm(b, 5);
             hipLaunchKernelGGL(k_resid, dim3((unsigned)(b->S / 64), (unsigned)((h.m + 3) / 4)), dim3(256), 0, st,
-                               t->dev.Jp.p, t->dev.xcol.p, b->Jx.p, b->R.p, b->x.p, b->rr.p, b->status.p, b->vote.p,
-                               b->rf.p, h.m, b->S);
+                               t->dev.Jp.p, t->dev.xcol.p, b->Jx.p, b->R.p, b->x.p, b->rr.p, b->ctl.p, h.m, b->S);
         }
         launch_lu<float, false>(b, n_active, b->fac32.p, b->rr.p, b->dvec.p, SEL_REFINE);
         {
             KTimer tm(b, 5);
             hipLaunchKernelGGL(k_refine_update, dim3((unsigned)b->nparts_r, (unsigned)(b->S / 64)), dim3(256), 0, st,
-                               b->x.p, b->dvec.p, b->rpart.p, b->status.p, b->vote.p, b->rf.p, h.m, b->S);
-            hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->counts.p, NSTAT);
+                               b->x.p, b->dvec.p, b->rpart.p, b->ctl.p, h.m, b->S);
+            hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->rcount.p, 1);
             hipLaunchKernelGGL(k_refine_decide, dim3(gS), dim3(256), 0, st, b->rpart.p, b->nparts_r, b->prevd.p,
-                               b->status.p, b->vote.p, b->rf.p, b->counts.p, t->mixed_rtol, (int)(k == t->mixed_kmax),
-                               b->S);
+                               b->ctl.p, b->rcount.p, t->mixed_rtol, (int)(k == t->mixed_kmax), b->S);
             HIPCHECK(hipGetLastError());
         }
         b->stats.refinement_steps += 1;
-        if (read_counts(b)[0] == 0) break;
+        HIPCHECK(hipMemcpyAsync(b->h_rcount, b->rcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (*b->h_rcount == 0) break;
     }
     launch_lu<double, true>(b, n_active, b->fac.p, b->R.p, b->x.p, SEL_F64);
 }
 
-void launch_step(p3smi_batch* b, int mask, int full) {
-    const p3smi::Topology& h = b->topo->host;
-    const DevTopo& d = b->topo->dev;
-    KTimer tm(b, 2);
-    hipLaunchKernelGGL(k_step, grid_items(h.n, b->S), dim3(256), 0, b->topo->stream, b->x.p, d.xva.p, d.xvm.p,
-                       b->status.p, mask, full, b->alpha.p, b->Vm.p, b->Va.p, b->cs.p, b->sn.p, b->Vm0.p, b->Va0.p,
-                       h.n, b->S);
-    HIPCHECK(hipGetLastError());
-}
-
-const int* launch_decide(p3smi_batch* b, double tol, int ls, int phase, int max_iter) {
-    KTimer tm(b, 3);
-    hipStream_t st = b->topo->stream;
-    const double sw = b->precision == P3SMI_PREC_MIXED_SWITCH ? b->topo->mixed_switch : 0.0;
-    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->counts.p, NSTAT);
-    hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->part.p, b->nparts,
-                       b->nrm.p, b->alpha.p, b->best_alpha.p, b->best_nrm.p, b->trial.p, b->status.p, b->iters.p,
-                       b->counts.p, b->vote.p, tol, ls, phase, max_iter, sw, b->S);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    return b->h_counts;
-}
-
 void check_solve_args(int max_iter, double tol, int precision) {
     if (max_iter <= 0) throw Error(P3SMI_E_ARG, "max_iter must be > 0");
+    if (max_iter > MAX_ITER_LIMIT) throw Error(P3SMI_E_ARG, "max_iter must be <= 65535");
     if (!(tol > 0.0)) throw Error(P3SMI_E_ARG, "tol must be > 0");
     if (precision != P3SMI_PREC_DOUBLE && precision != P3SMI_PREC_MIXED && precision != P3SMI_PREC_MIXED_SWITCH)
         throw Error(P3SMI_E_ARG, "precision must be 0 (double), 1 (mixed) or 2 (mixed_switch)");
 }
 
-// Start point of the NEW cases: polar state, mismatch and Jacobian, classification.
-const int* start_new(p3smi_batch* b, double tol, int line_search, int precision, int max_iter) {
-    const p3smi::Topology& h = b->topo->host;
-    {
+// precision class a case starts in
+unsigned start_class(int precision) { return precision == P3SMI_PREC_MIXED ? PR_F32 : PR_F64; }
+
+struct PassParams {
+    int max_iter, line_search, precision;
+    double tol;
+};
+
+// The kernels of one pass (see the file header); the LU only when cases are in STEP, k_begin
+// only when cases are in START.
+void pass_kernels(p3smi_batch* b, const PassParams& p, int n_solve, bool starting) {
+    p3smi_topology* t = b->topo;
+    const p3smi::Topology& h = t->host;
+    hipStream_t st = t->stream;
+    if (n_solve > 0) linear_solve(b, n_solve, p.precision);
+    if (starting) {
         KTimer tm(b, 4);
-        hipLaunchKernelGGL(k_init, grid_items(h.n, b->S), dim3(256), 0, b->topo->stream, b->V0.p, b->v0_stride,
-                           b->Vm.p, b->Va.p, b->cs.p, b->sn.p, b->status.p, b->vote.p,
-                           precision == P3SMI_PREC_DOUBLE ? 0 : 1, h.n, b->S);
+        hipLaunchKernelGGL(k_begin, grid_items(h.n, b->S), dim3(256), 0, st, b->V0.p, b->v0_stride, b->ctl.p,
+                           start_class(p.precision), b->Vm.p, b->Va.p, b->Ve.p, b->Vf.p, h.n, b->S);
         HIPCHECK(hipGetLastError());
     }
-    launch_eval(b, 1 << ST_NEW, 1 << ST_NEW);
-    return launch_decide(b, tol, line_search, 0, max_iter);
-}
-
-// ---- hipGraph replay of Newton iterations ---------------------------------------------
-// The kernels of one iteration depend only on device state (status words steer them), so an
-// iteration is recorded once per (LU launch shape, solve parameters, buffers) and replayed
-// with one graph launch and one synchronisation per phase: [LU, full step, evaluation,
-// decide, counts to the host], then [trial step, evaluation, decide, counts] while cases
-// backtrack, then [step, evaluation with Jacobian, decide, counts] at accepted trial points.
-// The status counters are cleared by a kernel (no memset node is recorded); schedules are
-// uploaded before recording. Kernel timers are not recorded: graphs run with timing off.
-void record_decide(p3smi_batch* b, double tol, int ls, int phase, int max_iter) {
-    hipStream_t st = b->topo->stream;
-    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->counts.p, NSTAT);
-    hipLaunchKernelGGL(k_decide, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->part.p, b->nparts,
-                       b->nrm.p, b->alpha.p, b->best_alpha.p, b->best_nrm.p, b->trial.p, b->status.p, b->iters.p,
-                       b->counts.p, b->vote.p, tol, ls, phase, max_iter, 0.0, b->S);
-    HIPCHECK(hipGetLastError());
+    if (n_solve > 0 || b->h_counts[CNT_TRIAL] > 0 || b->h_counts[CNT_COMMIT] > 0) {
+        KTimer tm(b, 2);
+        hipLaunchKernelGGL(k_advance, grid_items(h.n, b->S), dim3(256), 0, st, b->x.p, t->dev.xva.p, t->dev.xvm.p,
+                           b->ctl.p, b->alpha.p, b->Vm.p, b->Va.p, b->Ve.p, b->Vf.p, b->Vm0.p, b->Va0.p, h.n, b->S);
+        HIPCHECK(hipGetLastError());
+    }
+    launch_evaluate(b);
+    {
+        KTimer tm(b, 3);
+        CtlArgs a;
+        a.part = b->part.p;
+        a.nparts = t->dev.ntiles;
+        a.ctl = b->ctl.p;
+        a.nrm = b->nrm.p;
+        a.alpha = b->alpha.p;
+        a.best_alpha = b->best_alpha.p;
+        a.best_nrm = b->best_nrm.p;
+        a.tally = b->tally.p;
+        a.counts_out = b->counts.p;
+        a.tol = p.tol;
+        a.sw = p.precision == P3SMI_PREC_MIXED_SWITCH ? t->mixed_switch : 0.0;
+        a.line_search = p.line_search;
+        a.max_iter = p.max_iter;
+        a.S = b->S;
+        hipLaunchKernelGGL(k_control, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, a);
+        HIPCHECK(hipGetLastError());
+    }
     HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
 }
 
+// ---- hipGraph replay of passes ---------------------------------------------------------
+// A pass depends only on device state (the control words steer it), so it is recorded once per
+// (LU launch shape, whether it factors and whether it starts cases, solve parameters, buffers)
+// and replayed with one graph launch and one synchronisation. The stage counters reset
+// themselves in k_control (no memset node is recorded); schedules are uploaded before recording.
+// Kernel timers are not recorded: graphs run with timing off. FP64 solves only (the mixed
+// modes read refinement counters back between their launches).
 template <typename F>
-const int* replay(p3smi_batch* b, const std::string& key, F&& record) {
+void replay(p3smi_batch* b, const std::string& key, F&& record) {
     hipStream_t st = b->topo->stream;
     auto it = b->gcache.find(key);
     if (it == b->gcache.end()) {
@@ -2581,95 +2607,80 @@ const int* replay(p3smi_batch* b, const std::string& key, F&& record) {
         it = b->gcache.emplace(key, std::make_pair(g, x)).first;
     }
     HIPCHECK(hipGraphLaunch(it->second.second, st));
+}
+
+int live_cases(const int* cnt) { return cnt[CNT_START] + cnt[CNT_ACTIVE] + cnt[CNT_TRIAL] + cnt[CNT_COMMIT]; }
+
+// One pass; returns the stage counters after it (C ABI order).
+const int* run_pass(p3smi_batch* b, const PassParams& p) {
+    p3smi_topology* t = b->topo;
+    hipStream_t st = t->stream;
+    const int n_solve = b->h_counts[CNT_ACTIVE];
+    const bool starting = b->pending_start;
+    if (n_solve > 0) b->stats.newton_passes++;
+    if (b->h_counts[CNT_TRIAL] > 0) b->stats.line_search_passes++;
+    if (b->part.n < (size_t)t->dev.ntiles * b->S) b->part.alloc((size_t)t->dev.ntiles * b->S);  // never under capture
+    if (b->graphs && !b->timing && p.precision == P3SMI_PREC_DOUBLE) {
+        const LuConfig cfg = lu_config_for(t, n_solve);
+        if (n_solve > 0) {
+            (void)get_sched(t, cfg);  // upload before recording
+            b->stats.double_factorizations += n_solve;
+        }
+        char key[360];
+        std::snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%d/%d/%p/%d/%p/%p/%p/%p/%d/%d/%d/%p", n_solve > 0 ? cfg.key() : -1,
+                      cfg.G, cfg.NS, (int)starting, (int)(b->h_counts[CNT_TRIAL] + b->h_counts[CNT_COMMIT] > 0), p.tol,
+                      p.max_iter, p.line_search, (const void*)b->Ssp.p, b->s_stride, (const void*)b->Yg.p,
+                      (const void*)b->Yb.p, (const void*)b->pin.p, (const void*)b->V0.p, b->v0_stride,
+                      (int)b->per_case_y * 2 + (int)b->has_pin, t->dev.tile_rows, (const void*)b->part.p);
+        replay(b, key, [&] {
+            PassParams q = p;
+            const int64_t before = b->stats.double_factorizations;
+            pass_kernels(b, q, n_solve, starting);
+            b->stats.double_factorizations = before;  // counted above, once per replay
+        });
+    } else {
+        pass_kernels(b, p, n_solve, starting);
+    }
     HIPCHECK(hipStreamSynchronize(st));
+    b->pending_start = false;
     return b->h_counts;
 }
 
-const int* newton_iteration_graph(p3smi_batch* b, int n_active, double tol, int line_search, int max_iter) {
-    p3smi_topology* t = b->topo;
-    const int A = 1 << ST_ACTIVE, BT = 1 << ST_BACKTRACK, NJ = 1 << ST_NEEDJ;
-    const LuConfig cfg = n_active <= t->lu_few_max ? t->lu_few : t->lu_batch;
-    (void)get_sched(t, cfg);  // upload before recording
-    char key[320];
-    std::snprintf(key, sizeof key, "%d/%d/%d/%a/%d/%p/%d/%p/%p/%p/%d/%d", cfg.key(), cfg.G, cfg.NS, tol, max_iter,
-                  (const void*)b->Ssp.p, b->s_stride, (const void*)b->Yg.p, (const void*)b->Yb.p,
-                  (const void*)b->pin.p, (int)b->per_case_y, (int)b->has_pin);
-    const std::string base(key);
-    b->stats.newton_passes++;
-    b->stats.double_factorizations += n_active;
-    const int* cnt = replay(b, "it" + base + "/" + std::to_string(line_search), [&] {
-        launch_lu<double, true>(b, n_active, b->fac.p, b->R.p, b->x.p, SEL_ALL);
-        launch_step(b, A, 1);
-        launch_eval(b, A, A);
-        record_decide(b, tol, line_search, 1, max_iter);
-    });
-    while (cnt[ST_BACKTRACK] > 0) {
-        b->stats.line_search_passes++;
-        cnt = replay(b, "bt" + base, [&] {
-            launch_step(b, BT, 0);
-            launch_eval(b, BT, 0);
-            record_decide(b, tol, line_search, 2, max_iter);
-        });
-    }
-    if (cnt[ST_NEEDJ] > 0) {
-        cnt = replay(b, "nj" + base, [&] {
-            launch_step(b, NJ, 0);
-            launch_eval(b, NJ, NJ);
-            record_decide(b, tol, line_search, 3, max_iter);
-        });
-    }
-    return cnt;
-}
-
-// One Newton iteration of every ACTIVE case (p3s/cpp/nr_klu.cpp:595-708): linear solve,
-// full step scored by a full evaluation, backtracking passes while any case backtracks,
-// Jacobian rebuilt at accepted trial points.
-const int* newton_iteration(p3smi_batch* b, int n_active, double tol, int line_search, int precision, int max_iter) {
-    const int A = 1 << ST_ACTIVE, BT = 1 << ST_BACKTRACK, NJ = 1 << ST_NEEDJ;
-    if (b->graphs && !b->timing && precision == P3SMI_PREC_DOUBLE)
-        return newton_iteration_graph(b, n_active, tol, line_search, max_iter);
-    b->stats.newton_passes++;
-    linear_solve(b, n_active, precision);
-    launch_step(b, A, 1);
-    launch_eval(b, A, A);
-    const int* cnt = launch_decide(b, tol, line_search, 1, max_iter);
-    while (cnt[ST_BACKTRACK] > 0) {
-        b->stats.line_search_passes++;
-        launch_step(b, BT, 0);
-        launch_eval(b, BT, 0);
-        cnt = launch_decide(b, tol, line_search, 2, max_iter);
-    }
-    if (cnt[ST_NEEDJ] > 0) {
-        launch_step(b, NJ, 0);
-        launch_eval(b, NJ, NJ);
-        cnt = launch_decide(b, tol, line_search, 3, max_iter);
-    }
-    return cnt;
+void reset_cases(p3smi_batch* b, int precision) {
+    hipStream_t st = b->topo->stream;
+    hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->ctl.p, b->B, b->S,
+                       start_class(precision));
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 3);
+    HIPCHECK(hipGetLastError());
+    std::memset(b->h_counts, 0, NSTAT * sizeof(int));
+    b->h_counts[CNT_START] = b->B;
+    b->h_counts[CNT_IDLE] = b->S - b->B;
+    b->pending_start = true;
 }
 
 void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int precision) {
     if (!b->uploaded) throw Error(P3SMI_E_ARG, "batch has no inputs (upload first)");
     check_solve_args(max_iter, tol, precision);
     hipStream_t st = b->topo->stream;
+    ensure_tiles(b->topo);
     std::memset(&b->stats, 0, sizeof b->stats);
     b->precision = precision;
     b->solve_id = ++g_solve_ids;
     b->stats.cases = b->B;
     {
         KTimer tm(b, 4);
-        hipLaunchKernelGGL(k_status_init, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p, b->B,
-                           b->S);
-        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 3);
-        hipLaunchKernelGGL(k_clear_int, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->iters.p, b->S);
-        HIPCHECK(hipGetLastError());
+        reset_cases(b, precision);
     }
-    const int* cnt = start_new(b, tol, line_search, precision, max_iter);
-    int it = 0;
-    while (it < max_iter && cnt[ST_ACTIVE] > 0) {
-        ++it;
-        cnt = newton_iteration(b, cnt[ST_ACTIVE], tol, line_search, precision, max_iter);
+    const PassParams p{max_iter, line_search ? 1 : 0, precision, tol};
+    // a case needs at most 1 + max_iter * (1 + LS_MAX_TRIALS + 1) passes
+    const long long cap = 1 + (long long)max_iter * (LS_MAX_TRIALS + 2) + 2;
+    const int* cnt = b->h_counts;
+    long long passes = 0;
+    while (live_cases(cnt) > 0) {
+        if (++passes > cap) throw Error(P3SMI_E_INTERNAL, "Newton passes did not finish every case");
+        cnt = run_pass(b, p);
     }
-    b->stats.failed_factorizations = cnt[ST_FAILED];
+    b->stats.failed_factorizations = cnt[CNT_FAIL];
     int g[3];
     HIPCHECK(hipMemcpyAsync(g, b->guard.p, sizeof g, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -2687,18 +2698,17 @@ void download(p3smi_batch* b, double* V_out, size_t ld, size_t c0, int32_t* iter
     const p3smi::Topology& h = b->topo->host;
     hipStream_t st = b->topo->stream;
     b->Vout.alloc((size_t)2 * h.n * b->B);
-    hipLaunchKernelGGL(k_finalize, grid_items(h.n, b->S), dim3(256), 0, st, b->Vm.p, b->cs.p, b->sn.p,
+    hipLaunchKernelGGL(k_emit, grid_items(h.n, b->S), dim3(256), 0, st, b->Ve.p, b->Vf.p,
                        b->mask_unserved ? b->served.p : nullptr, b->Vout.p, h.n, b->S, b->B);
     HIPCHECK(hipGetLastError());
-    std::vector<int> status(b->B), it(b->B);
-    HIPCHECK(hipMemcpyAsync(status.data(), b->status.p, b->B * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(it.data(), b->iters.p, b->B * sizeof(int), hipMemcpyDeviceToHost, st));
+    std::vector<unsigned> ctl(b->B);
+    HIPCHECK(hipMemcpyAsync(ctl.data(), b->ctl.p, b->B * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     if (mismatch) HIPCHECK(hipMemcpyAsync(mismatch, b->nrm.p, b->B * sizeof(double), hipMemcpyDeviceToHost, st));
     if (V_out) download_cols(b->stage, V_out, b->Vout.p, h.n, ld, c0, b->B, 16, st);
     HIPCHECK(hipStreamSynchronize(st));
     for (int c = 0; c < b->B; ++c) {
-        if (iterations) iterations[c] = it[c];
-        if (converged) converged[c] = status[c] == ST_CONVERGED;
+        if (iterations) iterations[c] = iters_of(ctl[c]);
+        if (converged) converged[c] = stage_of(ctl[c]) == SG_CONV;
     }
 }
 
@@ -2718,7 +2728,7 @@ void check_v0(const double* V0, size_t rows, size_t cols, const char* who) {
 
 size_t bytes_per_case(const p3smi::Topology& h, bool per_case_y, bool pin, bool per_case_s) {
     size_t b = (size_t)h.n * 6 * 8 + (size_t)h.nnzJ * 8 + ((size_t)h.nnz_lu + 1) * 8 + (size_t)h.m * 16;
-    b += (size_t)eval_nparts(h.n) * 8 + 64;
+    b += (size_t)max_tiles(h) * 8 + 64;
     b += (size_t)h.n * 16 * 2;  // V0 / V out
     if (per_case_s) b += (size_t)h.n * 16;
     if (per_case_y) b += (size_t)h.nnzY * 16 * 2;  // G, B and the upload staging
@@ -2823,7 +2833,7 @@ int p3smi_topology_info(const p3smi_topology* t, p3smi_info* o) {
         o->nnz_lu = h.nnz_lu; o->n_pairs = h.n_pairs; o->n_levels = h.sched.nlev; o->n_blevels = h.sched.nblev;
         o->n_slots = h.sched.nslots; o->ordering = h.ordering; o->rcond0 = h.rcond0;
         o->lu_g = t->lu_batch.G; o->lu_ns = t->lu_batch.NS; o->lu_few_g = t->lu_few.G; o->lu_few_ns = t->lu_few.NS;
-        o->lu_few_max = t->lu_few_max; o->lu_group = t->lu_batch.grp;
+        o->lu_few_max = t->lu_few_max; o->lu_group = t->lu_batch.grp; o->eval_rows = t->eval_rows;
     });
 }
 
@@ -2867,6 +2877,15 @@ int p3smi_topology_set_lu_shape(p3smi_topology* t, int32_t g, int32_t ns, int32_
         t->lu_batch = a;
         t->lu_few = f;
         t->lu_few_max = few_max;
+    });
+}
+
+int p3smi_topology_set_eval_tile(p3smi_topology* t, int32_t rows) {
+    return guarded([&] {
+        if (!t) throw Error(P3SMI_E_ARG, "set_eval_tile: null topology");
+        if (rows < 1 || rows > 1024) throw Error(P3SMI_E_ARG, "set_eval_tile: rows must be in [1, 1024]");
+        std::lock_guard<std::mutex> lk(t->mu);
+        t->eval_rows = rows;
     });
 }
 
@@ -3077,7 +3096,7 @@ int p3smi_batch_vrange(p3smi_batch* b, double* vmin, double* vmax) {
         DevBuf<double> lo, hi;
         lo.alloc(b->B);
         hi.alloc(b->B);
-        hipLaunchKernelGGL(k_vrange, dim3((unsigned)((b->B + 255) / 256)), dim3(256), 0, st, b->Vm.p,
+        hipLaunchKernelGGL(k_vrange, dim3((unsigned)((b->B + 255) / 256)), dim3(256), 0, st, b->Ve.p, b->Vf.p,
                            b->mask_unserved ? b->served.p : nullptr, lo.p, hi.p, h.n, b->S, b->B);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(vmin, lo.p, b->B * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -3131,8 +3150,8 @@ int p3smi_batch_set_timing(p3smi_batch* b, int32_t on) {
 int p3smi_batch_bytes_per_case(const p3smi_batch* b, double* out4) {
     const p3smi::Topology& h = b->topo->host;
     const double n = h.n, m = h.m, nnzJ = h.nnzJ, lu = (double)h.nnz_lu;
-    // eval: Vm, cs, sn (24 n) + S (16 n) + J write (8 nnzJ) + R write (8 m) [+ per-case Y 16 nnzY]
-    out4[0] = 24 * n + (b->s_stride ? 16 * n : 0) + 8 * nnzJ + 8 * m + (b->per_case_y ? 16.0 * h.nnzY : 0.0);
+    // evaluation: Ve, Vf (16 n) + S (16 n) + J write (8 nnzJ) + R write (8 m) [+ per-case Y 16 nnzY]
+    out4[0] = 16 * n + (b->s_stride ? 16 * n : 0) + 8 * nnzJ + 8 * m + (b->per_case_y ? 16.0 * h.nnzY : 0.0);
     // lu: J read (8 nnzJ) + factor write (8 nnz_lu) + the substitutions' read of every factor
     // entry (8 nnz_lu: a 64-case group's factors, nnz_lu * 512 B, do not stay on chip between
     // the factor and solve phases) + R read (8 m) + x write/read (16 m)
@@ -3268,8 +3287,8 @@ int p3smi_solve_batch_contingency(p3smi_topology* t, const double* Yx_mag, const
             ang.alloc((size_t)h.nnzY * nc);
             upload_cols(b->stage, mag.p, Yx_mag, h.nnzY, L, c0, nc, 8, st);
             upload_cols(b->stage, ang.p, Yx_ang, h.nnzY, L, c0, nc, 8, st);
-            hipLaunchKernelGGL(k_y_polar, grid_items(h.nnzY, b->S), dim3(256), 0, st, mag.p, ang.p, b->Yg.p, b->Yb.p,
-                               h.nnzY, nc, nc, b->S);
+            hipLaunchKernelGGL(k_admittance_fill, grid_items(h.nnzY, b->S), dim3(256), 0, st, 1, mag.p, ang.p, nc, nc,
+                               b->Yg.p, b->Yb.p, h.nnzY, b->S);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipStreamSynchronize(st));
         };
@@ -3289,19 +3308,33 @@ int p3smi_solve_batch_contingency_stamp(p3smi_topology* t, const int32_t* stamp_
         for (int64_t e = 0; e < n_stamp; ++e)
             if (stamp_k[e] < 0 || stamp_k[e] >= h.nnzY || stamp_c[e] < 0 || stamp_c[e] >= L)
                 throw Error(P3SMI_E_ARG, "solve_batch_contingency_stamp: stamp index out of range");
-        DevBuf<int> dk, dc;
-        DevBuf<double> dv;
+        if (n_stamp > INT_MAX) throw Error(P3SMI_E_ARG, "solve_batch_contingency_stamp: too many stamp entries");
+        // the stamp as per-case delta lists (entries of a case in input order)
+        std::vector<int> dptr((size_t)L + 1, 0), dpos((size_t)std::max<int64_t>(n_stamp, 1));
+        std::vector<double> dval(2 * (size_t)std::max<int64_t>(n_stamp, 1));
+        for (int64_t e = 0; e < n_stamp; ++e) dptr[(size_t)stamp_c[e] + 1]++;
+        for (int c = 0; c < L; ++c) dptr[c + 1] += dptr[c];
+        {
+            std::vector<int> fill(dptr.begin(), dptr.end() - 1);
+            for (int64_t e = 0; e < n_stamp; ++e) {
+                const int q = fill[stamp_c[e]]++;
+                dpos[q] = stamp_k[e];
+                dval[2 * (size_t)q] = stamp_val[2 * e];
+                dval[2 * (size_t)q + 1] = stamp_val[2 * e + 1];
+            }
+        }
+        DevBuf<int> ddp, ddq;
+        DevBuf<double> ddv;
         ensure_device(t);
-        dk.upload(stamp_k, (size_t)n_stamp, t->stream);
-        dc.upload(stamp_c, (size_t)n_stamp, t->stream);
-        dv.upload(stamp_val, (size_t)2 * n_stamp, t->stream);
+        ddp.upload(dptr.data(), dptr.size(), t->stream);
+        ddq.upload(dpos.data(), dpos.size(), t->stream);
+        ddv.upload(dval.data(), dval.size(), t->stream);
         auto upy = [&](p3smi_batch* b, int c0, int nc) {
             hipStream_t st = t->stream;
-            hipLaunchKernelGGL(k_y_base, grid_items(h.nnzY, b->S), dim3(256), 0, st, t->dev.G.p, t->dev.B.p, b->Yg.p,
-                               b->Yb.p, h.nnzY, b->S);
-            if (n_stamp > 0)
-                hipLaunchKernelGGL(k_y_stamp, dim3((unsigned)((n_stamp + 255) / 256)), dim3(256), 0, st, dk.p, dc.p,
-                                   dv.p, b->Yg.p, b->Yb.p, (long long)n_stamp, c0, nc, b->S);
+            hipLaunchKernelGGL(k_admittance_fill, grid_items(h.nnzY, b->S), dim3(256), 0, st, 0, t->dev.G.p,
+                               t->dev.B.p, 0, 0, b->Yg.p, b->Yb.p, h.nnzY, b->S);
+            hipLaunchKernelGGL(k_admittance_delta, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, ddp.p, ddq.p,
+                               ddv.p, c0, nc, b->Yg.p, b->Yb.p, b->S);
             HIPCHECK(hipGetLastError());
         };
         contingency_common(t, S, V0, pin, L, upy, max_iter, tol, line_search, precision, V_out, iterations, converged,
@@ -3434,6 +3467,23 @@ int p3smi_outages_create(p3smi_topology* t, int32_t nbr, const int32_t* from, co
         o->ref.upload(ref, nref, st);
         o->gbus.upload(gb.data(), gb.size(), st);
         o->gz.upload(gzv.data(), gzv.size(), st);
+        // each case's Ybus deltas: the four stamp entries of every outaged branch, in group order
+        {
+            const int ng = n_cases ? gptr[n_cases] : 0;
+            std::vector<int> dptr((size_t)n_cases + 1), dpos(std::max(4 * ng, 1));
+            std::vector<double> dval(2 * (size_t)std::max(4 * ng, 1));
+            for (int c = 0; c <= n_cases; ++c) dptr[c] = 4 * gptr[c];
+            for (int q = 0; q < ng; ++q)
+                for (int j = 0; j < 4; ++j) {
+                    const int k = gbr[q];
+                    dpos[4 * q + j] = ypos[4 * k + j];
+                    dval[2 * (4 * (size_t)q + j)] = yval[2 * (4 * (size_t)k + j)];
+                    dval[2 * (4 * (size_t)q + j) + 1] = yval[2 * (4 * (size_t)k + j) + 1];
+                }
+            o->dptr.upload(dptr.data(), dptr.size(), st);
+            o->dpos.upload(dpos.data(), dpos.size(), st);
+            o->dval.upload(dval.data(), dval.size(), st);
+        }
         HIPCHECK(hipStreamSynchronize(st));
         *out = o.release();
     });
@@ -3524,10 +3574,10 @@ int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start
             run_islands(o, c0, nc, b->S, dvs.p, b->served.p, nullptr, b->pin.p, b->V0.p, o->npinned.p);
             b->Yg.alloc((size_t)h.nnzY * b->S);
             b->Yb.alloc((size_t)h.nnzY * b->S);
-            hipLaunchKernelGGL(k_y_base, grid_items(h.nnzY, b->S), dim3(256), 0, st, t->dev.G.p, t->dev.B.p, b->Yg.p,
-                               b->Yb.p, h.nnzY, b->S);
-            hipLaunchKernelGGL(k_case_y, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, o->gptr.p, o->gbr.p,
-                               o->ypos.p, o->yval.p, b->Yg.p, b->Yb.p, c0, nc, b->S);
+            hipLaunchKernelGGL(k_admittance_fill, grid_items(h.nnzY, b->S), dim3(256), 0, st, 0, t->dev.G.p,
+                               t->dev.B.p, 0, 0, b->Yg.p, b->Yb.p, h.nnzY, b->S);
+            hipLaunchKernelGGL(k_admittance_delta, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, o->dptr.p,
+                               o->dpos.p, o->dval.p, c0, nc, b->Yg.p, b->Yb.p, b->S);
             HIPCHECK(hipGetLastError());
             b->per_case_y = true;
             b->has_pin = true;
@@ -3584,7 +3634,7 @@ int p3smi_outages_branch_flows(p3smi_outages* o, const double* vn_from_kv, const
         const dim3 grid((unsigned)((L + 255) / 256), (unsigned)std::max(1, std::min(nbr, 65535)));
         if (nbr > 0)
             hipLaunchKernelGGL(k_branch_flows, grid, dim3(256), 0, st, o->f.p, o->t.p, o->yval.p, o->gptr.p, o->gbr.p,
-                               b->Vm.p, b->cs.p, b->sn.p, b->served.p, b->status.p, vnf.p, vnt.p, rat.p, tr.p, sn_mva,
+                               b->Ve.p, b->Vf.p, b->served.p, b->ctl.p, vnf.p, vnt.p, rat.p, tr.p, sn_mva,
                                nbr, L, b->S, full ? pfb.p : nullptr, full ? qfb.p : nullptr, full ? ptb.p : nullptr,
                                full ? qtb.p : nullptr, cur ? ifb.p : nullptr, cur ? itb.p : nullptr, ik.p, ld.p);
         HIPCHECK(hipGetLastError());
@@ -3655,7 +3705,7 @@ int p3smi_outages_voltage_violations(p3smi_outages* o, const double* vm_min, con
         HIPCHECK(hipMemsetAsync(cnt.p, 0, sizeof(int), st));
         ob.alloc(std::max(cap, 1)); oc.alloc(std::max(cap, 1)); ov.alloc(std::max(cap, 1)); ok.alloc(std::max(cap, 1));
         const dim3 grid((unsigned)((L + 255) / 256), (unsigned)std::min(h.n, 65535));
-        hipLaunchKernelGGL(k_vviol, grid, dim3(256), 0, st, b->Vm.p, b->cs.p, b->sn.p, b->served.p, b->status.p, lo.p,
+        hipLaunchKernelGGL(k_vviol, grid, dim3(256), 0, st, b->Ve.p, b->Vf.p, b->served.p, b->ctl.p, lo.p,
                            hi.p, ins.p, tol, h.n, L, b->S, cap, cnt.p, ob.p, oc.p, ov.p, ok.p);
         HIPCHECK(hipGetLastError());
         int total = 0;
@@ -3690,15 +3740,15 @@ int p3smi_batch_slots_init(p3smi_batch* b) {
         b->has_pin = false;
         b->mask_unserved = false;
         b->slot_mode = true;
-        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->status.p, ST_IDLE,
-                           b->S);
+        ensure_tiles(b->topo);
+        hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->ctl.p, 0, b->S, 0u);
         hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 3);
         HIPCHECK(hipGetLastError());
         std::memset(b->h_counts, 0, NSTAT * sizeof(int));
-        b->h_counts[ST_IDLE] = b->S;
+        b->h_counts[CNT_IDLE] = b->S;
         HIPCHECK(hipStreamSynchronize(st));
         b->uploaded = true;
-        b->pending_new = false;
+        b->pending_start = false;
     });
 }
 
@@ -3714,10 +3764,12 @@ int p3smi_batch_load(p3smi_batch* b, int32_t k, const int32_t* slots, const doub
         b->st_V.upload(V0, 2 * (size_t)h.n * k, st);
         b->st_slots.upload(slots, k, st);
         hipLaunchKernelGGL(k_load_slots, dim3((unsigned)((k + 63) / 64), (unsigned)((h.n + 3) / 4)), dim3(256), 0, st,
-                           b->st_S.p, b->st_V.p, b->st_slots.p, b->Ssp.p, b->V0.p, b->status.p, h.n, k, b->S);
+                           b->st_S.p, b->st_V.p, b->st_slots.p, b->Ssp.p, b->V0.p, b->ctl.p, h.n, k, b->S);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(st));
-        b->pending_new = true;
+        b->pending_start = true;
+        b->h_counts[CNT_START] += k;
+        b->h_counts[CNT_IDLE] -= k;
     });
 }
 
@@ -3728,11 +3780,8 @@ int p3smi_batch_pass(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_
         check_solve_args(max_iter, tol, precision);
         b->precision = precision;
         const int* cnt = b->h_counts;
-        if (b->pending_new) {
-            cnt = start_new(b, tol, line_search, precision, max_iter);
-            b->pending_new = false;
-        }
-        if (cnt[ST_ACTIVE] > 0) cnt = newton_iteration(b, cnt[ST_ACTIVE], tol, line_search, precision, max_iter);
+        if (live_cases(cnt) > 0 || b->pending_start)
+            cnt = run_pass(b, PassParams{max_iter, line_search ? 1 : 0, precision, tol});
         if (counts) std::copy(cnt, cnt + NSTAT, counts);
         collect_timing(b);
     });
@@ -3740,12 +3789,14 @@ int p3smi_batch_pass(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_
 
 int p3smi_batch_finished(p3smi_batch* b, int32_t* slots, int32_t* n) {
     return guarded([&] {
-        std::vector<int> status(b->B);
-        HIPCHECK(hipMemcpyAsync(status.data(), b->status.p, b->B * sizeof(int), hipMemcpyDeviceToHost, b->topo->stream));
+        std::vector<unsigned> ctl(b->B);
+        HIPCHECK(hipMemcpyAsync(ctl.data(), b->ctl.p, b->B * sizeof(unsigned), hipMemcpyDeviceToHost, b->topo->stream));
         HIPCHECK(hipStreamSynchronize(b->topo->stream));
         int k = 0;
-        for (int c = 0; c < b->B; ++c)
-            if (status[c] == ST_CONVERGED || status[c] == ST_FAILED || status[c] == ST_STOPPED) slots[k++] = c;
+        for (int c = 0; c < b->B; ++c) {
+            const unsigned sg = stage_of(ctl[c]);
+            if (sg == SG_CONV || sg == SG_FAIL || sg == SG_STOP) slots[k++] = c;
+        }
         *n = k;
     });
 }
@@ -3764,10 +3815,10 @@ int p3smi_batch_fetch(p3smi_batch* b, int32_t k, const int32_t* slots, double* V
         b->st_cv.alloc(k);
         b->st_mis.alloc(k);
         hipLaunchKernelGGL(k_fetch_slots, dim3((unsigned)((k + 63) / 64), (unsigned)((h.n + 3) / 4)), dim3(256), 0, st,
-                           b->st_slots.p, b->Vm.p, b->cs.p, b->sn.p, b->status.p, b->iters.p, b->nrm.p, b->st_out.p,
-                           b->st_it.p, b->st_cv.p, b->st_mis.p, h.n, k, b->S);
+                           b->st_slots.p, b->Ve.p, b->Vf.p, b->ctl.p, b->nrm.p, b->st_out.p, b->st_it.p, b->st_cv.p,
+                           b->st_mis.p, h.n, k, b->S);
         hipLaunchKernelGGL(k_release_slots, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, st, b->st_slots.p,
-                           b->status.p, k);
+                           b->ctl.p, k);
         HIPCHECK(hipGetLastError());
         if (V) HIPCHECK(hipMemcpyAsync(V, b->st_out.p, 2 * (size_t)h.n * k * sizeof(double), hipMemcpyDeviceToHost, st));
         if (iterations) HIPCHECK(hipMemcpyAsync(iterations, b->st_it.p, k * sizeof(int), hipMemcpyDeviceToHost, st));

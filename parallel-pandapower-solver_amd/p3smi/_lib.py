@@ -62,7 +62,7 @@ class Info(C.Structure):
         ("n_levels", C.c_int32), ("n_blevels", C.c_int32), ("n_slots", C.c_int32), ("ordering", C.c_int32),
         ("rcond0", C.c_double),
         ("lu_g", C.c_int32), ("lu_ns", C.c_int32), ("lu_few_g", C.c_int32), ("lu_few_ns", C.c_int32),
-        ("lu_few_max", C.c_int32), ("lu_group", C.c_int32),
+        ("lu_few_max", C.c_int32), ("lu_group", C.c_int32), ("eval_rows", C.c_int32),
     ]
 
 
@@ -82,7 +82,7 @@ class Stats(C.Structure):
 EXPORTS = [
     "p3smi_last_error", "p3smi_version", "p3smi_device_count", "p3smi_device_memory",
     "p3smi_topology_create", "p3smi_topology_free", "p3smi_topology_info", "p3smi_update_y",
-    "p3smi_topology_set_lu_shape", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_max_chunk", "p3smi_topology_set_mixed",
+    "p3smi_topology_set_lu_shape", "p3smi_topology_set_eval_tile", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_max_chunk", "p3smi_topology_set_mixed",
     "p3smi_host_jacobian", "p3smi_host_factor_solve", "p3smi_host_lu_emulate", "p3smi_host_maps",
     "p3smi_solve_batch", "p3smi_solve_batch_contingency", "p3smi_solve_batch_contingency_stamp",
     "p3smi_outages_branch_flows", "p3smi_outages_voltage_violations",
@@ -117,6 +117,7 @@ def lib():
     L.p3smi_update_y.argtypes = [P, P, _i32]
     L.p3smi_topology_set_mixed_switch.argtypes = [P, C.c_double]
     L.p3smi_topology_set_lu_shape.argtypes = [P, _i32, _i32, _i32, _i32, _i32, _i32]
+    L.p3smi_topology_set_eval_tile.argtypes = [P, _i32]
     L.p3smi_topology_set_mixed.argtypes = [P, _f64, _i32]
     L.p3smi_host_jacobian.argtypes = [P, P, P, P, P, P, P]
     L.p3smi_host_factor_solve.argtypes = [P, P, P, P, P]

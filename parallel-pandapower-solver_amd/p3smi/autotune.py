@@ -1,5 +1,5 @@
-"""Pick the LU launch shape per grid-size bucket and batch-size class by timing candidates
-on the device.
+"""Pick the LU launch shape and the evaluation's row tile per grid-size bucket and batch-size
+class by timing candidates on the device.
 
 The batched LU kernel has two shape parameters: g, the cases one workgroup factors
 together, and ns, its wavefronts (so 64 * ns / g task lists per case). Large g keeps
@@ -28,6 +28,8 @@ FEW_CANDIDATES = [(1, 8), (1, 16), (16, 8)]
 # dealing runs of a pivot's entries to one list (group > 0) lost to single entries on every
 # grid measured, so the tuner keeps to group 0 unless asked for more
 GROUP_CANDIDATES = [0]
+# rows per tile of the mismatch/Jacobian evaluation (a workgroup of 4 wavefronts per tile)
+EVAL_TILE_CANDIDATES = [8, 16, 32, 64]
 BUCKETS = [(500, "le500"), (5000, "le5k"), (20000, "le20k"), (10**9, "gt20k")]
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
 
@@ -87,8 +89,17 @@ def tune(solver, S, V0, reps: int = 2, log=None) -> dict:
             best_few = (1, 16, 0)
         fg, fns, fmax = best_few
         solver.set_lu_shape(bg, bns, fg, fns, fmax, bgrp)
+        tiles = {}
+        for rows in EVAL_TILE_CANDIDATES:
+            solver.set_eval_tile(rows)
+            _time_solve(solver, bh, 1)
+            tiles[rows] = _time_solve(solver, bh, reps)
+            if log:
+                log(f"[autotune] eval tile {rows} rows: {1e3 * tiles[rows]:.2f} ms")
+        best_rows = min(tiles, key=tiles.get)
+        solver.set_eval_tile(best_rows)
         return {"g": bg, "ns": bns, "group": bgrp, "few_g": fg, "few_ns": fns, "few_max": fmax,
-                "ms": 1e3 * min(results[(bg, bns, bgrp)], few.get(best_few, results[(bg, bns, bgrp)]))}
+                "eval_rows": best_rows, "ms": 1e3 * tiles[best_rows]}
     finally:
         L.p3smi_batch_free(bh)
 
@@ -114,4 +125,6 @@ def apply_cached(solver, cases: int = 1) -> bool:
     if not e:
         return False
     solver.set_lu_shape(e["g"], e["ns"], e["few_g"], e["few_ns"], e["few_max"], e.get("group", 0))
+    if "eval_rows" in e:
+        solver.set_eval_tile(e["eval_rows"])
     return True

@@ -97,6 +97,12 @@ class Solver:
                                                 int(group)))
         self.info = self._info()
 
+    def set_eval_tile(self, rows: int):
+        """Ybus rows per workgroup tile of the mismatch/Jacobian evaluation (results do not
+        depend on it)."""
+        check(lib().p3smi_topology_set_eval_tile(self._h, int(rows)))
+        self.info = self._info()
+
     def set_max_chunk(self, cases: int = 0):
         """Solve at most ``cases`` cases at once (a positive multiple of 64; 0: as many as 60%
         of the free HBM holds); larger calls run as nearly equal chunks one after the other

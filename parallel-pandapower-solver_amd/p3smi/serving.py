@@ -3,9 +3,10 @@
 A request is (topology, Sbus (n, B), V0 (n,) or (n, B)): B cases on one grid. The server
 keeps, per topology (keyed by a hash of Yp, Yj, Yx, pv, pq), a resident pool: the solver
 plan, Ybus and a device batch of slots. Cases of every request queued for a topology are
-loaded into free slots while the pool's other slots keep iterating; every pass advances
-all loaded cases by one Newton iteration; finished slots are fetched, their results
-scattered into their requests, and the slots refilled. A request completes when all its
+loaded into free slots while the pool's other slots keep iterating; every pass moves each
+loaded case one stage on (its start point, a Newton step, or a line-search trial);
+finished slots are fetched, their results scattered into their requests, and the slots
+refilled. A request completes when all its
 cases have. Each pool is driven by its own host thread on its own HIP stream, so passes of
 different topologies overlap on the device.
 
@@ -38,7 +39,6 @@ import numpy as np
 from . import _lib
 from .nr_klu import Solver, _prec
 
-_STATUS_ACTIVE, _STATUS_NEW = 0, 7
 
 
 def topology_key(Yp, Yj, Yx, pv, pq) -> str:

@@ -228,3 +228,25 @@ def test_update_Y_equals_a_rebuilt_solver(gpu):
     _compare(upd, ref, "update_Y")
     with pytest.raises(Exception, match="nnz"):
         s.update_Y(Yx2[:-1])
+
+
+def test_evaluation_tiles_do_not_change_the_bits(gpu):
+    """The mismatch/Jacobian evaluation's row tiles (rows per workgroup, staged in LDS up to 768
+    nonzeros) only change which workgroup evaluates a row: every shape gives the same bits."""
+    g = grids.config_grid("ts2000", 0)
+    Yp, Yj, Yx = g.csr()
+    S, V0 = grids.timeseries_daily(g, 128, seed=5), dc_initial_voltage(g)
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    first = None
+    for rows in (32, 1, 8, 64, 1024):
+        s.set_eval_tile(rows)
+        assert s.info["eval_rows"] == rows
+        r = s.solve_batch(S, V0, max_iter=20, tol=1e-8)
+        if first is None:
+            first = r
+            continue
+        np.testing.assert_array_equal(r["iterations"], first["iterations"])
+        np.testing.assert_array_equal(r["V"].view(np.float64), first["V"].view(np.float64))
+        np.testing.assert_array_equal(r["mismatch"], first["mismatch"])
+    with pytest.raises(Exception, match="rows"):
+        s.set_eval_tile(0)
