@@ -19,7 +19,9 @@
 // class and iteration count. The host runs passes; a pass is
 //     [LU of the cases in STEP] -> k_begin / k_advance -> k_evaluate -> k_control -> counters
 // and every kernel works only on the cases whose stage concerns it, so a case's backtracking
-// trial, another's next Newton iteration and a newly loaded case's start point share one pass.
+// trial, another's next Newton iteration and a newly loaded case's start point share one pass
+// (serving slots). A whole-batch solve holds the cases in STEP while any case is in its line
+// search, so that all cases take their k-th step in one LU launch.
 //
 // KERNELS:
 //   k_begin       start voltage -> state of the cases in START
@@ -238,6 +240,7 @@ struct EvalArgs {
     const int* row_q;
     const unsigned char* pin;  // (n, S) or null
     const unsigned* ctl;
+    int hold;                  // 1: the cases in STEP wait this pass (no LU ran for them)
     double* J;
     double* R;
     double* part;
@@ -266,7 +269,7 @@ __global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
     }
     __syncthreads();
     const unsigned sg = stage_of(a.ctl[c]);
-    const bool run = evaluated_stage(sg);
+    const bool run = evaluated_stage(sg) && !(a.hold && sg == SG_STEP);
     const bool jac = run && sg != SG_TRIAL;
     double worst = 0.0;
     if (run) {
@@ -537,8 +540,9 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     constexpr int TL = 64 * NS / G;
     // fewer than 64 cases per workgroup: lists differ within a wavefront, so their indices and
     // addresses sit in vector registers -- shorter chunks keep 16 wavefronts within 128
-    // registers (a quarter with one case per workgroup, half for FP32 factors otherwise)
-    constexpr int KDIV = G == 1 ? 4 : (G < 64 && SINGLE ? 2 : 1);
+    // registers: with one case per workgroup a quarter with FP64 factors, half with FP32 ones;
+    // half for FP32 factors with 16 or 32 cases per workgroup
+    constexpr int KDIV = G == 1 ? (SINGLE ? 2 : 4) : (G < 64 && SINGLE ? 2 : 1);
     constexpr int K = lu_chunk<real, real>() / KDIV;
     constexpr int KF = lu_chunk<real, double>() / KDIV;
     __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
@@ -1097,9 +1101,12 @@ __global__ __launch_bounds__(256) void k_refine_update(double* __restrict__ x, c
     }
 }
 
-// Per refining case: stop when |d| <= rtol |x|; fall back to FP64 for good when a step did not
-// at least halve |d| (refinement stalls: cond(J) times the FP32 rounding is too big) or when the
-// last allowed step still misses rtol. counts[0] = cases still refining.
+// Per refining case: stop when the correction |d| <= rtol |x|; fall back to FP64 for good when a
+// step did not at least halve |d| (refinement stalls: cond(J) times the FP32 rounding is too big)
+// or when the last allowed step still misses rtol. counts[0] += cases still refining, counts[1]
+// += fallbacks. Every step is refined to rtol, also far from the solution: on ill-conditioned
+// cases whose line search cuts steps, an inexact early step shifts the converged voltages by up
+// to |J^-1| tol (measured: 4e-9 with an inexact-Newton forcing term of 1e-4 ||F||).
 __global__ void k_refine_decide(const double* __restrict__ part, int nparts, double* __restrict__ prevd,
                                 unsigned* __restrict__ ctl, int* __restrict__ counts, double rtol, int last, int S) {
     __shared__ int cnt;
@@ -1117,6 +1124,7 @@ __global__ void k_refine_decide(const double* __restrict__ part, int nparts, dou
             ctl[c] = w & ~CTL_REFINING;
         } else if (!(dm <= 0.5 * prevd[c]) || last) {
             ctl[c] = set_prec(w & ~CTL_REFINING, PR_F64_ALWAYS);
+            atomicAdd(&counts[1], 1);
         } else {
             prevd[c] = dm;
             atomicAdd(&cnt, 1);
@@ -1126,12 +1134,17 @@ __global__ void k_refine_decide(const double* __restrict__ part, int nparts, dou
     if (threadIdx.x == 0 && cnt) atomicAdd(&counts[0], cnt);
 }
 
-// Start of a mixed-precision iteration: the FP32 cases in STEP refine, no previous correction.
-__global__ void k_refine_init(unsigned* __restrict__ ctl, double* __restrict__ prevd, int S) {
+// Start of a mixed-precision iteration: the FP32 cases in STEP refine, no previous correction;
+// counts[2] += cases in STEP of an FP64 class (factored in FP64 this iteration whatever happens).
+__global__ __launch_bounds__(256) void k_refine_init(unsigned* __restrict__ ctl, double* __restrict__ prevd,
+                                                     int* __restrict__ counts, int S) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned w = c < S ? ctl[c] : SG_IDLE;
+    const bool step = stage_of(w) == SG_STEP;
+    const bool on = step && prec_of(w) == PR_F32;
+    const int n64 = __popcll(__ballot(step && !on));
+    if ((threadIdx.x & 63) == 0 && n64) atomicAdd(&counts[2], n64);
     if (c >= S) return;
-    const unsigned w = ctl[c];
-    const bool on = stage_of(w) == SG_STEP && prec_of(w) == PR_F32;
     ctl[c] = on ? (w | CTL_REFINING) : (w & ~CTL_REFINING);
     prevd[c] = DBL_MAX;
 }
@@ -1152,14 +1165,14 @@ __global__ __launch_bounds__(256) void k_advance(const double* __restrict__ x, c
                                                  const double* __restrict__ alpha, double* __restrict__ Vm,
                                                  double* __restrict__ Va, double* __restrict__ Ve,
                                                  double* __restrict__ Vf, double* __restrict__ Vm0,
-                                                 double* __restrict__ Va0, int n, int S) {
+                                                 double* __restrict__ Va0, int hold, int n, int S) {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (bus >= n) return;
     const int iva = xva[bus], ivm = xvm[bus];
     if (iva < 0 && ivm < 0) return;
     const unsigned sg = stage_of(ctl[c]);
-    if (sg != SG_STEP && sg != SG_TRIAL && sg != SG_COMMIT) return;
+    if ((sg != SG_STEP || hold) && sg != SG_TRIAL && sg != SG_COMMIT) return;
     const size_t o = at(bus, S, c);
     double va, vm, step;
     if (sg == SG_STEP) {
@@ -1218,6 +1231,7 @@ struct CtlArgs {
     int* counts_out;     // NSTAT
     double tol, sw;      // sw > 0: switching mixed mode
     int line_search, max_iter, S;
+    int hold;            // 1: the cases in STEP waited this pass
 };
 
 __global__ __launch_bounds__(256) void k_control(CtlArgs a) {
@@ -1229,7 +1243,7 @@ __global__ __launch_bounds__(256) void k_control(CtlArgs a) {
     if (c < a.S) {
         unsigned w = a.ctl[c];
         unsigned sg = stage_of(w);
-        if (evaluated_stage(sg)) {
+        if (evaluated_stage(sg) && !(a.hold && sg == SG_STEP)) {
             double nw = 0.0;
             for (int t = 0; t < a.nparts; ++t) nw = nan_max(nw, a.part[at(t, a.S, c)]);
             if (!(nw < DBL_MAX)) nw = DBL_MAX;
@@ -2300,14 +2314,14 @@ void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     b->tally.alloc(NSTAT + 1);
     b->counts.alloc(NSTAT);
     b->guard.alloc(3);  // guard trips, FP32 and FP64 factorizations of the switching mode
-    b->rcount.alloc(1);
+    b->rcount.alloc(4);  // refining, fallbacks, FP64 classes, FP32 pivot failures before
     hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, t->stream, b->tally.p, NSTAT + 1);
     hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, t->stream, b->ctl.p, 0,
                        (int)S, 0u);
     HIPCHECK(hipGetLastError());
     b->mixed_ready = false;
     if (!b->h_counts) HIPCHECK(hipHostMalloc((void**)&b->h_counts, NSTAT * sizeof(int), hipHostMallocDefault));
-    if (!b->h_rcount) HIPCHECK(hipHostMalloc((void**)&b->h_rcount, sizeof(int), hipHostMallocDefault));
+    if (!b->h_rcount) HIPCHECK(hipHostMalloc((void**)&b->h_rcount, 4 * sizeof(int), hipHostMallocDefault));
     std::memset(b->h_counts, 0, NSTAT * sizeof(int));
     b->h_counts[CNT_IDLE] = (int)S;
     HIPCHECK(hipStreamSynchronize(t->stream));
@@ -2356,7 +2370,7 @@ void collect_timing(p3smi_batch* b) {
 
 dim3 grid_items(int items, int S) { return dim3((unsigned)(S / 64), (unsigned)((items + 3) / 4)); }
 
-void launch_evaluate(p3smi_batch* b) {
+void launch_evaluate(p3smi_batch* b, int hold) {
     p3smi_topology* t = b->topo;
     const p3smi::Topology& h = t->host;
     const DevTopo& d = t->dev;
@@ -2378,6 +2392,7 @@ void launch_evaluate(p3smi_batch* b) {
     a.row_q = d.row_q.p;
     a.pin = b->has_pin ? b->pin.p : nullptr;
     a.ctl = b->ctl.p;
+    a.hold = hold;
     a.J = b->Jx.p;
     a.R = b->R.p;
     a.part = b->part.p;
@@ -2485,12 +2500,15 @@ void linear_solve(p3smi_batch* b, int n_active, int precision) {
         return;
     }
     ensure_mixed(b);
-    b->stats.single_factorizations += n_active;  // cases whose FP32 pivots fail are refactored in FP64 below
     const unsigned gS = (unsigned)((b->S + 255) / 256);
     {
         KTimer tm(b, 4);
-        hipLaunchKernelGGL(k_refine_init, dim3(gS), dim3(256), 0, st, b->ctl.p, b->prevd.p, b->S);
+        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->rcount.p, 4);
+        hipLaunchKernelGGL(k_refine_init, dim3(gS), dim3(256), 0, st, b->ctl.p, b->prevd.p, b->rcount.p, b->S);
+        HIPCHECK(hipGetLastError());
     }
+    // rcount[3] = FP32 pivot failures so far (guard[2]), to tell this iteration's apart
+    HIPCHECK(hipMemcpyAsync(b->rcount.p + 3, b->guard.p + 2, sizeof(int), hipMemcpyDeviceToDevice, st));
     launch_lu<float, true>(b, n_active, b->fac32.p, b->R.p, b->x.p, SEL_F32);
     for (int k = 1; k <= t->mixed_kmax; ++k) {
         {
@@ -2503,17 +2521,28 @@ void linear_solve(p3smi_batch* b, int n_active, int precision) {
             KTimer tm(b, 5);
             hipLaunchKernelGGL(k_refine_update, dim3((unsigned)b->nparts_r, (unsigned)(b->S / 64)), dim3(256), 0, st,
                                b->x.p, b->dvec.p, b->rpart.p, b->ctl.p, h.m, b->S);
-            hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->rcount.p, 1);
+            hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->rcount.p, 1);  // [0] only
             hipLaunchKernelGGL(k_refine_decide, dim3(gS), dim3(256), 0, st, b->rpart.p, b->nparts_r, b->prevd.p,
                                b->ctl.p, b->rcount.p, t->mixed_rtol, (int)(k == t->mixed_kmax), b->S);
             HIPCHECK(hipGetLastError());
         }
         b->stats.refinement_steps += 1;
-        HIPCHECK(hipMemcpyAsync(b->h_rcount, b->rcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(b->h_rcount, b->rcount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        if (*b->h_rcount == 0) break;
+        if (b->h_rcount[0] == 0) break;
     }
-    launch_lu<double, true>(b, n_active, b->fac.p, b->R.p, b->x.p, SEL_F64);
+    int g2[2];
+    HIPCHECK(hipMemcpyAsync(&g2[0], b->rcount.p + 3, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&g2[1], b->guard.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(b->h_rcount, b->rcount.p, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const int fp64_class = b->h_rcount[2], fallbacks = b->h_rcount[1];
+    // FP64 factorizations of this iteration: the FP64 classes, stalled refinements, failed FP32 pivots
+    const int to_fp64 = fp64_class + fallbacks + (g2[1] - g2[0]);
+    b->stats.single_factorizations += n_active - fp64_class;
+    b->stats.double_factorizations += fp64_class + fallbacks;  // failed FP32 pivots: guard[2], at the end
+    // in the launch shape for their number
+    if (to_fp64 > 0) launch_lu<double, true>(b, to_fp64, b->fac.p, b->R.p, b->x.p, SEL_F64);
 }
 
 void check_solve_args(int max_iter, double tol, int precision) {
@@ -2530,11 +2559,12 @@ unsigned start_class(int precision) { return precision == P3SMI_PREC_MIXED ? PR_
 struct PassParams {
     int max_iter, line_search, precision;
     double tol;
+    bool lockstep;  // hold the cases in STEP while any case is in its line search
 };
 
 // The kernels of one pass (see the file header); the LU only when cases are in STEP, k_begin
 // only when cases are in START.
-void pass_kernels(p3smi_batch* b, const PassParams& p, int n_solve, bool starting) {
+void pass_kernels(p3smi_batch* b, const PassParams& p, int n_solve, bool starting, int hold) {
     p3smi_topology* t = b->topo;
     const p3smi::Topology& h = t->host;
     hipStream_t st = t->stream;
@@ -2545,13 +2575,14 @@ void pass_kernels(p3smi_batch* b, const PassParams& p, int n_solve, bool startin
                            start_class(p.precision), b->Vm.p, b->Va.p, b->Ve.p, b->Vf.p, h.n, b->S);
         HIPCHECK(hipGetLastError());
     }
-    if (n_solve > 0 || b->h_counts[CNT_TRIAL] > 0 || b->h_counts[CNT_COMMIT] > 0) {
+    if ((n_solve > 0 && !hold) || b->h_counts[CNT_TRIAL] > 0 || b->h_counts[CNT_COMMIT] > 0) {
         KTimer tm(b, 2);
         hipLaunchKernelGGL(k_advance, grid_items(h.n, b->S), dim3(256), 0, st, b->x.p, t->dev.xva.p, t->dev.xvm.p,
-                           b->ctl.p, b->alpha.p, b->Vm.p, b->Va.p, b->Ve.p, b->Vf.p, b->Vm0.p, b->Va0.p, h.n, b->S);
+                           b->ctl.p, b->alpha.p, b->Vm.p, b->Va.p, b->Ve.p, b->Vf.p, b->Vm0.p, b->Va0.p, hold, h.n,
+                           b->S);
         HIPCHECK(hipGetLastError());
     }
-    launch_evaluate(b);
+    launch_evaluate(b, hold);
     {
         KTimer tm(b, 3);
         CtlArgs a;
@@ -2569,6 +2600,7 @@ void pass_kernels(p3smi_batch* b, const PassParams& p, int n_solve, bool startin
         a.line_search = p.line_search;
         a.max_iter = p.max_iter;
         a.S = b->S;
+        a.hold = hold;
         hipLaunchKernelGGL(k_control, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, a);
         HIPCHECK(hipGetLastError());
     }
@@ -2615,7 +2647,9 @@ int live_cases(const int* cnt) { return cnt[CNT_START] + cnt[CNT_ACTIVE] + cnt[C
 const int* run_pass(p3smi_batch* b, const PassParams& p) {
     p3smi_topology* t = b->topo;
     hipStream_t st = t->stream;
-    const int n_solve = b->h_counts[CNT_ACTIVE];
+    const bool searching = b->h_counts[CNT_TRIAL] + b->h_counts[CNT_COMMIT] > 0;
+    const int hold = (p.lockstep && searching && b->h_counts[CNT_ACTIVE] > 0) ? 1 : 0;
+    const int n_solve = hold ? 0 : b->h_counts[CNT_ACTIVE];
     const bool starting = b->pending_start;
     if (n_solve > 0) b->stats.newton_passes++;
     if (b->h_counts[CNT_TRIAL] > 0) b->stats.line_search_passes++;
@@ -2628,18 +2662,18 @@ const int* run_pass(p3smi_batch* b, const PassParams& p) {
         }
         char key[360];
         std::snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%d/%d/%p/%d/%p/%p/%p/%p/%d/%d/%d/%p", n_solve > 0 ? cfg.key() : -1,
-                      cfg.G, cfg.NS, (int)starting, (int)(b->h_counts[CNT_TRIAL] + b->h_counts[CNT_COMMIT] > 0), p.tol,
+                      cfg.G, cfg.NS, (int)starting + 2 * hold, (int)searching, p.tol,
                       p.max_iter, p.line_search, (const void*)b->Ssp.p, b->s_stride, (const void*)b->Yg.p,
                       (const void*)b->Yb.p, (const void*)b->pin.p, (const void*)b->V0.p, b->v0_stride,
                       (int)b->per_case_y * 2 + (int)b->has_pin, t->dev.tile_rows, (const void*)b->part.p);
         replay(b, key, [&] {
             PassParams q = p;
             const int64_t before = b->stats.double_factorizations;
-            pass_kernels(b, q, n_solve, starting);
+            pass_kernels(b, q, n_solve, starting, hold);
             b->stats.double_factorizations = before;  // counted above, once per replay
         });
     } else {
-        pass_kernels(b, p, n_solve, starting);
+        pass_kernels(b, p, n_solve, starting, hold);
     }
     HIPCHECK(hipStreamSynchronize(st));
     b->pending_start = false;
@@ -2671,7 +2705,7 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
         KTimer tm(b, 4);
         reset_cases(b, precision);
     }
-    const PassParams p{max_iter, line_search ? 1 : 0, precision, tol};
+    const PassParams p{max_iter, line_search ? 1 : 0, precision, tol, true};
     // a case needs at most 1 + max_iter * (1 + LS_MAX_TRIALS + 1) passes
     const long long cap = 1 + (long long)max_iter * (LS_MAX_TRIALS + 2) + 2;
     const int* cnt = b->h_counts;
@@ -2688,6 +2722,8 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
     if (precision == P3SMI_PREC_MIXED_SWITCH) {
         b->stats.single_factorizations = g[1];
         b->stats.double_factorizations = g[2];
+    } else if (precision == P3SMI_PREC_MIXED) {
+        b->stats.double_factorizations += g[2];  // FP32 pivot failures refactored in FP64
     }
     collect_timing(b);
 }
@@ -3781,7 +3817,7 @@ int p3smi_batch_pass(p3smi_batch* b, int32_t max_iter, double tol, int32_t line_
         b->precision = precision;
         const int* cnt = b->h_counts;
         if (live_cases(cnt) > 0 || b->pending_start)
-            cnt = run_pass(b, PassParams{max_iter, line_search ? 1 : 0, precision, tol});
+            cnt = run_pass(b, PassParams{max_iter, line_search ? 1 : 0, precision, tol, false});
         if (counts) std::copy(cnt, cnt + NSTAT, counts);
         collect_timing(b);
     });
