@@ -55,10 +55,11 @@ typedef struct {
     int32_t cases, newton_passes, line_search_passes;
     int32_t failed_factorizations;
     int32_t guard_trips; /* factorizations whose pivot ratio fell below 1e-3 x the flat-start one */
-    int32_t reserved;
+    int32_t reruns;      /* cases run again from their start with pivoting after not converging */
     int64_t single_factorizations, double_factorizations, refinement_steps;
     double ms_eval, ms_lu, ms_step, ms_decide, ms_other, ms_refine;
-    int32_t n_eval, n_lu, n_step, n_decide, n_refine, reserved2;
+    int32_t n_eval, n_lu, n_step, n_decide, n_refine;
+    int32_t pivot_factorizations; /* fresh pivoting factorizations (the frozen plan's fallback) */
 } p3smi_stats;
 
 /* ---- library ------------------------------------------------------------------------ */
@@ -105,6 +106,14 @@ int p3smi_topology_set_eval_tile(p3smi_topology* t, int32_t rows);
  * once the correction is below rtol x the step, at most kmax steps, then falls back to an
  * FP64 factorization of that case. Defaults 1e-12 and 6. */
 int p3smi_topology_set_mixed(p3smi_topology* t, double rtol, int32_t kmax);
+/* Pivot-ratio guard of the frozen static-pivot plan (p3s/cpp/nr_klu.cpp:564-566, 609-620): a
+ * factorization whose min/max |pivot| falls below ratio x the flat-start one (or meets a zero
+ * pivot) is rejected, and the case factors with threshold partial pivoting (tolerance 1e-3,
+ * diagonal preferred, rows scaled) for the rest of its run, refactoring on that pivot order and
+ * pivoting again when a pivot vanishes. A case that ends without converging on the frozen plan
+ * is run again from its start with pivoting (p3s/cpp/nr_klu.cpp:844-857). Default 1e-3; 1e30
+ * rejects every frozen factorization (the reference's tests/cuda/test_lean_lu.py:224-244). */
+int p3smi_topology_set_guard(p3smi_topology* t, double ratio);
 /* Refresh Ybus values on the same pattern: nr_klu.Solver.update_Y (p3s/cpp/nr_klu.cpp:999-1011). */
 int p3smi_update_y(p3smi_topology* t, const double* Yx, int32_t nnz);
 

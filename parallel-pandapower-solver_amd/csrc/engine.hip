@@ -66,6 +66,9 @@ namespace {
 //   bits  4..7   backtracking trials taken in the current step (0..10)
 //   bits  8..9   precision class of the next factorization (mixed modes)
 //   bit   10     refining: the case's FP32 step is still being refined (mixed mode)
+//   bits 11..12  pivoting: frozen plan / needs a pivoting factorization / has its own pivot order
+//   bit   13     rerun: the case is on its run from scratch with pivoting (no further rerun)
+//   bit   14     pending: its step this iteration waits for the pivoting factorization
 //   bits 16..31  Newton iterations so far (linear solves)
 enum : unsigned {
     SG_IDLE = 0,    // free slot, or a padding column
@@ -81,6 +84,8 @@ enum : unsigned {
 // failed FP32 pivots); FP64 this once (an FP32 voter whose 64-case group takes an FP64 step)
 enum : unsigned { PR_F64 = 0, PR_F32 = 1, PR_F64_ALWAYS = 2, PR_F64_ONCE = 3 };
 constexpr unsigned CTL_REFINING = 1u << 10;
+enum : unsigned { PV_NONE = 0, PV_NEED = 1, PV_HAVE = 2 };
+constexpr unsigned CTL_RERUN = 1u << 13, CTL_PENDING = 1u << 14;
 constexpr int MAX_ITER_LIMIT = 65535;  // the iteration count has 16 bits
 
 __host__ __device__ constexpr unsigned stage_of(unsigned w) { return w & 15u; }
@@ -92,11 +97,32 @@ __host__ __device__ constexpr unsigned set_trials(unsigned w, unsigned t) { retu
 __host__ __device__ constexpr unsigned set_prec(unsigned w, unsigned p) { return (w & ~(3u << 8)) | (p << 8); }
 __host__ __device__ constexpr unsigned set_iters(unsigned w, int it) { return (w & 0xFFFFu) | ((unsigned)it << 16); }
 __host__ __device__ constexpr bool evaluated_stage(unsigned s) { return s >= SG_START && s <= SG_COMMIT; }
+__host__ __device__ constexpr unsigned piv_of(unsigned w) { return (w >> 11) & 3u; }
+__host__ __device__ constexpr unsigned set_piv(unsigned w, unsigned p) { return (w & ~(3u << 11)) | (p << 11); }
+// in STEP on the frozen plan, step not pending: what the static-pivot LU launches take
+__host__ __device__ constexpr bool frozen_step(unsigned w) {
+    return stage_of(w) == SG_STEP && piv_of(w) == PV_NONE && !(w & CTL_PENDING);
+}
+// a case this pass advances and evaluates: START, TRIAL, COMMIT, and STEP unless its step waits
+// for the pivoting factorization or the whole batch holds its STEP cases
+__host__ __device__ constexpr bool moves(unsigned w, int hold) {
+    return evaluated_stage(stage_of(w)) && !(stage_of(w) == SG_STEP && (hold || (w & CTL_PENDING)));
+}
+// The first run of a case ended without converging on the frozen plan: it starts again from its
+// start voltage, factoring with pivoting from its first iteration (p3s/cpp/nr_klu.cpp:844-857).
+__host__ __device__ constexpr unsigned rerun_word(unsigned w) {
+    return set_iters(set_piv(set_prec(set_trials(set_stage(w & ~(CTL_PENDING | CTL_REFINING), SG_START), 0), PR_F64_ALWAYS),
+                             PV_NEED),
+                     0) |
+           CTL_RERUN;
+}
 
 // Stage counters as the C ABI reports them (p3smi_batch_pass): active, converged, failed,
 // backtrack, needj, stopped, idle, new.
 constexpr int NSTAT = 8;
 enum : int { CNT_ACTIVE = 0, CNT_CONV, CNT_FAIL, CNT_TRIAL, CNT_COMMIT, CNT_STOP, CNT_IDLE, CNT_START };
+constexpr int CNT_PIVOT = NSTAT;   // internal: cases in STEP off the frozen plan (the pivoting solve)
+constexpr int NCOUNT = NSTAT + 1;  // counters k_control writes
 __host__ __device__ constexpr int counter_of(unsigned s) {
     return s == SG_START ? CNT_START : s == SG_STEP ? CNT_ACTIVE : s == SG_TRIAL ? CNT_TRIAL
          : s == SG_COMMIT ? CNT_COMMIT : s == SG_CONV ? CNT_CONV : s == SG_FAIL ? CNT_FAIL
@@ -133,10 +159,11 @@ __device__ __forceinline__ size_t at(int item, int S, int c) { return (size_t)it
 // largest of w and a, a NaN once seen kept
 __device__ __forceinline__ double nan_max(double w, double a) { return (a > w || a != a) ? a : w; }
 
-// Cases [0, B) start (START, no iterations, precision class p0), padding columns [B, S) idle.
-__global__ void k_ctl_reset(unsigned* __restrict__ ctl, int B, int S, unsigned p0) {
+// Cases [0, B) start (START, no iterations, precision class p0, and `extra`: the pivoting bits
+// when the frozen plan is not usable), padding columns [B, S) idle.
+__global__ void k_ctl_reset(unsigned* __restrict__ ctl, int B, int S, unsigned p0, unsigned extra) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < S) ctl[c] = c < B ? set_prec(SG_START, p0) : SG_IDLE;
+    if (c < S) ctl[c] = c < B ? (set_prec(SG_START, p0) | extra) : SG_IDLE;
 }
 
 __global__ void k_clear_int(int* p, int n) {
@@ -268,8 +295,9 @@ __global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
         }
     }
     __syncthreads();
-    const unsigned sg = stage_of(a.ctl[c]);
-    const bool run = evaluated_stage(sg) && !(a.hold && sg == SG_STEP);
+    const unsigned wc = a.ctl[c];
+    const unsigned sg = stage_of(wc);
+    const bool run = moves(wc, a.hold);
     const bool jac = run && sg != SG_TRIAL;
     double worst = 0.0;
     if (run) {
@@ -366,7 +394,7 @@ __global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
 enum : int { SEL_ALL = 0, SEL_F64 = 1, SEL_F32 = 2, SEL_REFINE = 3 };
 
 __device__ __forceinline__ bool selected(int sel, unsigned w) {
-    if (stage_of(w) != SG_STEP) return false;
+    if (!frozen_step(w)) return false;
     const unsigned p = prec_of(w);
     switch (sel) {
         case SEL_F64: return p != PR_F32;
@@ -386,7 +414,7 @@ __device__ __forceinline__ bool selected(int sel, unsigned w) {
 __global__ __launch_bounds__(256) void k_group_vote(unsigned* __restrict__ ctl, int* __restrict__ fcount, int S) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;  // blockDim 256: each wave is one group
     const unsigned w = c < S ? ctl[c] : SG_IDLE;
-    const bool act = stage_of(w) == SG_STEP;
+    const bool act = frozen_step(w);
     const bool f32 = act && prec_of(w) == PR_F32;
     const bool any64 = __ballot(act && !f32) != 0ull;
     if (f32 && any64) ctl[c] = set_prec(w, PR_F64_ONCE);
@@ -401,9 +429,10 @@ __global__ __launch_bounds__(256) void k_group_vote(unsigned* __restrict__ ctl, 
 // Pivot check at the end of a factorization (p3s/cpp/lean_lu.hpp:146-177, p3s/cpp/nr_klu.cpp:
 // 610-620), by the one thread that owns case c's statistics. FP32: a zero or non-finite pivot,
 // or a pivot ratio under the guard, moves the case to FP64 for good; it is factored in FP64 in
-// the same iteration (guard_trips[2] counts these). FP64: a zero or non-finite pivot fails the
-// case (its iteration counted); a ratio under the guard keeps the static-pivot step
-// (guard_trips[0]). Returns true when the case takes no step from these factors.
+// the same iteration (guard_trips[2] counts these). FP64: a zero or non-finite pivot or a ratio
+// under the guard rejects the frozen plan for the rest of the case's run (guard_trips[0]): its
+// step waits for the pivoting factorization (pending). Returns true when the case takes no step
+// from these factors.
 template <bool SINGLE>
 __device__ __forceinline__ bool pivot_verdict(unsigned* ctl, int c, double lo, double hi, double sum,
                                               double rcond_min, int* guard_trips) {
@@ -417,11 +446,11 @@ __device__ __forceinline__ bool pivot_verdict(unsigned* ctl, int c, double lo, d
         }
         return false;
     }
-    if (broken) {
-        ctl[c] = set_iters(set_stage(w, SG_FAIL), iters_of(w) + 1);
+    if (broken || lo < rcond_min * hi) {
+        ctl[c] = set_piv(w, PV_NEED) | CTL_PENDING;
+        atomicAdd(guard_trips, 1);
         return true;
     }
-    if (lo < rcond_min * hi) atomicAdd(guard_trips, 1);
     return false;
 }
 
@@ -1140,7 +1169,7 @@ __global__ __launch_bounds__(256) void k_refine_init(unsigned* __restrict__ ctl,
                                                      int* __restrict__ counts, int S) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned w = c < S ? ctl[c] : SG_IDLE;
-    const bool step = stage_of(w) == SG_STEP;
+    const bool step = frozen_step(w);
     const bool on = step && prec_of(w) == PR_F32;
     const int n64 = __popcll(__ballot(step && !on));
     if ((threadIdx.x & 63) == 0 && n64) atomicAdd(&counts[2], n64);
@@ -1171,8 +1200,9 @@ __global__ __launch_bounds__(256) void k_advance(const double* __restrict__ x, c
     if (bus >= n) return;
     const int iva = xva[bus], ivm = xvm[bus];
     if (iva < 0 && ivm < 0) return;
-    const unsigned sg = stage_of(ctl[c]);
-    if ((sg != SG_STEP || hold) && sg != SG_TRIAL && sg != SG_COMMIT) return;
+    const unsigned w = ctl[c];
+    const unsigned sg = stage_of(w);
+    if (sg == SG_START || !moves(w, hold)) return;
     const size_t o = at(bus, S, c);
     double va, vm, step;
     if (sg == SG_STEP) {
@@ -1216,7 +1246,8 @@ __device__ __forceinline__ unsigned switch_vote(unsigned p, double before, doubl
 //   COMMIT  the Jacobian rebuilt at the accepted point: converged, failed or iterating
 // nr_klu leaves its loop after the step of iteration max_iter without a further convergence
 // test, so such a case ends STOP. A non-finite norm counts as DBL_MAX, so no comparison
-// against it passes. The stage counters of the launch are summed over its workgroups; the last
+// against it passes. A case that ends STOP or FAIL on its first run starts again, factoring with
+// pivoting (rerun_word); the result of that run is the case's. The stage counters of the launch are summed over its workgroups; the last
 // workgroup to finish copies them to counts_out (what the host reads) and clears the sums for
 // the next launch, so no clearing kernel or memset is needed.
 struct CtlArgs {
@@ -1227,23 +1258,24 @@ struct CtlArgs {
     double* alpha;
     double* best_alpha;
     double* best_nrm;
-    int* tally;          // NSTAT sums + 1 finish counter, zero between launches
-    int* counts_out;     // NSTAT
+    int* tally;          // NCOUNT sums + 1 finish counter, zero between launches
+    int* counts_out;     // NCOUNT
+    int* reruns;         // += cases restarted with pivoting
     double tol, sw;      // sw > 0: switching mixed mode
     int line_search, max_iter, S;
     int hold;            // 1: the cases in STEP waited this pass
 };
 
 __global__ __launch_bounds__(256) void k_control(CtlArgs a) {
-    __shared__ int s_cnt[NSTAT];
+    __shared__ int s_cnt[NCOUNT];
     __shared__ int s_last;
-    if (threadIdx.x < NSTAT) s_cnt[threadIdx.x] = 0;
+    if (threadIdx.x < NCOUNT) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < a.S) {
         unsigned w = a.ctl[c];
         unsigned sg = stage_of(w);
-        if (evaluated_stage(sg) && !(a.hold && sg == SG_STEP)) {
+        if (moves(w, a.hold)) {
             double nw = 0.0;
             for (int t = 0; t < a.nparts; ++t) nw = nan_max(nw, a.part[at(t, a.S, c)]);
             if (!(nw < DBL_MAX)) nw = DBL_MAX;
@@ -1295,20 +1327,27 @@ __global__ __launch_bounds__(256) void k_control(CtlArgs a) {
                 sg = settle(nw, true);
                 w = set_trials(w, 0);
             }
-            a.ctl[c] = set_iters(set_prec(set_stage(w, sg), prec), it);
+            w = set_iters(set_prec(set_stage(w, sg), prec), it);
+            if ((sg == SG_STOP || sg == SG_FAIL) && !(w & CTL_RERUN)) {
+                w = rerun_word(w);
+                sg = SG_START;
+                atomicAdd(a.reruns, 1);
+            }
+            a.ctl[c] = w;
         }
         atomicAdd(&s_cnt[counter_of(sg)], 1);
+        if (sg == SG_STEP && piv_of(w) != PV_NONE) atomicAdd(&s_cnt[CNT_PIVOT], 1);
     }
     __syncthreads();
-    if (threadIdx.x < NSTAT && s_cnt[threadIdx.x]) atomicAdd(&a.tally[threadIdx.x], s_cnt[threadIdx.x]);
+    if (threadIdx.x < NCOUNT && s_cnt[threadIdx.x]) atomicAdd(&a.tally[threadIdx.x], s_cnt[threadIdx.x]);
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(&a.tally[NSTAT], 1) == (int)gridDim.x - 1;
+    if (threadIdx.x == 0) s_last = atomicAdd(&a.tally[NCOUNT], 1) == (int)gridDim.x - 1;
     __syncthreads();
     if (s_last) {
         __threadfence();
-        if (threadIdx.x < NSTAT) a.counts_out[threadIdx.x] = atomicExch(&a.tally[threadIdx.x], 0);
-        if (threadIdx.x == 0) atomicExch(&a.tally[NSTAT], 0);
+        if (threadIdx.x < NCOUNT) a.counts_out[threadIdx.x] = atomicExch(&a.tally[threadIdx.x], 0);
+        if (threadIdx.x == 0) atomicExch(&a.tally[NCOUNT], 0);
     }
 }
 
@@ -1327,6 +1366,326 @@ __global__ __launch_bounds__(256) void k_emit(const double* __restrict__ Ve, con
     V[2 * ((size_t)bus * B + c) + 1] = im;
 }
 
+
+// ===================================================================== pivoting factorization
+// The rescue of the frozen plan: a left-looking sparse LU with threshold partial pivoting
+// (Gilbert-Peierls: per column a depth-first reach through the columns of L gives the nonzero
+// rows in topological order, a sparse triangular solve, then the pivot), standing in for KLU's
+// klu_factor / klu_refactor / klu_solve at p3s/cpp/nr_klu.cpp:609-652. Rows are scaled by their
+// largest |a_ij| (KLU's default); the column order is the plan's minimum-degree order; a column's
+// pivot is its own row when that row is still free and holds at least PIVOT_TOL of the column's
+// largest candidate, else the largest (the first in reach order among equals). A refactorization
+// reuses the pivot order, row scales and pattern of the case's last factorization.
+//
+// One wavefront per case, each in a persistent slot of a pool (factors, pattern, work vectors,
+// slot-major). Lane 0 walks the depth-first search; the 64 lanes split every column update, the
+// pivot search and the store of an L column. The work vector w is zero between columns.
+constexpr double PIVOT_TOL = 1e-3;
+
+struct PivArgs {
+    const int* Cp;  // J by column (reference order): rows Ci, CSR entries Ce
+    const int* Ci;
+    const int* Ce;
+    const int* Jp;  // J rows (row scales)
+    const int* Q;   // column order
+    int m, S;
+    const double* J;  // [e * S + c]
+    const double* R;  // right-hand side, Jacobian row order [row * S + c]
+    double* x;        // solution, pivot order [k * S + c]
+    unsigned* ctl;
+    const int* list;  // the cases of this launch
+    const int* slot_of;
+    int* res;         // per list entry: 0 solved, 1 singular, 2 factor capacity exceeded
+    int* guard;       // [3] reruns, [4] pivoting factorizations
+    long long cap;    // L and U entries per slot
+    int *Lp, *Li, *Up, *Ui, *P, *pinv, *xi, *stk, *pstk, *flag;
+    double *Lx, *Ux, *Ud, *rs, *w, *y;
+};
+
+struct PivSlot {
+    int *Lp, *Li, *Up, *Ui, *P, *pinv, *xi, *stk, *pstk, *flag;
+    double *Lx, *Ux, *Ud, *rs, *w, *y;
+};
+
+__device__ __forceinline__ PivSlot piv_slot(const PivArgs& a, int s) {
+    const size_t m = (size_t)a.m, m1 = m + 1, cp = (size_t)a.cap;
+    return PivSlot{a.Lp + s * m1, a.Li + s * cp, a.Up + s * m1, a.Ui + s * cp, a.P + s * m,
+                   a.pinv + s * m,  a.xi + s * m, a.stk + s * m, a.pstk + s * m, a.flag + s * m,
+                   a.Lx + s * cp,  a.Ux + s * cp, a.Ud + s * m,  a.rs + s * m,   a.w + s * m,
+                   a.y + s * m};
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __threadfence_block();
+    __syncthreads();
+}
+
+// rows reachable from column col of A through the columns of L (lane 0): xi[top, m) in
+// topological order; flag[i] == k marks them
+__device__ int piv_reach(const PivArgs& a, const PivSlot& f, int col, int k) {
+    int top = a.m;
+    for (int p = a.Cp[col]; p < a.Cp[col + 1]; ++p) {
+        const int i0 = a.Ci[p];
+        if (f.flag[i0] == k) continue;
+        int head = 0;
+        f.stk[0] = i0;
+        while (head >= 0) {
+            const int i = f.stk[head];
+            const int j = f.pinv[i];
+            if (f.flag[i] != k) {
+                f.flag[i] = k;
+                f.pstk[head] = j < 0 ? 0 : f.Lp[j];
+            }
+            bool done = true;
+            if (j >= 0) {
+                const int pend = f.Lp[j + 1];
+                for (int q = f.pstk[head]; q < pend; ++q) {
+                    const int r = f.Li[q];
+                    if (f.flag[r] == k) continue;
+                    f.pstk[head] = q + 1;
+                    f.stk[++head] = r;
+                    done = false;
+                    break;
+                }
+            }
+            if (done) {
+                --head;
+                f.xi[--top] = i;
+            }
+        }
+    }
+    return top;
+}
+
+// w[i] -= L(i, j) xj over column j of L, split over the lanes
+__device__ __forceinline__ void piv_axpy(const PivSlot& f, int j, double xj, int lane) {
+    for (int q = f.Lp[j] + lane; q < f.Lp[j + 1]; q += 64) f.w[f.Li[q]] -= f.Lx[q] * xj;
+}
+
+// A(:, col) / row scale into w
+__device__ __forceinline__ void piv_scatter(const PivArgs& a, const PivSlot& f, int c, int col, int lane) {
+    for (int p = a.Cp[col] + lane; p < a.Cp[col + 1]; p += 64) {
+        const int i = a.Ci[p];
+        f.w[i] += a.J[at(a.Ce[p], a.S, c)] / f.rs[i];
+    }
+}
+
+// Fresh factorization. 0 = done, 1 = structurally or numerically singular, 2 = capacity.
+__device__ int piv_factor(const PivArgs& a, const PivSlot& f, int c, int lane) {
+    __shared__ int s_top;
+    const int m = a.m;
+    for (int i = lane; i < m; i += 64) {
+        double r = 0.0;
+        for (int e = a.Jp[i]; e < a.Jp[i + 1]; ++e) r = fmax(r, fabs(a.J[at(e, a.S, c)]));
+        f.rs[i] = (r > 0.0 && r <= DBL_MAX) ? r : 1.0;
+        f.pinv[i] = -1;
+        f.flag[i] = -1;
+        f.w[i] = 0.0;
+    }
+    if (lane == 0) f.Lp[0] = f.Up[0] = 0;
+    wave_sync();
+    long long lnz = 0, unz = 0;
+    for (int k = 0; k < m; ++k) {
+        const int col = a.Q[k];
+        if (lane == 0) s_top = piv_reach(a, f, col, k);
+        wave_sync();
+        const int top = s_top;
+        if (unz + (m - top) > a.cap || lnz + (m - top) > a.cap) return 2;
+        piv_scatter(a, f, c, col, lane);
+        wave_sync();
+        for (int q = top; q < m; ++q) {
+            const int i = f.xi[q];
+            const int j = f.pinv[i];
+            if (j < 0) continue;
+            const double xj = f.w[i];
+            if (lane == 0) {
+                f.Ui[unz] = j;
+                f.Ux[unz] = xj;
+            }
+            ++unz;
+            piv_axpy(f, j, xj, lane);
+            wave_sync();
+        }
+        if (lane == 0) f.Up[k + 1] = (int)unz;
+        // pivot search over the free rows of the reach: the largest |w|, the first in reach order
+        double amax = -1.0;
+        int piv = -1, pq = INT_MAX;
+        for (int q = top + lane; q < m; q += 64) {
+            const int i = f.xi[q];
+            if (f.pinv[i] >= 0) continue;
+            const double v = fabs(f.w[i]);
+            if (v > amax) {
+                amax = v;
+                piv = i;
+                pq = q;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double oa = __shfl_xor(amax, off);
+            const int op = __shfl_xor(piv, off), oq = __shfl_xor(pq, off);
+            if (oa > amax || (oa == amax && oq < pq)) {
+                amax = oa;
+                piv = op;
+                pq = oq;
+            }
+        }
+        if (f.pinv[col] < 0 && f.flag[col] == k && fabs(f.w[col]) >= PIVOT_TOL * amax) piv = col;
+        if (piv < 0 || !(amax > 0.0) || !(amax <= DBL_MAX)) return 1;
+        const double d = f.w[piv];
+        wave_sync();
+        if (lane == 0) {
+            f.Ud[k] = d;
+            f.P[k] = piv;
+            f.pinv[piv] = k;
+        }
+        wave_sync();
+        // column k of L: the free rows of the reach in reach order, divided by the pivot
+        for (int base = top; base < m; base += 64) {
+            const int q = base + lane;
+            bool take = false;
+            int i = 0;
+            double v = 0.0;
+            if (q < m) {
+                i = f.xi[q];
+                if (f.pinv[i] < 0) {
+                    take = true;
+                    v = f.w[i] / d;
+                }
+                f.w[i] = 0.0;
+            }
+            const unsigned long long bal = __ballot(take);
+            const int before = __popcll(bal & ((1ull << lane) - 1ull));
+            if (take) {
+                f.Li[lnz + before] = i;
+                f.Lx[lnz + before] = v;
+            }
+            lnz += __popcll(bal);
+        }
+        if (lane == 0) f.Lp[k + 1] = (int)lnz;
+        wave_sync();
+    }
+    return 0;
+}
+
+// Numeric refactorization on the case's pivot order and pattern. false: a zero or non-finite
+// pivot (w is left dirty; a fresh factorization clears it).
+__device__ bool piv_refactor(const PivArgs& a, const PivSlot& f, int c, int lane) {
+    const int m = a.m;
+    for (int i = lane; i < m; i += 64) f.w[i] = 0.0;
+    wave_sync();
+    for (int k = 0; k < m; ++k) {
+        piv_scatter(a, f, c, a.Q[k], lane);
+        wave_sync();
+        for (int q = f.Up[k]; q < f.Up[k + 1]; ++q) {
+            const int j = f.Ui[q];
+            const int i = f.P[j];
+            const double xj = __shfl(f.w[i], 0);
+            if (lane == 0) {
+                f.Ux[q] = xj;
+                f.w[i] = 0.0;
+            }
+            piv_axpy(f, j, xj, lane);
+            wave_sync();
+        }
+        const int pk = f.P[k];
+        const double d = __shfl(f.w[pk], 0);
+        if (!(fabs(d) > 0.0) || !(fabs(d) <= DBL_MAX)) return false;
+        if (lane == 0) {
+            f.w[pk] = 0.0;
+            f.Ud[k] = d;
+        }
+        for (int q = f.Lp[k] + lane; q < f.Lp[k + 1]; q += 64) {
+            const int i = f.Li[q];
+            f.Lx[q] = f.w[i] / d;
+            f.w[i] = 0.0;
+        }
+        wave_sync();
+    }
+    return true;
+}
+
+// x = A^-1 R with the case's factors (klu_solve): y = (R / rs) permuted by P, L y, U z, then
+// z in column order, which is the plan's pivot order.
+__device__ void piv_solve(const PivArgs& a, const PivSlot& f, int c, int lane) {
+    const int m = a.m;
+    for (int k = lane; k < m; k += 64) {
+        const int i = f.P[k];
+        f.y[k] = a.R[at(i, a.S, c)] / f.rs[i];
+    }
+    wave_sync();
+    for (int k = 0; k < m; ++k) {
+        const double yk = f.y[k];
+        if (yk != 0.0)
+            for (int q = f.Lp[k] + lane; q < f.Lp[k + 1]; q += 64) f.y[f.pinv[f.Li[q]]] -= f.Lx[q] * yk;
+        wave_sync();
+    }
+    for (int k = m - 1; k >= 0; --k) {
+        const double zk = __shfl(f.y[k], 0) / f.Ud[k];
+        if (lane == 0) f.y[k] = zk;
+        if (zk != 0.0)
+            for (int q = f.Up[k] + lane; q < f.Up[k + 1]; q += 64) f.y[f.Ui[q]] -= f.Ux[q] * zk;
+        wave_sync();
+    }
+    for (int k = lane; k < m; k += 64) a.x[at(k, a.S, c)] = f.y[k];
+}
+
+// The pivoting solve of every case in the list (cases in STEP off the frozen plan): a fresh
+// factorization for a case that needs one, else a refactorization on its order -- a fresh one
+// after all when that meets a zero pivot (p3s/cpp/nr_klu.cpp:629-651) -- then the solve. A case
+// whose fresh factorization is singular ends FAIL with this iteration counted, or, on its first
+// run, starts again with pivoting (as k_control restarts it). res 2: the slot's capacity was too
+// small for a fresh factorization; the case is left as it was, for a larger pool.
+__global__ __launch_bounds__(64) void k_pivot_solve(PivArgs a) {
+    const int lane = threadIdx.x;
+    const int c = a.list[blockIdx.x];
+    const int slot = a.slot_of[c];
+    if (slot < 0) {  // no slot assigned: the host reports it
+        if (lane == 0) a.res[blockIdx.x] = 3;
+        return;
+    }
+    const PivSlot f = piv_slot(a, slot);
+    const unsigned w0 = a.ctl[c];
+    bool fresh = piv_of(w0) != PV_HAVE;
+    bool ok = !fresh && piv_refactor(a, f, c, lane);
+    if (!ok) {
+        fresh = true;
+        const int r = piv_factor(a, f, c, lane);
+        if (r == 2) {
+            if (lane == 0) {
+                a.ctl[c] = set_piv(w0, PV_NEED) | CTL_PENDING;  // still waits for its step
+                a.res[blockIdx.x] = 2;
+            }
+            return;
+        }
+        if (lane == 0) atomicAdd(&a.guard[4], 1);
+        ok = r == 0;
+    }
+    if (!ok) {
+        if (lane == 0) {
+            if (w0 & CTL_RERUN) {
+                a.ctl[c] = set_iters(set_stage(w0 & ~CTL_PENDING, SG_FAIL), iters_of(w0) + 1);
+            } else {
+                a.ctl[c] = rerun_word(w0);
+                atomicAdd(&a.guard[3], 1);
+            }
+            a.res[blockIdx.x] = 1;
+        }
+        return;
+    }
+    piv_solve(a, f, c, lane);
+    if (lane == 0) {
+        a.ctl[c] = set_piv(w0 & ~CTL_PENDING, PV_HAVE);
+        a.res[blockIdx.x] = 0;
+    }
+}
+
+// The cases in STEP off the frozen plan, appended to list (count[0] = how many).
+__global__ void k_pivot_list(const unsigned* __restrict__ ctl, int S, int* __restrict__ list, int* __restrict__ count) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= S) return;
+    const unsigned w = ctl[c];
+    if (stage_of(w) == SG_STEP && piv_of(w) != PV_NONE) list[atomicAdd(count, 1)] = c;
+}
 
 // Time-series injections on the device (p3s/timeseries.py:36-141): for bus i and case t,
 // S[i, t] = s0[i] + sum_k coef[k] * (X[row[k], t] - base[k]) over the contributions k of
@@ -1543,8 +1902,8 @@ __global__ void k_vviol(const double* __restrict__ Ve, const double* __restrict_
 // Copy k staged cases (S and V0, (n, k) complex) into slot columns and mark them START.
 __global__ __launch_bounds__(256) void k_load_slots(const double* __restrict__ Sst, const double* __restrict__ Vst,
                                                     const int* __restrict__ slots, double* __restrict__ Ssp,
-                                                    double* __restrict__ V0, unsigned* __restrict__ ctl, int n, int k,
-                                                    int S) {
+                                                    double* __restrict__ V0, unsigned* __restrict__ ctl, unsigned extra,
+                                                    int n, int k, int S) {
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (j >= k || bus >= n) return;
@@ -1554,7 +1913,7 @@ __global__ __launch_bounds__(256) void k_load_slots(const double* __restrict__ S
     Ssp[dst + 1] = Sst[src + 1];
     V0[dst] = Vst[src];
     V0[dst + 1] = Vst[src + 1];
-    if (bus == 0) ctl[c] = SG_START;
+    if (bus == 0) ctl[c] = SG_START | extra;
 }
 
 // Results of k slots into (n, k) complex V, iterations, flags, final mismatch.
@@ -1924,6 +2283,7 @@ struct DevSched {  // one level schedule dealt to TL task lists
 
 struct DevTopo {
     DevBuf<int> Yp, Yj, Ydiag, row_p, row_q, posJ, xva, xvm, Jp, xcol;
+    DevBuf<int> Cp, Ci, Ce, perm;  // the Jacobian by column and the column order (pivoting LU)
     DevBuf<double> G, B;
     DevBuf<int> tiles;    // k_evaluate row tiles: first row of each, then n
     int ntiles = 0, tile_rows = 0;
@@ -1955,10 +2315,24 @@ struct p3smi_topology {
     int mixed_kmax = 6;           // refinement steps before the FP64 fallback
     int max_chunk = 0;            // one-call entry points: cases per chunk at most (0: sized to free HBM)
     int eval_rows = 32;           // k_evaluate: rows per tile at most (the autotuner may change it)
+    double guard_ratio = REUSE_RCOND_RATIO;  // frozen plan rejected below guard_ratio x rcond0
     hipStream_t stream = nullptr;
     std::unique_ptr<p3smi_batch> cached;  // workspace of the one-call entry points
     std::mutex mu;
     ~p3smi_topology();
+};
+
+// Slots of the pivoting factorization (see k_pivot_solve): per slot the L and U factors (cap
+// entries each), pattern, pivot order, row scales and work vectors, slot-major. A case keeps its
+// slot while it runs off the frozen plan (its factors persist between iterations).
+struct PivotPool {
+    int slots = 0, m = 0;
+    long long cap = 0;
+    DevBuf<int> Lp, Li, Up, Ui, P, pinv, xi, stk, pstk, flag;
+    DevBuf<double> Lx, Ux, Ud, rs, w, y;
+    DevBuf<int> slot_of, list, res, count;
+    std::vector<int> h_slot_of;  // host mirror of slot_of
+    std::vector<int> free;       // free slots
 };
 
 struct p3smi_batch {
@@ -1994,6 +2368,7 @@ struct p3smi_batch {
     DevBuf<double> sbX;                 // element series of the last device Sbus build
     bool graphs = false;                // replay captured Newton iterations (FP64, timers off)
     std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gcache;
+    PivotPool piv;                      // pivoting factorizations (allocated on first use)
     p3smi_stats stats{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
     size_t ev_used = 0;
@@ -2060,6 +2435,10 @@ void ensure_device(p3smi_topology* t) {
     std::vector<int> xcol(h.nnzJ);
     for (int e = 0; e < h.nnzJ; ++e) xcol[e] = h.pinv[h.Jj[e]];
     d.xcol.upload(xcol.data(), xcol.size(), st);
+    d.Cp.upload(h.Cp.data(), h.Cp.size(), st);
+    d.Ci.upload(h.Ci.data(), h.Ci.size(), st);
+    d.Ce.upload(h.Ce.data(), h.Ce.size(), st);
+    d.perm.upload(h.perm.data(), h.perm.size(), st);
     HIPCHECK(hipStreamSynchronize(st));
     d.ready = true;
 }
@@ -2311,19 +2690,23 @@ void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     b->part.alloc((size_t)t->dev.ntiles * S);
     b->pstat.alloc(2 * S);
     b->ctl.alloc(S);
-    b->tally.alloc(NSTAT + 1);
-    b->counts.alloc(NSTAT);
-    b->guard.alloc(3);  // guard trips, FP32 and FP64 factorizations of the switching mode
+    b->tally.alloc(NCOUNT + 1);
+    b->counts.alloc(NCOUNT);
+    // guard trips, FP32 and FP64 factorizations of the switching mode, reruns, pivoting factorizations
+    b->guard.alloc(5);
     b->rcount.alloc(4);  // refining, fallbacks, FP64 classes, FP32 pivot failures before
-    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, t->stream, b->tally.p, NSTAT + 1);
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, t->stream, b->tally.p, NCOUNT + 1);
     hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, t->stream, b->ctl.p, 0,
-                       (int)S, 0u);
+                       (int)S, 0u, 0u);
     HIPCHECK(hipGetLastError());
     b->mixed_ready = false;
-    if (!b->h_counts) HIPCHECK(hipHostMalloc((void**)&b->h_counts, NSTAT * sizeof(int), hipHostMallocDefault));
+    if (!b->h_counts) HIPCHECK(hipHostMalloc((void**)&b->h_counts, NCOUNT * sizeof(int), hipHostMallocDefault));
     if (!b->h_rcount) HIPCHECK(hipHostMalloc((void**)&b->h_rcount, 4 * sizeof(int), hipHostMallocDefault));
-    std::memset(b->h_counts, 0, NSTAT * sizeof(int));
+    std::memset(b->h_counts, 0, NCOUNT * sizeof(int));
     b->h_counts[CNT_IDLE] = (int)S;
+    b->piv.h_slot_of.assign(S, -1);
+    b->piv.free.clear();
+    for (int k = b->piv.slots - 1; k >= 0; --k) b->piv.free.push_back(k);
     HIPCHECK(hipStreamSynchronize(t->stream));
 }
 
@@ -2410,7 +2793,7 @@ void launch_evaluate(p3smi_batch* b, int hold) {
 template <int G, int NS, typename real, bool FACTOR>
 void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* rhs, double* xout, int sel) {
     const p3smi::Topology& h = b->topo->host;
-    const double rcond_min = REUSE_RCOND_RATIO * h.rcond0;
+    const double rcond_min = b->topo->guard_ratio * h.rcond0;
     if constexpr (G == 64) {
         if (ds.blk) {
             hipLaunchKernelGGL((k_lu_blk<NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
@@ -2553,6 +2936,142 @@ void check_solve_args(int max_iter, double tol, int precision) {
         throw Error(P3SMI_E_ARG, "precision must be 0 (double), 1 (mixed) or 2 (mixed_switch)");
 }
 
+// ---- pivoting factorizations ------------------------------------------------------------
+// Control bits a case starts with: none on a usable frozen plan; otherwise it factors with
+// pivoting from its first iteration and has no rerun (nothing frozen was reused,
+// p3s/cpp/nr_klu.cpp:844-857).
+unsigned start_extra(const p3smi_topology* t) { return t->host.lean_valid ? 0u : (set_piv(0u, PV_NEED) | CTL_RERUN); }
+
+// rows x width array -> rows' x width' keeping every row's leading min(width, width') entries
+template <typename T>
+void regrow(DevBuf<T>& buf, size_t rows, size_t width, size_t rows2, size_t width2, hipStream_t st) {
+    DevBuf<T> nb;
+    nb.alloc(rows2 * width2);
+    if (buf.p && rows && width)
+        HIPCHECK(hipMemcpy2DAsync(nb.p, width2 * sizeof(T), buf.p, width * sizeof(T), std::min(width, width2) * sizeof(T),
+                                  std::min(rows, rows2), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    std::swap(buf.p, nb.p);
+    std::swap(buf.n, nb.n);
+}
+
+// At least `slots` slots of `cap` factor entries each; existing slots keep their factors.
+void pivot_pool_ensure(p3smi_batch* b, int slots, long long cap) {
+    PivotPool& P = b->piv;
+    const size_t m = (size_t)b->topo->host.m;
+    hipStream_t st = b->topo->stream;
+    if (P.slots > 0 && P.m != (int)m) throw Error(P3SMI_E_INTERNAL, "pivot pool: dimension changed");
+    if (slots <= P.slots && cap <= P.cap) return;
+    const size_t s0 = P.slots, s1 = std::max(slots, P.slots);
+    const size_t c0 = (size_t)P.cap, c1 = (size_t)std::max(cap, P.cap);
+    for (auto* v : {&P.Lp, &P.Up}) regrow(*v, s0, m + 1, s1, m + 1, st);
+    for (auto* v : {&P.P, &P.pinv, &P.xi, &P.stk, &P.pstk, &P.flag}) regrow(*v, s0, m, s1, m, st);
+    for (auto* v : {&P.Ud, &P.rs, &P.w, &P.y}) regrow(*v, s0, m, s1, m, st);
+    for (auto* v : {&P.Li, &P.Ui}) regrow(*v, s0, c0, s1, c1, st);
+    for (auto* v : {&P.Lx, &P.Ux}) regrow(*v, s0, c0, s1, c1, st);
+    for (size_t k = s1; k-- > s0;) P.free.push_back((int)k);
+    P.slots = (int)s1;
+    P.cap = (long long)c1;
+    P.m = (int)m;
+}
+
+void pivot_release(p3smi_batch* b, int c) {
+    PivotPool& P = b->piv;
+    if (c >= 0 && c < (int)P.h_slot_of.size() && P.h_slot_of[c] >= 0) {
+        P.free.push_back(P.h_slot_of[c]);
+        P.h_slot_of[c] = -1;
+    }
+}
+
+void pivot_release_all(p3smi_batch* b) {
+    PivotPool& P = b->piv;
+    P.h_slot_of.assign(b->S, -1);
+    P.free.clear();
+    for (int k = P.slots - 1; k >= 0; --k) P.free.push_back(k);
+}
+
+// The pivoting solves of this pass: every case in STEP off the frozen plan gets its step from
+// k_pivot_solve (slots assigned here; a pool too small for a fresh factorization grows and the
+// case is factored again). Returns the number of cases solved.
+int pivot_solve(p3smi_batch* b) {
+    p3smi_topology* t = b->topo;
+    const p3smi::Topology& h = t->host;
+    hipStream_t st = t->stream;
+    PivotPool& P = b->piv;
+    KTimer tm(b, 1);
+    if (P.list.n < (size_t)b->S) {
+        P.list.alloc(b->S);
+        P.res.alloc(b->S);
+        P.slot_of.alloc(b->S);
+    }
+    P.count.alloc(1);
+    if (P.h_slot_of.size() != (size_t)b->S) pivot_release_all(b);
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, P.count.p, 1);
+    hipLaunchKernelGGL(k_pivot_list, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->ctl.p, b->S, P.list.p,
+                       P.count.p);
+    HIPCHECK(hipGetLastError());
+    int n = 0;
+    HIPCHECK(hipMemcpyAsync(&n, P.count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (n <= 0) return 0;
+    if (n > b->S) throw Error(P3SMI_E_INTERNAL, "pivot list overflow");
+    std::vector<int> list(n);
+    HIPCHECK(hipMemcpyAsync(list.data(), P.list.p, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    int lacking = 0;
+    for (int c : list) lacking += P.h_slot_of[c] < 0;
+    if (lacking > (int)P.free.size()) {
+        const int want = P.slots + lacking - (int)P.free.size();
+        pivot_pool_ensure(b, (want + 15) / 16 * 16, std::max<long long>(P.cap, std::max<long long>(h.nnz_lu, h.m)));
+    }
+    for (int c : list)
+        if (P.h_slot_of[c] < 0) {
+            P.h_slot_of[c] = P.free.back();
+            P.free.pop_back();
+        }
+    HIPCHECK(hipMemcpyAsync(P.slot_of.p, P.h_slot_of.data(), b->S * sizeof(int), hipMemcpyHostToDevice, st));
+    std::vector<int> res(n);
+    for (;;) {
+        PivArgs a;
+        a.Cp = t->dev.Cp.p;
+        a.Ci = t->dev.Ci.p;
+        a.Ce = t->dev.Ce.p;
+        a.Jp = t->dev.Jp.p;
+        a.Q = t->dev.perm.p;
+        a.m = h.m;
+        a.S = b->S;
+        a.J = b->Jx.p;
+        a.R = b->R.p;
+        a.x = b->x.p;
+        a.ctl = b->ctl.p;
+        a.list = P.list.p;
+        a.slot_of = P.slot_of.p;
+        a.res = P.res.p;
+        a.guard = b->guard.p;
+        a.cap = P.cap;
+        a.Lp = P.Lp.p; a.Li = P.Li.p; a.Up = P.Up.p; a.Ui = P.Ui.p; a.P = P.P.p; a.pinv = P.pinv.p;
+        a.xi = P.xi.p; a.stk = P.stk.p; a.pstk = P.pstk.p; a.flag = P.flag.p;
+        a.Lx = P.Lx.p; a.Ux = P.Ux.p; a.Ud = P.Ud.p; a.rs = P.rs.p; a.w = P.w.p; a.y = P.y.p;
+        hipLaunchKernelGGL(k_pivot_solve, dim3((unsigned)n), dim3(64), 0, st, a);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(res.data(), P.res.p, n * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        std::vector<int> again;
+        for (int q = 0; q < n; ++q) {
+            if (res[q] == 3) throw Error(P3SMI_E_INTERNAL, "pivoting factorization: a case without a slot");
+            if (res[q] == 2) again.push_back(list[q]);
+        }
+        if (again.empty()) break;
+        // fill beyond the pool's capacity: a pool with twice the capacity, the same slots
+        if (P.cap > (long long)h.m * (long long)h.m) throw Error(P3SMI_E_INTERNAL, "pivoting factorization: fill beyond m^2");
+        pivot_pool_ensure(b, P.slots, 2 * P.cap);
+        list = again;
+        n = (int)list.size();
+        HIPCHECK(hipMemcpyAsync(P.list.p, list.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    return (int)list.size();
+}
+
 // precision class a case starts in
 unsigned start_class(int precision) { return precision == P3SMI_PREC_MIXED ? PR_F32 : PR_F64; }
 
@@ -2564,11 +3083,12 @@ struct PassParams {
 
 // The kernels of one pass (see the file header); the LU only when cases are in STEP, k_begin
 // only when cases are in START.
-void pass_kernels(p3smi_batch* b, const PassParams& p, int n_solve, bool starting, int hold) {
+void pass_kernels(p3smi_batch* b, const PassParams& p, int n_solve, bool starting, int hold, bool pivoting) {
     p3smi_topology* t = b->topo;
     const p3smi::Topology& h = t->host;
     hipStream_t st = t->stream;
     if (n_solve > 0) linear_solve(b, n_solve, p.precision);
+    if (pivoting) pivot_solve(b);
     if (starting) {
         KTimer tm(b, 4);
         hipLaunchKernelGGL(k_begin, grid_items(h.n, b->S), dim3(256), 0, st, b->V0.p, b->v0_stride, b->ctl.p,
@@ -2601,10 +3121,11 @@ void pass_kernels(p3smi_batch* b, const PassParams& p, int n_solve, bool startin
         a.max_iter = p.max_iter;
         a.S = b->S;
         a.hold = hold;
+        a.reruns = b->guard.p + 3;
         hipLaunchKernelGGL(k_control, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, a);
         HIPCHECK(hipGetLastError());
     }
-    HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NSTAT * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NCOUNT * sizeof(int), hipMemcpyDeviceToHost, st));
 }
 
 // ---- hipGraph replay of passes ---------------------------------------------------------
@@ -2650,11 +3171,14 @@ const int* run_pass(p3smi_batch* b, const PassParams& p) {
     const bool searching = b->h_counts[CNT_TRIAL] + b->h_counts[CNT_COMMIT] > 0;
     const int hold = (p.lockstep && searching && b->h_counts[CNT_ACTIVE] > 0) ? 1 : 0;
     const int n_solve = hold ? 0 : b->h_counts[CNT_ACTIVE];
-    const bool starting = b->pending_start;
+    const bool starting = b->pending_start || b->h_counts[CNT_START] > 0;
+    // cases off the frozen plan take their step from the pivoting factorization (host round trips:
+    // this pass runs kernel by kernel)
+    const bool pivoting = !hold && b->h_counts[CNT_PIVOT] > 0;
     if (n_solve > 0) b->stats.newton_passes++;
     if (b->h_counts[CNT_TRIAL] > 0) b->stats.line_search_passes++;
     if (b->part.n < (size_t)t->dev.ntiles * b->S) b->part.alloc((size_t)t->dev.ntiles * b->S);  // never under capture
-    if (b->graphs && !b->timing && p.precision == P3SMI_PREC_DOUBLE) {
+    if (b->graphs && !b->timing && p.precision == P3SMI_PREC_DOUBLE && !pivoting) {
         const LuConfig cfg = lu_config_for(t, n_solve);
         if (n_solve > 0) {
             (void)get_sched(t, cfg);  // upload before recording
@@ -2669,11 +3193,11 @@ const int* run_pass(p3smi_batch* b, const PassParams& p) {
         replay(b, key, [&] {
             PassParams q = p;
             const int64_t before = b->stats.double_factorizations;
-            pass_kernels(b, q, n_solve, starting, hold);
+            pass_kernels(b, q, n_solve, starting, hold, false);
             b->stats.double_factorizations = before;  // counted above, once per replay
         });
     } else {
-        pass_kernels(b, p, n_solve, starting, hold);
+        pass_kernels(b, p, n_solve, starting, hold, pivoting);
     }
     HIPCHECK(hipStreamSynchronize(st));
     b->pending_start = false;
@@ -2683,13 +3207,14 @@ const int* run_pass(p3smi_batch* b, const PassParams& p) {
 void reset_cases(p3smi_batch* b, int precision) {
     hipStream_t st = b->topo->stream;
     hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->ctl.p, b->B, b->S,
-                       start_class(precision));
-    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 3);
+                       start_class(precision), start_extra(b->topo));
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 5);
     HIPCHECK(hipGetLastError());
-    std::memset(b->h_counts, 0, NSTAT * sizeof(int));
+    std::memset(b->h_counts, 0, NCOUNT * sizeof(int));
     b->h_counts[CNT_START] = b->B;
     b->h_counts[CNT_IDLE] = b->S - b->B;
     b->pending_start = true;
+    pivot_release_all(b);
 }
 
 void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int precision) {
@@ -2706,8 +3231,10 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
         reset_cases(b, precision);
     }
     const PassParams p{max_iter, line_search ? 1 : 0, precision, tol, true};
-    // a case needs at most 1 + max_iter * (1 + LS_MAX_TRIALS + 1) passes
-    const long long cap = 1 + (long long)max_iter * (LS_MAX_TRIALS + 2) + 2;
+    // a run of a case needs at most 1 + max_iter * (1 + 1 + LS_MAX_TRIALS + 1) passes (start; per
+    // iteration a pending pass after a guard trip, the step, the trials and the commit), and a case
+    // has at most two runs
+    const long long cap = 2 * (1 + (long long)max_iter * (LS_MAX_TRIALS + 3)) + 2;
     const int* cnt = b->h_counts;
     long long passes = 0;
     while (live_cases(cnt) > 0) {
@@ -2715,10 +3242,12 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
         cnt = run_pass(b, p);
     }
     b->stats.failed_factorizations = cnt[CNT_FAIL];
-    int g[3];
+    int g[5];
     HIPCHECK(hipMemcpyAsync(g, b->guard.p, sizeof g, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     b->stats.guard_trips = g[0];
+    b->stats.reruns = g[3];
+    b->stats.pivot_factorizations = g[4];
     if (precision == P3SMI_PREC_MIXED_SWITCH) {
         b->stats.single_factorizations = g[1];
         b->stats.double_factorizations = g[2];
@@ -2913,6 +3442,15 @@ int p3smi_topology_set_lu_shape(p3smi_topology* t, int32_t g, int32_t ns, int32_
         t->lu_batch = a;
         t->lu_few = f;
         t->lu_few_max = few_max;
+    });
+}
+
+int p3smi_topology_set_guard(p3smi_topology* t, double ratio) {
+    return guarded([&] {
+        if (!t) throw Error(P3SMI_E_ARG, "set_guard: null topology");
+        if (!(ratio >= 0.0) || !std::isfinite(ratio)) throw Error(P3SMI_E_ARG, "set_guard: ratio must be finite and >= 0");
+        std::lock_guard<std::mutex> lk(t->mu);
+        t->guard_ratio = ratio;
     });
 }
 
@@ -3147,10 +3685,12 @@ int p3smi_batch_stats(const p3smi_batch* b, p3smi_stats* out) {
         if (b->slot_mode) {
             // passes of a slot batch do not read the device counters back: read them here
             // (guard trips; FP32 and FP64 factorizations of the switching mode) since slots_init
-            int g[3];
+            int g[5];
             HIPCHECK(hipMemcpyAsync(g, b->guard.p, sizeof g, hipMemcpyDeviceToHost, b->topo->stream));
             HIPCHECK(hipStreamSynchronize(b->topo->stream));
             out->guard_trips = g[0];
+            out->reruns = g[3];
+            out->pivot_factorizations = g[4];
             if (b->precision == P3SMI_PREC_MIXED_SWITCH) {
                 out->single_factorizations = g[1];
                 out->double_factorizations = g[2];
@@ -3214,6 +3754,8 @@ static void add_stats(p3smi_stats& a, const p3smi_stats& s) {
     a.ms_decide += s.ms_decide;
     a.ms_other += s.ms_other;
     a.ms_refine += s.ms_refine;
+    a.reruns += s.reruns;
+    a.pivot_factorizations += s.pivot_factorizations;
     a.n_eval += s.n_eval;
     a.n_lu += s.n_lu;
     a.n_step += s.n_step;
@@ -3777,8 +4319,10 @@ int p3smi_batch_slots_init(p3smi_batch* b) {
         b->mask_unserved = false;
         b->slot_mode = true;
         ensure_tiles(b->topo);
-        hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->ctl.p, 0, b->S, 0u);
-        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 3);
+        hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->ctl.p, 0, b->S, 0u,
+                           0u);
+        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 5);
+        pivot_release_all(b);
         HIPCHECK(hipGetLastError());
         std::memset(b->h_counts, 0, NSTAT * sizeof(int));
         b->h_counts[CNT_IDLE] = b->S;
@@ -3800,7 +4344,8 @@ int p3smi_batch_load(p3smi_batch* b, int32_t k, const int32_t* slots, const doub
         b->st_V.upload(V0, 2 * (size_t)h.n * k, st);
         b->st_slots.upload(slots, k, st);
         hipLaunchKernelGGL(k_load_slots, dim3((unsigned)((k + 63) / 64), (unsigned)((h.n + 3) / 4)), dim3(256), 0, st,
-                           b->st_S.p, b->st_V.p, b->st_slots.p, b->Ssp.p, b->V0.p, b->ctl.p, h.n, k, b->S);
+                           b->st_S.p, b->st_V.p, b->st_slots.p, b->Ssp.p, b->V0.p, b->ctl.p, start_extra(b->topo), h.n,
+                           k, b->S);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(st));
         b->pending_start = true;
@@ -3855,6 +4400,7 @@ int p3smi_batch_fetch(p3smi_batch* b, int32_t k, const int32_t* slots, double* V
                            b->st_mis.p, h.n, k, b->S);
         hipLaunchKernelGGL(k_release_slots, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, st, b->st_slots.p,
                            b->ctl.p, k);
+        for (int j = 0; j < k; ++j) pivot_release(b, slots[j]);
         HIPCHECK(hipGetLastError());
         if (V) HIPCHECK(hipMemcpyAsync(V, b->st_out.p, 2 * (size_t)h.n * k * sizeof(double), hipMemcpyDeviceToHost, st));
         if (iterations) HIPCHECK(hipMemcpyAsync(iterations, b->st_it.p, k * sizeof(int), hipMemcpyDeviceToHost, st));

@@ -168,6 +168,34 @@ static void build_jacobian_pattern(Topology& t) {
             t.Jb[e] = rows[r][q].second & 3;
             t.posJ[rows[r][q].second] = e;
         }
+    // by column: column c of bus j takes, for the buses i of Ybus column j in ascending order,
+    // first the P rows of those with one, then their Q rows
+    std::vector<std::vector<int>> ycol(t.n);
+    for (int i = 0; i < t.n; ++i)
+        for (int k = t.Yp[i]; k < t.Yp[i + 1]; ++k) ycol[t.Yj[k]].push_back(i);
+    auto entry = [&](int r, int c) {
+        const auto b = t.Jj.begin() + t.Jp[r], e = t.Jj.begin() + t.Jp[r + 1];
+        const auto it = std::lower_bound(b, e, c);
+        if (it == e || *it != c) throw std::runtime_error("Jacobian pattern: missing column entry");
+        return (int)(it - t.Jj.begin());
+    };
+    t.Cp.assign(t.m + 1, 0);
+    t.Ci.clear();
+    t.Ce.clear();
+    t.Ci.reserve(t.nnzJ);
+    t.Ce.reserve(t.nnzJ);
+    for (int c = 0; c < t.m; ++c) {
+        const int j = c < t.npvpq ? t.pvpq[c] : t.pq[c - t.npvpq];
+        for (int pass = 0; pass < 2; ++pass)
+            for (int i : ycol[j]) {
+                const int r = pass == 0 ? t.row_p[i] : t.row_q[i];
+                if (r < 0) continue;
+                t.Ci.push_back(r);
+                t.Ce.push_back(entry(r, c));
+            }
+        t.Cp[c + 1] = (int)t.Ci.size();
+    }
+    if ((int)t.Ci.size() != t.nnzJ) throw std::runtime_error("Jacobian pattern: column and row views differ");
 }
 
 // Symmetric static-pivot LU: order, elimination tree, strict column patterns S_p,
@@ -1006,15 +1034,15 @@ void host_lu_emulate(const Topology& t, bool blocks, int nslots, bool single, co
 }
 
 // ------------------------------------------------------------------ entry points
+// min/max |pivot| of the static-pivot LU of the Jacobian at flat start, or 0 when a pivot is zero
+// or non-finite (the frozen order is then not used: p3s/cpp/nr_klu.cpp:718-729 leaves the lean plan
+// invalid and every solve factors with pivoting)
 static double reference_pivot_ratio(const Topology& t) {
     std::vector<double> Vm(t.n, 1.0), Va(t.n, 0.0), S(2 * (size_t)t.n, 0.0), R(t.m), J(t.nnzJ), fac(t.nnz_lu);
     host_eval(t, Vm.data(), Va.data(), nullptr, nullptr, S.data(), nullptr, R.data(), J.data());
     double lo, hi, sum;
     host_factor(t, J.data(), fac.data(), &lo, &hi, &sum);
-    if (!std::isfinite(sum) || !(lo > 0.0) || !(hi > 0.0))
-        throw std::runtime_error(
-            "static-pivot LU: the Jacobian has a zero or non-finite pivot at flat start; the frozen pivot order "
-            "cannot be used for this grid");
+    if (!std::isfinite(sum) || !(lo > 0.0) || !(hi > 0.0)) return 0.0;
     return lo / hi;
 }
 
@@ -1026,11 +1054,13 @@ void build_topology(Topology& t, const int* Yp, const int* Yj, const double* Yx,
     build_lu(t, ordering);
     build_schedule(t, nslots);
     t.rcond0 = reference_pivot_ratio(t);
+    t.lean_valid = t.rcond0 > 0.0;
 }
 
 void update_values(Topology& t, const double* Yx) {
     for (int k = 0; k < t.nnzY; ++k) { t.G[k] = Yx[2 * k]; t.B[k] = Yx[2 * k + 1]; }
     t.rcond0 = reference_pivot_ratio(t);
+    t.lean_valid = t.rcond0 > 0.0;
 }
 
 }  // namespace p3smi

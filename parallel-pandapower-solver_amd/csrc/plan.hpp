@@ -124,6 +124,10 @@ struct Topology {
     std::vector<int> row_p, row_q;        // bus -> Jacobian row of its P / Q equation, -1 = none
     // ---- Jacobian pattern, CSR by equation, columns ascending
     std::vector<int> Jp, Jj, Jk, Jb;      // entry -> (column, Ybus nonzero, block)
+    // ---- the same pattern by column, in the order of p3s/cpp/nr_klu.cpp:230-285 (within a
+    // column the P rows, then the Q rows, each in ascending bus order): Cp (m + 1), Ci the row and
+    // Ce the CSR entry of every column slot (the pivoting factorization reads columns this way)
+    std::vector<int> Cp, Ci, Ce;
     std::vector<int> posJ;                // (4 * nnzY) Jacobian entry of block b of nonzero k, -1 = none
     // ---- static-pivot LU plan
     std::vector<int> perm, pinv;          // perm[p] = Jacobian row/column eliminated p-th
@@ -138,6 +142,8 @@ struct Topology {
     int64_t nnz_lu = 0, n_pairs = 0;
     Schedule sched;
     double rcond0 = 0.0;                  // min/max |pivot| of the intact grid at flat start
+    bool lean_valid = true;               // false: the flat-start Jacobian has a zero or non-finite
+                                          // static pivot; every solve then factors with pivoting
     int ordering = 0;
 };
 

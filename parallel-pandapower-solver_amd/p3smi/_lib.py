@@ -69,12 +69,12 @@ class Info(C.Structure):
 class Stats(C.Structure):
     _fields_ = [
         ("cases", C.c_int32), ("newton_passes", C.c_int32), ("line_search_passes", C.c_int32),
-        ("failed_factorizations", C.c_int32), ("guard_trips", C.c_int32), ("reserved", C.c_int32),
+        ("failed_factorizations", C.c_int32), ("guard_trips", C.c_int32), ("reruns", C.c_int32),
         ("single_factorizations", C.c_int64), ("double_factorizations", C.c_int64), ("refinement_steps", C.c_int64),
         ("ms_eval", C.c_double), ("ms_lu", C.c_double), ("ms_step", C.c_double), ("ms_decide", C.c_double),
         ("ms_other", C.c_double), ("ms_refine", C.c_double),
         ("n_eval", C.c_int32), ("n_lu", C.c_int32), ("n_step", C.c_int32), ("n_decide", C.c_int32),
-        ("n_refine", C.c_int32), ("reserved2", C.c_int32),
+        ("n_refine", C.c_int32), ("pivot_factorizations", C.c_int32),
     ]
 
 
@@ -82,7 +82,7 @@ class Stats(C.Structure):
 EXPORTS = [
     "p3smi_last_error", "p3smi_version", "p3smi_device_count", "p3smi_device_memory",
     "p3smi_topology_create", "p3smi_topology_free", "p3smi_topology_info", "p3smi_update_y",
-    "p3smi_topology_set_lu_shape", "p3smi_topology_set_eval_tile", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_max_chunk", "p3smi_topology_set_mixed",
+    "p3smi_topology_set_lu_shape", "p3smi_topology_set_eval_tile", "p3smi_topology_set_guard", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_max_chunk", "p3smi_topology_set_mixed",
     "p3smi_host_jacobian", "p3smi_host_factor_solve", "p3smi_host_lu_emulate", "p3smi_host_maps",
     "p3smi_solve_batch", "p3smi_solve_batch_contingency", "p3smi_solve_batch_contingency_stamp",
     "p3smi_outages_branch_flows", "p3smi_outages_voltage_violations",
@@ -118,6 +118,7 @@ def lib():
     L.p3smi_topology_set_mixed_switch.argtypes = [P, C.c_double]
     L.p3smi_topology_set_lu_shape.argtypes = [P, _i32, _i32, _i32, _i32, _i32, _i32]
     L.p3smi_topology_set_eval_tile.argtypes = [P, _i32]
+    L.p3smi_topology_set_guard.argtypes = [P, _f64]
     L.p3smi_topology_set_mixed.argtypes = [P, _f64, _i32]
     L.p3smi_host_jacobian.argtypes = [P, P, P, P, P, P, P]
     L.p3smi_host_factor_solve.argtypes = [P, P, P, P, P]

@@ -115,8 +115,16 @@ class Solver:
 
     @property
     def lean_active(self) -> bool:
-        """True: every factorization runs on the frozen static-pivot plan."""
-        return True
+        """True when solves factor on the frozen static-pivot plan (p3s/cpp/nr_klu.cpp:1308): its
+        flat-start factorization had no zero or non-finite pivot. Otherwise every case factors
+        with pivoting."""
+        return bool(self.info["rcond0"] > 0.0)
+
+    def set_guard(self, ratio: float = 1e-3):
+        """Pivot-ratio guard of the frozen plan (REUSE_RCOND_RATIO, p3s/cpp/nr_klu.cpp:564-566):
+        a factorization whose pivot ratio falls below ratio x the flat-start one switches the
+        case to the pivoting factorization; 1e30 rejects every frozen factorization."""
+        check(lib().p3smi_topology_set_guard(self._h, float(ratio)))
 
     @property
     def m(self) -> int:
