@@ -114,6 +114,15 @@ int p3smi_topology_set_mixed(p3smi_topology* t, double rtol, int32_t kmax);
  * is run again from its start with pivoting (p3s/cpp/nr_klu.cpp:844-857). Default 1e-3; 1e30
  * rejects every frozen factorization (the reference's tests/cuda/test_lean_lu.py:224-244). */
 int p3smi_topology_set_guard(p3smi_topology* t, double ratio);
+/* DC-power-flow start on the device (p3s/timeseries.py:200-244 with the reduced solve of
+ * p3s/NewtonPowerflowCpp.py:422-440): solves B'(pvpq, pvpq) theta = P(pvpq) - B'(pvpq, ref)
+ * theta_ref with the engine's static-pivot LU, theta_ref the angles of V_flat at the reference
+ * buses, and returns V_out (n complex) = V_flat with |V_flat| e^(j theta) at pv and pq buses (a
+ * singular system gives angle 0; a non-finite result returns V_flat). Bdc (nnz): the DC
+ * susceptances (diagonal + sum 1/x, off-diagonal -1/x) on the Ybus CSR pattern, zero where no
+ * branch stamps; P (n): the active injections; ref (nref): reference buses. */
+int p3smi_dc_start(p3smi_topology* t, const double* Bdc, const double* P, const double* V_flat, int32_t nref,
+                   const int32_t* ref, double* V_out);
 /* Refresh Ybus values on the same pattern: nr_klu.Solver.update_Y (p3s/cpp/nr_klu.cpp:999-1011). */
 int p3smi_update_y(p3smi_topology* t, const double* Yx, int32_t nnz);
 

@@ -2369,6 +2369,7 @@ struct p3smi_batch {
     bool graphs = false;                // replay captured Newton iterations (FP64, timers off)
     std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gcache;
     PivotPool piv;                      // pivoting factorizations (allocated on first use)
+    double guard_override = -1.0;       // >= 0: the frozen plan's guard ratio for this batch's LU
     p3smi_stats stats{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
     size_t ev_used = 0;
@@ -2793,7 +2794,7 @@ void launch_evaluate(p3smi_batch* b, int hold) {
 template <int G, int NS, typename real, bool FACTOR>
 void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* rhs, double* xout, int sel) {
     const p3smi::Topology& h = b->topo->host;
-    const double rcond_min = b->topo->guard_ratio * h.rcond0;
+    const double rcond_min = (b->guard_override >= 0.0 ? b->guard_override : b->topo->guard_ratio) * h.rcond0;
     if constexpr (G == 64) {
         if (ds.blk) {
             hipLaunchKernelGGL((k_lu_blk<NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
@@ -3460,6 +3461,75 @@ int p3smi_topology_set_eval_tile(p3smi_topology* t, int32_t rows) {
         if (rows < 1 || rows > 1024) throw Error(P3SMI_E_ARG, "set_eval_tile: rows must be in [1, 1024]");
         std::lock_guard<std::mutex> lk(t->mu);
         t->eval_rows = rows;
+    });
+}
+
+int p3smi_dc_start(p3smi_topology* t, const double* Bdc, const double* P, const double* V_flat, int32_t nref,
+                   const int32_t* ref, double* V_out) {
+    return guarded([&] {
+        if (!t || !Bdc || !P || !V_flat || !V_out || (nref > 0 && !ref) || nref < 0)
+            throw Error(P3SMI_E_ARG, "dc_start: null argument");
+        std::lock_guard<std::mutex> lk(t->mu);
+        const p3smi::Topology& h = t->host;
+        const int n = h.n;
+        std::vector<double> theta(n, 0.0);
+        std::vector<char> is_ref(n, 0);
+        for (int q = 0; q < nref; ++q) {
+            if (ref[q] < 0 || ref[q] >= n) throw Error(P3SMI_E_ARG, "dc_start: reference bus out of range");
+            is_ref[ref[q]] = 1;
+            theta[ref[q]] = std::atan2(V_flat[2 * (size_t)ref[q] + 1], V_flat[2 * (size_t)ref[q]]);
+        }
+        // the matrix [B'(pvpq, pvpq) 0; 0 I] on the Jacobian's pattern and the right-hand side
+        // P(pvpq) - B'(pvpq, ref) theta_ref (p3s/NewtonPowerflowCpp.py:422-440)
+        auto bus_of = [&](int r) { return r < h.npvpq ? h.pvpq[r] : h.pq[r - h.npvpq]; };
+        std::vector<double> Jv(h.nnzJ), R(h.m, 0.0);
+        for (int r = 0; r < h.m; ++r)
+            for (int e = h.Jp[r]; e < h.Jp[r + 1]; ++e) {
+                const int blk = h.Jb[e];
+                Jv[e] = blk == 0 ? Bdc[h.Jk[e]] : (blk == 3 && bus_of(r) == bus_of(h.Jj[e])) ? 1.0 : 0.0;
+            }
+        for (int r = 0; r < h.npvpq; ++r) {
+            const int i = h.pvpq[r];
+            double rhs = P[i];
+            for (int k = h.Yp[i]; k < h.Yp[i + 1]; ++k)
+                if (is_ref[h.Yj[k]]) rhs -= Bdc[k] * theta[h.Yj[k]];
+            R[r] = rhs;
+        }
+        // one static-pivot factorization and solve on a one-case batch (B' is symmetric positive
+        // definite on a grid whose islands all hold a reference, so no pivot ratio guard applies)
+        p3smi_batch b;
+        batch_alloc(&b, t, 1);
+        b.guard_override = 0.0;
+        hipStream_t st = t->stream;
+        const size_t pitch = (size_t)b.S * sizeof(double);
+        HIPCHECK(hipMemcpy2DAsync(b.Jx.p, pitch, Jv.data(), sizeof(double), sizeof(double), h.nnzJ,
+                                  hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpy2DAsync(b.R.p, pitch, R.data(), sizeof(double), sizeof(double), h.m, hipMemcpyHostToDevice, st));
+        const unsigned w = SG_STEP;
+        HIPCHECK(hipMemcpyAsync(b.ctl.p, &w, sizeof w, hipMemcpyHostToDevice, st));
+        launch_lu<double, true>(&b, 1, b.fac.p, b.R.p, b.x.p, SEL_ALL);
+        std::vector<double> x(h.m);
+        unsigned w1 = 0;
+        HIPCHECK(hipMemcpy2DAsync(x.data(), sizeof(double), b.x.p, pitch, sizeof(double), h.m, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(&w1, b.ctl.p, sizeof w1, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        const bool singular = (w1 & CTL_PENDING) != 0;  // a zero or non-finite pivot
+        // angles at pv and pq buses (a singular solve gives NaN, taken as 0 as numpy's
+        // nan_to_num does, p3s/timeseries.py:200-244), magnitudes of the start kept
+        for (int i = 0; i < 2 * n; ++i) V_out[i] = V_flat[i];
+        bool finite = true;
+        for (int r = 0; r < h.npvpq; ++r) {
+            const int i = h.pvpq[r];
+            double th = singular ? 0.0 : x[h.pinv[r]];
+            if (std::isnan(th)) th = 0.0;
+            else if (std::isinf(th)) th = th > 0 ? DBL_MAX : -DBL_MAX;
+            const double vm = std::hypot(V_flat[2 * (size_t)i], V_flat[2 * (size_t)i + 1]);
+            V_out[2 * (size_t)i] = vm * std::cos(th);
+            V_out[2 * (size_t)i + 1] = vm * std::sin(th);
+            finite = finite && std::isfinite(V_out[2 * (size_t)i]) && std::isfinite(V_out[2 * (size_t)i + 1]);
+        }
+        if (!finite)
+            for (int i = 0; i < 2 * n; ++i) V_out[i] = V_flat[i];
     });
 }
 

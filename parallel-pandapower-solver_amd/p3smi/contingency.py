@@ -159,7 +159,7 @@ def solve_contingencies(grid, reslack: bool = False, tol: float = 1e-8, max_iter
 
     cs = from_grid(grid, groups=groups, reslack=reslack)
     if init == "dc":
-        v_start = dc_initial_voltage(grid)
+        v_start = dc_initial_voltage(grid, solver=cs.solver)  # the DC solve on the device
     elif init == "flat":
         v_start = grid.flat_start()
     else:

@@ -137,6 +137,34 @@ class Solver:
         self.Yx = Yx
         self.info = self._info()
 
+    def dc_start(self, Bdc, P, V_flat, ref) -> np.ndarray:
+        """DC-power-flow start voltage computed on the device: B'(pvpq, pvpq) theta = P(pvpq) -
+        B'(pvpq, ref) theta_ref solved with the engine's LU, |V_flat| kept at pv and pq buses
+        (p3s/timeseries.py:200-244). Bdc: the DC susceptance matrix (n, n), its nonzeros within
+        the Ybus pattern."""
+        B = sp.csr_matrix(Bdc, dtype=np.float64)
+        B.sum_duplicates()
+        B.sort_indices()
+        if B.shape != (self.n, self.n):
+            raise RuntimeError("dc_start: Bdc must be (n, n)")
+        n = np.int64(self.n)
+        ykeys = np.repeat(np.arange(self.n, dtype=np.int64), np.diff(self.Yp)) * n + self.Yj
+        bkeys = np.repeat(np.arange(self.n, dtype=np.int64), np.diff(B.indptr)) * n + B.indices
+        pos = np.searchsorted(ykeys, bkeys)
+        if bkeys.size and ((pos >= ykeys.size).any() or (ykeys[np.minimum(pos, ykeys.size - 1)] != bkeys).any()):
+            raise RuntimeError("dc_start: Bdc has a nonzero outside the Ybus pattern")
+        vals = np.zeros(self.nnz, dtype=np.float64)
+        vals[pos] = B.data
+        Pv = np.ascontiguousarray(P, dtype=np.float64).ravel()
+        Vf = _c128(V_flat).ravel()
+        ref = np.ascontiguousarray(ref, dtype=np.int32).ravel()
+        if Pv.shape[0] != self.n or Vf.shape[0] != self.n:
+            raise RuntimeError("dc_start: P and V_flat must be (n,)")
+        out = np.empty(self.n, dtype=np.complex128)
+        check(lib().p3smi_dc_start(self._h, ptr(vals), ptr(Pv), ptr(Vf), int(ref.size), ptr(ref), ptr(out)),
+              "dc_start: ")
+        return out
+
     # ---------------------------------------------------------------- solves
     def solve(self, Sbus, V0, max_iter: int = 30, tol: float = 1e-8, line_search: bool = True,
               precision: str = "double"):

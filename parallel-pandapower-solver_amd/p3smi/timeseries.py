@@ -6,7 +6,8 @@ p3s/NewtonPowerflowCpp.py:422-440): solve B[pvpq, pvpq] theta = P[pvpq] -
 B[pvpq, ref] theta_ref with the DC susceptance matrix of the branches (diagonal
 +sum 1/x, off-diagonal -1/x, p3s/models/TransmissionLineModel.py:146-148), keep the start
 magnitudes at PV and PQ buses and take the DC angles; fall back to the flat start when
-the DC angles are not finite. It is a one-off host step per topology, like the reference's.
+the DC angles are not finite. Given a solver, the solve runs on the device
+(``Solver.dc_start``: the engine's own LU on the DC matrix); otherwise with scipy on the host.
 """
 
 from __future__ import annotations
@@ -54,10 +55,13 @@ def dc_initial_voltage_arrays(Bdc, P, V_flat, ref, pv, pq) -> np.ndarray:
     return voltage
 
 
-def dc_initial_voltage(grid, Sbus=None) -> np.ndarray:
-    """DC start voltage of a synthetic grid (base-case injections unless ``Sbus`` given)."""
+def dc_initial_voltage(grid, Sbus=None, solver=None) -> np.ndarray:
+    """DC start voltage of a synthetic grid (base-case injections unless ``Sbus`` given): on
+    the device through ``solver`` (an ``nr_klu.Solver`` of the grid), else on the host."""
     S = grid.sbus() if Sbus is None else np.asarray(Sbus)
     Bdc = dc_susceptance(grid.n, grid.f, grid.t, grid.x)
+    if solver is not None:
+        return solver.dc_start(Bdc, S.real, grid.flat_start(), grid.ref)
     return dc_initial_voltage_arrays(Bdc, S.real, grid.flat_start(), grid.ref, grid.pv, grid.pq)
 
 
