@@ -72,7 +72,8 @@ def measured_traffic(kernel: str, workload: str, precision: str):
 def stats_acc():
     return {"ms_eval": 0.0, "ms_lu": 0.0, "ms_step": 0.0, "ms_decide": 0.0, "ms_other": 0.0, "ms_refine": 0.0,
             "n_eval": 0, "n_lu": 0, "n_step": 0, "n_decide": 0, "n_refine": 0, "double_factorizations": 0,
-            "single_factorizations": 0, "refinement_steps": 0, "newton_passes": 0, "line_search_passes": 0}
+            "single_factorizations": 0, "refinement_steps": 0, "newton_passes": 0, "line_search_passes": 0,
+            "ms_lu_batch": 0.0, "n_lu_batch": 0, "lu_batch_cases": 0}
 
 
 def add_stats(acc, L, bh):
@@ -85,29 +86,44 @@ def add_stats(acc, L, bh):
 
 
 def roofline(acc, bpc, steps, workload, precision):
-    """Dominant kernel's achieved algorithmic GB/s over its event-timed launches."""
+    """Dominant kernel's achieved algorithmic GB/s over its event-timed launches.
+
+    In an FP64 solve that is the LU's factor + solve launch of the batch shape, timed apart by
+    the engine (ms_lu_batch over n_lu_batch launches, lu_batch_cases factorizations in them):
+    bytes per case x cases in those launches / their time. The launches of the few-cases shape
+    (the last stragglers of a solve) are in kernel_ms_per_step only. In mixed precision the LU
+    launches also carry the refinement solves, which the byte count leaves out."""
+    batch = precision == "double" and acc["n_lu_batch"] > 0
     kern = "lu" if acc["ms_lu"] >= acc["ms_eval"] else "eval"
-    # one Jacobian evaluation and one factorization per case-iteration; in mixed precision the
-    # LU launches also carry the refinement solves, which the byte count leaves out
-    facts = acc["double_factorizations"] + acc["single_factorizations"]
-    bytes_total = facts * (bpc[1] if kern == "lu" else bpc[0])
-    ms = acc[f"ms_{kern}"]
-    n_launch = acc[f"n_{kern}"]
+    if kern == "lu" and batch:
+        bytes_total = acc["lu_batch_cases"] * bpc[1]
+        ms, n_launch = acc["ms_lu_batch"], acc["n_lu_batch"]
+    else:
+        facts = acc["double_factorizations"] + acc["single_factorizations"]
+        bytes_total = facts * (bpc[1] if kern == "lu" else bpc[0])
+        ms, n_launch = acc[f"ms_{kern}"], acc[f"n_{kern}"]
     achieved = bytes_total / (ms / 1e3) / 1e9 if ms > 0 else 0.0
-    traffic = measured_traffic(f"k_{kern}", workload, precision)
-    return {
-        "kernel": f"k_{kern}",
+    name = "k_lu_batch" if kern == "lu" and batch else f"k_{kern}"
+    traffic = measured_traffic(name, workload, precision)
+    avg_ms = ms / max(n_launch, 1)
+    out = {
+        "kernel": name,
         "bound": "hbm",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "avg_launch_ms": round(ms / max(n_launch, 1), 4),
+        "avg_launch_ms": round(avg_ms, 4),
+        "launches": int(n_launch),
+        "cases_per_launch": round(bytes_total / bpc[1 if kern == "lu" else 0] / max(n_launch, 1), 1),
         "bytes_per_case": float(bpc[1] if kern == "lu" else bpc[0]),
         "kernel_ms_per_step": {k: round(acc[f"ms_{k}"] / steps, 3)
                                for k in ("eval", "lu", "refine", "step", "decide", "other")},
     }
+    if traffic:
+        out["traffic_GB_per_s"] = round(traffic / (avg_ms / 1e3) / 1e9, 1)
+    return out
 
 
 def base_out(args, value, times, t_total, precision, config):
@@ -274,8 +290,16 @@ def run_timeseries(args) -> dict:
         add_stats(acc, L, bh)
         log(f"[bench] step {k + 1}/{args.steps}: {1e3 * times[-1]:.1f} ms")
     t_total = time.perf_counter() - t_all0
-    # the same steps with each Newton iteration replayed as a captured hipGraph (timers off)
     _lib.check(L.p3smi_batch_set_timing(bh, 0))
+    if args.main_only:  # for profiling: the timed steps and nothing else
+        bpc = np.empty(4, dtype=np.float64)
+        L.p3smi_batch_bytes_per_case(bh, _lib.ptr(bpc))
+        out = base_out(args, T * args.steps / t_total, times, t_total, args.precision,
+                       {"workload": f"{args.config}: {desc}, timed steps only"})
+        out["roofline"] = roofline(acc, bpc, args.steps, args.config, args.precision)
+        L.p3smi_batch_free(bh)
+        return out
+    # the same steps with each Newton iteration replayed as a captured hipGraph (timers off)
     _lib.check(L.p3smi_batch_set_graphs(bh, 1))
     solve()
     tg0 = time.perf_counter()
@@ -370,7 +394,8 @@ def run_n1(args) -> dict:
     args.warmup = max(1, args.warmup)
     for w in range(args.warmup):
         solve(code[main])
-        solve(code[other])
+        if not args.main_only:
+            solve(code[other])
         log(f"[bench] warmup {w + 1}/{args.warmup} done")
     bh = C.c_void_p()
     _lib.check(L.p3smi_topology_last_batch(solver._h, C.byref(bh)))
@@ -389,12 +414,18 @@ def run_n1(args) -> dict:
         return acc, times, time.perf_counter() - t_all0, int(cv.astype(bool).sum()), it.copy()
 
     acc, times, t_total, n_conv, it_main = timed(main)
+    bpc = np.empty(4, dtype=np.float64)
+    L.p3smi_batch_bytes_per_case(bh, _lib.ptr(bpc))
+    if args.main_only:  # for profiling: the main precision's timed steps and nothing else
+        _lib.check(L.p3smi_batch_set_timing(bh, 0))
+        out = base_out(args, n_conv * args.steps / t_total, times, t_total, main,
+                       {"workload": f"n1_10k: {Lc} single-branch outages, main precision only"})
+        out["roofline"] = roofline(acc, bpc, args.steps, "n1_10k", main)
+        return out
     acc2, times2, t_total2, n_conv2, it_other = timed(other)
     third = ({"double", "mixed", "mixed_switch"} - {main, other}).pop()
     solve(code[third])  # warm-up of the third precision
     acc3, _, t_total3, n_conv3, it_third = timed(third)
-    bpc = np.empty(4, dtype=np.float64)
-    L.p3smi_batch_bytes_per_case(bh, _lib.ptr(bpc))
     _lib.check(L.p3smi_batch_set_timing(bh, 0))
 
     # ratings for the post-processing screen: 125 % of each branch's loading in the intact
@@ -711,23 +742,19 @@ def run_serve64(args) -> dict:
     return out
 
 
-def calibrate_copy() -> int:
-    """One device-to-device copy of 1 GiB, three times: the reference reading for the HBM
-    traffic counters (FETCH_SIZE, WRITE_SIZE per dispatch against 1 GiB read and written)."""
-    hip = C.CDLL("libamdhip64.so")
+def calibrate() -> int:
+    """The HBM reference patterns of the library, 1 GiB read and 1 GiB written per launch,
+    three timed launches each after a warm-up: a linear copy and a copy of 512-byte rows
+    between permuted positions. Under rocprofv3 --pmc their FETCH_SIZE and WRITE_SIZE per
+    dispatch against the known 1 GiB give the scale of the counters for each access pattern."""
+    from p3smi import _lib
+
     n = 1 << 30
-    a, b = C.c_void_p(), C.c_void_p()
-    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
-        print("bench.py: hipMalloc failed", file=sys.stderr)
-        return 1
-    ok = hip.hipMemset(a, 1, C.c_size_t(n)) == 0
-    for _ in range(3):
-        ok = ok and hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0
-    ok = ok and hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
-    print(json.dumps({"calibration": "device copy", "bytes": n, "repeats": 3, "ok": bool(ok)}), flush=True)
-    return 0 if ok else 1
+    for pattern, name in ((0, "linear copy"), (1, "512-byte rows, permuted")):
+        ms = _lib.hbm_probe(pattern, n, 3)
+        print(json.dumps({"calibration": name, "bytes_read": n, "bytes_written": n, "ms_per_launch": round(ms, 4),
+                          "GB_per_s": round(2 * n / (ms / 1e3) / 1e9, 1)}), flush=True)
+    return 0
 
 
 def main() -> int:
@@ -741,13 +768,15 @@ def main() -> int:
     ap.add_argument("--precision", default="double", choices=["double", "mixed", "mixed_switch"],
                     help="precision of the line's value (n1_10k times the other two beside it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--main-only", action="store_true",
+                    help="time the main solve only (no compare legs, graph replay, screens or CPU baseline)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0,
                     help="target duration of one timed CPU baseline pass (best of 3 after a warm-up)")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only run a 1 GiB device copy (reference for the HBM traffic counters)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run the HBM reference patterns (scale of the traffic counters)")
     args = ap.parse_args()
-    if args.calibrate_copy:
-        return calibrate_copy()
+    if args.calibrate:
+        return calibrate()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported (use --gpus 1)",
               file=sys.stderr)

@@ -60,6 +60,11 @@ typedef struct {
     double ms_eval, ms_lu, ms_step, ms_decide, ms_other, ms_refine;
     int32_t n_eval, n_lu, n_step, n_decide, n_refine;
     int32_t pivot_factorizations; /* fresh pivoting factorizations (the frozen plan's fallback) */
+    /* FP64 factor + solve launches of the batch LU shape (part of ms_lu / n_lu) and the cases
+     * they factored: the dominant kernel of a whole-batch FP64 solve, timed apart */
+    double ms_lu_batch;
+    int32_t n_lu_batch;
+    int64_t lu_batch_cases;
 } p3smi_stats;
 
 /* ---- library ------------------------------------------------------------------------ */
@@ -70,6 +75,13 @@ int p3smi_device_count(int32_t* count);
 /* Free and total device memory in bytes (no counterpart in the reference; the serving
  * layer sizes its admission budget from it). Fails without a device. */
 int p3smi_device_memory(int64_t* free_bytes, int64_t* total_bytes);
+/* HBM reference patterns (no counterpart in the reference; the measurement of the kernels'
+ * HBM traffic scales the hardware counters by them). Allocates two device buffers of `bytes`
+ * (a power of two, 1 MiB .. 64 GiB), fills the source, then times `repeats` launches (after one
+ * warm-up) of pattern 0, a linear copy, or 1, a copy of 512-byte rows between permuted row
+ * positions (every source byte read once, every destination byte written once), and returns
+ * the average milliseconds per launch. */
+int p3smi_hbm_probe(int32_t pattern, int64_t bytes, int32_t repeats, double* ms_per_launch);
 
 /* ---- topology (replaces nr_klu.Solver.__init__, p3s/cpp/nr_klu.cpp:966-994, and
  *      PolarNewtonSolverCUDA.__init__, p3s/cuda/nr_polar_solver.py:282-376) ---------------
@@ -91,7 +103,8 @@ int p3smi_topology_set_mixed_switch(p3smi_topology* t, double mismatch);
  * chunks of nearly equal size, whole multiples of 64, solved one after the other. */
 int p3smi_topology_set_max_chunk(p3smi_topology* t, int32_t cases);
 /* LU launch shape (no counterpart in the reference; set by the autotuner): a workgroup of
- * 64*ns threads factors g consecutive cases (g in {1, 16, 32, 64}, ns in {8, 16});
+ * 64*ns threads factors g consecutive cases (g in {1, 16, 32, 64}, ns in {8, 16}, and ns = 4
+ * with g = 1);
  * (few_g, few_ns) is used instead while at most few_max cases are still iterating. group 0
  * with g = 64 treats 2x2 blocks of paired pivots (the P and Q unknowns of a PQ bus) as one
  * task; group > 0 deals runs of up to `group` entries of one pivot's U row / L column to one

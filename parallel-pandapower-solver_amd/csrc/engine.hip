@@ -570,10 +570,14 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     // fewer than 64 cases per workgroup: lists differ within a wavefront, so their indices and
     // addresses sit in vector registers -- shorter chunks keep 16 wavefronts within 128
     // registers: with one case per workgroup a quarter with FP64 factors, half with FP32 ones;
-    // half for FP32 factors with 16 or 32 cases per workgroup
-    constexpr int KDIV = G == 1 ? (SINGLE ? 2 : 4) : (G < 64 && SINGLE ? 2 : 1);
-    constexpr int K = lu_chunk<real, real>() / KDIV;
-    constexpr int KF = lu_chunk<real, double>() / KDIV;
+    // half for FP32 factors with 16 or 32 cases per workgroup. One case on 4 wavefronts (a
+    // straggler, a single solve) has registers to spare for chunks twice the batch length:
+    // the long Crout sums near the top of the elimination tree are serial chains whose time
+    // is the load latency per chunk, so the deeper chunks shorten the critical path.
+    constexpr int KDIV = G == 1 ? (NS == 4 ? 1 : (SINGLE ? 2 : 4)) : (G < 64 && SINGLE ? 2 : 1);
+    constexpr int KMUL = G == 1 && NS == 4 ? 2 : 1;
+    constexpr int K = lu_chunk<real, real>() * KMUL / KDIV;
+    constexpr int KF = lu_chunk<real, double>() * KMUL / KDIV;
     __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
     __shared__ int sbad[G];
     const int lane = threadIdx.x % G;
@@ -1182,6 +1186,15 @@ template <typename T>
 __global__ void k_zero(T* p, size_t n) {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i < n) p[i] = (T)0;
+}
+
+// One case's column of an interleaved array and a dense vector: put (dense -> column c of the
+// [item * S + c] layout) or take (column -> dense), n items.
+__global__ void k_column(const double* __restrict__ src, double* __restrict__ dst, int n, int S, int c, int take) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (take) dst[i] = src[(size_t)i * S + c];
+    else dst[(size_t)i * S + c] = src[i];
 }
 
 // Newton move of the cases this pass advances (p3s/cpp/nr_klu.cpp:665-700): STEP takes the full
@@ -2167,6 +2180,7 @@ struct Staging {
     static constexpr size_t MIN_BYTES = (size_t)16 << 20;  // smaller transfers copy directly
     char* buf[NSLOT] = {};
     hipEvent_t ev[NSLOT] = {};
+    DevBuf<char> xfer;  // device side of narrow-row transfers (h2d_rows, d2h_rows)
     Staging() = default;
     Staging(const Staging&) = delete;
     Staging& operator=(const Staging&) = delete;
@@ -2209,6 +2223,54 @@ void unpack_range(char* base, const char* packed, size_t ldb, size_t rowb, size_
     }
 }
 
+// Rows of `width` bytes between pageable host memory and the device (host pitch spitch or
+// dpitch). A 2D copy from or to pageable memory moves row by row, so rows narrower than
+// WIDE_ROW (a few cases' worth) are packed on the host, copied in one piece, and spread or
+// gathered on the device by a device-to-device 2D copy. Both return once the host side is
+// done with.
+constexpr size_t WIDE_ROW = 4096;
+
+void h2d_rows(Staging& sg, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows,
+              hipStream_t st) {
+    if (rows == 0 || width == 0) return;
+    if (width >= WIDE_ROW) {
+        HIPCHECK(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyHostToDevice, st));
+        return;
+    }
+    const size_t nb = width * rows;
+    std::vector<char> packed(nb);
+    pack_range(packed.data(), (const char*)src, spitch, width, 0, nb);
+    if (dpitch == width) {
+        HIPCHECK(hipMemcpyAsync(dst, packed.data(), nb, hipMemcpyHostToDevice, st));
+    } else {
+        if (sg.xfer.n < nb) sg.xfer.alloc(nb);
+        HIPCHECK(hipMemcpyAsync(sg.xfer.p, packed.data(), nb, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpy2DAsync(dst, dpitch, sg.xfer.p, width, width, rows, hipMemcpyDeviceToDevice, st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+void d2h_rows(Staging& sg, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows,
+              hipStream_t st) {
+    if (rows == 0 || width == 0) return;
+    if (width >= WIDE_ROW) {
+        HIPCHECK(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        return;
+    }
+    const size_t nb = width * rows;
+    std::vector<char> packed(nb);
+    if (spitch == width) {
+        HIPCHECK(hipMemcpyAsync(packed.data(), src, nb, hipMemcpyDeviceToHost, st));
+    } else {
+        if (sg.xfer.n < nb) sg.xfer.alloc(nb);
+        HIPCHECK(hipMemcpy2DAsync(sg.xfer.p, width, src, spitch, width, rows, hipMemcpyDeviceToDevice, st));
+        HIPCHECK(hipMemcpyAsync(packed.data(), sg.xfer.p, nb, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    unpack_range((char*)dst, packed.data(), dpitch, width, 0, nb);
+}
+
 // Column block [c0, c0 + cols) of a (rows, ld) host array of `elem` bytes -> device (rows, cols),
 // queued on `st`.
 void upload_cols(Staging& sg, void* dst, const void* src, size_t rows, size_t ld, size_t c0, size_t cols,
@@ -2216,7 +2278,8 @@ void upload_cols(Staging& sg, void* dst, const void* src, size_t rows, size_t ld
     const size_t rowb = cols * elem, ldb = ld * elem;
     const char* base = (const char*)src + c0 * elem;
     if (rows * rowb < Staging::MIN_BYTES || rowb > Staging::SLOT) {
-        HIPCHECK(hipMemcpy2DAsync(dst, rowb, base, ldb, rowb, rows, hipMemcpyHostToDevice, st));
+        if (rowb == ldb) HIPCHECK(hipMemcpyAsync(dst, base, rows * rowb, hipMemcpyHostToDevice, st));
+        else h2d_rows(sg, dst, rowb, base, ldb, rowb, rows, st);
         return;
     }
     sg.ensure();
@@ -2242,8 +2305,12 @@ void download_cols(Staging& sg, void* dst, const void* src, size_t rows, size_t 
     const size_t rowb = cols * elem, ldb = ld * elem;
     char* base = (char*)dst + c0 * elem;
     if (rows * rowb < Staging::MIN_BYTES || rowb > Staging::SLOT) {
-        HIPCHECK(hipMemcpy2DAsync(base, ldb, src, rowb, rowb, rows, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
+        if (rowb == ldb) {
+            HIPCHECK(hipMemcpyAsync(base, src, rows * rowb, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+        } else {
+            d2h_rows(sg, base, ldb, src, rowb, rowb, rows, st);
+        }
         return;
     }
     sg.ensure();
@@ -2318,6 +2385,7 @@ struct p3smi_topology {
     double guard_ratio = REUSE_RCOND_RATIO;  // frozen plan rejected below guard_ratio x rcond0
     hipStream_t stream = nullptr;
     std::unique_ptr<p3smi_batch> cached;  // workspace of the one-call entry points
+    std::unique_ptr<p3smi_batch> dc;      // one-case workspace of the DC start
     std::mutex mu;
     ~p3smi_topology();
 };
@@ -2408,6 +2476,7 @@ struct p3smi_outages {
 
 p3smi_topology::~p3smi_topology() {
     cached.reset();
+    dc.reset();
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -2711,7 +2780,8 @@ void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     HIPCHECK(hipStreamSynchronize(t->stream));
 }
 
-// kernel timing (optional): kinds 0 eval, 1 lu, 2 step, 3 decide, 4 other, 5 refine
+// kernel timing (optional): kinds 0 eval, 1 lu, 2 step, 3 decide, 4 other, 5 refine, 6 lu of the
+// batch shape in FP64
 struct KTimer {
     p3smi_batch* b;
     int kind;
@@ -2743,6 +2813,12 @@ void collect_timing(p3smi_batch* b) {
         switch (b->ev[i].first) {
             case 0: b->stats.ms_eval += ms; b->stats.n_eval++; break;
             case 1: b->stats.ms_lu += ms; b->stats.n_lu++; break;
+            case 6:
+                b->stats.ms_lu += ms;
+                b->stats.n_lu++;
+                b->stats.ms_lu_batch += ms;
+                b->stats.n_lu_batch++;
+                break;
             case 2: b->stats.ms_step += ms; b->stats.n_step++; break;
             case 3: b->stats.ms_decide += ms; b->stats.n_decide++; break;
             case 5: b->stats.ms_refine += ms; b->stats.n_refine++; break;
@@ -2813,7 +2889,7 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* 
 
 bool valid_lu_config(LuConfig c) {
     const bool g = c.G == 64 || c.G == 32 || c.G == 16 || c.G == 1;
-    const bool n = c.NS == 8 || c.NS == 16;
+    const bool n = c.NS == 8 || c.NS == 16 || (c.NS == 4 && c.G == 1);
     return g && n;
 }
 
@@ -2827,7 +2903,11 @@ void launch_lu(p3smi_batch* b, int n_active, real* fac, const double* rhs, doubl
     p3smi_topology* t = b->topo;
     const LuConfig cfg = lu_config_for(t, n_active);
     const DevSched& ds = get_sched(t, cfg);
-    KTimer tm(b, 1);
+    // FP64 factor + solve launches of the batch shape over every stepping case are timed apart
+    // (kind 6): the dominant kernel of a whole-batch FP64 solve, with their cases counted
+    const bool batch64 = FACTOR && std::is_same<real, double>::value && sel == SEL_ALL && n_active > t->lu_few_max;
+    if (batch64 && b->timing) b->stats.lu_batch_cases += n_active;
+    KTimer tm(b, batch64 ? 6 : 1);
 #define P3SMI_LU_CASE(GG, NN)                                                                \
     if (cfg.G == GG && cfg.NS == NN) {                                                       \
         launch_lu_cfg<GG, NN, real, FACTOR>(b, ds, fac, rhs, xout, sel);                     \
@@ -2835,7 +2915,7 @@ void launch_lu(p3smi_batch* b, int n_active, real* fac, const double* rhs, doubl
         return;                                                                              \
     }
 #define P3SMI_LU_G(GG) P3SMI_LU_CASE(GG, 8) P3SMI_LU_CASE(GG, 16)
-    P3SMI_LU_G(64) P3SMI_LU_G(32) P3SMI_LU_G(16) P3SMI_LU_G(1)
+    P3SMI_LU_G(64) P3SMI_LU_G(32) P3SMI_LU_G(16) P3SMI_LU_G(1) P3SMI_LU_CASE(1, 4)
 #undef P3SMI_LU_G
 #undef P3SMI_LU_CASE
     throw Error(P3SMI_E_ARG, "unsupported LU launch shape");
@@ -3374,6 +3454,80 @@ int p3smi_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
     });
 }
 
+// HBM reference patterns with known byte counts, for scaling the FETCH_SIZE / WRITE_SIZE
+// counters and for the bandwidth a pattern can reach. Rows are 512 bytes: one factor entry
+// (or one Jacobian entry, one voltage component) of a 64-case group, the unit the LU and the
+// evaluation move. Both kernels read every source byte once and write every destination byte
+// once.
+__global__ void __launch_bounds__(256) k_probe_fill(double2* p, size_t n2) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_double2((double)(i & 1023), 1.0);
+}
+
+__global__ void __launch_bounds__(256) k_probe_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                    size_t n2) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// One wavefront per row: row r of dst takes row (r * A) mod R of src and lands at row
+// (r * B) mod R; A and B are odd and R a power of two, so both maps are permutations and
+// consecutive rows fall far apart.
+__global__ void __launch_bounds__(256) k_probe_rows(const double* __restrict__ src, double* __restrict__ dst,
+                                                    uint64_t rows) {
+    const uint64_t mask = rows - 1;
+    const int lane = threadIdx.x & 63;
+    for (uint64_t r = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; r < rows;
+         r += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
+        const uint64_t a = (r * 0x9E3779B1ull) & mask, b = (r * 0x85EBCA77ull) & mask;
+        dst[b * 64 + lane] = src[a * 64 + lane];
+    }
+}
+
+int p3smi_hbm_probe(int32_t pattern, int64_t bytes, int32_t repeats, double* ms_per_launch) {
+    return guarded([&] {
+        if (!ms_per_launch) throw Error(P3SMI_E_ARG, "hbm_probe: null argument");
+        if (pattern != 0 && pattern != 1) throw Error(P3SMI_E_ARG, "hbm_probe: pattern must be 0 (copy) or 1 (rows)");
+        if (bytes < (1 << 20) || bytes > (int64_t(1) << 36) || (bytes & (bytes - 1)))
+            throw Error(P3SMI_E_ARG, "hbm_probe: bytes must be a power of two in [1 MiB, 64 GiB]");
+        if (repeats < 1 || repeats > 1000) throw Error(P3SMI_E_ARG, "hbm_probe: repeats must be in [1, 1000]");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw Error(P3SMI_E_DEVICE, "hbm_probe: no HIP device available");
+        DevBuf<double> src, dst;
+        const size_t nd = (size_t)bytes / sizeof(double);
+        src.alloc(nd);
+        dst.alloc(nd);
+        hipStream_t st = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        std::unique_ptr<void, void (*)(void*)> st_guard(st, [](void* s) { (void)hipStreamDestroy((hipStream_t)s); });
+        HIPCHECK(hipEventCreate(&e0));
+        std::unique_ptr<void, void (*)(void*)> e0_guard(e0, [](void* e) { (void)hipEventDestroy((hipEvent_t)e); });
+        HIPCHECK(hipEventCreate(&e1));
+        std::unique_ptr<void, void (*)(void*)> e1_guard(e1, [](void* e) { (void)hipEventDestroy((hipEvent_t)e); });
+        const size_t n2 = nd / 2;
+        const uint64_t rows = (uint64_t)bytes / 512;
+        k_probe_fill<<<4096, 256, 0, st>>>((double2*)src.p, n2);
+        HIPCHECK(hipGetLastError());
+        auto launch = [&] {
+            if (pattern == 0)
+                k_probe_copy<<<8192, 256, 0, st>>>((const double2*)src.p, (double2*)dst.p, n2);
+            else
+                k_probe_rows<<<8192, 256, 0, st>>>(src.p, dst.p, rows);
+        };
+        launch();  // warm-up
+        HIPCHECK(hipEventRecord(e0, st));
+        for (int i = 0; i < repeats; ++i) launch();
+        HIPCHECK(hipEventRecord(e1, st));
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_launch = (double)ms / repeats;
+    });
+}
+
 int p3smi_topology_create(const int32_t* Yp, const int32_t* Yj, const double* Yx, int32_t n, int32_t nnz,
                           const int32_t* pv, int32_t npv, const int32_t* pq, int32_t npq, int32_t ordering,
                           int32_t n_slots, p3smi_topology** out) {
@@ -3497,20 +3651,35 @@ int p3smi_dc_start(p3smi_topology* t, const double* Bdc, const double* P, const 
         }
         // one static-pivot factorization and solve on a one-case batch (B' is symmetric positive
         // definite on a grid whose islands all hold a reference, so no pivot ratio guard applies)
-        p3smi_batch b;
-        batch_alloc(&b, t, 1);
+        // (the workspace is kept for later calls; the dense vectors go up and down in one copy
+        // each and are put into / taken from column 0 on the device)
+        if (!t->dc) {
+            std::unique_ptr<p3smi_batch> nb(new p3smi_batch());
+            batch_alloc(nb.get(), t, 1);
+            t->dc = std::move(nb);
+        }
+        p3smi_batch& b = *t->dc;
         b.guard_override = 0.0;
         hipStream_t st = t->stream;
-        const size_t pitch = (size_t)b.S * sizeof(double);
-        HIPCHECK(hipMemcpy2DAsync(b.Jx.p, pitch, Jv.data(), sizeof(double), sizeof(double), h.nnzJ,
-                                  hipMemcpyHostToDevice, st));
-        HIPCHECK(hipMemcpy2DAsync(b.R.p, pitch, R.data(), sizeof(double), sizeof(double), h.m, hipMemcpyHostToDevice, st));
+        const int S = (int)b.S;
+        DevBuf<double> dense;
+        dense.alloc((size_t)std::max(h.nnzJ, h.m));
+        auto put = [&](const std::vector<double>& v, double* col) {
+            HIPCHECK(hipMemcpyAsync(dense.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_column, dim3((unsigned)((v.size() + 255) / 256)), dim3(256), 0, st, dense.p, col,
+                               (int)v.size(), S, 0, 0);
+            HIPCHECK(hipGetLastError());
+        };
+        put(Jv, b.Jx.p);
+        put(R, b.R.p);
         const unsigned w = SG_STEP;
         HIPCHECK(hipMemcpyAsync(b.ctl.p, &w, sizeof w, hipMemcpyHostToDevice, st));
         launch_lu<double, true>(&b, 1, b.fac.p, b.R.p, b.x.p, SEL_ALL);
         std::vector<double> x(h.m);
         unsigned w1 = 0;
-        HIPCHECK(hipMemcpy2DAsync(x.data(), sizeof(double), b.x.p, pitch, sizeof(double), h.m, hipMemcpyDeviceToHost, st));
+        hipLaunchKernelGGL(k_column, dim3((unsigned)((h.m + 255) / 256)), dim3(256), 0, st, b.x.p, dense.p, h.m, S, 0, 1);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(x.data(), dense.p, (size_t)h.m * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipMemcpyAsync(&w1, b.ctl.p, sizeof w1, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         const bool singular = (w1 & CTL_PENDING) != 0;  // a zero or non-finite pivot
@@ -3831,6 +4000,9 @@ static void add_stats(p3smi_stats& a, const p3smi_stats& s) {
     a.n_step += s.n_step;
     a.n_decide += s.n_decide;
     a.n_refine += s.n_refine;
+    a.ms_lu_batch += s.ms_lu_batch;
+    a.n_lu_batch += s.n_lu_batch;
+    a.lu_batch_cases += s.lu_batch_cases;
 }
 
 static int solve_chunked(p3smi_topology* t, int T, bool per_case_y, bool pin_given, bool per_case_s,
@@ -3914,7 +4086,7 @@ static void contingency_common(p3smi_topology* t, const double* S, const double*
             // pin (n, L) -> device (n, S): zero the padding columns
             b->pin.alloc((size_t)h.n * b->S);
             HIPCHECK(hipMemsetAsync(b->pin.p, 0, (size_t)h.n * b->S, st));
-            HIPCHECK(hipMemcpy2DAsync(b->pin.p, b->S, pin + c0, L, nc, h.n, hipMemcpyHostToDevice, st));
+            h2d_rows(b->stage, b->pin.p, b->S, pin + c0, L, nc, h.n, st);
         }
     };
     solve_chunked(t, L, true, pin != nullptr, false, up, max_iter, tol, line_search, precision, V_out, iters, conv,
@@ -4231,7 +4403,7 @@ int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start
             b->has_pin = true;
             b->mask_unserved = true;
             if (served) {
-                HIPCHECK(hipMemcpy2DAsync(served + c0, L, b->served.p, b->S, nc, h.n, hipMemcpyDeviceToHost, st));
+                d2h_rows(b->stage, served + c0, L, b->served.p, b->S, nc, h.n, st);
             }
         };
         o->res_batch = nullptr;

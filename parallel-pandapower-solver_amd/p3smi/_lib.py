@@ -75,12 +75,13 @@ class Stats(C.Structure):
         ("ms_other", C.c_double), ("ms_refine", C.c_double),
         ("n_eval", C.c_int32), ("n_lu", C.c_int32), ("n_step", C.c_int32), ("n_decide", C.c_int32),
         ("n_refine", C.c_int32), ("pivot_factorizations", C.c_int32),
+        ("ms_lu_batch", C.c_double), ("n_lu_batch", C.c_int32), ("lu_batch_cases", C.c_int64),
     ]
 
 
 # every symbol include/p3smi.h declares
 EXPORTS = [
-    "p3smi_last_error", "p3smi_version", "p3smi_device_count", "p3smi_device_memory",
+    "p3smi_last_error", "p3smi_version", "p3smi_device_count", "p3smi_device_memory", "p3smi_hbm_probe",
     "p3smi_topology_create", "p3smi_topology_free", "p3smi_topology_info", "p3smi_update_y",
     "p3smi_topology_set_lu_shape", "p3smi_topology_set_eval_tile", "p3smi_topology_set_guard", "p3smi_dc_start", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_max_chunk", "p3smi_topology_set_mixed",
     "p3smi_host_jacobian", "p3smi_host_factor_solve", "p3smi_host_lu_emulate", "p3smi_host_maps",
@@ -110,6 +111,7 @@ def lib():
     L.p3smi_version.restype = C.c_int
     L.p3smi_device_count.argtypes = [P]
     L.p3smi_device_memory.argtypes = [P, P]
+    L.p3smi_hbm_probe.argtypes = [_i32, C.c_int64, _i32, P]
     L.p3smi_topology_create.argtypes = [P, P, P, _i32, _i32, P, _i32, P, _i32, _i32, _i32, P]
     L.p3smi_topology_free.argtypes = [P]
     L.p3smi_topology_free.restype = None
@@ -186,3 +188,11 @@ def device_memory() -> tuple:
     f, t = C.c_int64(0), C.c_int64(0)
     check(lib().p3smi_device_memory(C.byref(f), C.byref(t)), "device_memory: ")
     return int(f.value), int(t.value)
+
+
+def hbm_probe(pattern: int, nbytes: int, repeats: int = 3) -> float:
+    """Average milliseconds per launch of an HBM reference pattern over two buffers of
+    ``nbytes`` (0: linear copy, 1: 512-byte rows between permuted positions)."""
+    ms = C.c_double(0.0)
+    check(lib().p3smi_hbm_probe(int(pattern), int(nbytes), int(repeats), C.byref(ms)), "hbm_probe: ")
+    return float(ms.value)

@@ -23,7 +23,7 @@ import numpy as np
 from . import _lib
 
 BATCH_CANDIDATES = [(64, 8), (64, 16), (32, 8), (32, 16), (16, 8), (16, 16), (1, 8), (1, 16)]
-FEW_CANDIDATES = [(1, 8), (1, 16), (16, 8)]
+FEW_CANDIDATES = [(1, 4), (1, 8), (1, 16), (16, 8)]
 # group 0 is one entry per task, except at g = 64 where it is the 2x2-block schedule;
 # dealing runs of a pivot's entries to one list (group > 0) lost to single entries on every
 # grid measured, so the tuner keeps to group 0 unless asked for more
