@@ -1994,6 +1994,9 @@ constexpr unsigned long long INF_BITS = 0x7ff0000000000000ull;
 // 355-430, p3s/contingency/solver_cpp.py:119-134). Column cl = blockIdx.x of the outputs
 // (stride S): served, ref bus (or -1), pin (re-slack references and unserved buses) and the
 // start voltage (|V| + 0j at a re-slack reference, 1 + 0j at unserved buses).
+// LDS: the union-find parents and island references of the case (8 n bytes) live in the
+// workgroup's dynamic LDS, so the unions are LDS atomics; else in ufbuf (2 n ints per case).
+template <bool LDS>
 __global__ __launch_bounds__(256) void k_islands(
     int n, int nbr, const int* __restrict__ f, const int* __restrict__ t, const unsigned char* __restrict__ live,
     const int* __restrict__ gptr, const int* __restrict__ gbr, int nref, const int* __restrict__ ref, int ngen,
@@ -2002,11 +2005,12 @@ __global__ __launch_bounds__(256) void k_islands(
     int* __restrict__ refbus, unsigned char* __restrict__ pin, double* __restrict__ V0, int* __restrict__ npinned,
     int S) {
     __shared__ int gm[MAX_GROUP];
+    extern __shared__ int s_uf[];
     const int cl = blockIdx.x;
     const int cs = c0 + cl;
     const int g0 = gptr[cs], gsz = gptr[cs + 1] - g0;
     for (int q = threadIdx.x; q < gsz; q += blockDim.x) gm[q] = gbr[g0 + q];
-    int* parent = ufbuf + (size_t)cl * 2 * n;
+    int* parent = LDS ? s_uf : ufbuf + (size_t)cl * 2 * n;
     int* cref = parent + n;
     for (int v = threadIdx.x; v < n; v += blockDim.x) {
         uf_st(&parent[v], v);
@@ -4174,18 +4178,50 @@ bool islands_need_z(const p3smi_outages* o) { return o->reslack && o->ngen; }
 // Islands + solver inputs of cases [c0, c0 + nc) into column-strided (stride S) outputs, in
 // launches of at most `chunk` cases. uf holds chunk * 2n ints (union-find parents and island
 // references), z (with re-slacking) chunk * n words of generator |Z|.
+// Whether a case's union-find (8 n bytes) fits the LDS of one workgroup of the current device
+// next to the kernel's static 1 KiB; the answer is asked of the device once.
+bool islands_in_lds(int n) {
+    static std::mutex mu;
+    static std::map<int, int> max_lds;  // device -> LDS bytes per workgroup
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    int lim = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = max_lds.find(dev);
+        if (it == max_lds.end()) {
+            if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) lim = 0;
+            max_lds[dev] = lim;
+        } else {
+            lim = it->second;
+        }
+    }
+    return (size_t)8 * n + sizeof(int) * MAX_GROUP <= (size_t)lim;
+}
+
 void launch_islands(p3smi_outages* o, int c0, int nc, int S, int chunk, int* uf, unsigned long long* z,
                     const double* dVstart, unsigned char* served, int* refbus, unsigned char* pin, double* V0,
                     int* npinned) {
     p3smi_topology* t = o->topo;
     const int n = t->host.n;
     hipStream_t st = t->stream;
+    const bool lds = islands_in_lds(n);
+    const size_t lds_bytes = lds ? (size_t)8 * n : 0;
+    if (lds && lds_bytes > (size_t)64 * 1024)
+        HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_islands<true>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+    if (!lds && !uf) throw Error(P3SMI_E_INTERNAL, "islands: no union-find scratch");
     for (int a = 0; a < nc; a += chunk) {
         const int k = std::min(chunk, nc - a);
-        hipLaunchKernelGGL(k_islands, dim3((unsigned)k), dim3(256), 0, st, n, o->nbr, o->f.p, o->t.p, o->live.p,
-                           o->gptr.p, o->gbr.p, o->nref, o->ref.p, o->ngen, o->gbus.p, o->gz.p, o->reslack, c0 + a,
-                           uf, islands_need_z(o) ? z : nullptr, dVstart, served + a, refbus ? refbus + a : nullptr,
-                           pin ? pin + a : nullptr, V0 ? V0 + 2 * (size_t)a : nullptr, npinned + a, S);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3((unsigned)k), dim3(256), lds_bytes, st, n, o->nbr, o->f.p, o->t.p,
+                               o->live.p, o->gptr.p, o->gbr.p, o->nref, o->ref.p, o->ngen, o->gbus.p, o->gz.p,
+                               o->reslack, c0 + a, uf, islands_need_z(o) ? z : nullptr, dVstart, served + a,
+                               refbus ? refbus + a : nullptr, pin ? pin + a : nullptr,
+                               V0 ? V0 + 2 * (size_t)a : nullptr, npinned + a, S);
+        };
+        if (lds) go(k_islands<true>);
+        else go(k_islands<false>);
         HIPCHECK(hipGetLastError());
     }
 }
@@ -4193,7 +4229,7 @@ void launch_islands(p3smi_outages* o, int c0, int nc, int S, int chunk, int* uf,
 void run_islands(p3smi_outages* o, int c0, int nc, int S, const double* dVstart, unsigned char* served,
                  int* refbus, unsigned char* pin, double* V0, int* npinned) {
     const int n = o->topo->host.n;
-    o->scratch.alloc((size_t)ISLAND_CHUNK * 2 * n);
+    if (!islands_in_lds(n)) o->scratch.alloc((size_t)ISLAND_CHUNK * 2 * n);
     if (islands_need_z(o)) o->zscratch.alloc((size_t)ISLAND_CHUNK * n);
     launch_islands(o, c0, nc, S, ISLAND_CHUNK, o->scratch.p, o->zscratch.p, dVstart, served, refbus, pin, V0,
                    npinned);
