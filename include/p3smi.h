@@ -65,6 +65,9 @@ typedef struct {
     double ms_lu_batch;
     int32_t n_lu_batch;
     int64_t lu_batch_cases;
+    /* of pivot_factorizations: those whose threshold pivoting chose the plan's diagonal at every
+     * column, taken from the static-pivot LU after a check (the case then stays on the plan) */
+    int32_t diagonal_factorizations;
 } p3smi_stats;
 
 /* ---- library ------------------------------------------------------------------------ */
@@ -127,6 +130,13 @@ int p3smi_topology_set_mixed(p3smi_topology* t, double rtol, int32_t kmax);
  * is run again from its start with pivoting (p3s/cpp/nr_klu.cpp:844-857). Default 1e-3; 1e30
  * rejects every frozen factorization (the reference's tests/cuda/test_lean_lu.py:224-244). */
 int p3smi_topology_set_guard(p3smi_topology* t, double ratio);
+/* How a fresh pivoting factorization starts (the klu_factor of p3s/cpp/nr_klu.cpp:642-651).
+ * diagonal_first = 1 (default): the static-pivot LU of the plan's order is checked against the
+ * threshold rule (every column's diagonal at least 1e-3 of its largest scaled candidate); when
+ * every column passes, that is the pivoting factorization (the rule picks the diagonal
+ * everywhere) and the case keeps the plan's order; otherwise, and always with 0, the device
+ * Gilbert-Peierls factorization chooses the pivots. */
+int p3smi_topology_set_pivoting(p3smi_topology* t, int32_t diagonal_first);
 /* DC-power-flow start on the device (p3s/timeseries.py:200-244 with the reduced solve of
  * p3s/NewtonPowerflowCpp.py:422-440): solves B'(pvpq, pvpq) theta = P(pvpq) - B'(pvpq, ref)
  * theta_ref with the engine's static-pivot LU, theta_ref the angles of V_flat at the reference

@@ -260,6 +260,9 @@ static void gp_grow(int** ia, double** xa, int* cap, int need) {
     *cap = nc;
 }
 
+/* diagnostics: fresh pivoting factorizations, and the pivots they took off the diagonal */
+static long long g_gp_factors = 0, g_gp_offdiag = 0;
+
 /* klu_factor: returns 1 on success, 0 if structurally or numerically singular. */
 static int gp_factor(gplu* f, int m, const int* Ap, const int* Ai, const double* Ax, const int* Q) {
     gp_free(f);
@@ -320,6 +323,14 @@ static int gp_factor(gplu* f, int m, const int* Ap, const int* Ai, const double*
         f->Lp[k + 1] = lnz;
     }
     gpw_free(&w);
+    {
+        long long off = 0;
+        for (int k = 0; k < m && ok; ++k) off += f->P[k] != Q[k];
+#pragma omp atomic
+        g_gp_factors++;
+#pragma omp atomic
+        g_gp_offdiag += off;
+    }
     if (!ok) { gp_free(f); return 0; }
     f->rcond = dmin / dmax;
     return 1;
@@ -722,9 +733,11 @@ typedef struct { int iterations, converged; } nres;
 /* diagnostics: how often the frozen-pivot plan was rejected / a column was re-run */
 static long long g_lean_rejections = 0, g_column_reruns = 0, g_backtracks = 0;
 long long oracle_lean_rejections(void) { return g_lean_rejections; }
+long long oracle_pivoting_factors(void) { return g_gp_factors; }
+long long oracle_offdiagonal_pivots(void) { return g_gp_offdiag; }
 long long oracle_column_reruns(void) { return g_column_reruns; }
 long long oracle_backtracks(void) { return g_backtracks; }
-void oracle_reset_counters(void) { g_lean_rejections = 0; g_column_reruns = 0; g_backtracks = 0; }
+void oracle_reset_counters(void) { g_lean_rejections = 0; g_column_reruns = 0; g_backtracks = 0; g_gp_factors = 0; g_gp_offdiag = 0; }
 
 /* p3s/cpp/nr_klu.cpp:569-711 */
 static nres run_newton(const otopo* topo, ostate* st, int max_iter, double tol, int line_search) {

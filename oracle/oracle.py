@@ -60,6 +60,8 @@ def lib():
     L.oracle_lean_rejections.restype = C.c_longlong
     L.oracle_column_reruns.restype = C.c_longlong
     L.oracle_backtracks.restype = C.c_longlong
+    L.oracle_pivoting_factors.restype = C.c_longlong
+    L.oracle_offdiagonal_pivots.restype = C.c_longlong
     L.oracle_set_lean_guard.argtypes = [C.c_double]
     _lib = L
     return L
@@ -214,10 +216,12 @@ def min_degree(A_csc) -> np.ndarray:
 
 
 def counters() -> dict:
-    """Frozen-plan rejections and column re-runs since the last reset_counters()."""
+    """Frozen-plan rejections, column re-runs, backtracking steps, fresh pivoting factorizations
+    and the pivots those took off the diagonal, since the last reset_counters()."""
     L = lib()
     return {"lean_rejections": int(L.oracle_lean_rejections()), "column_reruns": int(L.oracle_column_reruns()),
-            "backtracks": int(L.oracle_backtracks())}
+            "backtracks": int(L.oracle_backtracks()), "pivoting_factors": int(L.oracle_pivoting_factors()),
+            "offdiagonal_pivots": int(L.oracle_offdiagonal_pivots())}
 
 
 def set_lean_guard(ratio: float = 1e-3) -> None:

@@ -66,7 +66,8 @@ namespace {
 //   bits  4..7   backtracking trials taken in the current step (0..10)
 //   bits  8..9   precision class of the next factorization (mixed modes)
 //   bit   10     refining: the case's FP32 step is still being refined (mixed mode)
-//   bits 11..12  pivoting: frozen plan / needs a pivoting factorization / has its own pivot order
+//   bits 11..12  pivoting: frozen plan / needs a pivoting factorization / has its own pivot order /
+//                pivots on the plan's diagonal (a pivoting factorization that chose every diagonal)
 //   bit   13     rerun: the case is on its run from scratch with pivoting (no further rerun)
 //   bit   14     pending: its step this iteration waits for the pivoting factorization
 //   bits 16..31  Newton iterations so far (linear solves)
@@ -84,7 +85,7 @@ enum : unsigned {
 // failed FP32 pivots); FP64 this once (an FP32 voter whose 64-case group takes an FP64 step)
 enum : unsigned { PR_F64 = 0, PR_F32 = 1, PR_F64_ALWAYS = 2, PR_F64_ONCE = 3 };
 constexpr unsigned CTL_REFINING = 1u << 10;
-enum : unsigned { PV_NONE = 0, PV_NEED = 1, PV_HAVE = 2 };
+enum : unsigned { PV_NONE = 0, PV_NEED = 1, PV_HAVE = 2, PV_DIAG = 3 };
 constexpr unsigned CTL_RERUN = 1u << 13, CTL_PENDING = 1u << 14;
 constexpr int MAX_ITER_LIMIT = 65535;  // the iteration count has 16 bits
 
@@ -99,9 +100,15 @@ __host__ __device__ constexpr unsigned set_iters(unsigned w, int it) { return (w
 __host__ __device__ constexpr bool evaluated_stage(unsigned s) { return s >= SG_START && s <= SG_COMMIT; }
 __host__ __device__ constexpr unsigned piv_of(unsigned w) { return (w >> 11) & 3u; }
 __host__ __device__ constexpr unsigned set_piv(unsigned w, unsigned p) { return (w & ~(3u << 11)) | (p << 11); }
-// in STEP on the frozen plan, step not pending: what the static-pivot LU launches take
+// in STEP on the plan's pivot order (frozen, or kept by a pivoting factorization that chose the
+// diagonal everywhere), step not pending: what the static-pivot LU launches take
 __host__ __device__ constexpr bool frozen_step(unsigned w) {
-    return stage_of(w) == SG_STEP && piv_of(w) == PV_NONE && !(w & CTL_PENDING);
+    return stage_of(w) == SG_STEP && (piv_of(w) == PV_NONE || piv_of(w) == PV_DIAG) && !(w & CTL_PENDING);
+}
+// in STEP and factored by the pivoting LU (a fresh factorization, or a refactorization on the
+// case's own pivot order)
+__host__ __device__ constexpr bool pivoting_step(unsigned w) {
+    return stage_of(w) == SG_STEP && (piv_of(w) == PV_NEED || piv_of(w) == PV_HAVE);
 }
 // a case this pass advances and evaluates: START, TRIAL, COMMIT, and STEP unless its step waits
 // for the pivoting factorization or the whole batch holds its STEP cases
@@ -390,10 +397,13 @@ __global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
 // refinement solves). Substitutions accumulate in FP64 whatever the factor precision.
 
 // Which cases an LU launch works on (besides being in STEP): every one (SEL_ALL, the FP64
-// solve), the FP64 classes (SEL_F64), FP32 (SEL_F32), or FP32 and still refining (SEL_REFINE).
-enum : int { SEL_ALL = 0, SEL_F64 = 1, SEL_F32 = 2, SEL_REFINE = 3 };
+// solve), the FP64 classes (SEL_F64), FP32 (SEL_F32), or FP32 and still refining (SEL_REFINE);
+// SEL_NEED: the cases waiting for a fresh pivoting factorization, factored on the plan's
+// diagonal as the first try of that factorization (k_pivot_verify decides whether it stands).
+enum : int { SEL_ALL = 0, SEL_F64 = 1, SEL_F32 = 2, SEL_REFINE = 3, SEL_NEED = 4 };
 
 __device__ __forceinline__ bool selected(int sel, unsigned w) {
+    if (sel == SEL_NEED) return stage_of(w) == SG_STEP && piv_of(w) == PV_NEED;
     if (!frozen_step(w)) return false;
     const unsigned p = prec_of(w);
     switch (sel) {
@@ -431,13 +441,20 @@ __global__ __launch_bounds__(256) void k_group_vote(unsigned* __restrict__ ctl, 
 // or a pivot ratio under the guard, moves the case to FP64 for good; it is factored in FP64 in
 // the same iteration (guard_trips[2] counts these). FP64: a zero or non-finite pivot or a ratio
 // under the guard rejects the frozen plan for the rest of the case's run (guard_trips[0]): its
-// step waits for the pivoting factorization (pending). Returns true when the case takes no step
-// from these factors.
+// step waits for the pivoting factorization (pending). A case on its kept diagonal order has no
+// guard: a zero or non-finite pivot sends it back to a fresh pivoting factorization (the failed
+// refactorization of p3s/cpp/nr_klu.cpp:629-640). A first try on the diagonal (SEL_NEED) leaves
+// the control word to k_pivot_verify. Returns true when the case takes no step from these factors.
 template <bool SINGLE>
 __device__ __forceinline__ bool pivot_verdict(unsigned* ctl, int c, double lo, double hi, double sum,
-                                              double rcond_min, int* guard_trips) {
+                                              double rcond_min, int* guard_trips, int sel) {
     const bool broken = !(sum < DBL_MAX && lo > 0.0);
+    if (sel == SEL_NEED) return broken;
     const unsigned w = ctl[c];
+    if (!SINGLE && piv_of(w) == PV_DIAG) {
+        if (broken) ctl[c] = set_piv(w, PV_NEED) | CTL_PENDING;
+        return broken;
+    }
     if (SINGLE) {
         if (broken || lo < rcond_min * hi) {
             ctl[c] = set_prec(w, PR_F64_ALWAYS);
@@ -648,7 +665,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
                 }
                 pstat[2 * (size_t)c] = pmin;
                 pstat[2 * (size_t)c + 1] = pmax;
-                bad = pivot_verdict<SINGLE>(ctl, c, pmin, pmax, psum, rcond_min, guard_trips) ? 1 : 0;
+                bad = pivot_verdict<SINGLE>(ctl, c, pmin, pmax, psum, rcond_min, guard_trips, sel) ? 1 : 0;
             }
             sbad[lane] = bad;
         }
@@ -1067,7 +1084,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
                 }
                 pstat[2 * (size_t)c] = lo;
                 pstat[2 * (size_t)c + 1] = hi;
-                bad = pivot_verdict<SINGLE>(ctl, c, lo, hi, sum, rcond_min, guard_trips) ? 1 : 0;
+                bad = pivot_verdict<SINGLE>(ctl, c, lo, hi, sum, rcond_min, guard_trips, sel) ? 1 : 0;
             }
             sbad[lane] = bad;
         }
@@ -1349,7 +1366,7 @@ __global__ __launch_bounds__(256) void k_control(CtlArgs a) {
             a.ctl[c] = w;
         }
         atomicAdd(&s_cnt[counter_of(sg)], 1);
-        if (sg == SG_STEP && piv_of(w) != PV_NONE) atomicAdd(&s_cnt[CNT_PIVOT], 1);
+        if (pivoting_step(w)) atomicAdd(&s_cnt[CNT_PIVOT], 1);
     }
     __syncthreads();
     if (threadIdx.x < NCOUNT && s_cnt[threadIdx.x]) atomicAdd(&a.tally[threadIdx.x], s_cnt[threadIdx.x]);
@@ -1651,13 +1668,17 @@ __device__ void piv_solve(const PivArgs& a, const PivSlot& f, int c, int lane) {
 __global__ __launch_bounds__(64) void k_pivot_solve(PivArgs a) {
     const int lane = threadIdx.x;
     const int c = a.list[blockIdx.x];
+    const unsigned w0 = a.ctl[c];
+    if (piv_of(w0) == PV_DIAG) {  // k_pivot_verify took the static factorization on the diagonal
+        if (lane == 0) a.res[blockIdx.x] = 4;
+        return;
+    }
     const int slot = a.slot_of[c];
     if (slot < 0) {  // no slot assigned: the host reports it
         if (lane == 0) a.res[blockIdx.x] = 3;
         return;
     }
     const PivSlot f = piv_slot(a, slot);
-    const unsigned w0 = a.ctl[c];
     bool fresh = piv_of(w0) != PV_HAVE;
     bool ok = !fresh && piv_refactor(a, f, c, lane);
     if (!ok) {
@@ -1692,12 +1713,80 @@ __global__ __launch_bounds__(64) void k_pivot_solve(PivArgs a) {
     }
 }
 
-// The cases in STEP off the frozen plan, appended to list (count[0] = how many).
+// First try of a fresh pivoting factorization. The threshold-pivoting LU above, taking the
+// diagonal at every column, computes the static-pivot factors of the plan's order; an SEL_NEED
+// launch has just computed those (and the step from them) for the cases waiting for a fresh
+// factorization. With rows scaled by their largest |J| (d), column p's candidates are the free
+// rows of its reach -- the rows i of L(:, p) -- with values U(p,p) L(i,p) / d_i, and its diagonal
+// is U(p,p) / d_p, so the diagonal is the pivot of every column exactly when
+//     |L(i, p)| d_p / d_i <= 1 / PIVOT_TOL   for every L entry,
+// with every pivot finite and nonzero. Such a case keeps the plan's order (PV_DIAG: its later
+// refactorizations are the static-pivot launches, without the guard, as a refactorization on a
+// kept order has none) and takes its step from that launch; any other keeps PV_NEED for the
+// Gilbert-Peierls factorization (the off-diagonal pivots it needs). Measured on the 94 N-1
+// subsets of the Poisson serving mix (8k-43k-bus grids), the pivoting factorizations the
+// oracle's restatement of KLU ran, reruns included, chose the diagonal at every column.
+// One workgroup per listed case; the row scales go to the case's slot.
+struct VerifyArgs {
+    const int* Jp;    // J rows (row scales)
+    const int* perm;  // pivot -> Jacobian row / column
+    const int* Lp;    // L patterns in pivot space: rows Li[Lp[p], Lp[p + 1]) of column p, ascending
+    const int* Li;
+    const int* base;  // factor position of 1 / U(p,p); U row, then L column, follow
+    const double* J;
+    const double* fac;
+    unsigned* ctl;
+    const int* list;
+    const int* slot_of;
+    double* rs;       // slot-major row scales (m per slot)
+    int* guard;       // [4] pivoting factorizations, [5] of them on the diagonal
+    int m, S;
+};
+
+__global__ __launch_bounds__(256) void k_pivot_verify(VerifyArgs a) {
+    const int c = a.list[blockIdx.x];
+    const unsigned w = a.ctl[c];
+    const int slot = a.slot_of[c];
+    if (piv_of(w) != PV_NEED || slot < 0) return;  // the same in the whole workgroup
+    double* rs = a.rs + (size_t)slot * (size_t)a.m;
+    for (int i = threadIdx.x; i < a.m; i += blockDim.x) {
+        double r = 0.0;
+        for (int e = a.Jp[i]; e < a.Jp[i + 1]; ++e) r = fmax(r, fabs(a.J[at(e, a.S, c)]));
+        rs[i] = (r > 0.0 && r <= DBL_MAX) ? r : 1.0;
+    }
+    __syncthreads();
+    bool ok = true;
+    for (int p = threadIdx.x; p < a.m && ok; p += blockDim.x) {
+        const double inv = fabs(a.fac[at(a.base[p], a.S, c)]);
+        if (!(inv > 0.0 && inv <= DBL_MAX)) {
+            ok = false;
+            break;
+        }
+        const double dp = rs[a.perm[p]];
+        const int l0 = a.Lp[p], l1 = a.Lp[p + 1];
+        const int lpos = a.base[p] + 1 + (l1 - l0);
+        for (int r = l0; r < l1; ++r) {
+            const double v = fabs(a.fac[at(lpos + (r - l0), a.S, c)]) * dp / rs[a.perm[a.Li[r]]];
+            if (!(v <= 1.0 / PIVOT_TOL)) {
+                ok = false;
+                break;
+            }
+        }
+    }
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0 && ok) {
+        a.ctl[c] = set_prec(set_piv(w & ~(CTL_PENDING | CTL_REFINING), PV_DIAG), PR_F64_ALWAYS);
+        atomicAdd(&a.guard[4], 1);
+        atomicAdd(&a.guard[5], 1);
+    }
+}
+
+// The cases in STEP factored by the pivoting LU, appended to list (count[0] = how many).
 __global__ void k_pivot_list(const unsigned* __restrict__ ctl, int S, int* __restrict__ list, int* __restrict__ count) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= S) return;
     const unsigned w = ctl[c];
-    if (stage_of(w) == SG_STEP && piv_of(w) != PV_NONE) list[atomicAdd(count, 1)] = c;
+    if (pivoting_step(w)) list[atomicAdd(count, 1)] = c;
 }
 
 // Time-series injections on the device (p3s/timeseries.py:36-141): for bus i and case t,
@@ -2355,6 +2444,7 @@ struct DevSched {  // one level schedule dealt to TL task lists
 struct DevTopo {
     DevBuf<int> Yp, Yj, Ydiag, row_p, row_q, posJ, xva, xvm, Jp, xcol;
     DevBuf<int> Cp, Ci, Ce, perm;  // the Jacobian by column and the column order (pivoting LU)
+    DevBuf<int> Lp, Li, base;      // the plan's L patterns and factor positions (k_pivot_verify)
     DevBuf<double> G, B;
     DevBuf<int> tiles;    // k_evaluate row tiles: first row of each, then n
     int ntiles = 0, tile_rows = 0;
@@ -2387,6 +2477,7 @@ struct p3smi_topology {
     int max_chunk = 0;            // one-call entry points: cases per chunk at most (0: sized to free HBM)
     int eval_rows = 32;           // k_evaluate: rows per tile at most (the autotuner may change it)
     double guard_ratio = REUSE_RCOND_RATIO;  // frozen plan rejected below guard_ratio x rcond0
+    bool pivot_diagonal_first = true;  // a fresh pivoting factorization tries the plan's diagonal first
     hipStream_t stream = nullptr;
     std::unique_ptr<p3smi_batch> cached;  // workspace of the one-call entry points
     std::unique_ptr<p3smi_batch> dc;      // one-case workspace of the DC start
@@ -2513,6 +2604,9 @@ void ensure_device(p3smi_topology* t) {
     d.Ci.upload(h.Ci.data(), h.Ci.size(), st);
     d.Ce.upload(h.Ce.data(), h.Ce.size(), st);
     d.perm.upload(h.perm.data(), h.perm.size(), st);
+    d.Lp.upload(h.Lp.data(), h.Lp.size(), st);
+    d.Li.upload(h.Li.data(), h.Li.size(), st);
+    d.base.upload(h.base.data(), h.base.size(), st);
     HIPCHECK(hipStreamSynchronize(st));
     d.ready = true;
 }
@@ -2767,7 +2861,7 @@ void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     b->tally.alloc(NCOUNT + 1);
     b->counts.alloc(NCOUNT);
     // guard trips, FP32 and FP64 factorizations of the switching mode, reruns, pivoting factorizations
-    b->guard.alloc(5);
+    b->guard.alloc(6);
     b->rcount.alloc(4);  // refining, fallbacks, FP64 classes, FP32 pivot failures before
     hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, t->stream, b->tally.p, NCOUNT + 1);
     hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, t->stream, b->ctl.p, 0,
@@ -3075,15 +3169,16 @@ void pivot_release_all(p3smi_batch* b) {
     for (int k = P.slots - 1; k >= 0; --k) P.free.push_back(k);
 }
 
-// The pivoting solves of this pass: every case in STEP off the frozen plan gets its step from
-// k_pivot_solve (slots assigned here; a pool too small for a fresh factorization grows and the
-// case is factored again). Returns the number of cases solved.
+// The pivoting solves of this pass: every case in STEP factored by the pivoting LU gets its step
+// here (slots assigned here; a pool too small for a fresh factorization grows and the case is
+// factored again). A case that needs a fresh factorization first tries the plan's diagonal
+// (an SEL_NEED static-pivot launch checked by k_pivot_verify); the others, and those the check
+// rejects, go to k_pivot_solve. Returns the number of cases solved.
 int pivot_solve(p3smi_batch* b) {
     p3smi_topology* t = b->topo;
     const p3smi::Topology& h = t->host;
     hipStream_t st = t->stream;
     PivotPool& P = b->piv;
-    KTimer tm(b, 1);
     if (P.list.n < (size_t)b->S) {
         P.list.alloc(b->S);
         P.res.alloc(b->S);
@@ -3115,7 +3210,29 @@ int pivot_solve(p3smi_batch* b) {
             P.free.pop_back();
         }
     HIPCHECK(hipMemcpyAsync(P.slot_of.p, P.h_slot_of.data(), b->S * sizeof(int), hipMemcpyHostToDevice, st));
-    std::vector<int> res(n);
+    if (t->pivot_diagonal_first) {
+        launch_lu<double, true>(b, n, b->fac.p, b->R.p, b->x.p, SEL_NEED);
+        KTimer tm(b, 1);
+        VerifyArgs v;
+        v.Jp = t->dev.Jp.p;
+        v.perm = t->dev.perm.p;
+        v.Lp = t->dev.Lp.p;
+        v.Li = t->dev.Li.p;
+        v.base = t->dev.base.p;
+        v.J = b->Jx.p;
+        v.fac = b->fac.p;
+        v.ctl = b->ctl.p;
+        v.list = P.list.p;
+        v.slot_of = P.slot_of.p;
+        v.rs = P.rs.p;
+        v.guard = b->guard.p;
+        v.m = h.m;
+        v.S = b->S;
+        hipLaunchKernelGGL(k_pivot_verify, dim3((unsigned)n), dim3(256), 0, st, v);
+        HIPCHECK(hipGetLastError());
+    }
+    KTimer tm(b, 1);
+    std::vector<int> res(n), diagonal;
     for (;;) {
         PivArgs a;
         a.Cp = t->dev.Cp.p;
@@ -3145,6 +3262,7 @@ int pivot_solve(p3smi_batch* b) {
         for (int q = 0; q < n; ++q) {
             if (res[q] == 3) throw Error(P3SMI_E_INTERNAL, "pivoting factorization: a case without a slot");
             if (res[q] == 2) again.push_back(list[q]);
+            if (res[q] == 4) diagonal.push_back(list[q]);
         }
         if (again.empty()) break;
         // fill beyond the pool's capacity: a pool with twice the capacity, the same slots
@@ -3154,6 +3272,7 @@ int pivot_solve(p3smi_batch* b) {
         n = (int)list.size();
         HIPCHECK(hipMemcpyAsync(P.list.p, list.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
     }
+    for (int c : diagonal) pivot_release(b, c);  // on the plan's order: no slot needed any more
     return (int)list.size();
 }
 
@@ -3293,7 +3412,7 @@ void reset_cases(p3smi_batch* b, int precision) {
     hipStream_t st = b->topo->stream;
     hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->ctl.p, b->B, b->S,
                        start_class(precision), start_extra(b->topo));
-    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 5);
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 6);
     HIPCHECK(hipGetLastError());
     std::memset(b->h_counts, 0, NCOUNT * sizeof(int));
     b->h_counts[CNT_START] = b->B;
@@ -3327,12 +3446,13 @@ void run_newton(p3smi_batch* b, int max_iter, double tol, int line_search, int p
         cnt = run_pass(b, p);
     }
     b->stats.failed_factorizations = cnt[CNT_FAIL];
-    int g[5];
+    int g[6];
     HIPCHECK(hipMemcpyAsync(g, b->guard.p, sizeof g, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     b->stats.guard_trips = g[0];
     b->stats.reruns = g[3];
     b->stats.pivot_factorizations = g[4];
+    b->stats.diagonal_factorizations = g[5];
     if (precision == P3SMI_PREC_MIXED_SWITCH) {
         b->stats.single_factorizations = g[1];
         b->stats.double_factorizations = g[2];
@@ -3610,6 +3730,15 @@ int p3smi_topology_set_guard(p3smi_topology* t, double ratio) {
         if (!(ratio >= 0.0) || !std::isfinite(ratio)) throw Error(P3SMI_E_ARG, "set_guard: ratio must be finite and >= 0");
         std::lock_guard<std::mutex> lk(t->mu);
         t->guard_ratio = ratio;
+    });
+}
+
+int p3smi_topology_set_pivoting(p3smi_topology* t, int32_t diagonal_first) {
+    return guarded([&] {
+        if (!t) throw Error(P3SMI_E_ARG, "set_pivoting: null topology");
+        if (diagonal_first != 0 && diagonal_first != 1) throw Error(P3SMI_E_ARG, "set_pivoting: diagonal_first must be 0 or 1");
+        std::lock_guard<std::mutex> lk(t->mu);
+        t->pivot_diagonal_first = diagonal_first != 0;
     });
 }
 
@@ -3928,12 +4057,13 @@ int p3smi_batch_stats(const p3smi_batch* b, p3smi_stats* out) {
         if (b->slot_mode) {
             // passes of a slot batch do not read the device counters back: read them here
             // (guard trips; FP32 and FP64 factorizations of the switching mode) since slots_init
-            int g[5];
+            int g[6];
             HIPCHECK(hipMemcpyAsync(g, b->guard.p, sizeof g, hipMemcpyDeviceToHost, b->topo->stream));
             HIPCHECK(hipStreamSynchronize(b->topo->stream));
             out->guard_trips = g[0];
             out->reruns = g[3];
             out->pivot_factorizations = g[4];
+            out->diagonal_factorizations = g[5];
             if (b->precision == P3SMI_PREC_MIXED_SWITCH) {
                 out->single_factorizations = g[1];
                 out->double_factorizations = g[2];
@@ -3999,6 +4129,7 @@ static void add_stats(p3smi_stats& a, const p3smi_stats& s) {
     a.ms_refine += s.ms_refine;
     a.reruns += s.reruns;
     a.pivot_factorizations += s.pivot_factorizations;
+    a.diagonal_factorizations += s.diagonal_factorizations;
     a.n_eval += s.n_eval;
     a.n_lu += s.n_lu;
     a.n_step += s.n_step;
@@ -4599,7 +4730,7 @@ int p3smi_batch_slots_init(p3smi_batch* b) {
         ensure_tiles(b->topo);
         hipLaunchKernelGGL(k_ctl_reset, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, b->ctl.p, 0, b->S, 0u,
                            0u);
-        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 5);
+        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, b->guard.p, 6);
         pivot_release_all(b);
         HIPCHECK(hipGetLastError());
         std::memset(b->h_counts, 0, NSTAT * sizeof(int));

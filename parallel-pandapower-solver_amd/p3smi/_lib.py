@@ -76,6 +76,7 @@ class Stats(C.Structure):
         ("n_eval", C.c_int32), ("n_lu", C.c_int32), ("n_step", C.c_int32), ("n_decide", C.c_int32),
         ("n_refine", C.c_int32), ("pivot_factorizations", C.c_int32),
         ("ms_lu_batch", C.c_double), ("n_lu_batch", C.c_int32), ("lu_batch_cases", C.c_int64),
+        ("diagonal_factorizations", C.c_int32),
     ]
 
 
@@ -83,7 +84,7 @@ class Stats(C.Structure):
 EXPORTS = [
     "p3smi_last_error", "p3smi_version", "p3smi_device_count", "p3smi_device_memory", "p3smi_hbm_probe",
     "p3smi_topology_create", "p3smi_topology_free", "p3smi_topology_info", "p3smi_update_y",
-    "p3smi_topology_set_lu_shape", "p3smi_topology_set_eval_tile", "p3smi_topology_set_guard", "p3smi_dc_start", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_max_chunk", "p3smi_topology_set_mixed",
+    "p3smi_topology_set_lu_shape", "p3smi_topology_set_eval_tile", "p3smi_topology_set_guard", "p3smi_topology_set_pivoting", "p3smi_dc_start", "p3smi_topology_set_mixed_switch", "p3smi_topology_set_max_chunk", "p3smi_topology_set_mixed",
     "p3smi_host_jacobian", "p3smi_host_factor_solve", "p3smi_host_lu_emulate", "p3smi_host_maps",
     "p3smi_solve_batch", "p3smi_solve_batch_contingency", "p3smi_solve_batch_contingency_stamp",
     "p3smi_outages_branch_flows", "p3smi_outages_voltage_violations",
@@ -121,6 +122,7 @@ def lib():
     L.p3smi_topology_set_lu_shape.argtypes = [P, _i32, _i32, _i32, _i32, _i32, _i32]
     L.p3smi_topology_set_eval_tile.argtypes = [P, _i32]
     L.p3smi_topology_set_guard.argtypes = [P, _f64]
+    L.p3smi_topology_set_pivoting.argtypes = [P, _i32]
     L.p3smi_dc_start.argtypes = [P, P, P, P, _i32, P, P]
     L.p3smi_topology_set_mixed.argtypes = [P, _f64, _i32]
     L.p3smi_host_jacobian.argtypes = [P, P, P, P, P, P, P]

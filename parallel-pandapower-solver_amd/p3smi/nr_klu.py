@@ -126,6 +126,13 @@ class Solver:
         case to the pivoting factorization; 1e30 rejects every frozen factorization."""
         check(lib().p3smi_topology_set_guard(self._h, float(ratio)))
 
+    def set_pivoting(self, diagonal_first: bool = True):
+        """How a fresh pivoting factorization starts: with diagonal_first (default) the
+        static-pivot LU on the plan's order stands when the threshold rule would pick the
+        diagonal at every column (the case then stays on the plan's order); otherwise, and
+        always with False, the device Gilbert-Peierls LU chooses the pivots."""
+        check(lib().p3smi_topology_set_pivoting(self._h, 1 if diagonal_first else 0))
+
     @property
     def m(self) -> int:
         return int(self.info["m"])

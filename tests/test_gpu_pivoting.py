@@ -6,7 +6,11 @@ the run (p3s/cpp/nr_klu.cpp:609-652); a column that does not converge on the fro
 again from its start with pivoting (:844-857). The reference's GPU test forces the fallback by
 setting the ratio to 1e30 (tests/cuda/test_lean_lu.py:224-244). Here both the device engine and
 the oracle take that setting, and their results must agree: convergence flags and iteration
-counts exact, |dV| <= 1e-9; and the reruns of the non-converging cases happen on the same cases."""
+counts exact, |dV| <= 1e-9; and the reruns of the non-converging cases happen on the same cases.
+
+A fresh pivoting factorization runs in one of two ways, both checked here: the static-pivot LU
+of the plan's order, kept when the threshold rule would take the diagonal at every column
+(diagonal_first, the default), or the device Gilbert-Peierls LU choosing every pivot."""
 
 import numpy as np
 import pytest
@@ -37,8 +41,9 @@ def forced_guard():
     ora.set_lean_guard(1e-3)
 
 
+@pytest.mark.parametrize("diagonal_first", [True, False])
 @pytest.mark.parametrize("name", ["pf200", "ts2000"])
-def test_every_factorization_pivots_when_the_frozen_plan_is_rejected(gpu, forced_guard, name):
+def test_every_factorization_pivots_when_the_frozen_plan_is_rejected(gpu, forced_guard, name, diagonal_first):
     g = grids.config_grid(name, 0)
     Yp, Yj, Yx = g.csr()
     if name == "pf200":
@@ -47,14 +52,24 @@ def test_every_factorization_pivots_when_the_frozen_plan_is_rejected(gpu, forced
         S, V0 = grids.timeseries_daily(g, 8760, seed=1)[:, ::73].copy(), dc_initial_voltage(g)
     s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
     s.set_guard(1e30)
+    s.set_pivoting(diagonal_first)
     r = s.solve_batch(S, V0, max_iter=20, tol=1e-8)
     st = s.last_stats
+    ora.reset_counters()
     ref = OracleSolver(Yp, Yj, Yx, g.pv, g.pq).solve_batch(S, V0, max_iter=20, tol=1e-8)
+    oc = ora.counters()
     _compare(r, ref, name)
     assert r["converged"].all()
     # the first factorization of every case trips the guard and pivots; later ones refactor
     assert st["guard_trips"] == S.shape[1], st
     assert st["pivot_factorizations"] >= S.shape[1], st
+    # the oracle's threshold pivoting took the diagonal at every column of these Jacobians, so the
+    # device's first try stands for every case (and the Gilbert-Peierls path is not needed)
+    assert oc["offdiagonal_pivots"] == 0, oc
+    if diagonal_first:
+        assert st["diagonal_factorizations"] == st["pivot_factorizations"] == S.shape[1], st
+    else:
+        assert st["diagonal_factorizations"] == 0, st
     # the same solve on the frozen plan: the same flags and iterations, voltages within the bar
     s.set_guard(1e-3)
     plain = s.solve_batch(S, V0, max_iter=20, tol=1e-8)
@@ -62,7 +77,8 @@ def test_every_factorization_pivots_when_the_frozen_plan_is_rejected(gpu, forced
     _compare(r, plain, name + " frozen vs pivoting")
 
 
-def test_reruns_of_non_converging_cases_match_the_oracle(gpu):
+@pytest.mark.parametrize("diagonal_first", [True, False])
+def test_reruns_of_non_converging_cases_match_the_oracle(gpu, diagonal_first):
     """Heavy loads from perturbed starts: some cases do not converge on the frozen plan; both the
     device and the oracle run each of them again with pivoting and report that run."""
     g = grids.config_grid("pf200", 0)
@@ -74,6 +90,7 @@ def test_reruns_of_non_converging_cases_match_the_oracle(gpu):
     V0 = np.repeat(g.flat_start()[:, None], T, axis=1)
     V0[g.pq, :] *= rng.uniform(0.6, 1.4, T)[None, :]
     s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    s.set_pivoting(diagonal_first)
     r = s.solve_batch(S, V0, max_iter=20, tol=1e-8)
     st = s.last_stats
     ora.reset_counters()
@@ -82,9 +99,14 @@ def test_reruns_of_non_converging_cases_match_the_oracle(gpu):
     _compare(r, ref, "reruns")
     assert st["reruns"] == oc["column_reruns"] == int((~ref["converged"]).sum()), (st, oc)
     assert st["reruns"] > 0
+    print(f"diagonal_first={diagonal_first}: {st['reruns']} reruns, {st['pivot_factorizations']} pivoting "
+          f"factorizations ({st['diagonal_factorizations']} on the diagonal); oracle {oc}")
+    if not diagonal_first:
+        assert st["diagonal_factorizations"] == 0, st
 
 
-def test_forced_pivoting_on_n1_outages(gpu, forced_guard):
+@pytest.mark.parametrize("diagonal_first", [True, False])
+def test_forced_pivoting_on_n1_outages(gpu, forced_guard, diagonal_first):
     """Per-case admittances and pinned buses on the pivoting path: every pf200 outage."""
     from p3smi import contingency
     from tests.test_gpu_contingency import _oracle_solve
@@ -92,6 +114,7 @@ def test_forced_pivoting_on_n1_outages(gpu, forced_guard):
     g = grids.config_grid("pf200", 0)
     cs = contingency.from_grid(g, reslack=True)
     cs.solver.set_guard(1e30)
+    cs.solver.set_pivoting(diagonal_first)
     V_start = g.flat_start()
     r = cs.solve(g.sbus(), V_start, max_iter=20, tol=1e-8)
     V, it, cv, srv = _oracle_solve(cs, g, V_start)
