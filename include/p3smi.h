@@ -327,6 +327,26 @@ int p3smi_batch_finished(p3smi_batch* b, int32_t* slots, int32_t* n);
 int p3smi_batch_fetch(p3smi_batch* b, int32_t k, const int32_t* slots, double* V, int32_t* iterations,
                       uint8_t* converged, double* mismatch);
 
+/* ---- batch groups: uneven batching in one launch per kernel ---------------------------
+ * Replaces the reference's one-topology-at-a-time batch calls (p3s/cpp/nr_klu.cpp:1049-1166,
+ * the same Newton per case) for a set of topologies solved together. Each member is a batch
+ * created and uploaded as for p3smi_batch_solve (its topology, S, V0); members may differ in
+ * grid size, case count and LU launch shape. p3smi_group_solve runs every member's whole-batch
+ * FP64 Newton solve at once: each pass is one launch per kernel (per LU launch shape and
+ * evaluation variant) over all members, the passes decided on the device and captured
+ * passes_per_graph to a graph launch, so the host waits once per graph launch. A member whose
+ * case needs a pivoting factorization takes that pass alone on the host side and rejoins. Every
+ * case's result (p3smi_batch_download per member) is bit-identical to p3smi_batch_solve of its
+ * batch alone. A group does not own its batches; free the group before them. */
+typedef struct p3smi_group p3smi_group;
+int p3smi_group_create(p3smi_batch* const* members, int32_t count, p3smi_group** out);
+void p3smi_group_free(p3smi_group* g);
+int p3smi_group_solve(p3smi_group* g, int32_t max_iter, double tol, int32_t line_search, int32_t precision,
+                      int32_t passes_per_graph);
+/* Passes of the last group solve: [0] device passes launched (graph launches x passes per
+ * graph), [1] passes run on the host side for pivoting factorizations, [2] graph launches. */
+int p3smi_group_stats(const p3smi_group* g, int64_t* out3);
+
 #ifdef __cplusplus
 }
 #endif

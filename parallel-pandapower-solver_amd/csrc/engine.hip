@@ -52,6 +52,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include <unistd.h>
@@ -136,6 +137,44 @@ __host__ __device__ constexpr int counter_of(unsigned s) {
          : s == SG_STOP ? CNT_STOP : CNT_IDLE;
 }
 
+// Pass plan of a batch whose passes run on the device (group solves, several passes per graph
+// launch): what the next pass does, written by the last workgroup of k_control from the stage
+// counters, read by every kernel of the pass. The host writes the first plan of each launch.
+enum : int {
+    PL_HOLD = 0,     // the cases in STEP wait (lock-step: some case is in its line search)
+    PL_LU = 1,       // LU launch shape of the cases in STEP: 0 none, 1 the batch shape, 2 the few-case one
+    PL_HALT = 2,     // nothing to do on the device: no live case, or a case waits for a pivoting solve
+    PL_START = 3,    // cases in START
+    PL_ADVANCE = 4,  // k_advance has work (a step, a trial or a commit)
+    NPLAN = 8
+};
+// The plan for the pass after one whose counters are cnt (the decisions of the host's pass loop).
+__host__ __device__ inline void plan_from_counts(const int* cnt, int few_max, int* plan) {
+    const int live = cnt[CNT_START] + cnt[CNT_ACTIVE] + cnt[CNT_TRIAL] + cnt[CNT_COMMIT];
+    const bool searching = cnt[CNT_TRIAL] + cnt[CNT_COMMIT] > 0;
+    const bool hold = searching && cnt[CNT_ACTIVE] > 0;
+    const int n_solve = hold ? 0 : cnt[CNT_ACTIVE];
+    plan[PL_HOLD] = hold ? 1 : 0;
+    plan[PL_LU] = n_solve == 0 ? 0 : (n_solve <= few_max ? 2 : 1);
+    plan[PL_HALT] = (live == 0 || (!hold && cnt[CNT_PIVOT] > 0)) ? 1 : 0;
+    plan[PL_START] = cnt[CNT_START] > 0 ? 1 : 0;
+    plan[PL_ADVANCE] = (n_solve > 0 || searching) ? 1 : 0;
+    for (int k = PL_ADVANCE + 1; k < NPLAN; ++k) plan[k] = 0;
+}
+
+// Workgroups of a launch over several batches ("segments"): segment s owns the workgroups
+// [off[s], off[s + 1]), off[0] = 0, each range a multiple of 8 so that a segment's workgroups
+// are dealt to the 8 XCDs as those of a launch of its own. Returns s (uniform in a workgroup).
+__device__ __forceinline__ int seg_find(const int* __restrict__ off, int nseg, int b) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 constexpr double ARMIJO_C = 1e-4;
 constexpr int LS_MAX_TRIALS = 10;
 constexpr double REUSE_RCOND_RATIO = 1e-3;
@@ -180,24 +219,49 @@ __global__ void k_clear_int(int* p, int n) {
 
 // Start voltages of the cases in START: V0 (n) shared (v0_stride 0) or (n, v0_stride) complex,
 // column c. The rectangular state is V0 itself; the polar one its modulus and argument. Sets the
-// case's precision class p0 (one thread per case writes it).
-__global__ __launch_bounds__(256) void k_begin(const double* __restrict__ V0, int v0_stride,
-                                               unsigned* __restrict__ ctl, unsigned p0, double* __restrict__ Vm,
-                                               double* __restrict__ Va, double* __restrict__ Ve,
-                                               double* __restrict__ Vf, int n, int S) {
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (bus >= n) return;
+// case's precision class p0 (one thread per case writes it). Workgroup (bx, by) of gy along the
+// buses takes the 64 cases of group bx and the buses by * 4 + wave, stepping by gy * 4.
+struct BeginArgs {
+    const double* V0;
+    int v0_stride;
+    unsigned* ctl;
+    unsigned p0;
+    double *Vm, *Va, *Ve, *Vf;
+    int n, S;
+    int gx, gy;        // workgroups along the cases and along the buses
+    const int* plan;   // passes driven on the device: skip unless the plan has cases in START
+};
+
+__device__ __forceinline__ void begin_body(const double* __restrict__ V0, int v0_stride, unsigned* __restrict__ ctl,
+                                           unsigned p0, double* __restrict__ Vm, double* __restrict__ Va,
+                                           double* __restrict__ Ve, double* __restrict__ Vf, int n, int S, int bx,
+                                           int by, int gy) {
+    const int c = bx * 64 + (threadIdx.x & 63);
     const unsigned w = ctl[c];
     if (stage_of(w) != SG_START) return;
-    const size_t src = 2 * (v0_stride ? (size_t)bus * v0_stride + c : (size_t)bus);
-    const double re = V0[src], im = V0[src + 1];
-    const size_t o = at(bus, S, c);
-    Ve[o] = re;
-    Vf[o] = im;
-    Vm[o] = hypot(re, im);
-    Va[o] = atan2(im, re);
-    if (bus == 0) ctl[c] = set_prec(w, p0);  // the stage stays START: other threads may read it
+    for (int bus = by * 4 + (int)(threadIdx.x >> 6); bus < n; bus += gy * 4) {
+        const size_t src = 2 * (v0_stride ? (size_t)bus * v0_stride + c : (size_t)bus);
+        const double re = V0[src], im = V0[src + 1];
+        const size_t o = at(bus, S, c);
+        Ve[o] = re;
+        Vf[o] = im;
+        Vm[o] = hypot(re, im);
+        Va[o] = atan2(im, re);
+        if (bus == 0) ctl[c] = set_prec(w, p0);  // the stage stays START: other threads may read it
+    }
+}
+
+__global__ __launch_bounds__(256) void k_begin(BeginArgs a) {
+    begin_body(a.V0, a.v0_stride, a.ctl, a.p0, a.Vm, a.Va, a.Ve, a.Vf, a.n, a.S, blockIdx.x, blockIdx.y, gridDim.y);
+}
+
+__global__ __launch_bounds__(256) void k_begin_multi(const BeginArgs* __restrict__ tab, const int* __restrict__ off,
+                                                     int nseg) {
+    const int s = seg_find(off, nseg, blockIdx.x);
+    const BeginArgs& a = tab[s];
+    const int b = blockIdx.x - off[s];
+    if (b >= a.gx * a.gy || a.plan[PL_HALT] || !a.plan[PL_START]) return;
+    begin_body(a.V0, a.v0_stride, a.ctl, a.p0, a.Vm, a.Va, a.Ve, a.Vf, a.n, a.S, b % a.gx, b / a.gx, a.gy);
 }
 
 // Per-case admittance G + jB at [k * S + c] of columns [0, nc): mode 0 copies the shared base
@@ -279,10 +343,12 @@ struct EvalArgs {
     double* R;
     double* part;
     int n, S;
+    int gx;                    // row tiles (workgroups along x)
+    const int* plan;           // passes driven on the device: the hold comes from it
 };
 
 template <bool CASE_Y, bool PIN>
-__global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
+__device__ __forceinline__ void evaluate_body(const EvalArgs& a, int hold, int bx, int by) {
     __shared__ int s_col[EV_NZ];
     __shared__ int4 s_pos[EV_NZ];
     __shared__ double2 s_y[CASE_Y ? 1 : EV_NZ];
@@ -290,8 +356,8 @@ __global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int S = a.S;
-    const int c = blockIdx.y * 64 + lane;
-    const int r0 = a.tile_ptr[blockIdx.x], r1 = a.tile_ptr[blockIdx.x + 1];
+    const int c = by * 64 + lane;
+    const int r0 = a.tile_ptr[bx], r1 = a.tile_ptr[bx + 1];
     const int k0 = a.Yp[r0], k1 = a.Yp[r1];
     const bool staged = k1 - k0 <= EV_NZ;
     if (staged) {
@@ -304,7 +370,7 @@ __global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
     __syncthreads();
     const unsigned wc = a.ctl[c];
     const unsigned sg = stage_of(wc);
-    const bool run = moves(wc, a.hold);
+    const bool run = moves(wc, hold);
     const bool jac = run && sg != SG_TRIAL;
     double worst = 0.0;
     if (run) {
@@ -374,8 +440,23 @@ __global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
     __syncthreads();
     if (wave == 0 && run) {
         for (int v = 1; v < EV_WAVES; ++v) worst = nan_max(worst, s_worst[v][lane]);
-        a.part[at(blockIdx.x, S, c)] = worst;
+        a.part[at(bx, S, c)] = worst;
     }
+}
+
+template <bool CASE_Y, bool PIN>
+__global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
+    evaluate_body<CASE_Y, PIN>(a, a.hold, blockIdx.x, blockIdx.y);
+}
+
+template <bool CASE_Y, bool PIN>
+__global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate_multi(const EvalArgs* __restrict__ tab,
+                                                                  const int* __restrict__ off, int nseg) {
+    const int s = seg_find(off, nseg, blockIdx.x);
+    const EvalArgs& a = tab[s];
+    const int b = blockIdx.x - off[s];
+    if (b >= a.gx * (a.S / 64) || a.plan[PL_HALT]) return;
+    evaluate_body<CASE_Y, PIN>(a, a.plan[PL_HOLD], b % a.gx, b / a.gx);
 }
 
 // Summation order shared by every LU kernel, so that a case's factors and step are the same
@@ -573,15 +654,48 @@ struct In2 { double v, d; };
 // is uniform (scalar); with G = 32 or 16 the two or four lists of a wavefront walk their
 // streams side by side (divergent only where their task boundaries differ), which gives a
 // batch of few cases more lists per case and more workgroups for the compute units.
+// The arguments of an LU launch over one batch (either kernel's schedule arrays), for launches
+// that cover several batches (k_lu_stream_multi, k_lu_blk_multi: one table entry per batch).
+struct LuArgs {
+    const int* ptrA;
+    const int4* taskA;
+    const int* ptrB;
+    const int4* taskB;
+    const int* diagB;    // one-entry tasks
+    const int2* pairs;   // one-entry tasks
+    const int4* steps;   // 2x2 blocks
+    const int* ptrF;
+    const int4* fwd;
+    const int2* fent;    // one-entry tasks
+    const int4* fsteps;  // 2x2 blocks
+    const int* ptrW;
+    const int4* bwd;
+    const int2* bent;    // one-entry tasks
+    const int4* bsteps;  // 2x2 blocks
+    const double* Jx;
+    void* fac;
+    const double* rhs;
+    double* x;
+    unsigned* ctl;
+    double* pstat;
+    int* guard;
+    int sel;
+    double rcond_min;
+    int nlev, nblev, S;
+    int gx;             // workgroups over the batch's cases (S / G)
+    const int* plan;    // passes driven on the device: run when plan[PL_LU] == want
+    int want;
+};
+
 template <int G, int NS, typename real, bool FACTOR>
-__global__ __launch_bounds__(64 * NS) void k_lu_stream(
+__device__ __forceinline__ void lu_stream_body(
     const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
     const int4* __restrict__ taskB, const int* __restrict__ diagB, const int2* __restrict__ pairs,
     const int* __restrict__ ptrF, const int4* __restrict__ fwd, const int2* __restrict__ fent,
     const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int2* __restrict__ bent,
     const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
     unsigned* __restrict__ ctl, double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min,
-    int nlev, int nblev, int S) {
+    int nlev, int nblev, int S, int bx, int gx) {
     constexpr bool SINGLE = sizeof(real) == 4;
     constexpr int TL = 64 * NS / G;
     // fewer than 64 cases per workgroup: lists differ within a wavefront, so their indices and
@@ -602,8 +716,8 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     // With one case per workgroup, a factor row's 128-byte line holds 16 cases (FP64), and
     // workgroups are dealt to the 8 XCDs in turn: XCD x takes the consecutive cases
     // [x * n / 8, (x + 1) * n / 8) (workgroups x, x + 8, ...), so the cases of a line share an L2.
-    int wg = blockIdx.x;
-    if (G == 1 && (gridDim.x & 7) == 0) wg = (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+    int wg = bx;
+    if (G == 1 && (gx & 7) == 0) wg = (bx & 7) * (gx >> 3) + (bx >> 3);
     const int c = wg * G + lane;
     bool act = selected(sel, ctl[c]);
     if (!__syncthreads_or(act)) return;
@@ -677,6 +791,32 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
         if (act) seg_stream<KF, double, In2>(bwd, ptrW[i0], ptrW[i0 + 1], bent, fac, x, S, c, startW, finishW);
         __syncthreads();
     }
+}
+
+template <int G, int NS, typename real, bool FACTOR>
+__global__ __launch_bounds__(64 * NS) void k_lu_stream(
+    const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
+    const int4* __restrict__ taskB, const int* __restrict__ diagB, const int2* __restrict__ pairs,
+    const int* __restrict__ ptrF, const int4* __restrict__ fwd, const int2* __restrict__ fent,
+    const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int2* __restrict__ bent,
+    const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
+    unsigned* __restrict__ ctl, double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min,
+    int nlev, int nblev, int S) {
+    lu_stream_body<G, NS, real, FACTOR>(ptrA, taskA, ptrB, taskB, diagB, pairs, ptrF, fwd, fent, ptrW, bwd, bent, Jx,
+                                        fac, rhs, x, ctl, pstat, guard_trips, sel, rcond_min, nlev, nblev, S,
+                                        blockIdx.x, gridDim.x);
+}
+
+template <int G, int NS, typename real, bool FACTOR>
+__global__ __launch_bounds__(64 * NS) void k_lu_stream_multi(const LuArgs* __restrict__ tab,
+                                                             const int* __restrict__ off, int nseg) {
+    const int s = seg_find(off, nseg, blockIdx.x);
+    const LuArgs& a = tab[s];
+    const int b = blockIdx.x - off[s];
+    if (b >= a.gx || a.plan[PL_HALT] || a.plan[PL_LU] != a.want) return;
+    lu_stream_body<G, NS, real, FACTOR>(a.ptrA, a.taskA, a.ptrB, a.taskB, a.diagB, a.pairs, a.ptrF, a.fwd, a.fent,
+                                        a.ptrW, a.bwd, a.bent, a.Jx, static_cast<real*>(a.fac), a.rhs, a.x, a.ctl,
+                                        a.pstat, a.guard, a.sel, a.rcond_min, a.nlev, a.nblev, a.S, b, a.gx);
 }
 
 // ---- 2x2 supernode blocks, G = 64 ----------------------------------------------------
@@ -1034,20 +1174,21 @@ __device__ __forceinline__ void bblk_stream(const int4* __restrict__ tasks, int 
 // rows by pivot group (which read L(p+1, p), written by the diagonal block of a pair in
 // sub-level A).
 template <int NS, typename real, bool FACTOR>
-__global__ __launch_bounds__(64 * NS) void k_lu_blk(
+__device__ __forceinline__ void lu_blk_body(
     const int* __restrict__ ptrA, const int4* __restrict__ blkA, const int* __restrict__ ptrB,
     const int4* __restrict__ blkB, const int4* __restrict__ steps, const int* __restrict__ ptrF,
     const int4* __restrict__ fwd, const int4* __restrict__ fsteps, const int* __restrict__ ptrW,
     const int4* __restrict__ bwd, const int4* __restrict__ bsteps, const double* __restrict__ Jx,
     real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x, unsigned* __restrict__ ctl,
-    double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min, int nlev, int nblev, int S) {
+    double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min, int nlev, int nblev, int S,
+    int bx) {
     constexpr bool SINGLE = sizeof(real) == 4;
     constexpr int G = 64, TL = NS;
     __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
     __shared__ int sbad[G];
     const int lane = threadIdx.x % G;
     const int slot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G));
-    const int c = blockIdx.x * G + lane;
+    const int c = bx * G + lane;
     bool act = selected(sel, ctl[c]);
     if (!__syncthreads_or(act)) return;
 
@@ -1096,6 +1237,30 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
         if (act) bblk_stream<4, real>(bwd, ptrW[i0], ptrW[i0 + 1], bsteps, fac, x, S, c);
         __syncthreads();
     }
+}
+
+template <int NS, typename real, bool FACTOR>
+__global__ __launch_bounds__(64 * NS) void k_lu_blk(
+    const int* __restrict__ ptrA, const int4* __restrict__ blkA, const int* __restrict__ ptrB,
+    const int4* __restrict__ blkB, const int4* __restrict__ steps, const int* __restrict__ ptrF,
+    const int4* __restrict__ fwd, const int4* __restrict__ fsteps, const int* __restrict__ ptrW,
+    const int4* __restrict__ bwd, const int4* __restrict__ bsteps, const double* __restrict__ Jx,
+    real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x, unsigned* __restrict__ ctl,
+    double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min, int nlev, int nblev, int S) {
+    lu_blk_body<NS, real, FACTOR>(ptrA, blkA, ptrB, blkB, steps, ptrF, fwd, fsteps, ptrW, bwd, bsteps, Jx, fac, rhs, x,
+                                  ctl, pstat, guard_trips, sel, rcond_min, nlev, nblev, S, blockIdx.x);
+}
+
+template <int NS, typename real, bool FACTOR>
+__global__ __launch_bounds__(64 * NS) void k_lu_blk_multi(const LuArgs* __restrict__ tab, const int* __restrict__ off,
+                                                          int nseg) {
+    const int s = seg_find(off, nseg, blockIdx.x);
+    const LuArgs& a = tab[s];
+    const int b = blockIdx.x - off[s];
+    if (b >= a.gx || a.plan[PL_HALT] || a.plan[PL_LU] != a.want) return;
+    lu_blk_body<NS, real, FACTOR>(a.ptrA, a.taskA, a.ptrB, a.taskB, a.steps, a.ptrF, a.fwd, a.fsteps, a.ptrW, a.bwd,
+                                  a.bsteps, a.Jx, static_cast<real*>(a.fac), a.rhs, a.x, a.ctl, a.pstat, a.guard,
+                                  a.sel, a.rcond_min, a.nlev, a.nblev, a.S, b);
 }
 
 // Refinement residual r = R - J x in FP64 (J in equation-row CSR order, x in pivot order:
@@ -1218,42 +1383,71 @@ __global__ void k_column(const double* __restrict__ src, double* __restrict__ ds
 // step (alpha 1) from the current point and saves that point as the step's base; TRIAL and
 // COMMIT move to base + alpha dx. The solve's x is in pivot order: x[xva[bus]] the angle step,
 // x[xvm[bus]] the relative magnitude step dVm / Vm (magnitude columns are in ln |V|), so
-// Vm = Vm0 (1 + alpha x). One thread per (bus, case); the slack has neither and stays.
-__global__ __launch_bounds__(256) void k_advance(const double* __restrict__ x, const int* __restrict__ xva,
-                                                 const int* __restrict__ xvm, const unsigned* __restrict__ ctl,
-                                                 const double* __restrict__ alpha, double* __restrict__ Vm,
-                                                 double* __restrict__ Va, double* __restrict__ Ve,
-                                                 double* __restrict__ Vf, double* __restrict__ Vm0,
-                                                 double* __restrict__ Va0, int hold, int n, int S) {
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int bus = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (bus >= n) return;
-    const int iva = xva[bus], ivm = xvm[bus];
-    if (iva < 0 && ivm < 0) return;
+// Vm = Vm0 (1 + alpha x). One thread per (bus, case); the slack has neither and stays. Workgroup
+// (bx, by) of gy along the buses: the 64 cases of group bx, buses by * 4 + wave stepping by gy * 4.
+struct AdvanceArgs {
+    const double* x;
+    const int *xva, *xvm;
+    const unsigned* ctl;
+    const double* alpha;
+    double *Vm, *Va, *Ve, *Vf, *Vm0, *Va0;
+    int hold, n, S;
+    int gx, gy;
+    const int* plan;  // passes driven on the device: the hold and whether to run come from it
+};
+
+__device__ __forceinline__ void advance_body(const double* __restrict__ x, const int* __restrict__ xva,
+                                             const int* __restrict__ xvm, const unsigned* __restrict__ ctl,
+                                             const double* __restrict__ alpha, double* __restrict__ Vm,
+                                             double* __restrict__ Va, double* __restrict__ Ve,
+                                             double* __restrict__ Vf, double* __restrict__ Vm0,
+                                             double* __restrict__ Va0, int hold, int n, int S, int bx, int by,
+                                             int gy) {
+    const int c = bx * 64 + (threadIdx.x & 63);
     const unsigned w = ctl[c];
     const unsigned sg = stage_of(w);
     if (sg == SG_START || !moves(w, hold)) return;
-    const size_t o = at(bus, S, c);
-    double va, vm, step;
-    if (sg == SG_STEP) {
-        va = Va[o];
-        vm = Vm[o];
-        Va0[o] = va;
-        Vm0[o] = vm;
-        step = 1.0;
-    } else {
-        va = Va0[o];
-        vm = Vm0[o];
-        step = alpha[c];
+    const double al = sg == SG_STEP ? 1.0 : alpha[c];
+    for (int bus = by * 4 + (int)(threadIdx.x >> 6); bus < n; bus += gy * 4) {
+        const int iva = xva[bus], ivm = xvm[bus];
+        if (iva < 0 && ivm < 0) continue;
+        const size_t o = at(bus, S, c);
+        double va, vm, step;
+        if (sg == SG_STEP) {
+            va = Va[o];
+            vm = Vm[o];
+            Va0[o] = va;
+            Vm0[o] = vm;
+            step = 1.0;
+        } else {
+            va = Va0[o];
+            vm = Vm0[o];
+            step = al;
+        }
+        if (iva >= 0) va += step * x[at(iva, S, c)];
+        if (ivm >= 0) vm = fma(vm, step * x[at(ivm, S, c)], vm);
+        double s, co;
+        sincos(va, &s, &co);
+        Va[o] = va;
+        Vm[o] = vm;
+        Ve[o] = vm * co;
+        Vf[o] = vm * s;
     }
-    if (iva >= 0) va += step * x[at(iva, S, c)];
-    if (ivm >= 0) vm = fma(vm, step * x[at(ivm, S, c)], vm);
-    double s, co;
-    sincos(va, &s, &co);
-    Va[o] = va;
-    Vm[o] = vm;
-    Ve[o] = vm * co;
-    Vf[o] = vm * s;
+}
+
+__global__ __launch_bounds__(256) void k_advance(AdvanceArgs a) {
+    advance_body(a.x, a.xva, a.xvm, a.ctl, a.alpha, a.Vm, a.Va, a.Ve, a.Vf, a.Vm0, a.Va0, a.hold, a.n, a.S, blockIdx.x,
+                 blockIdx.y, gridDim.y);
+}
+
+__global__ __launch_bounds__(256) void k_advance_multi(const AdvanceArgs* __restrict__ tab,
+                                                       const int* __restrict__ off, int nseg) {
+    const int s = seg_find(off, nseg, blockIdx.x);
+    const AdvanceArgs& a = tab[s];
+    const int b = blockIdx.x - off[s];
+    if (b >= a.gx * a.gy || a.plan[PL_HALT] || !a.plan[PL_ADVANCE]) return;
+    advance_body(a.x, a.xva, a.xvm, a.ctl, a.alpha, a.Vm, a.Va, a.Ve, a.Vf, a.Vm0, a.Va0, a.plan[PL_HOLD], a.n, a.S,
+                 b % a.gx, b / a.gx, a.gy);
 }
 
 // Precision vote of the switching mixed mode after an accepted step (the reference GPU
@@ -1294,18 +1488,22 @@ struct CtlArgs {
     double tol, sw;      // sw > 0: switching mixed mode
     int line_search, max_iter, S;
     int hold;            // 1: the cases in STEP waited this pass
+    int gx;              // workgroups (S / 256 rounded up)
+    int few_max;         // LU: the few-case shape for at most this many cases in STEP
+    int* plan;           // passes driven on the device: the hold comes from it, and the last
+                         // workgroup writes the plan of the next pass there
 };
 
-__global__ __launch_bounds__(256) void k_control(CtlArgs a) {
+__device__ __forceinline__ void control_body(const CtlArgs& a, int hold, int bx, int gx) {
     __shared__ int s_cnt[NCOUNT];
     __shared__ int s_last;
     if (threadIdx.x < NCOUNT) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = bx * blockDim.x + threadIdx.x;
     if (c < a.S) {
         unsigned w = a.ctl[c];
         unsigned sg = stage_of(w);
-        if (moves(w, a.hold)) {
+        if (moves(w, hold)) {
             double nw = 0.0;
             for (int t = 0; t < a.nparts; ++t) nw = nan_max(nw, a.part[at(t, a.S, c)]);
             if (!(nw < DBL_MAX)) nw = DBL_MAX;
@@ -1372,13 +1570,34 @@ __global__ __launch_bounds__(256) void k_control(CtlArgs a) {
     if (threadIdx.x < NCOUNT && s_cnt[threadIdx.x]) atomicAdd(&a.tally[threadIdx.x], s_cnt[threadIdx.x]);
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(&a.tally[NCOUNT], 1) == (int)gridDim.x - 1;
+    if (threadIdx.x == 0) s_last = atomicAdd(&a.tally[NCOUNT], 1) == gx - 1;
     __syncthreads();
     if (s_last) {
         __threadfence();
-        if (threadIdx.x < NCOUNT) a.counts_out[threadIdx.x] = atomicExch(&a.tally[threadIdx.x], 0);
+        if (threadIdx.x < NCOUNT) {
+            const int v = atomicExch(&a.tally[threadIdx.x], 0);
+            a.counts_out[threadIdx.x] = v;
+            s_cnt[threadIdx.x] = v;
+        }
         if (threadIdx.x == 0) atomicExch(&a.tally[NCOUNT], 0);
+        __syncthreads();
+        if (a.plan && threadIdx.x == 0) {
+            int pl[NPLAN];
+            plan_from_counts(s_cnt, a.few_max, pl);
+            for (int k = 0; k < NPLAN; ++k) a.plan[k] = pl[k];
+        }
     }
+}
+
+__global__ __launch_bounds__(256) void k_control(CtlArgs a) { control_body(a, a.hold, blockIdx.x, gridDim.x); }
+
+__global__ __launch_bounds__(256) void k_control_multi(const CtlArgs* __restrict__ tab, const int* __restrict__ off,
+                                                       int nseg) {
+    const int s = seg_find(off, nseg, blockIdx.x);
+    const CtlArgs& a = tab[s];
+    const int b = blockIdx.x - off[s];
+    if (b >= a.gx || a.plan[PL_HALT]) return;
+    control_body(a, a.plan[PL_HOLD], b, a.gx);
 }
 
 // Complex V (n, B) of the cases' state, the layout of the numpy result; NaN at buses a served
@@ -2928,12 +3147,10 @@ void collect_timing(p3smi_batch* b) {
 
 dim3 grid_items(int items, int S) { return dim3((unsigned)(S / 64), (unsigned)((items + 3) / 4)); }
 
-void launch_evaluate(p3smi_batch* b, int hold) {
+EvalArgs eval_args(p3smi_batch* b, int hold) {
     p3smi_topology* t = b->topo;
     const p3smi::Topology& h = t->host;
     const DevTopo& d = t->dev;
-    if (b->part.n < (size_t)d.ntiles * b->S) b->part.alloc((size_t)d.ntiles * b->S);
-    KTimer tm(b, 0);
     EvalArgs a;
     a.tile_ptr = d.tiles.p;
     a.Yp = d.Yp.p;
@@ -2956,6 +3173,17 @@ void launch_evaluate(p3smi_batch* b, int hold) {
     a.part = b->part.p;
     a.n = h.n;
     a.S = b->S;
+    a.gx = d.ntiles;
+    a.plan = nullptr;
+    return a;
+}
+
+void launch_evaluate(p3smi_batch* b, int hold) {
+    p3smi_topology* t = b->topo;
+    const DevTopo& d = t->dev;
+    if (b->part.n < (size_t)d.ntiles * b->S) b->part.alloc((size_t)d.ntiles * b->S);
+    KTimer tm(b, 0);
+    const EvalArgs a = eval_args(b, hold);
     const dim3 grid((unsigned)d.ntiles, (unsigned)(b->S / 64));
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(64 * EV_WAVES), 0, t->stream, a); };
     if (b->per_case_y && b->has_pin) go(k_evaluate<true, true>);
@@ -2983,6 +3211,43 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* 
                        b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.diagB.p, ds.pairs.p,
                        ds.ptrF.p, ds.fwd.p, ds.fent.p, ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout,
                        b->ctl.p, b->pstat.p, b->guard.p, sel, rcond_min, ds.nlev, ds.nblev, b->S);
+}
+
+// The arguments of an LU launch of shape cfg over batch b (factor + solve of the selected cases).
+LuArgs lu_args(p3smi_batch* b, LuConfig cfg, const DevSched& ds, void* fac, const double* rhs, double* xout, int sel) {
+    const p3smi::Topology& h = b->topo->host;
+    LuArgs a{};
+    a.ptrA = ds.ptrA.p;
+    a.taskA = ds.taskA.p;
+    a.ptrB = ds.ptrB.p;
+    a.taskB = ds.taskB.p;
+    a.diagB = ds.diagB.p;
+    a.pairs = ds.pairs.p;
+    a.steps = ds.steps.p;
+    a.ptrF = ds.ptrF.p;
+    a.fwd = ds.fwd.p;
+    a.fent = ds.fent.p;
+    a.fsteps = ds.fsteps.p;
+    a.ptrW = ds.ptrW.p;
+    a.bwd = ds.bwd.p;
+    a.bent = ds.bent.p;
+    a.bsteps = ds.bsteps.p;
+    a.Jx = b->Jx.p;
+    a.fac = fac;
+    a.rhs = rhs;
+    a.x = xout;
+    a.ctl = b->ctl.p;
+    a.pstat = b->pstat.p;
+    a.guard = b->guard.p;
+    a.sel = sel;
+    a.rcond_min = (b->guard_override >= 0.0 ? b->guard_override : b->topo->guard_ratio) * h.rcond0;
+    a.nlev = ds.nlev;
+    a.nblev = ds.nblev;
+    a.S = b->S;
+    a.gx = b->S / cfg.G;
+    a.plan = nullptr;
+    a.want = 0;
+    return a;
 }
 
 bool valid_lu_config(LuConfig c) {
@@ -3279,11 +3544,77 @@ int pivot_solve(p3smi_batch* b) {
 // precision class a case starts in
 unsigned start_class(int precision) { return precision == P3SMI_PREC_MIXED ? PR_F32 : PR_F64; }
 
+BeginArgs begin_args(p3smi_batch* b, int precision, dim3 grid) {
+    BeginArgs a;
+    a.V0 = b->V0.p;
+    a.v0_stride = b->v0_stride;
+    a.ctl = b->ctl.p;
+    a.p0 = start_class(precision);
+    a.Vm = b->Vm.p;
+    a.Va = b->Va.p;
+    a.Ve = b->Ve.p;
+    a.Vf = b->Vf.p;
+    a.n = b->topo->host.n;
+    a.S = b->S;
+    a.gx = (int)grid.x;
+    a.gy = (int)grid.y;
+    a.plan = nullptr;
+    return a;
+}
+
+AdvanceArgs advance_args(p3smi_batch* b, int hold, dim3 grid) {
+    const DevTopo& d = b->topo->dev;
+    AdvanceArgs a;
+    a.x = b->x.p;
+    a.xva = d.xva.p;
+    a.xvm = d.xvm.p;
+    a.ctl = b->ctl.p;
+    a.alpha = b->alpha.p;
+    a.Vm = b->Vm.p;
+    a.Va = b->Va.p;
+    a.Ve = b->Ve.p;
+    a.Vf = b->Vf.p;
+    a.Vm0 = b->Vm0.p;
+    a.Va0 = b->Va0.p;
+    a.hold = hold;
+    a.n = b->topo->host.n;
+    a.S = b->S;
+    a.gx = (int)grid.x;
+    a.gy = (int)grid.y;
+    a.plan = nullptr;
+    return a;
+}
+
 struct PassParams {
     int max_iter, line_search, precision;
     double tol;
     bool lockstep;  // hold the cases in STEP while any case is in its line search
 };
+
+CtlArgs ctl_args(p3smi_batch* b, const PassParams& p, int hold) {
+    p3smi_topology* t = b->topo;
+    CtlArgs a;
+    a.part = b->part.p;
+    a.nparts = t->dev.ntiles;
+    a.ctl = b->ctl.p;
+    a.nrm = b->nrm.p;
+    a.alpha = b->alpha.p;
+    a.best_alpha = b->best_alpha.p;
+    a.best_nrm = b->best_nrm.p;
+    a.tally = b->tally.p;
+    a.counts_out = b->counts.p;
+    a.tol = p.tol;
+    a.sw = p.precision == P3SMI_PREC_MIXED_SWITCH ? t->mixed_switch : 0.0;
+    a.line_search = p.line_search;
+    a.max_iter = p.max_iter;
+    a.S = b->S;
+    a.hold = hold;
+    a.reruns = b->guard.p + 3;
+    a.gx = (b->S + 255) / 256;
+    a.few_max = t->lu_few_max;
+    a.plan = nullptr;
+    return a;
+}
 
 // The kernels of one pass (see the file header); the LU only when cases are in STEP, k_begin
 // only when cases are in START.
@@ -3295,38 +3626,21 @@ void pass_kernels(p3smi_batch* b, const PassParams& p, int n_solve, bool startin
     if (pivoting) pivot_solve(b);
     if (starting) {
         KTimer tm(b, 4);
-        hipLaunchKernelGGL(k_begin, grid_items(h.n, b->S), dim3(256), 0, st, b->V0.p, b->v0_stride, b->ctl.p,
-                           start_class(p.precision), b->Vm.p, b->Va.p, b->Ve.p, b->Vf.p, h.n, b->S);
+        const dim3 grid = grid_items(h.n, b->S);
+        hipLaunchKernelGGL(k_begin, grid, dim3(256), 0, st, begin_args(b, p.precision, grid));
         HIPCHECK(hipGetLastError());
     }
     if ((n_solve > 0 && !hold) || b->h_counts[CNT_TRIAL] > 0 || b->h_counts[CNT_COMMIT] > 0) {
         KTimer tm(b, 2);
-        hipLaunchKernelGGL(k_advance, grid_items(h.n, b->S), dim3(256), 0, st, b->x.p, t->dev.xva.p, t->dev.xvm.p,
-                           b->ctl.p, b->alpha.p, b->Vm.p, b->Va.p, b->Ve.p, b->Vf.p, b->Vm0.p, b->Va0.p, hold, h.n,
-                           b->S);
+        const dim3 grid = grid_items(h.n, b->S);
+        hipLaunchKernelGGL(k_advance, grid, dim3(256), 0, st, advance_args(b, hold, grid));
         HIPCHECK(hipGetLastError());
     }
     launch_evaluate(b, hold);
     {
         KTimer tm(b, 3);
-        CtlArgs a;
-        a.part = b->part.p;
-        a.nparts = t->dev.ntiles;
-        a.ctl = b->ctl.p;
-        a.nrm = b->nrm.p;
-        a.alpha = b->alpha.p;
-        a.best_alpha = b->best_alpha.p;
-        a.best_nrm = b->best_nrm.p;
-        a.tally = b->tally.p;
-        a.counts_out = b->counts.p;
-        a.tol = p.tol;
-        a.sw = p.precision == P3SMI_PREC_MIXED_SWITCH ? t->mixed_switch : 0.0;
-        a.line_search = p.line_search;
-        a.max_iter = p.max_iter;
-        a.S = b->S;
-        a.hold = hold;
-        a.reruns = b->guard.p + 3;
-        hipLaunchKernelGGL(k_control, dim3((unsigned)((b->S + 255) / 256)), dim3(256), 0, st, a);
+        const CtlArgs a = ctl_args(b, p, hold);
+        hipLaunchKernelGGL(k_control, dim3((unsigned)a.gx), dim3(256), 0, st, a);
         HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipMemcpyAsync(b->h_counts, b->counts.p, NCOUNT * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -4818,6 +5132,336 @@ int p3smi_batch_fetch(p3smi_batch* b, int32_t k, const int32_t* slots, double* V
         if (mismatch) HIPCHECK(hipMemcpyAsync(mismatch, b->st_mis.p, k * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         if (converged) std::copy(cv.begin(), cv.end(), converged);
+    });
+}
+
+}  // extern "C"
+
+// ===================================================================== batch groups
+// Batches of different topologies and sizes solved together (uneven batching): every pass is one
+// launch per kernel -- per LU launch shape and per evaluation variant -- over all the batches,
+// each launch's workgroups dealt to the batches by a segment table (seg_find). The passes run on
+// the device: each batch's k_control writes the plan of its next pass (plan_from_counts, the
+// decisions of the host's pass loop), every kernel reads it, and a batch with nothing to do
+// halts. K passes are captured into one graph, so the host synchronises once per K passes.
+// A batch with a case waiting for a pivoting factorization halts; the host runs that pass for it
+// alone (run_pass) and the group goes on. FP64 solves of whole batches (lock-step passes).
+struct GroupLaunch {
+    int kind = 0;  // LU: 0 one-entry tasks (k_lu_stream_multi), 1 2x2 blocks (k_lu_blk_multi);
+                   // evaluation: 2 * per-case Ybus + pin mask
+    int G = 0, NS = 0;
+    DevBuf<char> tab;
+    DevBuf<int> off;
+    int nseg = 0, blocks = 0;
+};
+
+struct p3smi_group {
+    std::vector<p3smi_batch*> mem;
+    hipStream_t stream = nullptr;
+    DevBuf<int> plans;   // NPLAN per member
+    DevBuf<int> counts;  // NCOUNT per member (k_control's counters)
+    int* h_plans = nullptr;
+    int* h_counts = nullptr;
+    std::vector<std::unique_ptr<GroupLaunch>> lu, eval;
+    GroupLaunch begin, adv, ctl;
+    std::string built;   // solve parameters and buffers the tables were built for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_passes = 0;
+    long long device_passes = 0, eager_passes = 0, graph_launches = 0;
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_passes = 0;
+    }
+    ~p3smi_group() {
+        drop_graph();
+        if (h_plans) (void)hipHostFree(h_plans);
+        if (h_counts) (void)hipHostFree(h_counts);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Segment table of one launch: args[i] with blocks[i] workgroups, ranges padded to 8.
+template <class A>
+void group_table(GroupLaunch& L, const std::vector<A>& args, const std::vector<int>& blocks, hipStream_t st) {
+    std::vector<int> off(args.size() + 1, 0);
+    for (size_t i = 0; i < args.size(); ++i) {
+        if (blocks[i] <= 0) throw Error(P3SMI_E_INTERNAL, "group: a segment without workgroups");
+        off[i + 1] = off[i] + (blocks[i] + 7) / 8 * 8;
+    }
+    L.tab.upload(reinterpret_cast<const char*>(args.data()), args.size() * sizeof(A), st);
+    L.off.upload(off.data(), off.size(), st);
+    L.nseg = (int)args.size();
+    L.blocks = off.back();
+    HIPCHECK(hipStreamSynchronize(st));  // the host vectors go out of scope
+}
+
+// workgroups along the buses of k_begin_multi / k_advance_multi (each thread steps through the rest)
+int group_bus_blocks(int n) { return std::min((n + 3) / 4, 64); }
+
+void group_build(p3smi_group* g, const PassParams& p) {
+    hipStream_t st = g->stream;
+    const int nm = (int)g->mem.size();
+    g->lu.clear();
+    g->eval.clear();
+    g->drop_graph();
+    std::map<std::tuple<int, int, int>, std::pair<std::vector<LuArgs>, std::vector<int>>> lu;
+    std::map<int, std::pair<std::vector<EvalArgs>, std::vector<int>>> ev;
+    std::vector<BeginArgs> bg;
+    std::vector<AdvanceArgs> ad;
+    std::vector<CtlArgs> ct;
+    std::vector<int> bgb, adb, ctb;
+    for (int i = 0; i < nm; ++i) {
+        p3smi_batch* b = g->mem[i];
+        p3smi_topology* t = b->topo;
+        const p3smi::Topology& h = t->host;
+        const int* plan = g->plans.p + (size_t)i * NPLAN;
+        const LuConfig cfgs[2] = {t->lu_batch, t->lu_few};
+        for (int w = 0; w < 2; ++w) {
+            const LuConfig cfg = cfgs[w];
+            if (!valid_lu_config(cfg)) throw Error(P3SMI_E_ARG, "group: unsupported LU launch shape");
+            LuArgs a = lu_args(b, cfg, get_sched(t, cfg), b->fac.p, b->R.p, b->x.p, SEL_ALL);
+            a.plan = plan;
+            a.want = w + 1;
+            auto& e = lu[std::make_tuple(cfg.blk() ? 1 : 0, cfg.G, cfg.NS)];
+            e.first.push_back(a);
+            e.second.push_back(a.gx);
+        }
+        const dim3 gi((unsigned)(b->S / 64), (unsigned)group_bus_blocks(h.n));
+        BeginArgs ba = begin_args(b, p.precision, gi);
+        ba.plan = plan;
+        bg.push_back(ba);
+        bgb.push_back(ba.gx * ba.gy);
+        AdvanceArgs aa = advance_args(b, 0, gi);
+        aa.plan = plan;
+        ad.push_back(aa);
+        adb.push_back(aa.gx * aa.gy);
+        EvalArgs ea = eval_args(b, 0);
+        ea.plan = plan;
+        auto& e = ev[2 * (int)b->per_case_y + (int)b->has_pin];
+        e.first.push_back(ea);
+        e.second.push_back(t->dev.ntiles * (b->S / 64));
+        CtlArgs ca = ctl_args(b, p, 0);
+        ca.counts_out = g->counts.p + (size_t)i * NCOUNT;
+        ca.plan = g->plans.p + (size_t)i * NPLAN;
+        ct.push_back(ca);
+        ctb.push_back(ca.gx);
+    }
+    for (auto& kv : lu) {
+        std::unique_ptr<GroupLaunch> L(new GroupLaunch());
+        L->kind = std::get<0>(kv.first);
+        L->G = std::get<1>(kv.first);
+        L->NS = std::get<2>(kv.first);
+        group_table(*L, kv.second.first, kv.second.second, st);
+        g->lu.push_back(std::move(L));
+    }
+    for (auto& kv : ev) {
+        std::unique_ptr<GroupLaunch> L(new GroupLaunch());
+        L->kind = kv.first;
+        group_table(*L, kv.second.first, kv.second.second, st);
+        g->eval.push_back(std::move(L));
+    }
+    group_table(g->begin, bg, bgb, st);
+    group_table(g->adv, ad, adb, st);
+    group_table(g->ctl, ct, ctb, st);
+}
+
+template <int G, int NS>
+void group_lu_launch(const GroupLaunch& L, hipStream_t st) {
+    const LuArgs* tab = reinterpret_cast<const LuArgs*>(L.tab.p);
+    if constexpr (G == 64) {
+        if (L.kind == 1) {
+            hipLaunchKernelGGL((k_lu_blk_multi<NS, double, true>), dim3((unsigned)L.blocks), dim3(64 * NS), 0, st, tab,
+                               L.off.p, L.nseg);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_lu_stream_multi<G, NS, double, true>), dim3((unsigned)L.blocks), dim3(64 * NS), 0, st, tab,
+                       L.off.p, L.nseg);
+}
+
+// One pass over every member (the kernels a batch's plan does not ask for return at once).
+void group_pass(p3smi_group* g) {
+    hipStream_t st = g->stream;
+    for (const auto& L : g->lu) {
+#define P3SMI_GLU(GG, NN)                          \
+    if (L->G == GG && L->NS == NN) {               \
+        group_lu_launch<GG, NN>(*L, st);           \
+        continue;                                  \
+    }
+        P3SMI_GLU(64, 8) P3SMI_GLU(64, 16) P3SMI_GLU(32, 8) P3SMI_GLU(32, 16) P3SMI_GLU(16, 8) P3SMI_GLU(16, 16)
+        P3SMI_GLU(1, 4) P3SMI_GLU(1, 8) P3SMI_GLU(1, 16)
+#undef P3SMI_GLU
+        throw Error(P3SMI_E_ARG, "group: unsupported LU launch shape");
+    }
+    hipLaunchKernelGGL(k_begin_multi, dim3((unsigned)g->begin.blocks), dim3(256), 0, st,
+                       reinterpret_cast<const BeginArgs*>(g->begin.tab.p), g->begin.off.p, g->begin.nseg);
+    hipLaunchKernelGGL(k_advance_multi, dim3((unsigned)g->adv.blocks), dim3(256), 0, st,
+                       reinterpret_cast<const AdvanceArgs*>(g->adv.tab.p), g->adv.off.p, g->adv.nseg);
+    for (const auto& L : g->eval) {
+        const EvalArgs* tab = reinterpret_cast<const EvalArgs*>(L->tab.p);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3((unsigned)L->blocks), dim3(64 * EV_WAVES), 0, st, tab, L->off.p, L->nseg);
+        };
+        if (L->kind == 3) go(k_evaluate_multi<true, true>);
+        else if (L->kind == 2) go(k_evaluate_multi<true, false>);
+        else if (L->kind == 1) go(k_evaluate_multi<false, true>);
+        else go(k_evaluate_multi<false, false>);
+    }
+    hipLaunchKernelGGL(k_control_multi, dim3((unsigned)g->ctl.blocks), dim3(256), 0, st,
+                       reinterpret_cast<const CtlArgs*>(g->ctl.tab.p), g->ctl.off.p, g->ctl.nseg);
+    HIPCHECK(hipGetLastError());
+}
+
+void group_solve(p3smi_group* g, int max_iter, double tol, int line_search, int precision, int K) {
+    check_solve_args(max_iter, tol, precision);
+    if (precision != P3SMI_PREC_DOUBLE) throw Error(P3SMI_E_ARG, "group solve: precision must be 0 (double)");
+    if (K < 1 || K > 64) throw Error(P3SMI_E_ARG, "group solve: passes per graph must be in [1, 64]");
+    const int nm = (int)g->mem.size();
+    const PassParams p{max_iter, line_search ? 1 : 0, precision, tol, true};
+    std::string key = std::to_string(max_iter) + "/" + std::to_string(tol) + "/" + std::to_string(line_search);
+    for (p3smi_batch* b : g->mem) {
+        p3smi_topology* t = b->topo;
+        if (!b->uploaded) throw Error(P3SMI_E_ARG, "group solve: a batch has no inputs (upload first)");
+        if (b->slot_mode) throw Error(P3SMI_E_ARG, "group solve: a batch is in slot mode");
+        ensure_tiles(t);
+        if (b->part.n < (size_t)t->dev.ntiles * b->S) b->part.alloc((size_t)t->dev.ntiles * b->S);
+        (void)get_sched(t, t->lu_batch);
+        (void)get_sched(t, t->lu_few);
+        std::memset(&b->stats, 0, sizeof b->stats);
+        b->precision = precision;
+        b->solve_id = ++g_solve_ids;
+        b->stats.cases = b->B;
+        reset_cases(b, precision);
+        HIPCHECK(hipStreamSynchronize(t->stream));
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "|%p/%p/%p/%p/%d/%d/%d/%d", (void*)b, (void*)b->part.p, (void*)b->Ssp.p,
+                      (void*)b->V0.p, t->lu_batch.key(), t->lu_few.key(), t->lu_few_max, t->dev.tile_rows);
+        key += buf;
+    }
+    if (g->built != key) {
+        group_build(g, p);
+        g->built = key;
+    }
+    if (g->graph_passes != K) {
+        g->drop_graph();
+        HIPCHECK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int k = 0; k < K; ++k) group_pass(g);
+        } catch (...) {
+            hipGraph_t gr = nullptr;
+            (void)hipStreamEndCapture(g->stream, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+        }
+        HIPCHECK(hipStreamEndCapture(g->stream, &g->graph));
+        if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+            g->drop_graph();
+            throw Error(P3SMI_E_DEVICE, "group solve: hipGraphInstantiate failed");
+        }
+        g->graph_passes = K;
+    }
+    // every run of a case takes at most 1 + max_iter * (LS_MAX_TRIALS + 3) passes, two runs per case
+    const long long cap = 2 * (1 + (long long)max_iter * (LS_MAX_TRIALS + 3)) + 2;
+    long long rounds = 0;
+    g->device_passes = g->eager_passes = g->graph_launches = 0;
+    for (;;) {
+        if (++rounds > cap) throw Error(P3SMI_E_INTERNAL, "group solve: passes did not finish every case");
+        bool any_live = false, any_device = false;
+        for (int i = 0; i < nm; ++i) {
+            p3smi_batch* b = g->mem[i];
+            int* pl = g->h_plans + (size_t)i * NPLAN;
+            plan_from_counts(b->h_counts, b->topo->lu_few_max, pl);
+            if (live_cases(b->h_counts) == 0) continue;
+            any_live = true;
+            if (pl[PL_HALT]) {  // a case waits for its pivoting factorization: this pass on the host
+                run_pass(b, p);
+                g->eager_passes++;
+                plan_from_counts(b->h_counts, b->topo->lu_few_max, pl);
+            }
+            if (!pl[PL_HALT]) any_device = true;
+        }
+        if (!any_live) break;
+        if (!any_device) continue;
+        HIPCHECK(hipMemcpyAsync(g->plans.p, g->h_plans, (size_t)nm * NPLAN * sizeof(int), hipMemcpyHostToDevice,
+                                g->stream));
+        HIPCHECK(hipGraphLaunch(g->exec, g->stream));
+        HIPCHECK(hipMemcpyAsync(g->h_counts, g->counts.p, (size_t)nm * NCOUNT * sizeof(int), hipMemcpyDeviceToHost,
+                                g->stream));
+        HIPCHECK(hipStreamSynchronize(g->stream));
+        g->graph_launches++;
+        g->device_passes += K;
+        for (int i = 0; i < nm; ++i) {
+            p3smi_batch* b = g->mem[i];
+            if (g->h_plans[(size_t)i * NPLAN + PL_HALT]) continue;  // the device did not touch it
+            std::copy(g->h_counts + (size_t)i * NCOUNT, g->h_counts + (size_t)(i + 1) * NCOUNT, b->h_counts);
+            b->pending_start = false;
+        }
+    }
+    for (p3smi_batch* b : g->mem) {
+        int gd[6];
+        HIPCHECK(hipMemcpyAsync(gd, b->guard.p, sizeof gd, hipMemcpyDeviceToHost, g->stream));
+        HIPCHECK(hipStreamSynchronize(g->stream));
+        b->stats.failed_factorizations = b->h_counts[CNT_FAIL];
+        b->stats.guard_trips = gd[0];
+        b->stats.reruns = gd[3];
+        b->stats.pivot_factorizations = gd[4];
+        b->stats.diagonal_factorizations = gd[5];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int p3smi_group_create(p3smi_batch* const* members, int32_t count, p3smi_group** out) {
+    return guarded([&] {
+        if (!out) throw Error(P3SMI_E_ARG, "group: null output");
+        *out = nullptr;
+        if (count <= 0 || !members) throw Error(P3SMI_E_ARG, "group: needs at least one batch");
+        std::unique_ptr<p3smi_group> g(new p3smi_group());
+        for (int i = 0; i < count; ++i) {
+            if (!members[i]) throw Error(P3SMI_E_ARG, "group: null batch");
+            for (p3smi_batch* b : g->mem)
+                if (b == members[i]) throw Error(P3SMI_E_ARG, "group: a batch appears twice");
+            g->mem.push_back(members[i]);
+        }
+        HIPCHECK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+        g->plans.alloc((size_t)count * NPLAN);
+        g->counts.alloc((size_t)count * NCOUNT);
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&g->h_plans), (size_t)count * NPLAN * sizeof(int)));
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&g->h_counts), (size_t)count * NCOUNT * sizeof(int)));
+        *out = g.release();
+    });
+}
+
+void p3smi_group_free(p3smi_group* g) { delete g; }
+
+int p3smi_group_solve(p3smi_group* g, int32_t max_iter, double tol, int32_t line_search, int32_t precision,
+                      int32_t passes_per_graph) {
+    return guarded([&] {
+        if (!g) throw Error(P3SMI_E_ARG, "group solve: null group");
+        std::vector<std::unique_lock<std::mutex>> locks;
+        std::vector<p3smi_topology*> topos;
+        for (p3smi_batch* b : g->mem)
+            if (std::find(topos.begin(), topos.end(), b->topo) == topos.end()) topos.push_back(b->topo);
+        std::sort(topos.begin(), topos.end());
+        for (p3smi_topology* t : topos) locks.emplace_back(t->mu);
+        group_solve(g, max_iter, tol, line_search, precision, passes_per_graph);
+    });
+}
+
+int p3smi_group_stats(const p3smi_group* g, int64_t* out3) {
+    return guarded([&] {
+        if (!g || !out3) throw Error(P3SMI_E_ARG, "group stats: null argument");
+        out3[0] = g->device_passes;
+        out3[1] = g->eager_passes;
+        out3[2] = g->graph_launches;
     });
 }
 

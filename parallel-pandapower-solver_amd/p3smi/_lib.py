@@ -92,6 +92,7 @@ EXPORTS = [
     "p3smi_topology_release_cache", "p3smi_topology_last_batch", "p3smi_batch_vrange", "p3smi_batch_stats", "p3smi_batch_set_graphs", "p3smi_topology_set_graphs", "p3smi_batch_set_timing", "p3smi_batch_bytes_per_case",
     "p3smi_outages_create", "p3smi_outages_free", "p3smi_outages_islands", "p3smi_outages_islands_noalloc", "p3smi_outages_solve",
     "p3smi_batch_slots_init", "p3smi_batch_load", "p3smi_batch_pass", "p3smi_batch_finished", "p3smi_batch_fetch",
+    "p3smi_group_create", "p3smi_group_free", "p3smi_group_solve", "p3smi_group_stats",
 ]
 
 _lib = None
@@ -165,6 +166,11 @@ def lib():
     L.p3smi_batch_pass.argtypes = [P, _i32, _f64, _i32, _i32, P]
     L.p3smi_batch_finished.argtypes = [P, P, P]
     L.p3smi_batch_fetch.argtypes = [P, _i32, P, P, P, P, P]
+    L.p3smi_group_create.argtypes = [P, _i32, P]
+    L.p3smi_group_free.argtypes = [P]
+    L.p3smi_group_free.restype = None
+    L.p3smi_group_solve.argtypes = [P, _i32, _f64, _i32, _i32, _i32]
+    L.p3smi_group_stats.argtypes = [P, P]
     _lib = L
     return L
 

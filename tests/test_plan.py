@@ -195,3 +195,16 @@ def test_block_emulation_rejects_bad_arguments():
     d = s.jacobian_rows_at(g.sbus(), g.flat_start())
     with pytest.raises(RuntimeError, match="nslots"):
         s.host_lu_emulate(d["Jx"], -d["F"], blocks=True, nslots=0)
+
+
+def test_group_rejects_bad_arguments_without_a_device():
+    import ctypes as C
+
+    from p3smi import _lib
+
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.p3smi_group_create(None, 0, C.byref(h)) != 0
+    assert b"at least one batch" in L.p3smi_last_error()
+    assert L.p3smi_group_solve(None, 20, 1e-8, 1, 0, 8) != 0
+    assert b"null group" in L.p3smi_last_error()
