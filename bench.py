@@ -645,6 +645,7 @@ def run_multi48(args) -> dict:
                      "V0": np.ascontiguousarray(dc_initial_voltage(g))})
         solvers.append(nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq))
     log(f"[bench] multi48: sizes {min(g.n for g in fam)}..{max(g.n for g in fam)} buses, {int(counts.sum())} cases")
+    grp = run_group(args, jobs, solvers)
     res = {}
     for prec in (args.precision, "mixed" if args.precision == "double" else "double"):
         srv = Server(tol=TOL, max_iter=MAX_ITER, precision=prec, max_slots=256)
@@ -668,13 +669,50 @@ def run_multi48(args) -> dict:
                           "(batches of 1-256 per grid, N(1, 0.1) load scaling), DC start, all submitted at once to "
                           "the continuous-batching server", "grids": 48, "cases": int(counts.sum()),
               "precision": args.precision}
-    out = base_out(args, main["value"], main["walls"], sum(main["walls"]), args.precision, config)
-    out["latency_ms"] = main["lat"]
-    out.update({"converged": main["conv"],
-                "compare": {"precision": "mixed" if args.precision == "double" else "double",
-                            "value": round(other["value"], 1), "latency_ms": other["lat"]},
-                "sequential_fp64_one_at_a_time": seq})
+    config["engine"] = "batch group: every grid's cases in one launch per kernel per pass, the passes planned on " \
+                       f"the device and captured {grp['passes_per_graph']} to a graph launch"
+    out = base_out(args, grp["value"], grp["walls"], sum(grp["walls"]), "double", config)
+    out.update({"converged": grp["conv"], "group": {k: v for k, v in grp.items() if k not in ("walls",)},
+                "server": {"precision": args.precision, "value": round(main["value"], 1), "latency_ms": main["lat"],
+                           "converged": main["conv"]},
+                "server_compare": {"precision": "mixed" if args.precision == "double" else "double",
+                                   "value": round(other["value"], 1), "latency_ms": other["lat"]},
+                "sequential_fp64_one_at_a_time": seq,
+                "group_over_sequential": round(grp["value"] / max(seq["value"], 1e-9), 3)})
     return out
+
+
+def run_group(args, jobs, solvers) -> dict:
+    """All jobs in one batch group (FP64): inputs resident, each timed step one group solve of
+    every case. Every job's result is checked bit-identical to its batch solved alone."""
+    from p3smi import autotune
+    from p3smi.group import Group
+
+    for job, s in zip(jobs, solvers):
+        autotune.apply_cached(s, job["S"].shape[1])
+    K = 8
+    grp = Group([(s, job["S"], job["V0"]) for job, s in zip(jobs, solvers)])
+    try:
+        for w in range(args.warmup):
+            grp.solve(max_iter=MAX_ITER, tol=TOL, passes_per_graph=K)
+        walls, res = [], None
+        for k in range(args.steps):
+            t0 = time.perf_counter()
+            res = grp.solve(max_iter=MAX_ITER, tol=TOL, passes_per_graph=K)
+            walls.append(time.perf_counter() - t0)
+            log(f"[bench] group step {k + 1}/{args.steps}: {1e3 * walls[-1]:.1f} ms")
+        passes = grp.passes()
+    finally:
+        grp.close()
+    same = True
+    for job, s, r in zip(jobs, solvers, res):
+        a = s.solve_batch(job["S"], job["V0"], max_iter=MAX_ITER, tol=TOL)
+        same &= bool(np.array_equal(a["iterations"], r["iterations"]) and np.array_equal(a["converged"], r["converged"])
+                     and np.array_equal(a["V"].view(np.float64), r["V"].view(np.float64)))
+    conv = sum(int(r["converged"].sum()) for r in res)
+    return {"value": round(conv * len(walls) / sum(walls), 1), "walls": walls, "conv": conv,
+            "ms_per_solve": round(1e3 * sum(walls) / len(walls), 3), "passes_per_graph": K, "passes": passes,
+            "bit_identical_to_alone": same}
 
 
 def run_serve64(args) -> dict:
