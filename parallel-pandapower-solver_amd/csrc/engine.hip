@@ -873,7 +873,7 @@ __device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q
         const int kind = w0.x & 255;
         BlkX in;
         in.x[0] = in.x[1] = in.x[2] = 0.0;
-        if (kind >= p3smi::BLK_U2) in.x[0] = (double)fac[at(w1.z, S, c)];  // L(p+1,p) or 1/U(p,p)
+        if (kind >= p3smi::BLK_U2 && kind <= p3smi::BLK_L2) in.x[0] = (double)fac[at(w1.z, S, c)];  // L(p+1,p) or 1/U(p,p)
         if (kind == p3smi::BLK_L2) {
             in.x[1] = (double)fac[at(w1.z + 1, S, c)];  // U(p, p+1)
             in.x[2] = (double)fac[at(w1.w, S, c)];      // 1/U(p+1, p+1)
@@ -926,6 +926,12 @@ __device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q
                 if (nr == 2) fac[at(w1.x + 1, S, c)] = ((real)in.j[2] + acc[2]) * inv;
                 break;
             }
+            case p3smi::BLK_PART:  // a chunk of a cut block: its sums, negated, for the next chunk
+                fac[at(w1.x, S, c)] = -acc[0];
+                if (nc == 2) fac[at(w1.x + 1, S, c)] = -acc[1];
+                if (nr == 2) fac[at(w1.x + 2, S, c)] = -acc[2];
+                if (nr == 2 && nc == 2) fac[at(w1.x + 3, S, c)] = -acc[3];
+                break;
             default: {  // BLK_L2
                 const real inv0 = (real)ix.x[0], u01 = (real)ix.x[1], inv1 = (real)ix.x[2];
                 const real l00 = ((real)in.j[0] + acc[0]) * inv0;
@@ -1368,6 +1374,12 @@ template <typename T>
 __global__ void k_zero(T* p, size_t n) {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i < n) p[i] = (T)0;
+}
+
+template <typename T>
+__global__ void k_one(T* p, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (T)1;
 }
 
 // One case's column of an interleaved array and a dense vector: put (dense -> column c of the
@@ -2841,23 +2853,34 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
     const p3smi::BlockSchedule s = p3smi::make_block_schedule(h, TL);
     const int64_t nlu = h.nnz_lu, nj = h.nnzJ;
     auto bad = [](const char* what) { throw Error(P3SMI_E_INTERNAL, std::string("block schedule: ") + what); };
-    // steps: positions in [0, nnz_lu] (nnz_lu = the zero row); the second row / column is -1
-    // exactly when the block has one row / column
+    // steps: positions in the factor arena [0, fac_rows) (nnz_lu = the zero row, nnz_lu + 1 the one
+    // row, then the partial slots); the second row / column is -1 exactly when the block has one
+    // row / column
     const int64_t nsteps = (int64_t)s.steps.size() / 4;
+    if (nlu + 2 + 4 * (int64_t)s.nparts > h.fac_rows) bad("partial slots beyond the factor arena");
     for (int64_t k = 0; k < nsteps; ++k)
         for (int w = 0; w < 4; ++w) {
             const int v = s.steps[4 * k + w];
-            if (v > nlu || v < ((w & 1) ? -1 : 0)) bad("step position out of range");
+            if (v >= h.fac_rows || v < ((w & 1) ? -1 : 0)) bad("step position out of range");
         }
     auto pack = [&](const std::vector<p3smi::BlockTask>& ts, const std::vector<int>& ptr) {
         std::vector<int4> out(3 * ts.size());
         for (size_t i = 0; i < ts.size(); ++i) {
             const p3smi::BlockTask& k = ts[i];
-            if (k.kind < p3smi::BLK_DIAG1 || k.kind > p3smi::BLK_L2 || k.nr < 1 || k.nr > 2 || k.nc < 1 || k.nc > 2)
+            if (k.kind < p3smi::BLK_DIAG1 || k.kind > p3smi::BLK_PART || k.nr < 1 || k.nr > 2 || k.nc < 1 || k.nc > 2)
                 bad("task shape");
             if (k.sbeg < 0 || k.send < k.sbeg || k.send > nsteps) bad("step range");
             for (int e = k.sbeg; e < k.send; ++e)
                 if ((s.steps[4 * e + 1] < 0) != (k.nr == 1) || (s.steps[4 * e + 3] < 0) != (k.nc == 1)) bad("step shape");
+            if (k.kind == p3smi::BLK_PART) {  // writes its partial slot, four rows past the one row
+                if (k.tgt0 < nlu + 2 || k.tgt0 + 3 >= h.fac_rows) bad("partial slot");
+                for (int j = 0; j < 4; ++j)
+                    if (k.src[j] != -1) bad("partial chunk with a Jacobian source");
+                out[3 * i] = make_int4(k.kind | (k.nr << 8) | (k.nc << 16), 0, k.sbeg, k.send);
+                out[3 * i + 1] = make_int4(k.tgt0, 0, 0, 0);
+                out[3 * i + 2] = make_int4(-1, -1, -1, -1);
+                continue;
+            }
             const bool two = k.kind == p3smi::BLK_DIAG2 || k.kind == p3smi::BLK_U2 || k.kind == p3smi::BLK_L2;
             const int ext = (k.kind == p3smi::BLK_U1 || k.kind == p3smi::BLK_U2) ? k.nc - 1
                             : (k.kind == p3smi::BLK_L1 || k.kind == p3smi::BLK_L2) ? k.nr - 1 : 0;
@@ -3031,6 +3054,13 @@ void zero_row(T* p, size_t n, hipStream_t st) {
     HIPCHECK(hipGetLastError());
 }
 
+// Set n elements to 1 (the factor arena's one row).
+template <typename T>
+void one_row(T* p, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_one<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    HIPCHECK(hipGetLastError());
+}
+
 // Row tiles of k_evaluate: consecutive rows, at most `rows` of them and at most EV_NZ Ybus
 // nonzeros, so that a tile's CSR segment fits the kernel's LDS (a single row with more
 // nonzeros is a tile of its own and read from global memory).
@@ -3069,8 +3099,10 @@ void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     const size_t S = b->S;
     for (auto* v : {&b->Vm, &b->Va, &b->Ve, &b->Vf, &b->Vm0, &b->Va0}) v->alloc((size_t)h.n * S);
     b->Jx.alloc((size_t)h.nnzJ * S);
-    b->fac.alloc(((size_t)h.nnz_lu + 1) * S);  // + the zero row the 2x2-block steps read
+    // the factor arena: + the zero row and the one row the 2x2-block steps read, + partial slots
+    b->fac.alloc((size_t)h.fac_rows * S);
     zero_row(b->fac.p + (size_t)h.nnz_lu * S, S, t->stream);
+    one_row(b->fac.p + ((size_t)h.nnz_lu + 1) * S, S, t->stream);
     b->R.alloc((size_t)h.m * S);
     b->x.alloc((size_t)h.m * S);
     for (auto* v : {&b->nrm, &b->alpha, &b->best_alpha, &b->best_nrm}) v->alloc(S);
@@ -3288,8 +3320,9 @@ void ensure_mixed(p3smi_batch* b) {
     if (b->mixed_ready) return;
     const p3smi::Topology& h = b->topo->host;
     const size_t S = b->S;
-    b->fac32.alloc(((size_t)h.nnz_lu + 1) * S);  // + the zero row
+    b->fac32.alloc((size_t)h.fac_rows * S);  // + the zero and one rows, partial slots
     zero_row(b->fac32.p + (size_t)h.nnz_lu * S, S, b->topo->stream);
+    one_row(b->fac32.p + ((size_t)h.nnz_lu + 1) * S, S, b->topo->stream);
     b->rr.alloc((size_t)h.m * S);
     b->dvec.alloc((size_t)h.m * S);
     b->prevd.alloc(S);
@@ -3811,7 +3844,7 @@ void check_v0(const double* V0, size_t rows, size_t cols, const char* who) {
 }
 
 size_t bytes_per_case(const p3smi::Topology& h, bool per_case_y, bool pin, bool per_case_s) {
-    size_t b = (size_t)h.n * 6 * 8 + (size_t)h.nnzJ * 8 + ((size_t)h.nnz_lu + 1) * 8 + (size_t)h.m * 16;
+    size_t b = (size_t)h.n * 6 * 8 + (size_t)h.nnzJ * 8 + (size_t)h.fac_rows * 8 + (size_t)h.m * 16;
     b += (size_t)max_tiles(h) * 8 + 64;
     b += (size_t)h.n * 16 * 2;  // V0 / V out
     if (per_case_s) b += (size_t)h.n * 16;

@@ -3,6 +3,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <iterator>
@@ -514,7 +515,10 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
     // blocks per level and sub-level, each with its shared pivots
     struct Blk {
         BlockTask k;
-        int r0, c0, p0;  // lead row, lead column, lead pivot of the owning group
+        int r0, c0, p0;        // lead row, lead column, lead pivot of the owning group
+        int tb = 0, te = -1;   // chunk: shared pivots [tb, te) of the block's list (te -1: all)
+        int pin = -1;          // chunk: partial sums to start from (part id), -1: from zero
+        int pout = -1;         // chunk other than the last: part id its finish stores (BLK_PART)
     };
     // the pairs' second pivots never start a block; their own rows and columns are in the
     // pair's blocks
@@ -622,19 +626,102 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
             }
         }
     };
+    // ---- long sums cut into chunks (see BLK_SPLIT_MIN): a chunk of shared pivots can run in
+    // sub-level index sl (2 l for A, 2 l + 1 for B) when every pivot's L and U rows are final
+    // before it, i.e. after sub-level B of the level of that pivot's group
+    const int one = zero + 1;
+    std::vector<int> ready(m);
+    for (int p = 0; p < m; ++p) {
+        const int lead = gsize[p] == 0 ? p - 1 : p;
+        ready[p] = 2 * t.height[lead + (gsize[lead] == 2 ? 1 : 0)] + 2;
+    }
+    struct Part { int sub; };  // a partial slot use: written in sub-level `sub`, read in sub + 1
+    std::vector<Part> parts;
+    std::vector<std::pair<int, Blk>> chunks;  // (sub-level index, chunk task)
+    auto cut = [&](Blk& b, int sub) {
+        shared(b);
+        const int len = (int)tl.size();
+        if (len <= BLK_SPLIT_MIN) return;
+        std::vector<std::array<int, 3>> ch;  // (sub-level, beg, end), the last chunk first
+        int end = len, sl = sub;
+        while (end > 0) {
+            int beg = std::max(0, end - BLK_CHUNK);
+            if (beg < BLK_TAIL) beg = 0;
+            int need = 0;
+            for (int i = beg; i < end; ++i) need = std::max(need, ready[tl[i]]);
+            if (sl < 0 || need > sl) return;  // cannot run this early: the block stays whole
+            ch.push_back({sl, beg, end});
+            end = beg;
+            --sl;
+        }
+        if (ch.size() < 2) return;
+        int prev = -1;
+        for (size_t k = ch.size(); k-- > 1;) {  // earlier chunks first
+            Blk c = b;
+            c.k.kind = BLK_PART;
+            c.k.tgt1 = c.k.aux0 = c.k.aux1 = 0;
+            c.k.src[0] = c.k.src[1] = c.k.src[2] = c.k.src[3] = -1;
+            c.tb = ch[k][1];
+            c.te = ch[k][2];
+            c.pin = prev;
+            c.pout = (int)parts.size();
+            parts.push_back({ch[k][0]});
+            prev = c.pout;
+            chunks.push_back({ch[k][0], c});
+        }
+        b.tb = ch[0][1];
+        b.te = ch[0][2];
+        b.pin = prev;
+        ++s.ncut;
+    };
+    for (int l = 0; l < s.nlev; ++l) {
+        for (Blk& b : levA[l]) cut(b, 2 * l);
+        for (Blk& b : levB[l]) cut(b, 2 * l + 1);
+    }
+    for (auto& c : chunks) (c.first % 2 == 0 ? levA : levB)[c.first / 2].push_back(c.second);
+    // partial slots: a part written in sub-level w is read in w + 1, so parts written in
+    // sub-levels of one parity can share slots; slots [0, nE) for even, [nE, nE + nO) for odd
+    std::vector<int> nsub(2 * (size_t)s.nlev + 1, 0), prank(parts.size());
+    for (size_t q = 0; q < parts.size(); ++q) prank[q] = nsub[parts[q].sub]++;
+    int nE = 0, nO = 0;
+    for (size_t w = 0; w < nsub.size(); ++w) (w % 2 == 0 ? nE : nO) = std::max(w % 2 == 0 ? nE : nO, nsub[w]);
+    s.nparts = nE + nO;
+    auto part_pos = [&](int q) {
+        const int slot = parts[q].sub % 2 == 0 ? prank[q] : nE + prank[q];
+        return zero + 2 + 4 * slot;
+    };
+
     auto weight = [&](const Blk& b) {  // row loads of the task
         shared(b);
-        return (int64_t)2 * b.k.nr * b.k.nc + (int64_t)tl.size() * (b.k.nr + b.k.nc);
+        const int64_t nt = (b.te < 0 ? (int64_t)tl.size() : (int64_t)(b.te - b.tb)) +
+                           (b.pin >= 0 ? (b.k.nc == 2 ? 2 : 1) : 0);
+        return (int64_t)2 * b.k.nr * b.k.nc + nt * (b.k.nr + b.k.nc);
     };
     auto emit = [&](Blk b, std::vector<BlockTask>& out) {
         shared(b);
         b.k.sbeg = (int)(s.steps.size() / 4);
-        for (int tp : tl) {
+        if (b.pin >= 0) {  // the partial sums of the previous chunk, back exactly (see BLK_SPLIT_MIN)
+            const int P = part_pos(b.pin);
+            s.steps.push_back(P);
+            s.steps.push_back(b.k.nr == 2 ? P + 2 : -1);
+            s.steps.push_back(one);
+            s.steps.push_back(b.k.nc == 2 ? zero : -1);
+            if (b.k.nc == 2) {
+                s.steps.push_back(P + 1);
+                s.steps.push_back(b.k.nr == 2 ? P + 3 : -1);
+                s.steps.push_back(zero);
+                s.steps.push_back(one);
+            }
+        }
+        const int i1 = b.te < 0 ? (int)tl.size() : b.te;
+        for (int i = b.tb; i < i1; ++i) {
+            const int tp = tl[i];
             s.steps.push_back(in_S(tp, b.r0) ? posL(b.r0, tp) : zero);
             s.steps.push_back(b.k.nr == 2 ? (in_S(tp, b.r0 + 1) ? posL(b.r0 + 1, tp) : zero) : -1);
             s.steps.push_back(in_S(tp, b.c0) ? posU(tp, b.c0) : zero);
             s.steps.push_back(b.k.nc == 2 ? (in_S(tp, b.c0 + 1) ? posU(tp, b.c0 + 1) : zero) : -1);
         }
+        if (b.pout >= 0) b.k.tgt0 = part_pos(b.pout);
         b.k.send = (int)(s.steps.size() / 4);
         out.push_back(b.k);
     };
@@ -824,17 +911,31 @@ struct Emu {
     std::vector<char> written;
     std::vector<double> x;
     std::vector<char> solved;
+    std::vector<int> wsub;  // partial slots: sub-level of the last write
+    int sub = 0;            // the sub-level running (2 level + 0 for A, 1 for B)
     int64_t bad = 0;
     double lo = std::numeric_limits<double>::max(), hi = 0.0;
     explicit Emu(const Topology& tt)
-        : t(tt), fac(tt.nnz_lu + 1, (real)0), written(tt.nnz_lu + 1, 0), x(tt.m, 0.0), solved(tt.m, 0) {
+        : t(tt), fac(std::max<int64_t>(tt.fac_rows, tt.nnz_lu + 2), (real)0), written(fac.size(), 0),
+          x(tt.m, 0.0), solved(tt.m, 0), wsub(fac.size(), -1) {
         written[t.nnz_lu] = 1;  // the zero row is readable
+        written[t.nnz_lu + 1] = 1;  // and the one row
+        fac[t.nnz_lu + 1] = (real)1;
     }
+    bool partial(int64_t pos) const { return pos >= t.nnz_lu + 2 && pos < (int64_t)fac.size(); }
     real rd(int pos) {
-        if (pos < 0 || pos > t.nnz_lu || !written[pos]) { ++bad; return (real)0; }
+        if (pos < 0 || pos >= (int64_t)fac.size() || !written[pos]) { ++bad; return (real)0; }
+        if (partial(pos) && wsub[pos] >= sub) ++bad;  // a partial sum read in the sub-level that writes it
         return fac[pos];
     }
     void wr(int pos, real v) {
+        if (partial(pos)) {  // partial slots are reused once read
+            if (wsub[pos] >= sub) { ++bad; return; }
+            written[pos] = 1;
+            wsub[pos] = sub;
+            fac[pos] = v;
+            return;
+        }
         if (pos < 0 || pos >= t.nnz_lu || written[pos]) { ++bad; return; }
         written[pos] = 1;
         fac[pos] = v;
@@ -947,6 +1048,12 @@ void emulate_blocks(const Topology& t, int ns, const double* J, const double* R,
                 if (k.nr == 2) E.wr(k.tgt0 + 1, (j[2] + acc[2]) * inv);
                 break;
             }
+            case BLK_PART:  // the chunk's sums, negated, for the next chunk of the block
+                E.wr(k.tgt0, -acc[0]);
+                if (k.nc == 2) E.wr(k.tgt0 + 1, -acc[1]);
+                if (k.nr == 2) E.wr(k.tgt0 + 2, -acc[2]);
+                if (k.nr == 2 && k.nc == 2) E.wr(k.tgt0 + 3, -acc[3]);
+                break;
             default: {
                 const real inv0 = E.rd(k.aux0), u01 = E.rd(k.aux0 + 1), inv1 = E.rd(k.aux1);
                 const real l00 = (j[0] + acc[0]) * inv0;
@@ -961,7 +1068,9 @@ void emulate_blocks(const Topology& t, int ns, const double* J, const double* R,
         }
     };
     for (int l = 0; l < s.nlev; ++l) {
+        E.sub = 2 * l;
         E.lists(s.ptrA, l, ns, [&](int q) { run(s.taskA[q]); });
+        E.sub = 2 * l + 1;
         E.lists(s.ptrB, l, ns, [&](int q) { run(s.taskB[q]); });
     }
     for (int l = 0; l < s.nlev; ++l)
@@ -1016,7 +1125,7 @@ void emulate(const Topology& t, bool blocks, int ns, const double* J, const doub
     if (blocks) emulate_blocks<real>(t, ns, J, R, E, checks);
     else emulate_one<real>(t, ns, J, R, E, checks);
     for (int64_t i = 0; i < t.nnz_lu; ++i) E.bad += !E.written[i];
-    if (E.fac[t.nnz_lu] != (real)0) ++E.bad;
+    if (E.fac[t.nnz_lu] != (real)0 || E.fac[t.nnz_lu + 1] != (real)1) ++E.bad;
     for (int i = 0; i < t.m; ++i) E.bad += !E.solved[i];
     std::copy(E.x.begin(), E.x.end(), x);
     *pmin = E.lo;
@@ -1053,6 +1162,9 @@ void build_topology(Topology& t, const int* Yp, const int* Yj, const double* Yx,
     build_jacobian_pattern(t);
     build_lu(t, ordering);
     build_schedule(t, nslots);
+    // the factor arena: entries, the zero and one rows, the block schedule's partial slots (the
+    // cuts and slots do not depend on the number of task lists)
+    t.fac_rows = t.nnz_lu + 2 + 4 * (int64_t)make_block_schedule(t, 1).nparts;
     t.rcond0 = reference_pivot_ratio(t);
     t.lean_valid = t.rcond0 > 0.0;
 }

@@ -67,7 +67,23 @@ struct Schedule {
 // term is an exact zero, so every entry sums the same nonzero terms in the same ascending
 // order as the one-entry-per-task schedule (identical bits while the factors are finite).
 // The contribution of t = p to the second row or column of a pair comes last, in the finish.
-enum BlockKind { BLK_DIAG1 = 0, BLK_DIAG2, BLK_U1, BLK_U2, BLK_L1, BLK_L2 };
+enum BlockKind { BLK_DIAG1 = 0, BLK_DIAG2, BLK_U1, BLK_U2, BLK_L1, BLK_L2, BLK_PART };
+
+// ---- chunks of long block sums
+// A block whose sum runs over many pivots t would keep one task list busy while the others wait
+// at the sub-level's barrier (the single long dot products at the top of the elimination tree).
+// Such a sum is cut into chunks of consecutive t, in ascending order, placed in consecutive
+// earlier sub-levels (each chunk where all its operands are already final). A chunk other than
+// the last is a BLK_PART task: its finish stores its four accumulators, negated, to a partial
+// slot (four rows of the factor arena past the zero row). The next chunk starts with one or two
+// load steps that bring them back exactly: with -p in the L operands and the one row (1.0) or the
+// zero row in the U operands, fma(p, 1, 0) = p and fma(p', 0, acc) = acc (an accumulator that
+// starts at +0 and adds products with fma never becomes -0). Every entry therefore sums the same
+// terms in the same order as an uncut block, so the factors keep their bits. Partial slots are
+// reused once read; the cuts do not depend on the number of task lists.
+constexpr int BLK_SPLIT_MIN = 32;  // sums over more pivots than this are cut
+constexpr int BLK_CHUNK = 16;      // pivots per chunk (a leftover under BLK_TAIL joins its neighbour)
+constexpr int BLK_TAIL = 8;
 
 struct BlockTask {
     int kind;        // BlockKind
@@ -103,6 +119,8 @@ struct BwdBlock {
 struct BlockSchedule {
     int nslots = 0, nlev = 0, nblev = 0;
     int npairs = 0;  // pivot pairs
+    int nparts = 0;  // partial slots (4 factor-arena rows each, from nnz_lu + 2)
+    int ncut = 0;    // blocks cut into chunks
     // per (level, slot): sub-level A = diagonal blocks and U blocks of single pivots;
     // sub-level B = the other U and L blocks and the forward rows.
     std::vector<int> ptrA, ptrB, ptrF, ptrW;
@@ -140,6 +158,8 @@ struct Topology {
     std::vector<int> prange;              // (nnz_lu + 1) pairs of factor position d: [prange[d], prange[d+1])
     std::vector<int> Fp, Fpos, Fcol;      // row i of L: entries [Fp[i], Fp[i+1]) = (position, column)
     int64_t nnz_lu = 0, n_pairs = 0;
+    int64_t fac_rows = 0;                 // factor arena rows: nnz_lu entries, the zero row (nnz_lu), the
+                                          // one row (nnz_lu + 1), the block schedule's partial slots
     Schedule sched;
     double rcond0 = 0.0;                  // min/max |pivot| of the intact grid at flat start
     bool lean_valid = true;               // false: the flat-start Jacobian has a zero or non-finite
