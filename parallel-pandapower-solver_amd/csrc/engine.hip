@@ -649,6 +649,10 @@ __device__ __forceinline__ void seg_stream(const int4* __restrict__ tasks, int q
 struct In1 { double v; };
 struct In2 { double v, d; };
 
+// task marker of the one-entry schedule: a chunk of a cut sum (its sum goes, negated, to its
+// partial slot; the next chunk's first operand pair (slot, one row) brings it back exactly)
+constexpr int LU_PART = INT_MIN;
+
 // The LU with one-entry tasks. A workgroup owns G consecutive cases (G = 64, 32, 16 or 1) and
 // TL = 64 * NS / G task lists; lane t works on case t % G and list t / G. With G = 64 a wavefront is one list and every loop bound
 // is uniform (scalar); with G = 32 or 16 the two or four lists of a wavefront walk their
@@ -723,12 +727,18 @@ __device__ __forceinline__ void lu_stream_body(
     if (!__syncthreads_or(act)) return;
 
     double pmin = DBL_MAX, pmax = 0.0, psum = 0.0;
-    // sub-level A: pivots (y = -src - 2) and U entries (y = Jacobian entry or -1 for fill)
+    // sub-level A: pivots (y = -src - 2) and U entries (y = Jacobian entry or -1 for fill); in
+    // either sub-level y = LU_PART marks a chunk of a cut sum, which stores its sum negated
     auto startA = [&](int, int4 t) -> In1 {
+        if (t.y == LU_PART) return {0.0};
         const int src = t.y < -1 ? -t.y - 2 : t.y;
         return {src >= 0 ? Jx[at(src, S, c)] : 0.0};
     };
     auto finishA = [&](int4 t, In1 in, real sum) {
+        if (t.y == LU_PART) {
+            fac[at(t.x, S, c)] = -sum;
+            return;
+        }
         real v = (real)in.v + sum;
         if (t.y < -1) {
             const double ab = fabs((double)v);
@@ -743,13 +753,21 @@ __device__ __forceinline__ void lu_stream_body(
     auto startB = [&](int q, int4 t) -> In2 {
         return {t.y >= 0 ? Jx[at(t.y, S, c)] : 0.0, (double)fac[at(diagB[q], S, c)]};
     };
-    auto finishB = [&](int4 t, In2 in, real sum) { fac[at(t.x, S, c)] = ((real)in.v + sum) * (real)in.d; };
-    // forward rows (row, right-hand side index): y = rhs - sum L y
-    auto startF = [&](int, int4 t) -> In1 { return {rhs[at(t.y, S, c)]}; };
-    auto finishF = [&](int4 t, In1 in, double sum) { x[at(t.x, S, c)] = in.v + sum; };
-    // backward rows (row, pivot position): x = (y - sum U x) / U(p,p)
-    auto startW = [&](int, int4 t) -> In2 { return {x[at(t.x, S, c)], (double)fac[at(t.y, S, c)]}; };
-    auto finishW = [&](int4 t, In2 in, double sum) { x[at(t.x, S, c)] = (in.v + sum) * in.d; };
+    auto finishB = [&](int4 t, In2 in, real sum) {
+        fac[at(t.x, S, c)] = t.y == LU_PART ? -sum : ((real)in.v + sum) * (real)in.d;
+    };
+    // forward rows (row, right-hand side index): y = rhs - sum L y; a chunk of a cut row
+    // (y = LU_PART) stores its sum negated to its solution-vector slot (row index >= m)
+    auto startF = [&](int, int4 t) -> In1 { return {t.y == LU_PART ? 0.0 : rhs[at(t.y, S, c)]}; };
+    auto finishF = [&](int4 t, In1 in, double sum) { x[at(t.x, S, c)] = t.y == LU_PART ? -sum : in.v + sum; };
+    // backward rows (row, pivot position): x = (y - sum U x) / U(p,p); chunks as forward
+    auto startW = [&](int, int4 t) -> In2 {
+        if (t.y == LU_PART) return {0.0, 0.0};
+        return {x[at(t.x, S, c)], (double)fac[at(t.y, S, c)]};
+    };
+    auto finishW = [&](int4 t, In2 in, double sum) {
+        x[at(t.x, S, c)] = t.y == LU_PART ? -sum : (in.v + sum) * in.d;
+    };
 
     for (int l = 0; l < nlev; ++l) {
         const int i0 = l * TL + slot;
@@ -3000,38 +3018,62 @@ const DevSched& get_sched(p3smi_topology* t, LuConfig cfg) {
     std::vector<int4> ta(s.taskA.size()), tb(s.taskB.size()), fw(s.fwd.size()), bw(s.bwd.size());
     std::vector<int2> pr, fe, be;
     pr.reserve((size_t)h.n_pairs);
+    const int64_t one = h.nnz_lu + 1;
     auto put_pairs = [&](const p3smi::Task& k) {
         const int b0 = (int)pr.size();
+        if (k.pin >= 0) {  // the previous chunk's sum, negated, times the one row
+            if (k.pin < one + 1 || k.pin >= h.fac_rows) throw Error(P3SMI_E_INTERNAL, "schedule: partial slot");
+            pr.push_back(make_int2(k.pin, (int)one));
+        }
         for (int e = k.pbeg; e < k.pend; ++e) pr.push_back(make_int2(h.pa[e], h.pb[e]));
         return std::make_pair(b0, (int)pr.size());
+    };
+    auto dst_of = [&](const p3smi::Task& k) {
+        if (k.part ? (k.dst < one + 1 || k.dst >= h.fac_rows) : (k.dst < 0 || k.dst >= h.nnz_lu))
+            throw Error(P3SMI_E_INTERNAL, "schedule: target position");
+        return k.dst;
     };
     for (size_t i = 0; i < ta.size(); ++i) {
         const auto& k = s.taskA[i];
         int src = k.asrc;
-        if (s.isdiagA[i]) {
+        if (k.part) {
+            src = LU_PART;
+        } else if (s.isdiagA[i]) {
             if (src < 0) throw Error(P3SMI_E_INTERNAL, "pivot without a Jacobian entry");
             src = -src - 2;
         }
         const auto r = put_pairs(k);
-        ta[i] = make_int4(k.dst, src, r.first, r.second);
+        ta[i] = make_int4(dst_of(k), src, r.first, r.second);
     }
     for (size_t i = 0; i < tb.size(); ++i) {
-        const auto r = put_pairs(s.taskB[i]);
-        tb[i] = make_int4(s.taskB[i].dst, s.taskB[i].asrc, r.first, r.second);
+        const auto& k = s.taskB[i];
+        const auto r = put_pairs(k);
+        tb[i] = make_int4(dst_of(k), k.part ? LU_PART : k.asrc, r.first, r.second);
     }
+    // a cut row's chunks pass their sums on, negated, through solution-vector slots [m, x_rows):
+    // (one row, slot) brings the previous chunk's back exactly
+    auto xslot = [&](int v) {
+        if (v < h.m || v >= h.x_rows) throw Error(P3SMI_E_INTERNAL, "schedule: solution-vector slot");
+        return v;
+    };
     for (size_t i = 0; i < fw.size(); ++i) {
         const auto& f = s.fwd[i];
         const int b0 = (int)fe.size();
+        if (f.pin >= 0) fe.push_back(make_int2((int)one, xslot(f.pin)));
         for (int e = f.ebeg; e < f.eend; ++e) fe.push_back(make_int2(h.Fpos[e], h.Fcol[e]));
-        fw[i] = make_int4(f.row, f.rsrc, b0, (int)fe.size());
+        fw[i] = f.dst >= 0 ? make_int4(xslot(f.dst), LU_PART, b0, (int)fe.size())
+                           : make_int4(f.row, f.rsrc, b0, (int)fe.size());
     }
     for (size_t i = 0; i < bw.size(); ++i) {
         const auto& w = s.bwd[i];
         const int b0 = (int)be.size();
         // columns descending: every backward row sums in this order (the 2x2-block kernel adds a
-        // pair's own term U(p, p+1) x(p+1) last)
-        for (int e = w.lend - 1; e >= w.lbeg; --e) be.push_back(make_int2(w.base + 1 + (e - w.lbeg), h.Li[e]));
-        bw[i] = make_int4(w.row, w.base, b0, (int)be.size());
+        // pair's own term U(p, p+1) x(p+1) last); a chunk walks its part of the row
+        if (w.pin >= 0) be.push_back(make_int2((int)one, xslot(w.pin)));
+        const int l0 = h.Lp[w.row];
+        for (int e = w.lend - 1; e >= w.lbeg; --e) be.push_back(make_int2(w.base + 1 + (e - l0), h.Li[e]));
+        bw[i] = w.dst >= 0 ? make_int4(xslot(w.dst), LU_PART, b0, (int)be.size())
+                           : make_int4(w.row, w.base, b0, (int)be.size());
     }
     if (pr.empty()) pr.push_back(make_int2(0, 0));  // keep every stream a valid pointer
     if (fe.empty()) fe.push_back(make_int2(0, 0));
@@ -3104,7 +3146,7 @@ void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     zero_row(b->fac.p + (size_t)h.nnz_lu * S, S, t->stream);
     one_row(b->fac.p + ((size_t)h.nnz_lu + 1) * S, S, t->stream);
     b->R.alloc((size_t)h.m * S);
-    b->x.alloc((size_t)h.m * S);
+    b->x.alloc((size_t)h.x_rows * S);  // + the slots of cut substitution rows
     for (auto* v : {&b->nrm, &b->alpha, &b->best_alpha, &b->best_nrm}) v->alloc(S);
     b->part.alloc((size_t)t->dev.ntiles * S);
     b->pstat.alloc(2 * S);
@@ -3324,7 +3366,7 @@ void ensure_mixed(p3smi_batch* b) {
     zero_row(b->fac32.p + (size_t)h.nnz_lu * S, S, b->topo->stream);
     one_row(b->fac32.p + ((size_t)h.nnz_lu + 1) * S, S, b->topo->stream);
     b->rr.alloc((size_t)h.m * S);
-    b->dvec.alloc((size_t)h.m * S);
+    b->dvec.alloc((size_t)h.x_rows * S);  // an LU solution too: + the slots of cut rows
     b->prevd.alloc(S);
     b->nparts_r = (h.m + 63) / 64;
     b->rpart.alloc(2 * (size_t)b->nparts_r * S);

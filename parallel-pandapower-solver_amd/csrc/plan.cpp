@@ -363,65 +363,228 @@ Schedule make_schedule(const Topology& t, int nslots, int group) {
     s.nblev = 0;
     for (int p = 0; p < m; ++p) s.nblev = std::max(s.nblev, t.depth[p] + 1);
 
-    // items per level: A = diagonal + U entries + forward rows (one balance), B = L entries
-    std::vector<std::vector<int>> levA(s.nlev), levB(s.nlev), levF(s.nlev), levW(s.nblev);
+    // items per level: A = diagonal + U entries + forward rows (one balance), B = L entries;
+    // an item is an entry's sum or a chunk of it: operand pairs [e0, e1), partial slots in / out
+    struct Item {
+        int d, e0, e1;
+        int pin = -1, pout = -1;  // part ids
+    };
+    std::vector<std::vector<Item>> levA(s.nlev), levB(s.nlev);
+    std::vector<std::vector<int>> levF(s.nlev), levW(s.nblev);
     for (int p = 0; p < m; ++p) {
         const int h = t.height[p];
         const int c = t.Lp[p + 1] - t.Lp[p];
-        levA[h].push_back(t.base[p]);
-        for (int r = 0; r < c; ++r) levA[h].push_back(t.base[p] + 1 + r);
-        for (int r = 0; r < c; ++r) levB[h].push_back(t.base[p] + 1 + c + r);
+        auto item = [&](int d) { return Item{d, prange[d], prange[d + 1]}; };
+        levA[h].push_back(item(t.base[p]));
+        for (int r = 0; r < c; ++r) levA[h].push_back(item(t.base[p] + 1 + r));
+        for (int r = 0; r < c; ++r) levB[h].push_back(item(t.base[p] + 1 + c + r));
         levF[h].push_back(p);
         levW[t.depth[p]].push_back(p);
     }
+    // ---- long sums cut into chunks (ONE_SPLIT_MIN): a chunk can run in sub-level index sl
+    // (2 l for A, 2 l + 1 for B) once the L and U entries of all its pairs' pivots are final,
+    // i.e. after sub-level B of the pivot's level
+    std::vector<int> ready_pos(t.nnz_lu, 0);
+    for (int p = 0; p < m; ++p)
+        for (int q = t.base[p]; q < t.base[p + 1]; ++q) ready_pos[q] = 2 * t.height[p] + 2;
+    std::vector<int> part_sub;  // sub-level that writes each part (read in the next one)
+    std::vector<std::pair<int, Item>> chunks;
+    auto cut = [&](Item& it, int sub) {
+        const int len = it.e1 - it.e0;
+        if (len <= ONE_SPLIT_MIN) return;
+        std::vector<std::array<int, 3>> ch;  // (sub-level, beg, end), the last chunk first
+        int end = len, sl = sub;
+        while (end > 0) {
+            int beg = std::max(0, end - ONE_CHUNK);
+            if (beg < ONE_TAIL) beg = 0;
+            int need = 0;
+            for (int e = it.e0 + beg; e < it.e0 + end; ++e) need = std::max(need, ready_pos[t.pa[e]]);
+            if (sl < 0 || need > sl) return;  // cannot run this early: the entry stays whole
+            ch.push_back({sl, beg, end});
+            end = beg;
+            --sl;
+        }
+        if (ch.size() < 2) return;
+        int prev = -1;
+        for (size_t k = ch.size(); k-- > 1;) {  // earlier chunks first
+            Item c{it.d, it.e0 + ch[k][1], it.e0 + ch[k][2]};
+            c.pin = prev;
+            c.pout = (int)part_sub.size();
+            part_sub.push_back(ch[k][0]);
+            prev = c.pout;
+            chunks.push_back({ch[k][0], c});
+        }
+        it.e1 = it.e0 + ch[0][2];
+        it.e0 = it.e0 + ch[0][1];
+        it.pin = prev;
+        ++s.ncut;
+    };
+    for (int l = 0; l < s.nlev; ++l) {
+        for (Item& it : levA[l]) cut(it, 2 * l);
+        for (Item& it : levB[l]) cut(it, 2 * l + 1);
+    }
+    for (auto& c : chunks) (c.first % 2 == 0 ? levA : levB)[c.first / 2].push_back(c.second);
+    // partial slots: parts written in sub-levels of one parity share slots (read in the next)
+    std::vector<int> nsub(2 * (size_t)s.nlev + 1, 0), prank(part_sub.size());
+    for (size_t q = 0; q < part_sub.size(); ++q) prank[q] = nsub[part_sub[q]]++;
+    int nE = 0, nO = 0;
+    for (size_t w = 0; w < nsub.size(); ++w) (w % 2 == 0 ? nE : nO) = std::max(w % 2 == 0 ? nE : nO, nsub[w]);
+    s.nparts = nE + nO;
+    const int zero = (int)t.nnz_lu, one = zero + 1;
+    auto part_pos = [&](int q) { return zero + 2 + (part_sub[q] % 2 == 0 ? prank[q] : nE + prank[q]); };
+
+    // ---- forward and backward rows cut the same way, their partial sums in solution-vector
+    // slots (rows m, m + 1, ...; a chunk's first pair (one row, slot) brings one back). Forward
+    // rows run in sub-level A: a chunk over columns j can run in sub-level A of level l once
+    // L(i, j) and y_j are final, l > height(j). Backward rows (columns descending) run one
+    // level per depth: a chunk can run at depth level d once every x_j is, d > depth(j).
+    struct SItem {
+        int p, e0, e1;            // forward: entries [e0, e1) of Fp; backward: S_p columns [e0, e1)
+        int pin = -1, pout = -1;  // slot ids
+    };
+    std::vector<std::vector<SItem>> levFi(s.nlev), levWi(s.nblev);
+    for (int l = 0; l < s.nlev; ++l)
+        for (int p : levF[l]) levFi[l].push_back(SItem{p, t.Fp[p], t.Fp[p + 1]});
+    for (int l = 0; l < s.nblev; ++l)
+        for (int p : levW[l]) levWi[l].push_back(SItem{p, t.Lp[p], t.Lp[p + 1]});
+    std::vector<int> fslot_lev, bslot_lev;  // level that writes each slot (read at the next)
+    std::vector<std::pair<int, SItem>> fchunks, bchunks;
+    // forward: chunks of consecutive entries at levels l, l - 1, ... (sub-level A of each)
+    auto cut_fwd = [&](SItem& it, int l) {
+        const int len = it.e1 - it.e0;
+        if (len <= ONE_SPLIT_MIN) return;
+        std::vector<std::array<int, 3>> ch;
+        int end = len, lv = l;
+        while (end > 0) {
+            int beg = std::max(0, end - ONE_CHUNK);
+            if (beg < ONE_TAIL) beg = 0;
+            int need = 0;
+            for (int e = it.e0 + beg; e < it.e0 + end; ++e) need = std::max(need, t.height[t.Fcol[e]] + 1);
+            if (lv < 0 || need > lv) return;
+            ch.push_back({lv, beg, end});
+            end = beg;
+            --lv;
+        }
+        if (ch.size() < 2) return;
+        int prev = -1;
+        for (size_t k = ch.size(); k-- > 1;) {
+            SItem c{it.p, it.e0 + ch[k][1], it.e0 + ch[k][2]};
+            c.pin = prev;
+            c.pout = (int)fslot_lev.size();
+            fslot_lev.push_back(ch[k][0]);
+            prev = c.pout;
+            fchunks.push_back({ch[k][0], c});
+        }
+        it.e1 = it.e0 + ch[0][2];
+        it.e0 = it.e0 + ch[0][1];
+        it.pin = prev;
+    };
+    // backward: the row's columns in descending order, chunks of that sequence at depths d, d - 1,
+    // ...; a chunk is the ascending range [e0, e1) of S_p walked downwards
+    auto cut_bwd = [&](SItem& it, int d) {
+        const int len = it.e1 - it.e0;
+        if (len <= ONE_SPLIT_MIN) return;
+        std::vector<std::array<int, 3>> ch;  // (depth, k0, k1): positions k in the descending walk
+        int end = len, lv = d;
+        while (end > 0) {
+            int beg = std::max(0, end - ONE_CHUNK);
+            if (beg < ONE_TAIL) beg = 0;
+            int need = 0;
+            for (int k = beg; k < end; ++k) need = std::max(need, t.depth[t.Li[it.e1 - 1 - k]] + 1);
+            if (lv < 0 || need > lv) return;
+            ch.push_back({lv, beg, end});
+            end = beg;
+            --lv;
+        }
+        if (ch.size() < 2) return;
+        const int e1 = it.e1;
+        int prev = -1;
+        for (size_t k = ch.size(); k-- > 1;) {
+            SItem c{it.p, e1 - ch[k][2], e1 - ch[k][1]};
+            c.pin = prev;
+            c.pout = (int)bslot_lev.size();
+            bslot_lev.push_back(ch[k][0]);
+            prev = c.pout;
+            bchunks.push_back({ch[k][0], c});
+        }
+        it.e0 = e1 - ch[0][2];
+        it.e1 = e1 - ch[0][1];
+        it.pin = prev;
+    };
+    for (int l = 0; l < s.nlev; ++l)
+        for (SItem& it : levFi[l]) cut_fwd(it, l);
+    for (int l = 0; l < s.nblev; ++l)
+        for (SItem& it : levWi[l]) cut_bwd(it, l);
+    for (auto& c : fchunks) levFi[c.first].push_back(c.second);
+    for (auto& c : bchunks) levWi[c.first].push_back(c.second);
+    // slots: written at level v and read at v + 1, so levels of one parity share them; forward
+    // slots are all read before the backward rows start, which reuse the same rows
+    auto slot_map = [&](const std::vector<int>& lev_of, int nlevels, std::vector<int>& slot) {
+        std::vector<int> cnt((size_t)nlevels + 1, 0);
+        slot.assign(lev_of.size(), 0);
+        for (size_t q = 0; q < lev_of.size(); ++q) slot[q] = cnt[lev_of[q]]++;
+        int e = 0, o = 0;
+        for (int v = 0; v <= nlevels; ++v) (v % 2 == 0 ? e : o) = std::max(v % 2 == 0 ? e : o, cnt[v]);
+        for (size_t q = 0; q < lev_of.size(); ++q) slot[q] = t.m + (lev_of[q] % 2 == 0 ? slot[q] : e + slot[q]);
+        return e + o;
+    };
+    std::vector<int> fslot, bslot;
+    s.nxparts = std::max(slot_map(fslot_lev, s.nlev, fslot), slot_map(bslot_lev, s.nblev, bslot));
+
     std::vector<int> lA, sA, lB, sB, lF, sF, lW, sW;
-    std::vector<int> itemsA, itemsB, itemsF, itemsW;
+    std::vector<Item> itemsA, itemsB;
+    std::vector<SItem> itemsF, itemsW;
     // dealing units: single entries (group 0) or runs of up to `group` consecutive entries
     // of one pivot's U row / L column, which share one operand of every pair (L(p,t) along a
-    // U row, U(t,p) down an L column) and so reuse it from cache within one task list
-    auto units_of = [&](const std::vector<int>& items, std::vector<std::vector<int>>& units) {
+    // U row, U(t,p) down an L column) and so reuse it from cache within one task list; a
+    // chunk of a cut sum is a unit of its own
+    auto whole = [&](const Item& it) { return it.pin < 0 && it.pout < 0; };
+    auto units_of = [&](const std::vector<Item>& items, std::vector<std::vector<Item>>& units) {
         units.clear();
         for (size_t i = 0; i < items.size(); ++i) {
-            const int d = items[i];
-            const bool cont = group > 0 && !units.empty() && (int)units.back().size() < group &&
-                              units.back().back() + 1 == d;
-            if (cont) units.back().push_back(d);
-            else units.push_back({d});
+            const Item& it = items[i];
+            const bool cont = group > 0 && !units.empty() && (int)units.back().size() < group && whole(it) &&
+                              whole(units.back().back()) && units.back().back().d + 1 == it.d;
+            if (cont) units.back().push_back(it);
+            else units.push_back({it});
         }
     };
-    std::vector<std::vector<int>> ua, ub;
+    auto weight = [&](const Item& it) { return (int64_t)2 + (it.e1 - it.e0) + (it.pin >= 0 ? 1 : 0); };
+    std::vector<std::vector<Item>> ua, ub;
     for (int l = 0; l < s.nlev; ++l) {
         units_of(levA[l], ua);
         std::vector<int64_t> load(nslots, 0);
         std::vector<int64_t> w;
         for (auto& u : ua) {
             int64_t wt = 0;
-            for (int d : u) wt += 2 + (prange[d + 1] - prange[d]);
+            for (const Item& it : u) wt += weight(it);
             w.push_back(wt);
         }
-        for (int p : levF[l]) w.push_back(2 + (t.Fp[p + 1] - t.Fp[p]));
+        for (const SItem& it : levFi[l]) w.push_back(2 + (it.e1 - it.e0) + (it.pin >= 0 ? 1 : 0));
         std::vector<int> slot = lpt_deal(w, nslots, load);
         for (size_t i = 0; i < ua.size(); ++i)
-            for (int d : ua[i]) { itemsA.push_back(d); lA.push_back(l); sA.push_back(slot[i]); }
-        for (size_t i = 0; i < levF[l].size(); ++i) {
-            itemsF.push_back(levF[l][i]); lF.push_back(l); sF.push_back(slot[ua.size() + i]);
+            for (const Item& it : ua[i]) { itemsA.push_back(it); lA.push_back(l); sA.push_back(slot[i]); }
+        for (size_t i = 0; i < levFi[l].size(); ++i) {
+            itemsF.push_back(levFi[l][i]); lF.push_back(l); sF.push_back(slot[ua.size() + i]);
         }
         units_of(levB[l], ub);
         std::vector<int64_t> loadB(nslots, 0), wb;
         for (auto& u : ub) {
             int64_t wt = 0;
-            for (int d : u) wt += 2 + (prange[d + 1] - prange[d]);
+            for (const Item& it : u) wt += weight(it);
             wb.push_back(wt);
         }
         std::vector<int> slotB = lpt_deal(wb, nslots, loadB);
         for (size_t i = 0; i < ub.size(); ++i)
-            for (int d : ub[i]) { itemsB.push_back(d); lB.push_back(l); sB.push_back(slotB[i]); }
+            for (const Item& it : ub[i]) { itemsB.push_back(it); lB.push_back(l); sB.push_back(slotB[i]); }
     }
     for (int l = 0; l < s.nblev; ++l) {
         std::vector<int64_t> load(nslots, 0), w;
-        for (int p : levW[l]) w.push_back(2 + (t.Lp[p + 1] - t.Lp[p]));
+        for (const SItem& it : levWi[l]) w.push_back(2 + (it.e1 - it.e0) + (it.pin >= 0 ? 1 : 0));
         std::vector<int> slot = lpt_deal(w, nslots, load);
-        for (size_t i = 0; i < levW[l].size(); ++i) { itemsW.push_back(levW[l][i]); lW.push_back(l); sW.push_back(slot[i]); }
+        for (size_t i = 0; i < levWi[l].size(); ++i) {
+            itemsW.push_back(levWi[l][i]); lW.push_back(l); sW.push_back(slot[i]);
+        }
     }
     // pivot of each factor position (for L entries: the diagonal they are scaled by)
     std::vector<int> diag_of(t.nnz_lu, -1);
@@ -431,27 +594,46 @@ Schedule make_schedule(const Topology& t, int nslots, int group) {
         is_diag[t.base[p]] = 1;
         for (int r = 0; r < c; ++r) diag_of[t.base[p] + 1 + c + r] = t.base[p];
     }
+    auto task_of = [&](const Item& it) {
+        Task k{it.d, asrc[it.d], it.e0, it.e1};
+        if (it.pin >= 0) k.pin = part_pos(it.pin);
+        if (it.pout >= 0) {
+            k.dst = part_pos(it.pout);
+            k.asrc = -1;
+            k.part = 1;
+        }
+        return k;
+    };
     auto ordA = group_by_slot(lA, sA, s.nlev, nslots, s.ptrA);
     for (int i : ordA) {
-        const int d = itemsA[i];
-        s.taskA.push_back({d, asrc[d], prange[d], prange[d + 1]});
-        s.isdiagA.push_back((unsigned char)is_diag[d]);
+        const Item& it = itemsA[i];
+        const Task k = task_of(it);
+        s.taskA.push_back(k);
+        s.isdiagA.push_back((unsigned char)(k.part ? 0 : is_diag[it.d]));
     }
     auto ordB = group_by_slot(lB, sB, s.nlev, nslots, s.ptrB);
     for (int i : ordB) {
-        const int d = itemsB[i];
-        s.taskB.push_back({d, asrc[d], prange[d], prange[d + 1]});
-        s.diagB.push_back(diag_of[d]);
+        const Item& it = itemsB[i];
+        const Task k = task_of(it);
+        s.taskB.push_back(k);
+        s.diagB.push_back(k.part ? zero : diag_of[it.d]);
     }
+    (void)one;
     auto ordF = group_by_slot(lF, sF, s.nlev, nslots, s.ptrF);
     for (int i : ordF) {
-        const int p = itemsF[i];
-        s.fwd.push_back({p, t.perm[p], t.Fp[p], t.Fp[p + 1]});
+        const SItem& it = itemsF[i];
+        FwdRow f{it.p, t.perm[it.p], it.e0, it.e1};
+        if (it.pin >= 0) f.pin = fslot[it.pin];
+        if (it.pout >= 0) f.dst = fslot[it.pout];
+        s.fwd.push_back(f);
     }
     auto ordW = group_by_slot(lW, sW, s.nblev, nslots, s.ptrW);
     for (int i : ordW) {
-        const int p = itemsW[i];
-        s.bwd.push_back({p, t.base[p], t.Lp[p], t.Lp[p + 1]});
+        const SItem& it = itemsW[i];
+        BwdRow b{it.p, t.base[it.p], it.e0, it.e1};
+        if (it.pin >= 0) b.pin = bslot[it.pin];
+        if (it.pout >= 0) b.dst = bslot[it.pout];
+        s.bwd.push_back(b);
     }
     return s;
 }
@@ -859,14 +1041,17 @@ void host_eval(const Topology& t, const double* Vm, const double* Va, const doub
 void host_factor(const Topology& t, const double* J, double* fac, double* pmin, double* pmax, double* psum) {
     const Schedule& s = t.sched;
     double lo = std::numeric_limits<double>::infinity(), hi = 0.0, sum = 0.0;
+    // every entry's whole sum, at the level of its task (the chunks of a cut sum are skipped:
+    // the last one sits where the whole entry would)
     auto run = [&](const Task& tk) {
         double acc = tk.asrc >= 0 ? J[tk.asrc] : 0.0;
-        for (int e = tk.pbeg; e < tk.pend; ++e) acc -= fac[t.pa[e]] * fac[t.pb[e]];
+        for (int e = t.prange[tk.dst]; e < t.prange[tk.dst + 1]; ++e) acc -= fac[t.pa[e]] * fac[t.pb[e]];
         return acc;
     };
     for (int l = 0; l < s.nlev; ++l) {
         for (int q = s.ptrA[(size_t)l * s.nslots]; q < s.ptrA[(size_t)(l + 1) * s.nslots]; ++q) {
             const Task& tk = s.taskA[q];
+            if (tk.part) continue;
             double acc = run(tk);
             if (s.isdiagA[q]) {
                 const double a = std::fabs(acc);
@@ -877,6 +1062,7 @@ void host_factor(const Topology& t, const double* J, double* fac, double* pmin, 
         }
         for (int q = s.ptrB[(size_t)l * s.nslots]; q < s.ptrB[(size_t)(l + 1) * s.nslots]; ++q) {
             const Task& tk = s.taskB[q];
+            if (tk.part) continue;
             fac[tk.dst] = run(tk) * fac[s.diagB[q]];
         }
     }
@@ -884,19 +1070,23 @@ void host_factor(const Topology& t, const double* J, double* fac, double* pmin, 
 }
 
 void host_solve(const Topology& t, const double* fac, const double* R, double* x) {
+    // every row's whole sum at the level of its last chunk
     const Schedule& s = t.sched;
     for (int l = 0; l < s.nlev; ++l)
         for (int q = s.ptrF[(size_t)l * s.nslots]; q < s.ptrF[(size_t)(l + 1) * s.nslots]; ++q) {
             const FwdRow& f = s.fwd[q];
+            if (f.dst >= 0) continue;
             double acc = R[f.rsrc];
-            for (int e = f.ebeg; e < f.eend; ++e) acc -= fac[t.Fpos[e]] * x[t.Fcol[e]];
+            for (int e = t.Fp[f.row]; e < t.Fp[f.row + 1]; ++e) acc -= fac[t.Fpos[e]] * x[t.Fcol[e]];
             x[f.row] = acc;
         }
     for (int l = 0; l < s.nblev; ++l)
         for (int q = s.ptrW[(size_t)l * s.nslots]; q < s.ptrW[(size_t)(l + 1) * s.nslots]; ++q) {
             const BwdRow& b = s.bwd[q];
+            if (b.dst >= 0) continue;
             double acc = x[b.row];
-            for (int e = b.lend - 1; e >= b.lbeg; --e) acc -= fac[b.base + 1 + (e - b.lbeg)] * x[t.Li[e]];
+            const int l0 = t.Lp[b.row];
+            for (int e = t.Lp[b.row + 1] - 1; e >= l0; --e) acc -= fac[b.base + 1 + (e - l0)] * x[t.Li[e]];
             x[b.row] = acc * fac[b.base];
         }
 }
@@ -915,9 +1105,12 @@ struct Emu {
     int sub = 0;            // the sub-level running (2 level + 0 for A, 1 for B)
     int64_t bad = 0;
     double lo = std::numeric_limits<double>::max(), hi = 0.0;
+    std::vector<int> xw;    // solution-vector slots: level of the last write (-1 none)
+    int xlev = 0;           // the substitution level running
     explicit Emu(const Topology& tt)
         : t(tt), fac(std::max<int64_t>(tt.fac_rows, tt.nnz_lu + 2), (real)0), written(fac.size(), 0),
-          x(tt.m, 0.0), solved(tt.m, 0), wsub(fac.size(), -1) {
+          x(std::max<int64_t>(tt.x_rows, tt.m), 0.0), solved(x.size(), 0), wsub(fac.size(), -1),
+          xw(x.size(), -1) {
         written[t.nnz_lu] = 1;  // the zero row is readable
         written[t.nnz_lu + 1] = 1;  // and the one row
         fac[t.nnz_lu + 1] = (real)1;
@@ -950,52 +1143,75 @@ struct Emu {
         if (!solved[i]) ++bad;
         return x[i];
     }
+    double xslot(int i) {  // a cut row's partial sum, written at an earlier level
+        if (i < t.m || i >= (int)x.size() || xw[i] < 0 || xw[i] >= xlev) { ++bad; return 0.0; }
+        return x[i];
+    }
+    void set_slot(int i, double v) {
+        if (i < t.m || i >= (int)x.size() || xw[i] >= xlev) { ++bad; return; }
+        xw[i] = xlev;
+        x[i] = v;
+    }
     template <typename F>
     void lists(const std::vector<int>& ptr, int l, int ns, F f) {  // lists of one sub-level, last first
         for (int sl = ns - 1; sl >= 0; --sl)
             for (int q = ptr[(size_t)l * ns + sl]; q < ptr[(size_t)l * ns + sl + 1]; ++q) f(q);
-    }
-    void backward_row(int row, int base, int lbeg, int lend) {
-        double acc = 0.0;
-        for (int e = lend - 1; e >= lbeg; --e) acc = std::fma(-(double)rd(base + 1 + (e - lbeg)), xv(t.Li[e]), acc);
-        x[row] = (x[row] + acc) * (double)rd(base);
     }
 };
 
 template <typename real>
 void emulate_one(const Topology& t, int ns, const double* J, const double* R, Emu<real>& E, int64_t* checks) {
     const Schedule s = make_schedule(t, ns, 0);
+    const int one = (int)t.nnz_lu + 1;
     auto dot = [&](const Task& k) {
         real acc = 0;
+        if (k.pin >= 0) acc = std::fma(-E.rd(k.pin), E.rd(one), acc);  // the previous chunk's sum
         for (int e = k.pbeg; e < k.pend; ++e) acc = std::fma(-E.rd(t.pa[e]), E.rd(t.pb[e]), acc);
         return acc;
     };
     for (int l = 0; l < s.nlev; ++l) {
+        E.sub = 2 * l;
         E.lists(s.ptrA, l, ns, [&](int q) {
             const Task& k = s.taskA[q];
+            if (k.part) { E.wr(k.dst, -dot(k)); return; }
             real v = (real)(k.asrc >= 0 ? J[k.asrc] : 0.0) + dot(k);
             E.wr(k.dst, s.isdiagA[q] ? E.pivot(v) : v);
         });
+        E.sub = 2 * l + 1;
         E.lists(s.ptrB, l, ns, [&](int q) {
             const Task& k = s.taskB[q];
+            if (k.part) { E.wr(k.dst, -dot(k)); return; }
             E.wr(k.dst, ((real)(k.asrc >= 0 ? J[k.asrc] : 0.0) + dot(k)) * (real)(double)E.rd(s.diagB[q]));
         });
     }
-    for (int l = 0; l < s.nlev; ++l)
+    for (int l = 0; l < s.nlev; ++l) {
+        E.xlev = l;
         E.lists(s.ptrF, l, ns, [&](int q) {
             const FwdRow& f = s.fwd[q];
             double acc = 0.0;
+            if (f.pin >= 0) acc = std::fma(-(double)E.rd(one), E.xslot(f.pin), acc);
             for (int e = f.ebeg; e < f.eend; ++e) acc = std::fma(-(double)E.rd(t.Fpos[e]), E.xv(t.Fcol[e]), acc);
+            if (f.dst >= 0) { E.set_slot(f.dst, -acc); return; }
             E.x[f.row] = R[f.rsrc] + acc;
             E.solved[f.row] = 1;
         });
+    }
     std::fill(E.solved.begin(), E.solved.end(), 0);
-    for (int l = 0; l < s.nblev; ++l)
+    std::fill(E.xw.begin(), E.xw.end(), -1);
+    for (int l = 0; l < s.nblev; ++l) {
+        E.xlev = l;
         E.lists(s.ptrW, l, ns, [&](int q) {
             const BwdRow& b = s.bwd[q];
-            E.backward_row(b.row, b.base, b.lbeg, b.lend);
+            double acc = 0.0;
+            if (b.pin >= 0) acc = std::fma(-(double)E.rd(one), E.xslot(b.pin), acc);
+            const int l0 = t.Lp[b.row];
+            for (int e = b.lend - 1; e >= b.lbeg; --e)
+                acc = std::fma(-(double)E.rd(b.base + 1 + (e - l0)), E.xv(t.Li[e]), acc);
+            if (b.dst >= 0) { E.set_slot(b.dst, -acc); return; }
+            E.x[b.row] = (E.x[b.row] + acc) * (double)E.rd(b.base);
             E.solved[b.row] = 1;
         });
+    }
     checks[1] = 0;
     checks[2] = t.n_pairs;
 }
@@ -1127,7 +1343,7 @@ void emulate(const Topology& t, bool blocks, int ns, const double* J, const doub
     for (int64_t i = 0; i < t.nnz_lu; ++i) E.bad += !E.written[i];
     if (E.fac[t.nnz_lu] != (real)0 || E.fac[t.nnz_lu + 1] != (real)1) ++E.bad;
     for (int i = 0; i < t.m; ++i) E.bad += !E.solved[i];
-    std::copy(E.x.begin(), E.x.end(), x);
+    std::copy(E.x.begin(), E.x.begin() + t.m, x);
     *pmin = E.lo;
     *pmax = E.hi;
     checks[0] = E.bad;
@@ -1164,7 +1380,9 @@ void build_topology(Topology& t, const int* Yp, const int* Yj, const double* Yx,
     build_schedule(t, nslots);
     // the factor arena: entries, the zero and one rows, the block schedule's partial slots (the
     // cuts and slots do not depend on the number of task lists)
-    t.fac_rows = t.nnz_lu + 2 + 4 * (int64_t)make_block_schedule(t, 1).nparts;
+    const Schedule one = make_schedule(t, 1, 0);
+    t.fac_rows = t.nnz_lu + 2 + std::max<int64_t>(4 * (int64_t)make_block_schedule(t, 1).nparts, one.nparts);
+    t.x_rows = (int64_t)t.m + one.nxparts;
     t.rcond0 = reference_pivot_ratio(t);
     t.lean_valid = t.rcond0 > 0.0;
 }

@@ -27,18 +27,33 @@
 
 namespace p3smi {
 
-struct Task {      // one factor entry
-    int dst;       // factor position written
+struct Task {      // one factor entry (or a chunk of its sum, see ONE_SPLIT_MIN)
+    int dst;       // factor position written (a chunk other than the last: its partial slot)
     int asrc;      // Jacobian entry it starts from, -1 = fill (starts from 0)
     int pbeg, pend;  // operand pairs [pbeg, pend)
+    int pin = -1;  // chunk: partial slot of the previous chunk (its sum negated), loaded first
+    int part = 0;  // chunk other than the last: stores its sum, negated, to dst
 };
+
+// Long sums of the one-entry schedule (one accumulator per entry) are cut as the 2x2 blocks'
+// are (BLK_SPLIT_MIN below): chunks of consecutive operand pairs in earlier sub-levels, the
+// partial sum passed on negated through a slot of the factor arena and brought back by the
+// pair (slot, one row). These slots and the block schedule's are never used by one launch,
+// so they share the arena rows past the one row.
+constexpr int ONE_SPLIT_MIN = 32;  // sums over more operand pairs than this are cut
+constexpr int ONE_CHUNK = 16;
+constexpr int ONE_TAIL = 8;
 
 struct FwdRow {    // forward substitution row i: y_i = R[rsrc] - sum L(i,j) y_j
     int row, rsrc, ebeg, eend;
+    int pin = -1;  // chunk: solution-vector slot (row index >= m) holding the previous chunk's sum, negated
+    int dst = -1;  // chunk other than the last: the slot its sum goes to, negated
 };
 
 struct BwdRow {    // backward substitution row i: x_i = (y_i - sum U(i,j) x_j) / U(i,i), j descending
-    int row, base, lbeg, lend;
+    int row, base, lbeg, lend;  // a chunk: columns [lbeg, lend) of row's S_row (still descending)
+    int pin = -1;  // chunk: solution-vector slot of the previous chunk's sum, negated
+    int dst = -1;  // chunk other than the last: the slot its sum goes to, negated
 };
 
 struct Schedule {
@@ -53,6 +68,9 @@ struct Schedule {
     std::vector<unsigned char> isdiagA;  // taskA entry is a pivot
     std::vector<FwdRow> fwd;
     std::vector<BwdRow> bwd;
+    int nparts = 0;  // partial slots (one factor-arena row each, from nnz_lu + 2)
+    int ncut = 0;    // entries cut into chunks
+    int nxparts = 0; // solution-vector slots of cut forward / backward rows (rows m, m + 1, ...)
 };
 
 // ---- 2x2 supernode blocks
@@ -159,7 +177,8 @@ struct Topology {
     std::vector<int> Fp, Fpos, Fcol;      // row i of L: entries [Fp[i], Fp[i+1]) = (position, column)
     int64_t nnz_lu = 0, n_pairs = 0;
     int64_t fac_rows = 0;                 // factor arena rows: nnz_lu entries, the zero row (nnz_lu), the
-                                          // one row (nnz_lu + 1), the block schedule's partial slots
+                                          // one row (nnz_lu + 1), the schedules' partial slots
+    int64_t x_rows = 0;                   // solution-vector rows: m, then the slots of cut substitution rows
     Schedule sched;
     double rcond0 = 0.0;                  // min/max |pivot| of the intact grid at flat start
     bool lean_valid = true;               // false: the flat-start Jacobian has a zero or non-finite
