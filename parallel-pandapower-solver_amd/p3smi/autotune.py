@@ -128,3 +128,41 @@ def apply_cached(solver, cases: int = 1) -> bool:
     if "eval_rows" in e:
         solver.set_eval_tile(e["eval_rows"])
     return True
+
+
+# Representative batches of each (size bucket, case class) for the synthetic grid family:
+# (bucket key, grid size in buses, cases). The "many" class of the > 20k bucket is tuned at
+# 2,048 cases (a 50k-bus case needs ~20 MB of workspace).
+REPRESENTATIVE = [("le500:few", 200, 96), ("le5k:few", 2000, 512), ("le5k:many", 2000, 8760),
+                  ("le20k:few", 10000, 256), ("le20k:many", 10000, 13499), ("gt20k:few", 40000, 64),
+                  ("gt20k:many", 40000, 1024)]
+CONFIG_OF = {200: "pf200", 2000: "ts2000", 10000: "n1_10k"}  # the benchmark grids of those sizes
+
+
+def main(argv=None) -> int:
+    """Re-tune every (size bucket, case class) on synthetic grids and write the cache:
+    python -m p3smi.autotune [cache path]."""
+    import sys
+
+    from . import grids, nr_klu
+    from .timeseries import dc_initial_voltage
+
+    argv = sys.argv[1:] if argv is None else argv
+    out = argv[0] if argv else CACHE
+    entries = {}
+    for name, n_bus, cases in REPRESENTATIVE:
+        g = grids.config_grid(CONFIG_OF[n_bus], 0) if n_bus in CONFIG_OF else grids.sized_grid(n_bus, 0)
+        Yp, Yj, Yx = g.csr()
+        s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+        S = grids.timeseries_scaled(g, cases, seed=1, sigma=0.1)
+        t0 = time.perf_counter()
+        entries[name] = tune(s, S, dc_initial_voltage(g, solver=s), reps=2, log=lambda m: print(m, flush=True))
+        print(f"[autotune] {name}: {g.n} buses, {cases} cases -> {entries[name]} "
+              f"({time.perf_counter() - t0:.1f} s)", flush=True)
+    with open(out, "w") as fh:
+        json.dump(entries, fh, indent=1, sort_keys=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
