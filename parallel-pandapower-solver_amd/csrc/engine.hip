@@ -2305,7 +2305,7 @@ __device__ __forceinline__ void uf_st(int* p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ int uf_find(int* parent, int x) {
+__device__ __forceinline__ int uf_find(int* parent, int x) {
     for (;;) {
         const int p = uf_ld(&parent[x]);
         if (p == x) return x;
@@ -2315,7 +2315,7 @@ __device__ int uf_find(int* parent, int x) {
     }
 }
 
-__device__ void uf_unite(int* parent, int a, int b) {
+__device__ __forceinline__ void uf_unite(int* parent, int a, int b) {
     for (;;) {
         a = uf_find(parent, a);
         b = uf_find(parent, b);
@@ -2334,6 +2334,15 @@ constexpr unsigned long long INF_BITS = 0x7ff0000000000000ull;
 // start voltage (|V| + 0j at a re-slack reference, 1 + 0j at unserved buses).
 // LDS: the union-find parents and island references of the case (8 n bytes) live in the
 // workgroup's dynamic LDS, so the unions are LDS atomics; else in ufbuf (2 n ints per case).
+// Only the workgroup's own threads share these arrays (and its |Z| scratch), so a barrier
+// between phases is all the ordering they need; a device-scope fence adds nothing but its
+// cost to the LDS form.
+template <bool LDS>
+__device__ __forceinline__ void phase_end() {
+    if (!LDS) __threadfence();
+    __syncthreads();
+}
+
 template <bool LDS>
 __global__ __launch_bounds__(256) void k_islands(
     int n, int nbr, const int* __restrict__ f, const int* __restrict__ t, const unsigned char* __restrict__ live,
@@ -2354,36 +2363,30 @@ __global__ __launch_bounds__(256) void k_islands(
         uf_st(&parent[v], v);
         uf_st(&cref[v], INT_MAX);
     }
-    __threadfence();
-    __syncthreads();
+    phase_end<LDS>();
     for (int e = threadIdx.x; e < nbr; e += blockDim.x) {
         if (!live[e]) continue;
         bool out = false;
         for (int q = 0; q < gsz; ++q) out |= gm[q] == e;
         if (!out) uf_unite(parent, f[e], t[e]);
     }
-    __threadfence();
-    __syncthreads();
+    phase_end<LDS>();
     for (int v = threadIdx.x; v < n; v += blockDim.x) uf_st(&parent[v], uf_find(parent, v));
-    __threadfence();
-    __syncthreads();
+    phase_end<LDS>();
     // the lowest-index slack bus of each island
     for (int i = threadIdx.x; i < nref; i += blockDim.x) atomicMin(&cref[uf_ld(&parent[ref[i]])], ref[i]);
-    __threadfence();
-    __syncthreads();
+    phase_end<LDS>();
     unsigned long long* zb = zscratch ? zscratch + (size_t)cl * n : nullptr;
     if (reslack && ngen > 0) {
         // islands without a slack: the lowest generator |Z|, then the lowest bus index among
         // the generators within np.isclose of it (rtol 1e-5, atol 1e-8)
         for (int v = threadIdx.x; v < n; v += blockDim.x) zb[v] = INF_BITS;
-        __threadfence();
-        __syncthreads();
+        phase_end<LDS>();
         for (int i = threadIdx.x; i < ngen; i += blockDim.x) {
             const int r = uf_ld(&parent[gbus[i]]);
             if (uf_ld(&cref[r]) == INT_MAX) atomicMin(&zb[r], (unsigned long long)__double_as_longlong(gz[i]));
         }
-        __threadfence();
-        __syncthreads();
+        phase_end<LDS>();
         for (int i = threadIdx.x; i < ngen; i += blockDim.x) {
             const int r = uf_ld(&parent[gbus[i]]);
             const unsigned long long zr = __hip_atomic_load(&zb[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2391,8 +2394,7 @@ __global__ __launch_bounds__(256) void k_islands(
             const double best = __longlong_as_double((long long)zr);
             if (fabs(gz[i] - best) <= 1e-8 + 1e-5 * fabs(best)) atomicMin(&cref[r], gbus[i]);
         }
-        __threadfence();
-        __syncthreads();
+        phase_end<LDS>();
     }
     for (int v = threadIdx.x; v < n; v += blockDim.x) {
         const int r = uf_ld(&parent[v]);
