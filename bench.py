@@ -6,8 +6,8 @@ Workloads (``--config``):
 * ``n1_10k`` (default): the 10,000-bus grid (spanning tree + 3,500 chords, 800 PV buses),
   every single-branch outage (13,499 cases) with re-slacking of islands, DC start, tol 1e-8
   p.u., max 20 iterations. One step is the whole N-1 solve on the device: islands
-  (union-find), per-case Ybus, Newton. ``value`` is the plain FP64 form; the mixed-precision
-  forms are timed beside it (``compare``, ``compare_2``, ``mixed_over_fp64``).
+  (union-find), then Newton on the base Ybus minus each case's stamps. ``value`` is the plain
+  FP64 form; the mixed-precision forms are timed beside it (``compare``, ``compare_2``, ``mixed_over_fp64``).
 * ``ts2000``: the 2,000-bus synthetic grid (spanning tree + 600 chords, 200 PV buses, one
   slack), 8,760 hourly load cases (daily sinusoid x N(1, 0.05) per load, PV output x
   U(0, 1)), DC-power-flow start.

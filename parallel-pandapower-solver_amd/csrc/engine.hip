@@ -33,7 +33,7 @@
 //                 vote) and the stage counters the host reads back
 //   k_lu_blk, k_lu_stream   static-pivot LU + forward/backward substitution (see below)
 //   k_resid, k_refine_*     FP64 residual and update of the mixed-precision refinement
-//   k_islands, k_admittance_*   N-1: islands, references, pins, per-case Ybus
+//   k_islands, k_admittance_fill   N-1: islands, references, pins; per-case Ybus from magnitude/angle
 //   k_emit        state -> complex V (n, B), NaN at unserved buses
 #include <hip/hip_runtime.h>
 
@@ -179,7 +179,7 @@ constexpr double ARMIJO_C = 1e-4;
 constexpr int LS_MAX_TRIALS = 10;
 constexpr double REUSE_RCOND_RATIO = 1e-3;
 // k_evaluate: workgroups of 4 wavefronts over one row tile; a tile's CSR segment (column,
-// Jacobian positions, shared admittance) is staged in LDS when it has at most EV_NZ nonzeros
+// Jacobian positions) is staged in LDS when it has at most EV_NZ nonzeros
 // (the host cuts tiles to fit, except around a single row with more).
 constexpr int EV_WAVES = 4;
 constexpr int EV_NZ = 768;
@@ -287,21 +287,6 @@ __global__ __launch_bounds__(256) void k_admittance_fill(int mode, const double*
     }
 }
 
-// Per-case admittance deltas: case c (column c - c0 of the batch) subtracts its entries
-// [dptr[c], dptr[c + 1]) = (Ybus position, complex value) in list order -- an outage case's
-// branch stamps in group order (p3s/contingency/case_generator.py:319-325). One thread per case.
-__global__ void k_admittance_delta(const int* __restrict__ dptr, const int* __restrict__ dpos,
-                                   const double* __restrict__ dval, int c0, int nc, double* __restrict__ G,
-                                   double* __restrict__ Bv, int S) {
-    const int cl = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cl >= nc) return;
-    for (int q = dptr[c0 + cl], q1 = dptr[c0 + cl + 1]; q < q1; ++q) {
-        const size_t o = at(dpos[q], S, cl);
-        G[o] -= dval[2 * (size_t)q];
-        Bv[o] -= dval[2 * (size_t)q + 1];
-    }
-}
-
 // Mismatch, Jacobian and per-tile infinity norm of every case that takes an evaluation this
 // pass (stages START, STEP, TRIAL, COMMIT; TRIAL without the Jacobian).
 //
@@ -330,6 +315,12 @@ struct EvalArgs {
     const int4* posJ;
     const double* G;   // (nnz) shared, or (nnz, S) per case
     const double* B;
+    // per-case deltas on the shared admittance (YM 2): case c subtracts the values dval of its
+    // entries [dptr[dc0 + c], dptr[dc0 + c + 1]) at the Ybus positions dpos, in list order
+    const int* dptr;
+    const int* dpos;
+    const double* dval;  // complex
+    int dc0;
     const double* Ve;
     const double* Vf;
     const double* Ssp;  // complex (n) shared (s_stride 0) or (n, s_stride)
@@ -347,11 +338,19 @@ struct EvalArgs {
     const int* plan;           // passes driven on the device: the hold comes from it
 };
 
-template <bool CASE_Y, bool PIN>
+// YM: the admittance of the cases -- 0 shared, 1 per case ((nnz, S) arrays), 2 shared with
+// per-case deltas (an outage case's stamps, subtracted as the evaluation reads the row: the
+// same subtractions in the same order as a per-case copy would hold, without the copy).
+enum : int { YM_SHARED = 0, YM_CASE = 1, YM_DELTA = 2 };
+
+template <int YM, bool PIN>
 __device__ __forceinline__ void evaluate_body(const EvalArgs& a, int hold, int bx, int by) {
+    constexpr bool CASE_Y = YM == YM_CASE;
+    // a shared admittance entry is read with a wave-uniform global load rather than staged in LDS:
+    // the smaller block footprint keeps more waves resident for the voltage gathers (measured
+    // 3-4% faster evaluations on the 2000- and 10000-bus workloads)
     __shared__ int s_col[EV_NZ];
     __shared__ int4 s_pos[EV_NZ];
-    __shared__ double2 s_y[CASE_Y ? 1 : EV_NZ];
     __shared__ double s_worst[EV_WAVES][64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -364,7 +363,6 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs& a, int hold, int b
         for (int k = k0 + (int)threadIdx.x; k < k1; k += 64 * EV_WAVES) {
             s_col[k - k0] = a.Yj[k];
             s_pos[k - k0] = a.posJ[k];
-            if (!CASE_Y) s_y[k - k0] = make_double2(a.G[k], a.B[k]);
         }
     }
     __syncthreads();
@@ -373,6 +371,14 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs& a, int hold, int b
     const bool run = moves(wc, hold);
     const bool jac = run && sg != SG_TRIAL;
     double worst = 0.0;
+    // YM_DELTA: the case's deltas, and whether any falls in this tile's nonzeros
+    int d0 = 0, d1 = 0;
+    bool dtile = false;
+    if (YM == YM_DELTA && run) {
+        d0 = a.dptr[a.dc0 + c];
+        d1 = a.dptr[a.dc0 + c + 1];
+        for (int d = d0; d < d1; ++d) dtile |= a.dpos[d] >= k0 && a.dpos[d] < k1;
+    }
     if (run) {
         for (int row = r0 + wave; row < r1; row += EV_WAVES) {
             const size_t oi = at(row, S, c);
@@ -386,13 +392,16 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs& a, int hold, int b
                 if (CASE_Y) {
                     g = a.G[at(k0 + q, S, c)];
                     b = a.B[at(k0 + q, S, c)];
-                } else if (staged) {
-                    g = s_y[q].x;
-                    b = s_y[q].y;
                 } else {
                     g = a.G[k0 + q];
                     b = a.B[k0 + q];
                 }
+                if (YM == YM_DELTA && dtile)
+                    for (int d = d0; d < d1; ++d)
+                        if (a.dpos[d] == k0 + q) {
+                            g -= a.dval[2 * (size_t)d];
+                            b -= a.dval[2 * (size_t)d + 1];
+                        }
                 const size_t ok = at(col, S, c);
                 const double ek = a.Ve[ok], fk = a.Vf[ok];
                 const double tr = g * ek - b * fk, ti = g * fk + b * ek;  // Y_ik V_k
@@ -444,19 +453,19 @@ __device__ __forceinline__ void evaluate_body(const EvalArgs& a, int hold, int b
     }
 }
 
-template <bool CASE_Y, bool PIN>
+template <int YM, bool PIN>
 __global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate(EvalArgs a) {
-    evaluate_body<CASE_Y, PIN>(a, a.hold, blockIdx.x, blockIdx.y);
+    evaluate_body<YM, PIN>(a, a.hold, blockIdx.x, blockIdx.y);
 }
 
-template <bool CASE_Y, bool PIN>
+template <int YM, bool PIN>
 __global__ __launch_bounds__(64 * EV_WAVES) void k_evaluate_multi(const EvalArgs* __restrict__ tab,
                                                                   const int* __restrict__ off, int nseg) {
     const int s = seg_find(off, nseg, blockIdx.x);
     const EvalArgs& a = tab[s];
     const int b = blockIdx.x - off[s];
     if (b >= a.gx * (a.S / 64) || a.plan[PL_HALT]) return;
-    evaluate_body<CASE_Y, PIN>(a, a.plan[PL_HOLD], b % a.gx, b / a.gx);
+    evaluate_body<YM, PIN>(a, a.plan[PL_HOLD], b % a.gx, b / a.gx);
 }
 
 // Summation order shared by every LU kernel, so that a case's factors and step are the same
@@ -2766,6 +2775,12 @@ struct p3smi_batch {
     DevBuf<unsigned char> served;       // per-case served mask (N-1 outage sets)
     int s_stride = 0, v0_stride = 0;
     bool per_case_y = false, has_pin = false, uploaded = false, mask_unserved = false;
+    // per-case admittance deltas on the shared Ybus (outage sets, stamps): null = none
+    const int* ydptr = nullptr;
+    const int* ydpos = nullptr;
+    const double* ydval = nullptr;
+    int ydc0 = 0;
+    int ymode() const { return per_case_y ? 1 : (ydptr ? 2 : 0); }
     int* h_counts = nullptr;            // pinned host mirror of counts (stage counters, C ABI order)
     int* h_rcount = nullptr;            // pinned host mirror of rcount
     bool pending_start = false;         // cases in START since the last pass
@@ -3235,6 +3250,10 @@ EvalArgs eval_args(p3smi_batch* b, int hold) {
     a.posJ = reinterpret_cast<const int4*>(d.posJ.p);
     a.G = b->per_case_y ? b->Yg.p : d.G.p;
     a.B = b->per_case_y ? b->Yb.p : d.B.p;
+    a.dptr = b->ydptr;
+    a.dpos = b->ydpos;
+    a.dval = b->ydval;
+    a.dc0 = b->ydc0;
     a.Ve = b->Ve.p;
     a.Vf = b->Vf.p;
     a.Ssp = b->Ssp.p;
@@ -3262,10 +3281,14 @@ void launch_evaluate(p3smi_batch* b, int hold) {
     const EvalArgs a = eval_args(b, hold);
     const dim3 grid((unsigned)d.ntiles, (unsigned)(b->S / 64));
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(64 * EV_WAVES), 0, t->stream, a); };
-    if (b->per_case_y && b->has_pin) go(k_evaluate<true, true>);
-    else if (b->per_case_y) go(k_evaluate<true, false>);
-    else if (b->has_pin) go(k_evaluate<false, true>);
-    else go(k_evaluate<false, false>);
+    switch (2 * b->ymode() + (int)b->has_pin) {
+        case 0: go(k_evaluate<YM_SHARED, false>); break;
+        case 1: go(k_evaluate<YM_SHARED, true>); break;
+        case 2: go(k_evaluate<YM_CASE, false>); break;
+        case 3: go(k_evaluate<YM_CASE, true>); break;
+        case 4: go(k_evaluate<YM_DELTA, false>); break;
+        default: go(k_evaluate<YM_DELTA, true>); break;
+    }
     HIPCHECK(hipGetLastError());
 }
 
@@ -3784,7 +3807,7 @@ const int* run_pass(p3smi_batch* b, const PassParams& p) {
                       cfg.G, cfg.NS, (int)starting + 2 * hold, (int)searching, p.tol,
                       p.max_iter, p.line_search, (const void*)b->Ssp.p, b->s_stride, (const void*)b->Yg.p,
                       (const void*)b->Yb.p, (const void*)b->pin.p, (const void*)b->V0.p, b->v0_stride,
-                      (int)b->per_case_y * 2 + (int)b->has_pin, t->dev.tile_rows, (const void*)b->part.p);
+                      b->ymode() * 2 + (int)b->has_pin, t->dev.tile_rows, (const void*)b->part.p);
         replay(b, key, [&] {
             PassParams q = p;
             const int64_t before = b->stats.double_factorizations;
@@ -4313,6 +4336,7 @@ int p3smi_batch_upload(p3smi_batch* b, const double* S, const double* V0, int32_
         }
         b->v0_stride = v0_per_case ? b->B : 0;
         b->per_case_y = false;
+        b->ydptr = nullptr;
         b->has_pin = false;
         b->mask_unserved = false;
         HIPCHECK(hipStreamSynchronize(st));
@@ -4579,6 +4603,7 @@ int p3smi_solve_batch(p3smi_topology* t, const double* S, const double* V0, int3
                 b->v0_stride = 0;
             }
             b->per_case_y = false;
+            b->ydptr = nullptr;
             b->has_pin = false;
             b->mask_unserved = false;
         };
@@ -4587,10 +4612,13 @@ int p3smi_solve_batch(p3smi_topology* t, const double* S, const double* V0, int3
     });
 }
 
+// upload_y null: the cases' admittances are the shared Ybus minus per-case deltas (dptr, dpos,
+// dval, device arrays indexed by the case), subtracted as the evaluation reads them
 static void contingency_common(p3smi_topology* t, const double* S, const double* V0, const uint8_t* pin, int L,
                                const std::function<void(p3smi_batch*, int, int)>& upload_y, int max_iter, double tol,
                                int line_search, int precision, double* V_out, int32_t* iters, uint8_t* conv,
-                               double* mismatch) {
+                               double* mismatch, const int* dptr = nullptr, const int* dpos = nullptr,
+                               const double* dval = nullptr) {
     const p3smi::Topology& h = t->host;
     if (L <= 0) throw Error(P3SMI_E_ARG, "solve_batch_contingency: L (cases) must be > 0");
     if (max_iter <= 0) throw Error(P3SMI_E_ARG, "solve_batch_contingency: max_iter must be > 0");
@@ -4602,10 +4630,19 @@ static void contingency_common(p3smi_topology* t, const double* S, const double*
         b->V0.alloc((size_t)2 * h.n * nc);
         upload_cols(b->stage, b->V0.p, V0, h.n, L, c0, nc, 16, st);
         b->v0_stride = nc;
-        b->Yg.alloc((size_t)h.nnzY * b->S);
-        b->Yb.alloc((size_t)h.nnzY * b->S);
-        upload_y(b, c0, nc);
-        b->per_case_y = true;
+        if (upload_y) {
+            b->Yg.alloc((size_t)h.nnzY * b->S);
+            b->Yb.alloc((size_t)h.nnzY * b->S);
+            upload_y(b, c0, nc);
+            b->per_case_y = true;
+            b->ydptr = nullptr;
+        } else {
+            b->per_case_y = false;
+            b->ydptr = dptr;
+            b->ydpos = dpos;
+            b->ydval = dval;
+            b->ydc0 = c0;
+        }
         b->has_pin = pin != nullptr;
         b->mask_unserved = false;
         if (pin) {
@@ -4615,8 +4652,8 @@ static void contingency_common(p3smi_topology* t, const double* S, const double*
             h2d_rows(b->stage, b->pin.p, b->S, pin + c0, L, nc, h.n, st);
         }
     };
-    solve_chunked(t, L, true, pin != nullptr, false, up, max_iter, tol, line_search, precision, V_out, iters, conv,
-                  mismatch);
+    solve_chunked(t, L, (bool)upload_y, pin != nullptr, false, up, max_iter, tol, line_search, precision, V_out, iters,
+                  conv, mismatch);
 }
 
 int p3smi_solve_batch_contingency(p3smi_topology* t, const double* Yx_mag, const double* Yx_ang, const double* S,
@@ -4675,16 +4712,9 @@ int p3smi_solve_batch_contingency_stamp(p3smi_topology* t, const int32_t* stamp_
         ddp.upload(dptr.data(), dptr.size(), t->stream);
         ddq.upload(dpos.data(), dpos.size(), t->stream);
         ddv.upload(dval.data(), dval.size(), t->stream);
-        auto upy = [&](p3smi_batch* b, int c0, int nc) {
-            hipStream_t st = t->stream;
-            hipLaunchKernelGGL(k_admittance_fill, grid_items(h.nnzY, b->S), dim3(256), 0, st, 0, t->dev.G.p,
-                               t->dev.B.p, 0, 0, b->Yg.p, b->Yb.p, h.nnzY, b->S);
-            hipLaunchKernelGGL(k_admittance_delta, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, ddp.p, ddq.p,
-                               ddv.p, c0, nc, b->Yg.p, b->Yb.p, b->S);
-            HIPCHECK(hipGetLastError());
-        };
-        contingency_common(t, S, V0, pin, L, upy, max_iter, tol, line_search, precision, V_out, iterations, converged,
-                           mismatch);
+        // the cases' admittances: the shared Ybus minus their stamps, subtracted in the evaluation
+        contingency_common(t, S, V0, pin, L, nullptr, max_iter, tol, line_search, precision, V_out, iterations,
+                           converged, mismatch, ddp.p, ddq.p, ddv.p);
     });
 }
 
@@ -4950,14 +4980,13 @@ int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start
             o->npinned.alloc(b->S);
             HIPCHECK(hipMemsetAsync(o->npinned.p, 0, sizeof(int) * b->S, st));
             run_islands(o, c0, nc, b->S, dvs.p, b->served.p, nullptr, b->pin.p, b->V0.p, o->npinned.p);
-            b->Yg.alloc((size_t)h.nnzY * b->S);
-            b->Yb.alloc((size_t)h.nnzY * b->S);
-            hipLaunchKernelGGL(k_admittance_fill, grid_items(h.nnzY, b->S), dim3(256), 0, st, 0, t->dev.G.p,
-                               t->dev.B.p, 0, 0, b->Yg.p, b->Yb.p, h.nnzY, b->S);
-            hipLaunchKernelGGL(k_admittance_delta, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, o->dptr.p,
-                               o->dpos.p, o->dval.p, c0, nc, b->Yg.p, b->Yb.p, b->S);
-            HIPCHECK(hipGetLastError());
-            b->per_case_y = true;
+            // each case's Ybus is the base minus its outaged stamps (case_generator.py:319-325),
+            // subtracted in group order as the evaluation reads the row
+            b->per_case_y = false;
+            b->ydptr = o->dptr.p;
+            b->ydpos = o->dpos.p;
+            b->ydval = o->dval.p;
+            b->ydc0 = c0;
             b->has_pin = true;
             b->mask_unserved = true;
             if (served) {
@@ -4965,7 +4994,7 @@ int p3smi_outages_solve(p3smi_outages* o, const double* S, const double* V_start
             }
         };
         o->res_batch = nullptr;
-        const int chunks = solve_chunked(t, L, true, true, false, up, max_iter, tol, line_search, precision, V_out,
+        const int chunks = solve_chunked(t, L, false, true, false, up, max_iter, tol, line_search, precision, V_out,
                                          iterations, converged, mismatch);
         if (chunks == 1) {  // the whole set's results stay on the device for post-processing
             o->res_batch = t->cached.get();
@@ -5115,6 +5144,7 @@ int p3smi_batch_slots_init(p3smi_batch* b) {
         b->s_stride = b->S;
         b->v0_stride = b->S;
         b->per_case_y = false;
+        b->ydptr = nullptr;
         b->has_pin = false;
         b->mask_unserved = false;
         b->slot_mode = true;
@@ -5225,7 +5255,7 @@ int p3smi_batch_fetch(p3smi_batch* b, int32_t k, const int32_t* slots, double* V
 // alone (run_pass) and the group goes on. FP64 solves of whole batches (lock-step passes).
 struct GroupLaunch {
     int kind = 0;  // LU: 0 one-entry tasks (k_lu_stream_multi), 1 2x2 blocks (k_lu_blk_multi);
-                   // evaluation: 2 * per-case Ybus + pin mask
+                   // evaluation: 2 * admittance mode (YM_*) + pin mask
     int G = 0, NS = 0;
     DevBuf<char> tab;
     DevBuf<int> off;
@@ -5320,7 +5350,7 @@ void group_build(p3smi_group* g, const PassParams& p) {
         adb.push_back(aa.gx * aa.gy);
         EvalArgs ea = eval_args(b, 0);
         ea.plan = plan;
-        auto& e = ev[2 * (int)b->per_case_y + (int)b->has_pin];
+        auto& e = ev[2 * b->ymode() + (int)b->has_pin];
         e.first.push_back(ea);
         e.second.push_back(t->dev.ntiles * (b->S / 64));
         CtlArgs ca = ctl_args(b, p, 0);
@@ -5385,10 +5415,14 @@ void group_pass(p3smi_group* g) {
         auto go = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3((unsigned)L->blocks), dim3(64 * EV_WAVES), 0, st, tab, L->off.p, L->nseg);
         };
-        if (L->kind == 3) go(k_evaluate_multi<true, true>);
-        else if (L->kind == 2) go(k_evaluate_multi<true, false>);
-        else if (L->kind == 1) go(k_evaluate_multi<false, true>);
-        else go(k_evaluate_multi<false, false>);
+        switch (L->kind) {
+            case 0: go(k_evaluate_multi<YM_SHARED, false>); break;
+            case 1: go(k_evaluate_multi<YM_SHARED, true>); break;
+            case 2: go(k_evaluate_multi<YM_CASE, false>); break;
+            case 3: go(k_evaluate_multi<YM_CASE, true>); break;
+            case 4: go(k_evaluate_multi<YM_DELTA, false>); break;
+            default: go(k_evaluate_multi<YM_DELTA, true>); break;
+        }
     }
     hipLaunchKernelGGL(k_control_multi, dim3((unsigned)g->ctl.blocks), dim3(256), 0, st,
                        reinterpret_cast<const CtlArgs*>(g->ctl.tab.p), g->ctl.off.p, g->ctl.nseg);
