@@ -16,8 +16,10 @@ Workloads (``--config``):
   the continuous-batching server (one slot pool per grid, pools overlapping on the device),
   both precisions, against the plain FP64 one-grid-at-a-time loop.
 * ``serve64``: Poisson arrivals (``--rate``, ``--requests``) of time-series and N-1 jobs
-  over 64 cached topologies of 100..50,000 buses; p50/p99 request latency and throughput
-  against the plain FP64 one-request-at-a-time loop.
+  over 64 cached topologies of 100..50,000 buses; p50/p99 request latency (from each
+  request's scheduled arrival) and throughput of the server scheduling shortest work first,
+  beside the same server running every worker at once and the plain FP64
+  one-request-at-a-time loop.
 
 A step is one complete batched solve of all cases from the start voltage, with the
 inputs already resident in device memory. The JSON line reports converged cases/s over
@@ -569,7 +571,8 @@ def _counts(rng, k, total, lo, hi):
 
 def _run_requests(srv, jobs, arrivals=None):
     """Submit jobs (at their arrival offsets, seconds, if given), wait for all. Returns the
-    requests, the wall time from the first submission to the last completion."""
+    requests, the wall time from the first submission to the last completion. A request's
+    latency runs from its scheduled arrival to its completion."""
     reqs = []
     t0 = time.perf_counter()
     for k, job in enumerate(jobs):
@@ -581,6 +584,10 @@ def _run_requests(srv, jobs, arrivals=None):
             reqs.append(srv.submit(*job["topo"], job["S"], job["V0"]))
         else:
             reqs.append(srv.submit_contingency(job["key"], job["cases"], job["S"], job["V0"]))
+        if arrivals is not None:  # latency counts from the scheduled arrival: a late submit is the server's
+            r = reqs[-1]
+            r.submit_lag = time.perf_counter() - (t0 + arrivals[k])
+            r.t_submit = min(r.t_submit, t0 + arrivals[k])
     for r in reqs:
         r.wait()
     return reqs, max(r.t_done for r in reqs) - t0
@@ -589,6 +596,18 @@ def _run_requests(srv, jobs, arrivals=None):
 def _latency(reqs):
     lat = np.array([r.latency for r in reqs])
     return {"p50": round(1e3 * float(np.percentile(lat, 50)), 3), "p99": round(1e3 * float(np.percentile(lat, 99)), 3)}
+
+
+def _latency_by_size(reqs, jobs, fam, edges=(0, 1000, 10000, 10 ** 9)):
+    """p50 latency (ms) of the requests per grid-size band (buses)."""
+    n = np.array([fam[j["gid"]].n for j in jobs])
+    lat = np.array([r.latency for r in reqs])
+    out = {}
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        sel = (n >= lo) & (n < hi)
+        if sel.any():
+            out[(lo, hi)] = round(1e3 * float(np.percentile(lat[sel], 50)), 1)
+    return out
 
 
 def _sequential(jobs, solvers, n1sets, arrivals=None):
@@ -647,8 +666,9 @@ def run_multi48(args) -> dict:
     log(f"[bench] multi48: sizes {min(g.n for g in fam)}..{max(g.n for g in fam)} buses, {int(counts.sum())} cases")
     grp = run_group(args, jobs, solvers)
     res = {}
-    for prec in (args.precision, "mixed" if args.precision == "double" else "double"):
-        srv = Server(tol=TOL, max_iter=MAX_ITER, precision=prec, max_slots=256)
+    other_prec = "mixed" if args.precision == "double" else "double"
+    for prec, prio in ((args.precision, True), (other_prec, True), (args.precision, False)):
+        srv = Server(tol=TOL, max_iter=MAX_ITER, precision=prec, max_slots=256, priority=prio)
         for w in range(args.warmup):
             _run_requests(srv, jobs)
             log(f"[bench] {prec} warmup {w + 1}/{args.warmup} done")
@@ -658,13 +678,14 @@ def run_multi48(args) -> dict:
             walls.append(wall)
             lats += [r.latency for r in reqs]
             conv = sum(int(r.converged.sum()) for r in reqs)
-            log(f"[bench] {prec} step {k + 1}/{args.steps}: {1e3 * wall:.1f} ms")
+            log(f"[bench] {prec} {'shortest work first' if prio else 'all at once'} step {k + 1}/{args.steps}: "
+                f"{1e3 * wall:.1f} ms")
         srv.close()
-        res[prec] = {"value": conv * len(walls) / sum(walls), "walls": walls, "conv": conv,
+        res[prec if prio else "all_at_once"] = {"value": conv * len(walls) / sum(walls), "walls": walls, "conv": conv,
                      "lat": {"p50": round(1e3 * float(np.percentile(lats, 50)), 3),
                              "p99": round(1e3 * float(np.percentile(lats, 99)), 3)}}
     seq = _sequential(jobs, solvers, None)
-    main, other = res[args.precision], res["mixed" if args.precision == "double" else "double"]
+    main, other, fair = res[args.precision], res[other_prec], res["all_at_once"]
     config = {"workload": "multi48: 48 synthetic grids, sizes log-uniform 100-20000 buses, 4096 load-scaled cases "
                           "(batches of 1-256 per grid, N(1, 0.1) load scaling), DC start, all submitted at once to "
                           "the continuous-batching server", "grids": 48, "cases": int(counts.sum()),
@@ -675,8 +696,10 @@ def run_multi48(args) -> dict:
     out.update({"converged": grp["conv"], "group": {k: v for k, v in grp.items() if k not in ("walls",)},
                 "server": {"precision": args.precision, "value": round(main["value"], 1), "latency_ms": main["lat"],
                            "converged": main["conv"]},
-                "server_compare": {"precision": "mixed" if args.precision == "double" else "double",
-                                   "value": round(other["value"], 1), "latency_ms": other["lat"]},
+                "server_compare": {"precision": other_prec, "value": round(other["value"], 1),
+                                   "latency_ms": other["lat"]},
+                "server_all_workers_at_once": {"precision": args.precision, "value": round(fair["value"], 1),
+                                               "latency_ms": fair["lat"]},
                 "sequential_fp64_one_at_a_time": seq,
                 "group_over_sequential": round(grp["value"] / max(seq["value"], 1e-9), 3)})
     return out
@@ -715,12 +738,40 @@ def run_group(args, jobs, solvers) -> dict:
             "bit_identical_to_alone": same}
 
 
-def run_serve64(args) -> dict:
-    """Poisson arrivals of (grid, batch) jobs over 64 cached topologies (100..50,000 buses,
-    log-uniform), batch sizes log-uniform 1..1,024, 70 % time-series and 30 % N-1 subsets, at
-    `--rate` requests/s; against the plain FP64 one-request-at-a-time loop."""
-    from p3smi import contingency, grids, nr_klu
+def _serve64_runner(args, fam, topos, starts, jobs, arrivals):
+    """serve(priority, **server options) -> (requests, wall s, outage sets): one Server over the
+    64 topologies, its topology cache warmed, the jobs submitted at their arrival offsets."""
+    from p3smi import contingency, grids
     from p3smi.serving import Server, topology_key
+
+    for j in jobs:
+        if j["kind"] == "n1":
+            j["key"] = topology_key(*topos[j["gid"]])
+    warm = [{"kind": "ts", "gid": i, "topo": topos[i], "V0": starts[i],
+             "S": np.ascontiguousarray(grids.timeseries_scaled(fam[i], 8, seed=7, sigma=0.1))} for i in range(len(fam))]
+
+    def serve(priority, **kw):
+        srv = Server(tol=TOL, max_iter=MAX_ITER, precision=args.precision, max_slots=1024, priority=priority, **kw)
+        n1sets = {}
+        for gid, g in enumerate(fam):
+            key = srv.register_outages(*topos[gid], g.f, g.t, g.branch_stamps(),
+                                       contingency.single_branch_outages(g.nbr), g.ref, g.gen_z, True)
+            n1sets[gid] = srv.outage_sets[key]
+        _run_requests(srv, warm)  # warm the topology cache: one small request per grid
+        reqs, wall = _run_requests(srv, jobs, arrivals)
+        srv.close()
+        lat = _latency(reqs)
+        log(f"[bench] served ({'shortest work first' if priority else 'all workers at once'}): {1e3 * wall:.1f} ms, "
+            f"p50 {lat['p50']:.1f} ms, p99 {lat['p99']:.1f} ms; p50 by grid size: "
+            + ", ".join(f"{lo}-{hi} buses {v}" for (lo, hi), v in _latency_by_size(reqs, jobs, fam).items()))
+        return reqs, wall, n1sets
+
+    return serve
+
+
+def _serve64_jobs(args):
+    """The 64 topologies, their solvers and DC starts, and the jobs with their arrival offsets."""
+    from p3smi import grids, nr_klu
     from p3smi.timeseries import dc_initial_voltage
 
     rng = np.random.default_rng(5)
@@ -749,24 +800,24 @@ def run_serve64(args) -> dict:
     total = sum(j["S"].shape[1] if j["kind"] == "ts" else j["cases"].size for j in jobs)
     log(f"[bench] serve64: {nreq} requests at {args.rate} req/s, {total} cases, "
         f"{sum(j['kind'] == 'n1' for j in jobs)} N-1 subsets")
-    srv = Server(tol=TOL, max_iter=MAX_ITER, precision=args.precision, max_slots=1024)
-    n1sets = {}
-    for gid, g in enumerate(fam):
-        key = srv.register_outages(*topos[gid], g.f, g.t, g.branch_stamps(),
-                                   contingency.single_branch_outages(g.nbr), g.ref, g.gen_z, True)
-        n1sets[gid] = srv.outage_sets[key]
-    for j in jobs:
-        if j["kind"] == "n1":
-            j["key"] = topology_key(*topos[j["gid"]])
-    # warm the topology cache: one small request per grid (plans, schedules, pools)
-    warm = [{"kind": "ts", "gid": i, "topo": topos[i], "V0": starts[i],
-             "S": np.ascontiguousarray(grids.timeseries_scaled(fam[i], 8, seed=7, sigma=0.1))} for i in range(len(fam))]
-    _run_requests(srv, warm)
-    log("[bench] topology cache warm")
-    reqs, wall = _run_requests(srv, jobs, arrivals)
+    return fam, topos, starts, solvers, jobs, arrivals, total
+
+
+def run_serve64(args) -> dict:
+    """Poisson arrivals of (grid, batch) jobs over 64 cached topologies (100..50,000 buses,
+    log-uniform), batch sizes log-uniform 1..1,024, 70 % time-series and 30 % N-1 subsets, at
+    `--rate` requests/s, served shortest work first; beside it the same server with every worker
+    running at once, and the plain FP64 one-request-at-a-time loop."""
+    fam, topos, starts, solvers, jobs, arrivals, total = _serve64_jobs(args)
+    nreq = len(jobs)
+    serve = _serve64_runner(args, fam, topos, starts, jobs, arrivals)
+    reqs, wall, n1sets = serve(True)
     conv = sum(int(r.converged.sum()) for r in reqs)
-    srv.close()
-    log(f"[bench] served: {1e3 * wall:.1f} ms")
+    fair, fair_wall, _ = serve(False)
+    fair_conv = sum(int(r.converged.sum()) for r in fair)
+    same = all(np.array_equal(a.V.view(np.float64), b.V.view(np.float64), equal_nan=True)
+               and np.array_equal(a.iterations, b.iterations) and np.array_equal(a.converged, b.converged)
+               for a, b in zip(reqs, fair))
     seq = _sequential(jobs, solvers, n1sets, arrivals)
     config = {"workload": f"serve64: Poisson arrivals at {args.rate} req/s, {nreq} requests over 64 cached synthetic "
                           "topologies (100-50000 buses, log-uniform), batches log-uniform 1-1024 cases, 70% time "
@@ -776,7 +827,13 @@ def run_serve64(args) -> dict:
     out["latency_ms"] = _latency(reqs)
     out["steps"] = 1
     out["ms_per_step"] = round(1e3 * wall, 3)
-    out.update({"converged": conv, "sequential_fp64_one_at_a_time": seq})
+    out["latency_ms_by_grid_size_p50"] = {f"{lo}-{hi}": v for (lo, hi), v in _latency_by_size(reqs, jobs, fam).items()}
+    out["submit_lag_ms_max"] = round(1e3 * max(getattr(r, "submit_lag", 0.0) for r in reqs), 2)
+    out.update({"converged": conv,
+                "all_workers_at_once": {"value": round(fair_conv / fair_wall, 1), "wall_s": round(fair_wall, 3),
+                                        "latency_ms": _latency(fair), "converged": fair_conv,
+                                        "results_bit_identical": bool(same)},
+                "sequential_fp64_one_at_a_time": seq})
     return out
 
 
