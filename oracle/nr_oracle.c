@@ -29,7 +29,10 @@
  *
  * The reference delegates ordering and pivoting to SuiteSparse KLU 7.14.1
  * (p3s/cpp/ci/build_klu.py:49-50), which is not available here. Its role is restated:
- *   - ordering: minimum degree on the pattern of A + A^T (KLU default: AMD);
+ *   - ordering: a fill-reducing minimum degree order (KLU default: AMD), taken on the buses
+ *     (the quotient graph of the Jacobian pattern), each bus's P equation before its Q
+ *     equation -- the order the device plan factors in, so that the frozen plan's pivot
+ *     ratios, and with them the reuse guard's decisions, are the same on both;
  *   - klu_factor: left-looking Gilbert-Peierls LU with threshold partial pivoting
  *     (tolerance 0.001, diagonal preferred, rows scaled by their max |a_ij|, as KLU's
  *     defaults do), KLU's pivot growth estimate rcond = min|u_kk| / max|u_kk|;
@@ -182,6 +185,35 @@ static void o_min_degree(int m, const int* Ap, const int* Ai, int* perm) {
     }
     for (int i = 0; i < m; ++i) free(adj[i].a);
     free(adj); free(mark); free(gone); free(h);
+}
+
+/* Minimum degree on the buses: bus k = pvpq[k] (its P equation is row k), adjacent when
+ * any of their equations couple in the Jacobian pattern; then each bus's P equation
+ * before its Q equation (row npvpq + its position in pq). Q[k] = column eliminated k-th. */
+static void o_bus_min_degree(int m, int npvpq, const int* Jp, const int* Ji, const int* pvpq, const int* pq_pos,
+                             const int* pvpq_pos, const int* pq, int* Q) {
+    int* kb = NEWV(int, m); /* equation -> bus index */
+    for (int v = 0; v < m; ++v) kb[v] = v < npvpq ? v : pvpq_pos[pq[v - npvpq]];
+    ivec* col = NEWV(ivec, npvpq);
+    for (int j = 0; j < m; ++j)
+        for (int p = Jp[j]; p < Jp[j + 1]; ++p)
+            if (kb[Ji[p]] != kb[j]) iv_push(&col[kb[j]], kb[Ji[p]]);
+    int* Bp = NEWV(int, npvpq + 1);
+    for (int b = 0; b < npvpq; ++b) Bp[b + 1] = Bp[b] + col[b].n;
+    int* Bi = NEWV(int, Bp[npvpq] + 1);
+    for (int b = 0; b < npvpq; ++b) {
+        if (col[b].n) memcpy(Bi + Bp[b], col[b].a, (size_t)col[b].n * sizeof(int));
+        free(col[b].a);
+    }
+    int* order = NEWV(int, npvpq);
+    o_min_degree(npvpq, Bp, Bi, order);
+    int k = 0;
+    for (int q = 0; q < npvpq; ++q) {
+        const int b = order[q];
+        Q[k++] = b;
+        if (pq_pos[pvpq[b]] >= 0) Q[k++] = npvpq + pq_pos[pvpq[b]];
+    }
+    free(kb); free(col); free(Bp); free(Bi); free(order);
 }
 
 /* ================================================================== pivoting LU
@@ -872,7 +904,8 @@ static void build_lean_plan(otopo* t) {
     state_free(&s);
 }
 
-/* Yx given as interleaved (re, im) pairs. ordering: 0 = minimum degree, 2 = natural. */
+/* Yx given as interleaved (re, im) pairs. ordering: 0 = minimum degree on the buses (P
+ * before Q of each bus), 2 = natural. */
 void* oracle_topo_create(const int* Yp, const int* Yj, const double* Yx, int n, int nnzY, const int* pv, int npv,
                          const int* pq, int npq, int ordering, char* err, int errlen) {
     if (n <= 0 || nnzY <= 0 || Yp[n] != nnzY) { set_err(err, errlen, "invalid Ybus CSR"); return NULL; }
@@ -898,7 +931,7 @@ void* oracle_topo_create(const int* Yp, const int* Yj, const double* Yx, int n, 
     build_J_pattern(t);
     t->Q = NEWV(int, t->m);
     if (ordering == 2) for (int k = 0; k < t->m; ++k) t->Q[k] = k;
-    else o_min_degree(t->m, t->Jp, t->Ji, t->Q);
+    else o_bus_min_degree(t->m, t->npvpq, t->Jp, t->Ji, t->pvpq, t->pq_pos, t->pvpq_pos, t->pq, t->Q);
     build_lean_plan(t);
     return t;
 }
