@@ -35,6 +35,7 @@ CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json"
 
 
 MANY_CASES = 8192
+FEW_PROBE = 1  # cases of the batch on which the few-case shapes are timed
 
 
 def bucket(n_bus: int) -> str:
@@ -76,18 +77,38 @@ def tune(solver, S, V0, reps: int = 2, log=None) -> dict:
                 if log:
                     log(f"[autotune] batch g={g} ns={ns} group={grp}: {1e3 * results[(g, ns, grp)]:.2f} ms")
         bg, bns, bgrp = min(results, key=results.get)
-        few = {}
-        for fg, fns in FEW_CANDIDATES:
-            for fmax in (64, 256, 1024):
-                solver.set_lu_shape(bg, bns, fg, fns, fmax, bgrp)
-                _time_solve(solver, bh, 1)
-                few[(fg, fns, fmax)] = _time_solve(solver, bh, reps)
+        # the few-case shape runs single solves and the last iterating cases of a batch, both
+        # latency bound: in a whole batch's time it is lost in the noise, so its (g, ns) is
+        # timed on FEW_PROBE cases that it factors alone (one: a 16-case probe picked 16
+        # wavefronts, which doubled a one-outage call on the 10k grid against 4), and only
+        # the threshold on the whole batch
+        k = min(FEW_PROBE, S.shape[1])
+        probe = {}
+        sh = C.c_void_p()
+        _lib.check(L.p3smi_batch_create(solver._h, k, C.byref(sh)))
+        try:
+            Sk = np.ascontiguousarray(S[:, :k])
+            V0k = np.ascontiguousarray(V0[:, :k]) if V0.ndim == 2 else V0
+            _lib.check(L.p3smi_batch_upload(sh, _lib.ptr(Sk), _lib.ptr(V0k), int(V0k.ndim == 2)))
+            for fg, fns in FEW_CANDIDATES:
+                solver.set_lu_shape(bg, bns, fg, fns, 1 << 30, bgrp)
+                _time_solve(solver, sh, 1)
+                probe[(fg, fns)] = _time_solve(solver, sh, reps)
                 if log:
-                    log(f"[autotune] few g={fg} ns={fns} max={fmax}: {1e3 * few[(fg, fns, fmax)]:.2f} ms")
-        best_few = min(few, key=few.get)
-        if few[best_few] >= results[(bg, bns, bgrp)]:
-            best_few = (1, 16, 0)
-        fg, fns, fmax = best_few
+                    log(f"[autotune] few g={fg} ns={fns} on {k} cases: {1e3 * probe[(fg, fns)]:.2f} ms")
+        finally:
+            L.p3smi_batch_free(sh)
+        fg, fns = min(probe, key=probe.get)
+        few = {}
+        for fmax in (64, 256, 1024):
+            solver.set_lu_shape(bg, bns, fg, fns, fmax, bgrp)
+            _time_solve(solver, bh, 1)
+            few[fmax] = _time_solve(solver, bh, reps)
+            if log:
+                log(f"[autotune] few g={fg} ns={fns} max={fmax}: {1e3 * few[fmax]:.2f} ms")
+        fmax = min(few, key=few.get)
+        if few[fmax] >= results[(bg, bns, bgrp)]:
+            fmax = 0
         solver.set_lu_shape(bg, bns, fg, fns, fmax, bgrp)
         tiles = {}
         for rows in EVAL_TILE_CANDIDATES:
