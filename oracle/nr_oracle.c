@@ -88,8 +88,10 @@ static void iv_push(ivec* v, int x) {
 
 /* ================================================================== ordering
  * Minimum degree on the symmetric pattern of A + A^T (diagonal ignored), on the explicit
- * elimination graph: eliminating v joins its remaining neighbours into a clique. Ties go
- * to the lower index, so the order is deterministic. perm[k] = column eliminated k-th. */
+ * elimination graph: eliminating v joins its remaining neighbours into a clique. A node's
+ * degree is its number of live neighbours, or with weights w (may be NULL) the sum of
+ * theirs. Ties go to the lower index, so the order is deterministic. perm[k] = column
+ * eliminated k-th. */
 typedef struct { int deg, node; } heapent;
 static int he_less(heapent a, heapent b) { return a.deg < b.deg || (a.deg == b.deg && a.node < b.node); }
 static void heap_push(heapent* h, int* hn, heapent e) {
@@ -115,7 +117,14 @@ static heapent heap_pop(heapent* h, int* hn) {
     return top;
 }
 
-static void o_min_degree(int m, const int* Ap, const int* Ai, int* perm) {
+static int o_degree(const ivec* adj, int v, const int* wt) {
+    if (!wt) return adj[v].n;
+    int d = 0;
+    for (int q = 0; q < adj[v].n; ++q) d += wt[adj[v].a[q]];
+    return d;
+}
+
+static void o_min_degree_w(int m, const int* Ap, const int* Ai, const int* wt, int* perm) {
     ivec* adj = NEWV(ivec, m);
     int* mark = NEWV(int, m);
     char* gone = NEWV(char, m);
@@ -139,13 +148,13 @@ static void o_min_degree(int m, const int* Ap, const int* Ai, int* perm) {
     size_t hcap = (size_t)8 * m + 16;
     heapent* h = (heapent*)malloc(hcap * sizeof(heapent));
     int hn = 0;
-    for (int i = 0; i < m; ++i) heap_push(h, &hn, (heapent){adj[i].n, i});
+    for (int i = 0; i < m; ++i) heap_push(h, &hn, (heapent){o_degree(adj, i, wt), i});
     int stamp = 0;
     for (int k = 0; k < m; ++k) {
         heapent e;
         for (;;) {
             e = heap_pop(h, &hn);
-            if (!gone[e.node] && e.deg == adj[e.node].n) break;
+            if (!gone[e.node] && e.deg == o_degree(adj, e.node, wt)) break;
         }
         int v = e.node;
         perm[k] = v;
@@ -171,13 +180,13 @@ static void o_min_degree(int m, const int* Ap, const int* Ai, int* perm) {
                 hcap *= 2;
                 h = (heapent*)realloc(h, hcap * sizeof(heapent));
             }
-            heap_push(h, &hn, (heapent){adj[u].n, u});
+            heap_push(h, &hn, (heapent){o_degree(adj, u, wt), u});
         }
         /* rebuild the heap when stale entries dominate */
         if ((size_t)hn > (size_t)4 * (m - k) + 64) {
             int w = 0;
             for (int q = 0; q < hn; ++q)
-                if (!gone[h[q].node] && h[q].deg == adj[h[q].node].n) h[w++] = h[q];
+                if (!gone[h[q].node] && h[q].deg == o_degree(adj, h[q].node, wt)) h[w++] = h[q];
             int hn2 = 0;
             for (int q = 0; q < w; ++q) heap_push(h, &hn2, h[q]);
             hn = hn2;
@@ -187,8 +196,11 @@ static void o_min_degree(int m, const int* Ap, const int* Ai, int* perm) {
     free(adj); free(mark); free(gone); free(h);
 }
 
+static void o_min_degree(int m, const int* Ap, const int* Ai, int* perm) { o_min_degree_w(m, Ap, Ai, NULL, perm); }
+
 /* Minimum degree on the buses: bus k = pvpq[k] (its P equation is row k), adjacent when
- * any of their equations couple in the Jacobian pattern; then each bus's P equation
+ * any of their equations couple in the Jacobian pattern, a bus's degree the number of
+ * equations of its neighbours (2 for a PQ bus, 1 for a PV bus); then each bus's P equation
  * before its Q equation (row npvpq + its position in pq). Q[k] = column eliminated k-th. */
 static void o_bus_min_degree(int m, int npvpq, const int* Jp, const int* Ji, const int* pvpq, const int* pq_pos,
                              const int* pvpq_pos, const int* pq, int* Q) {
@@ -205,15 +217,17 @@ static void o_bus_min_degree(int m, int npvpq, const int* Jp, const int* Ji, con
         if (col[b].n) memcpy(Bi + Bp[b], col[b].a, (size_t)col[b].n * sizeof(int));
         free(col[b].a);
     }
+    int* wt = NEWV(int, npvpq);
+    for (int b = 0; b < npvpq; ++b) wt[b] = pq_pos[pvpq[b]] >= 0 ? 2 : 1;
     int* order = NEWV(int, npvpq);
-    o_min_degree(npvpq, Bp, Bi, order);
+    o_min_degree_w(npvpq, Bp, Bi, wt, order);
     int k = 0;
     for (int q = 0; q < npvpq; ++q) {
         const int b = order[q];
         Q[k++] = b;
         if (pq_pos[pvpq[b]] >= 0) Q[k++] = npvpq + pq_pos[pvpq[b]];
     }
-    free(kb); free(col); free(Bp); free(Bi); free(order);
+    free(kb); free(col); free(Bp); free(Bi); free(wt); free(order);
 }
 
 /* ================================================================== pivoting LU

@@ -19,13 +19,20 @@ namespace {
 // ------------------------------------------------------------------ minimum degree
 // Greedy minimum degree on the explicit elimination graph of a symmetric pattern
 // (adjacency without the diagonal). Eliminating v turns its live neighbours into a
-// clique. Degree ties go to the lowest index (deterministic order).
-std::vector<int> minimum_degree(int m, std::vector<std::vector<int>> adj) {
+// clique. A node's degree is the number of its live neighbours, or with weights `wt` the
+// sum of theirs. Degree ties go to the lowest index (deterministic order).
+std::vector<int> minimum_degree(int m, std::vector<std::vector<int>> adj, const std::vector<int>* wt = nullptr) {
     std::vector<char> dead(m, 0);
     std::vector<int> stamp(m, -1);
+    auto degree = [&](int v) {
+        if (!wt) return (int)adj[v].size();
+        int d = 0;
+        for (int u : adj[v]) d += (*wt)[u];
+        return d;
+    };
     using Key = std::pair<int, int>;  // (degree, node)
     std::priority_queue<Key, std::vector<Key>, std::greater<Key>> pq;
-    for (int i = 0; i < m; ++i) pq.push({(int)adj[i].size(), i});
+    for (int i = 0; i < m; ++i) pq.push({degree(i), i});
     std::vector<int> order;
     order.reserve(m);
     int tick = 0;
@@ -33,7 +40,7 @@ std::vector<int> minimum_degree(int m, std::vector<std::vector<int>> adj) {
         Key top = pq.top();
         pq.pop();
         int v = top.second;
-        if (dead[v] || top.first != (int)adj[v].size()) continue;  // stale entry
+        if (dead[v] || top.first != degree(v)) continue;  // stale entry
         dead[v] = 1;
         order.push_back(v);
         std::vector<int> nb;
@@ -56,7 +63,7 @@ std::vector<int> minimum_degree(int m, std::vector<std::vector<int>> adj) {
                     stamp[x] = tick;
                     au.push_back(x);
                 }
-            pq.push({(int)au.size(), u});
+            pq.push({degree(u), u});
         }
     }
     return order;
@@ -202,10 +209,15 @@ static void build_jacobian_pattern(Topology& t) {
 // Minimum degree on the buses, then each bus's P equation before its Q equation. The two
 // equations of a bus couple to the same neighbours, so eliminated side by side they form a
 // pivot pair (parent p + 1, S_p = {p + 1} + S_{p+1}) at every PQ bus, and the 2x2-block
-// schedule loads one row of operands for both. Minimum degree over single equations put
-// the two side by side at 89% of the PQ buses (10k grid) and gave 0.7-2% less fill, but its
-// batch LU was 9-15% slower than on the bus order (n1_10k 40.7 against 34.5 ms per launch,
-// ts2000 4.62 against 4.21 ms).
+// schedule loads one row of operands for both. A bus's degree counts the equations of its
+// neighbours (2 for a PQ bus, 1 for a PV bus), as the fill it causes does. The batch LU per
+// launch, each step measured against the previous one on one machine:
+//   - minimum degree over single equations (the two side by side at 89% of the PQ buses of
+//     the 10k grid) -> on the buses counting neighbour buses: n1_10k 40.7 -> 34.5 ms,
+//     ts2000 4.62 -> 4.21 ms, with 0.7-2% more fill;
+//   - counting neighbour buses -> counting neighbour equations: n1_10k 34.5 -> 32.0 ms,
+//     ts2000 4.04 -> 3.91 ms; on n1_10k 4.5% less fill, 17% fewer operand pairs and 176
+//     levels against 204.
 std::vector<int> bus_minimum_degree(const Topology& t, const std::vector<std::vector<int>>& adj) {
     const int m = t.m;
     std::vector<int> bus_of(m, -1);
@@ -232,7 +244,9 @@ std::vector<int> bus_minimum_degree(const Topology& t, const std::vector<std::ve
     }
     std::vector<int> perm;
     perm.reserve(m);
-    for (int k : minimum_degree(nb, std::move(badj))) {
+    std::vector<int> wt(nb);
+    for (int k = 0; k < nb; ++k) wt[k] = t.row_q[buses[k]] >= 0 ? 2 : 1;
+    for (int k : minimum_degree(nb, std::move(badj), &wt)) {
         const int b = buses[k];
         if (t.row_p[b] >= 0) perm.push_back(t.row_p[b]);
         if (t.row_q[b] >= 0) perm.push_back(t.row_q[b]);
