@@ -1030,25 +1030,40 @@ __device__ __forceinline__ void blk_stream(const int4* __restrict__ tasks, int q
 }
 
 // Forward rows of pivot groups (2 int4 per task: {row, nr, sbeg, send}, {rsrc0, rsrc1,
-// lpos, -}; steps int4 {L(row, j), L(row+1, j) or -1, j, -}): a step loads L(R, j) and y(j)
+// lpos, pin}; steps int4 {L(row, j), L(row+1, j) or -1, j, -}): a step loads L(R, j) and y(j)
 // once for the group's rows; the pair's term L(row+1, row) y(row) comes last. Chunks of
-// K steps in two buffers, as blk_stream.
+// K steps in two buffers, as blk_stream. A row cut into pieces: pin >= 0 is the solution-vector
+// slot whose nr rows hold the previous piece's sums, the start of this piece's; a piece other
+// than the last has nr + 4 in place of nr and its slot in place of row, and stores its sums
+// there instead of finishing the row.
 template <int K, typename real>
 __device__ __forceinline__ void fblk_stream(const int4* __restrict__ tasks, int q0, int q1, const int4* __restrict__ st,
                                             const real* __restrict__ fac, double* __restrict__ x,
                                             const double* __restrict__ rhs, int S, int c) {
     if (q0 >= q1) return;
-    struct FIn { double r0, r1, l10; };
+    struct FIn { double r0, r1, l10, p0, p1; };
     auto start = [&](int q) -> FIn {
         const int4 w0 = tasks[2 * q], w1 = tasks[2 * q + 1];
-        FIn in;
-        in.r0 = rhs[at(w1.x, S, c)];
-        in.r1 = w0.y == 2 ? rhs[at(w1.y, S, c)] : 0.0;
-        in.l10 = w0.y == 2 ? (double)fac[at(w1.z, S, c)] : 0.0;
+        const int nr = w0.y > 2 ? w0.y - 4 : w0.y;
+        FIn in{0.0, 0.0, 0.0, 0.0, 0.0};
+        if (w0.y <= 2) {
+            in.r0 = rhs[at(w1.x, S, c)];
+            in.r1 = nr == 2 ? rhs[at(w1.y, S, c)] : 0.0;
+            in.l10 = nr == 2 ? (double)fac[at(w1.z, S, c)] : 0.0;
+        }
+        if (w1.w >= 0) {
+            in.p0 = x[at(w1.w, S, c)];
+            if (nr == 2) in.p1 = x[at(w1.w + 1, S, c)];
+        }
         return in;
     };
     auto finish = [&](int q, const FIn& in, double a0, double a1) {
         const int4 w0 = tasks[2 * q];
+        if (w0.y > 2) {  // a piece of a cut row: its sums for the next piece
+            x[at(w0.x, S, c)] = a0;
+            if (w0.y == 6) x[at(w0.x + 1, S, c)] = a1;
+            return;
+        }
         const double y0 = in.r0 + a0;
         x[at(w0.x, S, c)] = y0;
         if (w0.y == 2) x[at(w0.x + 1, S, c)] = in.r1 + fma(-in.l10, y0, a1);
@@ -1059,14 +1074,15 @@ __device__ __forceinline__ void fblk_stream(const int4* __restrict__ tasks, int 
     FIn inn = in;
     if (q + 1 < q1) inn = start(q + 1);
     const int eend = tasks[2 * (q1 - 1)].w;
-    double acc0 = 0.0, acc1 = 0.0;
+    double acc0 = in.p0, acc1 = in.p1;
     auto advance = [&]() {
         finish(q, in, acc0, acc1);
-        acc0 = acc1 = 0.0;
         ++q;
         if (q < q1) {
             tk = tasks[2 * q];
             in = inn;
+            acc0 = in.p0;
+            acc1 = in.p1;
             if (q + 1 < q1) inn = start(q + 1);
         }
     };
@@ -1112,29 +1128,41 @@ __device__ __forceinline__ void fblk_stream(const int4* __restrict__ tasks, int 
 }
 
 // Backward rows of pivot groups (2 int4 per task: {row, nr, sbeg, send}, {1/U(row,row),
-// 1/U(row+1,row+1), -, -} positions; steps int4 {U(row, j), U(row+1, j) or -1, j, -} over
+// 1/U(row+1,row+1), pin, -} positions; steps int4 {U(row, j), U(row+1, j) or -1, j, -} over
 // the columns of the group's last row, descending): a step loads x(j) once for the group's
-// rows; for a pair, x(row+1) first, then U(row, row+1) x(row+1) last in row's sum.
+// rows; for a pair, x(row+1) first, then U(row, row+1) x(row+1) last in row's sum. Cut rows
+// as fblk_stream (pin in the third field of the second int4).
 template <int K, typename real>
 __device__ __forceinline__ void bblk_stream(const int4* __restrict__ tasks, int q0, int q1, const int4* __restrict__ st,
                                             const real* __restrict__ fac, double* __restrict__ x, int S, int c) {
     if (q0 >= q1) return;
-    struct BIn { double y0, y1, inv0, inv1, u01; };
+    struct BIn { double y0, y1, inv0, inv1, u01, p0, p1; };
     auto start = [&](int q) -> BIn {
         const int4 w0 = tasks[2 * q], w1 = tasks[2 * q + 1];
-        BIn in;
-        in.y0 = x[at(w0.x, S, c)];
-        in.inv0 = (double)fac[at(w1.x, S, c)];
-        in.y1 = in.inv1 = in.u01 = 0.0;
-        if (w0.y == 2) {
-            in.y1 = x[at(w0.x + 1, S, c)];
-            in.inv1 = (double)fac[at(w1.y, S, c)];
-            in.u01 = (double)fac[at(w1.x + 1, S, c)];
+        const int nr = w0.y > 2 ? w0.y - 4 : w0.y;
+        BIn in{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (w0.y <= 2) {
+            in.y0 = x[at(w0.x, S, c)];
+            in.inv0 = (double)fac[at(w1.x, S, c)];
+            if (nr == 2) {
+                in.y1 = x[at(w0.x + 1, S, c)];
+                in.inv1 = (double)fac[at(w1.y, S, c)];
+                in.u01 = (double)fac[at(w1.x + 1, S, c)];
+            }
+        }
+        if (w1.z >= 0) {
+            in.p0 = x[at(w1.z, S, c)];
+            if (nr == 2) in.p1 = x[at(w1.z + 1, S, c)];
         }
         return in;
     };
     auto finish = [&](int q, const BIn& in, double a0, double a1) {
         const int4 w0 = tasks[2 * q];
+        if (w0.y > 2) {  // a piece of a cut row: its sums for the next piece
+            x[at(w0.x, S, c)] = a0;
+            if (w0.y == 6) x[at(w0.x + 1, S, c)] = a1;
+            return;
+        }
         if (w0.y == 2) {
             const double x1 = (in.y1 + a1) * in.inv1;
             x[at(w0.x + 1, S, c)] = x1;
@@ -1149,14 +1177,15 @@ __device__ __forceinline__ void bblk_stream(const int4* __restrict__ tasks, int 
     BIn inn = in;
     if (q + 1 < q1) inn = start(q + 1);
     const int eend = tasks[2 * (q1 - 1)].w;
-    double acc0 = 0.0, acc1 = 0.0;
+    double acc0 = in.p0, acc1 = in.p1;
     auto advance = [&]() {
         finish(q, in, acc0, acc1);
-        acc0 = acc1 = 0.0;
         ++q;
         if (q < q1) {
             tk = tasks[2 * q];
             in = inn;
+            acc0 = in.p0;
+            acc1 = in.p1;
             if (q + 1 < q1) inn = start(q + 1);
         }
     };
@@ -2954,10 +2983,13 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
     std::vector<int4> st4(std::max<int64_t>(nsteps, 1), make_int4(0, -1, 0, -1));
     for (int64_t k = 0; k < nsteps; ++k)
         st4[k] = make_int4(s.steps[4 * k], s.steps[4 * k + 1], s.steps[4 * k + 2], s.steps[4 * k + 3]);
-    // forward rows by pivot group: {row, nr, sbeg, send}, {rsrc0, rsrc1, lpos, 0}; steps
-    // {L(row, j), L(row+1, j) or -1, j, 0}
+    // forward rows by pivot group: {row, nr, sbeg, send}, {rsrc0, rsrc1, lpos, pin}; a piece of
+    // a cut row other than the last {slot, nr + 4, sbeg, send}; steps {L(row, j), L(row+1, j)
+    // or -1, j, 0}
     const int64_t nfs = (int64_t)s.fsteps.size() / 3;
-    std::vector<int4> fw(2 * std::max<size_t>(s.fwd.size(), 1), make_int4(0, 1, 0, 0));
+    const int64_t xr = h.x_rows;  // cut rows' slots: rows [m, x_rows) of the solution vector
+    auto slot_ok = [&](int v, int nr) { return v >= h.m && v + nr <= xr; };
+    std::vector<int4> fw(2 * std::max<size_t>(s.fwd.size(), 1), make_int4(0, 1, 0, -1));
     std::vector<int4> fs(std::max<int64_t>(nfs, 1), make_int4(0, -1, 0, 0));
     for (int64_t k = 0; k < nfs; ++k) {
         const int la = s.fsteps[3 * k], lb = s.fsteps[3 * k + 1], j = s.fsteps[3 * k + 2];
@@ -2971,16 +3003,18 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
         if (f.sbeg < 0 || f.send < f.sbeg || f.send > nfs) bad("forward step range");
         for (int e = f.sbeg; e < f.send; ++e)
             if ((s.fsteps[3 * e + 1] < 0) != (f.nr == 1)) bad("forward step shape");
-        fw[2 * i] = make_int4(f.row, f.nr, f.sbeg, f.send);
-        fw[2 * i + 1] = make_int4(f.rsrc0, f.nr == 2 ? f.rsrc1 : 0, f.nr == 2 ? f.lpos : 0, 0);
+        if (f.pin != -1 && !slot_ok(f.pin, f.nr)) bad("forward slot");
+        if (f.pout != -1 && !slot_ok(f.pout, f.nr)) bad("forward slot");
+        fw[2 * i] = f.pout >= 0 ? make_int4(f.pout, f.nr + 4, f.sbeg, f.send) : make_int4(f.row, f.nr, f.sbeg, f.send);
+        fw[2 * i + 1] = make_int4(f.rsrc0, f.nr == 2 ? f.rsrc1 : 0, f.nr == 2 ? f.lpos : 0, f.pin);
     }
     for (size_t r = 0; r + 1 < s.ptrF.size(); ++r)
         for (int q = s.ptrF[r]; q + 1 < s.ptrF[r + 1]; ++q)
             if (s.fwd[q].send != s.fwd[q + 1].sbeg) bad("forward steps not contiguous");
-    // backward rows by pivot group: {row, nr, sbeg, send}, {base0, base1, 0, 0}; steps
-    // {U(row, j), U(row+1, j) or -1, j, 0}
+    // backward rows by pivot group: {row, nr, sbeg, send}, {base0, base1, pin, 0}; pieces of cut
+    // rows as the forward ones; steps {U(row, j), U(row+1, j) or -1, j, 0}
     const int64_t nbs = (int64_t)s.bsteps.size() / 3;
-    std::vector<int4> bw(2 * std::max<size_t>(s.bwd.size(), 1), make_int4(0, 1, 0, 0));
+    std::vector<int4> bw(2 * std::max<size_t>(s.bwd.size(), 1), make_int4(0, 1, -1, 0));
     std::vector<int4> bs4(std::max<int64_t>(nbs, 1), make_int4(0, -1, 0, 0));
     for (int64_t k = 0; k < nbs; ++k) {
         const int ua = s.bsteps[3 * k], ub = s.bsteps[3 * k + 1], j = s.bsteps[3 * k + 2];
@@ -2995,8 +3029,10 @@ std::unique_ptr<DevSched> upload_block_sched(p3smi_topology* t, int TL) {
         if (w.sbeg < 0 || w.send < w.sbeg || w.send > nbs) bad("backward step range");
         for (int e = w.sbeg; e < w.send; ++e)
             if ((s.bsteps[3 * e + 1] < 0) != (w.nr == 1)) bad("backward step shape");
-        bw[2 * i] = make_int4(w.row, w.nr, w.sbeg, w.send);
-        bw[2 * i + 1] = make_int4(w.base0, w.nr == 2 ? w.base1 : 0, 0, 0);
+        if (w.pin != -1 && !slot_ok(w.pin, w.nr)) bad("backward slot");
+        if (w.pout != -1 && !slot_ok(w.pout, w.nr)) bad("backward slot");
+        bw[2 * i] = w.pout >= 0 ? make_int4(w.pout, w.nr + 4, w.sbeg, w.send) : make_int4(w.row, w.nr, w.sbeg, w.send);
+        bw[2 * i + 1] = make_int4(w.base0, w.nr == 2 ? w.base1 : 0, w.pin, 0);
     }
     for (size_t r = 0; r + 1 < s.ptrW.size(); ++r)
         for (int q = s.ptrW[r]; q + 1 < s.ptrW[r + 1]; ++q)

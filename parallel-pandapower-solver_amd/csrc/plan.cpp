@@ -962,9 +962,89 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
         b.k.send = (int)(s.steps.size() / 4);
         out.push_back(b.k);
     };
+    // ---- forward and backward rows of a pivot group, cut like the blocks: a row's steps (the
+    // columns j < p of its L rows ascending; the columns of S_last descending) in chunks at the
+    // levels before the row's own, each where its y_j (forward: after the level of j's group)
+    // or x_j (backward: after the depth level of j's group) are final
+    struct RItem {
+        int p, k0, k1;            // group lead, steps [k0, k1) of the row's column list
+        int pin = -1, pout = -1;  // slot ids
+    };
+    std::vector<int> lastg(m);  // the last pivot of each pivot's group
+    for (int p = 0; p < m; ++p) lastg[p] = gsize[p] == 2 ? p + 1 : p;
+    std::vector<std::vector<int>> fcols(m), bcols(m);
+    for (int p = 0; p < m; ++p) {
+        if (gsize[p] == 0) continue;
+        merge(rows[p], gsize[p] == 2 ? &rows[p + 1] : nullptr, ra);
+        for (int j : ra)
+            if (j < p) fcols[p].push_back(j);
+        const int last = lastg[p];
+        for (int q = t.Lp[last + 1] - 1; q >= t.Lp[last]; --q) bcols[p].push_back(t.Li[q]);
+    }
+    std::vector<std::vector<RItem>> levFi(s.nlev), levWi(s.nblev);
+    for (int l = 0; l < s.nlev; ++l)
+        for (int p : levF[l]) levFi[l].push_back(RItem{p, 0, (int)fcols[p].size()});
+    for (int l = 0; l < s.nblev; ++l)
+        for (int p : levW[l]) levWi[l].push_back(RItem{p, 0, (int)bcols[p].size()});
+    std::vector<int> fslot_lev, bslot_lev;  // level that writes each slot (read at the next)
+    auto cut_row = [&](RItem& it, int l, const std::vector<int>& cols, const std::vector<int>& lev_of,
+                       std::vector<int>& slot_lev, std::vector<std::pair<int, RItem>>& out) {
+        const int len = it.k1 - it.k0;
+        if (len <= BLK_SPLIT_MIN) return false;
+        std::vector<std::array<int, 3>> ch;  // (level, beg, end), the last piece first
+        int end = len, lv = l;
+        while (end > 0) {
+            int beg = std::max(0, end - BLK_CHUNK);
+            if (beg < BLK_TAIL) beg = 0;
+            int need = 0;
+            for (int k = beg; k < end; ++k) need = std::max(need, lev_of[lastg[cols[it.k0 + k]]] + 1);
+            if (lv < 0 || need > lv) return false;  // cannot run this early: the row stays whole
+            ch.push_back({lv, beg, end});
+            end = beg;
+            --lv;
+        }
+        if (ch.size() < 2) return false;
+        int prev = -1;
+        for (size_t k = ch.size(); k-- > 1;) {  // earlier chunks first
+            RItem c{it.p, it.k0 + ch[k][1], it.k0 + ch[k][2]};
+            c.pin = prev;
+            c.pout = (int)slot_lev.size();
+            slot_lev.push_back(ch[k][0]);
+            prev = c.pout;
+            out.push_back({ch[k][0], c});
+        }
+        it.k1 = it.k0 + ch[0][2];
+        it.k0 = it.k0 + ch[0][1];
+        it.pin = prev;
+        return true;
+    };
+    {
+        std::vector<std::pair<int, RItem>> fch, bch;
+        for (int l = 0; l < s.nlev; ++l)
+            for (RItem& it : levFi[l]) s.nfcut += cut_row(it, l, fcols[it.p], t.height, fslot_lev, fch);
+        for (int l = 0; l < s.nblev; ++l)
+            for (RItem& it : levWi[l]) s.nbcut += cut_row(it, l, bcols[it.p], t.depth, bslot_lev, bch);
+        for (auto& c : fch) levFi[c.first].push_back(c.second);
+        for (auto& c : bch) levWi[c.first].push_back(c.second);
+    }
+    // slots of two solution-vector rows each: written at level v and read at v + 1, so levels of
+    // one parity share them; the forward slots are all read before the backward rows start,
+    // which reuse the same rows
+    auto xslots = [&](const std::vector<int>& lev_of, int nlevels, std::vector<int>& row) {
+        std::vector<int> cnt((size_t)nlevels + 1, 0), rank(lev_of.size());
+        for (size_t q = 0; q < lev_of.size(); ++q) rank[q] = cnt[lev_of[q]]++;
+        int e = 0, o = 0;
+        for (int v = 0; v <= nlevels; ++v) (v % 2 == 0 ? e : o) = std::max(v % 2 == 0 ? e : o, cnt[v]);
+        row.resize(lev_of.size());
+        for (size_t q = 0; q < lev_of.size(); ++q) row[q] = m + 2 * (lev_of[q] % 2 == 0 ? rank[q] : e + rank[q]);
+        return 2 * (e + o);
+    };
+    std::vector<int> fslot, bslot;
+    s.nxparts = std::max(xslots(fslot_lev, s.nlev, fslot), xslots(bslot_lev, s.nblev, bslot));
+
     std::vector<int> lA, sA, lB, sB, lF, sF, lW, sW;
     std::vector<const Blk*> itA, itB;
-    std::vector<int> itF, itW;
+    std::vector<const RItem*> itF, itW;
     for (int l = 0; l < s.nlev; ++l) {
         std::vector<int64_t> load(nslots, 0), w;
         for (const Blk& b : levA[l]) w.push_back(weight(b));
@@ -972,27 +1052,21 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
         for (size_t i = 0; i < levA[l].size(); ++i) { itA.push_back(&levA[l][i]); lA.push_back(l); sA.push_back(slot[i]); }
         std::vector<int64_t> loadB(nslots, 0), wb;
         for (const Blk& b : levB[l]) wb.push_back(weight(b));
-        for (int p : levF[l]) {
-            const int nr = gsize[p] == 2 ? 2 : 1;
-            merge(rows[p], nr == 2 ? &rows[p + 1] : nullptr, ra);
-            int64_t nstep = 0;
-            for (int j : ra) nstep += j < p;
-            wb.push_back(2 + nstep * (nr + 1));
+        for (const RItem& it : levFi[l]) {
+            const int nr = gsize[it.p] == 2 ? 2 : 1;
+            wb.push_back(2 + (int64_t)(it.k1 - it.k0) * (nr + 1));
         }
         std::vector<int> slotB = lpt_deal(wb, nslots, loadB);
         for (size_t i = 0; i < levB[l].size(); ++i) { itB.push_back(&levB[l][i]); lB.push_back(l); sB.push_back(slotB[i]); }
-        for (size_t i = 0; i < levF[l].size(); ++i) {
-            itF.push_back(levF[l][i]); lF.push_back(l); sF.push_back(slotB[levB[l].size() + i]);
+        for (size_t i = 0; i < levFi[l].size(); ++i) {
+            itF.push_back(&levFi[l][i]); lF.push_back(l); sF.push_back(slotB[levB[l].size() + i]);
         }
     }
     for (int l = 0; l < s.nblev; ++l) {
         std::vector<int64_t> load(nslots, 0), w;
-        for (int p : levW[l]) {
-            const int last = p + (gsize[p] == 2 ? 1 : 0);
-            w.push_back(2 + (int64_t)(t.Lp[last + 1] - t.Lp[last]) * (gsize[p] == 2 ? 3 : 2));
-        }
+        for (const RItem& it : levWi[l]) w.push_back(2 + (int64_t)(it.k1 - it.k0) * (gsize[it.p] == 2 ? 3 : 2));
         std::vector<int> slot = lpt_deal(w, nslots, load);
-        for (size_t i = 0; i < levW[l].size(); ++i) { itW.push_back(levW[l][i]); lW.push_back(l); sW.push_back(slot[i]); }
+        for (size_t i = 0; i < levWi[l].size(); ++i) { itW.push_back(&levWi[l][i]); lW.push_back(l); sW.push_back(slot[i]); }
     }
     // execution order: per level, sub-level A of every list, then sub-level B; steps follow it
     std::vector<int> ordA = group_by_slot(lA, sA, s.nlev, nslots, s.ptrA);
@@ -1002,7 +1076,8 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
         for (int q = s.ptrB[(size_t)l * nslots]; q < s.ptrB[(size_t)(l + 1) * nslots]; ++q) emit(*itB[ordB[q]], s.taskB);
     }
     for (int i : group_by_slot(lF, sF, s.nlev, nslots, s.ptrF)) {
-        const int p = itF[i];
+        const RItem& it = *itF[i];
+        const int p = it.p;
         const int nr = gsize[p] == 2 ? 2 : 1;
         FwdBlock f{};
         f.row = p;
@@ -1010,10 +1085,11 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
         f.rsrc0 = t.perm[p];
         f.rsrc1 = nr == 2 ? t.perm[p + 1] : -1;
         f.lpos = nr == 2 ? posL(p + 1, p) : -1;
+        f.pin = it.pin >= 0 ? fslot[it.pin] : -1;
+        f.pout = it.pout >= 0 ? fslot[it.pout] : -1;
         f.sbeg = (int)(s.fsteps.size() / 3);
-        merge(rows[p], nr == 2 ? &rows[p + 1] : nullptr, ra);
-        for (int j : ra) {
-            if (j >= p) continue;
+        for (int k = it.k0; k < it.k1; ++k) {  // columns j < p of the group's L rows, ascending
+            const int j = fcols[p][k];
             s.fsteps.push_back(in_S(j, p) ? posL(p, j) : zero);
             s.fsteps.push_back(nr == 2 ? (in_S(j, p + 1) ? posL(p + 1, j) : zero) : -1);
             s.fsteps.push_back(j);
@@ -1022,16 +1098,19 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
         s.fwd.push_back(f);
     }
     for (int i : group_by_slot(lW, sW, s.nblev, nslots, s.ptrW)) {
-        const int p = itW[i];
-        const int nr = gsize[p] == 2 ? 2 : 1, last = p + nr - 1;
+        const RItem& it = *itW[i];
+        const int p = it.p;
+        const int nr = gsize[p] == 2 ? 2 : 1;
         BwdBlock b{};
         b.row = p;
         b.nr = nr;
         b.base0 = t.base[p];
         b.base1 = nr == 2 ? t.base[p + 1] : -1;
+        b.pin = it.pin >= 0 ? bslot[it.pin] : -1;
+        b.pout = it.pout >= 0 ? bslot[it.pout] : -1;
         b.sbeg = (int)(s.bsteps.size() / 3);
-        for (int q = t.Lp[last + 1] - 1; q >= t.Lp[last]; --q) {  // columns of S_last, descending
-            const int j = t.Li[q];
+        for (int k = it.k0; k < it.k1; ++k) {  // columns of S_last, descending
+            const int j = bcols[p][k];
             s.bsteps.push_back(posU(p, j));
             s.bsteps.push_back(nr == 2 ? posU(p + 1, j) : -1);
             s.bsteps.push_back(j);
@@ -1344,16 +1423,26 @@ void emulate_blocks(const Topology& t, int ns, const double* J, const double* R,
         E.sub = 2 * l + 1;
         E.lists(s.ptrB, l, ns, [&](int q) { run(s.taskB[q]); });
     }
-    for (int l = 0; l < s.nlev; ++l)
+    for (int l = 0; l < s.nlev; ++l) {
+        E.xlev = l;
         E.lists(s.ptrF, l, ns, [&](int q) {
             const FwdBlock& f = s.fwd[q];
             double a0 = 0.0, a1 = 0.0;
+            if (f.pin >= 0) {  // the previous chunk's sums
+                a0 = E.xslot(f.pin);
+                if (f.nr == 2) a1 = E.xslot(f.pin + 1);
+            }
             for (int e = f.sbeg; e < f.send; ++e) {
                 const int* st = &s.fsteps[3 * (size_t)e];
                 if ((st[1] < 0) != (f.nr == 1)) ++E.bad;
                 const double y = E.xv(st[2]);
                 a0 = std::fma(-(double)E.rd(st[0]), y, a0);
                 if (st[1] >= 0) a1 = std::fma(-(double)E.rd(st[1]), y, a1);
+            }
+            if (f.pout >= 0) {  // a chunk: its sums for the next piece
+                E.set_slot(f.pout, a0);
+                if (f.nr == 2) E.set_slot(f.pout + 1, a1);
+                return;
             }
             const double y0 = R[f.rsrc0] + a0;
             E.x[f.row] = y0;
@@ -1363,17 +1452,29 @@ void emulate_blocks(const Topology& t, int ns, const double* J, const double* R,
                 E.solved[f.row + 1] = 1;
             }
         });
+    }
     std::fill(E.solved.begin(), E.solved.end(), 0);
-    for (int l = 0; l < s.nblev; ++l)
+    std::fill(E.xw.begin(), E.xw.end(), -1);
+    for (int l = 0; l < s.nblev; ++l) {
+        E.xlev = l;
         E.lists(s.ptrW, l, ns, [&](int q) {
             const BwdBlock& b = s.bwd[q];
             double a0 = 0.0, a1 = 0.0;
+            if (b.pin >= 0) {
+                a0 = E.xslot(b.pin);
+                if (b.nr == 2) a1 = E.xslot(b.pin + 1);
+            }
             for (int e = b.sbeg; e < b.send; ++e) {
                 const int* st = &s.bsteps[3 * (size_t)e];
                 if ((st[1] < 0) != (b.nr == 1)) ++E.bad;
                 const double xj = E.xv(st[2]);
                 a0 = std::fma(-(double)E.rd(st[0]), xj, a0);
                 if (st[1] >= 0) a1 = std::fma(-(double)E.rd(st[1]), xj, a1);
+            }
+            if (b.pout >= 0) {
+                E.set_slot(b.pout, a0);
+                if (b.nr == 2) E.set_slot(b.pout + 1, a1);
+                return;
             }
             if (b.nr == 2) {
                 const double x1 = (E.x[b.row + 1] + a1) * (double)E.rd(b.base1);
@@ -1385,6 +1486,7 @@ void emulate_blocks(const Topology& t, int ns, const double* J, const double* R,
             }
             E.solved[b.row] = 1;
         });
+    }
     checks[1] = s.npairs;
     checks[2] = (int64_t)(s.steps.size() / 4);
 }
@@ -1436,8 +1538,9 @@ void build_topology(Topology& t, const int* Yp, const int* Yj, const double* Yx,
     // the factor arena: entries, the zero and one rows, the block schedule's partial slots (the
     // cuts and slots do not depend on the number of task lists)
     const Schedule one = make_schedule(t, 1, 0);
-    t.fac_rows = t.nnz_lu + 2 + std::max<int64_t>(4 * (int64_t)make_block_schedule(t, 1).nparts, one.nparts);
-    t.x_rows = (int64_t)t.m + one.nxparts;
+    const BlockSchedule blk = make_block_schedule(t, 1);
+    t.fac_rows = t.nnz_lu + 2 + std::max<int64_t>(4 * (int64_t)blk.nparts, one.nparts);
+    t.x_rows = (int64_t)t.m + std::max(one.nxparts, blk.nxparts);
     t.rcond0 = reference_pivot_ratio(t);
     t.lean_valid = t.rcond0 > 0.0;
 }

@@ -99,6 +99,10 @@ enum BlockKind { BLK_DIAG1 = 0, BLK_DIAG2, BLK_U1, BLK_U2, BLK_L1, BLK_L2, BLK_P
 // starts at +0 and adds products with fma never becomes -0). Every entry therefore sums the same
 // terms in the same order as an uncut block, so the factors keep their bits. Partial slots are
 // reused once read; the cuts do not depend on the number of task lists.
+// Forward and backward rows of a pivot group are cut the same way, over their steps in order:
+// a chunk at the level before the next piece stores its one or two running sums to
+// solution-vector rows past m, and the next piece starts its sums from them (the same
+// additions in the same order as the uncut row).
 constexpr int BLK_SPLIT_MIN = 32;  // sums over more pivots than this are cut
 constexpr int BLK_CHUNK = 16;      // pivots per chunk (a leftover under BLK_TAIL joins its neighbour)
 constexpr int BLK_TAIL = 8;
@@ -123,6 +127,10 @@ struct FwdBlock {
     int rsrc0, rsrc1;      // right-hand side entries (Jacobian row order)
     int lpos;              // pair: position of L(row+1, row)
     int sbeg, send;        // forward steps [sbeg, send)
+    // a row cut into chunks (see BLK_SPLIT_MIN): pin = solution-vector slot (row >= m) whose
+    // nr rows hold the sums of the previous chunk to start from, -1 none; pout = the slot a
+    // chunk other than the last stores its sums to, -1 for the row's last piece
+    int pin = -1, pout = -1;
 };
 
 // Backward rows of a pivot group, columns j in descending order (the order of every
@@ -132,6 +140,7 @@ struct BwdBlock {
     int row, nr;           // first row (pivot index), 1 or 2 rows
     int base0, base1;      // 1/U(row,row) and (pair) 1/U(row+1,row+1) positions
     int sbeg, send;        // backward steps [sbeg, send)
+    int pin = -1, pout = -1;  // cut row: solution-vector slots, as FwdBlock
 };
 
 struct BlockSchedule {
@@ -139,6 +148,8 @@ struct BlockSchedule {
     int npairs = 0;  // pivot pairs
     int nparts = 0;  // partial slots (4 factor-arena rows each, from nnz_lu + 2)
     int ncut = 0;    // blocks cut into chunks
+    int nxparts = 0; // solution-vector rows past m holding the cut substitution rows' sums
+    int nfcut = 0, nbcut = 0;  // forward, backward rows cut into chunks
     // per (level, slot): sub-level A = diagonal blocks and U blocks of single pivots;
     // sub-level B = the other U and L blocks and the forward rows.
     std::vector<int> ptrA, ptrB, ptrF, ptrW;
