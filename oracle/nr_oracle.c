@@ -29,7 +29,7 @@
  *
  * The reference delegates ordering and pivoting to SuiteSparse KLU 7.14.1
  * (p3s/cpp/ci/build_klu.py:49-50), which is not available here. Its role is restated:
- *   - ordering: a fill-reducing minimum degree order (KLU default: AMD), taken on the buses
+ *   - ordering: a fill-reducing order (KLU default: AMD): minimum fill, taken on the buses
  *     (the quotient graph of the Jacobian pattern), each bus's P equation before its Q
  *     equation -- the order the device plan factors in, so that the frozen plan's pivot
  *     ratios, and with them the reuse guard's decisions, are the same on both;
@@ -198,10 +198,140 @@ static void o_min_degree_w(int m, const int* Ap, const int* Ai, const int* wt, i
 
 static void o_min_degree(int m, const int* Ap, const int* Ai, int* perm) { o_min_degree_w(m, Ap, Ai, NULL, perm); }
 
-/* Minimum degree on the buses: bus k = pvpq[k] (its P equation is row k), adjacent when
- * any of their equations couple in the Jacobian pattern, a bus's degree the number of
- * equations of its neighbours (2 for a PQ bus, 1 for a PV bus); then each bus's P equation
- * before its Q equation (row npvpq + its position in pq). Q[k] = column eliminated k-th. */
+/* Minimum fill on a weighted symmetric pattern (diagonal ignored), on the explicit
+ * elimination graph: eliminate the node whose clique adds the least weight (sum of wt_a wt_b
+ * over its non-adjacent live neighbour pairs), ties by weighted degree, then lower index.
+ * Keys of the clique and its neighbours are recomputed after each elimination (a binary heap
+ * with versions; stale entries are skipped). perm[k] = node eliminated k-th. */
+typedef struct { int64_t fill, deg; int node, ver; } fillent;
+static int fe_less(fillent a, fillent b) {
+    if (a.fill != b.fill) return a.fill < b.fill;
+    if (a.deg != b.deg) return a.deg < b.deg;
+    return a.node < b.node;
+}
+static void fheap_push(fillent** h, int* hn, int* cap, fillent e) {
+    if (*hn >= *cap) { *cap *= 2; *h = (fillent*)realloc(*h, (size_t)*cap * sizeof(fillent)); if (!*h) abort(); }
+    int i = (*hn)++;
+    (*h)[i] = e;
+    while (i > 0) {
+        int p = (i - 1) / 2;
+        if (!fe_less((*h)[i], (*h)[p])) break;
+        fillent t = (*h)[i]; (*h)[i] = (*h)[p]; (*h)[p] = t; i = p;
+    }
+}
+static fillent fheap_pop(fillent* h, int* hn) {
+    fillent top = h[0];
+    h[0] = h[--(*hn)];
+    int i = 0;
+    for (;;) {
+        int l = 2 * i + 1, r = l + 1, s = i;
+        if (l < *hn && fe_less(h[l], h[s])) s = l;
+        if (r < *hn && fe_less(h[r], h[s])) s = r;
+        if (s == i) break;
+        fillent t = h[i]; h[i] = h[s]; h[s] = t; i = s;
+    }
+    return top;
+}
+static void o_min_fill_w(int m, const int* Ap, const int* Ai, const int* wt, int* perm) {
+    ivec* adj = NEWV(ivec, m);
+    int* mark = NEWV(int, m);
+    int* stamp = NEWV(int, m);
+    int* ver = NEWV(int, m);
+    char* gone = NEWV(char, m);
+    for (int i = 0; i < m; ++i) mark[i] = stamp[i] = -1;
+    for (int j = 0; j < m; ++j)
+        for (int p = Ap[j]; p < Ap[j + 1]; ++p) {
+            int i = Ai[p];
+            if (i == j) continue;
+            iv_push(&adj[i], j);
+            iv_push(&adj[j], i);
+        }
+    for (int i = 0; i < m; ++i) { /* dedupe */
+        int w = 0;
+        for (int q = 0; q < adj[i].n; ++q) {
+            int u = adj[i].a[q];
+            if (mark[u] != i) { mark[u] = i; adj[i].a[w++] = u; }
+        }
+        adj[i].n = w;
+    }
+    for (int i = 0; i < m; ++i) mark[i] = -1;
+    int tick = 0, st = 0;
+    #define O_FILL_KEY(v, F, D) do { \
+        (F) = 0; (D) = 0; \
+        for (int q_ = 0; q_ < adj[v].n; ++q_) (D) += wt[adj[v].a[q_]]; \
+        for (int i_ = 0; i_ < adj[v].n; ++i_) { \
+            int a_ = adj[v].a[i_]; ++tick; \
+            for (int r_ = 0; r_ < adj[a_].n; ++r_) mark[adj[a_].a[r_]] = tick; \
+            for (int j_ = i_ + 1; j_ < adj[v].n; ++j_) \
+                if (mark[adj[v].a[j_]] != tick) (F) += (int64_t)wt[a_] * wt[adj[v].a[j_]]; \
+        } } while (0)
+    int cap = 2 * m + 16, hn = 0;
+    fillent* h = (fillent*)malloc((size_t)cap * sizeof(fillent));
+    for (int v = 0; v < m; ++v) {
+        int64_t f, d;
+        O_FILL_KEY(v, f, d);
+        fheap_push(&h, &hn, &cap, (fillent){f, d, v, 0});
+    }
+    ivec touched = {0, 0, 0};
+    for (int k = 0; k < m; ++k) {
+        fillent e;
+        for (;;) {
+            e = fheap_pop(h, &hn);
+            if (!gone[e.node] && e.ver == ver[e.node]) break;
+        }
+        int v = e.node;
+        perm[k] = v;
+        gone[v] = 1;
+        ivec nb = adj[v];
+        adj[v].a = NULL; adj[v].n = adj[v].cap = 0;
+        for (int q = 0; q < nb.n; ++q) {
+            int u = nb.a[q];
+            if (gone[u]) continue;
+            ++st;
+            int w = 0;
+            for (int r = 0; r < adj[u].n; ++r) { /* drop v and eliminated nodes */
+                int x = adj[u].a[r];
+                if (gone[x]) continue;
+                adj[u].a[w++] = x;
+                stamp[x] = st;
+            }
+            adj[u].n = w;
+            stamp[u] = st;
+            for (int r = 0; r < nb.n; ++r) {
+                int x = nb.a[r];
+                if (!gone[x] && stamp[x] != st) { stamp[x] = st; iv_push(&adj[u], x); }
+            }
+        }
+        /* keys change for the clique and its neighbours */
+        ++st;
+        touched.n = 0;
+        for (int q = 0; q < nb.n; ++q) {
+            int u = nb.a[q];
+            if (gone[u]) continue;
+            if (stamp[u] != st) { stamp[u] = st; iv_push(&touched, u); }
+            for (int r = 0; r < adj[u].n; ++r) {
+                int x = adj[u].a[r];
+                if (stamp[x] != st) { stamp[x] = st; iv_push(&touched, x); }
+            }
+        }
+        for (int q = 0; q < touched.n; ++q) {
+            int u = touched.a[q];
+            int64_t f, d;
+            O_FILL_KEY(u, f, d);
+            fheap_push(&h, &hn, &cap, (fillent){f, d, u, ++ver[u]});
+        }
+        free(nb.a);
+    }
+    #undef O_FILL_KEY
+    free(touched.a);
+    for (int i = 0; i < m; ++i) free(adj[i].a);
+    free(adj); free(mark); free(stamp); free(ver); free(gone); free(h);
+}
+
+/* Minimum fill on the buses: bus k = pvpq[k] (its P equation is row k), adjacent when any of
+ * their equations couple in the Jacobian pattern, a bus weighing its equations (2 for a PQ
+ * bus, 1 for a PV bus); then each bus's P equation before its Q equation (row npvpq + its
+ * position in pq). Q[k] = column eliminated k-th. */
 static void o_bus_min_degree(int m, int npvpq, const int* Jp, const int* Ji, const int* pvpq, const int* pq_pos,
                              const int* pvpq_pos, const int* pq, int* Q) {
     int* kb = NEWV(int, m); /* equation -> bus index */
@@ -220,7 +350,7 @@ static void o_bus_min_degree(int m, int npvpq, const int* Jp, const int* Ji, con
     int* wt = NEWV(int, npvpq);
     for (int b = 0; b < npvpq; ++b) wt[b] = pq_pos[pvpq[b]] >= 0 ? 2 : 1;
     int* order = NEWV(int, npvpq);
-    o_min_degree_w(npvpq, Bp, Bi, wt, order);
+    o_min_fill_w(npvpq, Bp, Bi, wt, order);
     int k = 0;
     for (int q = 0; q < npvpq; ++q) {
         const int b = order[q];

@@ -10,64 +10,12 @@
 #include <limits>
 #include <numeric>
 #include <queue>
+#include <tuple>
 #include <stdexcept>
 
 namespace p3smi {
 
 namespace {
-
-// ------------------------------------------------------------------ minimum degree
-// Greedy minimum degree on the explicit elimination graph of a symmetric pattern
-// (adjacency without the diagonal). Eliminating v turns its live neighbours into a
-// clique. A node's degree is the number of its live neighbours, or with weights `wt` the
-// sum of theirs. Degree ties go to the lowest index (deterministic order).
-std::vector<int> minimum_degree(int m, std::vector<std::vector<int>> adj, const std::vector<int>* wt = nullptr) {
-    std::vector<char> dead(m, 0);
-    std::vector<int> stamp(m, -1);
-    auto degree = [&](int v) {
-        if (!wt) return (int)adj[v].size();
-        int d = 0;
-        for (int u : adj[v]) d += (*wt)[u];
-        return d;
-    };
-    using Key = std::pair<int, int>;  // (degree, node)
-    std::priority_queue<Key, std::vector<Key>, std::greater<Key>> pq;
-    for (int i = 0; i < m; ++i) pq.push({degree(i), i});
-    std::vector<int> order;
-    order.reserve(m);
-    int tick = 0;
-    while ((int)order.size() < m) {
-        Key top = pq.top();
-        pq.pop();
-        int v = top.second;
-        if (dead[v] || top.first != degree(v)) continue;  // stale entry
-        dead[v] = 1;
-        order.push_back(v);
-        std::vector<int> nb;
-        nb.swap(adj[v]);
-        for (int u : nb) {
-            if (dead[u]) continue;
-            ++tick;
-            auto& au = adj[u];
-            size_t w = 0;
-            for (size_t r = 0; r < au.size(); ++r) {
-                int x = au[r];
-                if (dead[x]) continue;
-                au[w++] = x;
-                stamp[x] = tick;
-            }
-            au.resize(w);
-            stamp[u] = tick;
-            for (int x : nb)
-                if (!dead[x] && stamp[x] != tick) {
-                    stamp[x] = tick;
-                    au.push_back(x);
-                }
-            pq.push({degree(u), u});
-        }
-    }
-    return order;
-}
 
 // Longest-processing-time dealing: tasks (weights w) of one level go, heaviest first, to
 // the currently least-loaded of `ns` lists. `load` carries over between calls so that two
@@ -206,19 +154,101 @@ static void build_jacobian_pattern(Topology& t) {
     if ((int)t.Ci.size() != t.nnzJ) throw std::runtime_error("Jacobian pattern: column and row views differ");
 }
 
-// Minimum degree on the buses, then each bus's P equation before its Q equation. The two
+// ------------------------------------------------------------------ ordering
+// Minimum fill on a weighted graph: eliminate the node whose clique adds the least weight
+// (sum of w_a w_b over its non-adjacent live neighbour pairs), ties by weighted degree (sum of
+// its live neighbours' w), then lowest index. After an elimination only the clique and its
+// neighbours change keys; they are pushed again with a new version, older entries skipped.
+std::vector<int> minimum_fill(int m, std::vector<std::vector<int>> adj, const std::vector<int>& wt) {
+    std::vector<char> dead(m, 0);
+    std::vector<int> mark(m, -1);
+    int tick = 0;
+    auto cost = [&](int v, int64_t& fill, int64_t& deg) {
+        fill = 0;
+        deg = 0;
+        const auto& nv = adj[v];
+        for (int a : nv) deg += wt[a];
+        for (size_t i = 0; i < nv.size(); ++i) {
+            const int a = nv[i];
+            ++tick;
+            for (int x : adj[a]) mark[x] = tick;
+            for (size_t j = i + 1; j < nv.size(); ++j)
+                if (mark[nv[j]] != tick) fill += (int64_t)wt[a] * wt[nv[j]];
+        }
+    };
+    using Key = std::tuple<int64_t, int64_t, int, int>;  // fill, degree, node, version
+    std::priority_queue<Key, std::vector<Key>, std::greater<Key>> pq;
+    std::vector<int> ver(m, 0);
+    for (int v = 0; v < m; ++v) {
+        int64_t f, d;
+        cost(v, f, d);
+        pq.push({f, d, v, 0});
+    }
+    std::vector<int> order;
+    order.reserve(m);
+    std::vector<int> stamp(m, -1);
+    int st = 0;
+    while ((int)order.size() < m) {
+        auto [f, d, v, vv] = pq.top();
+        pq.pop();
+        if (dead[v] || vv != ver[v]) continue;
+        dead[v] = 1;
+        order.push_back(v);
+        std::vector<int> nb;
+        nb.swap(adj[v]);
+        std::vector<int> touched;
+        for (int u : nb) {
+            if (dead[u]) continue;
+            ++st;
+            auto& au = adj[u];
+            size_t w = 0;
+            for (size_t r = 0; r < au.size(); ++r) {
+                int x = au[r];
+                if (dead[x]) continue;
+                au[w++] = x;
+                stamp[x] = st;
+            }
+            au.resize(w);
+            stamp[u] = st;
+            for (int x : nb)
+                if (!dead[x] && stamp[x] != st) {
+                    stamp[x] = st;
+                    au.push_back(x);
+                }
+        }
+        // fill changes for the clique and its neighbours
+        ++st;
+        for (int u : nb) {
+            if (dead[u]) continue;
+            if (stamp[u] != st) { stamp[u] = st; touched.push_back(u); }
+            for (int x : adj[u]) if (stamp[x] != st) { stamp[x] = st; touched.push_back(x); }
+        }
+        for (int u : touched) {
+            int64_t f2, d2;
+            cost(u, f2, d2);
+            pq.push({f2, d2, u, ++ver[u]});
+        }
+    }
+    return order;
+}
+
+// Minimum fill on the buses, then each bus's P equation before its Q equation. The two
 // equations of a bus couple to the same neighbours, so eliminated side by side they form a
 // pivot pair (parent p + 1, S_p = {p + 1} + S_{p+1}) at every PQ bus, and the 2x2-block
-// schedule loads one row of operands for both. A bus's degree counts the equations of its
-// neighbours (2 for a PQ bus, 1 for a PV bus), as the fill it causes does. The batch LU per
-// launch, each step measured against the previous one on one machine:
+// schedule loads one row of operands for both. A bus weighs its equations (2 for a PQ bus, 1
+// for a PV bus): a fill edge between buses a and b is w_a w_b fill entries of the Jacobian's
+// factors. The batch LU per launch, each step measured against the previous one on one
+// machine (the LU is bound by its operand reads, so fewer operand pairs pay):
 //   - minimum degree over single equations (the two side by side at 89% of the PQ buses of
-//     the 10k grid) -> on the buses counting neighbour buses: n1_10k 40.7 -> 34.5 ms,
-//     ts2000 4.62 -> 4.21 ms, with 0.7-2% more fill;
-//   - counting neighbour buses -> counting neighbour equations: n1_10k 34.5 -> 32.0 ms,
-//     ts2000 4.04 -> 3.91 ms; on n1_10k 4.5% less fill, 17% fewer operand pairs and 176
-//     levels against 204.
-std::vector<int> bus_minimum_degree(const Topology& t, const std::vector<std::vector<int>>& adj) {
+//     the 10k grid) -> minimum degree on the buses counting neighbour buses: n1_10k
+//     40.7 -> 34.5 ms, ts2000 4.62 -> 4.21 ms, with 0.7-2% more fill;
+//   - counting neighbour equations in the degree: n1_10k 34.5 -> 32.0 ms, ts2000 4.04 ->
+//     3.91 ms; on n1_10k 4.5% less fill, 17% fewer operand pairs, 176 levels against 204;
+//   - minimum fill instead of minimum degree: n1_10k 31.8 -> 29.2 ms, ts2000 3.84 -> 3.73 ms;
+//     on n1_10k 3.2% less fill, 15% fewer operand pairs, 168 levels. Multiple elimination
+//     (an independent set of minimum-degree buses per round) gave 2% fewer pairs and more
+//     levels, and was not kept.
+std::vector<int> bus_minimum_fill(const Topology& t, const std::vector<std::vector<int>>& adj) {
     const int m = t.m;
     std::vector<int> bus_of(m, -1);
     for (int i = 0; i < t.n; ++i) {
@@ -246,7 +276,7 @@ std::vector<int> bus_minimum_degree(const Topology& t, const std::vector<std::ve
     perm.reserve(m);
     std::vector<int> wt(nb);
     for (int k = 0; k < nb; ++k) wt[k] = t.row_q[buses[k]] >= 0 ? 2 : 1;
-    for (int k : minimum_degree(nb, std::move(badj), &wt)) {
+    for (int k : minimum_fill(nb, std::move(badj), wt)) {
         const int b = buses[k];
         if (t.row_p[b] >= 0) perm.push_back(t.row_p[b]);
         if (t.row_q[b] >= 0) perm.push_back(t.row_q[b]);
@@ -272,7 +302,7 @@ static void build_lu(Topology& t, int ordering) {
         t.perm.resize(m);
         std::iota(t.perm.begin(), t.perm.end(), 0);
     } else {
-        t.perm = bus_minimum_degree(t, adj);
+        t.perm = bus_minimum_fill(t, adj);
     }
     t.pinv.assign(m, -1);
     for (int p = 0; p < m; ++p) t.pinv[t.perm[p]] = p;
