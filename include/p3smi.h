@@ -89,8 +89,8 @@ int p3smi_hbm_probe(int32_t pattern, int64_t bytes, int32_t repeats, double* ms_
 /* ---- topology (replaces nr_klu.Solver.__init__, p3s/cpp/nr_klu.cpp:966-994, and
  *      PolarNewtonSolverCUDA.__init__, p3s/cuda/nr_polar_solver.py:282-376) ---------------
  * Yp (n+1), Yj (nnz): CSR pattern of Ybus, columns sorted per row; Yx (nnz) complex.
- * pv (npv), pq (npq): bus index sets. ordering: 0 minimum degree (on the buses, each bus's P
- * equation before its Q equation), 2 natural.
+ * pv (npv), pq (npq): bus index sets. ordering: 0 fill-reducing (minimum fill on the buses,
+ * each bus's P equation before its Q equation), 2 natural.
  * n_slots: task lists per workgroup of the LU kernel (0 = default).
  * The host plan is built here; device buffers are uploaded on first use. */
 int p3smi_topology_create(const int32_t* Yp, const int32_t* Yj, const double* Yx, int32_t n, int32_t nnz,

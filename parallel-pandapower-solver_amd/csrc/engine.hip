@@ -4108,7 +4108,7 @@ int p3smi_topology_create(const int32_t* Yp, const int32_t* Yj, const double* Yx
     return guarded([&] {
         if (!out) throw Error(P3SMI_E_ARG, "out is NULL");
         *out = nullptr;
-        if (ordering != 0 && ordering != 2) throw Error(P3SMI_E_ARG, "ordering must be 0 (minimum degree) or 2 (natural)");
+        if (ordering != 0 && ordering != 2) throw Error(P3SMI_E_ARG, "ordering must be 0 (fill-reducing) or 2 (natural)");
         if (n_slots == 0) n_slots = 8;
         if (n_slots != 1 && n_slots != 2 && n_slots != 4 && n_slots != 8 && n_slots != 16)
             throw Error(P3SMI_E_ARG, "n_slots must be 1, 2, 4, 8 or 16");

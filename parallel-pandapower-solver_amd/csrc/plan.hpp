@@ -8,7 +8,7 @@
 //   * the Jacobian pattern (CSR by equation) and the map from each Ybus nonzero's four
 //     derivative blocks to Jacobian entries (block ids 0 dP/dVa, 1 dP/dVm, 2 dQ/dVa,
 //     3 dQ/dVm, as p3s/cpp/nr_klu.cpp:104-108);
-//   * a symmetric fill-reducing order (minimum degree on the Jacobian graph), the
+//   * a symmetric fill-reducing order (minimum fill on the buses, P before Q), the
 //     elimination tree and the static-pivot factor pattern;
 //   * the level schedule the GPU executes: every factor entry is one task (a dot product
 //     over precomputed operand pairs), tasks of one elimination-tree height run
@@ -197,7 +197,8 @@ struct Topology {
     int ordering = 0;
 };
 
-// Build everything. ordering: 0 = minimum degree, 2 = natural. Throws std::runtime_error.
+// Build everything. ordering: 0 = fill-reducing (minimum fill on the buses), 2 = natural.
+// Throws std::runtime_error.
 void build_topology(Topology& t, const int* Yp, const int* Yj, const double* Yx, int n, int nnzY, const int* pv,
                     int npv, const int* pq, int npq, int ordering, int nslots);
 

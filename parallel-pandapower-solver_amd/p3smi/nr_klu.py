@@ -58,7 +58,7 @@ class Solver:
         if self.Yx.shape[0] != self.nnz:
             raise RuntimeError("Solver: Yx and Yj lengths differ")
         if ordering not in (0, 2):
-            raise RuntimeError("Solver: ordering must be 0 (minimum degree) or 2 (natural)")
+            raise RuntimeError("Solver: ordering must be 0 (fill-reducing) or 2 (natural)")
         h = C.c_void_p()
         check(lib().p3smi_topology_create(ptr(self.Yp), ptr(self.Yj), ptr(self.Yx), self.n, self.nnz, ptr(self.pv),
                                           int(self.pv.size), ptr(self.pq), int(self.pq.size), int(ordering),
