@@ -182,6 +182,8 @@ constexpr double REUSE_RCOND_RATIO = 1e-3;
 // Jacobian positions) is staged in LDS when it has at most EV_NZ nonzeros
 // (the host cuts tiles to fit, except around a single row with more).
 constexpr int EV_WAVES = 4;
+// compact one-case LU arenas per batch (LuCompact): the stragglers of a pass and single solves
+constexpr int LU_COMPACT_SLOTS = 16;
 constexpr int EV_NZ = 768;
 
 struct Error : std::runtime_error {
@@ -210,6 +212,15 @@ __device__ __forceinline__ double nan_max(double w, double a) { return (a > w ||
 __global__ void k_ctl_reset(unsigned* __restrict__ ctl, int B, int S, unsigned p0, unsigned extra) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < S) ctl[c] = c < B ? (set_prec(SG_START, p0) | extra) : SG_IDLE;
+}
+
+// The zero and one rows of every compact one-case arena (LuCompact): slot w's rows start at
+// w * frows.
+__global__ void k_compact_rows(double* __restrict__ p, int cap, long long frows, long long zero) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= cap) return;
+    p[(size_t)w * (size_t)frows + (size_t)zero] = 0.0;
+    p[(size_t)w * (size_t)frows + (size_t)zero + 1] = 1.0;
 }
 
 __global__ void k_clear_int(int* p, int n) {
@@ -700,15 +711,31 @@ struct LuArgs {
     int want;
 };
 
+// Compact arenas of one-case launches (G = 1, FP64 factor + solve): a workgroup that takes a
+// slot (w < cap, from a counter the host clears before the launch) factors and substitutes in
+// rows of stride 1 of its own -- fac rows [w * frows, (w + 1) * frows), x rows [w * xrows, ...)
+// -- instead of the batch's rows of stride S, where each of its operands is alone in a
+// 128-byte line: its factors (8 nnz_LU bytes) then stay in one L2 instead of spanning
+// nnz_LU lines. The same arithmetic in the same order; the solution goes to the batch's x at
+// the end. Nothing reads the factors of such a launch afterwards. cap 0: batch rows.
+struct LuCompact {
+    double* fac = nullptr;  // cap x frows, the zero and one rows of every slot set
+    double* x = nullptr;    // cap x xrows
+    int* ctr = nullptr;
+    int cap = 0;
+    long long frows = 0, xrows = 0;
+    int m = 0;
+};
+
 template <int G, int NS, typename real, bool FACTOR>
 __device__ __forceinline__ void lu_stream_body(
     const int* __restrict__ ptrA, const int4* __restrict__ taskA, const int* __restrict__ ptrB,
     const int4* __restrict__ taskB, const int* __restrict__ diagB, const int2* __restrict__ pairs,
     const int* __restrict__ ptrF, const int4* __restrict__ fwd, const int2* __restrict__ fent,
     const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int2* __restrict__ bent,
-    const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
+    const double* __restrict__ Jx, real* __restrict__ fac_b, const double* __restrict__ rhs, double* __restrict__ x_b,
     unsigned* __restrict__ ctl, double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min,
-    int nlev, int nblev, int S, int bx, int gx) {
+    int nlev, int nblev, int S, int bx, int gx, const LuCompact& cmp = LuCompact{}) {
     constexpr bool SINGLE = sizeof(real) == 4;
     constexpr int TL = 64 * NS / G;
     // fewer than 64 cases per workgroup: lists differ within a wavefront, so their indices and
@@ -734,6 +761,24 @@ __device__ __forceinline__ void lu_stream_body(
     const int c = wg * G + lane;
     bool act = selected(sel, ctl[c]);
     if (!__syncthreads_or(act)) return;
+    // factor and solution rows: the batch's (stride S, column c) or a compact slot (stride 1)
+    real* __restrict__ fac = fac_b;
+    double* __restrict__ x = x_b;
+    int Sf = S, cf = c;
+    if constexpr (G == 1 && FACTOR && !SINGLE) {
+        if (cmp.cap > 0) {
+            __shared__ int s_slot;
+            if (threadIdx.x == 0) s_slot = atomicAdd(cmp.ctr, 1);
+            __syncthreads();
+            const int w = s_slot;
+            if (w < cmp.cap) {
+                fac = reinterpret_cast<real*>(cmp.fac + (size_t)w * (size_t)cmp.frows);
+                x = cmp.x + (size_t)w * (size_t)cmp.xrows;
+                Sf = 1;
+                cf = 0;
+            }
+        }
+    }
 
     double pmin = DBL_MAX, pmax = 0.0, psum = 0.0;
     // sub-level A: pivots (y = -src - 2) and U entries (y = Jacobian entry or -1 for fill); in
@@ -745,7 +790,7 @@ __device__ __forceinline__ void lu_stream_body(
     };
     auto finishA = [&](int4 t, In1 in, real sum) {
         if (t.y == LU_PART) {
-            fac[at(t.x, S, c)] = -sum;
+            fac[at(t.x, Sf, cf)] = -sum;
             return;
         }
         real v = (real)in.v + sum;
@@ -756,38 +801,38 @@ __device__ __forceinline__ void lu_stream_body(
             psum += ab;
             v = (real)1 / v;
         }
-        fac[at(t.x, S, c)] = v;
+        fac[at(t.x, Sf, cf)] = v;
     };
     // sub-level B: L entries, scaled by the pivot inverse diagB[q]
     auto startB = [&](int q, int4 t) -> In2 {
-        return {t.y >= 0 ? Jx[at(t.y, S, c)] : 0.0, (double)fac[at(diagB[q], S, c)]};
+        return {t.y >= 0 ? Jx[at(t.y, S, c)] : 0.0, (double)fac[at(diagB[q], Sf, cf)]};
     };
     auto finishB = [&](int4 t, In2 in, real sum) {
-        fac[at(t.x, S, c)] = t.y == LU_PART ? -sum : ((real)in.v + sum) * (real)in.d;
+        fac[at(t.x, Sf, cf)] = t.y == LU_PART ? -sum : ((real)in.v + sum) * (real)in.d;
     };
     // forward rows (row, right-hand side index): y = rhs - sum L y; a chunk of a cut row
     // (y = LU_PART) stores its sum negated to its solution-vector slot (row index >= m)
     auto startF = [&](int, int4 t) -> In1 { return {t.y == LU_PART ? 0.0 : rhs[at(t.y, S, c)]}; };
-    auto finishF = [&](int4 t, In1 in, double sum) { x[at(t.x, S, c)] = t.y == LU_PART ? -sum : in.v + sum; };
+    auto finishF = [&](int4 t, In1 in, double sum) { x[at(t.x, Sf, cf)] = t.y == LU_PART ? -sum : in.v + sum; };
     // backward rows (row, pivot position): x = (y - sum U x) / U(p,p); chunks as forward
     auto startW = [&](int, int4 t) -> In2 {
         if (t.y == LU_PART) return {0.0, 0.0};
-        return {x[at(t.x, S, c)], (double)fac[at(t.y, S, c)]};
+        return {x[at(t.x, Sf, cf)], (double)fac[at(t.y, Sf, cf)]};
     };
     auto finishW = [&](int4 t, In2 in, double sum) {
-        x[at(t.x, S, c)] = t.y == LU_PART ? -sum : (in.v + sum) * in.d;
+        x[at(t.x, Sf, cf)] = t.y == LU_PART ? -sum : (in.v + sum) * in.d;
     };
 
     for (int l = 0; l < nlev; ++l) {
         const int i0 = l * TL + slot;
         if (act) {
             if (FACTOR)
-                seg_stream<K, real, In1>(taskA, ptrA[i0], ptrA[i0 + 1], pairs, fac, fac, S, c, startA, finishA);
-            seg_stream<KF, double, In1>(fwd, ptrF[i0], ptrF[i0 + 1], fent, fac, x, S, c, startF, finishF);
+                seg_stream<K, real, In1>(taskA, ptrA[i0], ptrA[i0 + 1], pairs, fac, fac, Sf, cf, startA, finishA);
+            seg_stream<KF, double, In1>(fwd, ptrF[i0], ptrF[i0 + 1], fent, fac, x, Sf, cf, startF, finishF);
         }
         __syncthreads();
         if (FACTOR) {
-            if (act) seg_stream<K, real, In2>(taskB, ptrB[i0], ptrB[i0 + 1], pairs, fac, fac, S, c, startB, finishB);
+            if (act) seg_stream<K, real, In2>(taskB, ptrB[i0], ptrB[i0 + 1], pairs, fac, fac, Sf, cf, startB, finishB);
             __syncthreads();
         }
     }
@@ -815,9 +860,11 @@ __device__ __forceinline__ void lu_stream_body(
     }
     for (int l = 0; l < nblev; ++l) {
         const int i0 = l * TL + slot;
-        if (act) seg_stream<KF, double, In2>(bwd, ptrW[i0], ptrW[i0 + 1], bent, fac, x, S, c, startW, finishW);
+        if (act) seg_stream<KF, double, In2>(bwd, ptrW[i0], ptrW[i0 + 1], bent, fac, x, Sf, cf, startW, finishW);
         __syncthreads();
     }
+    if (x != x_b && act)  // a compact slot: the solution to the batch's rows
+        for (int r = (int)threadIdx.x; r < cmp.m; r += (int)blockDim.x) x_b[at(r, S, c)] = x[r];
 }
 
 template <int G, int NS, typename real, bool FACTOR>
@@ -828,10 +875,10 @@ __global__ __launch_bounds__(64 * NS) void k_lu_stream(
     const int* __restrict__ ptrW, const int4* __restrict__ bwd, const int2* __restrict__ bent,
     const double* __restrict__ Jx, real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x,
     unsigned* __restrict__ ctl, double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min,
-    int nlev, int nblev, int S) {
+    int nlev, int nblev, int S, LuCompact cmp) {
     lu_stream_body<G, NS, real, FACTOR>(ptrA, taskA, ptrB, taskB, diagB, pairs, ptrF, fwd, fent, ptrW, bwd, bent, Jx,
                                         fac, rhs, x, ctl, pstat, guard_trips, sel, rcond_min, nlev, nblev, S,
-                                        blockIdx.x, gridDim.x);
+                                        blockIdx.x, gridDim.x, cmp);
 }
 
 template <int G, int NS, typename real, bool FACTOR>
@@ -2792,6 +2839,9 @@ struct p3smi_batch {
     int B = 0, S = 0;
     DevBuf<double> Vm, Va, Ve, Vf, Vm0, Va0, Jx, fac, R, x;  // polar, rectangular and base state
     DevBuf<float> fac32;                // FP32 factors (mixed precision)
+    DevBuf<double> cmp_fac, cmp_x;      // compact one-case arenas (LuCompact), cmp_cap slots
+    DevBuf<int> cmp_ctr;
+    int cmp_cap = 0;
     DevBuf<double> rr, dvec, prevd, rpart;  // refinement residual, correction, norms
     int nparts_r = 0;
     bool mixed_ready = false;
@@ -3200,6 +3250,14 @@ void batch_alloc(p3smi_batch* b, p3smi_topology* t, int cases) {
     one_row(b->fac.p + ((size_t)h.nnz_lu + 1) * S, S, t->stream);
     b->R.alloc((size_t)h.m * S);
     b->x.alloc((size_t)h.x_rows * S);  // + the slots of cut substitution rows
+    // compact arenas of one-case FP64 launches (LuCompact): one per case, at most
+    // LU_COMPACT_SLOTS (a launch with more cases runs the others in the batch's rows)
+    b->cmp_cap = std::min(cases, LU_COMPACT_SLOTS);
+    b->cmp_fac.alloc((size_t)h.fac_rows * (size_t)b->cmp_cap);
+    b->cmp_x.alloc((size_t)h.x_rows * (size_t)b->cmp_cap);
+    b->cmp_ctr.alloc(1);
+    hipLaunchKernelGGL(k_compact_rows, dim3((unsigned)((b->cmp_cap + 63) / 64)), dim3(64), 0, t->stream, b->cmp_fac.p,
+                       b->cmp_cap, (long long)h.fac_rows, (long long)h.nnz_lu);
     for (auto* v : {&b->nrm, &b->alpha, &b->best_alpha, &b->best_nrm}) v->alloc(S);
     b->part.alloc((size_t)t->dev.ntiles * S);
     b->pstat.alloc(2 * S);
@@ -3342,10 +3400,23 @@ void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* 
         }
     }
     if (ds.blk) throw Error(P3SMI_E_INTERNAL, "2x2-block schedule launched with G != 64");
+    LuCompact cmp;  // one-case FP64 factor + solve whose factors nothing reads later: compact arenas
+    if constexpr (G == 1 && FACTOR && std::is_same<real, double>::value) {
+        if ((sel == SEL_ALL || sel == SEL_F64) && b->cmp_cap > 0 && xout == b->x.p) {
+            hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, b->topo->stream, b->cmp_ctr.p, 1);
+            cmp.fac = b->cmp_fac.p;
+            cmp.x = b->cmp_x.p;
+            cmp.ctr = b->cmp_ctr.p;
+            cmp.cap = b->cmp_cap;
+            cmp.frows = h.fac_rows;
+            cmp.xrows = h.x_rows;
+            cmp.m = h.m;
+        }
+    }
     hipLaunchKernelGGL((k_lu_stream<G, NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
                        b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.diagB.p, ds.pairs.p,
                        ds.ptrF.p, ds.fwd.p, ds.fent.p, ds.ptrW.p, ds.bwd.p, ds.bent.p, b->Jx.p, fac, rhs, xout,
-                       b->ctl.p, b->pstat.p, b->guard.p, sel, rcond_min, ds.nlev, ds.nblev, b->S);
+                       b->ctl.p, b->pstat.p, b->guard.p, sel, rcond_min, ds.nlev, ds.nblev, b->S, cmp);
 }
 
 // The arguments of an LU launch of shape cfg over batch b (factor + solve of the selected cases).
