@@ -106,9 +106,10 @@ def tune(solver, S, V0, reps: int = 2, log=None) -> dict:
             few[fmax] = _time_solve(solver, bh, reps)
             if log:
                 log(f"[autotune] few g={fg} ns={fns} max={fmax}: {1e3 * few[fmax]:.2f} ms")
+        # a threshold of 0 (no few-case launches) is not offered: the whole batch's time cannot
+        # tell the thresholds apart, and without the few-case shape a single solve takes the
+        # batch shape (a one-outage call on the 10k grid: 69 against 24 ms)
         fmax = min(few, key=few.get)
-        if few[fmax] >= results[(bg, bns, bgrp)]:
-            fmax = 0
         solver.set_lu_shape(bg, bns, fg, fns, fmax, bgrp)
         tiles = {}
         for rows in EVAL_TILE_CANDIDATES:
