@@ -1290,15 +1290,16 @@ __device__ __forceinline__ void lu_blk_body(
     const int4* __restrict__ bwd, const int4* __restrict__ bsteps, const double* __restrict__ Jx,
     real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x, unsigned* __restrict__ ctl,
     double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min, int nlev, int nblev, int S,
-    int bx) {
+    int bx, int cpg) {
     constexpr bool SINGLE = sizeof(real) == 4;
     constexpr int G = 64, TL = NS;
     __shared__ double smin[TL][G], smax[TL][G], ssum[TL][G];
     __shared__ int sbad[G];
     const int lane = threadIdx.x % G;
     const int slot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G));
-    const int c = bx * G + lane;
-    bool act = selected(sel, ctl[c]);
+    // cases bx * cpg + [0, cpg): cpg of the 64 lanes carry one case each
+    const int c = bx * cpg + lane;
+    bool act = lane < cpg && c < S && selected(sel, ctl[c]);
     if (!__syncthreads_or(act)) return;
 
     // pivot statistics live in LDS (a diagonal block updates them; registers are scarce here)
@@ -1355,9 +1356,10 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk(
     const int4* __restrict__ fwd, const int4* __restrict__ fsteps, const int* __restrict__ ptrW,
     const int4* __restrict__ bwd, const int4* __restrict__ bsteps, const double* __restrict__ Jx,
     real* __restrict__ fac, const double* __restrict__ rhs, double* __restrict__ x, unsigned* __restrict__ ctl,
-    double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min, int nlev, int nblev, int S) {
+    double* __restrict__ pstat, int* __restrict__ guard_trips, int sel, double rcond_min, int nlev, int nblev, int S,
+    int cpg) {
     lu_blk_body<NS, real, FACTOR>(ptrA, blkA, ptrB, blkB, steps, ptrF, fwd, fsteps, ptrW, bwd, bsteps, Jx, fac, rhs, x,
-                                  ctl, pstat, guard_trips, sel, rcond_min, nlev, nblev, S, blockIdx.x);
+                                  ctl, pstat, guard_trips, sel, rcond_min, nlev, nblev, S, blockIdx.x, cpg);
 }
 
 template <int NS, typename real, bool FACTOR>
@@ -1369,7 +1371,7 @@ __global__ __launch_bounds__(64 * NS) void k_lu_blk_multi(const LuArgs* __restri
     if (b >= a.gx || a.plan[PL_HALT] || a.plan[PL_LU] != a.want) return;
     lu_blk_body<NS, real, FACTOR>(a.ptrA, a.taskA, a.ptrB, a.taskB, a.steps, a.ptrF, a.fwd, a.fsteps, a.ptrW, a.bwd,
                                   a.bsteps, a.Jx, static_cast<real*>(a.fac), a.rhs, a.x, a.ctl, a.pstat, a.guard,
-                                  a.sel, a.rcond_min, a.nlev, a.nblev, a.S, b);
+                                  a.sel, a.rcond_min, a.nlev, a.nblev, a.S, b, 64);
 }
 
 // Refinement residual r = R - J x in FP64 (J in equation-row CSR order, x in pivot order:
@@ -2814,6 +2816,7 @@ struct p3smi_topology {
     int eval_rows = 32;           // k_evaluate: rows per tile at most (the autotuner may change it)
     double guard_ratio = REUSE_RCOND_RATIO;  // frozen plan rejected below guard_ratio x rcond0
     bool pivot_diagonal_first = true;  // a fresh pivoting factorization tries the plan's diagonal first
+    int n_cu = 256;                    // compute units of the current device (set with the stream)
     hipStream_t stream = nullptr;
     std::unique_ptr<p3smi_batch> cached;  // workspace of the one-call entry points
     std::unique_ptr<p3smi_batch> dc;      // one-case workspace of the DC start
@@ -2928,6 +2931,12 @@ void ensure_device(p3smi_topology* t) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         throw Error(P3SMI_E_DEVICE, "no HIP device available (the MI355X engine needs a GPU)");
     if (!t->stream) HIPCHECK(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    {
+        int dev = 0, ncu = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (ncu > 0) t->n_cu = ncu;
+    }
     const p3smi::Topology& h = t->host;
     hipStream_t st = t->stream;
     DevTopo& d = t->dev;
@@ -3386,16 +3395,29 @@ void launch_evaluate(p3smi_batch* b, int hold) {
     HIPCHECK(hipGetLastError());
 }
 
+// Cases per workgroup of a 2x2-block LU launch. A workgroup's time is set by its chain of
+// dependent loads, not by the width of its rows, so more workgroups of fewer cases finish
+// sooner while each still has a compute unit of its own. 48 FP64 cases make rows of three
+// 128-byte lines (2,000-bus grid, 8,760 cases: 3.84 -> 3.76 ms per launch); 32 cases on that
+// batch need 274 workgroups, more than the 256 compute units (6.3 ms). FP32 rows keep 64 cases
+// (48 would start every other row mid-line).
+template <typename real>
+int blk_cases(const p3smi_batch* b) {
+    if (!std::is_same<real, double>::value) return 64;
+    return (b->S + 47) / 48 <= b->topo->n_cu ? 48 : 64;
+}
+
 template <int G, int NS, typename real, bool FACTOR>
 void launch_lu_cfg(p3smi_batch* b, const DevSched& ds, real* fac, const double* rhs, double* xout, int sel) {
     const p3smi::Topology& h = b->topo->host;
     const double rcond_min = (b->guard_override >= 0.0 ? b->guard_override : b->topo->guard_ratio) * h.rcond0;
     if constexpr (G == 64) {
         if (ds.blk) {
-            hipLaunchKernelGGL((k_lu_blk<NS, real, FACTOR>), dim3((unsigned)(b->S / G)), dim3(64 * NS), 0,
-                               b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.steps.p, ds.ptrF.p,
-                               ds.fwd.p, ds.fsteps.p, ds.ptrW.p, ds.bwd.p, ds.bsteps.p, b->Jx.p, fac, rhs, xout,
-                               b->ctl.p, b->pstat.p, b->guard.p, sel, rcond_min, ds.nlev, ds.nblev, b->S);
+            const int cpg = blk_cases<real>(b);
+            hipLaunchKernelGGL((k_lu_blk<NS, real, FACTOR>), dim3((unsigned)((b->S + cpg - 1) / cpg)), dim3(64 * NS),
+                               0, b->topo->stream, ds.ptrA.p, ds.taskA.p, ds.ptrB.p, ds.taskB.p, ds.steps.p,
+                               ds.ptrF.p, ds.fwd.p, ds.fsteps.p, ds.ptrW.p, ds.bwd.p, ds.bsteps.p, b->Jx.p, fac, rhs,
+                               xout, b->ctl.p, b->pstat.p, b->guard.p, sel, rcond_min, ds.nlev, ds.nblev, b->S, cpg);
             return;
         }
     }

@@ -905,15 +905,27 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
     struct Part { int sub; };  // a partial slot use: written in sub-level `sub`, read in sub + 1
     std::vector<Part> parts;
     std::vector<std::pair<int, Blk>> chunks;  // (sub-level index, chunk task)
+    auto fine_cut = [](int len, int64_t total) {  // see BLK_FINE_MIN
+        return len > BLK_FINE_MIN && total < (int64_t)BLK_REF_LISTS * BLK_FINE_MIN &&
+               (int64_t)len * BLK_REF_LISTS > total;
+    };
+    // pivots summed per sub-level, before any cut (BLK_FINE_MIN)
+    std::vector<int64_t> subtot(2 * (size_t)s.nlev, 0);
+    for (int l = 0; l < s.nlev; ++l) {
+        for (Blk& b : levA[l]) subtot[2 * l] += (shared(b), (int64_t)tl.size());
+        for (Blk& b : levB[l]) subtot[2 * l + 1] += (shared(b), (int64_t)tl.size());
+    }
     auto cut = [&](Blk& b, int sub) {
         shared(b);
         const int len = (int)tl.size();
-        if (len <= BLK_SPLIT_MIN) return;
+        const bool fine = fine_cut(len, subtot[sub]);
+        const int chunk = fine ? BLK_FINE_CHUNK : BLK_CHUNK, tail = fine ? BLK_FINE_TAIL : BLK_TAIL;
+        if (len <= (fine ? BLK_FINE_MIN : BLK_SPLIT_MIN)) return;
         std::vector<std::array<int, 3>> ch;  // (sub-level, beg, end), the last chunk first
         int end = len, sl = sub;
         while (end > 0) {
-            int beg = std::max(0, end - BLK_CHUNK);
-            if (beg < BLK_TAIL) beg = 0;
+            int beg = std::max(0, end - chunk);
+            if (beg < tail) beg = 0;
             int need = 0;
             for (int i = beg; i < end; ++i) need = std::max(need, ready[tl[i]]);
             if (sl < 0 || need > sl) return;  // cannot run this early: the block stays whole
@@ -1017,15 +1029,26 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
     for (int l = 0; l < s.nblev; ++l)
         for (int p : levW[l]) levWi[l].push_back(RItem{p, 0, (int)bcols[p].size()});
     std::vector<int> fslot_lev, bslot_lev;  // level that writes each slot (read at the next)
+    // steps per level before any cut: forward rows with the blocks of their sub-level B
+    std::vector<int64_t> ftot(s.nlev, 0), wtot(s.nblev, 0);
+    for (int l = 0; l < s.nlev; ++l) {
+        ftot[l] = subtot[2 * l + 1];
+        for (const RItem& it : levFi[l]) ftot[l] += it.k1 - it.k0;
+    }
+    for (int l = 0; l < s.nblev; ++l)
+        for (const RItem& it : levWi[l]) wtot[l] += it.k1 - it.k0;
     auto cut_row = [&](RItem& it, int l, const std::vector<int>& cols, const std::vector<int>& lev_of,
-                       std::vector<int>& slot_lev, std::vector<std::pair<int, RItem>>& out) {
+                       std::vector<int>& slot_lev, std::vector<std::pair<int, RItem>>& out,
+                       const std::vector<int64_t>& tot) {
         const int len = it.k1 - it.k0;
-        if (len <= BLK_SPLIT_MIN) return false;
+        const bool fine = fine_cut(len, tot[l]);
+        const int chunk = fine ? BLK_FINE_CHUNK : BLK_CHUNK, tail = fine ? BLK_FINE_TAIL : BLK_TAIL;
+        if (len <= (fine ? BLK_FINE_MIN : BLK_SPLIT_MIN)) return false;
         std::vector<std::array<int, 3>> ch;  // (level, beg, end), the last piece first
         int end = len, lv = l;
         while (end > 0) {
-            int beg = std::max(0, end - BLK_CHUNK);
-            if (beg < BLK_TAIL) beg = 0;
+            int beg = std::max(0, end - chunk);
+            if (beg < tail) beg = 0;
             int need = 0;
             for (int k = beg; k < end; ++k) need = std::max(need, lev_of[lastg[cols[it.k0 + k]]] + 1);
             if (lv < 0 || need > lv) return false;  // cannot run this early: the row stays whole
@@ -1051,9 +1074,9 @@ BlockSchedule make_block_schedule(const Topology& t, int nslots) {
     {
         std::vector<std::pair<int, RItem>> fch, bch;
         for (int l = 0; l < s.nlev; ++l)
-            for (RItem& it : levFi[l]) s.nfcut += cut_row(it, l, fcols[it.p], t.height, fslot_lev, fch);
+            for (RItem& it : levFi[l]) s.nfcut += cut_row(it, l, fcols[it.p], t.height, fslot_lev, fch, ftot);
         for (int l = 0; l < s.nblev; ++l)
-            for (RItem& it : levWi[l]) s.nbcut += cut_row(it, l, bcols[it.p], t.depth, bslot_lev, bch);
+            for (RItem& it : levWi[l]) s.nbcut += cut_row(it, l, bcols[it.p], t.depth, bslot_lev, bch, wtot);
         for (auto& c : fch) levFi[c.first].push_back(c.second);
         for (auto& c : bch) levWi[c.first].push_back(c.second);
     }

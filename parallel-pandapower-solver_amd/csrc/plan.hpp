@@ -106,6 +106,16 @@ enum BlockKind { BLK_DIAG1 = 0, BLK_DIAG2, BLK_U1, BLK_U2, BLK_L1, BLK_L2, BLK_P
 constexpr int BLK_SPLIT_MIN = 32;  // sums over more pivots than this are cut
 constexpr int BLK_CHUNK = 16;      // pivots per chunk (a leftover under BLK_TAIL joins its neighbour)
 constexpr int BLK_TAIL = 8;
+// In a thin sub-level (fewer than BLK_REF_LISTS x BLK_FINE_MIN pivots summed by all its tasks)
+// a sum that sets the sub-level's length -- more pivots than its tasks hold on average over
+// BLK_REF_LISTS lists -- is cut finer, from BLK_FINE_MIN pivots on into chunks of
+// BLK_FINE_CHUNK. Such sums form the chain at the top of the elimination tree, where a level
+// holds one or two pivots and the other lists wait on them. Wider sub-levels keep the coarse
+// cut: their time follows the steps of all their tasks, and each cut adds steps.
+constexpr int BLK_REF_LISTS = 16;
+constexpr int BLK_FINE_MIN = 16;
+constexpr int BLK_FINE_CHUNK = 8;
+constexpr int BLK_FINE_TAIL = 4;
 
 struct BlockTask {
     int kind;        // BlockKind
