@@ -149,6 +149,25 @@ def test_every_lu_shape_gives_the_same_bits(gpu, precision, name):
         assert np.array_equal(r["mismatch"], first["mismatch"]), (gg, ns, grp)
 
 
+def test_block_lu_group_width_gives_the_same_bits(gpu):
+    """The FP64 block LU runs 48 cases per workgroup while a batch's groups of 48 fit one per
+    compute unit (a 512-case batch on 256 compute units), 64 otherwise (12,800 cases). The
+    same cases give the same bits either way, wherever they sit in the batch."""
+    g = grids.config_grid("ts2000", 0)
+    Yp, Yj, Yx = g.csr()
+    S = grids.timeseries_daily(g, 512, seed=5)
+    V0 = dc_initial_voltage(g)
+    s = nr_klu.Solver(Yp, Yj, Yx, g.pv, g.pq)
+    s.set_lu_shape(64, 16, 1, 16, 0, 0)  # few_max 0: every launch uses the batch shape
+    small = s.solve_batch(S, V0, max_iter=20, tol=1e-8)
+    big = s.solve_batch(np.tile(S, (1, 25)), V0, max_iter=20, tol=1e-8)
+    for k in (0, 7, 24):
+        part = slice(512 * k, 512 * (k + 1))
+        assert np.array_equal(big["iterations"][part], small["iterations"]), k
+        assert np.array_equal(big["V"][:, part].view(np.float64), small["V"].view(np.float64)), k
+        assert np.array_equal(big["mismatch"][part], small["mismatch"]), k
+
+
 def test_switching_mixed_mode(gpu):
     """The reference GPU solver's mixed rule (p3s/cuda/nr_polar_kernels.cu:209-288): FP32 steps
     while the mismatch is above 1e-3 p.u., FP64 after. The converged voltages meet the 1e-9 bar
