@@ -3395,12 +3395,13 @@ void launch_evaluate(p3smi_batch* b, int hold) {
     HIPCHECK(hipGetLastError());
 }
 
-// Cases per workgroup of a 2x2-block LU launch. A workgroup's time is set by its chain of
-// dependent loads, not by the width of its rows, so more workgroups of fewer cases finish
-// sooner while each still has a compute unit of its own. 48 FP64 cases make rows of three
-// 128-byte lines (2,000-bus grid, 8,760 cases: 3.84 -> 3.76 ms per launch); 32 cases on that
-// batch need 274 workgroups, more than the 256 compute units (6.3 ms). FP32 rows keep 64 cases
-// (48 would start every other row mid-line).
+// Cases per workgroup of a 2x2-block LU launch: 48 FP64 cases (rows of three 128-byte lines)
+// while the batch's groups of 48 fit one per compute unit, else 64. A workgroup's time is set
+// by its chain of dependent loads and hardly depends on the width of its rows; narrower rows
+// over more compute units ease the load on the memory system (2,000-bus grid, 8,760 cases:
+// 183 workgroups, 3.84 -> 3.76 ms per launch). 32 cases on that batch need 274 workgroups,
+// more than the 256 compute units: 6.3 ms. FP32 rows keep 64 cases (48 would start every
+// other row mid-line).
 template <typename real>
 int blk_cases(const p3smi_batch* b) {
     if (!std::is_same<real, double>::value) return 64;
